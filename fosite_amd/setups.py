@@ -1,0 +1,144 @@
+"""Configurations and initial data for the finite-volume workloads.
+
+A configuration is a plain dict whose keys follow the reference's Dict keys
+(mesh/physics/fluxes/timedisc/boundary/sources sub-dictionaries flattened):
+
+  physics   16 EULER_ISOTHERM | 17 EULER        (physics_base.f90:496-498)
+  geometry  1 CARTESIAN | 31 LOGCYLINDRICAL     (geometry_base.f90:345-357)
+  inum, jnum, xmin, xmax, ymin, ymax, gparam    (mesh_base.f90:295-372)
+  bc        [western, eastern, southern, northern] boundary ids
+            1 NO_GRADIENTS, 2 PERIODIC, 13 ABSORBING (boundary_base.f90:110-130)
+  limiter, theta, recvars                       (reconstruction_linear.f90:71-78,
+                                                 reconstruction_base.f90:117)
+  method, order, cfl, tol_rel, tol_abs, stoptime, dtlimit, beta
+                                                (timedisc_base.f90:187-191, 315-352)
+  fargo     0 off | 2 user-supplied background velocity (mesh_base.f90:372-423)
+  gravity, mass, GN                             (gravity_pointmass.f90:132-301)
+
+Initial data is returned as primitive variables on all cells (ghosts
+included), the isothermal sound speeds at cell centres and faces, and the
+FARGO background velocity per x-ring, i.e. exactly what a fosite setup
+program assigns in its InitData (e.g. examples/mmsn.f90:258-269).
+"""
+import math
+
+import numpy as np
+
+EULER_ISOTHERM, EULER = 16, 17
+CARTESIAN, LOGCYLINDRICAL = 1, 31
+NO_GRADIENTS, PERIODIC, ABSORBING = 1, 2, 13
+MINMOD, MONOCENT, SWEBY, SUPERBEE, OSPRE, VANLEER, NOLIMIT = 1, 2, 3, 4, 5, 7, 8
+PRIMITIVE, CONSERVATIVE = 1, 2
+RK_FEHLBERG, CASH_KARP, DORMAND_PRINCE, SSPRK = 2, 3, 4, 5
+GNUM = 2
+
+DEFAULTS = dict(gamma=1.4, gparam=1.0, limiter=MINMOD, theta=1.0, recvars=CONSERVATIVE,
+                method=SSPRK, order=5, cfl=0.4, tol_rel=0.01, tol_abs=[0.001] * 4,
+                dtlimit=-1.0, beta=0.08, fargo=0, gravity=0, mass=1.0, GN=1.0)
+
+
+def full(cfg):
+    """Configuration with the reference's defaults filled in."""
+    out = dict(DEFAULTS)
+    out.update(cfg)
+    return out
+
+
+def coords(cfg):
+    """Cell-centre curvilinear coordinates per ring/column (ghosts included),
+    the reference's formula xmin + (2i-1)/2 * dx (mesh_base.f90:650-653)."""
+    dx = (cfg["xmax"] - cfg["xmin"]) / cfg["inum"]
+    dy = (cfg["ymax"] - cfg["ymin"]) / cfg["jnum"]
+    fi = np.arange(cfg["inum"] + 2 * GNUM) - 1
+    fj = np.arange(cfg["jnum"] + 2 * GNUM) - 1
+    xc = cfg["xmin"] + ((2 * fi - 1) * 0.5) * dx
+    yc = cfg["ymin"] + ((2 * fj - 1) * 0.5) * dy
+    return xc, yc, dx, dy
+
+
+def _exp(a):
+    # libm exp per value: the reference's scale factors use EXP from libm
+    return np.array([math.exp(float(v)) for v in a])
+
+
+def splitmix64_uniform(seed, n):
+    """U(0,1) samples u[k] for k = 0..n-1 from SplitMix64 keyed by (seed, k)."""
+    with np.errstate(over="ignore"):
+        z = (np.uint64(seed) + np.arange(1, n + 1, dtype=np.uint64)
+             * np.uint64(0x9E3779B97F4A7C15))
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+# ----------------------------------------------------------------- config 1
+
+def riemann2d_config(n=256, steps=200):
+    """2-D Euler four-quadrant Riemann problem on [0,1]^2 (Lax-Liu config 3)."""
+    return full(dict(physics=EULER, gamma=1.4, geometry=CARTESIAN, inum=n, jnum=n,
+                     xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0,
+                     bc=[NO_GRADIENTS] * 4, limiter=VANLEER, recvars=CONSERVATIVE,
+                     method=SSPRK, order=3, cfl=0.4, stoptime=1.0, steps=steps))
+
+
+def riemann2d_initial(cfg):
+    """Quadrant states (rho,u,v,p) split at (0.5,0.5) by cell centre."""
+    xc, yc, _, _ = coords(cfg)
+    X, Y = np.meshgrid(xc, yc)
+    east, north = X >= 0.5, Y >= 0.5
+    states = {  # (east, north): (rho, u, v, p)
+        (True, True): (1.5, 0.0, 0.0, 1.5),
+        (False, True): (0.5323, 1.206, 0.0, 0.3),
+        (False, False): (0.138, 1.206, 1.206, 0.029),
+        (True, False): (0.5323, 0.0, 1.206, 0.3),
+    }
+    pv = np.zeros((4,) + X.shape)
+    for (e, nn), st in states.items():
+        m = (east == e) & (north == nn)
+        for l in range(4):
+            pv[l][m] = st[l]
+    return dict(pvar=pv)
+
+
+# ----------------------------------------------------------------- config 2
+
+def kepler_disk_config(ni=512, nj=1024, steps=1000, cs_ratio=0.05):
+    """Isothermal Keplerian disk on a logcylindrical grid, r in [1,10]."""
+    return full(dict(physics=EULER_ISOTHERM, geometry=LOGCYLINDRICAL, inum=ni, jnum=nj,
+                     xmin=math.log(1.0), xmax=math.log(10.0), ymin=0.0, ymax=2 * math.pi,
+                     gparam=1.0, bc=[ABSORBING, ABSORBING, PERIODIC, PERIODIC],
+                     limiter=VANLEER, recvars=CONSERVATIVE, method=SSPRK, order=3,
+                     cfl=0.3, stoptime=1.0e3, fargo=2, gravity=1, mass=1.0, GN=1.0,
+                     steps=steps, cs_ratio=cs_ratio, seed=20261202))
+
+
+def kepler_disk_initial(cfg):
+    """Sigma ~ r^-1 (1 + 0.01 U(-1,1)), v_r = 0, v_phi = sqrt(GM/r),
+    cs = cs_ratio * sqrt(GM/r) at centres and faces, w(i) = sqrt(GM/r_i)."""
+    xc, yc, dx, _ = coords(cfg)
+    gp, GM = cfg["gparam"], cfg["GN"] * cfg["mass"]
+    NI, NJ = len(xc), len(yc)
+    r = gp * _exp(xc)
+    rw = gp * _exp(xc + (-(0.5 * dx)))
+    re = gp * _exp(xc + 0.5 * dx)
+    u = splitmix64_uniform(cfg.get("seed", 20261202), NI * NJ).reshape(NJ, NI)
+    sigma = (1.0 / r)[None, :] * (1.0 + 0.01 * (2.0 * u - 1.0))
+    vk = np.sqrt(GM / r)
+    pv = np.zeros((3, NJ, NI))
+    pv[0] = sigma
+    pv[2] = vk[None, :]
+    q = cfg.get("cs_ratio", 0.05)
+    cs = np.broadcast_to((q * np.sqrt(GM / r))[None, :], (NJ, NI)).copy()
+    fcs = np.empty((4, NJ, NI))
+    fcs[0] = (q * np.sqrt(GM / rw))[None, :]
+    fcs[1] = (q * np.sqrt(GM / re))[None, :]
+    fcs[2] = cs
+    fcs[3] = cs
+    return dict(pvar=pv, bccsound=cs, fcsound=fcs, w=vk.copy())
+
+
+def initial(cfg):
+    if cfg["physics"] == EULER:
+        return riemann2d_initial(cfg)
+    return kepler_disk_initial(cfg)

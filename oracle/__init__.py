@@ -1,0 +1,165 @@
+"""CPU oracle for fosite's unsplit finite-volume step -- TEST INFRASTRUCTURE.
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's cpu_baseline leg may
+import this package, and only as the checker.  The product (``fosite_amd``)
+never imports it.
+
+``Oracle`` wraps ``oracle/build/libfvoracle.so`` (the C restatement in
+``fvref.c``) with the same configuration vocabulary as the product.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "build", "libfvoracle.so")
+
+
+def build(force=False):
+    """Compile the C restatement (no FMA contraction, IEEE semantics)."""
+    os.makedirs(os.path.dirname(LIB), exist_ok=True)
+    src = os.path.join(HERE, "fvref.c")
+    if not force and os.path.exists(LIB) and os.path.getmtime(LIB) >= os.path.getmtime(src):
+        return LIB
+    subprocess.check_call(["gcc", "-O2", "-ffp-contract=off", "-fno-fast-math", "-fPIC",
+                           "-shared", "-o", LIB, src, "-lm"])
+    return LIB
+
+
+class _Cfg(ctypes.Structure):
+    _fields_ = [("physics", ctypes.c_int), ("gamma", ctypes.c_double),
+                ("geometry", ctypes.c_int), ("inum", ctypes.c_int), ("jnum", ctypes.c_int),
+                ("xmin", ctypes.c_double), ("xmax", ctypes.c_double),
+                ("ymin", ctypes.c_double), ("ymax", ctypes.c_double),
+                ("gparam", ctypes.c_double), ("bc", ctypes.c_int * 4),
+                ("limiter", ctypes.c_int), ("theta", ctypes.c_double),
+                ("recvars", ctypes.c_int), ("method", ctypes.c_int), ("order", ctypes.c_int),
+                ("cfl", ctypes.c_double), ("tol_rel", ctypes.c_double),
+                ("tol_abs", ctypes.c_double * 4), ("stoptime", ctypes.c_double),
+                ("dtlimit", ctypes.c_double), ("beta", ctypes.c_double),
+                ("fargo", ctypes.c_int), ("gravity", ctypes.c_int),
+                ("mass", ctypes.c_double), ("GN", ctypes.c_double)]
+
+
+class _Scalars(ctypes.Structure):
+    _fields_ = [("iter", ctypes.c_int), ("n_adj", ctypes.c_int), ("brk", ctypes.c_int),
+                ("dtcause", ctypes.c_int), ("time", ctypes.c_double), ("dt", ctypes.c_double),
+                ("dtold", ctypes.c_double), ("maxerrold", ctypes.c_double)]
+
+
+_lib = None
+
+
+def _load():
+    global _lib
+    if _lib is None:
+        build()
+        L = ctypes.CDLL(LIB)
+        P = ctypes.POINTER(ctypes.c_double)
+        L.fvr_create.restype = ctypes.c_void_p
+        L.fvr_create.argtypes = [ctypes.POINTER(_Cfg)]
+        L.fvr_destroy.argtypes = [ctypes.c_void_p]
+        L.fvr_ncells.argtypes = [ctypes.c_void_p]
+        L.fvr_vnum.argtypes = [ctypes.c_void_p]
+        L.fvr_mesh.argtypes = [ctypes.c_void_p, ctypes.c_int, P]
+        L.fvr_set_sound_speeds.argtypes = [ctypes.c_void_p, P, P]
+        L.fvr_set_background_velocity.argtypes = [ctypes.c_void_p, P]
+        L.fvr_set_pvar.argtypes = [ctypes.c_void_p, P]
+        L.fvr_first_step.argtypes = [ctypes.c_void_p]
+        L.fvr_step.argtypes = [ctypes.c_void_p]
+        L.fvr_get.argtypes = [ctypes.c_void_p, P, P, ctypes.POINTER(_Scalars)]
+        L.fvr_compute_rhs.argtypes = [ctypes.c_void_p, P]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+MESH_IDS = ["dAx_w", "dAx_e", "dAy_s", "dAy_n", "dydzdV", "dzdxdV", "dxdydV", "dlx", "dly",
+            "cxyx", "cyxy", "czxz", "czyz", "rdx_w", "rdx_e", "rdx_s", "rdx_n", "radius",
+            "radius_w", "radius_e", "radius_s", "radius_n", "posvec_x", "posvec_y",
+            "bcenter_x", "bcenter_y", "accel_x", "accel_y"]
+
+
+class Oracle:
+    """One oracle simulation; ``cfg`` is a ``fosite_amd.config`` style dict."""
+
+    def __init__(self, cfg):
+        L = _load()
+        c = _Cfg()
+        c.physics = cfg["physics"]
+        c.gamma = cfg.get("gamma", 1.4)
+        c.geometry = cfg["geometry"]
+        c.inum, c.jnum = cfg["inum"], cfg["jnum"]
+        c.xmin, c.xmax, c.ymin, c.ymax = cfg["xmin"], cfg["xmax"], cfg["ymin"], cfg["ymax"]
+        c.gparam = cfg.get("gparam", 1.0)
+        for d, b in enumerate(cfg["bc"]):
+            c.bc[d] = b
+        c.limiter = cfg.get("limiter", 7)
+        c.theta = cfg.get("theta", 1.0)
+        c.recvars = cfg.get("recvars", 2)
+        c.method, c.order = cfg.get("method", 5), cfg.get("order", 3)
+        c.cfl = cfg.get("cfl", 0.4)
+        c.tol_rel = cfg.get("tol_rel", 0.01)
+        ta = cfg.get("tol_abs", [0.001] * 4)
+        for l in range(4):
+            c.tol_abs[l] = ta[l] if l < len(ta) else 0.001
+        c.stoptime = cfg["stoptime"]
+        c.dtlimit = cfg.get("dtlimit", -1.0)
+        c.beta = cfg.get("beta", 0.08)
+        c.fargo = cfg.get("fargo", 0)
+        c.gravity = cfg.get("gravity", 0)
+        c.mass = cfg.get("mass", 1.0)
+        c.GN = cfg.get("GN", 1.0)
+        self._h = L.fvr_create(ctypes.byref(c))
+        if not self._h:
+            raise ValueError("oracle: unsupported configuration")
+        self.L = L
+        self.N = L.fvr_ncells(self._h)
+        self.V = L.fvr_vnum(self._h)
+        self.NI = cfg["inum"] + 4
+        self.NJ = cfg["jnum"] + 4
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self.L.fvr_destroy(h)
+            self._h = None
+
+    def mesh(self, name):
+        out = np.empty(self.N)
+        self.L.fvr_mesh(self._h, MESH_IDS.index(name), _ptr(out))
+        return out.reshape(self.NJ, self.NI)
+
+    def set_initial(self, pvar, bccsound=None, fcsound=None, w=None):
+        if bccsound is not None:
+            b = np.ascontiguousarray(bccsound, dtype=np.float64)
+            f = np.ascontiguousarray(fcsound, dtype=np.float64)
+            self.L.fvr_set_sound_speeds(self._h, _ptr(b), _ptr(f))
+        if w is not None:
+            ww = np.ascontiguousarray(w, dtype=np.float64)
+            self.L.fvr_set_background_velocity(self._h, _ptr(ww))
+        p = np.ascontiguousarray(pvar, dtype=np.float64)
+        self.L.fvr_set_pvar(self._h, _ptr(p))
+
+    def first_step(self):
+        return self.L.fvr_first_step(self._h)
+
+    def step(self):
+        return self.L.fvr_step(self._h)
+
+    def state(self):
+        cv = np.empty((self.V, self.NJ, self.NI))
+        pv = np.empty((self.V, self.NJ, self.NI))
+        sc = _Scalars()
+        self.L.fvr_get(self._h, _ptr(cv), _ptr(pv), ctypes.byref(sc))
+        return cv, pv, {k: getattr(sc, k) for k, _ in _Scalars._fields_}
+
+    def rhs(self):
+        out = np.empty((self.V, self.NJ, self.NI))
+        self.L.fvr_compute_rhs(self._h, _ptr(out))
+        return out

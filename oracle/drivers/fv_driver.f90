@@ -1,0 +1,176 @@
+!#############################################################################
+!# fv_driver: setup program for the reference library (test infrastructure) #
+!#############################################################################
+!> Runs the reference finite-volume step on a configuration read from a
+!! namelist file and dumps the state after FirstStep and after selected Step()
+!! calls.  Used only to produce parity fixtures for the oracle; never shipped.
+!!
+!! Follows the reference's setup-program pattern (tests/riemann2d.f90:520-676:
+!! Dict config -> Sim%Setup -> initial data -> Step loop) and fosite%FirstStep /
+!! fosite%Step (fosite.f90:258-464).  Step() is called exactly k_nsteps times
+!! after an explicit FirstStep (Run() would do one extra step, fosite.f90:359).
+!!
+!! Input:  namelist /fvcfg/ (see below); binary initial data file with
+!!         pvar(IGMIN:IGMAX,JGMIN:JGMAX,1,VNUM), then for isothermal k_physics
+!!         bccsound(cells) and fcsound(cells,4), then for k_fargo=2 w(IGMIN:IGMAX).
+!! Output: <k_prefix>_mesh.bin (metric arrays), <k_prefix>_steps.bin (records).
+PROGRAM fv_driver
+  USE fosite_mod
+  USE physics_eulerisotherm_mod, ONLY : physics_eulerisotherm
+  IMPLICIT NONE
+  !--------------------------------------------------------------------------!
+  INTEGER :: k_physics = 17, k_geometry = 1, k_inum = 32, k_jnum = 32
+  REAL    :: k_gamma = 1.4, k_xmin = 0., k_xmax = 1., k_ymin = 0., k_ymax = 1., k_gparam = 1.
+  INTEGER :: k_bcw = 1, k_bce = 1, k_bcs = 1, k_bcn = 1
+  INTEGER :: k_limiter = 7, k_recvars = 2
+  REAL    :: k_theta = 1.0
+  INTEGER :: k_method = 5, k_order = 3
+  REAL    :: k_cfl = 0.4, k_tolrel = 0.01, k_tolabs(4) = 0.001, k_stoptime = 1.0
+  REAL    :: k_dtlimit = -1.0, k_beta = 0.08
+  INTEGER :: k_fargo = 0, k_gravity = 0, k_units = 0
+  REAL    :: k_mass = 1.0
+  INTEGER :: k_nsteps = 10, k_dumpevery = 1
+  CHARACTER(LEN=512) :: k_infile = "", k_prefix = "out"
+  NAMELIST /fvcfg/ k_physics, k_geometry, k_inum, k_jnum, k_gamma, k_xmin, k_xmax, k_ymin, k_ymax, &
+       k_gparam, k_bcw, k_bce, k_bcs, k_bcn, k_limiter, k_recvars, k_theta, k_method, k_order,   &
+       k_cfl, k_tolrel, k_tolabs, k_stoptime, k_dtlimit, k_beta, k_fargo, k_gravity, k_units,   &
+       k_mass, k_nsteps, k_dumpevery, k_infile, k_prefix
+  !--------------------------------------------------------------------------!
+  CLASS(fosite), ALLOCATABLE :: Sim
+  TYPE(Dict_TYP), POINTER    :: config, mesh, phys, bnd, flx, tdisc, dfile, &
+                                src, pmass
+  CHARACTER(LEN=512)         :: cfgname
+  INTEGER                    :: u, n, vnum, ncell
+  LOGICAL                    :: brk
+  REAL, ALLOCATABLE          :: pv(:), cs(:), fcs(:), w(:)
+  !--------------------------------------------------------------------------!
+  CALL GET_COMMAND_ARGUMENT(1, cfgname)
+  OPEN(NEWUNIT=u, FILE=TRIM(cfgname), STATUS="old")
+  READ(u, NML=fvcfg)
+  CLOSE(u)
+
+  ALLOCATE(Sim)
+  CALL Sim%InitFosite()
+
+  mesh => Dict("meshtype" / MIDPOINT, "geometry" / k_geometry, &
+       "inum" / k_inum, "jnum" / k_jnum, "knum" / 1, &
+       "xmin" / k_xmin, "xmax" / k_xmax, "ymin" / k_ymin, "ymax" / k_ymax, &
+       "zmin" / 0.0, "zmax" / 0.0, "gparam" / k_gparam)
+  IF (k_fargo.GT.0) CALL SetAttr(mesh, "fargo/method", k_fargo)
+  bnd => Dict("western" / k_bcw, "eastern" / k_bce, "southern" / k_bcs, &
+       "northern" / k_bcn, "bottomer" / NO_GRADIENTS, "topper" / NO_GRADIENTS)
+  IF (k_physics.EQ.EULER) THEN
+    phys => Dict("problem" / k_physics, "gamma" / k_gamma)
+  ELSE
+    phys => Dict("problem" / k_physics)
+  END IF
+  IF (k_units.GT.0) CALL SetAttr(phys, "units", k_units)
+  flx => Dict("order" / LINEAR, "fluxtype" / KT, "variables" / k_recvars, &
+       "limiter" / k_limiter, "theta" / k_theta)
+  tdisc => Dict("method" / k_method, "order" / k_order, "cfl" / k_cfl, &
+       "stoptime" / k_stoptime, "tol_rel" / k_tolrel, "beta" / k_beta, &
+       "maxiter" / 100000000)
+  IF (k_physics.EQ.EULER) THEN
+    CALL SetAttr(tdisc, "tol_abs", k_tolabs(1:4))
+  ELSE
+    CALL SetAttr(tdisc, "tol_abs", k_tolabs(1:3))
+  END IF
+  IF (k_dtlimit.GT.0.0) CALL SetAttr(tdisc, "dtlimit", k_dtlimit)
+  dfile => Dict("fileformat" / GNUPLOT, "filename" / (TRIM(k_prefix) // "_data"), &
+       "count" / 1)
+  config => Dict("mesh" / mesh, "physics" / phys, "boundary" / bnd, &
+       "fluxes" / flx, "timedisc" / tdisc, "datafile" / dfile)
+  IF (k_gravity.GT.0) THEN
+    pmass => Dict("gtype" / POINTMASS, "potential" / NEWTON, "mass" / k_mass, &
+         "outbound" / 0)
+    src => Dict("grav/stype" / GRAVITY, "grav/pmass" / pmass)
+    CALL SetAttr(config, "sources", src)
+  END IF
+  Sim%config => config
+  CALL Sim%Setup()
+
+  ! ---- initial data ----
+  vnum  = Sim%Physics%VNUM
+  ncell = SIZE(Sim%Timedisc%pvar%data1d) / vnum
+  ALLOCATE(pv(ncell*vnum))
+  OPEN(NEWUNIT=u, FILE=TRIM(k_infile), ACCESS="stream", FORM="unformatted", STATUS="old")
+  READ(u) pv
+  Sim%Timedisc%pvar%data1d(:) = pv(:)
+  SELECT TYPE(p => Sim%Physics)
+  TYPE IS(physics_eulerisotherm)
+    ALLOCATE(cs(ncell), fcs(ncell*Sim%Mesh%NFACES))
+    READ(u) cs
+    READ(u) fcs
+    p%bccsound%data1d(:) = cs(:)
+    p%fcsound%data1d(:)  = fcs(:)
+  END SELECT
+  IF (k_fargo.EQ.2) THEN
+    ALLOCATE(w(Sim%Mesh%IGMAX-Sim%Mesh%IGMIN+1))
+    READ(u) w
+    Sim%Timedisc%w(:,1) = w(:)
+  END IF
+  CLOSE(u)
+  CALL Sim%Physics%Convert2Conservative(Sim%Timedisc%pvar, Sim%Timedisc%cvar)
+
+  CALL DumpMesh(TRIM(k_prefix) // "_mesh.bin")
+
+  ! ---- integration ----
+  OPEN(NEWUNIT=u, FILE=TRIM(k_prefix) // "_steps.bin", ACCESS="stream", &
+       FORM="unformatted", STATUS="replace")
+  CALL Sim%FirstStep()
+  CALL DumpState(u, 0)
+  DO n = 1, k_nsteps
+    brk = Sim%Step()
+    IF (MOD(n, k_dumpevery).EQ.0 .OR. n.EQ.k_nsteps .OR. brk .OR. Sim%Timedisc%break) &
+      CALL DumpState(u, n)
+    IF (brk .OR. Sim%Timedisc%break) EXIT
+  END DO
+  CLOSE(u)
+  CALL Sim%Finalize()
+  DEALLOCATE(Sim)
+
+CONTAINS
+
+  !> one record: int32 step, int32 iter, int32 n_adj, int32 break,
+  !! float64 time, dt (next), dtold, maxerrold, then cvar, pvar (data1d)
+  SUBROUTINE DumpState(u, n)
+    INTEGER, INTENT(IN) :: u, n
+    INTEGER(KIND=4)     :: ibuf(4)
+    REAL(KIND=8)        :: rbuf(4)
+    ibuf(1) = n
+    ibuf(2) = Sim%iter
+    ibuf(3) = Sim%Timedisc%n_adj
+    ibuf(4) = MERGE(1, 0, Sim%Timedisc%break)
+    rbuf(1) = Sim%Timedisc%time
+    rbuf(2) = Sim%Timedisc%dt
+    rbuf(3) = Sim%Timedisc%dtold
+    rbuf(4) = Sim%Timedisc%maxerrold
+    WRITE(u) ibuf, rbuf
+    WRITE(u) Sim%Timedisc%cvar%data1d
+    WRITE(u) Sim%Timedisc%pvar%data1d
+  END SUBROUTINE DumpState
+
+  !> metric arrays the GPU context is built from (all cells incl. ghosts)
+  SUBROUTINE DumpMesh(fname)
+    CHARACTER(LEN=*), INTENT(IN) :: fname
+    INTEGER :: v, f
+    OPEN(NEWUNIT=v, FILE=fname, ACCESS="stream", FORM="unformatted", STATUS="replace")
+    WRITE(v) Sim%Mesh%dx, Sim%Mesh%dy, Sim%Mesh%invdx, Sim%Mesh%invdy
+    WRITE(v) Sim%Mesh%dAxdydz(:,:,:,1:2)
+    WRITE(v) Sim%Mesh%dAydzdx(:,:,:,1:2)
+    WRITE(v) Sim%Mesh%dydzdV%data3d, Sim%Mesh%dzdxdV%data3d, Sim%Mesh%dxdydV%data3d
+    WRITE(v) Sim%Mesh%dlx%data3d, Sim%Mesh%dly%data3d
+    WRITE(v) Sim%Mesh%cxyx%bcenter, Sim%Mesh%cyxy%bcenter, &
+             Sim%Mesh%czxz%bcenter, Sim%Mesh%czyz%bcenter
+    DO f = 1, 4
+      WRITE(v) Sim%Mesh%curv%faces(:,:,:,f,(f+1)/2) - Sim%Mesh%curv%bcenter(:,:,:,(f+1)/2)
+    END DO
+    WRITE(v) Sim%Mesh%radius%bcenter
+    DO f = 1, 4
+      WRITE(v) Sim%Mesh%radius%faces(:,:,:,f)
+    END DO
+    WRITE(v) Sim%Mesh%posvec%bcenter(:,:,:,1), Sim%Mesh%posvec%bcenter(:,:,:,2)
+    CLOSE(v)
+  END SUBROUTINE DumpMesh
+
+END PROGRAM fv_driver

@@ -1,0 +1,93 @@
+"""Run the compiled reference (oracle/_ref/fv_driver) -- TEST INFRASTRUCTURE.
+
+Writes a namelist and the initial-data file, runs the reference's
+FirstStep + nsteps x Step (fosite.f90:258-464) through our setup program
+oracle/drivers/fv_driver.f90, and parses the dumped records.
+"""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DRIVER = os.path.join(HERE, "_ref", "fv_driver")
+
+
+def available():
+    return os.path.exists(DRIVER)
+
+
+def _nml(cfg, nsteps, dump_every, infile, prefix):
+    ta = list(cfg.get("tol_abs", [0.001] * 4)) + [0.001] * 4
+    vals = dict(k_physics=cfg["physics"], k_geometry=cfg["geometry"], k_inum=cfg["inum"],
+                k_jnum=cfg["jnum"], k_gamma=cfg.get("gamma", 1.4), k_xmin=cfg["xmin"],
+                k_xmax=cfg["xmax"], k_ymin=cfg["ymin"], k_ymax=cfg["ymax"],
+                k_gparam=cfg.get("gparam", 1.0), k_bcw=cfg["bc"][0], k_bce=cfg["bc"][1],
+                k_bcs=cfg["bc"][2], k_bcn=cfg["bc"][3], k_limiter=cfg.get("limiter", 1),
+                k_recvars=cfg.get("recvars", 2), k_theta=cfg.get("theta", 1.0),
+                k_method=cfg.get("method", 5), k_order=cfg.get("order", 5),
+                k_cfl=cfg.get("cfl", 0.4), k_tolrel=cfg.get("tol_rel", 0.01),
+                k_stoptime=cfg["stoptime"], k_dtlimit=cfg.get("dtlimit", -1.0),
+                k_beta=cfg.get("beta", 0.08), k_fargo=cfg.get("fargo", 0),
+                k_gravity=cfg.get("gravity", 0), k_units=3 if cfg.get("gravity", 0) else 0,
+                k_mass=cfg.get("mass", 1.0), k_nsteps=nsteps, k_dumpevery=dump_every)
+    lines = ["&fvcfg"]
+    for k, v in vals.items():
+        lines.append(" %s = %s" % (k, repr(float(v)) if isinstance(v, float) else v))
+    lines.append(" k_tolabs = " + ", ".join(repr(float(x)) for x in ta[:4]))
+    lines.append(' k_infile = "%s"' % infile)
+    lines.append(' k_prefix = "%s"' % prefix)
+    lines.append("/")
+    return "\n".join(lines) + "\n"
+
+
+def run(cfg, init, nsteps, dump_every=1, workdir=None):
+    """Returns (mesh dict, list of step records {step, iter, n_adj, brk, time,
+    dt, dtold, maxerrold, cvar, pvar})."""
+    NI, NJ = cfg["inum"] + 4, cfg["jnum"] + 4
+    N = NI * NJ
+    V = 4 if cfg["physics"] == 17 else 3
+    tmp = workdir or tempfile.mkdtemp(prefix="fvref_")
+    infile = os.path.join(tmp, "init.bin")
+    prefix = os.path.join(tmp, "run")
+    with open(infile, "wb") as f:
+        f.write(np.ascontiguousarray(init["pvar"], dtype="<f8").tobytes())
+        if V == 3:
+            f.write(np.ascontiguousarray(init["bccsound"], dtype="<f8").tobytes())
+            f.write(np.ascontiguousarray(init["fcsound"], dtype="<f8").tobytes())
+        if cfg.get("fargo", 0) == 2:
+            f.write(np.ascontiguousarray(init["w"], dtype="<f8").tobytes())
+    nml = os.path.join(tmp, "cfg.nml")
+    with open(nml, "w") as f:
+        f.write(_nml(cfg, nsteps, dump_every, infile, prefix))
+    r = subprocess.run([DRIVER, nml], cwd=tmp, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("reference driver failed:\n" + r.stdout[-3000:] + r.stderr[-3000:])
+    m = np.fromfile(prefix + "_mesh.bin", dtype="<f8")
+    o = 4
+    mesh = dict(dx=m[0], dy=m[1], invdx=m[2], invdy=m[3])
+    def take(name, k=1):
+        nonlocal o
+        a = m[o:o + k * N].reshape((k, NJ, NI) if k > 1 else (NJ, NI))
+        o += k * N
+        mesh[name] = a
+    take("dAx", 2); take("dAy", 2)
+    for n in ("dydzdV", "dzdxdV", "dxdydV", "dlx", "dly", "cxyx", "cyxy", "czxz", "czyz"):
+        take(n)
+    for n in ("rdx_w", "rdx_e", "rdx_s", "rdx_n", "radius", "radius_w", "radius_e",
+              "radius_s", "radius_n", "posvec_x", "posvec_y"):
+        take(n)
+    recs = []
+    raw = open(prefix + "_steps.bin", "rb").read()
+    off = 0
+    rec = 48 + 2 * V * N * 8
+    while off + rec <= len(raw):
+        ib = np.frombuffer(raw, "<i4", 4, off)
+        rb = np.frombuffer(raw, "<f8", 4, off + 16)
+        cv = np.frombuffer(raw, "<f8", V * N, off + 48).reshape(V, NJ, NI).copy()
+        pv = np.frombuffer(raw, "<f8", V * N, off + 48 + V * N * 8).reshape(V, NJ, NI).copy()
+        recs.append(dict(step=int(ib[0]), iter=int(ib[1]), n_adj=int(ib[2]), brk=int(ib[3]),
+                         time=rb[0], dt=rb[1], dtold=rb[2], maxerrold=rb[3], cvar=cv, pvar=pv))
+        off += rec
+    return mesh, recs
