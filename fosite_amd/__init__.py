@@ -1,0 +1,195 @@
+"""fosite_amd -- fosite's unsplit finite-volume step on AMD MI355X (gfx950).
+
+Python host for ``libfosite_fv.so`` (C ABI declared in include/fosite_fv.h).
+The object API follows fosite's driver (fosite.f90): a ``Fosite`` is built
+from a configuration dict (see ``fosite_amd.setups``), receives its initial
+data (InitData), then runs FirstStep / Step / Run.  All array work happens on
+the GPU; this module only marshals configuration and host arrays.
+
+The library is built in-tree by ``fosite_amd.build.build()``; importing this
+package never compiles anything and raises if the library is missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import setups
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libfosite_fv.so")
+
+__all__ = ["Fosite", "FvConfig", "mesh_array", "setups", "load_library", "LIBPATH"]
+
+
+class FvConfig(ctypes.Structure):
+    """Mirror of ``fv_config`` (include/fosite_fv.h)."""
+    _fields_ = [("physics", ctypes.c_int), ("gamma", ctypes.c_double),
+                ("geometry", ctypes.c_int), ("inum", ctypes.c_int), ("jnum", ctypes.c_int),
+                ("xmin", ctypes.c_double), ("xmax", ctypes.c_double),
+                ("ymin", ctypes.c_double), ("ymax", ctypes.c_double),
+                ("gparam", ctypes.c_double), ("bc", ctypes.c_int * 4),
+                ("limiter", ctypes.c_int), ("theta", ctypes.c_double),
+                ("recvars", ctypes.c_int), ("method", ctypes.c_int), ("order", ctypes.c_int),
+                ("cfl", ctypes.c_double), ("tol_rel", ctypes.c_double),
+                ("tol_abs", ctypes.c_double * 4), ("stoptime", ctypes.c_double),
+                ("dtlimit", ctypes.c_double), ("beta", ctypes.c_double),
+                ("fargo", ctypes.c_int), ("gravity", ctypes.c_int),
+                ("mass", ctypes.c_double), ("GN", ctypes.c_double), ("dtype", ctypes.c_int)]
+
+    @classmethod
+    def from_dict(cls, cfg):
+        cfg = setups.full(cfg)
+        c = cls()
+        for name, _ in cls._fields_:
+            if name in ("bc", "tol_abs"):
+                continue
+            if name == "dtype":
+                c.dtype = {"fp64": 0, 0: 0}.get(cfg.get("dtype", 0), -1)
+                continue
+            setattr(c, name, cfg[name])
+        for d in range(4):
+            c.bc[d] = cfg["bc"][d]
+        ta = list(cfg["tol_abs"]) + [0.001] * 4
+        for l in range(4):
+            c.tol_abs[l] = ta[l]
+        return c
+
+
+class FvScalars(ctypes.Structure):
+    _fields_ = [("iter", ctypes.c_int), ("n_adj", ctypes.c_int), ("brk", ctypes.c_int),
+                ("dtcause", ctypes.c_int), ("time", ctypes.c_double), ("dt", ctypes.c_double),
+                ("dtold", ctypes.c_double), ("maxerrold", ctypes.c_double)]
+
+
+# every entry point of include/fosite_fv.h
+EXPORTS = ["fv_mesh_array", "fv_create", "fv_destroy", "fv_dims", "fv_set_sound_speeds",
+           "fv_set_background_velocity", "fv_set_pvar", "fv_first_step", "fv_step", "fv_run",
+           "fv_get_state", "fv_get_scalars", "fv_last_run_ms", "fv_last_error"]
+
+_lib = None
+
+
+def load_library():
+    """Load the in-tree HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise RuntimeError("libfosite_fv.so not built: run fosite_amd.build.build() "
+                           "(or __graft_entry__.build()) first")
+    L = ctypes.CDLL(LIBPATH)
+    P = ctypes.POINTER(ctypes.c_double)
+    V = ctypes.c_void_p
+    L.fv_mesh_array.argtypes = [ctypes.POINTER(FvConfig), ctypes.c_int, P]
+    L.fv_create.argtypes = [ctypes.POINTER(V), ctypes.POINTER(FvConfig)]
+    L.fv_destroy.argtypes = [V]
+    L.fv_dims.argtypes = [V, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                          ctypes.POINTER(ctypes.c_int)]
+    L.fv_set_sound_speeds.argtypes = [V, P, P]
+    L.fv_set_background_velocity.argtypes = [V, P]
+    L.fv_set_pvar.argtypes = [V, P]
+    L.fv_first_step.argtypes = [V, ctypes.POINTER(ctypes.c_int)]
+    L.fv_step.argtypes = [V, ctypes.POINTER(ctypes.c_int)]
+    L.fv_run.argtypes = [V, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    L.fv_get_state.argtypes = [V, P, P]
+    L.fv_get_scalars.argtypes = [V, ctypes.POINTER(FvScalars)]
+    L.fv_last_run_ms.argtypes = [V]
+    L.fv_last_run_ms.restype = ctypes.c_double
+    L.fv_last_error.argtypes = [V]
+    L.fv_last_error.restype = ctypes.c_char_p
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def mesh_array(cfg, name):
+    """Host-side mesh metric array (no GPU needed), shape (NJ, NI)."""
+    ids = ["dAx_w", "dAx_e", "dAy_s", "dAy_n", "dydzdV", "dzdxdV", "dxdydV", "dlx", "dly",
+           "cxyx", "cyxy", "czxz", "czyz", "rdx_w", "rdx_e", "rdx_s", "rdx_n", "radius",
+           "radius_w", "radius_e", "radius_s", "radius_n", "posvec_x", "posvec_y",
+           "bcenter_x", "bcenter_y", "accel_x", "accel_y"]
+    L = load_library()
+    c = FvConfig.from_dict(cfg)
+    out = np.empty((cfg["jnum"] + 4, cfg["inum"] + 4))
+    rc = L.fv_mesh_array(ctypes.byref(c), ids.index(name), _ptr(out))
+    if rc:
+        raise ValueError("fv_mesh_array failed (%d)" % rc)
+    return out
+
+
+class Fosite:
+    """One simulation on the current HIP device (fosite.f90 ``fosite`` type)."""
+
+    def __init__(self, cfg):
+        self.L = load_library()
+        self.cfg = setups.full(cfg)
+        c = FvConfig.from_dict(self.cfg)
+        h = ctypes.c_void_p()
+        rc = self.L.fv_create(ctypes.byref(h), ctypes.byref(c))
+        if rc:
+            raise RuntimeError("fv_create failed (%d); see stderr" % rc)
+        self._h = h
+        ni, nj, v = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self.L.fv_dims(h, ctypes.byref(ni), ctypes.byref(nj), ctypes.byref(v))
+        self.NI, self.NJ, self.V = ni.value, nj.value, v.value
+
+    def _chk(self, rc, what):
+        if rc:
+            raise RuntimeError("%s failed (%d): %s" % (what, rc,
+                               self.L.fv_last_error(self._h).decode()))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.L.fv_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_initial(self, pvar, bccsound=None, fcsound=None, w=None):
+        """InitData: primitive state (V, NJ, NI), isothermal sound speeds,
+        FARGO background velocity (NI,)."""
+        if bccsound is not None:
+            b = np.ascontiguousarray(bccsound, dtype=np.float64)
+            f = np.ascontiguousarray(fcsound, dtype=np.float64)
+            self._chk(self.L.fv_set_sound_speeds(self._h, _ptr(b), _ptr(f)), "fv_set_sound_speeds")
+        if w is not None and self.cfg.get("fargo", 0) == 2:
+            ww = np.ascontiguousarray(w, dtype=np.float64)
+            self._chk(self.L.fv_set_background_velocity(self._h, _ptr(ww)),
+                      "fv_set_background_velocity")
+        p = np.ascontiguousarray(pvar, dtype=np.float64)
+        assert p.shape == (self.V, self.NJ, self.NI), p.shape
+        self._chk(self.L.fv_set_pvar(self._h, _ptr(p)), "fv_set_pvar")
+
+    def first_step(self):
+        b = ctypes.c_int()
+        self._chk(self.L.fv_first_step(self._h, ctypes.byref(b)), "fv_first_step")
+        return bool(b.value)
+
+    def step(self):
+        d = ctypes.c_int()
+        self._chk(self.L.fv_step(self._h, ctypes.byref(d)), "fv_step")
+        return bool(d.value)
+
+    def run(self, nsteps):
+        t = ctypes.c_int()
+        self._chk(self.L.fv_run(self._h, int(nsteps), ctypes.byref(t)), "fv_run")
+        return t.value
+
+    def last_run_ms(self):
+        return self.L.fv_last_run_ms(self._h)
+
+    def state(self):
+        cv = np.empty((self.V, self.NJ, self.NI))
+        pv = np.empty((self.V, self.NJ, self.NI))
+        self._chk(self.L.fv_get_state(self._h, _ptr(cv), _ptr(pv)), "fv_get_state")
+        return cv, pv
+
+    def scalars(self):
+        s = FvScalars()
+        self._chk(self.L.fv_get_scalars(self._h, ctypes.byref(s)), "fv_get_scalars")
+        return {k: getattr(s, k) for k, _ in FvScalars._fields_}

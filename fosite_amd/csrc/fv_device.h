@@ -1,0 +1,84 @@
+// fv_device.h -- data shared by the host control code and the HIP kernels of
+// the finite-volume step.  Plain structs passed by value as kernel arguments.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace fv {
+
+constexpr int GH = 2;      // ghost layers per side (mesh/mesh_base.f90:462)
+constexpr int MAXK = 7;    // RK stages (Dormand-Prince has 7)
+constexpr int TILE_X = 64; // threads per tile row (one wavefront)
+
+enum Physics { EULER_ISOTHERM = 16, EULER = 17 };
+enum Geometry { CARTESIAN = 1, LOGCYLINDRICAL = 31 };
+enum Bc { NO_GRADIENTS = 1, PERIODIC = 2, ABSORBING = 13 };
+enum Limiter { MINMOD = 1, MONOCENT = 2, SWEBY = 3, SUPERBEE = 4, OSPRE = 5, VANLEER = 7, NOLIMIT = 8 };
+
+// Per-ring (x index i, ghost-inclusive) metric and physics arrays.  On the
+// supported geometries every metric quantity depends on i only
+// (mesh/mesh_midpoint.f90:97-145 with geometry_logcylindrical.f90:310-319),
+// except the y reconstruction offsets, which depend on j only.
+enum RingId {
+  R_DAXW, R_DAYS, R_DYDZDV, R_DZDXDV, R_DXDYDV, R_DLX, R_DLY, R_CXYX, R_CYXY, R_CZXZ, R_CZYZ,
+  R_RDXW, R_RDXE, R_ACCX, R_ACCY, R_CS, R_CSW, R_CSE, R_CSS, R_CSN, R_W, R_COUNT
+};
+enum ColId { C_RDYS, C_RDYN, C_COUNT };
+
+struct Geom {
+  int NI, NJ, P;        // ghost-inclusive cells per row / column, row pitch (elements)
+  long FS;              // field stride (elements)
+  int inum, jnum;
+  double invdx, invdy, gamma, pmin, rhomin;
+  int limiter;
+  double theta;
+  int bc[4];
+  int fargo, gravity, curvilinear;
+  const double *ring;   // [R_COUNT][NI]
+  const double *col;    // [C_COUNT][NJ]
+};
+
+// One right-hand-side evaluation (timedisc_base.f90:1073-1271) on the stage
+// state  S = Y - sum_q c[q] * k[q]  (timedisc_rkfehlberg.f90:277-302).
+struct StageArgs {
+  const double *Y;
+  const double *k[MAXK];
+  double c[MAXK];
+  int nk;
+  const double *gsrc;   // ghost cells of the stage state (written by the ghost kernel)
+  double *gdst;
+  double *kout;         // rhs (interior cells)
+  double *yout;         // transformed state (interior), or null
+  int add_bg, sub_bg;   // FARGO frame change at entry / before the fluxes
+  int pmin_clamp;       // Euler CHECK_PMIN round trip (timedisc_base.f90:1150-1164)
+  int want_cfl, want_check;
+  double *part;         // per-block partial results
+  int part_off;
+};
+
+// pointwise final combination + embedded error (timedisc_rkfehlberg.f90:199-266,
+// timedisc_base.f90:1012-1053)
+struct CombineArgs {
+  const double *Y;
+  const double *k[MAXK];
+  double ch[MAXK], cl[MAXK];
+  int nk;
+  double *ynew;
+  double tol_rel, tol_abs[4];
+  double *part;
+};
+
+// partial slot layout: part[2*b] = max(invdt or err), part[2*b+1] = flag bits
+enum Flag { F_INVALID = 1, F_RHOMIN = 2, F_PMIN = 4 };
+
+// kernel launchers (fv_kernels.hip)
+int rhs_blocks(const Geom &g);
+int ghost_blocks(const Geom &g);
+int combine_blocks(const Geom &g);
+void launch_ghost(const Geom &g, const StageArgs &a, int V, hipStream_t s);
+void launch_rhs(const Geom &g, const StageArgs &a, int V, hipStream_t s);
+void launch_combine(const Geom &g, const CombineArgs &a, int V, hipStream_t s);
+void launch_remap(const Geom &g, const double *u, double *z, double dt, int V, hipStream_t s);
+void launch_frame(const Geom &g, const double *src, double *dst, int sign, int V, hipStream_t s);
+void launch_reduce(const double *part, int n, double *out, hipStream_t s);
+
+}  // namespace fv

@@ -1,0 +1,109 @@
+/*
+ * fosite_fv.h -- C ABI of the MI355X finite-volume step (libfosite_fv.so).
+ *
+ * One context = one fosite simulation (Mesh + Physics + Fluxes + Boundary +
+ * Sources + Timedisc) whose state lives on the current HIP device.  All entry
+ * points return 0 on success and a negative code on failure; the message is
+ * available from fv_last_error().  The reference has no C interface: each
+ * entry point below replaces the Fortran type-bound procedure cited beside it
+ * (paths relative to the fosite source tree).
+ *
+ * Arrays cross the boundary in the reference's marray layout
+ * (mesh/marray_base.f90:255-300): ghost cells included (2 per side in x and
+ * y), cell index i fastest, then j, then the field (state vectors) or the
+ * face (W,E,S,N) for face arrays.  Fields are ordered density, x-momentum,
+ * y-momentum[, energy] (conservative) or density, x-velocity, y-velocity
+ * [, pressure] (primitive), physics/physics_euler.f90:2603-2613.
+ */
+#ifndef FOSITE_FV_H
+#define FOSITE_FV_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fv_ctx fv_ctx;
+
+/* Configuration: the Dict keys of a fosite setup program (e.g.
+ * tests/riemann2d.f90:586-670), flattened.  Integer ids are the reference's. */
+typedef struct fv_config {
+  int physics;      /* 16 EULER_ISOTHERM, 17 EULER (physics/physics_base.f90:496-498) */
+  double gamma;     /* physics/physics_euler.f90:152 */
+  int geometry;     /* 1 CARTESIAN, 31 LOGCYLINDRICAL (mesh/geometry_base.f90:345-357) */
+  int inum, jnum;   /* resolution, knum = 1 (mesh/mesh_base.f90:350-352) */
+  double xmin, xmax, ymin, ymax, gparam;
+  int bc[4];        /* western, eastern, southern, northern:
+                       1 NO_GRADIENTS, 2 PERIODIC, 13 ABSORBING (boundary/boundary_base.f90:110-130) */
+  int limiter;      /* 1 minmod, 2 mc, 3 sweby, 4 superbee, 5 ospre, 7 van leer, 8 no limit
+                       (fluxes/reconstruction_linear.f90:71-78) */
+  double theta;     /* limiter parameter for mc and sweby */
+  int recvars;      /* 2 CONSERVATIVE (1 PRIMITIVE is rejected) (fluxes/reconstruction_base.f90:117) */
+  int method, order;/* 2 RK_FEHLBERG, 4 DORMAND_PRINCE, 5 SSPRK (timedisc/timedisc_base.f90:187-191) */
+  double cfl, tol_rel, tol_abs[4], stoptime, dtlimit, beta; /* timedisc_base.f90:315-352;
+                       dtlimit <= 0 selects the default EPSILON*stoptime */
+  int fargo;        /* 0 off, 2 user-supplied background velocity (mesh/mesh_base.f90:372-423) */
+  int gravity;      /* 1: point mass at the origin, Newtonian, no accretion
+                       (sources/gravity_pointmass.f90:132-301 with outbound = 0) */
+  double mass, GN;  /* point mass and gravitational constant (GEOMETRICAL units: GN = 1) */
+  int dtype;        /* 0 = FP64 (the plain drop-in) */
+} fv_config;
+
+typedef struct fv_scalars {
+  int iter, n_adj, brk, dtcause;
+  double time, dt, dtold, maxerrold;
+} fv_scalars;
+
+/* Mesh metric query, host only (no context, no device): the array the
+ * reference's Mesh holds for configuration `cfg`, all cells incl. ghosts.
+ * Ids: 0 dAxdydz(W) 1 dAxdydz(E) 2 dAydzdx(S) 3 dAydzdx(N) 4 dydzdV 5 dzdxdV
+ * 6 dxdydV 7 dlx 8 dly 9 cxyx 10 cyxy 11 czxz 12 czyz 13..16 reconstruction
+ * offsets W,E,S,N 17 radius(bcenter) 18..21 radius(faces W,E,S,N)
+ * 22 posvec_x 23 posvec_y 24 bcenter x 25 bcenter y 26 accel_x 27 accel_y.
+ * Replaces mesh/mesh_midpoint.f90:73-146 (InitMesh_midpoint) and
+ * sources/gravity_pointmass.f90:263-282 (static acceleration). */
+int fv_mesh_array(const fv_config *cfg, int id, double *out);
+
+/* fosite%Setup (fosite.f90:152-255): validates the configuration, builds the
+ * mesh metric on the host and allocates the device-resident state. */
+int fv_create(fv_ctx **ctx, const fv_config *cfg);
+/* fosite%Finalize (fosite.f90:680-691) */
+int fv_destroy(fv_ctx *ctx);
+/* ghost-inclusive cell counts and number of conservative fields */
+int fv_dims(const fv_ctx *ctx, int *ni, int *nj, int *vnum);
+
+/* physics_eulerisotherm%SetSoundSpeeds_center / _faces
+ * (physics/physics_eulerisotherm.f90:288-309); isothermal physics only.
+ * bccsound: cells; fcsound: cells x 4 faces. */
+int fv_set_sound_speeds(fv_ctx *ctx, const double *bccsound, const double *fcsound);
+/* Timedisc%w for FARGO type 2 (timedisc/timedisc_base.f90:389,420):
+ * one value per x-ring, IGMIN..IGMAX. */
+int fv_set_background_velocity(fv_ctx *ctx, const double *w);
+/* InitData: Timedisc%pvar = pvar, then Physics%Convert2Conservative
+ * (physics/physics_eulerisotherm.f90:426-449). */
+int fv_set_pvar(fv_ctx *ctx, const double *pvar);
+
+/* fosite%FirstStep (fosite.f90:258-348): full RHS with data checks, CFL
+ * time step, initial output frame change.  Returns the break flag in *brk. */
+int fv_first_step(fv_ctx *ctx, int *brk);
+/* fosite%Step (fosite.f90:367-464): stop-time test, output-time clipping
+ * (io/fileio_base.f90:604-620), Timedisc%IntegrationStep
+ * (timedisc/timedisc_base.f90:649-750), next CFL time step
+ * (timedisc/timedisc_base.f90:818-909).  *done = 1 when the run ended
+ * (stop time reached or break). */
+int fv_step(fv_ctx *ctx, int *done);
+/* nsteps x fv_step; *taken = steps executed before the run ended */
+int fv_run(fv_ctx *ctx, int nsteps, int *taken);
+
+/* Device -> host copy of Timedisc%cvar (all cells) and of the primitive
+ * variables Convert2Primitive(cvar) (either pointer may be NULL). */
+int fv_get_state(fv_ctx *ctx, double *cvar, double *pvar);
+int fv_get_scalars(const fv_ctx *ctx, fv_scalars *out);
+/* wall time of the device work issued by the last fv_run, in ms
+ * (HIP events on the context's stream) */
+double fv_last_run_ms(const fv_ctx *ctx);
+const char *fv_last_error(const fv_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
