@@ -65,7 +65,8 @@ class FvScalars(ctypes.Structure):
 # every entry point of include/fosite_fv.h
 EXPORTS = ["fv_mesh_array", "fv_create", "fv_destroy", "fv_dims", "fv_set_sound_speeds",
            "fv_set_background_velocity", "fv_set_pvar", "fv_first_step", "fv_step", "fv_run",
-           "fv_get_state", "fv_get_scalars", "fv_last_run_ms", "fv_last_error"]
+           "fv_get_state", "fv_get_scalars", "fv_last_run_ms", "fv_last_error", "fv_profile",
+           "fv_kernel_stats"]
 
 _lib = None
 
@@ -98,6 +99,9 @@ def load_library():
     L.fv_last_run_ms.restype = ctypes.c_double
     L.fv_last_error.argtypes = [V]
     L.fv_last_error.restype = ctypes.c_char_p
+    L.fv_profile.argtypes = [V, ctypes.c_int]
+    L.fv_kernel_stats.argtypes = [V, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                  ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
     _lib = L
     return L
 
@@ -182,6 +186,17 @@ class Fosite:
 
     def last_run_ms(self):
         return self.L.fv_last_run_ms(self._h)
+
+    def profile(self, enable=True):
+        self._chk(self.L.fv_profile(self._h, int(enable)), "fv_profile")
+
+    KERNELS = {"rhs": 0, "rhs_remap": 1, "combine": 2, "frame": 3, "control": 4}
+
+    def kernel_stats(self, kernel):
+        n, ms, b = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+        self._chk(self.L.fv_kernel_stats(self._h, self.KERNELS[kernel], ctypes.byref(n),
+                                         ctypes.byref(ms), ctypes.byref(b)), "fv_kernel_stats")
+        return n.value, ms.value, b.value
 
     def state(self):
         cv = np.empty((self.V, self.NJ, self.NI))

@@ -11,6 +11,8 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <cstddef>
+#include <cstdlib>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -25,6 +27,8 @@ using namespace fv;
 namespace {
 
 constexpr int NBUF = 4;
+// kernel classes of the timing statistics (fv_kernel_stats)
+constexpr int K_RHS = 0, K_RHS_REMAP = 1, K_COMBINE = 2, K_FRAME = 3, K_CONTROL = 4, K_CLASSES = 5;
 // timedisc_base.f90:193-203
 constexpr int CHECK_NOTHING = 0, CHECK_ALL = ~0, CHECK_CSOUND = 1, CHECK_PMIN = 2;
 constexpr int DTCAUSE_ERRADJ = -1, DTCAUSE_SMALLERR = -2, DTCAUSE_FILEIO = -4;
@@ -103,10 +107,17 @@ struct fv_ctx {
   double *d_ring = nullptr, *d_col = nullptr;
   double *buf[NBUF] = {};
   int cvar = 0, cold = 0;
-  double *W = nullptr;
-  double *k[MAXK] = {};
-  double *part = nullptr, *d_red = nullptr, *h_red = nullptr;
-  int nb_rhs = 0, nb_ghost = 0, nb_comb = 0;
+  double *k[MAXK] = {};     // stage rhs k_2..k_s (k[0] unused)
+  double *k0[2] = {};       // coeff(1) = Timedisc%rhs, double-buffered for rollback
+  int k0cur = 0;
+  Ctl *d_ctl = nullptr, *h_ctl = nullptr;
+  struct GraphEntry { int x0, x1, t, k0, nsteps; hipGraphExec_t exec; };
+  std::vector<GraphEntry> graphs;
+  bool fast = true;         // device-controlled batches allowed
+  int batch_steps = 0, batch_halts = 0;
+  double *part_r = nullptr, *part_c = nullptr;  // per-block partials of k_rhs / k_combine
+  double *d_red = nullptr, *h_red = nullptr;
+  int nb_rhs = 0, nb_comb = 0;
   hipStream_t st = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   Geom g{};
@@ -119,6 +130,14 @@ struct fv_ctx {
   double last_ms = 0;
   double invdt = 0;
   int flags = 0;
+  // per-kernel timing (HIP events around each launch on the context stream)
+  bool prof = false;
+  std::vector<hipEvent_t> evpool;
+  size_t evused = 0;
+  struct Pending { int kind; size_t e; double bytes; };
+  std::vector<Pending> pend;
+  int st_launch[K_CLASSES] = {};
+  double st_ms[K_CLASSES] = {}, st_bytes[K_CLASSES] = {};
 };
 
 #define HIPCHK(x)                                                              \
@@ -143,12 +162,47 @@ int pick_free(const fv_ctx *ctx, int a, int b, int c) {
   return -1;
 }
 
+// kernel timing: returns the event slot recorded before a launch, or -1
+long prof_begin(fv_ctx *ctx) {
+  if (!ctx->prof) return -1;
+  while (ctx->evused + 2 > ctx->evpool.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return -1;
+    ctx->evpool.push_back(e);
+  }
+  size_t k = ctx->evused;
+  ctx->evused += 2;
+  (void)hipEventRecord(ctx->evpool[k], ctx->st);
+  return (long)k;
+}
+void prof_end(fv_ctx *ctx, long k, int kind, double bytes) {
+  if (k < 0) return;
+  (void)hipEventRecord(ctx->evpool[k + 1], ctx->st);
+  ctx->pend.push_back({kind, (size_t)k, bytes});
+}
+// after a synchronisation: fold the recorded intervals into the statistics
+void prof_collect(fv_ctx *ctx) {
+  for (auto &p : ctx->pend) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, ctx->evpool[p.e], ctx->evpool[p.e + 1]) == hipSuccess) {
+      ctx->st_launch[p.kind]++;
+      ctx->st_ms[p.kind] += ms;
+      ctx->st_bytes[p.kind] += p.bytes;
+    }
+  }
+  ctx->pend.clear();
+  ctx->evused = 0;
+}
+double cells(const fv_ctx *ctx) { return (double)ctx->c.inum * ctx->c.jnum; }
+double word_bytes(const fv_ctx *ctx) { return 8.0 * ctx->V * cells(ctx); }
+
 // reduce the per-block partials and fetch {max, flags} to the host
-int reduce(fv_ctx *ctx, int n, double *mx, int *flags) {
-  launch_reduce(ctx->part, n, ctx->d_red, ctx->st);
+int reduce(fv_ctx *ctx, const double *part, int n, double *mx, int *flags) {
+  launch_reduce(part, n, ctx->d_red, ctx->st);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(ctx->h_red, ctx->d_red, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->st));
   HIPCHK(hipStreamSynchronize(ctx->st));
+  prof_collect(ctx);
   *mx = ctx->h_red[0];
   *flags = (int)ctx->h_red[1];
   return 0;
@@ -157,48 +211,52 @@ int reduce(fv_ctx *ctx, int n, double *mx, int *flags) {
 StageArgs base_args(fv_ctx *ctx) {
   StageArgs a;
   std::memset(&a, 0, sizeof(a));
-  a.part = ctx->part;
+  a.part = ctx->part_r;
   return a;
 }
 
-// ComputeRHS (timedisc_base.f90:1073-1271) on the main state (pvar, cvar):
-// writes coeff(1) and, when the frame/PMIN round trip changes cvar, the new
-// cvar into a fresh buffer.  Also yields the CFL inverse time step and the
-// CheckData flags (timedisc_base.f90:1666-1742).
-int rhs_main(fv_ctx *ctx, int checkbm) {
-  const bool fargo = ctx->c.fargo == 2;
-  int add = 0, sub = 0;
-  if (fargo && ctx->fl_cvar && ctx->fl_pvar) {
+// FARGO frame flags of one ComputeRHS call (timedisc_base.f90:1098-1208):
+// returns add/sub and updates the flags of (p, c) like the reference does
+void rhs_flags(const fv_ctx *ctx, int &fp, int &fc, int &add, int &sub) {
+  add = sub = 0;
+  if (ctx->c.fargo != 2) return;
+  if (fc && fp) {
     add = 1;
-    ctx->fl_cvar = ctx->fl_pvar = 0;
+    fc = fp = 0;
   }
-  ctx->fl_pvar = ctx->fl_cvar;  // CenterBoundary
-  const int pmin = (ctx->V == 4 && (checkbm & CHECK_PMIN)) ? 1 : 0;
-  if (fargo && !ctx->fl_cvar && !ctx->fl_pvar) {
+  fp = fc;  // CenterBoundary
+  if (!fc && !fp) {
     sub = 1;
-    ctx->fl_cvar = ctx->fl_pvar = 1;
+    fc = fp = 1;
   }
-  const bool ident = !add && !sub && !pmin;
-  const int outb = ident ? ctx->cvar : pick_free(ctx, ctx->cvar, ctx->cold, -1);
+}
+
+// ComputeRHS on the main state (pvar, cvar) -> coeff(1) and the new cvar
+// (fresh buffer, ghost cells included); also the CFL inverse time step and
+// the CheckData flags (timedisc_base.f90:1666-1742).  remap = 1 evaluates it
+// on FargoAdvectionY(cvar) (the post-step remap, timedisc_base.f90:722-743).
+int rhs_main(fv_ctx *ctx, int checkbm, int remap) {
+  int add, sub;
+  rhs_flags(ctx, ctx->fl_pvar, ctx->fl_cvar, add, sub);
+  const int pmin = (ctx->V == 4 && (checkbm & CHECK_PMIN)) ? 1 : 0;
+  const int outb = pick_free(ctx, ctx->cvar, ctx->cold, -1);
   StageArgs a = base_args(ctx);
   a.Y = ctx->buf[ctx->cvar];
   a.nk = 0;
-  a.gsrc = a.gdst = ctx->buf[outb];
-  a.kout = ctx->k[0];
-  a.yout = ident ? nullptr : ctx->buf[outb];
+  a.dt_step = ctx->dt;
+  a.kout = ctx->k0[ctx->k0cur];
+  a.yout = ctx->buf[outb];
   a.add_bg = add;
   a.sub_bg = sub;
   a.pmin_clamp = pmin;
   a.want_cfl = 1;
   a.want_check = checkbm != CHECK_NOTHING;
-  a.part_off = ctx->nb_rhs;
-  launch_ghost(ctx->g, a, ctx->V, ctx->st);
-  HIPCHK(hipGetLastError());
-  a.part_off = 0;
-  launch_rhs(ctx->g, a, ctx->V, ctx->st);
+  long pk = prof_begin(ctx);
+  launch_rhs(ctx->g, a, ctx->V, remap ? IN_REMAP : IN_STAGE, ctx->st);
+  prof_end(ctx, pk, remap ? K_RHS_REMAP : K_RHS, 3.0 * word_bytes(ctx));
   HIPCHK(hipGetLastError());
   ctx->cvar = outb;
-  int rc = reduce(ctx, ctx->nb_rhs + ctx->nb_ghost, &ctx->invdt, &ctx->flags);
+  int rc = reduce(ctx, ctx->part_r, ctx->nb_rhs, &ctx->invdt, &ctx->flags);
   if (rc) return rc;
   if (checkbm != CHECK_NOTHING) {
     if ((checkbm & CHECK_CSOUND) && ctx->V == 3 && ctx->cs_min <= 0.) ctx->brk = 1;
@@ -207,33 +265,24 @@ int rhs_main(fv_ctx *ctx, int checkbm) {
   return 0;
 }
 
-// ComputeRHS on the stage state (ptmp, ctmp) with ctmp = cvar - sum a*dt*k
+// ComputeRHS on the stage state (ptmp, ctmp), ctmp = cvar - sum a*dt*k
 int rhs_stage(fv_ctx *ctx, int m, double dt) {
-  const bool fargo = ctx->c.fargo == 2;
-  int add = 0, sub = 0;
-  if (fargo && ctx->fl_ctmp && ctx->fl_ptmp) {
-    add = 1;
-    ctx->fl_ctmp = ctx->fl_ptmp = 0;
-  }
-  ctx->fl_ptmp = ctx->fl_ctmp;
-  if (fargo && !ctx->fl_ctmp && !ctx->fl_ptmp) {
-    sub = 1;
-    ctx->fl_ctmp = ctx->fl_ptmp = 1;
-  }
+  int add, sub;
+  rhs_flags(ctx, ctx->fl_ptmp, ctx->fl_ctmp, add, sub);
   StageArgs a = base_args(ctx);
   a.Y = ctx->buf[ctx->cvar];
   a.nk = m;
   for (int q = 0; q < m; q++) {
-    a.k[q] = ctx->k[q];
-    a.c[q] = ctx->tab.a[m][q] * dt;
+    a.k[q] = q ? ctx->k[q] : ctx->k0[ctx->k0cur];
+    a.a[q] = ctx->tab.a[m][q];
   }
-  a.gsrc = a.gdst = ctx->W;
+  a.dt = dt;
   a.kout = ctx->k[m];
   a.add_bg = add;
   a.sub_bg = sub;
-  launch_ghost(ctx->g, a, ctx->V, ctx->st);
-  HIPCHK(hipGetLastError());
-  launch_rhs(ctx->g, a, ctx->V, ctx->st);
+  long pk = prof_begin(ctx);
+  launch_rhs(ctx->g, a, ctx->V, IN_STAGE, ctx->st);
+  prof_end(ctx, pk, K_RHS, (m + 2.0) * word_bytes(ctx));
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -261,6 +310,23 @@ double adjust_timestep(fv_ctx *ctx, double maxerr, double dtold) {
   return dtnew;
 }
 
+CombineArgs combine_args(fv_ctx *ctx, int y, int k0, int out) {
+  CombineArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.Y = ctx->buf[y];
+  a.nk = ctx->tab.m;
+  for (int q = 0; q < ctx->tab.m; q++) {
+    a.k[q] = q ? ctx->k[q] : ctx->k0[k0];
+    a.bh[q] = ctx->tab.bh[q];
+    a.bl[q] = ctx->tab.bl[q];
+  }
+  a.ynew = ctx->buf[out];
+  a.tol_rel = ctx->c.tol_rel;
+  for (int l = 0; l < 4; l++) a.tol_abs[l] = ctx->c.tol_abs[l];
+  a.part = ctx->part_c;
+  return a;
+}
+
 // timedisc_rkfehlberg.f90:170-268 (boundary-flux accounting omitted: it only
 // feeds diagnostics and accretion, neither enabled)
 int solve_ode(fv_ctx *ctx, double *dt, double *err) {
@@ -270,25 +336,16 @@ int solve_ode(fv_ctx *ctx, double *dt, double *err) {
   }
   ctx->fl_ctmp = ctx->fl_cvar;  // ctmp = cvar
   const int outb = pick_free(ctx, ctx->cvar, ctx->cold, -1);
-  CombineArgs a;
-  std::memset(&a, 0, sizeof(a));
-  a.Y = ctx->buf[ctx->cvar];
-  a.nk = ctx->tab.m;
-  for (int q = 0; q < ctx->tab.m; q++) {
-    a.k[q] = ctx->k[q];
-    a.ch[q] = ctx->tab.bh[q] * *dt;
-    a.cl[q] = ctx->tab.bl[q] * *dt;
-  }
-  a.ynew = ctx->buf[outb];
-  a.tol_rel = ctx->c.tol_rel;
-  for (int l = 0; l < 4; l++) a.tol_abs[l] = ctx->c.tol_abs[l];
-  a.part = ctx->part;
+  CombineArgs a = combine_args(ctx, ctx->cvar, ctx->k0cur, outb);
+  a.dt = *dt;
+  long pk = prof_begin(ctx);
   launch_combine(ctx->g, a, ctx->V, ctx->st);
+  prof_end(ctx, pk, K_COMBINE, (ctx->tab.m + 2.0) * word_bytes(ctx));
   HIPCHK(hipGetLastError());
   ctx->cvar = outb;
   double e;
   int f;
-  int rc = reduce(ctx, ctx->nb_comb, &e, &f);
+  int rc = reduce(ctx, ctx->part_c, ctx->nb_comb, &e, &f);
   if (rc) return rc;
   if (ctx->c.tol_rel <= 1.0) {
     *err = e;
@@ -305,7 +362,9 @@ double calc_timestep(fv_ctx *ctx) {
 // out-of-place frame change of cvar (Physics%Add/SubtractBackgroundVelocityY)
 int frame_cvar(fv_ctx *ctx, int sign) {
   const int outb = pick_free(ctx, ctx->cvar, ctx->cold, -1);
+  long pk = prof_begin(ctx);
   launch_frame(ctx->g, ctx->buf[ctx->cvar], ctx->buf[outb], sign, ctx->V, ctx->st);
+  prof_end(ctx, pk, K_FRAME, 2.0 * 8.0 * ctx->V * (double)ctx->NI * ctx->NJ);
   HIPCHK(hipGetLastError());
   ctx->cvar = outb;
   return 0;
@@ -339,14 +398,14 @@ int integration_step(fv_ctx *ctx) {
     } else if (err >= 1.0) {  // RejectSolution (timedisc_base.f90:961-1009)
       ctx->cvar = ctx->cold;
       ctx->fl_cvar = ctx->fl_cold;
-      if ((rc = rhs_main(ctx, CHECK_NOTHING))) return rc;
+      if ((rc = rhs_main(ctx, CHECK_NOTHING, 0))) return rc;
       ctx->n_adj++;
       ctx->dtcause = DTCAUSE_ERRADJ;
       if (dt < ctx->dtmin) ctx->dtmin = dt;
     } else {  // AcceptSolution (timedisc_base.f90:911-959)
       time = time + dtold;
       if (time < ctx->time + ctx->dt) {
-        if ((rc = rhs_main(ctx, CHECK_ALL))) return rc;
+        if ((rc = rhs_main(ctx, CHECK_ALL, 0))) return rc;
         ctx->cold = ctx->cvar;
         ctx->fl_cold = ctx->fl_cvar;
       }
@@ -358,22 +417,12 @@ int integration_step(fv_ctx *ctx) {
     ctx->dt = time - ctx->time;
     ctx->time = time;
     ctx->dtold = dt;
-    if (ctx->c.fargo == 2) {
-      // CenterBoundary in the comoving frame, then FargoAdvectionY
-      // (timedisc_base.f90:722-743, 1935-2069)
-      StageArgs a = base_args(ctx);
-      a.Y = ctx->buf[ctx->cvar];
-      a.gsrc = a.gdst = ctx->buf[ctx->cvar];
-      ctx->fl_pvar = ctx->fl_cvar;
-      launch_ghost(ctx->g, a, ctx->V, ctx->st);
-      HIPCHK(hipGetLastError());
-      const int outb = pick_free(ctx, ctx->cvar, ctx->cold, -1);
-      launch_remap(ctx->g, ctx->buf[ctx->cvar], ctx->buf[outb], ctx->dt, ctx->V, ctx->st);
-      HIPCHK(hipGetLastError());
-      ctx->cvar = outb;
-      ctx->fl_pvar = ctx->fl_cvar;  // Convert2Primitive(cvar, pvar)
-    }
-    if ((rc = rhs_main(ctx, CHECK_ALL))) return rc;
+    const int remap = ctx->c.fargo == 2;
+    // FARGO: CenterBoundary in the comoving frame, FargoAdvectionY and
+    // Convert2Primitive (timedisc_base.f90:722-741) are folded into the
+    // end-of-step RHS evaluation, which reads the remapped state on the fly
+    if (remap) ctx->fl_pvar = ctx->fl_cvar;
+    if ((rc = rhs_main(ctx, CHECK_ALL, remap))) return rc;
     ctx->cold = ctx->cvar;
     ctx->fl_cold = ctx->fl_cvar;
   }
@@ -394,6 +443,149 @@ bool ring_from(fv_ctx *ctx, int id, const double *cells) {
     if (std::memcmp(cells + (size_t)j * NI, cells, sizeof(double) * NI)) return false;
   std::memcpy(&ctx->ring_h[(size_t)id * NI], cells, sizeof(double) * NI);
   return true;
+}
+
+// ------------------------------------------------- device-controlled batches
+// In the steady state of the integration (frame flags settled, cold = cvar,
+// no pending break), a step that is accepted at the first attempt needs no
+// host decision: the device runs stages, combination, accept test, end-of-step
+// RHS and CFL update, and a 2-step HIP graph (buffers alternate between two
+// state arrays) is launched repeatedly.  Steps that leave this path halt the
+// batch and are redone by the host path above.
+bool fast_eligible(const fv_ctx *ctx) {
+  if (!ctx->fast || ctx->iter < 1 || ctx->brk || ctx->cvar != ctx->cold) return false;
+  if (!(ctx->c.tol_rel < 1.0)) return false;  // AdjustTimestep active
+  // the stop test of the next Step() is the host's (fosite.f90:370-373)
+  if (std::fabs(ctx->c.stoptime - ctx->time) <= 1.0E-05 * ctx->c.stoptime) return false;
+  const int f = ctx->c.fargo == 2 ? 1 : 0;
+  return ctx->fl_cvar == f && ctx->fl_pvar == f && ctx->fl_ctmp == f && ctx->fl_ptmp == f &&
+         ctx->fl_cold == f;
+}
+
+// enqueue one device-controlled step: input state x_in/k0_in -> x_out/k0_out
+void enqueue_fast_step(fv_ctx *ctx, int xin, int xout, int tb, int k0in, int k0out) {
+  const int steady = ctx->c.fargo == 2 ? 1 : 0;
+  for (int m = 1; m < ctx->tab.m; m++) {
+    StageArgs a = base_args(ctx);
+    a.Y = ctx->buf[xin];
+    a.nk = m;
+    for (int q = 0; q < m; q++) {
+      a.k[q] = q ? ctx->k[q] : ctx->k0[k0in];
+      a.a[q] = ctx->tab.a[m][q];
+    }
+    a.ctl = ctx->d_ctl;
+    a.kout = ctx->k[m];
+    a.add_bg = a.sub_bg = steady;
+    long pk = prof_begin(ctx);
+    launch_rhs(ctx->g, a, ctx->V, IN_STAGE, ctx->st);
+    prof_end(ctx, pk, K_RHS, (m + 2.0) * word_bytes(ctx));
+  }
+  CombineArgs c = combine_args(ctx, xin, k0in, tb);
+  c.ctl = ctx->d_ctl;
+  long pk = prof_begin(ctx);
+  launch_combine(ctx->g, c, ctx->V, ctx->st);
+  prof_end(ctx, pk, K_COMBINE, (ctx->tab.m + 2.0) * word_bytes(ctx));
+  StageArgs a = base_args(ctx);
+  a.Y = ctx->buf[tb];
+  a.nk = 0;
+  a.ctl = ctx->d_ctl;
+  a.end_of_step = 1;
+  a.kout = ctx->k0[k0out];
+  a.yout = ctx->buf[xout];
+  a.add_bg = a.sub_bg = steady;
+  a.pmin_clamp = ctx->V == 4 ? 1 : 0;
+  a.want_cfl = a.want_check = 1;
+  const int remap = ctx->c.fargo == 2;
+  pk = prof_begin(ctx);
+  launch_rhs(ctx->g, a, ctx->V, remap ? IN_REMAP : IN_STAGE, ctx->st);
+  prof_end(ctx, pk, remap ? K_RHS_REMAP : K_RHS, 3.0 * word_bytes(ctx));
+  pk = prof_begin(ctx);
+  launch_control(ctx->d_ctl, ctx->part_c, ctx->nb_comb, ctx->part_r, ctx->nb_rhs, ctx->st);
+  prof_end(ctx, pk, K_CONTROL, 16.0 * (ctx->nb_comb + ctx->nb_rhs));
+}
+
+int get_graph(fv_ctx *ctx, int x0, int x1, int tb, int k0, hipGraphExec_t *out) {
+  for (auto &e : ctx->graphs)
+    if (e.x0 == x0 && e.x1 == x1 && e.t == tb && e.k0 == k0) {
+      *out = e.exec;
+      return 0;
+    }
+  hipGraph_t graph;
+  HIPCHK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
+  enqueue_fast_step(ctx, x0, x1, tb, k0, 1 - k0);
+  enqueue_fast_step(ctx, x1, x0, tb, 1 - k0, k0);
+  HIPCHK(hipStreamEndCapture(ctx->st, &graph));
+  hipGraphExec_t exec;
+  HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  (void)hipGraphDestroy(graph);
+  ctx->graphs.push_back({x0, x1, tb, k0, 2, exec});
+  *out = exec;
+  return 0;
+}
+
+// runs up to nmax steps on the device; *done_steps = steps completed there,
+// *halted = 1 if the batch stopped on a step the host must redo
+int run_batch(fv_ctx *ctx, int nmax, int *done_steps, int *halted) {
+  *done_steps = 0;
+  *halted = 0;
+  const int x0 = ctx->cvar;
+  const int x1 = pick_free(ctx, x0, -1, -1);
+  const int tb = pick_free(ctx, x0, x1, -1);
+  const int k0 = ctx->k0cur;
+  Ctl *h = ctx->h_ctl;
+  std::memset(h, 0, offsetof(Ctl, err_hist));
+  h->time = ctx->time;
+  h->dt = ctx->dt;
+  h->t_out = ctx->t_out;
+  h->stoptime = ctx->c.stoptime;
+  h->cfl = ctx->c.cfl;
+  h->dtlimit = ctx->dtlimit;
+  h->maxerrold = ctx->maxerrold;
+  h->beta = ctx->c.beta;
+  h->order = ctx->c.order;
+  h->nsteps = nmax;
+  hipGraphExec_t exec = nullptr;
+  if (!ctx->prof) {
+    int rc = get_graph(ctx, x0, x1, tb, k0, &exec);
+    if (rc) return rc;
+  }
+  HIPCHK(hipMemcpyAsync(ctx->d_ctl, h, offsetof(Ctl, err_hist), hipMemcpyHostToDevice, ctx->st));
+  if (exec) {
+    for (int s = 0; s < nmax; s += 2) HIPCHK(hipGraphLaunch(exec, ctx->st));
+  } else {  // timed: the same launches, issued one by one between events
+    for (int s = 0; s < nmax; s++)
+      enqueue_fast_step(ctx, s % 2 ? x1 : x0, s % 2 ? x0 : x1, tb, s % 2 ? 1 - k0 : k0, s % 2 ? k0 : 1 - k0);
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipMemcpyAsync(h, ctx->d_ctl, sizeof(Ctl), hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  prof_collect(ctx);
+  const int k = h->step;
+  // replay the host scalars of the k accepted steps exactly as the host path
+  // would have computed them (fosite.f90:367-464, timedisc_base.f90:649-750)
+  for (int s = 0; s < k; s++) {
+    const double dts = h->dt_hist[s];
+    if (dts < ctx->dtmin && ctx->dtcause != DTCAUSE_FILEIO) ctx->dtmin = dts;
+    const double dtnew = adjust_timestep(ctx, h->err_hist[s], dts);
+    const double t1 = ctx->time + dts;
+    ctx->iter++;
+    if (dtnew < ctx->dtlimit) return fail(ctx, "device step control missed a dtlimit break", -4);
+    ctx->dt = t1 - ctx->time;
+    ctx->time = t1;
+    ctx->dtold = dtnew;
+    ctx->dtcause = 0;
+    ctx->cvar = ctx->cold = (s % 2 == 0) ? x1 : x0;
+    ctx->k0cur = 1 - ctx->k0cur;
+  }
+  if (k > 0) {  // CalcTimestep of the last accepted step (before output clipping)
+    ctx->invdt = h->invdt;
+    ctx->dt = calc_timestep(ctx);
+  }
+  *halted = h->reason != H_BATCH;
+  ctx->batch_steps += k;
+  ctx->batch_halts += *halted;
+  *done_steps = k;
+  return 0;
 }
 
 }  // namespace
@@ -457,14 +649,16 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
       if (ctx->mesh.a[M_RDYS][(size_t)j * ctx->NI + i] != ctx->col_h[(size_t)C_RDYS * ctx->NJ + j])
         return bail("reconstruction offsets in y depend on i");
   }
+  if (const char *e = std::getenv("FV_RHS_ROWS")) set_rhs_rows(std::atoi(e));
+  if (const char *e = std::getenv("FV_HOST_CONTROL")) ctx->fast = std::atoi(e) == 0;
   // device resources
   const size_t sbytes = sizeof(double) * ctx->V * ctx->FS;
   auto dalloc = [&](double **p, size_t b) { return hipMalloc((void **)p, b) == hipSuccess; };
   bool ok = hipStreamCreate(&ctx->st) == hipSuccess && hipEventCreate(&ctx->e0) == hipSuccess &&
             hipEventCreate(&ctx->e1) == hipSuccess;
   for (int q = 0; q < NBUF && ok; q++) ok = dalloc(&ctx->buf[q], sbytes) && hipMemset(ctx->buf[q], 0, sbytes) == hipSuccess;
-  ok = ok && dalloc(&ctx->W, sbytes) && hipMemset(ctx->W, 0, sbytes) == hipSuccess;
-  for (int q = 0; q < ctx->tab.m && ok; q++) ok = dalloc(&ctx->k[q], sbytes) && hipMemset(ctx->k[q], 0, sbytes) == hipSuccess;
+  for (int q = 1; q < ctx->tab.m && ok; q++) ok = dalloc(&ctx->k[q], sbytes) && hipMemset(ctx->k[q], 0, sbytes) == hipSuccess;
+  for (int q = 0; q < 2 && ok; q++) ok = dalloc(&ctx->k0[q], sbytes) && hipMemset(ctx->k0[q], 0, sbytes) == hipSuccess;
   ok = ok && dalloc(&ctx->d_ring, sizeof(double) * R_COUNT * ctx->NI) && dalloc(&ctx->d_col, sizeof(double) * C_COUNT * ctx->NJ);
   Geom &g = ctx->g;
   g.NI = ctx->NI; g.NJ = ctx->NJ; g.P = ctx->P; g.FS = ctx->FS;
@@ -475,11 +669,12 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
   for (int d = 0; d < 4; d++) g.bc[d] = c.bc[d];
   g.fargo = c.fargo == 2; g.gravity = c.gravity != 0; g.curvilinear = c.geometry != CARTESIAN;
   ctx->nb_rhs = rhs_blocks(g);
-  ctx->nb_ghost = ghost_blocks(g);
   ctx->nb_comb = combine_blocks(g);
-  const int np = std::max(ctx->nb_rhs + ctx->nb_ghost, ctx->nb_comb);
-  ok = ok && dalloc(&ctx->part, sizeof(double) * 2 * np) && dalloc(&ctx->d_red, sizeof(double) * 2);
+  ok = ok && dalloc(&ctx->part_r, sizeof(double) * 2 * ctx->nb_rhs) &&
+       dalloc(&ctx->part_c, sizeof(double) * 2 * ctx->nb_comb) && dalloc(&ctx->d_red, sizeof(double) * 2);
+  ok = ok && hipMalloc((void **)&ctx->d_ctl, sizeof(Ctl)) == hipSuccess && hipMemset(ctx->d_ctl, 0, sizeof(Ctl)) == hipSuccess;
   ok = ok && hipHostMalloc((void **)&ctx->h_red, sizeof(double) * 2) == hipSuccess;
+  ok = ok && hipHostMalloc((void **)&ctx->h_ctl, sizeof(Ctl)) == hipSuccess;
   if (!ok) {
     fv_destroy(ctx);
     std::fprintf(stderr, "fv_create: device allocation failed\n");
@@ -501,14 +696,20 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
 
 extern "C" int fv_destroy(fv_ctx *ctx) {
   if (!ctx) return 0;
+  if (ctx->st) (void)hipStreamSynchronize(ctx->st);
+  for (auto &e : ctx->graphs) (void)hipGraphExecDestroy(e.exec);
   for (auto &b : ctx->buf) if (b) (void)hipFree(b);
   for (auto &b : ctx->k) if (b) (void)hipFree(b);
-  if (ctx->W) (void)hipFree(ctx->W);
-  if (ctx->part) (void)hipFree(ctx->part);
+  for (auto &b : ctx->k0) if (b) (void)hipFree(b);
+  if (ctx->part_r) (void)hipFree(ctx->part_r);
+  if (ctx->part_c) (void)hipFree(ctx->part_c);
   if (ctx->d_red) (void)hipFree(ctx->d_red);
+  if (ctx->d_ctl) (void)hipFree(ctx->d_ctl);
   if (ctx->d_ring) (void)hipFree(ctx->d_ring);
   if (ctx->d_col) (void)hipFree(ctx->d_col);
   if (ctx->h_red) (void)hipHostFree(ctx->h_red);
+  if (ctx->h_ctl) (void)hipHostFree(ctx->h_ctl);
+  for (auto e : ctx->evpool) (void)hipEventDestroy(e);
   if (ctx->e0) (void)hipEventDestroy(ctx->e0);
   if (ctx->e1) (void)hipEventDestroy(ctx->e1);
   if (ctx->st) (void)hipStreamDestroy(ctx->st);
@@ -575,7 +776,7 @@ extern "C" int fv_first_step(fv_ctx *ctx, int *brk) {
   if (ctx->c.fargo == 2 && !ctx->have_w) return fail(ctx, "fargo = 2 needs the background velocity");
   if (ctx->iter > -1) return fail(ctx, "FirstStep should only be called once");
   ctx->iter = 0;
-  int rc = rhs_main(ctx, CHECK_ALL);
+  int rc = rhs_main(ctx, CHECK_ALL, 0);
   if (rc) return rc;
   ctx->dt = calc_timestep(ctx);
   ctx->cold = ctx->cvar;
@@ -622,14 +823,23 @@ extern "C" int fv_run(fv_ctx *ctx, int nsteps, int *taken) {
   if (!ctx) return -1;
   int n = 0, done = 0, rc = 0;
   HIPCHK(hipEventRecord(ctx->e0, ctx->st));
-  for (; n < nsteps && !done; n++)
+  while (n < nsteps && !done) {
+    if (fast_eligible(ctx) && nsteps - n >= 2) {
+      int k = 0, halted = 0;
+      const int want = std::min(nsteps - n, CTL_HIST);
+      if ((rc = run_batch(ctx, want, &k, &halted))) break;
+      n += k;
+      if (!halted || n >= nsteps) continue;
+    }
     if ((rc = fv_step(ctx, &done))) break;
+    if (!done) n++;
+  }
   HIPCHK(hipEventRecord(ctx->e1, ctx->st));
   HIPCHK(hipEventSynchronize(ctx->e1));
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, ctx->e0, ctx->e1));
   ctx->last_ms = ms;
-  if (taken) *taken = done ? n - 1 : n;
+  if (taken) *taken = n;
   return rc;
 }
 
@@ -670,5 +880,22 @@ extern "C" int fv_get_scalars(const fv_ctx *ctx, fv_scalars *o) {
 }
 
 extern "C" double fv_last_run_ms(const fv_ctx *ctx) { return ctx ? ctx->last_ms : -1.0; }
+
+extern "C" int fv_profile(fv_ctx *ctx, int enable) {
+  if (!ctx) return -1;
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  prof_collect(ctx);
+  ctx->prof = enable != 0;
+  for (int q = 0; q < K_CLASSES; q++) ctx->st_launch[q] = 0, ctx->st_ms[q] = ctx->st_bytes[q] = 0;
+  return 0;
+}
+
+extern "C" int fv_kernel_stats(const fv_ctx *ctx, int kernel, int *launches, double *ms, double *alg_bytes) {
+  if (!ctx || kernel < 0 || kernel >= K_CLASSES) return -1;
+  if (launches) *launches = ctx->st_launch[kernel];
+  if (ms) *ms = ctx->st_ms[kernel];
+  if (alg_bytes) *alg_bytes = ctx->st_bytes[kernel];
+  return 0;
+}
 
 extern "C" const char *fv_last_error(const fv_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }

@@ -5,9 +5,10 @@
 
 namespace fv {
 
-constexpr int GH = 2;      // ghost layers per side (mesh/mesh_base.f90:462)
-constexpr int MAXK = 7;    // RK stages (Dormand-Prince has 7)
-constexpr int TILE_X = 64; // threads per tile row (one wavefront)
+constexpr int GH = 2;       // ghost layers per side (mesh/mesh_base.f90:462)
+constexpr int MAXK = 7;     // RK stages (Dormand-Prince has 7)
+constexpr int TILE_X = 64;  // threads per tile row (one wavefront)
+constexpr int CTL_HIST = 512;  // steps per device-controlled batch
 
 enum Physics { EULER_ISOTHERM = 16, EULER = 17 };
 enum Geometry { CARTESIAN = 1, LOGCYLINDRICAL = 31 };
@@ -37,22 +38,41 @@ struct Geom {
   const double *col;    // [C_COUNT][NJ]
 };
 
-// One right-hand-side evaluation (timedisc_base.f90:1073-1271) on the stage
-// state  S = Y - sum_q c[q] * k[q]  (timedisc_rkfehlberg.f90:277-302).
+// Device-side step control (one per context).  In a device-controlled batch
+// the kernels read the time step from here; a one-block control kernel at the
+// end of every step reduces the per-block partials of the final combination
+// (embedded error) and of the end-of-step RHS (CFL, data checks), performs the
+// accept test and computes the next time step.  Every step that the device
+// cannot decide alone (rejection, break, output, stop time) halts the batch
+// and the host redoes that step itself.
+enum HaltReason { H_NONE = 0, H_ERR = 1, H_DTLIMIT = 2, H_FLAGS = 3, H_OUTPUT = 4, H_STOP = 5, H_BATCH = 6 };
+struct Ctl {
+  double time, dt, dt_step, t_out, stoptime, cfl, dtlimit, maxerrold, beta, order;
+  double invdt;
+  int halt, reason, step, nsteps, flags, pad;
+  double err_hist[CTL_HIST], dt_hist[CTL_HIST];
+};
+
+enum InputMode { IN_STAGE = 0, IN_REMAP = 1 };
+
+// One right-hand-side evaluation (timedisc_base.f90:1073-1271) on the state
+//   IN_STAGE: S = Y - sum_q (a[q]*dt) * k[q]   (timedisc_rkfehlberg.f90:277-302)
+//   IN_REMAP: S = FargoAdvectionY(Y)           (timedisc_base.f90:1935-2069)
 struct StageArgs {
   const double *Y;
   const double *k[MAXK];
-  double c[MAXK];
+  double a[MAXK];       // Butcher coefficients of this stage
   int nk;
-  const double *gsrc;   // ghost cells of the stage state (written by the ghost kernel)
-  double *gdst;
+  double dt;            // time step (host-controlled) ...
+  double dt_step;       // ... and this%dt of the FARGO remap
+  Ctl *ctl;             // device-controlled: dt and halting come from here
+  int end_of_step;      // device-controlled end-of-step RHS: dt_step = (time+dt)-time
   double *kout;         // rhs (interior cells)
-  double *yout;         // transformed state (interior), or null
+  double *yout;         // transformed state incl. ghost cells, or null
   int add_bg, sub_bg;   // FARGO frame change at entry / before the fluxes
   int pmin_clamp;       // Euler CHECK_PMIN round trip (timedisc_base.f90:1150-1164)
   int want_cfl, want_check;
-  double *part;         // per-block partial results
-  int part_off;
+  double *part;         // per-block partials: [2*b] max 1/dt, [2*b+1] check flags
 };
 
 // pointwise final combination + embedded error (timedisc_rkfehlberg.f90:199-266,
@@ -60,25 +80,25 @@ struct StageArgs {
 struct CombineArgs {
   const double *Y;
   const double *k[MAXK];
-  double ch[MAXK], cl[MAXK];
+  double bh[MAXK], bl[MAXK];
   int nk;
+  double dt;
+  Ctl *ctl;
   double *ynew;
   double tol_rel, tol_abs[4];
-  double *part;
+  double *part;         // per-block partials: [2*b] max error, [2*b+1] = 0
 };
 
-// partial slot layout: part[2*b] = max(invdt or err), part[2*b+1] = flag bits
 enum Flag { F_INVALID = 1, F_RHOMIN = 2, F_PMIN = 4 };
 
 // kernel launchers (fv_kernels.hip)
 int rhs_blocks(const Geom &g);
-int ghost_blocks(const Geom &g);
 int combine_blocks(const Geom &g);
-void launch_ghost(const Geom &g, const StageArgs &a, int V, hipStream_t s);
-void launch_rhs(const Geom &g, const StageArgs &a, int V, hipStream_t s);
+void set_rhs_rows(int by);
+void launch_rhs(const Geom &g, const StageArgs &a, int V, int mode, hipStream_t s);
 void launch_combine(const Geom &g, const CombineArgs &a, int V, hipStream_t s);
-void launch_remap(const Geom &g, const double *u, double *z, double dt, int V, hipStream_t s);
 void launch_frame(const Geom &g, const double *src, double *dst, int sign, int V, hipStream_t s);
 void launch_reduce(const double *part, int n, double *out, hipStream_t s);
+void launch_control(Ctl *ctl, const double *part_c, int nc, const double *part_r, int nr, hipStream_t s);
 
 }  // namespace fv

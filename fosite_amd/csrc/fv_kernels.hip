@@ -5,17 +5,23 @@
 // the file is compiled with -ffp-contract=off, so IEEE +,-,*,/ and sqrt give
 // the reference's bits.  Citations are file:line in the fosite source tree.
 //
-// Kernels per right-hand-side evaluation:
-//   k_ghost  boundary conditions + corner fill + ghost prim->cons for the
-//            stage state (boundary_generic.f90:287-517), written to the ghost
-//            cells of a work array;
-//   k_rhs    fused: stage state on the fly (timedisc_rkfehlberg.f90:277-302),
-//            FARGO frame round trip, linear reconstruction, face conversions,
-//            wave speeds, physical and KT fluxes, geometric/gravity/FARGO
-//            sources, rhs assembly, CFL and data checks.  64 x BY threads per
-//            tile, LDS-staged halo, one cell per thread.
-// Per step: k_combine (final stage combination + embedded error norm),
-// k_remap (FARGO linear transport + integer shift), small reductions.
+// k_rhs (one launch per right-hand-side evaluation) is fully fused:
+//   * the input state of every loaded cell is formed on the fly -- either the
+//     RK stage state Y - sum a*dt*k (timedisc_rkfehlberg.f90:277-302) or, for
+//     the end-of-step evaluation under FARGO, the remapped state
+//     (timedisc_base.f90:1935-2069);
+//   * ghost cells are evaluated on the fly from interior cells with the
+//     boundary rules (boundary_generic.f90:287-517, boundary_nogradients.f90,
+//     boundary_periodic.f90, boundary_absorbing.f90), so no boundary pass and
+//     no intermediate state array exist;
+//   * FARGO frame round trip, Euler PMIN round trip, linear reconstruction,
+//     face conversions, wave speeds, physical and KT fluxes, geometric,
+//     gravity and FARGO sources, rhs assembly, CFL and data checks follow.
+//   Tiles of 64 x BY threads (one cell each) with the halo staged in LDS.
+// k_combine: final RK combination + embedded error norm (pointwise).
+// k_control: one block; in device-controlled batches it reduces the partials
+// of k_combine and of the end-of-step k_rhs, runs the accept test and the CFL
+// update.  Kernel boundaries order everything: no grid-wide fences.
 #include <float.h>
 #include <math.h>
 
@@ -56,180 +62,224 @@ __device__ __forceinline__ double sound_speed(double gamma, double rho, double p
   return sqrt(fmax2(2.0 * TINYD, gamma * pr / rho));
 }
 
-// ------------------------------------------------------------- stage state
-// S = Y - c0*k0 - c1*k1 - ... in the reference's accumulation order
-template <int V>
-__device__ __forceinline__ void stage_state(const Geom &g, const StageArgs &a, long ix, double *s) {
-#pragma unroll
-  for (int l = 0; l < V; l++) {
-    double v = a.Y[l * g.FS + ix];
-    for (int q = 0; q < a.nk; q++) v = v - a.c[q] * a.k[q][l * g.FS + ix];
-    s[l] = v;
-  }
-}
+// ------------------------------------------------------------ input state
+template <int V, int MODE>
+struct In {
+  const Geom &g;
+  const StageArgs &a;
+  double c[MAXK];  // a(m,q)*dt
+  double dts;      // this%dt of the FARGO remap
+  int nk;
 
-// ComputeRHS state transform on one interior cell (timedisc_base.f90:1098-1208):
-// ur = cvar after the frame change and CenterBoundary/PMIN (real frame),
-// pr = pvar at the same point, uc = cvar handed to the fluxes.
-template <int V>
-__device__ __forceinline__ void transform(const Geom &g, const StageArgs &a, int i, const double *s,
-                                          double *ur, double *pr, double *uc) {
-#pragma unroll
-  for (int l = 0; l < V; l++) ur[l] = s[l];
-  if (a.add_bg) ur[2] = ur[2] + ur[0] * ring(g, R_W, i);
-  c2p<V>(ur, pr, g.gamma);
-  if (V == 4 && a.pmin_clamp) {
-    pr[3] = fmax2(pr[3], g.pmin);
-    p2c<V>(pr, ur, g.gamma);
-  }
-#pragma unroll
-  for (int l = 0; l < V; l++) uc[l] = ur[l];
-  if (a.sub_bg) uc[2] = ur[2] - ur[0] * ring(g, R_W, i);
-}
-
-// pvar of an interior cell right after CenterBoundary's conversion
-template <int V>
-__device__ __forceinline__ void prim_interior(const Geom &g, const StageArgs &a, int i, int j, double *p) {
-  double s[V];
-  stage_state<V>(g, a, cidx(g, i, j), s);
-  if (a.add_bg) s[2] = s[2] + s[0] * ring(g, R_W, i);
-  c2p<V>(s, p, g.gamma);
-}
-
-// ------------------------------------------------------------- boundaries
-// absorbing boundary for isothermal physics along x: boundary_absorbing.f90:114-150,
-// physics_eulerisotherm.f90:1764-1849, 2027-2085, 2895-2969
-template <int V>
-__device__ void absorbing_x(const Geom &g, const StageArgs &a, int east, int j, double *p) {
-  const int imin = GH, imax = GH + g.inum - 1;
-  const int i1 = east ? imax : imin, i2 = east ? imax - 1 : imin + 1;
-  const int iL = i1 < i2 ? i1 : i2, iR = i1 < i2 ? i2 : i1;
-  const int delta = east ? 1 : -1;
-  double pL[V], pR[V];
-  prim_interior<V>(g, a, iL, j, pL);
-  prim_interior<V>(g, a, iR, j, pR);
-  const double *p1 = east ? pR : pL;  // values at i1
-  double cs = ring(g, R_CS, i1), u1 = p1[1];
-  double lam0 = u1 - cs, lam1 = u1, lam2 = u1 + cs;
-  double csf = ring(g, R_CSW, iR);
-  double dlnrho = log(pR[0] / pL[0]);
-  double ducs = (pR[1] - pL[1]) / csf;
-  double x0 = dlnrho - ducs, x1 = pR[2] - pL[2], x2 = dlnrho + ducs;
-  if (east) {
-    if (lam0 <= 0.0) x0 = 0.0;
-    if (lam1 <= 0.0) x1 = 0.0;
-    if (lam2 <= 0.0) x2 = 0.0;
-  } else {
-    if (lam0 >= 0.0) x0 = 0.0;
-    if (lam1 >= 0.0) x1 = 0.0;
-    if (lam2 >= 0.0) x2 = 0.0;
-  }
-  double cs2 = ring(g, east ? R_CSE : R_CSW, i1);
-  p[0] = p1[0] * exp(delta * 0.5 * (x2 + x0));
-  p[1] = u1 + delta * 0.5 * cs2 * (x2 - x0);
-  p[2] = p1[2] + delta * x1;
-}
-
-// pvar of the x-ghost cell (side, layer gl in 1..2) at interior row j
-template <int V>
-__device__ void ghost_we(const Geom &g, const StageArgs &a, int east, int gl, int j, double *p) {
-  const int imin = GH, imax = GH + g.inum - 1, bc = g.bc[east];
-  if (bc == NO_GRADIENTS)
-    prim_interior<V>(g, a, east ? imax : imin, j, p);
-  else if (bc == PERIODIC)
-    prim_interior<V>(g, a, east ? imin + gl - 1 : imax - gl + 1, j, p);
-  else
-    absorbing_x<V>(g, a, east, j, p);
-}
-// pvar of the y-ghost cell (side, layer gl) at interior column i
-template <int V>
-__device__ void ghost_sn(const Geom &g, const StageArgs &a, int north, int gl, int i, double *p) {
-  const int jmin = GH, jmax = GH + g.jnum - 1, bc = g.bc[2 + north];
-  if (bc == NO_GRADIENTS)
-    prim_interior<V>(g, a, i, north ? jmax : jmin, p);
-  else
-    prim_interior<V>(g, a, i, north ? jmin + gl - 1 : jmax - gl + 1, p);
-}
-
-__device__ __forceinline__ double blk_max(double v, double *red) {
-  // wave64 reduction then LDS across waves
-  for (int o = 32; o > 0; o >>= 1) {
-    double w = __shfl_xor(v, o, 64);
-    v = fmax2(v, w);
-  }
-  const int t = threadIdx.x + threadIdx.y * blockDim.x, nw = (blockDim.x * blockDim.y + 63) / 64;
-  __syncthreads();
-  if ((t & 63) == 0) red[t >> 6] = v;
-  __syncthreads();
-  double r = -DBL_MAX;
-  for (int k = 0; k < nw; k++) r = fmax2(r, red[k]);
-  return r;
-}
-__device__ __forceinline__ int blk_or(int v, int *red) {
-  for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, 64);
-  const int t = threadIdx.x + threadIdx.y * blockDim.x, nw = (blockDim.x * blockDim.y + 63) / 64;
-  __syncthreads();
-  if ((t & 63) == 0) red[t >> 6] = v;
-  __syncthreads();
-  int r = 0;
-  for (int k = 0; k < nw; k++) r |= red[k];
-  return r;
-}
-
-// CFL contribution of one cell (timedisc_base.f90:869-872 with
-// physics_eulerisotherm.f90:530-552); p = pvar after the frame change back
-__device__ __forceinline__ double cell_invdt(const Geom &g, int i, double u, double v, double cs) {
-  double mnx = fmin2(0., u - cs), mxx = fmax2(0., u + cs);
-  double mny = fmin2(0., v - cs), mxy = fmax2(0., v + cs);
-  return fmax2(mxx, -mnx) / ring(g, R_DLX, i) + fmax2(mxy, -mny) / ring(g, R_DLY, i);
-}
-
-// One thread per ghost cell: W/E slabs (interior rows), S/N slabs (interior
-// columns), 16 corner cells (SetCornerEdges, boundary_generic.f90:434-517).
-template <int V>
-__global__ __launch_bounds__(256) void k_ghost(Geom g, StageArgs a) {
-  __shared__ double redd[8];
-  __shared__ int redi[8];
-  const int imin = GH, imax = GH + g.inum - 1, jmin = GH, jmax = GH + g.jnum - 1;
-  const int nwe = 2 * 2 * g.jnum, nsn = 2 * 2 * g.inum;
-  int t = blockIdx.x * 256 + threadIdx.x;
-  int i = -1, j = -1;
-  double p[V];
-  bool active = true;
-  if (t < nwe) {  // x-ghosts: t -> (side, layer, j)
-    int east = t / (2 * g.jnum), r = t % (2 * g.jnum), gl = 1 + r / g.jnum;
-    j = jmin + r % g.jnum;
-    i = east ? imax + gl : imin - gl;
-    ghost_we<V>(g, a, east, gl, j, p);
-  } else if ((t -= nwe) < nsn) {
-    int north = t / (2 * g.inum), r = t % (2 * g.inum), gl = 1 + r / g.inum;
-    i = imin + r % g.inum;
-    j = north ? jmax + gl : jmin - gl;
-    ghost_sn<V>(g, a, north, gl, i, p);
-  } else if ((t -= nsn) < 16) {
-    int corner = t / 4, ga = 1 + (t & 1), gb = 1 + ((t >> 1) & 1);
-    int east = corner & 1, north = corner >> 1;
-    int ir = east ? imax : imin, jr = north ? jmax : jmin;
-    i = east ? imax + ga : imin - ga;
-    j = north ? jmax + gb : jmin - gb;
-    double px[V], py[V];
-    if (ga == 2 && gb == 1) {
-      ghost_we<V>(g, a, east, 2, jr, p);  // copy from the x-ghost at the boundary row
-    } else if (ga == 1 && gb == 2) {
-      ghost_sn<V>(g, a, north, 2, ir, p);  // copy from the y-ghost at the boundary column
-    } else {
-      ghost_we<V>(g, a, east, ga, jr, px);
-      ghost_sn<V>(g, a, north, gb, ir, py);
-#pragma unroll
-      for (int l = 0; l < V; l++) p[l] = 0.5 * (px[l] + py[l]);
+  __forceinline__ __device__ In(const Geom &g_, const StageArgs &a_) : g(g_), a(a_) {
+    double dt = a.dt;
+    dts = a.dt_step;
+    if (a.ctl) {  // device-controlled: this step's dt; dt_step as AcceptSolution sets it
+      const double t = a.ctl->time;
+      dt = a.ctl->dt;
+      dts = a.end_of_step ? (t + dt) - t : 0.0;
     }
-  } else {
-    active = false;
+    nk = a.nk;
+#pragma unroll
+    for (int q = 0; q < MAXK; q++) c[q] = a.a[q] * dt;
   }
-  double invdt = -DBL_MAX;
-  int flags = 0;
-  if (active) {
-    double u[V];
+
+  // stored state at an interior column and any row, y-ghost rows taken from
+  // the comoving-frame CenterBoundary (prim round trip, no frame change)
+  __forceinline__ __device__ void ycell(int i, int jj, double *u) const {
+    const int jmin = GH, jmax = GH + g.jnum - 1;
+    int js = jj;
+    bool ghost = false;
+    if (jj < jmin) {
+      ghost = true;
+      js = g.bc[2] == PERIODIC ? jj + g.jnum : jmin;
+    } else if (jj > jmax) {
+      ghost = true;
+      js = g.bc[3] == PERIODIC ? jj - g.jnum : jmax;
+    }
+    const long ix = cidx(g, i, js);
+#pragma unroll
+    for (int l = 0; l < V; l++) u[l] = a.Y[l * g.FS + ix];
+    if (ghost) {
+      double p[V];
+      c2p<V>(u, p, g.gamma);
+      p2c<V>(p, u, g.gamma);
+    }
+  }
+
+  // FargoAdvectionY for one interior cell: minmod-limited linear transport by
+  // the fractional shift, then CSHIFT(., -shift) over JMIN:JMAX
+  __forceinline__ __device__ void remap(int i, int j, double *s) const {
+    const int jmin = GH, J = g.jnum;
+    double d = ring(g, R_W, i) * dts / ring(g, R_DLY, i);
+    const double shift = round(d);
+    d = d - shift;
+    long jj = (long)(j - jmin) - (long)shift;
+    jj %= J;
+    if (jj < 0) jj += J;
+    const int js = jmin + (int)jj;
+    double v[5][V];
+#pragma unroll
+    for (int q = 0; q < 5; q++) ycell(i, js - 2 + q, v[q]);
+#pragma unroll
+    for (int l = 0; l < V; l++) {
+      double dq[4], dql[3];
+#pragma unroll
+      for (int q = 0; q < 4; q++) dq[q] = v[q + 1][l] - v[q][l];
+#pragma unroll
+      for (int q = 0; q < 3; q++)
+        dql[q] = (fsign(1.0, dq[q]) * fsign(1.0, dq[q + 1]) > 0.)
+                     ? fsign(fmin2(fabs(dq[q]), fabs(dq[q + 1])), dq[q])
+                     : 0.;
+      double f0, f1;  // flux(js-1), flux(js)
+      if (d > 0.) {
+        f0 = v[1][l] + .5 * dql[0] * (1. - d);
+        f1 = v[2][l] + .5 * dql[1] * (1. - d);
+      } else {
+        f0 = v[2][l] - .5 * dql[1] * (1. + d);
+        f1 = v[3][l] - .5 * dql[2] * (1. + d);
+      }
+      s[l] = v[2][l] - d * (f1 - f0);
+    }
+  }
+
+  // input state at an interior cell
+  __forceinline__ __device__ void interior(int i, int j, double *s) const {
+    if (MODE == IN_REMAP) {
+      remap(i, j, s);
+    } else {
+      const long ix = cidx(g, i, j);
+      // all loads first (uniform branches only), then the stage sum in the
+      // reference's order
+      double kv[MAXK][V];
+#pragma unroll
+      for (int l = 0; l < V; l++) s[l] = a.Y[l * g.FS + ix];
+#pragma unroll
+      for (int q = 0; q < MAXK; q++)
+        if (q < nk) {
+#pragma unroll
+          for (int l = 0; l < V; l++) kv[q][l] = a.k[q][l * g.FS + ix];
+        }
+#pragma unroll
+      for (int q = 0; q < MAXK; q++)
+        if (q < nk) {
+#pragma unroll
+          for (int l = 0; l < V; l++) s[l] = s[l] - c[q] * kv[q][l];
+        }
+    }
+  }
+
+  // ComputeRHS transform (timedisc_base.f90:1098-1208): ur = cvar after the
+  // frame change, CenterBoundary and PMIN (real frame), pr = pvar at that
+  // point, uc = cvar handed to the fluxes
+  __forceinline__ __device__ void transform(int i, const double *s, double *ur, double *pr, double *uc) const {
+#pragma unroll
+    for (int l = 0; l < V; l++) ur[l] = s[l];
+    if (a.add_bg) ur[2] = ur[2] + ur[0] * ring(g, R_W, i);
+    c2p<V>(ur, pr, g.gamma);
+    if (V == 4 && a.pmin_clamp) {
+      pr[3] = fmax2(pr[3], g.pmin);
+      p2c<V>(pr, ur, g.gamma);
+    }
+#pragma unroll
+    for (int l = 0; l < V; l++) uc[l] = ur[l];
+    if (a.sub_bg) uc[2] = ur[2] - ur[0] * ring(g, R_W, i);
+  }
+
+  // pvar of an interior cell right after CenterBoundary's conversion
+  __forceinline__ __device__ void prim_interior(int i, int j, double *p) const {
+    double s[V];
+    interior(i, j, s);
+    if (a.add_bg) s[2] = s[2] + s[0] * ring(g, R_W, i);
+    c2p<V>(s, p, g.gamma);
+  }
+
+  // absorbing boundary, isothermal physics, x direction:
+  // boundary_absorbing.f90:114-150, physics_eulerisotherm.f90:1764-1849,
+  // 2027-2085, 2895-2969
+  __forceinline__ __device__ void absorbing_x(int east, int j, double *p) const {
+    const int imin = GH, imax = GH + g.inum - 1;
+    const int i1 = east ? imax : imin, i2 = east ? imax - 1 : imin + 1;
+    const int iL = i1 < i2 ? i1 : i2, iR = i1 < i2 ? i2 : i1;
+    const int delta = east ? 1 : -1;
+    double pL[V], pR[V];
+    prim_interior(iL, j, pL);
+    prim_interior(iR, j, pR);
+    double p1[V];  // element-wise select keeps both states in registers
+#pragma unroll
+    for (int l = 0; l < V; l++) p1[l] = east ? pR[l] : pL[l];
+    const double cs = ring(g, R_CS, i1), u1 = p1[1];
+    const double lam0 = u1 - cs, lam1 = u1, lam2 = u1 + cs;
+    const double csf = ring(g, R_CSW, iR);
+    const double dlnrho = log(pR[0] / pL[0]);
+    const double ducs = (pR[1] - pL[1]) / csf;
+    double x0 = dlnrho - ducs, x1 = pR[2] - pL[2], x2 = dlnrho + ducs;
+    if (east) {
+      if (lam0 <= 0.0) x0 = 0.0;
+      if (lam1 <= 0.0) x1 = 0.0;
+      if (lam2 <= 0.0) x2 = 0.0;
+    } else {
+      if (lam0 >= 0.0) x0 = 0.0;
+      if (lam1 >= 0.0) x1 = 0.0;
+      if (lam2 >= 0.0) x2 = 0.0;
+    }
+    const double cs2 = ring(g, east ? R_CSE : R_CSW, i1);
+    p[0] = p1[0] * exp(delta * 0.5 * (x2 + x0));
+    p[1] = u1 + delta * 0.5 * cs2 * (x2 - x0);
+    p[2] = p1[2] + delta * x1;
+  }
+
+  // x-ghost (side, layer 1..2) at interior row j
+  __forceinline__ __device__ void ghost_we(int east, int gl, int j, double *p) const {
+    const int imin = GH, imax = GH + g.inum - 1, bc = east ? g.bc[1] : g.bc[0];
+    if (bc == NO_GRADIENTS)
+      prim_interior(east ? imax : imin, j, p);
+    else if (bc == PERIODIC)
+      prim_interior(east ? imin + gl - 1 : imax - gl + 1, j, p);
+    else
+      absorbing_x(east, j, p);
+  }
+  // y-ghost (side, layer) at interior column i
+  __forceinline__ __device__ void ghost_sn(int north, int gl, int i, double *p) const {
+    const int jmin = GH, jmax = GH + g.jnum - 1, bc = north ? g.bc[3] : g.bc[2];
+    if (bc == NO_GRADIENTS)
+      prim_interior(i, north ? jmax : jmin, p);
+    else
+      prim_interior(i, north ? jmin + gl - 1 : jmax - gl + 1, p);
+  }
+
+  // pvar of any ghost cell after CenterBoundary: boundary rules in the order
+  // W, E, S, N, then SetCornerEdges (boundary_generic.f90:434-517)
+  __forceinline__ __device__ void ghost_prim(int i, int j, double *p) const {
+    const int imin = GH, imax = GH + g.inum - 1, jmin = GH, jmax = GH + g.jnum - 1;
+    const bool xg = i < imin || i > imax, yg = j < jmin || j > jmax;
+    if (xg && !yg) {
+      const int east = i > imax;
+      ghost_we(east, east ? i - imax : imin - i, j, p);
+    } else if (yg && !xg) {
+      const int north = j > jmax;
+      ghost_sn(north, north ? j - jmax : jmin - j, i, p);
+    } else {
+      const int east = i > imax, north = j > jmax;
+      const int ga = east ? i - imax : imin - i, gb = north ? j - jmax : jmin - j;
+      const int ir = east ? imax : imin, jr = north ? jmax : jmin;
+      if (ga == 2 && gb == 1) {
+        ghost_we(east, 2, jr, p);
+      } else if (ga == 1 && gb == 2) {
+        ghost_sn(north, 2, ir, p);
+      } else {
+        double px[V], py[V];
+        ghost_we(east, ga, jr, px);
+        ghost_sn(north, gb, ir, py);
+#pragma unroll
+        for (int l = 0; l < V; l++) p[l] = 0.5 * (px[l] + py[l]);
+      }
+    }
+  }
+
+  // final ghost state: PMIN round trip, prim->cons, data check, frame change
+  __forceinline__ __device__ void ghost_final(int i, int j, double *u, double *p, int &flags) const {
+    ghost_prim(i, j, p);
     if (V == 4 && a.pmin_clamp) p[3] = fmax2(p[3], g.pmin);
     p2c<V>(p, u, g.gamma);
     if (a.want_check) {
@@ -238,37 +288,67 @@ __global__ __launch_bounds__(256) void k_ghost(Geom g, StageArgs a) {
         if (bad(u[l]) || bad(p[l])) flags |= F_INVALID;
     }
     if (a.sub_bg) {
-      double w = ring(g, R_W, i);
+      const double w = ring(g, R_W, i);
       p[2] = p[2] - w;
       u[2] = u[2] - u[0] * w;
     }
-    if (a.want_cfl) {
-      double cs = (V == 4) ? sound_speed(g.gamma, p[0], p[3]) : ring(g, R_CS, i);
-      invdt = cell_invdt(g, i, p[1], p[2], cs);
-    }
-    const long ix = cidx(g, i, j);
-#pragma unroll
-    for (int l = 0; l < V; l++) a.gdst[l * g.FS + ix] = u[l];
   }
-  if (a.want_cfl || a.want_check) {
-    double m = blk_max(invdt, redd);
-    int f = blk_or(flags, redi);
-    if (threadIdx.x == 0) {
-      a.part[2 * (a.part_off + blockIdx.x)] = m;
-      a.part[2 * (a.part_off + blockIdx.x) + 1] = (double)f;
-    }
-  }
+};
+
+// CFL contribution of one cell (timedisc_base.f90:869-872 with
+// physics_eulerisotherm.f90:530-552); u, v = pvar after the frame change back
+__device__ __forceinline__ double cell_invdt(const Geom &g, int i, double u, double v, double cs) {
+  const double mnx = fmin2(0., u - cs), mxx = fmax2(0., u + cs);
+  const double mny = fmin2(0., v - cs), mxy = fmax2(0., v + cs);
+  return fmax2(mxx, -mnx) / ring(g, R_DLX, i) + fmax2(mxy, -mny) / ring(g, R_DLY, i);
 }
 
+// ------------------------------------------------------------ reductions
+__device__ __forceinline__ double wave_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax2(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ int wave_or(int v) {
+  for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, 64);
+  return v;
+}
+// block-wide max / or; every thread gets the result
+__device__ double blk_max(double v, double *red) {
+  v = wave_max(v);
+  const int t = threadIdx.x + threadIdx.y * blockDim.x, nw = (blockDim.x * blockDim.y + 63) / 64;
+  __syncthreads();
+  if ((t & 63) == 0) red[t >> 6] = v;
+  __syncthreads();
+  double r = -DBL_MAX;
+  for (int k = 0; k < nw; k++) r = fmax2(r, red[k]);
+  return r;
+}
+__device__ int blk_or(int v, int *red) {
+  v = wave_or(v);
+  const int t = threadIdx.x + threadIdx.y * blockDim.x, nw = (blockDim.x * blockDim.y + 63) / 64;
+  __syncthreads();
+  if ((t & 63) == 0) red[t >> 6] = v;
+  __syncthreads();
+  int r = 0;
+  for (int k = 0; k < nw; k++) r |= red[k];
+  return r;
+}
 // ----------------------------------------------------------- reconstruction
 // limiters, reconstruction_linear.f90:645-733
-__device__ __forceinline__ double limiter(const Geom &g, double a1, double a2) {
+__device__ __forceinline__ double slope(const Geom &g, double um, double u0, double up) {
+  const double a1 = u0 - um, a2 = up - u0;
   switch (g.limiter) {
     case MINMOD:
       return (fsign(1.0, a1) * fsign(1.0, a2) > 0) ? fsign(fmin2(fabs(a1), fabs(a2)), a1) : 0.;
+    case MONOCENT: {
+      const double b1 = g.theta * (u0 - um), b2 = g.theta * (up - u0), b3 = 0.5 * (up - um);
+      if ((fsign(1.0, b1) * fsign(1.0, b2) > 0) && (fsign(1.0, b2) * fsign(1.0, b3) > 0))
+        return fsign(fmin2(fmin2(fabs(b1), fabs(b2)), fabs(b3)), b1);
+      return 0.;
+    }
     case SWEBY:
     case SUPERBEE: {
-      double q = g.limiter == SUPERBEE ? 2.0 : g.theta;
+      const double q = g.limiter == SUPERBEE ? 2.0 : g.theta;
       return (fsign(1.0, a1) * fsign(1.0, a2) > 0)
                  ? fsign(fmax2(fmin2(q * fabs(a1), fabs(a2)), fmin2(fabs(a1), q * fabs(a2))), a1)
                  : 0.;
@@ -276,21 +356,12 @@ __device__ __forceinline__ double limiter(const Geom &g, double a1, double a2) {
     case OSPRE:
       return 1.5 * a1 * a2 * (a1 + a2) / (a1 * (a1 + 0.5 * a2) + a2 * (a2 + 0.5 * a1) + TINYD);
     case VANLEER: {
-      double b1 = fabs(a1), b2 = fabs(a2);
+      const double b1 = fabs(a1), b2 = fabs(a2);
       return (a1 * b2 + a2 * b1) / (b1 + b2 + TINYD);
     }
     default:
       return 0.5 * (a1 + a2);
   }
-}
-__device__ __forceinline__ double slope(const Geom &g, double um, double u0, double up) {
-  if (g.limiter == MONOCENT) {  // minmod3 of theta-scaled one-sided and central differences
-    double a1 = g.theta * (u0 - um), a2 = g.theta * (up - u0), a3 = 0.5 * (up - um);
-    if ((fsign(1.0, a1) * fsign(1.0, a2) > 0) && (fsign(1.0, a2) * fsign(1.0, a3) > 0))
-      return fsign(fmin2(fmin2(fabs(a1), fabs(a2)), fabs(a3)), a1);
-    return 0.;
-  }
-  return limiter(g, u0 - um, up - u0);
 }
 
 // KT flux across one interface from the two face states (fluxes_kt.f90:126-176,
@@ -307,9 +378,9 @@ __device__ __forceinline__ void kt_flux(const Geom &g, int d, const double *UL, 
     csL = sound_speed(g.gamma, pl[0], pl[3]);
     csR = sound_speed(g.gamma, pr[0], pr[3]);
   }
-  double vL = pl[un], vR = pr[un];
-  double amin = fmin2(fmin2(0.0, fmin2(0., vR - csR)), fmin2(0., vL - csL));
-  double amax = fmax2(fmax2(0.0, fmax2(0., vR + csR)), fmax2(0., vL + csL));
+  const double vL = pl[un], vR = pr[un];
+  const double amin = fmin2(fmin2(0.0, fmin2(0., vR - csR)), fmin2(0., vL - csL));
+  const double amax = fmax2(fmax2(0.0, fmax2(0., vR + csR)), fmax2(0., vL + csL));
   double fL[V], fR[V];
   fL[0] = pl[0] * vL;
   fR[0] = pr[0] * vR;
@@ -324,63 +395,132 @@ __device__ __forceinline__ void kt_flux(const Geom &g, int d, const double *UL, 
   }
   fL[ut] = UL[ut] * vL;
   fR[ut] = UR[ut] * vR;
-  double fac = dA / (amax - amin);
+  const double fac = dA / (amax - amin);
 #pragma unroll
   for (int l = 0; l < V; l++) F[l] = fac * (amax * fL[l] - amin * fR[l] + amin * amax * (UR[l] - UL[l]));
 }
 
+// ------------------------------------------------------------ step control
+// PI controller (timedisc_base.f90:759-814), used on the device only to stop
+// a batch when the step could fall under dtlimit; the host redoes such steps
+__device__ double pi_dtnew(const Ctl *c, double maxerr, double dtold) {
+  double dttmp = maxerr > 0. ? 0.9 * dtold * exp(-log(maxerr) / c->order) : 4. * dtold;
+  double dtnew = c->maxerrold > 0. ? dttmp * exp(-c->beta * log(c->maxerrold / maxerr)) : dttmp;
+  if (maxerr < 1.0 && 4. * dtold < dtnew) dtnew = 4. * dtold;
+  return dtnew;
+}
+__device__ void halt(Ctl *c, int reason) {
+  c->halt = 1;
+  c->reason = reason;
+}
+// after SolveODE: accept when the embedded error is finite and < 1
+// (timedisc_base.f90:686-713, AcceptSolution 911-959)
+__device__ void control_accept(Ctl *c, double err) {
+  const int s = c->step;
+  c->err_hist[s] = err;
+  c->dt_hist[s] = c->dt;
+  if (!(err < 1.0)) return halt(c, H_ERR);  // NaN, Inf or a rejection
+  if (pi_dtnew(c, err, c->dt) < c->dtlimit * (1.0 + 1.0e-6)) return halt(c, H_DTLIMIT);
+  c->maxerrold = err;
+  const double t1 = c->time + c->dt;
+  c->dt_step = t1 - c->time;
+  c->time = t1;
+}
+// after the end-of-step RHS: data checks, output test, next CFL step,
+// stop-time test and output-time clipping (fosite.f90:383-462,
+// io/fileio_base.f90:604-620)
+__device__ void control_next(Ctl *c, double invdt, int flags) {
+  if (flags) return halt(c, H_FLAGS);
+  const double t = c->time;
+  if (fabs(c->t_out - t) <= 1.0E-5 * c->t_out) return halt(c, H_OUTPUT);
+  double dt = c->cfl / invdt;
+  if (fabs(c->stoptime - t) <= 1.0E-05 * c->stoptime) return halt(c, H_STOP);
+  if ((t + dt) / c->t_out > 1.0)
+    dt = c->t_out - t;
+  else if ((t + 1.5 * dt) / c->t_out > 1.0)
+    dt = 0.5 * (c->t_out - t);
+  c->invdt = invdt;
+  c->dt = dt;
+  c->step = c->step + 1;
+  if (c->step >= c->nsteps) halt(c, H_BATCH);
+}
+
 // -------------------------------------------------------------- fused RHS
-template <int V, int BY>
+template <int V, int BY, int MODE>
 __global__ __launch_bounds__(TILE_X *BY) void k_rhs(Geom g, StageArgs a) {
   constexpr int BX = TILE_X, SX = BX + 2, SY = BY + 2;
   constexpr int NUC = V * SY * SX, NF = 2 * V * BY * BX, NL = NUC > NF ? NUC : NF;
+  constexpr int NT = BX * BY;
   __shared__ double lds[NL];
   __shared__ double redd[BY];
   __shared__ int redi[BY];
-  double *uc_s = lds;                  // [V][SY][SX] flux-input state
-  double *fa_s = lds;                  // [V][BY][BX] W faces, then x fluxes
-  double *fb_s = lds + V * BY * BX;    // [V][BY][BX] S faces, then y fluxes
+  double *uc_s = lds;                // [V][SY][SX] flux-input state
+  double *fa_s = lds;                // [V][BY][BX] W faces, then x fluxes
+  double *fb_s = lds + V * BY * BX;  // [V][BY][BX] S faces, then y fluxes
 
-  const int tx = threadIdx.x, ty = threadIdx.y;
+  if (a.ctl && a.ctl->halt) return;
+  const In<V, MODE> in(g, a);
+
+  const int tx = threadIdx.x, ty = threadIdx.y, t = tx + BX * ty;
   const int i0 = GH + blockIdx.x * (BX - 2), j0 = GH + blockIdx.y * (BY - 2);
-  const int fi = i0 - 1 + tx, fj = j0 - 1 + ty;  // this thread's cell
-  const int imax = GH + g.inum - 1, jmax = GH + g.jnum - 1;
+  const int fi = i0 - 1 + tx, fj = j0 - 1 + ty;
+  const int imin = GH, jmin = GH, imax = GH + g.inum - 1, jmax = GH + g.jnum - 1;
   const bool out = tx >= 1 && tx <= BX - 2 && ty >= 1 && ty <= BY - 2 && fi <= imax && fj <= jmax;
 
   // ---- phase 1: flux-input state of the tile and its halo into LDS
   double ur[V], pr[V], uc[V];
-  auto load = [&](int i, int j, int sx, int sy, bool own) {
-    double u[V];
-    if (i < 0 || j < 0 || i >= g.NI || j >= g.NJ) {
-#pragma unroll
-      for (int l = 0; l < V; l++) u[l] = 0.0;
-    } else if (i < GH || j < GH || i > imax || j > jmax) {
-      const long ix = cidx(g, i, j);
-#pragma unroll
-      for (int l = 0; l < V; l++) u[l] = a.gsrc[l * g.FS + ix];
-    } else {
+  auto stage_in = [&](int i, int j, double *u, bool own) {
+    const bool xin = i >= imin && i <= imax, yin = j >= jmin && j <= jmax;
+    if (xin && yin) {
       double s[V], r[V], p[V];
-      stage_state<V>(g, a, cidx(g, i, j), s);
-      transform<V>(g, a, i, s, r, p, u);
+      in.interior(i, j, s);
+      in.transform(i, s, r, p, u);
       if (own) {
 #pragma unroll
-        for (int l = 0; l < V; l++) { ur[l] = r[l]; pr[l] = p[l]; }
+        for (int l = 0; l < V; l++) ur[l] = r[l], pr[l] = p[l];
       }
-    }
+    } else if ((xin || yin) && i >= 0 && j >= 0 && i < g.NI && j < g.NJ) {
+      double p[V];
+      int f = 0;
+      in.ghost_final(i, j, u, p, f);
+    } else {
 #pragma unroll
-    for (int l = 0; l < V; l++) uc_s[(l * SY + sy) * SX + sx] = u[l];
-    if (own) {
-#pragma unroll
-      for (int l = 0; l < V; l++) uc[l] = u[l];
+      for (int l = 0; l < V; l++) u[l] = 0.0;  // corner or outside: never read
     }
   };
-  load(fi, fj, tx + 1, ty + 1, true);
-  if (tx == 0 && ty >= 1 && ty <= BY - 2) load(fi - 1, fj, 0, ty + 1, false);
-  if (tx == BX - 1 && ty >= 1 && ty <= BY - 2) load(fi + 1, fj, SX - 1, ty + 1, false);
-  if (ty == 0 && tx >= 1 && tx <= BX - 2) load(fi, fj - 1, tx + 1, 0, false);
-  if (ty == BY - 1 && tx >= 1 && tx <= BX - 2) load(fi, fj + 1, tx + 1, SY - 1, false);
+  // pass 0: the thread's own cell; pass 1: one halo cell (x columns i0-2 and
+  // i0+63 on the output rows, y rows j0-2 and j0+BY-1 on the output columns).
+  // One loop body keeps a single inlined copy of the state evaluation.
+  constexpr int NXH = 2 * (BY - 2), NH = NXH + 2 * (BX - 2);
+#pragma unroll 1
+  for (int pass = 0; pass < 2; pass++) {
+    int i = fi, j = fj, sx = tx + 1, sy = ty + 1;
+    if (pass) {
+      if (t >= NH) break;
+      if (t < NXH) {
+        const int side = t / (BY - 2), r = t % (BY - 2);
+        i = side ? i0 + BX - 1 : i0 - 2;
+        sx = side ? SX - 1 : 0;
+        j = j0 + r;
+        sy = r + 2;
+      } else {
+        const int k = t - NXH, side = k / (BX - 2), c = k % (BX - 2);
+        i = i0 + c;
+        sx = c + 2;
+        j = side ? j0 + BY - 1 : j0 - 2;
+        sy = side ? SY - 1 : 0;
+      }
+    }
+    double u[V];
+    stage_in(i, j, u, pass == 0);
+#pragma unroll
+    for (int l = 0; l < V; l++) {
+      uc_s[(l * SY + sy) * SX + sx] = u[l];
+      if (pass == 0) uc[l] = u[l];
+    }
+  }
 
-  // sources of the own cell, from the real-frame state (ur, pr)
+  // sources, data checks and CFL of the own interior cell
   double geo[V], ext[V];
   double invdt = -DBL_MAX;
   int flags = 0;
@@ -406,7 +546,7 @@ __global__ __launch_bounds__(TILE_X *BY) void k_rhs(Geom g, StageArgs a) {
     // FARGO source (physics_eulerisotherm.f90:1531-1586)
     if (g.fargo)
       geo[2] = geo[2] - ur[1] * 0.5 * (ring(g, R_W, fi + 1) - ring(g, R_W, fi - 1)) / ring(g, R_DLX, fi);
-    // point-mass gravity summed into the source list (sources_generic.f90:220-261,
+    // point-mass gravity through the source list (sources_generic.f90:220-261,
     // sources_gravity.f90:164-199, physics_eulerisotherm.f90:1201-1226,
     // physics_euler.f90:1849-1885)
     if (g.gravity) {
@@ -415,7 +555,7 @@ __global__ __launch_bounds__(TILE_X *BY) void k_rhs(Geom g, StageArgs a) {
       ext[2] = 0.0 + ur[0] * ay;
       if (V == 4) ext[3] = 0.0 + (ur[1] * ax + ur[2] * ay);
     }
-    // data checks (timedisc_base.f90:1666-1742) and CFL (timedisc_base.f90:818-909)
+    // CheckData (timedisc_base.f90:1666-1742) and CFL (timedisc_base.f90:818-909)
     if (a.want_check) {
       if (pr[0] < g.rhomin) flags |= F_RHOMIN;
       if (V == 4 && pr[3] < g.pmin) flags |= F_PMIN;
@@ -424,8 +564,8 @@ __global__ __launch_bounds__(TILE_X *BY) void k_rhs(Geom g, StageArgs a) {
         if (bad(pr[l]) || bad(ur[l])) flags |= F_INVALID;
     }
     if (a.want_cfl) {
-      double vy = a.sub_bg ? pr[2] - ring(g, R_W, fi) : pr[2];
-      double cs = (V == 4) ? sound_speed(g.gamma, pr[0], pr[3]) : ring(g, R_CS, fi);
+      const double vy = a.sub_bg ? pr[2] - ring(g, R_W, fi) : pr[2];
+      const double cs = (V == 4) ? sound_speed(g.gamma, pr[0], pr[3]) : ring(g, R_CS, fi);
       invdt = cell_invdt(g, fi, pr[1], vy, cs);
     }
   }
@@ -433,15 +573,16 @@ __global__ __launch_bounds__(TILE_X *BY) void k_rhs(Geom g, StageArgs a) {
 
   // ---- phase 2: limited slopes and face states
   const int sx = tx + 1, sy = ty + 1;
-  const bool xface = ty >= 1 && ty <= BY - 2;       // rows that need x fluxes
-  const bool yface = tx >= 1 && tx <= BX - 2;       // columns that need y fluxes
+  const bool xface = ty >= 1 && ty <= BY - 2;  // rows that need x fluxes
+  const bool yface = tx >= 1 && tx <= BX - 2;  // columns that need y fluxes
+  const int fic = fi < g.NI - 1 ? fi : g.NI - 2;
   double fw[V], fe[V], fs[V], fn[V];
   if (xface) {
-    const double dw = ring(g, R_RDXW, fi < g.NI ? fi : g.NI - 1), de = ring(g, R_RDXE, fi < g.NI ? fi : g.NI - 1);
+    const double dw = ring(g, R_RDXW, fic), de = ring(g, R_RDXE, fic);
 #pragma unroll
     for (int l = 0; l < V; l++) {
       const double *r = uc_s + (l * SY + sy) * SX;
-      double sl = g.invdx * slope(g, r[sx - 1], r[sx], r[sx + 1]);
+      const double sl = g.invdx * slope(g, r[sx - 1], r[sx], r[sx + 1]);
       fw[l] = uc[l] + sl * dw;
       fe[l] = uc[l] + sl * de;
     }
@@ -452,7 +593,7 @@ __global__ __launch_bounds__(TILE_X *BY) void k_rhs(Geom g, StageArgs a) {
 #pragma unroll
     for (int l = 0; l < V; l++) {
       const double *c = uc_s + l * SY * SX + sx;
-      double sl = g.invdy * slope(g, c[(sy - 1) * SX], c[sy * SX], c[(sy + 1) * SX]);
+      const double sl = g.invdy * slope(g, c[(sy - 1) * SX], c[sy * SX], c[(sy + 1) * SX]);
       fs[l] = uc[l] + sl * ds;
       fn[l] = uc[l] + sl * dn;
     }
@@ -465,20 +606,19 @@ __global__ __launch_bounds__(TILE_X *BY) void k_rhs(Geom g, StageArgs a) {
   }
   __syncthreads();
 
-  // ---- phase 3: interface fluxes (east interface of x-cells, north of y-cells)
+  // ---- phase 3: interface fluxes (east interface of x cells, north of y cells)
   double xf[V], yf[V];
-  const int fic = fi < g.NI - 1 ? fi : g.NI - 2;
   if (xface && tx <= BX - 2) {
-    double ur2[V];
+    double r2[V];
 #pragma unroll
-    for (int l = 0; l < V; l++) ur2[l] = fa_s[(l * BY + ty) * BX + tx + 1];
-    kt_flux<V>(g, 0, fe, ur2, ring(g, R_CSE, fic), ring(g, R_CSW, fic + 1), ring(g, R_DAXW, fic + 1), xf);
+    for (int l = 0; l < V; l++) r2[l] = fa_s[(l * BY + ty) * BX + tx + 1];
+    kt_flux<V>(g, 0, fe, r2, ring(g, R_CSE, fic), ring(g, R_CSW, fic + 1), ring(g, R_DAXW, fic + 1), xf);
   }
   if (yface && ty <= BY - 2) {
-    double un2[V];
+    double n2[V];
 #pragma unroll
-    for (int l = 0; l < V; l++) un2[l] = fb_s[(l * BY + ty + 1) * BX + tx];
-    kt_flux<V>(g, 1, fn, un2, ring(g, R_CSN, fic), ring(g, R_CSS, fic), ring(g, R_DAYS, fic), yf);
+    for (int l = 0; l < V; l++) n2[l] = fb_s[(l * BY + ty + 1) * BX + tx];
+    kt_flux<V>(g, 1, fn, n2, ring(g, R_CSN, fic), ring(g, R_CSS, fic), ring(g, R_DAYS, fic), yf);
   }
   __syncthreads();
 #pragma unroll
@@ -495,45 +635,96 @@ __global__ __launch_bounds__(TILE_X *BY) void k_rhs(Geom g, StageArgs a) {
     const long ix = cidx(g, fi, fj);
 #pragma unroll
     for (int l = 0; l < V; l++) {
-      double xw = fa_s[(l * BY + ty) * BX + tx - 1], ys = fb_s[(l * BY + ty - 1) * BX + tx];
-      double r = dydzdV * (xf[l] - xw) + dzdxdV * (yf[l] - ys) + dxdydV * (0.0 - 0.0) - geo[l] - ext[l];
-      a.kout[l * g.FS + ix] = r;
+      const double xw = fa_s[(l * BY + ty) * BX + tx - 1], ys = fb_s[(l * BY + ty - 1) * BX + tx];
+      a.kout[l * g.FS + ix] = dydzdV * (xf[l] - xw) + dzdxdV * (yf[l] - ys) + dxdydV * (0.0 - 0.0) - geo[l] - ext[l];
       if (a.yout) a.yout[l * g.FS + ix] = uc[l];
     }
   }
-  if (a.want_cfl || a.want_check) {
-    double m = blk_max(invdt, redd);
-    int f = blk_or(flags, redi);
-    if (tx == 0 && ty == 0) {
-      const int b = a.part_off + blockIdx.x + blockIdx.y * gridDim.x;
-      a.part[2 * b] = m;
-      a.part[2 * b + 1] = (double)f;
+
+  if (!(a.want_cfl || a.want_check)) return;
+  // ---- ghost cells owned by edge blocks: data checks, CFL (ghosts included,
+  // timedisc_base.f90:869-872) and the ghost cells of the new state
+  {
+    constexpr int NXG = 2 * (BY - 2), NYG = 2 * (BX - 2);
+    const bool west = blockIdx.x == 0, east = blockIdx.x == gridDim.x - 1;
+    const bool south = blockIdx.y == 0, north = blockIdx.y == gridDim.y - 1;
+    int i = -1, j = -1;
+    if (t < 2 * NXG) {  // x ghosts on the output rows
+      const int side = t / NXG, r = t % NXG, gl = 1 + r / (BY - 2);
+      j = j0 + r % (BY - 2);
+      if ((side ? east : west) && j <= jmax) i = side ? imax + gl : imin - gl;
+    } else if (t < 2 * NXG + 2 * NYG) {  // y ghosts on the output columns
+      const int k = t - 2 * NXG, side = k / NYG, r = k % NYG, gl = 1 + r / (BX - 2);
+      i = i0 + r % (BX - 2);
+      if ((side ? north : south) && i <= imax) j = side ? jmax + gl : jmin - gl;
+      else i = -1;
+    } else if (t < 2 * NXG + 2 * NYG + 16) {  // corners
+      const int k = t - 2 * NXG - 2 * NYG, c = k / 4, ga = 1 + (k & 1), gb = 1 + ((k >> 1) & 1);
+      const int ce = c & 1, cn = c >> 1;
+      if ((ce ? east : west) && (cn ? north : south)) {
+        i = ce ? imax + ga : imin - ga;
+        j = cn ? jmax + gb : jmin - gb;
+      }
     }
+    if (i >= 0 && j >= 0) {
+      double u[V], p[V];
+      in.ghost_final(i, j, u, p, flags);
+      if (a.want_cfl) {
+        const double cs = (V == 4) ? sound_speed(g.gamma, p[0], p[3]) : ring(g, R_CS, i);
+        invdt = fmax2(invdt, cell_invdt(g, i, p[1], p[2], cs));
+      }
+      if (a.yout) {
+        const long ix = cidx(g, i, j);
+#pragma unroll
+        for (int l = 0; l < V; l++) a.yout[l * g.FS + ix] = u[l];
+      }
+    }
+  }
+  const double m = blk_max(invdt, redd);
+  const int f = blk_or(flags, redi);
+  if (t == 0) {
+    const int b = blockIdx.x + blockIdx.y * gridDim.x;
+    a.part[2 * b] = m;
+    a.part[2 * b + 1] = (double)f;
   }
 }
 
 // ------------------------------------------------------ final combination
-template <int V>
+// ynew = Y - sum_q (bh_q*dt) k_q and the error estimate from the embedded
+// solution (timedisc_rkfehlberg.f90:199-266, timedisc_base.f90:1012-1053);
+// NK = number of stages, so every load of a cell is issued before any use
+template <int V, int NK>
 __global__ __launch_bounds__(256) void k_combine(Geom g, CombineArgs a) {
   __shared__ double redd[4];
+  if (a.ctl && a.ctl->halt) return;
+  const double dt = a.ctl ? a.ctl->dt : a.dt;
+  double ch[NK], cl[NK];
+#pragma unroll
+  for (int q = 0; q < NK; q++) ch[q] = a.bh[q] * dt, cl[q] = a.bl[q] * dt;
   const int i = GH + blockIdx.x * 64 + threadIdx.x, j = GH + blockIdx.y * 4 + threadIdx.y;
   double e = -DBL_MAX;
   if (i < GH + g.inum && j < GH + g.jnum) {
     const long ix = cidx(g, i, j);
+    double y[V], k[NK][V];
 #pragma unroll
     for (int l = 0; l < V; l++) {
-      double y = a.Y[l * g.FS + ix], yh = y, yl = y;
-      for (int q = 0; q < a.nk; q++) {
-        double k = a.k[q][l * g.FS + ix];
-        yl = yl - a.cl[q] * k;
-        yh = yh - a.ch[q] * k;
+      y[l] = a.Y[l * g.FS + ix];
+#pragma unroll
+      for (int q = 0; q < NK; q++) k[q][l] = a.k[q][l * g.FS + ix];
+    }
+#pragma unroll
+    for (int l = 0; l < V; l++) {
+      double yh = y[l], yl = y[l];
+#pragma unroll
+      for (int q = 0; q < NK; q++) {
+        yl = yl - cl[q] * k[q][l];
+        yh = yh - ch[q] * k[q][l];
       }
       a.ynew[l * g.FS + ix] = yh;
-      double c = fabs(yh - yl) / (a.tol_rel * fabs(yh) + a.tol_abs[l]);
-      e = fmax2(e, c);
+      e = fmax2(e, fabs(yh - yl) / (a.tol_rel * fabs(yh) + a.tol_abs[l]));
     }
   }
-  double m = blk_max(e, redd);
+  const double m = blk_max(e, redd);
   if (threadIdx.x == 0 && threadIdx.y == 0) {
     const int b = blockIdx.x + blockIdx.y * gridDim.x;
     a.part[2 * b] = m;
@@ -541,47 +732,27 @@ __global__ __launch_bounds__(256) void k_combine(Geom g, CombineArgs a) {
   }
 }
 
-// ------------------------------------------------------------ FARGO remap
-// timedisc_base.f90:1935-2069: minmod-limited linear transport by the
-// fractional shift, then the integer circular shift CSHIFT(., -shift).
-// One thread per interior output cell; x-ghost rings are rebuilt by the next
-// boundary update, so only interior rings are transported.
-template <int V>
-__global__ __launch_bounds__(256) void k_remap(Geom g, const double *u, double *z, double dt) {
-  const int i = GH + blockIdx.x * 64 + threadIdx.x, j = GH + blockIdx.y * 4 + threadIdx.y;
-  if (i >= GH + g.inum || j >= GH + g.jnum) return;
-  const int jmin = GH, J = g.jnum;
-  double d = ring(g, R_W, i) * dt / ring(g, R_DLY, i);
-  double shift = round(d);
-  d = d - shift;
-  long sh = (long)shift;
-  // source row of the integer shift: result(j) = a(j - shift) cyclically
-  long jj = (long)(j - jmin) - sh;
-  jj %= J;
-  if (jj < 0) jj += J;
-  const int js = jmin + (int)jj;
-#pragma unroll
-  for (int l = 0; l < V; l++) {
-    const double *c = u + l * g.FS + i;
-    double v[5];  // rows js-2 .. js+2
-#pragma unroll
-    for (int q = 0; q < 5; q++) v[q] = c[(long)(js - 2 + q) * g.P];
-    double dq[4];
-#pragma unroll
-    for (int q = 0; q < 4; q++) dq[q] = v[q + 1] - v[q];  // dq(js-2+q)
-    double dql[3];                                     // dql(js-1+q)
-#pragma unroll
-    for (int q = 0; q < 3; q++)
-      dql[q] = (fsign(1.0, dq[q]) * fsign(1.0, dq[q + 1]) > 0.) ? fsign(fmin2(fabs(dq[q]), fabs(dq[q + 1])), dq[q]) : 0.;
-    double f0, f1;  // flux(js-1), flux(js)
-    if (d > 0.) {
-      f0 = v[1] + .5 * dql[0] * (1. - d);
-      f1 = v[2] + .5 * dql[1] * (1. - d);
-    } else {
-      f0 = v[2] - .5 * dql[1] * (1. + d);
-      f1 = v[3] - .5 * dql[2] * (1. + d);
-    }
-    z[l * g.FS + cidx(g, i, j)] = v[2] - d * (f1 - f0);
+// Step control of a device-controlled batch (one block): embedded error of
+// the step from the k_combine partials, accept test, then CFL and data checks
+// from the end-of-step k_rhs partials and the next time step.
+__global__ __launch_bounds__(1024) void k_control(Ctl *ctl, const double *part_c, int nc, const double *part_r,
+                                                  int nr) {
+  __shared__ double redd[16];
+  __shared__ int redi[16];
+  if (ctl->halt) return;
+  double e = -DBL_MAX, m = -DBL_MAX;
+  int f = 0;
+  for (int b = threadIdx.x; b < nc; b += blockDim.x) e = fmax2(e, part_c[2 * b]);
+  for (int b = threadIdx.x; b < nr; b += blockDim.x) {
+    m = fmax2(m, part_r[2 * b]);
+    f |= (int)part_r[2 * b + 1];
+  }
+  e = blk_max(e, redd);
+  m = blk_max(m, redd);
+  f = blk_or(f, redi);
+  if (threadIdx.x == 0) {
+    control_accept(ctl, e);
+    if (!ctl->halt) control_next(ctl, m, f);
   }
 }
 
@@ -620,42 +791,50 @@ __global__ __launch_bounds__(256) void k_reduce(const double *part, int n, doubl
 }
 
 // -------------------------------------------------------------- launchers
-constexpr int RHS_BY = 8;
+static int g_by = 8;
+void set_rhs_rows(int by) { g_by = (by == 16) ? 16 : 8; }
 
 int rhs_blocks(const Geom &g) {
-  return ((g.inum + TILE_X - 3) / (TILE_X - 2)) * ((g.jnum + RHS_BY - 3) / (RHS_BY - 2));
+  return ((g.inum + TILE_X - 3) / (TILE_X - 2)) * ((g.jnum + g_by - 3) / (g_by - 2));
 }
-int ghost_blocks(const Geom &g) { return (4 * g.jnum + 4 * g.inum + 16 + 255) / 256; }
 int combine_blocks(const Geom &g) { return ((g.inum + 63) / 64) * ((g.jnum + 3) / 4); }
 
-void launch_ghost(const Geom &g, const StageArgs &a, int V, hipStream_t s) {
-  dim3 grid(ghost_blocks(g)), block(256);
-  if (V == 3)
-    hipLaunchKernelGGL(k_ghost<3>, grid, block, 0, s, g, a);
+template <int V, int BY>
+static void rhs_by(const Geom &g, const StageArgs &a, int mode, hipStream_t s) {
+  dim3 grid((g.inum + TILE_X - 3) / (TILE_X - 2), (g.jnum + BY - 3) / (BY - 2)), block(TILE_X, BY);
+  if (mode == IN_REMAP)
+    hipLaunchKernelGGL((k_rhs<V, BY, IN_REMAP>), grid, block, 0, s, g, a);
   else
-    hipLaunchKernelGGL(k_ghost<4>, grid, block, 0, s, g, a);
+    hipLaunchKernelGGL((k_rhs<V, BY, IN_STAGE>), grid, block, 0, s, g, a);
 }
-void launch_rhs(const Geom &g, const StageArgs &a, int V, hipStream_t s) {
-  dim3 grid((g.inum + TILE_X - 3) / (TILE_X - 2), (g.jnum + RHS_BY - 3) / (RHS_BY - 2));
-  dim3 block(TILE_X, RHS_BY);
-  if (V == 3)
-    hipLaunchKernelGGL((k_rhs<3, RHS_BY>), grid, block, 0, s, g, a);
-  else
-    hipLaunchKernelGGL((k_rhs<4, RHS_BY>), grid, block, 0, s, g, a);
+void launch_rhs(const Geom &g, const StageArgs &a, int V, int mode, hipStream_t s) {
+  if (V == 3) {
+    if (g_by == 16) rhs_by<3, 16>(g, a, mode, s);
+    else rhs_by<3, 8>(g, a, mode, s);
+  } else {
+    if (g_by == 16) rhs_by<4, 16>(g, a, IN_STAGE, s);
+    else rhs_by<4, 8>(g, a, IN_STAGE, s);
+  }
+}
+template <int V>
+static void combine_v(const Geom &g, const CombineArgs &a, hipStream_t s) {
+  dim3 grid((g.inum + 63) / 64, (g.jnum + 3) / 4), block(64, 4);
+  switch (a.nk) {
+    case 3: hipLaunchKernelGGL((k_combine<V, 3>), grid, block, 0, s, g, a); break;
+    case 5: hipLaunchKernelGGL((k_combine<V, 5>), grid, block, 0, s, g, a); break;
+    case 6: hipLaunchKernelGGL((k_combine<V, 6>), grid, block, 0, s, g, a); break;
+    case 7: hipLaunchKernelGGL((k_combine<V, 7>), grid, block, 0, s, g, a); break;
+    default: break;  // the host accepts only tableaux with 3, 5, 6 or 7 stages
+  }
 }
 void launch_combine(const Geom &g, const CombineArgs &a, int V, hipStream_t s) {
-  dim3 grid((g.inum + 63) / 64, (g.jnum + 3) / 4), block(64, 4);
   if (V == 3)
-    hipLaunchKernelGGL(k_combine<3>, grid, block, 0, s, g, a);
+    combine_v<3>(g, a, s);
   else
-    hipLaunchKernelGGL(k_combine<4>, grid, block, 0, s, g, a);
+    combine_v<4>(g, a, s);
 }
-void launch_remap(const Geom &g, const double *u, double *z, double dt, int V, hipStream_t s) {
-  dim3 grid((g.inum + 63) / 64, (g.jnum + 3) / 4), block(64, 4);
-  if (V == 3)
-    hipLaunchKernelGGL(k_remap<3>, grid, block, 0, s, g, u, z, dt);
-  else
-    hipLaunchKernelGGL(k_remap<4>, grid, block, 0, s, g, u, z, dt);
+void launch_control(Ctl *ctl, const double *part_c, int nc, const double *part_r, int nr, hipStream_t s) {
+  hipLaunchKernelGGL(k_control, dim3(1), dim3(1024), 0, s, ctl, part_c, nc, part_r, nr);
 }
 void launch_frame(const Geom &g, const double *src, double *dst, int sign, int V, hipStream_t s) {
   dim3 grid((g.NI + 63) / 64, (g.NJ + 3) / 4), block(64, 4);

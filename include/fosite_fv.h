@@ -103,6 +103,19 @@ int fv_get_scalars(const fv_ctx *ctx, fv_scalars *out);
 double fv_last_run_ms(const fv_ctx *ctx);
 const char *fv_last_error(const fv_ctx *ctx);
 
+/* Kernel timing (no reference counterpart; measurement support).
+ * fv_profile(ctx, 1) resets the statistics and brackets every kernel launch
+ * with HIP events on the context's stream (launches are then issued one by
+ * one instead of as HIP graphs).  Kernel classes: 0 fused RHS on an RK stage
+ * state, 1 fused RHS on the FARGO-remapped state (end of step), 2 final RK
+ * combination with embedded error, 3 FARGO frame change, 4 step control of
+ * the device-controlled batches (one block).  fv_kernel_stats
+ * returns the number of launches, their summed device time in ms and their
+ * algorithmic bytes (state words each launch must read and write, 8 bytes
+ * per word, interior cells). */
+int fv_profile(fv_ctx *ctx, int enable);
+int fv_kernel_stats(const fv_ctx *ctx, int kernel, int *launches, double *ms, double *alg_bytes);
+
 #ifdef __cplusplus
 }
 #endif

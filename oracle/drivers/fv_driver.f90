@@ -41,6 +41,7 @@ PROGRAM fv_driver
                                 src, pmass
   CHARACTER(LEN=512)         :: cfgname
   INTEGER                    :: u, n, vnum, ncell
+  INTEGER(KIND=8)            :: c0, c1, crate, ctot
   LOGICAL                    :: brk
   REAL, ALLOCATABLE          :: pv(:), cs(:), fcs(:), w(:)
   !--------------------------------------------------------------------------!
@@ -119,13 +120,20 @@ PROGRAM fv_driver
        FORM="unformatted", STATUS="replace")
   CALL Sim%FirstStep()
   CALL DumpState(u, 0)
+  ctot = 0
   DO n = 1, k_nsteps
+    CALL SYSTEM_CLOCK(c0, crate)
     brk = Sim%Step()
+    CALL SYSTEM_CLOCK(c1)
+    ctot = ctot + (c1 - c0)
     IF (MOD(n, k_dumpevery).EQ.0 .OR. n.EQ.k_nsteps .OR. brk .OR. Sim%Timedisc%break) &
       CALL DumpState(u, n)
     IF (brk .OR. Sim%Timedisc%break) EXIT
   END DO
   CLOSE(u)
+  ! wall time of the Step() calls only (setup, FirstStep and dumps excluded)
+  WRITE(*,'(A,I0,A,ES24.16)') "FV_DRIVER_STEP_SECONDS ", MIN(n, k_nsteps), " ", &
+       REAL(ctot, 8) / REAL(crate, 8)
   CALL Sim%Finalize()
   DEALLOCATE(Sim)
 

@@ -5,6 +5,7 @@ FirstStep + nsteps x Step (fosite.f90:258-464) through our setup program
 oracle/drivers/fv_driver.f90, and parses the dumped records.
 """
 import os
+import shutil
 import subprocess
 import tempfile
 
@@ -64,6 +65,11 @@ def run(cfg, init, nsteps, dump_every=1, workdir=None):
     r = subprocess.run([DRIVER, nml], cwd=tmp, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("reference driver failed:\n" + r.stdout[-3000:] + r.stderr[-3000:])
+    step_seconds = None
+    for line in r.stdout.splitlines():
+        if line.startswith("FV_DRIVER_STEP_SECONDS"):
+            _, ns, sec = line.split()
+            step_seconds = (int(ns), float(sec))
     m = np.fromfile(prefix + "_mesh.bin", dtype="<f8")
     o = 4
     mesh = dict(dx=m[0], dy=m[1], invdx=m[2], invdy=m[3])
@@ -90,4 +96,7 @@ def run(cfg, init, nsteps, dump_every=1, workdir=None):
         recs.append(dict(step=int(ib[0]), iter=int(ib[1]), n_adj=int(ib[2]), brk=int(ib[3]),
                          time=rb[0], dt=rb[1], dtold=rb[2], maxerrold=rb[3], cvar=cv, pvar=pv))
         off += rec
+    mesh["step_seconds"] = step_seconds
+    if workdir is None:
+        shutil.rmtree(tmp, ignore_errors=True)
     return mesh, recs

@@ -1,0 +1,79 @@
+"""Write the golden fixtures of the FV step from the compiled reference.
+
+Runs oracle/_ref/fv_driver -- the reference's own Fortran sources compiled
+by oracle/build_ref.py, driven through FirstStep + Step() (fosite.f90:258-464)
+-- on small seeded configurations and stores, per case, the configuration,
+the step scalars after every step and the conservative state after chosen
+steps, plus the reference's mesh arrays for one Cartesian and one
+logcylindrical mesh.  Output: tests/golden/fv_golden.npz (data only).
+
+Usage (needs the reference checkout and its build):
+    python oracle/build_ref.py && python tests/golden/make_golden.py
+"""
+import json
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from fosite_amd import setups as S  # noqa: E402
+from oracle import refrun  # noqa: E402
+
+OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "fv_golden.npz")
+
+E = S.riemann2d_config(n=16)
+D = S.kepler_disk_config(ni=16, nj=32)
+
+# name: (config, steps, steps whose state is stored)
+CASES = {
+    "euler_vanleer_ssprk3_24": (S.riemann2d_config(n=24), 12, (1, 6, 12)),
+    "euler_minmod_ssprk5": (dict(E, limiter=S.MINMOD, order=5), 6, (1, 6)),
+    "euler_mc_1.2": (dict(E, limiter=S.MONOCENT, theta=1.2), 6, (6,)),
+    "euler_sweby_1.5": (dict(E, limiter=S.SWEBY, theta=1.5), 6, (6,)),
+    "euler_superbee": (dict(E, limiter=S.SUPERBEE), 6, (6,)),
+    "euler_ospre": (dict(E, limiter=S.OSPRE), 6, (6,)),
+    "euler_nolimit": (dict(E, limiter=S.NOLIMIT), 3, (3,)),
+    "euler_periodic_rkf3": (dict(E, bc=[S.PERIODIC] * 4, method=S.RK_FEHLBERG, order=3), 8, (8,)),
+    "euler_dp5": (dict(E, method=S.DORMAND_PRINCE, order=5, tol_rel=1e-4), 6, (6,)),
+    "euler_rkf5": (dict(E, method=S.RK_FEHLBERG, order=5), 6, (6,)),
+    "euler_ragged_20x12": (dict(E, inum=20, jnum=12), 8, (8,)),
+    "disk_fargo_absorbing_16x32": (D, 10, (1, 10)),
+    "disk_no_fargo": (dict(D, fargo=0), 8, (8,)),
+    "disk_nograd_radial": (dict(D, bc=[S.NO_GRADIENTS, S.NO_GRADIENTS, S.PERIODIC, S.PERIODIC]), 8, (8,)),
+    "disk_cfl_0.9": (dict(D, cfl=0.9), 12, (12,)),
+}
+MESHES = {"cartesian_16": E, "logcyl_16x32": D}
+SCALARS = ("iter", "n_adj", "brk", "time", "dt", "dtold", "maxerrold")
+
+
+def main():
+    if not refrun.available():
+        sys.exit("reference build missing: run oracle/build_ref.py first")
+    out = {}
+    for name, (cfg, steps, keep) in CASES.items():
+        init = S.initial(cfg)
+        _, recs = refrun.run(cfg, init, steps, dump_every=1)
+        assert [r["step"] for r in recs] == list(range(steps + 1)), name
+        out[name + "/config"] = np.frombuffer(json.dumps(
+            {k: v for k, v in cfg.items()}).encode(), dtype=np.uint8)
+        out[name + "/steps"] = np.array([steps])
+        for k in SCALARS:
+            out[name + "/" + k] = np.array([r[k] for r in recs])
+        for s in keep:
+            out["%s/cvar_%d" % (name, s)] = recs[s]["cvar"]
+        print(name, "n_adj", recs[-1]["n_adj"], "time %.6e" % recs[-1]["time"])
+    for name, cfg in MESHES.items():
+        mesh, _ = refrun.run(cfg, S.initial(cfg), 0, dump_every=1)
+        for k, v in mesh.items():
+            if isinstance(v, np.ndarray):
+                out["mesh_%s/%s" % (name, k)] = v
+        out["mesh_%s/config" % name] = np.frombuffer(json.dumps(cfg).encode(), dtype=np.uint8)
+    np.savez_compressed(OUT, **out)
+    print("wrote", OUT, os.path.getsize(OUT), "bytes")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,43 @@
+"""Access to the golden fixtures written by tests/golden/make_golden.py from
+the compiled reference (data only: configurations, per-step scalars, states,
+mesh arrays)."""
+import json
+import os
+
+import numpy as np
+
+PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "fv_golden.npz")
+SCALARS = ("iter", "n_adj", "brk", "time", "dt", "dtold", "maxerrold")
+
+_data = None
+
+
+def data():
+    global _data
+    if _data is None:
+        _data = dict(np.load(PATH, allow_pickle=False))
+    return _data
+
+
+def cases():
+    return sorted({k.split("/")[0] for k in data() if not k.startswith("mesh_")})
+
+
+def case(name):
+    d = data()
+    cfg = json.loads(bytes(d[name + "/config"]).decode())
+    steps = int(d[name + "/steps"][0])
+    scalars = {k: d[name + "/" + k] for k in SCALARS}
+    states = {int(k.split("_")[-1]): d[k] for k in d if k.startswith(name + "/cvar_")}
+    return cfg, steps, scalars, states
+
+
+def meshes():
+    return sorted({k.split("/")[0][5:] for k in data() if k.startswith("mesh_")})
+
+
+def mesh(name):
+    d = data()
+    cfg = json.loads(bytes(d["mesh_%s/config" % name]).decode())
+    arrays = {k.split("/")[1]: d[k] for k in d if k.startswith("mesh_%s/" % name) and not k.endswith("/config")}
+    return cfg, arrays

@@ -42,3 +42,36 @@ def run_pair(cfg, nsteps, every=1):
             assert d1 == d2
             break
     sim.close()
+
+
+def run_batched(cfg, nsteps, chunk):
+    """Like run_pair, but the HIP side advances with Fosite.run(chunk) (the
+    device-controlled batches) while the oracle steps one at a time.
+    Yields (steps so far, gpu_cvar, gpu_scalars, oracle_cvar, oracle_scalars,
+    run ended)."""
+    import fosite_amd
+    import oracle
+    init = setups.initial(cfg)
+    sim = fosite_amd.Fosite(cfg)
+    ref = oracle.Oracle(cfg)
+    args = (init["pvar"], init.get("bccsound"), init.get("fcsound"), init.get("w"))
+    sim.set_initial(*args)
+    ref.set_initial(*args)
+    sim.first_step()
+    ref.first_step()
+    n = 0
+    while n < nsteps:
+        k = min(chunk, nsteps - n)
+        taken = sim.run(k)
+        for _ in range(taken):
+            assert not ref.step()
+        n += taken
+        ended = taken < k
+        if ended:
+            assert ref.step(), "oracle did not end where the HIP run ended"
+        cv, _ = sim.state()
+        rc, _, rs = ref.state()
+        yield n, cv, sim.scalars(), rc, rs, ended
+        if ended:
+            break
+    sim.close()

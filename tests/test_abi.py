@@ -1,0 +1,66 @@
+"""The C ABI library on the CPU: it loads, exports every entry point that
+include/fosite_fv.h declares, computes the host-side mesh metric exactly like
+the reference, and rejects unsupported configurations before touching a GPU."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import fosite_amd
+from fosite_amd import setups as S
+from tests import golden_cases as G
+from tests.test_oracle_golden import MESH_ARRAYS
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "fosite_fv.h")
+
+
+def declared():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(fv_[a-z_0-9]+)\s*\(", src)))
+
+
+def test_header_declares_the_python_export_list():
+    assert declared() == sorted(fosite_amd.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    L = fosite_amd.load_library()
+    for name in declared():
+        assert hasattr(L, name), name
+
+
+@pytest.mark.parametrize("name", G.meshes())
+def test_host_mesh_matches_reference_fixture(name):
+    cfg, arrays = G.mesh(name)
+    for ours, (key, idx) in MESH_ARRAYS.items():
+        ref = arrays[key] if idx is None else arrays[key][idx]
+        assert np.array_equal(fosite_amd.mesh_array(cfg, ours), ref), (name, ours)
+
+
+BAD = {
+    "physics": dict(physics=99),
+    "geometry": dict(geometry=3),
+    "primitive reconstruction": dict(recvars=S.PRIMITIVE),
+    "limiter": dict(limiter=6),
+    "unpaired periodic": dict(bc=[S.PERIODIC, S.NO_GRADIENTS, S.NO_GRADIENTS, S.NO_GRADIENTS]),
+    "absorbing in y": dict(bc=[S.NO_GRADIENTS, S.NO_GRADIENTS, S.ABSORBING, S.NO_GRADIENTS]),
+    "fargo on cartesian": dict(fargo=2),
+    "tableau": dict(method=S.SSPRK, order=4),
+    "tiny mesh": dict(inum=3),
+    "dtype": dict(dtype=1),
+}
+
+
+@pytest.mark.parametrize("what", list(BAD))
+def test_create_rejects_unsupported_configuration(what):
+    L = fosite_amd.load_library()
+    cfg = dict(S.riemann2d_config(n=16), **BAD[what])
+    c = fosite_amd.FvConfig.from_dict(cfg)
+    if what == "dtype":
+        c.dtype = 1
+    h = ctypes.c_void_p()
+    assert L.fv_create(ctypes.byref(h), ctypes.byref(c)) == -1
+    assert not h.value

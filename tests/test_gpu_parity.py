@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 from fosite_amd import setups as S
-from tests.parity import rel_l1, run_pair
+from tests.parity import rel_l1, run_batched, run_pair
 
 E = S.riemann2d_config(n=48)
 D = S.kepler_disk_config(ni=40, nj=64)
@@ -67,3 +67,40 @@ def test_gpu_config1_full_size_bitexact():
         assert np.array_equal(cv[I], rc[I]), (n, rel_l1(cv, rc))
         last = n
     assert last == 200
+
+
+# Fosite.run() advances in device-controlled batches (HIP graphs, step control
+# on the device, host scalars replayed); steps the device cannot decide alone
+# (rejections, output and stop times, breaks) are redone by the host path.
+BATCH_CASES = {
+    "euler_vanleer_48": (E, 90, 40, 0.0),
+    "euler_dp5": (dict(E, method=S.DORMAND_PRINCE, order=5, tol_rel=1e-4), 40, 16, 0.0),
+    "euler_rkf5_periodic": (dict(E, method=S.RK_FEHLBERG, order=5, bc=[S.PERIODIC] * 4), 40, 13, 0.0),
+    "euler_stop_inside_batch": (dict(E, stoptime=0.05), 400, 300, 0.0),
+    "disk_nograd_radial": (dict(D, bc=[S.NO_GRADIENTS, S.NO_GRADIENTS, S.PERIODIC, S.PERIODIC]), 60, 25, 0.0),
+    "disk_fargo_absorbing": (D, 90, 40, 1e-12),
+    "disk_cfl_0.9_rejects": (dict(D, cfl=0.9), 60, 30, 1e-12),
+    "disk_ragged_100x90": (dict(D, inum=100, jnum=90), 40, 17, 1e-12),
+}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", list(BATCH_CASES))
+def test_gpu_batched_run_matches_oracle(name):
+    cfg, nsteps, chunk, tol = BATCH_CASES[name]
+    print("batched case", name, cfg["inum"], cfg["jnum"], nsteps, chunk, flush=True)
+    I = (slice(None), slice(2, -2), slice(2, -2))
+    ended = False
+    for n, cv, sc, rc, rs, ended in run_batched(cfg, nsteps, chunk):
+        for key in ("iter", "n_adj", "brk"):
+            assert sc[key] == rs[key], (n, key, sc, rs)
+        if tol == 0.0:
+            for key in ("time", "dt", "dtold", "maxerrold"):
+                assert sc[key] == rs[key], (n, key, sc, rs)
+            assert np.array_equal(cv[I], rc[I]), (n, np.abs(cv[I] - rc[I]).max())
+        else:
+            for key in ("time", "dt", "dtold", "maxerrold"):
+                assert abs(sc[key] - rs[key]) <= tol * max(abs(rs[key]), 1e-300), (n, key, sc, rs)
+            assert (rel_l1(cv, rc) <= tol).all(), (n, rel_l1(cv, rc))
+    if name == "euler_stop_inside_batch":
+        assert ended

@@ -1,0 +1,149 @@
+"""The CPU oracle against the reference.
+
+* golden fixtures written from the compiled reference (tests/golden): every
+  step's scalars and the stored states must match bit for bit;
+* the live reference build (oracle/_ref, present only where the reference
+  checkout is) on a configuration outside the fixtures;
+* mathematical invariants the reference's algorithm fixes: exact transpose
+  symmetry of the four-quadrant Riemann problem, exact free-stream
+  preservation, mass conservation on a periodic box, the point-mass
+  acceleration in closed form."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from fosite_amd import setups as S
+from oracle import refrun
+from tests import golden_cases as G
+
+MESH_ARRAYS = {  # oracle.Oracle.mesh name -> reference dump key (index)
+    "dAx_w": ("dAx", 0), "dAx_e": ("dAx", 1), "dAy_s": ("dAy", 0), "dAy_n": ("dAy", 1),
+    "dydzdV": ("dydzdV", None), "dzdxdV": ("dzdxdV", None), "dxdydV": ("dxdydV", None),
+    "dlx": ("dlx", None), "dly": ("dly", None), "cxyx": ("cxyx", None), "cyxy": ("cyxy", None),
+    "czxz": ("czxz", None), "czyz": ("czyz", None), "rdx_w": ("rdx_w", None),
+    "rdx_e": ("rdx_e", None), "rdx_s": ("rdx_s", None), "rdx_n": ("rdx_n", None),
+    "radius": ("radius", None), "radius_w": ("radius_w", None), "radius_e": ("radius_e", None),
+    "radius_s": ("radius_s", None), "radius_n": ("radius_n", None),
+    "posvec_x": ("posvec_x", None), "posvec_y": ("posvec_y", None),
+}
+
+
+def run_oracle(cfg, steps, keep=()):
+    init = S.initial(cfg)
+    o = oracle.Oracle(cfg)
+    o.set_initial(init["pvar"], init.get("bccsound"), init.get("fcsound"), init.get("w"))
+    o.first_step()
+    rows, states = [], {}
+    cv, _, sc = o.state()
+    rows.append(sc)
+    if 0 in keep:
+        states[0] = cv
+    for n in range(1, steps + 1):
+        o.step()
+        cv, _, sc = o.state()
+        rows.append(sc)
+        if n in keep:
+            states[n] = cv
+    return rows, states
+
+
+@pytest.mark.parametrize("name", G.cases())
+def test_oracle_matches_reference_fixture(name):
+    cfg, steps, scalars, states = G.case(name)
+    rows, got = run_oracle(cfg, steps, keep=tuple(states))
+    for k in G.SCALARS:
+        assert np.array_equal(np.array([r[k] for r in rows]), scalars[k]), (name, k)
+    I = (slice(None), slice(2, -2), slice(2, -2))
+    for s, ref in states.items():
+        assert np.array_equal(got[s][I], ref[I]), (name, s, np.abs(got[s][I] - ref[I]).max())
+
+
+@pytest.mark.parametrize("name", G.meshes())
+def test_oracle_mesh_matches_reference_fixture(name):
+    cfg, arrays = G.mesh(name)
+    o = oracle.Oracle(cfg)
+    for ours, (key, idx) in MESH_ARRAYS.items():
+        ref = arrays[key] if idx is None else arrays[key][idx]
+        assert np.array_equal(o.mesh(ours), ref), (name, ours)
+
+
+def test_fixture_cases_exercise_rejections_and_every_limiter():
+    rejected = [n for n in G.cases() if G.case(n)[2]["n_adj"].max() > 0]
+    assert len(rejected) >= 3, rejected
+    limiters = {G.case(n)[0]["limiter"] for n in G.cases()}
+    assert limiters >= {S.MINMOD, S.MONOCENT, S.SWEBY, S.SUPERBEE, S.OSPRE, S.VANLEER, S.NOLIMIT}
+
+
+@pytest.mark.skipif(not refrun.available(), reason="reference build (oracle/_ref) absent")
+def test_oracle_matches_live_reference_outside_fixtures():
+    cfg = dict(S.riemann2d_config(n=14), jnum=10, bc=[S.PERIODIC, S.PERIODIC, S.NO_GRADIENTS, S.NO_GRADIENTS],
+               limiter=S.SWEBY, theta=1.3, method=S.SSPRK, order=5, cfl=0.5)
+    _, recs = refrun.run(cfg, S.initial(cfg), 7, dump_every=1)
+    rows, states = run_oracle(cfg, 7, keep=(7,))
+    for k in G.SCALARS:
+        assert [r[k] for r in rows] == [rec[k] for rec in recs], k
+    I = (slice(None), slice(2, -2), slice(2, -2))
+    assert np.array_equal(states[7][I], recs[7]["cvar"][I])
+
+
+def test_transpose_symmetry_is_exact():
+    """The quadrant data is symmetric under x <-> y with u <-> v; the unsplit
+    scheme adds the x and y flux differences with commutative FP additions,
+    so the discrete solution stays exactly symmetric."""
+    _, states = run_oracle(S.riemann2d_config(n=24), 10, keep=(10,))
+    cv = states[10]
+    assert np.array_equal(cv[0], cv[0].T)
+    assert np.array_equal(cv[1], cv[2].T)
+    assert np.array_equal(cv[3], cv[3].T)
+    assert not np.array_equal(cv[0][2:-2, 2:-2], cv[0][2, 2])  # not trivially uniform
+
+
+def test_free_stream_is_preserved_exactly():
+    cfg = dict(S.riemann2d_config(n=12), bc=[S.PERIODIC] * 4)
+    o = oracle.Oracle(cfg)
+    pv = np.zeros((4, 16, 16))
+    pv[0], pv[1], pv[2], pv[3] = 1.3, 0.7, -0.4, 2.1
+    o.set_initial(pv)
+    o.first_step()
+    cv0, _, _ = o.state()
+    for _ in range(5):
+        o.step()
+    cv, _, sc = o.state()
+    I = (slice(None), slice(2, -2), slice(2, -2))
+    assert sc["time"] > 0
+    assert np.array_equal(cv[I], cv0[I])
+
+
+def test_mass_is_conserved_on_a_periodic_box():
+    cfg = dict(S.riemann2d_config(n=20), bc=[S.PERIODIC] * 4)
+    o = oracle.Oracle(cfg)
+    init = S.initial(cfg)
+    o.set_initial(init["pvar"])
+    o.first_step()
+    cv0, _, _ = o.state()
+    for _ in range(10):
+        o.step()
+    cv, _, _ = o.state()
+    I = (slice(2, -2), slice(2, -2))
+    m0, m1 = cv0[0][I].sum(), cv[0][I].sum()
+    e0, e1 = cv0[3][I].sum(), cv[3][I].sum()
+    assert abs(m1 - m0) <= 1e-13 * m0 and abs(e1 - e0) <= 1e-13 * e0
+    assert not np.array_equal(cv[0][I], cv0[0][I])
+
+
+def test_point_mass_acceleration_closed_form():
+    """gravity_pointmass: a = -GM r_hat / r^2 at the cell centres of a
+    logcylindrical mesh, i.e. purely radial (x) with magnitude GM / r^2."""
+    cfg = S.kepler_disk_config(ni=16, nj=32)
+    o = oracle.Oracle(cfg)
+    r = o.mesh("radius")
+    ax, ay = o.mesh("accel_x"), o.mesh("accel_y")
+    GM = cfg["GN"] * cfg["mass"]
+    np.testing.assert_allclose(ax, -GM / r ** 2, rtol=1e-14, atol=0)
+    assert np.abs(ay).max() <= 1e-15 * np.abs(ax).max()
+    # radius of a logcylindrical cell centre: gparam * exp(x)
+    xc, _, _, _ = S.coords(cfg)
+    np.testing.assert_allclose(r[0], cfg["gparam"] * np.exp(xc), rtol=1e-15)
+    assert math.isclose(r[0, 2], math.exp(cfg["xmin"] + 0.5 * (cfg["xmax"] - cfg["xmin"]) / 16), rel_tol=1e-15)
