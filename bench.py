@@ -29,14 +29,15 @@ TRAFFIC_BYTES_PER_LAUNCH = None
 METRIC = "cell-updates/s & HBM GB/s vs plain FP64 at stated err; p50/p99 run latency"
 
 
-def b_alg(V=3, w=8, s=3):
-    """Algorithmic HBM bytes per cell-update of one s-stage step: stage m
-    (2..s) reads the state and m-1 stage slopes and writes one slope (m+1
-    words); the final combination reads the state and s slopes and writes the
-    new state (s+2); the end-of-step RHS (with the FARGO remap folded in)
-    reads the combined state and writes the first slope of the next step and
-    the remapped state (3).  w bytes per word, V conserved fields."""
-    return w * V * (sum(m + 1 for m in range(2, s + 1)) + (s + 2) + 3)
+def b_alg(V=3, w=8, s=3, fargo=True):
+    """Algorithmic HBM bytes per cell-update of one s-stage step as this
+    implementation moves them: stage m (2..s-1) reads the state and m-1 stage
+    slopes and writes one slope (m+1 words); the last stage also forms the
+    final combination, reading the state and s-1 slopes and writing the new
+    state (s+1); the FARGO remap reads and writes the state (2); the
+    end-of-step RHS reads it and writes the first slope of the next step and
+    the stored state (3).  w bytes per word, V conserved fields."""
+    return w * V * (sum(m + 1 for m in range(2, s)) + (s + 1) + (2 if fargo else 0) + 3)
 
 
 def cpu_baseline(cfg, init, steps):
@@ -92,7 +93,7 @@ def main():
     # launch bracketed by HIP events on the simulation's stream
     sim.profile(True)
     sim.run(a.profile_steps)
-    stats = {k: sim.kernel_stats(k) for k in ("rhs", "rhs_remap", "combine")}
+    stats = {k: sim.kernel_stats(k) for k in ("rhs", "rhs_combine", "rhs_end", "remap", "control")}
     sim.profile(False)
     n_rhs, ms_rhs, bytes_rhs = stats["rhs"]
     achieved = bytes_rhs / (ms_rhs * 1e-3) / 1e9
@@ -121,7 +122,7 @@ def main():
                    "sim_time_reached": sc["time"], "rejected_steps": sc["n_adj"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_BYTES_PER_LAUNCH,
-                     "kernel": "fv::k_rhs<3,8,IN_STAGE> (fused stage state + ghost cells + RHS)",
+                     "kernel": "fv::k_rhs<3,BY,VANLEER> on a stage state (fused stage sum, ghost cells, RHS)",
                      "launches": n_rhs, "mean_launch_us": 1e3 * ms_rhs / max(n_rhs, 1),
                      "alg_bytes_per_launch": bytes_rhs / max(n_rhs, 1),
                      "timing": "HIP events around each launch on the simulation stream, "

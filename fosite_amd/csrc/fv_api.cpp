@@ -28,7 +28,8 @@ namespace {
 
 constexpr int NBUF = 4;
 // kernel classes of the timing statistics (fv_kernel_stats)
-constexpr int K_RHS = 0, K_RHS_REMAP = 1, K_COMBINE = 2, K_FRAME = 3, K_CONTROL = 4, K_CLASSES = 5;
+constexpr int K_RHS = 0, K_RHS_END = 1, K_RHS_COMBINE = 2, K_FRAME = 3, K_CONTROL = 4, K_REMAP = 5,
+              K_CLASSES = 6;
 // timedisc_base.f90:193-203
 constexpr int CHECK_NOTHING = 0, CHECK_ALL = ~0, CHECK_CSOUND = 1, CHECK_PMIN = 2;
 constexpr int DTCAUSE_ERRADJ = -1, DTCAUSE_SMALLERR = -2, DTCAUSE_FILEIO = -4;
@@ -115,9 +116,9 @@ struct fv_ctx {
   std::vector<GraphEntry> graphs;
   bool fast = true;         // device-controlled batches allowed
   int batch_steps = 0, batch_halts = 0;
-  double *part_r = nullptr, *part_c = nullptr;  // per-block partials of k_rhs / k_combine
+  double *part_r = nullptr, *part_c = nullptr;  // per-block partials: CFL/checks, embedded error
   double *d_red = nullptr, *h_red = nullptr;
-  int nb_rhs = 0, nb_comb = 0;
+  int nb_rhs = 0;
   hipStream_t st = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   Geom g{};
@@ -239,11 +240,20 @@ int rhs_main(fv_ctx *ctx, int checkbm, int remap) {
   int add, sub;
   rhs_flags(ctx, ctx->fl_pvar, ctx->fl_cvar, add, sub);
   const int pmin = (ctx->V == 4 && (checkbm & CHECK_PMIN)) ? 1 : 0;
-  const int outb = pick_free(ctx, ctx->cvar, ctx->cold, -1);
+  int in = ctx->cvar;
+  if (remap) {  // FargoAdvectionY into a free buffer
+    const int rb = pick_free(ctx, ctx->cvar, ctx->cold, -1);
+    RemapArgs r{ctx->buf[ctx->cvar], ctx->buf[rb], ctx->dt, nullptr};
+    long pk = prof_begin(ctx);
+    launch_remap(ctx->g, r, ctx->V, ctx->st);
+    prof_end(ctx, pk, K_REMAP, 2.0 * word_bytes(ctx));
+    HIPCHK(hipGetLastError());
+    in = rb;
+  }
+  const int outb = pick_free(ctx, ctx->cvar, ctx->cold, in);
   StageArgs a = base_args(ctx);
-  a.Y = ctx->buf[ctx->cvar];
+  a.Y = ctx->buf[in];
   a.nk = 0;
-  a.dt_step = ctx->dt;
   a.kout = ctx->k0[ctx->k0cur];
   a.yout = ctx->buf[outb];
   a.add_bg = add;
@@ -252,8 +262,8 @@ int rhs_main(fv_ctx *ctx, int checkbm, int remap) {
   a.want_cfl = 1;
   a.want_check = checkbm != CHECK_NOTHING;
   long pk = prof_begin(ctx);
-  launch_rhs(ctx->g, a, ctx->V, remap ? IN_REMAP : IN_STAGE, ctx->st);
-  prof_end(ctx, pk, remap ? K_RHS_REMAP : K_RHS, 3.0 * word_bytes(ctx));
+  launch_rhs(ctx->g, a, ctx->V, ctx->st);
+  prof_end(ctx, pk, K_RHS_END, 3.0 * word_bytes(ctx));
   HIPCHK(hipGetLastError());
   ctx->cvar = outb;
   int rc = reduce(ctx, ctx->part_r, ctx->nb_rhs, &ctx->invdt, &ctx->flags);
@@ -265,24 +275,46 @@ int rhs_main(fv_ctx *ctx, int checkbm, int remap) {
   return 0;
 }
 
-// ComputeRHS on the stage state (ptmp, ctmp), ctmp = cvar - sum a*dt*k
-int rhs_stage(fv_ctx *ctx, int m, double dt) {
-  int add, sub;
-  rhs_flags(ctx, ctx->fl_ptmp, ctx->fl_ctmp, add, sub);
+// arguments of stage m (1..s-1) on the state y with coeff(1) in k0[k0],
+// the last stage (m = s-1) fused with the final combination into buffer out
+StageArgs stage_args(fv_ctx *ctx, int m, int y, int k0, int out) {
   StageArgs a = base_args(ctx);
-  a.Y = ctx->buf[ctx->cvar];
+  a.Y = ctx->buf[y];
   a.nk = m;
   for (int q = 0; q < m; q++) {
-    a.k[q] = q ? ctx->k[q] : ctx->k0[ctx->k0cur];
+    a.k[q] = q ? ctx->k[q] : ctx->k0[k0];
     a.a[q] = ctx->tab.a[m][q];
   }
+  if (m == ctx->tab.m - 1) {
+    a.ynew = ctx->buf[out];
+    for (int q = 0; q < ctx->tab.m; q++) {
+      a.bh[q] = ctx->tab.bh[q];
+      a.bl[q] = ctx->tab.bl[q];
+    }
+    a.tol_rel = ctx->c.tol_rel;
+    for (int l = 0; l < 4; l++) a.tol_abs[l] = ctx->c.tol_abs[l];
+    a.part_c = ctx->part_c;
+  } else {
+    a.kout = ctx->k[m];
+  }
+  return a;
+}
+void launch_stage(fv_ctx *ctx, const StageArgs &a, int m) {
+  const bool last = m == ctx->tab.m - 1;
+  long pk = prof_begin(ctx);
+  launch_rhs(ctx->g, a, ctx->V, ctx->st);
+  prof_end(ctx, pk, last ? K_RHS_COMBINE : K_RHS, (last ? m + 1.0 : m + 2.0) * word_bytes(ctx));
+}
+
+// ComputeRHS on the stage state (ptmp, ctmp), ctmp = cvar - sum a*dt*k
+int rhs_stage(fv_ctx *ctx, int m, double dt, int out) {
+  int add, sub;
+  rhs_flags(ctx, ctx->fl_ptmp, ctx->fl_ctmp, add, sub);
+  StageArgs a = stage_args(ctx, m, ctx->cvar, ctx->k0cur, out);
   a.dt = dt;
-  a.kout = ctx->k[m];
   a.add_bg = add;
   a.sub_bg = sub;
-  long pk = prof_begin(ctx);
-  launch_rhs(ctx->g, a, ctx->V, IN_STAGE, ctx->st);
-  prof_end(ctx, pk, K_RHS, (m + 2.0) * word_bytes(ctx));
+  launch_stage(ctx, a, m);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -310,42 +342,19 @@ double adjust_timestep(fv_ctx *ctx, double maxerr, double dtold) {
   return dtnew;
 }
 
-CombineArgs combine_args(fv_ctx *ctx, int y, int k0, int out) {
-  CombineArgs a;
-  std::memset(&a, 0, sizeof(a));
-  a.Y = ctx->buf[y];
-  a.nk = ctx->tab.m;
-  for (int q = 0; q < ctx->tab.m; q++) {
-    a.k[q] = q ? ctx->k[q] : ctx->k0[k0];
-    a.bh[q] = ctx->tab.bh[q];
-    a.bl[q] = ctx->tab.bl[q];
-  }
-  a.ynew = ctx->buf[out];
-  a.tol_rel = ctx->c.tol_rel;
-  for (int l = 0; l < 4; l++) a.tol_abs[l] = ctx->c.tol_abs[l];
-  a.part = ctx->part_c;
-  return a;
-}
-
 // timedisc_rkfehlberg.f90:170-268 (boundary-flux accounting omitted: it only
 // feeds diagnostics and accretion, neither enabled)
 int solve_ode(fv_ctx *ctx, double *dt, double *err) {
-  for (int m = 1; m < ctx->tab.m; m++) {
-    int rc = rhs_stage(ctx, m, *dt);
+  const int outb = pick_free(ctx, ctx->cvar, ctx->cold, -1);
+  for (int m = 1; m < ctx->tab.m; m++) {  // the last stage writes the combination
+    int rc = rhs_stage(ctx, m, *dt, outb);
     if (rc) return rc;
   }
   ctx->fl_ctmp = ctx->fl_cvar;  // ctmp = cvar
-  const int outb = pick_free(ctx, ctx->cvar, ctx->cold, -1);
-  CombineArgs a = combine_args(ctx, ctx->cvar, ctx->k0cur, outb);
-  a.dt = *dt;
-  long pk = prof_begin(ctx);
-  launch_combine(ctx->g, a, ctx->V, ctx->st);
-  prof_end(ctx, pk, K_COMBINE, (ctx->tab.m + 2.0) * word_bytes(ctx));
-  HIPCHK(hipGetLastError());
   ctx->cvar = outb;
   double e;
   int f;
-  int rc = reduce(ctx, ctx->part_c, ctx->nb_comb, &e, &f);
+  int rc = reduce(ctx, ctx->part_c, ctx->nb_rhs, &e, &f);
   if (rc) return rc;
   if (ctx->c.tol_rel <= 1.0) {
     *err = e;
@@ -463,48 +472,41 @@ bool fast_eligible(const fv_ctx *ctx) {
 }
 
 // enqueue one device-controlled step: input state x_in/k0_in -> x_out/k0_out
-void enqueue_fast_step(fv_ctx *ctx, int xin, int xout, int tb, int k0in, int k0out) {
+// (tb: combined state, rb: remapped state)
+void enqueue_fast_step(fv_ctx *ctx, int xin, int xout, int tb, int rb, int k0in, int k0out) {
   const int steady = ctx->c.fargo == 2 ? 1 : 0;
   for (int m = 1; m < ctx->tab.m; m++) {
-    StageArgs a = base_args(ctx);
-    a.Y = ctx->buf[xin];
-    a.nk = m;
-    for (int q = 0; q < m; q++) {
-      a.k[q] = q ? ctx->k[q] : ctx->k0[k0in];
-      a.a[q] = ctx->tab.a[m][q];
-    }
+    StageArgs a = stage_args(ctx, m, xin, k0in, tb);
     a.ctl = ctx->d_ctl;
-    a.kout = ctx->k[m];
     a.add_bg = a.sub_bg = steady;
-    long pk = prof_begin(ctx);
-    launch_rhs(ctx->g, a, ctx->V, IN_STAGE, ctx->st);
-    prof_end(ctx, pk, K_RHS, (m + 2.0) * word_bytes(ctx));
+    launch_stage(ctx, a, m);
   }
-  CombineArgs c = combine_args(ctx, xin, k0in, tb);
-  c.ctl = ctx->d_ctl;
-  long pk = prof_begin(ctx);
-  launch_combine(ctx->g, c, ctx->V, ctx->st);
-  prof_end(ctx, pk, K_COMBINE, (ctx->tab.m + 2.0) * word_bytes(ctx));
+  int y = tb;
+  if (ctx->c.fargo == 2) {
+    RemapArgs r{ctx->buf[tb], ctx->buf[rb], 0.0, ctx->d_ctl};
+    long pk = prof_begin(ctx);
+    launch_remap(ctx->g, r, ctx->V, ctx->st);
+    prof_end(ctx, pk, K_REMAP, 2.0 * word_bytes(ctx));
+    y = rb;
+  }
   StageArgs a = base_args(ctx);
-  a.Y = ctx->buf[tb];
+  a.Y = ctx->buf[y];
   a.nk = 0;
   a.ctl = ctx->d_ctl;
-  a.end_of_step = 1;
   a.kout = ctx->k0[k0out];
   a.yout = ctx->buf[xout];
   a.add_bg = a.sub_bg = steady;
   a.pmin_clamp = ctx->V == 4 ? 1 : 0;
   a.want_cfl = a.want_check = 1;
-  const int remap = ctx->c.fargo == 2;
+  long pk = prof_begin(ctx);
+  launch_rhs(ctx->g, a, ctx->V, ctx->st);
+  prof_end(ctx, pk, K_RHS_END, 3.0 * word_bytes(ctx));
   pk = prof_begin(ctx);
-  launch_rhs(ctx->g, a, ctx->V, remap ? IN_REMAP : IN_STAGE, ctx->st);
-  prof_end(ctx, pk, remap ? K_RHS_REMAP : K_RHS, 3.0 * word_bytes(ctx));
-  pk = prof_begin(ctx);
-  launch_control(ctx->d_ctl, ctx->part_c, ctx->nb_comb, ctx->part_r, ctx->nb_rhs, ctx->st);
-  prof_end(ctx, pk, K_CONTROL, 16.0 * (ctx->nb_comb + ctx->nb_rhs));
+  launch_control(ctx->d_ctl, ctx->part_c, ctx->part_r, ctx->nb_rhs, ctx->st);
+  prof_end(ctx, pk, K_CONTROL, 32.0 * ctx->nb_rhs);
 }
 
-int get_graph(fv_ctx *ctx, int x0, int x1, int tb, int k0, hipGraphExec_t *out) {
+int get_graph(fv_ctx *ctx, int x0, int x1, int tb, int rb, int k0, hipGraphExec_t *out) {
   for (auto &e : ctx->graphs)
     if (e.x0 == x0 && e.x1 == x1 && e.t == tb && e.k0 == k0) {
       *out = e.exec;
@@ -512,8 +514,8 @@ int get_graph(fv_ctx *ctx, int x0, int x1, int tb, int k0, hipGraphExec_t *out) 
     }
   hipGraph_t graph;
   HIPCHK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
-  enqueue_fast_step(ctx, x0, x1, tb, k0, 1 - k0);
-  enqueue_fast_step(ctx, x1, x0, tb, 1 - k0, k0);
+  enqueue_fast_step(ctx, x0, x1, tb, rb, k0, 1 - k0);
+  enqueue_fast_step(ctx, x1, x0, tb, rb, 1 - k0, k0);
   HIPCHK(hipStreamEndCapture(ctx->st, &graph));
   hipGraphExec_t exec;
   HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
@@ -531,6 +533,7 @@ int run_batch(fv_ctx *ctx, int nmax, int *done_steps, int *halted) {
   const int x0 = ctx->cvar;
   const int x1 = pick_free(ctx, x0, -1, -1);
   const int tb = pick_free(ctx, x0, x1, -1);
+  const int rb = pick_free(ctx, x0, x1, tb);
   const int k0 = ctx->k0cur;
   Ctl *h = ctx->h_ctl;
   std::memset(h, 0, offsetof(Ctl, err_hist));
@@ -546,7 +549,7 @@ int run_batch(fv_ctx *ctx, int nmax, int *done_steps, int *halted) {
   h->nsteps = nmax;
   hipGraphExec_t exec = nullptr;
   if (!ctx->prof) {
-    int rc = get_graph(ctx, x0, x1, tb, k0, &exec);
+    int rc = get_graph(ctx, x0, x1, tb, rb, k0, &exec);
     if (rc) return rc;
   }
   HIPCHK(hipMemcpyAsync(ctx->d_ctl, h, offsetof(Ctl, err_hist), hipMemcpyHostToDevice, ctx->st));
@@ -554,7 +557,7 @@ int run_batch(fv_ctx *ctx, int nmax, int *done_steps, int *halted) {
     for (int s = 0; s < nmax; s += 2) HIPCHK(hipGraphLaunch(exec, ctx->st));
   } else {  // timed: the same launches, issued one by one between events
     for (int s = 0; s < nmax; s++)
-      enqueue_fast_step(ctx, s % 2 ? x1 : x0, s % 2 ? x0 : x1, tb, s % 2 ? 1 - k0 : k0, s % 2 ? k0 : 1 - k0);
+      enqueue_fast_step(ctx, s % 2 ? x1 : x0, s % 2 ? x0 : x1, tb, rb, s % 2 ? 1 - k0 : k0, s % 2 ? k0 : 1 - k0);
     HIPCHK(hipGetLastError());
   }
   HIPCHK(hipMemcpyAsync(h, ctx->d_ctl, sizeof(Ctl), hipMemcpyDeviceToHost, ctx->st));
@@ -649,7 +652,6 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
       if (ctx->mesh.a[M_RDYS][(size_t)j * ctx->NI + i] != ctx->col_h[(size_t)C_RDYS * ctx->NJ + j])
         return bail("reconstruction offsets in y depend on i");
   }
-  if (const char *e = std::getenv("FV_RHS_ROWS")) set_rhs_rows(std::atoi(e));
   if (const char *e = std::getenv("FV_HOST_CONTROL")) ctx->fast = std::atoi(e) == 0;
   // device resources
   const size_t sbytes = sizeof(double) * ctx->V * ctx->FS;
@@ -669,9 +671,8 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
   for (int d = 0; d < 4; d++) g.bc[d] = c.bc[d];
   g.fargo = c.fargo == 2; g.gravity = c.gravity != 0; g.curvilinear = c.geometry != CARTESIAN;
   ctx->nb_rhs = rhs_blocks(g);
-  ctx->nb_comb = combine_blocks(g);
   ok = ok && dalloc(&ctx->part_r, sizeof(double) * 2 * ctx->nb_rhs) &&
-       dalloc(&ctx->part_c, sizeof(double) * 2 * ctx->nb_comb) && dalloc(&ctx->d_red, sizeof(double) * 2);
+       dalloc(&ctx->part_c, sizeof(double) * 2 * ctx->nb_rhs) && dalloc(&ctx->d_red, sizeof(double) * 2);
   ok = ok && hipMalloc((void **)&ctx->d_ctl, sizeof(Ctl)) == hipSuccess && hipMemset(ctx->d_ctl, 0, sizeof(Ctl)) == hipSuccess;
   ok = ok && hipHostMalloc((void **)&ctx->h_red, sizeof(double) * 2) == hipSuccess;
   ok = ok && hipHostMalloc((void **)&ctx->h_ctl, sizeof(Ctl)) == hipSuccess;

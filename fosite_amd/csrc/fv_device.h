@@ -7,7 +7,7 @@ namespace fv {
 
 constexpr int GH = 2;       // ghost layers per side (mesh/mesh_base.f90:462)
 constexpr int MAXK = 7;     // RK stages (Dormand-Prince has 7)
-constexpr int TILE_X = 64;  // threads per tile row (one wavefront)
+constexpr int TILE_X = 64;  // tile width in cells (one wavefront per tile row)
 constexpr int CTL_HIST = 512;  // steps per device-controlled batch
 
 enum Physics { EULER_ISOTHERM = 16, EULER = 17 };
@@ -53,52 +53,50 @@ struct Ctl {
   double err_hist[CTL_HIST], dt_hist[CTL_HIST];
 };
 
-enum InputMode { IN_STAGE = 0, IN_REMAP = 1 };
-
-// One right-hand-side evaluation (timedisc_base.f90:1073-1271) on the state
-//   IN_STAGE: S = Y - sum_q (a[q]*dt) * k[q]   (timedisc_rkfehlberg.f90:277-302)
-//   IN_REMAP: S = FargoAdvectionY(Y)           (timedisc_base.f90:1935-2069)
+// One right-hand-side evaluation (timedisc_base.f90:1073-1271) on the RK
+// stage state S = Y - sum_q (a[q]*dt) * k[q] (timedisc_rkfehlberg.f90:277-302).
+// With ynew set, the launch is the last stage of the step and also performs
+// the final combination and the embedded error estimate
+// (timedisc_rkfehlberg.f90:199-266, timedisc_base.f90:1012-1053) in place of
+// storing the stage's rhs.
 struct StageArgs {
   const double *Y;
   const double *k[MAXK];
   double a[MAXK];       // Butcher coefficients of this stage
   int nk;
-  double dt;            // time step (host-controlled) ...
-  double dt_step;       // ... and this%dt of the FARGO remap
+  double dt;            // time step (host-controlled)
   Ctl *ctl;             // device-controlled: dt and halting come from here
-  int end_of_step;      // device-controlled end-of-step RHS: dt_step = (time+dt)-time
-  double *kout;         // rhs (interior cells)
+  double *kout;         // rhs (interior cells); unused with ynew
   double *yout;         // transformed state incl. ghost cells, or null
   int add_bg, sub_bg;   // FARGO frame change at entry / before the fluxes
   int pmin_clamp;       // Euler CHECK_PMIN round trip (timedisc_base.f90:1150-1164)
   int want_cfl, want_check;
   double *part;         // per-block partials: [2*b] max 1/dt, [2*b+1] check flags
+  // fused final combination (last stage only)
+  double *ynew;
+  double bh[MAXK], bl[MAXK];
+  double tol_rel, tol_abs[4];
+  double *part_c;       // per-block partials: [2*b] max error, [2*b+1] = 0
 };
 
-// pointwise final combination + embedded error (timedisc_rkfehlberg.f90:199-266,
-// timedisc_base.f90:1012-1053)
-struct CombineArgs {
+// FargoAdvectionY on the combined state (timedisc_base.f90:1935-2069):
+// comoving-frame CenterBoundary for the y-ghost rows, then per ring the
+// integer shift and the minmod-limited fractional transport.
+struct RemapArgs {
   const double *Y;
-  const double *k[MAXK];
-  double bh[MAXK], bl[MAXK];
-  int nk;
-  double dt;
-  Ctl *ctl;
-  double *ynew;
-  double tol_rel, tol_abs[4];
-  double *part;         // per-block partials: [2*b] max error, [2*b+1] = 0
+  double *out;          // remapped interior cells
+  double dt_step;       // this%dt (host-controlled)
+  Ctl *ctl;             // device-controlled: dt_step = (time+dt)-time, halting
 };
 
 enum Flag { F_INVALID = 1, F_RHOMIN = 2, F_PMIN = 4 };
 
 // kernel launchers (fv_kernels.hip)
 int rhs_blocks(const Geom &g);
-int combine_blocks(const Geom &g);
-void set_rhs_rows(int by);
-void launch_rhs(const Geom &g, const StageArgs &a, int V, int mode, hipStream_t s);
-void launch_combine(const Geom &g, const CombineArgs &a, int V, hipStream_t s);
+void launch_rhs(const Geom &g, const StageArgs &a, int V, hipStream_t s);
+void launch_remap(const Geom &g, const RemapArgs &a, int V, hipStream_t s);
 void launch_frame(const Geom &g, const double *src, double *dst, int sign, int V, hipStream_t s);
 void launch_reduce(const double *part, int n, double *out, hipStream_t s);
-void launch_control(Ctl *ctl, const double *part_c, int nc, const double *part_r, int nr, hipStream_t s);
+void launch_control(Ctl *ctl, const double *part_c, const double *part_r, int nb, hipStream_t s);
 
 }  // namespace fv

@@ -6,10 +6,8 @@
 // the reference's bits.  Citations are file:line in the fosite source tree.
 //
 // k_rhs (one launch per right-hand-side evaluation) is fully fused:
-//   * the input state of every loaded cell is formed on the fly -- either the
-//     RK stage state Y - sum a*dt*k (timedisc_rkfehlberg.f90:277-302) or, for
-//     the end-of-step evaluation under FARGO, the remapped state
-//     (timedisc_base.f90:1935-2069);
+//   * the input state of every loaded cell is formed on the fly: the RK stage
+//     state Y - sum a*dt*k (timedisc_rkfehlberg.f90:277-302);
 //   * ghost cells are evaluated on the fly from interior cells with the
 //     boundary rules (boundary_generic.f90:287-517, boundary_nogradients.f90,
 //     boundary_periodic.f90, boundary_absorbing.f90), so no boundary pass and
@@ -17,11 +15,13 @@
 //   * FARGO frame round trip, Euler PMIN round trip, linear reconstruction,
 //     face conversions, wave speeds, physical and KT fluxes, geometric,
 //     gravity and FARGO sources, rhs assembly, CFL and data checks follow.
+//   * in the last stage of a step, the final RK combination and the embedded
+//     error norm (pointwise) replace the store of the stage's rhs.
 //   Tiles of 64 x BY threads (one cell each) with the halo staged in LDS.
-// k_combine: final RK combination + embedded error norm (pointwise).
+// k_remap: FARGO azimuthal shift of the combined state (y direction only).
 // k_control: one block; in device-controlled batches it reduces the partials
-// of k_combine and of the end-of-step k_rhs, runs the accept test and the CFL
-// update.  Kernel boundaries order everything: no grid-wide fences.
+// of the last stage and of the end-of-step k_rhs, runs the accept test and
+// the CFL update.  Kernel boundaries order everything: no grid-wide fences.
 #include <float.h>
 #include <math.h>
 
@@ -36,7 +36,8 @@ __device__ __forceinline__ double fmax2(double a, double b) { return (b > a) ? b
 __device__ __forceinline__ double fsign(double a, double b) { return copysign(fabs(a), b); }
 __device__ __forceinline__ bool bad(double x) { return (x != x) || (x > DBL_MAX); }
 
-__device__ __forceinline__ double ring(const Geom &g, int id, int i) { return g.ring[id * g.NI + i]; }
+// per-ring arrays: uniform base per id, 32-bit unsigned lane offset
+__device__ __forceinline__ double ring(const Geom &g, int id, int i) { return (g.ring + id * g.NI)[(unsigned)i]; }
 __device__ __forceinline__ long cidx(const Geom &g, int i, int j) { return (long)j * g.P + i; }
 
 // ---------------------------------------------------------------- physics
@@ -63,91 +64,24 @@ __device__ __forceinline__ double sound_speed(double gamma, double rho, double p
 }
 
 // ------------------------------------------------------------ input state
-template <int V, int MODE>
+template <int V>
 struct In {
   const Geom &g;
   const StageArgs &a;
+  double dt;
   double c[MAXK];  // a(m,q)*dt
-  double dts;      // this%dt of the FARGO remap
   int nk;
 
   __forceinline__ __device__ In(const Geom &g_, const StageArgs &a_) : g(g_), a(a_) {
-    double dt = a.dt;
-    dts = a.dt_step;
-    if (a.ctl) {  // device-controlled: this step's dt; dt_step as AcceptSolution sets it
-      const double t = a.ctl->time;
-      dt = a.ctl->dt;
-      dts = a.end_of_step ? (t + dt) - t : 0.0;
-    }
+    dt = a.ctl ? a.ctl->dt : a.dt;  // device-controlled: this step's dt
     nk = a.nk;
 #pragma unroll
     for (int q = 0; q < MAXK; q++) c[q] = a.a[q] * dt;
   }
 
-  // stored state at an interior column and any row, y-ghost rows taken from
-  // the comoving-frame CenterBoundary (prim round trip, no frame change)
-  __forceinline__ __device__ void ycell(int i, int jj, double *u) const {
-    const int jmin = GH, jmax = GH + g.jnum - 1;
-    int js = jj;
-    bool ghost = false;
-    if (jj < jmin) {
-      ghost = true;
-      js = g.bc[2] == PERIODIC ? jj + g.jnum : jmin;
-    } else if (jj > jmax) {
-      ghost = true;
-      js = g.bc[3] == PERIODIC ? jj - g.jnum : jmax;
-    }
-    const long ix = cidx(g, i, js);
-#pragma unroll
-    for (int l = 0; l < V; l++) u[l] = a.Y[l * g.FS + ix];
-    if (ghost) {
-      double p[V];
-      c2p<V>(u, p, g.gamma);
-      p2c<V>(p, u, g.gamma);
-    }
-  }
-
-  // FargoAdvectionY for one interior cell: minmod-limited linear transport by
-  // the fractional shift, then CSHIFT(., -shift) over JMIN:JMAX
-  __forceinline__ __device__ void remap(int i, int j, double *s) const {
-    const int jmin = GH, J = g.jnum;
-    double d = ring(g, R_W, i) * dts / ring(g, R_DLY, i);
-    const double shift = round(d);
-    d = d - shift;
-    long jj = (long)(j - jmin) - (long)shift;
-    jj %= J;
-    if (jj < 0) jj += J;
-    const int js = jmin + (int)jj;
-    double v[5][V];
-#pragma unroll
-    for (int q = 0; q < 5; q++) ycell(i, js - 2 + q, v[q]);
-#pragma unroll
-    for (int l = 0; l < V; l++) {
-      double dq[4], dql[3];
-#pragma unroll
-      for (int q = 0; q < 4; q++) dq[q] = v[q + 1][l] - v[q][l];
-#pragma unroll
-      for (int q = 0; q < 3; q++)
-        dql[q] = (fsign(1.0, dq[q]) * fsign(1.0, dq[q + 1]) > 0.)
-                     ? fsign(fmin2(fabs(dq[q]), fabs(dq[q + 1])), dq[q])
-                     : 0.;
-      double f0, f1;  // flux(js-1), flux(js)
-      if (d > 0.) {
-        f0 = v[1][l] + .5 * dql[0] * (1. - d);
-        f1 = v[2][l] + .5 * dql[1] * (1. - d);
-      } else {
-        f0 = v[2][l] - .5 * dql[1] * (1. + d);
-        f1 = v[3][l] - .5 * dql[2] * (1. + d);
-      }
-      s[l] = v[2][l] - d * (f1 - f0);
-    }
-  }
-
   // input state at an interior cell
   __forceinline__ __device__ void interior(int i, int j, double *s) const {
-    if (MODE == IN_REMAP) {
-      remap(i, j, s);
-    } else {
+    {
       const long ix = cidx(g, i, j);
       // all loads first (uniform branches only), then the stage sum in the
       // reference's order
@@ -167,6 +101,44 @@ struct In {
           for (int l = 0; l < V; l++) s[l] = s[l] - c[q] * kv[q][l];
         }
     }
+  }
+
+  // stage states of two interior cells (p0, p1 say which are wanted): all
+  // loads of both cells are issued before any arithmetic
+  __forceinline__ __device__ void interior2(bool p0, int ia, int ja, double *s0, bool p1, int ib, int jb,
+                                            double *s1) const {
+    const unsigned x0 = (unsigned)ja * (unsigned)g.P + (unsigned)ia;
+    const unsigned x1 = (unsigned)jb * (unsigned)g.P + (unsigned)ib;
+    double k0[MAXK][V], k1[MAXK][V];
+    if (p0) {
+#pragma unroll
+      for (int l = 0; l < V; l++) s0[l] = (a.Y + l * g.FS)[x0];
+#pragma unroll
+      for (int q = 0; q < MAXK; q++)
+        if (q < nk) {
+#pragma unroll
+          for (int l = 0; l < V; l++) k0[q][l] = (a.k[q] + l * g.FS)[x0];
+        }
+    }
+    if (p1) {
+#pragma unroll
+      for (int l = 0; l < V; l++) s1[l] = (a.Y + l * g.FS)[x1];
+#pragma unroll
+      for (int q = 0; q < MAXK; q++)
+        if (q < nk) {
+#pragma unroll
+          for (int l = 0; l < V; l++) k1[q][l] = (a.k[q] + l * g.FS)[x1];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < MAXK; q++)
+      if (q < nk) {
+#pragma unroll
+        for (int l = 0; l < V; l++) {
+          s0[l] = s0[l] - c[q] * k0[q][l];
+          s1[l] = s1[l] - c[q] * k1[q][l];
+        }
+      }
   }
 
   // ComputeRHS transform (timedisc_base.f90:1098-1208): ur = cvar after the
@@ -335,9 +307,11 @@ __device__ int blk_or(int v, int *red) {
 }
 // ----------------------------------------------------------- reconstruction
 // limiters, reconstruction_linear.f90:645-733
+// LIM = 0: limiter chosen at run time from g.limiter
+template <int LIM>
 __device__ __forceinline__ double slope(const Geom &g, double um, double u0, double up) {
   const double a1 = u0 - um, a2 = up - u0;
-  switch (g.limiter) {
+  switch (LIM ? LIM : g.limiter) {
     case MINMOD:
       return (fsign(1.0, a1) * fsign(1.0, a2) > 0) ? fsign(fmin2(fabs(a1), fabs(a2)), a1) : 0.;
     case MONOCENT: {
@@ -348,7 +322,7 @@ __device__ __forceinline__ double slope(const Geom &g, double um, double u0, dou
     }
     case SWEBY:
     case SUPERBEE: {
-      const double q = g.limiter == SUPERBEE ? 2.0 : g.theta;
+      const double q = (LIM ? LIM : g.limiter) == SUPERBEE ? 2.0 : g.theta;
       return (fsign(1.0, a1) * fsign(1.0, a2) > 0)
                  ? fsign(fmax2(fmin2(q * fabs(a1), fabs(a2)), fmin2(fabs(a1), q * fabs(a2))), a1)
                  : 0.;
@@ -446,77 +420,101 @@ __device__ void control_next(Ctl *c, double invdt, int flags) {
 }
 
 // -------------------------------------------------------------- fused RHS
-template <int V, int BY, int MODE>
-__global__ __launch_bounds__(TILE_X *BY) void k_rhs(Geom g, StageArgs a) {
-  constexpr int BX = TILE_X, SX = BX + 2, SY = BY + 2;
-  constexpr int NUC = V * SY * SX, NF = 2 * V * BY * BX, NL = NUC > NF ? NUC : NF;
-  constexpr int NT = BX * BY;
+// Tile: 64 x TY_RHS output cells, one thread each.  The footprint (2-cell
+// halo, corners excluded) is staged in LDS as the state the fluxes see;
+// threads with an extra job cover the halo cells and the faces and fluxes one
+// cell beyond the tile.  Phases, separated by barriers:
+//   1 stage state + transform of the own and one extra footprint cell (ghost
+//     cells on boundary tiles), sources, data checks and CFL of the own cell
+//   2 limited slopes and face states: own cell; rows -1 and TY; columns -1, 64
+//   3 KT interface fluxes: own east and north interfaces; west of column 0,
+//     south of row 0
+//   4 rhs assembly, or the final RK combination in the last stage; stores
+// Output-aligned tiles keep every lane of the flux phases busy and divide
+// power-of-two meshes exactly.
+constexpr int TY_RHS = 8;
+
+template <int V, int LIM>
+__global__ __launch_bounds__(TILE_X *TY_RHS, 4) void k_rhs(Geom g, StageArgs a) {
+  constexpr int TX = TILE_X, TY = TY_RHS, SX = TX + 4, SY = TY + 4;
+  constexpr int N_UC = V * SY * SX;
+  constexpr int XW = V * TY * (TX + 1), XE = V * TY, YS = V * (TY + 1) * TX, YN = V * TX;
+  constexpr int N_FACE = XW + XE + YS + YN;
+  constexpr int FX = V * TY * (TX + 1), FY = V * (TY + 1) * TX;
+  constexpr int N_FLUX = FX + FY;
+  constexpr int NL = N_UC > N_FACE ? (N_UC > N_FLUX ? N_UC : N_FLUX) : (N_FACE > N_FLUX ? N_FACE : N_FLUX);
   __shared__ double lds[NL];
-  __shared__ double redd[BY];
-  __shared__ int redi[BY];
-  double *uc_s = lds;                // [V][SY][SX] flux-input state
-  double *fa_s = lds;                // [V][BY][BX] W faces, then x fluxes
-  double *fb_s = lds + V * BY * BX;  // [V][BY][BX] S faces, then y fluxes
+  __shared__ double redd[TY];
+  __shared__ int redi[TY];
+  // footprint state, tile-relative row r in [-2, TY+1], column c in [-2, TX+1]
+  auto UC = [&](int l, int r, int c) -> double & { return lds[(l * SY + r + 2) * SX + c + 2]; };
+  // faces: W of columns 0..TX, E of column -1, S of rows 0..TY, N of row -1
+  auto FW = [&](int l, int r, int c) -> double & { return lds[(l * TY + r) * (TX + 1) + c]; };
+  auto FE1 = [&](int l, int r) -> double & { return lds[XW + l * TY + r]; };
+  auto FS = [&](int l, int r, int c) -> double & { return lds[XW + XE + (l * (TY + 1) + r) * TX + c]; };
+  auto FN1 = [&](int l, int c) -> double & { return lds[XW + XE + YS + l * TX + c]; };
+  // fluxes: x interface c-1/2 (c = 0..TX) of row r; y interface r-1/2 (r = 0..TY) of column c
+  auto FLX = [&](int l, int r, int c) -> double & { return lds[(l * TY + r) * (TX + 1) + c]; };
+  auto FLY = [&](int l, int r, int c) -> double & { return lds[FX + (l * (TY + 1) + r) * TX + c]; };
 
   if (a.ctl && a.ctl->halt) return;
-  const In<V, MODE> in(g, a);
-
-  const int tx = threadIdx.x, ty = threadIdx.y, t = tx + BX * ty;
-  const int i0 = GH + blockIdx.x * (BX - 2), j0 = GH + blockIdx.y * (BY - 2);
-  const int fi = i0 - 1 + tx, fj = j0 - 1 + ty;
+  const In<V> in(g, a);
+  const int tx = threadIdx.x, ty = threadIdx.y, t = tx + TX * ty;
+  const int i0 = GH + blockIdx.x * TX, j0 = GH + blockIdx.y * TY;
   const int imin = GH, jmin = GH, imax = GH + g.inum - 1, jmax = GH + g.jnum - 1;
-  const bool out = tx >= 1 && tx <= BX - 2 && ty >= 1 && ty <= BY - 2 && fi <= imax && fj <= jmax;
+  const int i = i0 + tx, j = j0 + ty;
+  const bool out = i <= imax && j <= jmax;  // own cell is an interior cell
 
-  // ---- phase 1: flux-input state of the tile and its halo into LDS
-  double ur[V], pr[V], uc[V];
-  auto stage_in = [&](int i, int j, double *u, bool own) {
-    const bool xin = i >= imin && i <= imax, yin = j >= jmin && j <= jmax;
-    if (xin && yin) {
-      double s[V], r[V], p[V];
-      in.interior(i, j, s);
-      in.transform(i, s, r, p, u);
-      if (own) {
+  // ---- phase 1: footprint state into LDS
+  // extra footprint cell: rows -2,-1,TY,TY+1 of the own column (t < 4*TX),
+  // then columns -2,-1,TX,TX+1 of rows 0..TY-1 (next 4*TY threads)
+  int er = 0, ec = 0;
+  bool has_e = false;
+  if (t < 4 * TX) {
+    er = ty < 2 ? ty - 2 : TY + ty - 2;
+    ec = tx;
+    has_e = true;
+  } else if (t < 4 * TX + 4 * TY) {
+    const int k = t - 4 * TX, q = k / TY;
+    ec = q < 2 ? q - 2 : TX + q - 2;
+    er = k % TY;
+    has_e = true;
+  }
+  const int ei = i0 + ec, ej = j0 + er;
+  const bool e_in = has_e && ei >= imin && ei <= imax && ej >= jmin && ej <= jmax;
+  double ur[V], pr[V];
+  {
+    double so[V], se[V], uo[V], ue[V], re[V], pe[V];
+    in.interior2(out, i, j, so, e_in, ei, ej, se);
+    if (out) {
+      in.transform(i, so, ur, pr, uo);
 #pragma unroll
-        for (int l = 0; l < V; l++) ur[l] = r[l], pr[l] = p[l];
-      }
-    } else if ((xin || yin) && i >= 0 && j >= 0 && i < g.NI && j < g.NJ) {
-      double p[V];
-      int f = 0;
-      in.ghost_final(i, j, u, p, f);
-    } else {
-#pragma unroll
-      for (int l = 0; l < V; l++) u[l] = 0.0;  // corner or outside: never read
+      for (int l = 0; l < V; l++) UC(l, ty, tx) = uo[l];
     }
-  };
-  // pass 0: the thread's own cell; pass 1: one halo cell (x columns i0-2 and
-  // i0+63 on the output rows, y rows j0-2 and j0+BY-1 on the output columns).
-  // One loop body keeps a single inlined copy of the state evaluation.
-  constexpr int NXH = 2 * (BY - 2), NH = NXH + 2 * (BX - 2);
+    if (e_in) {
+      in.transform(ei, se, re, pe, ue);
+#pragma unroll
+      for (int l = 0; l < V; l++) UC(l, er, ec) = ue[l];
+    }
+  }
+  const bool own_g = !out, e_g = has_e && !e_in;
+  if (own_g || e_g) {  // ghost cells or cells beyond the mesh: boundary tiles only
 #pragma unroll 1
-  for (int pass = 0; pass < 2; pass++) {
-    int i = fi, j = fj, sx = tx + 1, sy = ty + 1;
-    if (pass) {
-      if (t >= NH) break;
-      if (t < NXH) {
-        const int side = t / (BY - 2), r = t % (BY - 2);
-        i = side ? i0 + BX - 1 : i0 - 2;
-        sx = side ? SX - 1 : 0;
-        j = j0 + r;
-        sy = r + 2;
+    for (int p = 0; p < 2; p++) {
+      if (!(p ? e_g : own_g)) continue;
+      const int ii = p ? ei : i, jj = p ? ej : j;
+      const bool xin = ii >= imin && ii <= imax, yin = jj >= jmin && jj <= jmax;
+      double u[V];
+      if ((xin || yin) && ii >= 0 && jj >= 0 && ii < g.NI && jj < g.NJ) {
+        double pp[V];
+        int f = 0;
+        in.ghost_final(ii, jj, u, pp, f);
       } else {
-        const int k = t - NXH, side = k / (BX - 2), c = k % (BX - 2);
-        i = i0 + c;
-        sx = c + 2;
-        j = side ? j0 + BY - 1 : j0 - 2;
-        sy = side ? SY - 1 : 0;
-      }
-    }
-    double u[V];
-    stage_in(i, j, u, pass == 0);
 #pragma unroll
-    for (int l = 0; l < V; l++) {
-      uc_s[(l * SY + sy) * SX + sx] = u[l];
-      if (pass == 0) uc[l] = u[l];
+        for (int l = 0; l < V; l++) u[l] = 0.0;  // corner or beyond the mesh: never needed
+      }
+#pragma unroll
+      for (int l = 0; l < V; l++) UC(l, p ? er : ty, p ? ec : tx) = u[l];
     }
   }
 
@@ -530,14 +528,14 @@ __global__ __launch_bounds__(TILE_X *BY) void k_rhs(Geom g, StageArgs a) {
     // geometrical sources (physics_eulerisotherm.f90:689-832, elementals 3069-3088;
     // physics_euler.f90:1706-1846); zero on cartesian meshes
     if (g.curvilinear) {
-      const double cxyx = ring(g, R_CXYX, fi), cyxy = ring(g, R_CYXY, fi);
-      const double czxz = ring(g, R_CZXZ, fi), czyz = ring(g, R_CZYZ, fi);
+      const double cxyx = ring(g, R_CXYX, i), cyxy = ring(g, R_CYXY, i);
+      const double czxz = ring(g, R_CZXZ, i), czyz = ring(g, R_CZYZ, i);
       const double vx = pr[1], vy = pr[2], vz = 0.0, mx = ur[1], my = ur[2], mz = 0.0;
       double P;
       if (V == 4) {
         P = pr[3];
       } else {
-        const double cs = ring(g, R_CS, fi);
+        const double cs = ring(g, R_CS, i);
         P = pr[0] * (cs * cs);
       }
       geo[1] = my * (cyxy * vy - cxyx * vx) + mz * (czxz * vz - 0.0 * vx) + (cyxy + czxz) * P;
@@ -545,12 +543,12 @@ __global__ __launch_bounds__(TILE_X *BY) void k_rhs(Geom g, StageArgs a) {
     }
     // FARGO source (physics_eulerisotherm.f90:1531-1586)
     if (g.fargo)
-      geo[2] = geo[2] - ur[1] * 0.5 * (ring(g, R_W, fi + 1) - ring(g, R_W, fi - 1)) / ring(g, R_DLX, fi);
+      geo[2] = geo[2] - ur[1] * 0.5 * (ring(g, R_W, i + 1) - ring(g, R_W, i - 1)) / ring(g, R_DLX, i);
     // point-mass gravity through the source list (sources_generic.f90:220-261,
     // sources_gravity.f90:164-199, physics_eulerisotherm.f90:1201-1226,
     // physics_euler.f90:1849-1885)
     if (g.gravity) {
-      const double ax = ring(g, R_ACCX, fi), ay = ring(g, R_ACCY, fi);
+      const double ax = ring(g, R_ACCX, i), ay = ring(g, R_ACCY, i);
       ext[1] = 0.0 + ur[0] * ax;
       ext[2] = 0.0 + ur[0] * ay;
       if (V == 4) ext[3] = 0.0 + (ur[1] * ax + ur[2] * ay);
@@ -564,119 +562,207 @@ __global__ __launch_bounds__(TILE_X *BY) void k_rhs(Geom g, StageArgs a) {
         if (bad(pr[l]) || bad(ur[l])) flags |= F_INVALID;
     }
     if (a.want_cfl) {
-      const double vy = a.sub_bg ? pr[2] - ring(g, R_W, fi) : pr[2];
-      const double cs = (V == 4) ? sound_speed(g.gamma, pr[0], pr[3]) : ring(g, R_CS, fi);
-      invdt = cell_invdt(g, fi, pr[1], vy, cs);
+      const double vy = a.sub_bg ? pr[2] - ring(g, R_W, i) : pr[2];
+      const double cs = (V == 4) ? sound_speed(g.gamma, pr[0], pr[3]) : ring(g, R_CS, i);
+      invdt = cell_invdt(g, i, pr[1], vy, cs);
     }
   }
   __syncthreads();
 
-  // ---- phase 2: limited slopes and face states
-  const int sx = tx + 1, sy = ty + 1;
-  const bool xface = ty >= 1 && ty <= BY - 2;  // rows that need x fluxes
-  const bool yface = tx >= 1 && tx <= BX - 2;  // columns that need y fluxes
-  const int fic = fi < g.NI - 1 ? fi : g.NI - 2;
-  double fw[V], fe[V], fs[V], fn[V];
-  if (xface) {
-    const double dw = ring(g, R_RDXW, fic), de = ring(g, R_RDXE, fic);
+  // ---- phase 2: limited slopes and face states (reconstruction_linear.f90)
+  const int ir = i < g.NI - 1 ? i : g.NI - 2;  // ring index, kept inside the mesh arrays
+  const int jr = j < g.NJ ? j : g.NJ - 1;
+  double uc[V], fw[V], fe[V], fs[V], fn[V];
+  {
+    const double dw = ring(g, R_RDXW, ir), de = ring(g, R_RDXE, ir);
+    const double ds = g.col[C_RDYS * g.NJ + jr], dn = g.col[C_RDYN * g.NJ + jr];
 #pragma unroll
     for (int l = 0; l < V; l++) {
-      const double *r = uc_s + (l * SY + sy) * SX;
-      const double sl = g.invdx * slope(g, r[sx - 1], r[sx], r[sx + 1]);
-      fw[l] = uc[l] + sl * dw;
-      fe[l] = uc[l] + sl * de;
+      uc[l] = UC(l, ty, tx);
+      const double sx = g.invdx * slope<LIM>(g, UC(l, ty, tx - 1), uc[l], UC(l, ty, tx + 1));
+      fw[l] = uc[l] + sx * dw;
+      fe[l] = uc[l] + sx * de;
+      const double sy = g.invdy * slope<LIM>(g, UC(l, ty - 1, tx), uc[l], UC(l, ty + 1, tx));
+      fs[l] = uc[l] + sy * ds;
+      fn[l] = uc[l] + sy * dn;
     }
   }
-  if (yface) {
-    const int jc = fj < g.NJ ? fj : g.NJ - 1;
-    const double ds = g.col[C_RDYS * g.NJ + jc], dn = g.col[C_RDYN * g.NJ + jc];
+  // extra faces: N of row -1 and S of row TY (t < 2*TX); E of column -1 and
+  // W of column TX (next 2*TY threads)
+  double xface[V];
+  int xk = -1, xr = 0, xc = 0;
+  if (t < 2 * TX) {
+    xk = t / TX;  // 0: row -1 (N face), 1: row TY (S face)
+    xr = xk ? TY : -1;
+    xc = tx;
+    const int jj = j0 + xr, jc = jj < g.NJ ? jj : g.NJ - 1;
+    const double off = xk ? g.col[C_RDYS * g.NJ + jc] : g.col[C_RDYN * g.NJ + jc];
 #pragma unroll
     for (int l = 0; l < V; l++) {
-      const double *c = uc_s + l * SY * SX + sx;
-      const double sl = g.invdy * slope(g, c[(sy - 1) * SX], c[sy * SX], c[(sy + 1) * SX]);
-      fs[l] = uc[l] + sl * ds;
-      fn[l] = uc[l] + sl * dn;
+      const double c0 = UC(l, xr, xc);
+      xface[l] = c0 + g.invdy * slope<LIM>(g, UC(l, xr - 1, xc), c0, UC(l, xr + 1, xc)) * off;
+    }
+  } else if (t < 2 * TX + 2 * TY) {
+    const int k = t - 2 * TX;
+    xk = 2 + k / TY;  // 2: column -1 (E face), 3: column TX (W face)
+    xr = k % TY;
+    xc = xk == 3 ? TX : -1;
+    const int ii = i0 + xc, ic = ii < g.NI ? ii : g.NI - 1;
+    const double off = xk == 3 ? ring(g, R_RDXW, ic) : ring(g, R_RDXE, ic);
+#pragma unroll
+    for (int l = 0; l < V; l++) {
+      const double c0 = UC(l, xr, xc);
+      xface[l] = c0 + g.invdx * slope<LIM>(g, UC(l, xr, xc - 1), c0, UC(l, xr, xc + 1)) * off;
     }
   }
   __syncthreads();
 #pragma unroll
   for (int l = 0; l < V; l++) {
-    if (xface) fa_s[(l * BY + ty) * BX + tx] = fw[l];
-    if (yface) fb_s[(l * BY + ty) * BX + tx] = fs[l];
+    FW(l, ty, tx) = fw[l];
+    FS(l, ty, tx) = fs[l];
+    if (xk == 0) FN1(l, xc) = xface[l];
+    if (xk == 1) FS(l, TY, xc) = xface[l];
+    if (xk == 2) FE1(l, xr) = xface[l];
+    if (xk == 3) FW(l, xr, TX) = xface[l];
   }
   __syncthreads();
 
-  // ---- phase 3: interface fluxes (east interface of x cells, north of y cells)
-  double xf[V], yf[V];
-  if (xface && tx <= BX - 2) {
-    double r2[V];
+  // ---- phase 3: KT interface fluxes (fluxes_kt.f90:126-176)
+  double xf[V], yf[V], ef[V];
+  {
+    double r2[V], n2[V];
 #pragma unroll
-    for (int l = 0; l < V; l++) r2[l] = fa_s[(l * BY + ty) * BX + tx + 1];
-    kt_flux<V>(g, 0, fe, r2, ring(g, R_CSE, fic), ring(g, R_CSW, fic + 1), ring(g, R_DAXW, fic + 1), xf);
+    for (int l = 0; l < V; l++) {
+      r2[l] = FW(l, ty, tx + 1);
+      n2[l] = FS(l, ty + 1, tx);
+    }
+    kt_flux<V>(g, 0, fe, r2, ring(g, R_CSE, ir), ring(g, R_CSW, ir + 1), ring(g, R_DAXW, ir + 1), xf);
+    kt_flux<V>(g, 1, fn, n2, ring(g, R_CSN, ir), ring(g, R_CSS, ir), ring(g, R_DAYS, ir), yf);
   }
-  if (yface && ty <= BY - 2) {
-    double n2[V];
+  // extra fluxes: south interface of row 0 (t < TX), west interface of
+  // column 0 (threads 2*TX .. 2*TX+TY-1)
+  int fk = -1, fr = 0;
+  if (t < TX) {
+    fk = 0;
+    double s1[V], n1[V];
 #pragma unroll
-    for (int l = 0; l < V; l++) n2[l] = fb_s[(l * BY + ty + 1) * BX + tx];
-    kt_flux<V>(g, 1, fn, n2, ring(g, R_CSN, fic), ring(g, R_CSS, fic), ring(g, R_DAYS, fic), yf);
+    for (int l = 0; l < V; l++) {
+      n1[l] = FN1(l, tx);
+      s1[l] = FS(l, 0, tx);
+    }
+    kt_flux<V>(g, 1, n1, s1, ring(g, R_CSN, ir), ring(g, R_CSS, ir), ring(g, R_DAYS, ir), ef);
+  } else if (t >= 2 * TX && t < 2 * TX + TY) {
+    fk = 1;
+    fr = t - 2 * TX;
+    double e1[V], w1[V];
+#pragma unroll
+    for (int l = 0; l < V; l++) {
+      e1[l] = FE1(l, fr);
+      w1[l] = FW(l, fr, 0);
+    }
+    kt_flux<V>(g, 0, e1, w1, ring(g, R_CSE, i0 - 1), ring(g, R_CSW, i0), ring(g, R_DAXW, i0), ef);
   }
   __syncthreads();
 #pragma unroll
   for (int l = 0; l < V; l++) {
-    if (xface && tx <= BX - 2) fa_s[(l * BY + ty) * BX + tx] = xf[l];
-    if (yface && ty <= BY - 2) fb_s[(l * BY + ty) * BX + tx] = yf[l];
+    FLX(l, ty, tx + 1) = xf[l];
+    FLY(l, ty + 1, tx) = yf[l];
+    if (fk == 0) FLY(l, 0, tx) = ef[l];
+    if (fk == 1) FLX(l, fr, 0) = ef[l];
   }
   __syncthreads();
 
   // ---- phase 4: rhs assembly (timedisc_base.f90:1254-1271)
+  double err = -DBL_MAX;
   if (out) {
-    const double dydzdV = ring(g, R_DYDZDV, fi), dzdxdV = ring(g, R_DZDXDV, fi);
-    const double dxdydV = ring(g, R_DXDYDV, fi);
-    const long ix = cidx(g, fi, fj);
+    const double dydzdV = ring(g, R_DYDZDV, i), dzdxdV = ring(g, R_DZDXDV, i);
+    const double dxdydV = ring(g, R_DXDYDV, i);
+    const unsigned ix = (unsigned)j * (unsigned)g.P + (unsigned)i;
+    double rhs[V];
 #pragma unroll
     for (int l = 0; l < V; l++) {
-      const double xw = fa_s[(l * BY + ty) * BX + tx - 1], ys = fb_s[(l * BY + ty - 1) * BX + tx];
-      a.kout[l * g.FS + ix] = dydzdV * (xf[l] - xw) + dzdxdV * (yf[l] - ys) + dxdydV * (0.0 - 0.0) - geo[l] - ext[l];
-      if (a.yout) a.yout[l * g.FS + ix] = uc[l];
+      const double xw = FLX(l, ty, tx), ys = FLY(l, ty, tx);
+      rhs[l] = dydzdV * (xf[l] - xw) + dzdxdV * (yf[l] - ys) + dxdydV * (0.0 - 0.0) - geo[l] - ext[l];
+      if (a.yout) (a.yout + l * g.FS)[ix] = uc[l];
+    }
+    if (a.ynew) {
+      // final combination with this stage's rhs as k_s, and the error of the
+      // embedded solution (same operation order as the reference)
+      double y[V], kq[MAXK][V];
+#pragma unroll
+      for (int l = 0; l < V; l++) y[l] = (a.Y + l * g.FS)[ix];
+#pragma unroll
+      for (int q = 0; q < MAXK; q++)
+        if (q < in.nk) {
+#pragma unroll
+          for (int l = 0; l < V; l++) kq[q][l] = (a.k[q] + l * g.FS)[ix];
+        }
+#pragma unroll
+      for (int l = 0; l < V; l++) {
+        double yh = y[l], yl = y[l];
+#pragma unroll
+        for (int q = 0; q < MAXK; q++)
+          if (q < in.nk) {
+            yl = yl - (a.bl[q] * in.dt) * kq[q][l];
+            yh = yh - (a.bh[q] * in.dt) * kq[q][l];
+          }
+        yl = yl - (a.bl[in.nk] * in.dt) * rhs[l];
+        yh = yh - (a.bh[in.nk] * in.dt) * rhs[l];
+        (a.ynew + l * g.FS)[ix] = yh;
+        err = fmax2(err, fabs(yh - yl) / (a.tol_rel * fabs(yh) + a.tol_abs[l]));
+      }
+    } else {
+#pragma unroll
+      for (int l = 0; l < V; l++) (a.kout + l * g.FS)[ix] = rhs[l];
+    }
+  }
+  if (a.ynew) {
+    const double m = blk_max(err, redd);
+    if (t == 0) {
+      const int b = blockIdx.x + blockIdx.y * gridDim.x;
+      a.part_c[2 * b] = m;
+      a.part_c[2 * b + 1] = 0.0;
     }
   }
 
   if (!(a.want_cfl || a.want_check)) return;
-  // ---- ghost cells owned by edge blocks: data checks, CFL (ghosts included,
+  // ---- ghost cells owned by edge tiles: data checks, CFL (ghosts included,
   // timedisc_base.f90:869-872) and the ghost cells of the new state
   {
-    constexpr int NXG = 2 * (BY - 2), NYG = 2 * (BX - 2);
     const bool west = blockIdx.x == 0, east = blockIdx.x == gridDim.x - 1;
     const bool south = blockIdx.y == 0, north = blockIdx.y == gridDim.y - 1;
-    int i = -1, j = -1;
-    if (t < 2 * NXG) {  // x ghosts on the output rows
-      const int side = t / NXG, r = t % NXG, gl = 1 + r / (BY - 2);
-      j = j0 + r % (BY - 2);
-      if ((side ? east : west) && j <= jmax) i = side ? imax + gl : imin - gl;
-    } else if (t < 2 * NXG + 2 * NYG) {  // y ghosts on the output columns
-      const int k = t - 2 * NXG, side = k / NYG, r = k % NYG, gl = 1 + r / (BX - 2);
-      i = i0 + r % (BX - 2);
-      if ((side ? north : south) && i <= imax) j = side ? jmax + gl : jmin - gl;
-      else i = -1;
-    } else if (t < 2 * NXG + 2 * NYG + 16) {  // corners
-      const int k = t - 2 * NXG - 2 * NYG, c = k / 4, ga = 1 + (k & 1), gb = 1 + ((k >> 1) & 1);
+    int gi = -1, gj = -1;
+    if (t < 4 * TY) {  // x ghosts of the tile rows: side, layer, row
+      const int side = t / (2 * TY), gl = 1 + (t / TY) % 2, jj = j0 + t % TY;
+      if ((side ? east : west) && jj <= jmax) {
+        gi = side ? imax + gl : imin - gl;
+        gj = jj;
+      }
+    } else if (t < 4 * TY + 4 * TX) {  // y ghosts of the tile columns
+      const int k = t - 4 * TY, side = k / (2 * TX), gl = 1 + (k / TX) % 2, ii = i0 + k % TX;
+      if ((side ? north : south) && ii <= imax) {
+        gi = ii;
+        gj = side ? jmax + gl : jmin - gl;
+      }
+    } else if (t < 4 * TY + 4 * TX + 16) {  // corners
+      const int k = t - 4 * TY - 4 * TX, c = k / 4, ga = 1 + (k & 1), gb = 1 + ((k >> 1) & 1);
       const int ce = c & 1, cn = c >> 1;
       if ((ce ? east : west) && (cn ? north : south)) {
-        i = ce ? imax + ga : imin - ga;
-        j = cn ? jmax + gb : jmin - gb;
+        gi = ce ? imax + ga : imin - ga;
+        gj = cn ? jmax + gb : jmin - gb;
       }
     }
-    if (i >= 0 && j >= 0) {
+    if (gi >= 0) {
       double u[V], p[V];
-      in.ghost_final(i, j, u, p, flags);
+      in.ghost_final(gi, gj, u, p, flags);
       if (a.want_cfl) {
-        const double cs = (V == 4) ? sound_speed(g.gamma, p[0], p[3]) : ring(g, R_CS, i);
-        invdt = fmax2(invdt, cell_invdt(g, i, p[1], p[2], cs));
+        const double cs = (V == 4) ? sound_speed(g.gamma, p[0], p[3]) : ring(g, R_CS, gi);
+        invdt = fmax2(invdt, cell_invdt(g, gi, p[1], p[2], cs));
       }
       if (a.yout) {
-        const long ix = cidx(g, i, j);
+        const unsigned ix = (unsigned)gj * (unsigned)g.P + (unsigned)gi;
 #pragma unroll
-        for (int l = 0; l < V; l++) a.yout[l * g.FS + ix] = u[l];
+        for (int l = 0; l < V; l++) (a.yout + l * g.FS)[ix] = u[l];
       }
     }
   }
@@ -689,61 +775,89 @@ __global__ __launch_bounds__(TILE_X *BY) void k_rhs(Geom g, StageArgs a) {
   }
 }
 
-// ------------------------------------------------------ final combination
-// ynew = Y - sum_q (bh_q*dt) k_q and the error estimate from the embedded
-// solution (timedisc_rkfehlberg.f90:199-266, timedisc_base.f90:1012-1053);
-// NK = number of stages, so every load of a cell is issued before any use
-template <int V, int NK>
-__global__ __launch_bounds__(256) void k_combine(Geom g, CombineArgs a) {
-  __shared__ double redd[4];
-  if (a.ctl && a.ctl->halt) return;
-  const double dt = a.ctl ? a.ctl->dt : a.dt;
-  double ch[NK], cl[NK];
-#pragma unroll
-  for (int q = 0; q < NK; q++) ch[q] = a.bh[q] * dt, cl[q] = a.bl[q] * dt;
-  const int i = GH + blockIdx.x * 64 + threadIdx.x, j = GH + blockIdx.y * 4 + threadIdx.y;
-  double e = -DBL_MAX;
-  if (i < GH + g.inum && j < GH + g.jnum) {
-    const long ix = cidx(g, i, j);
-    double y[V], k[NK][V];
-#pragma unroll
-    for (int l = 0; l < V; l++) {
-      y[l] = a.Y[l * g.FS + ix];
-#pragma unroll
-      for (int q = 0; q < NK; q++) k[q][l] = a.k[q][l * g.FS + ix];
-    }
-#pragma unroll
-    for (int l = 0; l < V; l++) {
-      double yh = y[l], yl = y[l];
-#pragma unroll
-      for (int q = 0; q < NK; q++) {
-        yl = yl - cl[q] * k[q][l];
-        yh = yh - ch[q] * k[q][l];
-      }
-      a.ynew[l * g.FS + ix] = yh;
-      e = fmax2(e, fabs(yh - yl) / (a.tol_rel * fabs(yh) + a.tol_abs[l]));
-    }
+// ------------------------------------------------------------ FARGO remap
+// FargoAdvectionY (timedisc_base.f90:1935-2069) of one interior cell from
+// the combined state: per ring the shift d = w*dt/dly split into its nearest
+// integer and a fraction, minmod-limited linear transport by the fraction,
+// then CSHIFT(., -shift) over JMIN:JMAX.  Rows outside JMIN:JMAX come from
+// the comoving-frame CenterBoundary (prim round trip, no frame change).
+template <int V>
+__device__ __forceinline__ void ycell(const Geom &g, const double *Y, int i, int jj, double *u) {
+  const int jmin = GH, jmax = GH + g.jnum - 1;
+  int js = jj;
+  bool ghost = false;
+  if (jj < jmin) {
+    ghost = true;
+    js = g.bc[2] == PERIODIC ? jj + g.jnum : jmin;
+  } else if (jj > jmax) {
+    ghost = true;
+    js = g.bc[3] == PERIODIC ? jj - g.jnum : jmax;
   }
-  const double m = blk_max(e, redd);
-  if (threadIdx.x == 0 && threadIdx.y == 0) {
-    const int b = blockIdx.x + blockIdx.y * gridDim.x;
-    a.part[2 * b] = m;
-    a.part[2 * b + 1] = 0.0;
+  const long ix = cidx(g, i, js);
+#pragma unroll
+  for (int l = 0; l < V; l++) u[l] = Y[l * g.FS + ix];
+  if (ghost) {
+    double p[V];
+    c2p<V>(u, p, g.gamma);
+    p2c<V>(p, u, g.gamma);
+  }
+}
+
+template <int V>
+__global__ __launch_bounds__(256) void k_remap(Geom g, RemapArgs a) {
+  if (a.ctl && a.ctl->halt) return;
+  double dts = a.dt_step;
+  if (a.ctl) {  // this%dt as AcceptSolution will set it
+    const double t = a.ctl->time, dt = a.ctl->dt;
+    dts = (t + dt) - t;
+  }
+  const int i = GH + blockIdx.x * 64 + threadIdx.x, j = GH + blockIdx.y * 4 + threadIdx.y;
+  if (i >= GH + g.inum || j >= GH + g.jnum) return;
+  const int jmin = GH, J = g.jnum;
+  double d = ring(g, R_W, i) * dts / ring(g, R_DLY, i);
+  const double shift = round(d);
+  d = d - shift;
+  // source row: jmin + ((j - jmin - shift) mod J); fmod is exact
+  int jj = (j - jmin) - (int)fmod(shift, (double)J);
+  if (jj < 0) jj += J;
+  if (jj >= J) jj -= J;
+  const int js = jmin + jj;
+  double v[5][V];
+#pragma unroll
+  for (int q = 0; q < 5; q++) ycell<V>(g, a.Y, i, js - 2 + q, v[q]);
+  const long ix = cidx(g, i, j);
+#pragma unroll
+  for (int l = 0; l < V; l++) {
+    double dq[4], dql[3];
+#pragma unroll
+    for (int q = 0; q < 4; q++) dq[q] = v[q + 1][l] - v[q][l];
+#pragma unroll
+    for (int q = 0; q < 3; q++)
+      dql[q] = (fsign(1.0, dq[q]) * fsign(1.0, dq[q + 1]) > 0.) ? fsign(fmin2(fabs(dq[q]), fabs(dq[q + 1])), dq[q])
+                                                              : 0.;
+    double f0, f1;  // flux(js-1), flux(js)
+    if (d > 0.) {
+      f0 = v[1][l] + .5 * dql[0] * (1. - d);
+      f1 = v[2][l] + .5 * dql[1] * (1. - d);
+    } else {
+      f0 = v[2][l] - .5 * dql[1] * (1. + d);
+      f1 = v[3][l] - .5 * dql[2] * (1. + d);
+    }
+    a.out[l * g.FS + ix] = v[2][l] - d * (f1 - f0);
   }
 }
 
 // Step control of a device-controlled batch (one block): embedded error of
-// the step from the k_combine partials, accept test, then CFL and data checks
+// the step from the partials of the last stage, accept test, then CFL and data checks
 // from the end-of-step k_rhs partials and the next time step.
-__global__ __launch_bounds__(1024) void k_control(Ctl *ctl, const double *part_c, int nc, const double *part_r,
-                                                  int nr) {
+__global__ __launch_bounds__(1024) void k_control(Ctl *ctl, const double *part_c, const double *part_r, int nb) {
   __shared__ double redd[16];
   __shared__ int redi[16];
   if (ctl->halt) return;
   double e = -DBL_MAX, m = -DBL_MAX;
   int f = 0;
-  for (int b = threadIdx.x; b < nc; b += blockDim.x) e = fmax2(e, part_c[2 * b]);
-  for (int b = threadIdx.x; b < nr; b += blockDim.x) {
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    e = fmax2(e, part_c[2 * b]);
     m = fmax2(m, part_r[2 * b]);
     f |= (int)part_r[2 * b + 1];
   }
@@ -791,50 +905,38 @@ __global__ __launch_bounds__(256) void k_reduce(const double *part, int n, doubl
 }
 
 // -------------------------------------------------------------- launchers
-static int g_by = 8;
-void set_rhs_rows(int by) { g_by = (by == 16) ? 16 : 8; }
+int rhs_blocks(const Geom &g) { return ((g.inum + TILE_X - 1) / TILE_X) * ((g.jnum + TY_RHS - 1) / TY_RHS); }
 
-int rhs_blocks(const Geom &g) {
-  return ((g.inum + TILE_X - 3) / (TILE_X - 2)) * ((g.jnum + g_by - 3) / (g_by - 2));
-}
-int combine_blocks(const Geom &g) { return ((g.inum + 63) / 64) * ((g.jnum + 3) / 4); }
-
-template <int V, int BY>
-static void rhs_by(const Geom &g, const StageArgs &a, int mode, hipStream_t s) {
-  dim3 grid((g.inum + TILE_X - 3) / (TILE_X - 2), (g.jnum + BY - 3) / (BY - 2)), block(TILE_X, BY);
-  if (mode == IN_REMAP)
-    hipLaunchKernelGGL((k_rhs<V, BY, IN_REMAP>), grid, block, 0, s, g, a);
-  else
-    hipLaunchKernelGGL((k_rhs<V, BY, IN_STAGE>), grid, block, 0, s, g, a);
-}
-void launch_rhs(const Geom &g, const StageArgs &a, int V, int mode, hipStream_t s) {
-  if (V == 3) {
-    if (g_by == 16) rhs_by<3, 16>(g, a, mode, s);
-    else rhs_by<3, 8>(g, a, mode, s);
-  } else {
-    if (g_by == 16) rhs_by<4, 16>(g, a, IN_STAGE, s);
-    else rhs_by<4, 8>(g, a, IN_STAGE, s);
-  }
+template <int V, int LIM>
+static void rhs_lim(const Geom &g, const StageArgs &a, hipStream_t s) {
+  dim3 grid((g.inum + TILE_X - 1) / TILE_X, (g.jnum + TY_RHS - 1) / TY_RHS), block(TILE_X, TY_RHS);
+  hipLaunchKernelGGL((k_rhs<V, LIM>), grid, block, 0, s, g, a);
 }
 template <int V>
-static void combine_v(const Geom &g, const CombineArgs &a, hipStream_t s) {
-  dim3 grid((g.inum + 63) / 64, (g.jnum + 3) / 4), block(64, 4);
-  switch (a.nk) {
-    case 3: hipLaunchKernelGGL((k_combine<V, 3>), grid, block, 0, s, g, a); break;
-    case 5: hipLaunchKernelGGL((k_combine<V, 5>), grid, block, 0, s, g, a); break;
-    case 6: hipLaunchKernelGGL((k_combine<V, 6>), grid, block, 0, s, g, a); break;
-    case 7: hipLaunchKernelGGL((k_combine<V, 7>), grid, block, 0, s, g, a); break;
-    default: break;  // the host accepts only tableaux with 3, 5, 6 or 7 stages
-  }
-}
-void launch_combine(const Geom &g, const CombineArgs &a, int V, hipStream_t s) {
-  if (V == 3)
-    combine_v<3>(g, a, s);
+static void rhs_v(const Geom &g, const StageArgs &a, hipStream_t s) {
+  // compile-time limiters for the common choices, run-time switch otherwise
+  if (g.limiter == VANLEER)
+    rhs_lim<V, VANLEER>(g, a, s);
+  else if (g.limiter == MINMOD)
+    rhs_lim<V, MINMOD>(g, a, s);
   else
-    combine_v<4>(g, a, s);
+    rhs_lim<V, 0>(g, a, s);
 }
-void launch_control(Ctl *ctl, const double *part_c, int nc, const double *part_r, int nr, hipStream_t s) {
-  hipLaunchKernelGGL(k_control, dim3(1), dim3(1024), 0, s, ctl, part_c, nc, part_r, nr);
+void launch_rhs(const Geom &g, const StageArgs &a, int V, hipStream_t s) {
+  if (V == 3)
+    rhs_v<3>(g, a, s);
+  else
+    rhs_v<4>(g, a, s);
+}
+void launch_remap(const Geom &g, const RemapArgs &a, int V, hipStream_t s) {
+  dim3 grid((g.inum + 63) / 64, (g.jnum + 3) / 4), block(64, 4);
+  if (V == 3)
+    hipLaunchKernelGGL(k_remap<3>, grid, block, 0, s, g, a);
+  else
+    hipLaunchKernelGGL(k_remap<4>, grid, block, 0, s, g, a);
+}
+void launch_control(Ctl *ctl, const double *part_c, const double *part_r, int nb, hipStream_t s) {
+  hipLaunchKernelGGL(k_control, dim3(1), dim3(1024), 0, s, ctl, part_c, part_r, nb);
 }
 void launch_frame(const Geom &g, const double *src, double *dst, int sign, int V, hipStream_t s) {
   dim3 grid((g.NI + 63) / 64, (g.NJ + 3) / 4), block(64, 4);

@@ -107,9 +107,10 @@ const char *fv_last_error(const fv_ctx *ctx);
  * fv_profile(ctx, 1) resets the statistics and brackets every kernel launch
  * with HIP events on the context's stream (launches are then issued one by
  * one instead of as HIP graphs).  Kernel classes: 0 fused RHS on an RK stage
- * state, 1 fused RHS on the FARGO-remapped state (end of step), 2 final RK
- * combination with embedded error, 3 FARGO frame change, 4 step control of
- * the device-controlled batches (one block).  fv_kernel_stats
+ * state, 1 end-of-step RHS (CFL, data checks, next step's first slope),
+ * 2 last RK stage fused with the final combination and embedded error,
+ * 3 FARGO frame change, 4 step control of the device-controlled batches (one
+ * block), 5 FARGO azimuthal remap.  fv_kernel_stats
  * returns the number of launches, their summed device time in ms and their
  * algorithmic bytes (state words each launch must read and write, 8 bytes
  * per word, interior cells). */
