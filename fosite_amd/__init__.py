@@ -17,7 +17,7 @@ import numpy as np
 from . import setups
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIBPATH = os.path.join(HERE, "libfosite_fv.so")
+LIBPATH = os.environ.get("FOSITE_FV_LIB") or os.path.join(HERE, "libfosite_fv.so")
 
 __all__ = ["Fosite", "FvConfig", "mesh_array", "setups", "load_library", "LIBPATH"]
 

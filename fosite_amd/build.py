@@ -15,16 +15,49 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-ffp-contract=off", "-std=c++17", "-fP
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
-def build(verbose=False):
-    cmd = [HIPCC] + FLAGS + ["-o", OUT] + SRC
+def build(verbose=False, out=OUT, defines=()):
+    cmd = [HIPCC] + FLAGS + ["-D" + d for d in defines] + ["-o", out] + SRC
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stderr)
-        raise RuntimeError("hipcc failed building %s" % OUT)
+        raise RuntimeError("hipcc failed building %s" % out)
     if verbose:
-        print("built", OUT)
-    return OUT
+        print("built", out)
+    return out
+
+
+FC = os.environ.get("FC_AMD", "/opt/rocm/bin/amdflang")
+FDIR = os.path.join(HERE, "fortran")
+FBUILD = os.path.join(FDIR, "build")
+FSRC = ["fosite_fv_c.f90", "fosite_gpu.f90"]
+FPROG = "fv_run"
+
+
+def build_fortran(verbose=False):
+    """Fortran 2008 host (fosite_gpu_mod) and the fv_run setup program,
+    linked against the in-tree library (run-time path relative to the
+    executable)."""
+    os.makedirs(FBUILD, exist_ok=True)
+    objs = []
+    for f in FSRC + [FPROG + ".f90"]:
+        obj = os.path.join(FBUILD, f[:-4] + ".o")
+        cmd = [FC, "-O2", "-c", "-J", FBUILD, "-o", obj, os.path.join(FDIR, f)]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError("amdflang failed on %s" % f)
+        objs.append(obj)
+    exe = os.path.join(FBUILD, FPROG)
+    cmd = [FC, "-o", exe] + objs + ["-L" + HERE, "-lfosite_fv", "-Wl,-rpath,$ORIGIN/../.."]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("linking %s failed" % exe)
+    if verbose:
+        print("built", exe)
+    return exe
 
 
 if __name__ == "__main__":
     build(verbose=True)
+    build_fortran(verbose=True)

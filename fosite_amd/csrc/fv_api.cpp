@@ -670,6 +670,8 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
   g.limiter = c.limiter; g.theta = c.theta;
   for (int d = 0; d < 4; d++) g.bc[d] = c.bc[d];
   g.fargo = c.fargo == 2; g.gravity = c.gravity != 0; g.curvilinear = c.geometry != CARTESIAN;
+  g.tile_x = 64;
+  if (const char *e = std::getenv("FV_RHS_TILE")) g.tile_x = std::atoi(e) == 32 ? 32 : 64;
   ctx->nb_rhs = rhs_blocks(g);
   ok = ok && dalloc(&ctx->part_r, sizeof(double) * 2 * ctx->nb_rhs) &&
        dalloc(&ctx->part_c, sizeof(double) * 2 * ctx->nb_rhs) && dalloc(&ctx->d_red, sizeof(double) * 2);

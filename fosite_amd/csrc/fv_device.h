@@ -34,6 +34,7 @@ struct Geom {
   double theta;
   int bc[4];
   int fargo, gravity, curvilinear;
+  int tile_x;           // k_rhs tile width in cells: 64 or 32
   const double *ring;   // [R_COUNT][NI]
   const double *col;    // [C_COUNT][NJ]
 };

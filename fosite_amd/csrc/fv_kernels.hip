@@ -420,39 +420,51 @@ __device__ void control_next(Ctl *c, double invdt, int flags) {
 }
 
 // -------------------------------------------------------------- fused RHS
-// Tile: 64 x TY_RHS output cells, one thread each.  The footprint (2-cell
+// Tile: TX x TY_RHS output cells (TX = 64 or 32), one thread each.  The footprint (2-cell
 // halo, corners excluded) is staged in LDS as the state the fluxes see;
 // threads with an extra job cover the halo cells and the faces and fluxes one
 // cell beyond the tile.  Phases, separated by barriers:
 //   1 stage state + transform of the own and one extra footprint cell (ghost
 //     cells on boundary tiles), sources, data checks and CFL of the own cell
-//   2 limited slopes and face states: own cell; rows -1 and TY; columns -1, 64
+//   2 limited slopes and face states: own cell; rows -1 and TY; columns -1, TX
 //   3 KT interface fluxes: own east and north interfaces; west of column 0,
 //     south of row 0
 //   4 rhs assembly, or the final RK combination in the last stage; stores
+// Faces live in their own LDS region; the fluxes reuse the footprint region,
+// which is dead after phase 2, so three barriers separate the four phases.
 // Output-aligned tiles keep every lane of the flux phases busy and divide
-// power-of-two meshes exactly.
-constexpr int TY_RHS = 8;
+// power-of-two meshes exactly; at most 128 VGPRs, so that 4 waves share
+// each SIMD.
+#ifndef FV_RHS_ROWS
+#define FV_RHS_ROWS 8
+#endif
+#ifndef FV_RHS_WAVES_PER_SIMD
+#define FV_RHS_WAVES_PER_SIMD 4
+#endif
+constexpr int TY_RHS = FV_RHS_ROWS;
 
-template <int V, int LIM>
-__global__ __launch_bounds__(TILE_X *TY_RHS, 4) void k_rhs(Geom g, StageArgs a) {
-  constexpr int TX = TILE_X, TY = TY_RHS, SX = TX + 4, SY = TY + 4;
+template <int V, int LIM, int TX>
+__global__ __launch_bounds__(TX *TY_RHS, FV_RHS_WAVES_PER_SIMD) void k_rhs(Geom g, StageArgs a) {
+  constexpr int TY = TY_RHS, SX = TX + 4, SY = TY + 4;
   constexpr int N_UC = V * SY * SX;
   constexpr int XW = V * TY * (TX + 1), XE = V * TY, YS = V * (TY + 1) * TX, YN = V * TX;
   constexpr int N_FACE = XW + XE + YS + YN;
   constexpr int FX = V * TY * (TX + 1), FY = V * (TY + 1) * TX;
   constexpr int N_FLUX = FX + FY;
-  constexpr int NL = N_UC > N_FACE ? (N_UC > N_FLUX ? N_UC : N_FLUX) : (N_FACE > N_FLUX ? N_FACE : N_FLUX);
-  __shared__ double lds[NL];
+  // region A: footprint state (phases 1-2), then fluxes (phases 3-4);
+  // region B: face states (phases 2-3)
+  constexpr int NA = N_UC > N_FLUX ? N_UC : N_FLUX;
+  __shared__ double lds[NA + N_FACE];
   __shared__ double redd[TY];
   __shared__ int redi[TY];
   // footprint state, tile-relative row r in [-2, TY+1], column c in [-2, TX+1]
   auto UC = [&](int l, int r, int c) -> double & { return lds[(l * SY + r + 2) * SX + c + 2]; };
   // faces: W of columns 0..TX, E of column -1, S of rows 0..TY, N of row -1
-  auto FW = [&](int l, int r, int c) -> double & { return lds[(l * TY + r) * (TX + 1) + c]; };
-  auto FE1 = [&](int l, int r) -> double & { return lds[XW + l * TY + r]; };
-  auto FS = [&](int l, int r, int c) -> double & { return lds[XW + XE + (l * (TY + 1) + r) * TX + c]; };
-  auto FN1 = [&](int l, int c) -> double & { return lds[XW + XE + YS + l * TX + c]; };
+  double *fb = lds + NA;
+  auto FW = [&](int l, int r, int c) -> double & { return fb[(l * TY + r) * (TX + 1) + c]; };
+  auto FE1 = [&](int l, int r) -> double & { return fb[XW + l * TY + r]; };
+  auto FS = [&](int l, int r, int c) -> double & { return fb[XW + XE + (l * (TY + 1) + r) * TX + c]; };
+  auto FN1 = [&](int l, int c) -> double & { return fb[XW + XE + YS + l * TX + c]; };
   // fluxes: x interface c-1/2 (c = 0..TX) of row r; y interface r-1/2 (r = 0..TY) of column c
   auto FLX = [&](int l, int r, int c) -> double & { return lds[(l * TY + r) * (TX + 1) + c]; };
   auto FLY = [&](int l, int r, int c) -> double & { return lds[FX + (l * (TY + 1) + r) * TX + c]; };
@@ -482,23 +494,10 @@ __global__ __launch_bounds__(TILE_X *TY_RHS, 4) void k_rhs(Geom g, StageArgs a) 
   }
   const int ei = i0 + ec, ej = j0 + er;
   const bool e_in = has_e && ei >= imin && ei <= imax && ej >= jmin && ej <= jmax;
-  double ur[V], pr[V];
-  {
-    double so[V], se[V], uo[V], ue[V], re[V], pe[V];
-    in.interior2(out, i, j, so, e_in, ei, ej, se);
-    if (out) {
-      in.transform(i, so, ur, pr, uo);
-#pragma unroll
-      for (int l = 0; l < V; l++) UC(l, ty, tx) = uo[l];
-    }
-    if (e_in) {
-      in.transform(ei, se, re, pe, ue);
-#pragma unroll
-      for (int l = 0; l < V; l++) UC(l, er, ec) = ue[l];
-    }
-  }
+  // ghost cells or cells beyond the mesh (boundary tiles only) first, so that
+  // no value of the interior path is live across this register-hungry code
   const bool own_g = !out, e_g = has_e && !e_in;
-  if (own_g || e_g) {  // ghost cells or cells beyond the mesh: boundary tiles only
+  if (own_g || e_g) {
 #pragma unroll 1
     for (int p = 0; p < 2; p++) {
       if (!(p ? e_g : own_g)) continue;
@@ -518,6 +517,21 @@ __global__ __launch_bounds__(TILE_X *TY_RHS, 4) void k_rhs(Geom g, StageArgs a) 
     }
   }
 
+  double ur[V], pr[V];
+  {
+    double so[V], se[V], uo[V], ue[V], re[V], pe[V];
+    in.interior2(out, i, j, so, e_in, ei, ej, se);
+    if (out) {
+      in.transform(i, so, ur, pr, uo);
+#pragma unroll
+      for (int l = 0; l < V; l++) UC(l, ty, tx) = uo[l];
+    }
+    if (e_in) {
+      in.transform(ei, se, re, pe, ue);
+#pragma unroll
+      for (int l = 0; l < V; l++) UC(l, er, ec) = ue[l];
+    }
+  }
   // sources, data checks and CFL of the own interior cell
   double geo[V], ext[V];
   double invdt = -DBL_MAX;
@@ -572,7 +586,7 @@ __global__ __launch_bounds__(TILE_X *TY_RHS, 4) void k_rhs(Geom g, StageArgs a) 
   // ---- phase 2: limited slopes and face states (reconstruction_linear.f90)
   const int ir = i < g.NI - 1 ? i : g.NI - 2;  // ring index, kept inside the mesh arrays
   const int jr = j < g.NJ ? j : g.NJ - 1;
-  double uc[V], fw[V], fe[V], fs[V], fn[V];
+  double uc[V], fe[V], fn[V];
   {
     const double dw = ring(g, R_RDXW, ir), de = ring(g, R_RDXE, ir);
     const double ds = g.col[C_RDYS * g.NJ + jr], dn = g.col[C_RDYN * g.NJ + jr];
@@ -580,55 +594,49 @@ __global__ __launch_bounds__(TILE_X *TY_RHS, 4) void k_rhs(Geom g, StageArgs a) 
     for (int l = 0; l < V; l++) {
       uc[l] = UC(l, ty, tx);
       const double sx = g.invdx * slope<LIM>(g, UC(l, ty, tx - 1), uc[l], UC(l, ty, tx + 1));
-      fw[l] = uc[l] + sx * dw;
+      FW(l, ty, tx) = uc[l] + sx * dw;
       fe[l] = uc[l] + sx * de;
       const double sy = g.invdy * slope<LIM>(g, UC(l, ty - 1, tx), uc[l], UC(l, ty + 1, tx));
-      fs[l] = uc[l] + sy * ds;
+      FS(l, ty, tx) = uc[l] + sy * ds;
       fn[l] = uc[l] + sy * dn;
     }
   }
   // extra faces: N of row -1 and S of row TY (t < 2*TX); E of column -1 and
   // W of column TX (next 2*TY threads)
-  double xface[V];
-  int xk = -1, xr = 0, xc = 0;
   if (t < 2 * TX) {
-    xk = t / TX;  // 0: row -1 (N face), 1: row TY (S face)
-    xr = xk ? TY : -1;
-    xc = tx;
+    const int xk = t / TX;  // 0: row -1 (N face), 1: row TY (S face)
+    const int xr = xk ? TY : -1, xc = tx;
     const int jj = j0 + xr, jc = jj < g.NJ ? jj : g.NJ - 1;
     const double off = xk ? g.col[C_RDYS * g.NJ + jc] : g.col[C_RDYN * g.NJ + jc];
 #pragma unroll
     for (int l = 0; l < V; l++) {
       const double c0 = UC(l, xr, xc);
-      xface[l] = c0 + g.invdy * slope<LIM>(g, UC(l, xr - 1, xc), c0, UC(l, xr + 1, xc)) * off;
+      const double f = c0 + g.invdy * slope<LIM>(g, UC(l, xr - 1, xc), c0, UC(l, xr + 1, xc)) * off;
+      if (xk)
+        FS(l, TY, xc) = f;
+      else
+        FN1(l, xc) = f;
     }
   } else if (t < 2 * TX + 2 * TY) {
     const int k = t - 2 * TX;
-    xk = 2 + k / TY;  // 2: column -1 (E face), 3: column TX (W face)
-    xr = k % TY;
-    xc = xk == 3 ? TX : -1;
+    const int xk = k / TY;  // 0: column -1 (E face), 1: column TX (W face)
+    const int xr = k % TY, xc = xk ? TX : -1;
     const int ii = i0 + xc, ic = ii < g.NI ? ii : g.NI - 1;
-    const double off = xk == 3 ? ring(g, R_RDXW, ic) : ring(g, R_RDXE, ic);
+    const double off = xk ? ring(g, R_RDXW, ic) : ring(g, R_RDXE, ic);
 #pragma unroll
     for (int l = 0; l < V; l++) {
       const double c0 = UC(l, xr, xc);
-      xface[l] = c0 + g.invdx * slope<LIM>(g, UC(l, xr, xc - 1), c0, UC(l, xr, xc + 1)) * off;
+      const double f = c0 + g.invdx * slope<LIM>(g, UC(l, xr, xc - 1), c0, UC(l, xr, xc + 1)) * off;
+      if (xk)
+        FW(l, xr, TX) = f;
+      else
+        FE1(l, xr) = f;
     }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int l = 0; l < V; l++) {
-    FW(l, ty, tx) = fw[l];
-    FS(l, ty, tx) = fs[l];
-    if (xk == 0) FN1(l, xc) = xface[l];
-    if (xk == 1) FS(l, TY, xc) = xface[l];
-    if (xk == 2) FE1(l, xr) = xface[l];
-    if (xk == 3) FW(l, xr, TX) = xface[l];
   }
   __syncthreads();
 
   // ---- phase 3: KT interface fluxes (fluxes_kt.f90:126-176)
-  double xf[V], yf[V], ef[V];
+  double xf[V], yf[V];
   {
     double r2[V], n2[V];
 #pragma unroll
@@ -638,37 +646,35 @@ __global__ __launch_bounds__(TILE_X *TY_RHS, 4) void k_rhs(Geom g, StageArgs a) 
     }
     kt_flux<V>(g, 0, fe, r2, ring(g, R_CSE, ir), ring(g, R_CSW, ir + 1), ring(g, R_DAXW, ir + 1), xf);
     kt_flux<V>(g, 1, fn, n2, ring(g, R_CSN, ir), ring(g, R_CSS, ir), ring(g, R_DAYS, ir), yf);
+#pragma unroll
+    for (int l = 0; l < V; l++) {
+      FLX(l, ty, tx + 1) = xf[l];
+      FLY(l, ty + 1, tx) = yf[l];
+    }
   }
   // extra fluxes: south interface of row 0 (t < TX), west interface of
   // column 0 (threads 2*TX .. 2*TX+TY-1)
-  int fk = -1, fr = 0;
   if (t < TX) {
-    fk = 0;
-    double s1[V], n1[V];
+    double s1[V], n1[V], ef[V];
 #pragma unroll
     for (int l = 0; l < V; l++) {
       n1[l] = FN1(l, tx);
       s1[l] = FS(l, 0, tx);
     }
     kt_flux<V>(g, 1, n1, s1, ring(g, R_CSN, ir), ring(g, R_CSS, ir), ring(g, R_DAYS, ir), ef);
+#pragma unroll
+    for (int l = 0; l < V; l++) FLY(l, 0, tx) = ef[l];
   } else if (t >= 2 * TX && t < 2 * TX + TY) {
-    fk = 1;
-    fr = t - 2 * TX;
-    double e1[V], w1[V];
+    const int fr = t - 2 * TX;
+    double e1[V], w1[V], ef[V];
 #pragma unroll
     for (int l = 0; l < V; l++) {
       e1[l] = FE1(l, fr);
       w1[l] = FW(l, fr, 0);
     }
     kt_flux<V>(g, 0, e1, w1, ring(g, R_CSE, i0 - 1), ring(g, R_CSW, i0), ring(g, R_DAXW, i0), ef);
-  }
-  __syncthreads();
 #pragma unroll
-  for (int l = 0; l < V; l++) {
-    FLX(l, ty, tx + 1) = xf[l];
-    FLY(l, ty + 1, tx) = yf[l];
-    if (fk == 0) FLY(l, 0, tx) = ef[l];
-    if (fk == 1) FLX(l, fr, 0) = ef[l];
+    for (int l = 0; l < V; l++) FLX(l, fr, 0) = ef[l];
   }
   __syncthreads();
 
@@ -905,28 +911,31 @@ __global__ __launch_bounds__(256) void k_reduce(const double *part, int n, doubl
 }
 
 // -------------------------------------------------------------- launchers
-int rhs_blocks(const Geom &g) { return ((g.inum + TILE_X - 1) / TILE_X) * ((g.jnum + TY_RHS - 1) / TY_RHS); }
+int rhs_blocks(const Geom &g) { return ((g.inum + g.tile_x - 1) / g.tile_x) * ((g.jnum + TY_RHS - 1) / TY_RHS); }
 
-template <int V, int LIM>
+template <int V, int LIM, int TX>
 static void rhs_lim(const Geom &g, const StageArgs &a, hipStream_t s) {
-  dim3 grid((g.inum + TILE_X - 1) / TILE_X, (g.jnum + TY_RHS - 1) / TY_RHS), block(TILE_X, TY_RHS);
-  hipLaunchKernelGGL((k_rhs<V, LIM>), grid, block, 0, s, g, a);
+  dim3 grid((g.inum + TX - 1) / TX, (g.jnum + TY_RHS - 1) / TY_RHS), block(TX, TY_RHS);
+  hipLaunchKernelGGL((k_rhs<V, LIM, TX>), grid, block, 0, s, g, a);
 }
-template <int V>
+template <int V, int TX>
 static void rhs_v(const Geom &g, const StageArgs &a, hipStream_t s) {
   // compile-time limiters for the common choices, run-time switch otherwise
   if (g.limiter == VANLEER)
-    rhs_lim<V, VANLEER>(g, a, s);
+    rhs_lim<V, VANLEER, TX>(g, a, s);
   else if (g.limiter == MINMOD)
-    rhs_lim<V, MINMOD>(g, a, s);
+    rhs_lim<V, MINMOD, TX>(g, a, s);
   else
-    rhs_lim<V, 0>(g, a, s);
+    rhs_lim<V, 0, TX>(g, a, s);
 }
 void launch_rhs(const Geom &g, const StageArgs &a, int V, hipStream_t s) {
-  if (V == 3)
-    rhs_v<3>(g, a, s);
-  else
-    rhs_v<4>(g, a, s);
+  if (V == 3) {
+    if (g.tile_x == 32) rhs_v<3, 32>(g, a, s);
+    else rhs_v<3, 64>(g, a, s);
+  } else {
+    if (g.tile_x == 32) rhs_v<4, 32>(g, a, s);
+    else rhs_v<4, 64>(g, a, s);
+  }
 }
 void launch_remap(const Geom &g, const RemapArgs &a, int V, hipStream_t s) {
   dim3 grid((g.inum + 63) / 64, (g.jnum + 3) / 4), block(64, 4);

@@ -1,0 +1,150 @@
+!##############################################################################
+!# fosite_fv_c: ISO_C_BINDING interface of libfosite_fv.so                   #
+!##############################################################################
+!> Fortran 2008 declarations of every entry point of include/fosite_fv.h.
+!! The structs are BIND(C) mirrors of fv_config and fv_scalars; arrays cross
+!! the boundary as plain pointers in the marray layout (i fastest, then j,
+!! then field or face, ghost cells included).
+MODULE fosite_fv_c
+  USE, INTRINSIC :: iso_c_binding
+  IMPLICIT NONE
+  PRIVATE
+
+  TYPE, BIND(C), PUBLIC :: fv_config
+    INTEGER(c_int) :: physics = 17
+    REAL(c_double) :: gamma = 1.4_c_double
+    INTEGER(c_int) :: geometry = 1
+    INTEGER(c_int) :: inum = 32, jnum = 32
+    REAL(c_double) :: xmin = 0, xmax = 1, ymin = 0, ymax = 1, gparam = 1
+    INTEGER(c_int) :: bc(4) = 1
+    INTEGER(c_int) :: limiter = 1
+    REAL(c_double) :: theta = 1
+    INTEGER(c_int) :: recvars = 2
+    INTEGER(c_int) :: method = 5, order = 5
+    REAL(c_double) :: cfl = 0.4_c_double, tol_rel = 0.01_c_double
+    REAL(c_double) :: tol_abs(4) = 0.001_c_double
+    REAL(c_double) :: stoptime = 1, dtlimit = -1, beta = 0.08_c_double
+    INTEGER(c_int) :: fargo = 0, gravity = 0
+    REAL(c_double) :: mass = 1, GN = 1
+    INTEGER(c_int) :: dtype = 0
+  END TYPE fv_config
+
+  TYPE, BIND(C), PUBLIC :: fv_scalars
+    INTEGER(c_int) :: iter, n_adj, brk, dtcause
+    REAL(c_double) :: time, dt, dtold, maxerrold
+  END TYPE fv_scalars
+
+  PUBLIC :: fv_mesh_array, fv_create, fv_destroy, fv_dims, fv_set_sound_speeds, &
+       fv_set_background_velocity, fv_set_pvar, fv_first_step, fv_step, fv_run, &
+       fv_get_state, fv_get_scalars, fv_last_run_ms, fv_last_error, fv_profile, &
+       fv_kernel_stats, c_string
+
+  INTERFACE
+    INTEGER(c_int) FUNCTION fv_mesh_array(cfg, id, out) BIND(C, NAME="fv_mesh_array")
+      IMPORT :: c_int, c_double, fv_config
+      TYPE(fv_config), INTENT(IN) :: cfg
+      INTEGER(c_int), VALUE :: id
+      REAL(c_double), INTENT(OUT) :: out(*)
+    END FUNCTION fv_mesh_array
+    INTEGER(c_int) FUNCTION fv_create(ctx, cfg) BIND(C, NAME="fv_create")
+      IMPORT :: c_int, c_ptr, fv_config
+      TYPE(c_ptr), INTENT(OUT) :: ctx
+      TYPE(fv_config), INTENT(IN) :: cfg
+    END FUNCTION fv_create
+    INTEGER(c_int) FUNCTION fv_destroy(ctx) BIND(C, NAME="fv_destroy")
+      IMPORT :: c_int, c_ptr
+      TYPE(c_ptr), VALUE :: ctx
+    END FUNCTION fv_destroy
+    INTEGER(c_int) FUNCTION fv_dims(ctx, ni, nj, vnum) BIND(C, NAME="fv_dims")
+      IMPORT :: c_int, c_ptr
+      TYPE(c_ptr), VALUE :: ctx
+      INTEGER(c_int), INTENT(OUT) :: ni, nj, vnum
+    END FUNCTION fv_dims
+    INTEGER(c_int) FUNCTION fv_set_sound_speeds(ctx, bccsound, fcsound) BIND(C, NAME="fv_set_sound_speeds")
+      IMPORT :: c_int, c_ptr, c_double
+      TYPE(c_ptr), VALUE :: ctx
+      REAL(c_double), INTENT(IN) :: bccsound(*), fcsound(*)
+    END FUNCTION fv_set_sound_speeds
+    INTEGER(c_int) FUNCTION fv_set_background_velocity(ctx, w) BIND(C, NAME="fv_set_background_velocity")
+      IMPORT :: c_int, c_ptr, c_double
+      TYPE(c_ptr), VALUE :: ctx
+      REAL(c_double), INTENT(IN) :: w(*)
+    END FUNCTION fv_set_background_velocity
+    INTEGER(c_int) FUNCTION fv_set_pvar(ctx, pvar) BIND(C, NAME="fv_set_pvar")
+      IMPORT :: c_int, c_ptr, c_double
+      TYPE(c_ptr), VALUE :: ctx
+      REAL(c_double), INTENT(IN) :: pvar(*)
+    END FUNCTION fv_set_pvar
+    INTEGER(c_int) FUNCTION fv_first_step(ctx, brk) BIND(C, NAME="fv_first_step")
+      IMPORT :: c_int, c_ptr
+      TYPE(c_ptr), VALUE :: ctx
+      INTEGER(c_int), INTENT(OUT) :: brk
+    END FUNCTION fv_first_step
+    INTEGER(c_int) FUNCTION fv_step(ctx, done) BIND(C, NAME="fv_step")
+      IMPORT :: c_int, c_ptr
+      TYPE(c_ptr), VALUE :: ctx
+      INTEGER(c_int), INTENT(OUT) :: done
+    END FUNCTION fv_step
+    INTEGER(c_int) FUNCTION fv_run(ctx, nsteps, taken) BIND(C, NAME="fv_run")
+      IMPORT :: c_int, c_ptr
+      TYPE(c_ptr), VALUE :: ctx
+      INTEGER(c_int), VALUE :: nsteps
+      INTEGER(c_int), INTENT(OUT) :: taken
+    END FUNCTION fv_run
+    INTEGER(c_int) FUNCTION fv_get_state(ctx, cvar, pvar) BIND(C, NAME="fv_get_state")
+      IMPORT :: c_int, c_ptr
+      TYPE(c_ptr), VALUE :: ctx, cvar, pvar
+    END FUNCTION fv_get_state
+    INTEGER(c_int) FUNCTION fv_get_scalars(ctx, sc) BIND(C, NAME="fv_get_scalars")
+      IMPORT :: c_int, c_ptr, fv_scalars
+      TYPE(c_ptr), VALUE :: ctx
+      TYPE(fv_scalars), INTENT(OUT) :: sc
+    END FUNCTION fv_get_scalars
+    REAL(c_double) FUNCTION fv_last_run_ms(ctx) BIND(C, NAME="fv_last_run_ms")
+      IMPORT :: c_double, c_ptr
+      TYPE(c_ptr), VALUE :: ctx
+    END FUNCTION fv_last_run_ms
+    TYPE(c_ptr) FUNCTION fv_last_error(ctx) BIND(C, NAME="fv_last_error")
+      IMPORT :: c_ptr
+      TYPE(c_ptr), VALUE :: ctx
+    END FUNCTION fv_last_error
+    INTEGER(c_int) FUNCTION fv_profile(ctx, enable) BIND(C, NAME="fv_profile")
+      IMPORT :: c_int, c_ptr
+      TYPE(c_ptr), VALUE :: ctx
+      INTEGER(c_int), VALUE :: enable
+    END FUNCTION fv_profile
+    INTEGER(c_int) FUNCTION fv_kernel_stats(ctx, kernel, launches, ms, alg_bytes) BIND(C, NAME="fv_kernel_stats")
+      IMPORT :: c_int, c_ptr, c_double
+      TYPE(c_ptr), VALUE :: ctx
+      INTEGER(c_int), VALUE :: kernel
+      INTEGER(c_int), INTENT(OUT) :: launches
+      REAL(c_double), INTENT(OUT) :: ms, alg_bytes
+    END FUNCTION fv_kernel_stats
+    FUNCTION c_strlen(s) BIND(C, NAME="strlen")
+      IMPORT :: c_ptr, c_size_t
+      TYPE(c_ptr), VALUE :: s
+      INTEGER(c_size_t) :: c_strlen
+    END FUNCTION c_strlen
+  END INTERFACE
+
+CONTAINS
+
+  !> copy of a NUL-terminated C string
+  FUNCTION c_string(p) RESULT(s)
+    TYPE(c_ptr), INTENT(IN) :: p
+    CHARACTER(LEN=:), ALLOCATABLE :: s
+    CHARACTER(KIND=c_char), POINTER :: chars(:)
+    INTEGER :: n, k
+    IF (.NOT.C_ASSOCIATED(p)) THEN
+      s = ""
+      RETURN
+    END IF
+    n = INT(c_strlen(p))
+    CALL C_F_POINTER(p, chars, [n])
+    ALLOCATE(CHARACTER(LEN=n) :: s)
+    DO k = 1, n
+      s(k:k) = chars(k)
+    END DO
+  END FUNCTION c_string
+
+END MODULE fosite_fv_c
