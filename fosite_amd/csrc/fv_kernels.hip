@@ -457,6 +457,10 @@ __global__ __launch_bounds__(TX *TY_RHS, FV_RHS_WAVES_PER_SIMD) void k_rhs(Geom 
   __shared__ double lds[NA + N_FACE];
   __shared__ double redd[TY];
   __shared__ int redi[TY];
+  // CFL term and check flags of the ghost cells a thread evaluated (kept in
+  // LDS rather than registers across the interior path)
+  __shared__ double gcfl[TX * TY];
+  __shared__ int gflg[TX * TY];
   // footprint state, tile-relative row r in [-2, TY+1], column c in [-2, TX+1]
   auto UC = [&](int l, int r, int c) -> double & { return lds[(l * SY + r + 2) * SX + c + 2]; };
   // faces: W of columns 0..TX, E of column -1, S of rows 0..TY, N of row -1
@@ -495,10 +499,16 @@ __global__ __launch_bounds__(TX *TY_RHS, FV_RHS_WAVES_PER_SIMD) void k_rhs(Geom 
   const int ei = i0 + ec, ej = j0 + er;
   const bool e_in = has_e && ei >= imin && ei <= imax && ej >= jmin && ej <= jmax;
   // ghost cells or cells beyond the mesh (boundary tiles only) first, so that
-  // no value of the interior path is live across this register-hungry code
+  // no value of the interior path is live across this register-hungry code.
+  // Every non-corner ghost cell of the mesh lies in some tile's footprint, so
+  // the ghost part of the data checks and of the CFL reduction (ghosts
+  // included, timedisc_base.f90:869-872) and the ghost cells of the new state
+  // are produced here; a cell seen by two tiles contributes identical values.
   const bool own_g = !out, e_g = has_e && !e_in;
   if (own_g || e_g) {
-#pragma unroll 1
+    double ginv = -DBL_MAX;
+    int gf = 0;
+#pragma unroll
     for (int p = 0; p < 2; p++) {
       if (!(p ? e_g : own_g)) continue;
       const int ii = p ? ei : i, jj = p ? ej : j;
@@ -506,8 +516,16 @@ __global__ __launch_bounds__(TX *TY_RHS, FV_RHS_WAVES_PER_SIMD) void k_rhs(Geom 
       double u[V];
       if ((xin || yin) && ii >= 0 && jj >= 0 && ii < g.NI && jj < g.NJ) {
         double pp[V];
-        int f = 0;
-        in.ghost_final(ii, jj, u, pp, f);
+        in.ghost_final(ii, jj, u, pp, gf);
+        if (a.want_cfl) {
+          const double cs = (V == 4) ? sound_speed(g.gamma, pp[0], pp[3]) : ring(g, R_CS, ii);
+          ginv = fmax2(ginv, cell_invdt(g, ii, pp[1], pp[2], cs));
+        }
+        if (a.yout) {
+          const unsigned ix = (unsigned)jj * (unsigned)g.P + (unsigned)ii;
+#pragma unroll
+          for (int l = 0; l < V; l++) (a.yout + l * g.FS)[ix] = u[l];
+        }
       } else {
 #pragma unroll
         for (int l = 0; l < V; l++) u[l] = 0.0;  // corner or beyond the mesh: never needed
@@ -515,6 +533,8 @@ __global__ __launch_bounds__(TX *TY_RHS, FV_RHS_WAVES_PER_SIMD) void k_rhs(Geom 
 #pragma unroll
       for (int l = 0; l < V; l++) UC(l, p ? er : ty, p ? ec : tx) = u[l];
     }
+    gcfl[t] = ginv;
+    gflg[t] = gf;
   }
 
   double ur[V], pr[V];
@@ -732,33 +752,19 @@ __global__ __launch_bounds__(TX *TY_RHS, FV_RHS_WAVES_PER_SIMD) void k_rhs(Geom 
   }
 
   if (!(a.want_cfl || a.want_check)) return;
-  // ---- ghost cells owned by edge tiles: data checks, CFL (ghosts included,
-  // timedisc_base.f90:869-872) and the ghost cells of the new state
-  {
+  if (own_g || e_g) {  // ghost cells evaluated in phase 1
+    invdt = fmax2(invdt, gcfl[t]);
+    flags |= gflg[t];
+  }
+  // ---- the corner ghost cells (in no footprint) of the corner tiles:
+  // data checks, CFL and the new state (SetCornerEdges, boundary_generic.f90:434-517)
+  if (t < 16) {
     const bool west = blockIdx.x == 0, east = blockIdx.x == gridDim.x - 1;
     const bool south = blockIdx.y == 0, north = blockIdx.y == gridDim.y - 1;
-    int gi = -1, gj = -1;
-    if (t < 4 * TY) {  // x ghosts of the tile rows: side, layer, row
-      const int side = t / (2 * TY), gl = 1 + (t / TY) % 2, jj = j0 + t % TY;
-      if ((side ? east : west) && jj <= jmax) {
-        gi = side ? imax + gl : imin - gl;
-        gj = jj;
-      }
-    } else if (t < 4 * TY + 4 * TX) {  // y ghosts of the tile columns
-      const int k = t - 4 * TY, side = k / (2 * TX), gl = 1 + (k / TX) % 2, ii = i0 + k % TX;
-      if ((side ? north : south) && ii <= imax) {
-        gi = ii;
-        gj = side ? jmax + gl : jmin - gl;
-      }
-    } else if (t < 4 * TY + 4 * TX + 16) {  // corners
-      const int k = t - 4 * TY - 4 * TX, c = k / 4, ga = 1 + (k & 1), gb = 1 + ((k >> 1) & 1);
-      const int ce = c & 1, cn = c >> 1;
-      if ((ce ? east : west) && (cn ? north : south)) {
-        gi = ce ? imax + ga : imin - ga;
-        gj = cn ? jmax + gb : jmin - gb;
-      }
-    }
-    if (gi >= 0) {
+    const int c = t / 4, ga = 1 + (t & 1), gb = 1 + ((t >> 1) & 1);
+    const int ce = c & 1, cn = c >> 1;
+    if ((ce ? east : west) && (cn ? north : south)) {
+      const int gi = ce ? imax + ga : imin - ga, gj = cn ? jmax + gb : jmin - gb;
       double u[V], p[V];
       in.ghost_final(gi, gj, u, p, flags);
       if (a.want_cfl) {
