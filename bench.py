@@ -23,9 +23,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the stage RHS kernel from rocprofv3 FETCH_SIZE +
-# WRITE_SIZE (calibrated against a copy kernel); None until measured
-TRAFFIC_BYTES_PER_LAUNCH = None
+# HBM bytes per launch of the stage-2 RHS kernel on this workload: rocprofv3
+# FETCH_SIZE x2.0 + WRITE_SIZE x1.0 (factors from a 1 GiB copy kernel),
+# 15702 KiB and 13056 KiB per dispatch (profiles/k_rhs_disk512x1024_hbm_traffic.md)
+TRAFFIC_BYTES_PER_LAUNCH = 45.5e6
 METRIC = "cell-updates/s & HBM GB/s vs plain FP64 at stated err; p50/p99 run latency"
 
 
