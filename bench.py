@@ -62,6 +62,134 @@ def cpu_baseline(cfg, init, steps):
                 sample="%d steps of the %dx%d disk (C restatement, gcc -O2)" % (steps, ni, nj))
 
 
+def run_until(sim, nsteps=None):
+    """Advance in device-controlled batches until nsteps steps were taken or
+    the run ended; returns (steps, device ms)."""
+    taken, ms = 0, 0.0
+    while nsteps is None or taken < nsteps:
+        want = 512 if nsteps is None else min(512, nsteps - taken)
+        k = sim.run(want)
+        taken += k
+        ms += sim.last_run_ms()
+        if k < want:
+            break
+    return taken, ms
+
+
+def field_errors(a, b):
+    """Relative L1 over interior cells: density, momentum (vector), energy and
+    the two momentum components."""
+    import numpy as np
+    I = (slice(2, -2), slice(2, -2))
+    d, r = np.abs(a - b), np.abs(b)
+    out = {"density": float(d[0][I].sum() / r[0][I].sum()),
+           "momentum": float((d[1][I].sum() + d[2][I].sum()) / (r[1][I].sum() + r[2][I].sum())),
+           "x_momentum": float(d[1][I].sum() / r[1][I].sum()),
+           "y_momentum": float(d[2][I].sum() / r[2][I].sum())}
+    if a.shape[0] == 4:
+        out["energy"] = float(d[3][I].sum() / r[3][I].sum())
+    return out
+
+
+def precision_form(ni, nj, steps):
+    """The precision form on the Euler-disk workload: an FP64 run of `steps`
+    steps, the FP32 form run to the same simulation time, and an FP64 run with
+    a 100x tighter tolerance as the accuracy reference.  Reports both
+    throughputs, the relative L1 error of the FP32 run against the FP64 run
+    and of both against the reference, and the relative mass drift of each run
+    (FP64 sums of density x cell volume over the interior)."""
+    import fosite_amd
+    from fosite_amd import setups
+    cfg64 = setups.euler_disk_config(ni=ni, nj=nj, steps=steps)
+    init = setups.initial(cfg64)
+    vol = 1.0 / fosite_amd.mesh_array(cfg64, "dxdydV")[2:-2, 2:-2]
+
+    def mass(cv):
+        return float((cv[0, 2:-2, 2:-2] * vol).sum())
+
+    def run(cfg, nsteps=None):
+        sim = fosite_amd.Fosite(cfg)
+        sim.set_initial(init["pvar"])
+        sim.first_step()
+        cv0, _ = sim.state()
+        n, ms = run_until(sim, nsteps)
+        cv, _ = sim.state()
+        sc = sim.scalars()
+        sim.close()
+        return cv0, cv, sc, n, ms
+
+    cv0, cv64, sc64, n64, ms64 = run(cfg64, steps)
+    t_end = sc64["time"]
+    _, cv32, sc32, n32, ms32 = run(dict(cfg64, dtype="fp32", stoptime=t_end))
+    _, cvref, scref, nref, _ = run(dict(cfg64, stoptime=t_end, tol_rel=cfg64["tol_rel"] * 1e-2))
+    m0, m64, m32 = mass(cv0), mass(cv64), mass(cv32)
+    cells = ni * nj
+    return {"workload": "Euler disk, logcylindrical %dx%d, Dormand-Prince rtol 1e-4, absorbing r / "
+                        "periodic phi, point-mass gravity, %d FP64 steps" % (ni, nj, steps),
+            "fp64_cell_updates_per_s": cells * n64 / (ms64 * 1e-3),
+            "fp32_cell_updates_per_s": cells * n32 / (ms32 * 1e-3),
+            "speedup_fp32_over_fp64": (n32 / ms32) / (n64 / ms64),
+            "steps_fp64": n64, "steps_fp32": n32, "steps_reference": nref,
+            "t_end": t_end, "t_end_fp32": sc32["time"], "t_end_reference": scref["time"],
+            "rel_l1_fp32_vs_fp64": field_errors(cv32, cv64),
+            "rel_l1_fp64_vs_reference": field_errors(cv64, cvref),
+            "rel_l1_fp32_vs_reference": field_errors(cv32, cvref),
+            "reference": "FP64, rtol 1e-6",
+            "mass_drift_fp64": (m64 - m0) / m0, "mass_drift_fp32": (m32 - m0) / m0}
+
+
+def batched_form(n=64, steps=500, alone_steps=50):
+    """The batched form: n independent isothermal disks of uneven sizes in
+    one batch (every kernel of a step one launch over all disks, each round
+    one HIP graph launch, tiling chosen by the autotuner), `steps` SSPRK3
+    steps with FARGO.  Beside it: the same disks as an FP64 batch, and each
+    disk alone on the plain FP64 path (fv_run, `alone_steps` steps each)."""
+    import fosite_amd
+    from fosite_amd import setups
+
+    def make(cfgs):
+        sims = []
+        for c in cfgs:
+            init = setups.initial(c)
+            sim = fosite_amd.Fosite(c)
+            sim.set_initial(init["pvar"], init["bccsound"], init["fcsound"], init["w"])
+            sim.first_step()
+            sims.append(sim)
+        return sims
+
+    out = {}
+    for dtype in ("fp32", "fp64"):
+        cfgs = setups.batch_disks(n=n, steps=steps, dtype=dtype)
+        sims = make(cfgs)
+        b = fosite_amd.Batch(sims, autotune=True)
+        taken = b.run(steps)
+        info = b.info()
+        upd = sum(c["inum"] * c["jnum"] * k for c, k in zip(cfgs, taken))
+        out[dtype] = {"cell_updates_per_s": upd / (info["last_run_ms"] * 1e-3), "ms": info["last_run_ms"],
+                      "member_steps": sum(taken), "tiling": ["64 row-major", "64 XCD-chunked", "32 row-major",
+                                                             "32 XCD-chunked"][info["tiling"]],
+                      "tune_ms_per_step": info["tune_ms"], "device_steps": info["device_steps"],
+                      "host_steps": info["host_steps"], "halts": info["halts"]}
+        b.close()
+        for sim in sims:
+            sim.close()
+    cfgs = setups.batch_disks(n=n, steps=steps, dtype="fp64")
+    upd, ms = 0, 0.0
+    for c in cfgs:
+        (sim,) = make([c])
+        k = sim.run(alone_steps)
+        ms += sim.last_run_ms()
+        upd += c["inum"] * c["jnum"] * k
+        sim.close()
+    out["fp64_alone"] = {"cell_updates_per_s": upd / (ms * 1e-3), "ms": ms, "steps_per_disk": alone_steps}
+    cells = sum(c["inum"] * c["jnum"] for c in cfgs)
+    return {"workload": "%d isothermal Keplerian disks, radial cells from {128,256,384,512}, azimuthal = 2x, "
+                        "cs/vK U(0.03,0.08), FARGO, SSPRK3, %d steps (%d cells in all)" % (n, steps, cells),
+            "fp32_batch": out["fp32"], "fp64_batch": out["fp64"], "fp64_alone": out["fp64_alone"],
+            "speedup_fp32_batch_over_fp64_alone": out["fp32"]["cell_updates_per_s"] /
+            out["fp64_alone"]["cell_updates_per_s"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,6 +198,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--profile-steps", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--forms", action="store_true",
+                    help="also measure the precision form (Euler disk 1024x2048, DP5)")
+    ap.add_argument("--form-steps", type=int, default=500)
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU; the FV step shards by radial slabs "
@@ -131,6 +262,9 @@ def main():
         "kernels": kernels,
     }
     sim.close()
+    if a.forms:
+        out["precision_form"] = precision_form(1024, 2048, a.form_steps)
+        out["batched_form"] = batched_form()
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg, init, a.cpu_steps)
     print(json.dumps(out))

@@ -19,7 +19,7 @@ from . import setups
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.environ.get("FOSITE_FV_LIB") or os.path.join(HERE, "libfosite_fv.so")
 
-__all__ = ["Fosite", "FvConfig", "mesh_array", "setups", "load_library", "LIBPATH"]
+__all__ = ["Fosite", "Batch", "FvConfig", "mesh_array", "setups", "load_library", "LIBPATH"]
 
 
 class FvConfig(ctypes.Structure):
@@ -45,7 +45,7 @@ class FvConfig(ctypes.Structure):
             if name in ("bc", "tol_abs"):
                 continue
             if name == "dtype":
-                c.dtype = {"fp64": 0, 0: 0}.get(cfg.get("dtype", 0), -1)
+                c.dtype = {"fp64": 0, 0: 0, "fp32": 1, 1: 1}.get(cfg.get("dtype", 0), -1)
                 continue
             setattr(c, name, cfg[name])
         for d in range(4):
@@ -66,7 +66,21 @@ class FvScalars(ctypes.Structure):
 EXPORTS = ["fv_mesh_array", "fv_create", "fv_destroy", "fv_dims", "fv_set_sound_speeds",
            "fv_set_background_velocity", "fv_set_pvar", "fv_first_step", "fv_step", "fv_run",
            "fv_get_state", "fv_get_scalars", "fv_last_run_ms", "fv_last_error", "fv_profile",
-           "fv_kernel_stats"]
+           "fv_kernel_stats", "fv_batch_create", "fv_batch_add", "fv_batch_remove", "fv_batch_run",
+           "fv_batch_get_info", "fv_batch_destroy", "fv_batch_last_error"]
+
+
+class FvBatchOpts(ctypes.Structure):
+    """Mirror of ``fv_batch_opts``."""
+    _fields_ = [("autotune", ctypes.c_int), ("tiling", ctypes.c_int), ("max_graph_steps", ctypes.c_int)]
+
+
+class FvBatchInfo(ctypes.Structure):
+    """Mirror of ``fv_batch_info``."""
+    _fields_ = [("members", ctypes.c_int), ("tiling", ctypes.c_int), ("tuned", ctypes.c_int),
+                ("tune_ms", ctypes.c_double * 4), ("device_steps", ctypes.c_long),
+                ("host_steps", ctypes.c_long), ("halts", ctypes.c_long), ("rounds", ctypes.c_long),
+                ("last_run_ms", ctypes.c_double)]
 
 _lib = None
 
@@ -102,6 +116,14 @@ def load_library():
     L.fv_profile.argtypes = [V, ctypes.c_int]
     L.fv_kernel_stats.argtypes = [V, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                   ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    L.fv_batch_create.argtypes = [ctypes.POINTER(V), ctypes.POINTER(FvBatchOpts)]
+    L.fv_batch_add.argtypes = [V, V]
+    L.fv_batch_remove.argtypes = [V, V]
+    L.fv_batch_run.argtypes = [V, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    L.fv_batch_get_info.argtypes = [V, ctypes.POINTER(FvBatchInfo)]
+    L.fv_batch_destroy.argtypes = [V]
+    L.fv_batch_last_error.argtypes = [V]
+    L.fv_batch_last_error.restype = ctypes.c_char_p
     _lib = L
     return L
 
@@ -208,3 +230,62 @@ class Fosite:
         s = FvScalars()
         self._chk(self.L.fv_get_scalars(self._h, ctypes.byref(s)), "fv_get_scalars")
         return {k: getattr(s, k) for k, _ in FvScalars._fields_}
+
+
+class Batch:
+    """Many ``Fosite`` simulations advanced together (``fv_batch_*``): every
+    kernel of a step is one launch over all members, the steps of a round one
+    HIP graph.  Results are bit-identical to running each member alone.
+
+    tiling: 0/1 = 64-wide RHS tiles in row-major / XCD-chunked order, 2/3 =
+    the same with 32-wide tiles; with ``autotune`` the batch times all four
+    on its first device steps and keeps the fastest."""
+
+    def __init__(self, sims=(), autotune=True, tiling=0, max_graph_steps=0):
+        self.L = load_library()
+        o = FvBatchOpts(int(bool(autotune)), int(tiling), int(max_graph_steps))
+        h = ctypes.c_void_p()
+        rc = self.L.fv_batch_create(ctypes.byref(h), ctypes.byref(o))
+        if rc:
+            raise RuntimeError("fv_batch_create failed (%d); see stderr" % rc)
+        self._h = h
+        self.members = []
+        for s in sims:
+            self.add(s)
+
+    def _chk(self, rc, what):
+        if rc:
+            raise RuntimeError("%s failed (%d): %s" % (what, rc, self.L.fv_batch_last_error(self._h).decode()))
+
+    def add(self, sim):
+        self._chk(self.L.fv_batch_add(self._h, sim._h), "fv_batch_add")
+        self.members.append(sim)
+
+    def remove(self, sim):
+        self._chk(self.L.fv_batch_remove(self._h, sim._h), "fv_batch_remove")
+        self.members.remove(sim)
+
+    def run(self, nsteps):
+        """nsteps Steps of every member; returns the steps each member took."""
+        t = (ctypes.c_int * max(1, len(self.members)))()
+        self._chk(self.L.fv_batch_run(self._h, int(nsteps), t), "fv_batch_run")
+        return [t[k] for k in range(len(self.members))]
+
+    def info(self):
+        i = FvBatchInfo()
+        self._chk(self.L.fv_batch_get_info(self._h, ctypes.byref(i)), "fv_batch_get_info")
+        d = {k: getattr(i, k) for k, _ in FvBatchInfo._fields_}
+        d["tune_ms"] = list(i.tune_ms)
+        return d
+
+    def last_run_ms(self):
+        return self.info()["last_run_ms"]
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.L.fv_batch_destroy(self._h)
+            self._h = None
+            self.members = []
+
+    def __del__(self):
+        self.close()

@@ -6,21 +6,44 @@ exactly like the reference; packed FP32 instructions are disabled.
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = [os.path.join(HERE, "csrc", f) for f in ("fv_kernels.hip", "fv_api.cpp", "fv_mesh.cpp")]
+# kernel templates are instantiated per element type and kernel family in
+# separate translation units, compiled in parallel
+SRC = [os.path.join(HERE, "csrc", f) for f in (
+    "fv_inst_f64_single.hip", "fv_inst_f64_multi.hip", "fv_inst_f32_single.hip", "fv_inst_f32_multi.hip",
+    "fv_kernels.hip", "fv_api.cpp", "fv_mesh.cpp")]
 OUT = os.path.join(HERE, "libfosite_fv.so")
+OBJ = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-FLAGS = ["--offload-arch=gfx950", "-O3", "-ffp-contract=off", "-std=c++17", "-fPIC", "-shared",
+FLAGS = ["--offload-arch=gfx950", "-O3", "-ffp-contract=off", "-std=c++17", "-fPIC",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
-def build(verbose=False, out=OUT, defines=()):
-    cmd = [HIPCC] + FLAGS + ["-D" + d for d in defines] + ["-o", out] + SRC
+def _run(cmd, what):
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stderr)
-        raise RuntimeError("hipcc failed building %s" % out)
+        raise RuntimeError("hipcc failed on %s" % what)
+
+
+def build(verbose=False, out=OUT, defines=(), jobs=None):
+    """Compile every source to an object (in parallel), then link the shared
+    library; the result replaces out atomically."""
+    odir = os.path.join(OBJ, os.path.basename(out).replace(".", "_"))
+    os.makedirs(odir, exist_ok=True)
+    defs = ["-D" + d for d in defines]
+    objs = [os.path.join(odir, os.path.basename(f) + ".o") for f in SRC]
+    jobs = jobs or min(len(SRC), max(1, os.cpu_count() or 1))
+    with ThreadPoolExecutor(jobs) as ex:
+        futs = [ex.submit(_run, [HIPCC] + FLAGS + defs + ["-c", "-o", o, f], os.path.basename(f))
+                for f, o in zip(SRC, objs)]
+        for fu in futs:
+            fu.result()
+    tmp = out + ".tmp"
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs, "link")
+    os.replace(tmp, out)
     if verbose:
         print("built", out)
     return out

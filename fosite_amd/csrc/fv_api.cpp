@@ -116,6 +116,7 @@ struct fv_ctx {
   std::vector<GraphEntry> graphs;
   bool fast = true;         // device-controlled batches allowed
   int batch_steps = 0, batch_halts = 0;
+  fv_batch *batch = nullptr;  // batch this context belongs to
   double *part_r = nullptr, *part_c = nullptr;  // per-block partials: CFL/checks, embedded error
   double *d_red = nullptr, *h_red = nullptr;
   int nb_rhs = 0;
@@ -195,7 +196,12 @@ void prof_collect(fv_ctx *ctx) {
   ctx->evused = 0;
 }
 double cells(const fv_ctx *ctx) { return (double)ctx->c.inum * ctx->c.jnum; }
-double word_bytes(const fv_ctx *ctx) { return 8.0 * ctx->V * cells(ctx); }
+size_t elem_bytes(const fv_ctx *ctx) { return ctx->c.dtype == 1 ? sizeof(float) : sizeof(double); }
+double word_bytes(const fv_ctx *ctx) { return (double)elem_bytes(ctx) * ctx->V * cells(ctx); }
+// address of field l of a state buffer (element type by dtype)
+char *field(const fv_ctx *ctx, double *buf, int l) {
+  return reinterpret_cast<char *>(buf) + (size_t)l * ctx->FS * elem_bytes(ctx);
+}
 
 // reduce the per-block partials and fetch {max, flags} to the host
 int reduce(fv_ctx *ctx, const double *part, int n, double *mx, int *flags) {
@@ -471,39 +477,99 @@ bool fast_eligible(const fv_ctx *ctx) {
          ctx->fl_cold == f;
 }
 
-// enqueue one device-controlled step: input state x_in/k0_in -> x_out/k0_out
-// (tb: combined state, rb: remapped state)
-void enqueue_fast_step(fv_ctx *ctx, int xin, int xout, int tb, int rb, int k0in, int k0out) {
+// The launches of one device-controlled step: input state x_in/k0_in ->
+// x_out/k0_out (tb: combined state, rb: remapped state), step control in ctl
+struct FastStep {
+  StageArgs st[MAXK];  // stages 2..s (index m-1); the last one fused with the combination
+  bool remap;
+  RemapArgs rm;
+  StageArgs end;       // end-of-step ComputeRHS: CFL, data checks, new state with ghosts
+};
+void fast_step_args(fv_ctx *ctx, Ctl *ctl, int xin, int xout, int tb, int rb, int k0in, int k0out, FastStep &f) {
   const int steady = ctx->c.fargo == 2 ? 1 : 0;
   for (int m = 1; m < ctx->tab.m; m++) {
     StageArgs a = stage_args(ctx, m, xin, k0in, tb);
-    a.ctl = ctx->d_ctl;
+    a.ctl = ctl;
     a.add_bg = a.sub_bg = steady;
-    launch_stage(ctx, a, m);
+    f.st[m - 1] = a;
   }
   int y = tb;
-  if (ctx->c.fargo == 2) {
-    RemapArgs r{ctx->buf[tb], ctx->buf[rb], 0.0, ctx->d_ctl};
-    long pk = prof_begin(ctx);
-    launch_remap(ctx->g, r, ctx->V, ctx->st);
-    prof_end(ctx, pk, K_REMAP, 2.0 * word_bytes(ctx));
+  f.remap = ctx->c.fargo == 2;
+  if (f.remap) {
+    f.rm = RemapArgs{ctx->buf[tb], ctx->buf[rb], 0.0, ctl};
     y = rb;
   }
   StageArgs a = base_args(ctx);
   a.Y = ctx->buf[y];
   a.nk = 0;
-  a.ctl = ctx->d_ctl;
+  a.ctl = ctl;
   a.kout = ctx->k0[k0out];
   a.yout = ctx->buf[xout];
   a.add_bg = a.sub_bg = steady;
   a.pmin_clamp = ctx->V == 4 ? 1 : 0;
   a.want_cfl = a.want_check = 1;
+  f.end = a;
+}
+
+void enqueue_fast_step(fv_ctx *ctx, int xin, int xout, int tb, int rb, int k0in, int k0out) {
+  FastStep f;
+  fast_step_args(ctx, ctx->d_ctl, xin, xout, tb, rb, k0in, k0out, f);
+  for (int m = 1; m < ctx->tab.m; m++) launch_stage(ctx, f.st[m - 1], m);
+  if (f.remap) {
+    long pk = prof_begin(ctx);
+    launch_remap(ctx->g, f.rm, ctx->V, ctx->st);
+    prof_end(ctx, pk, K_REMAP, 2.0 * word_bytes(ctx));
+  }
   long pk = prof_begin(ctx);
-  launch_rhs(ctx->g, a, ctx->V, ctx->st);
+  launch_rhs(ctx->g, f.end, ctx->V, ctx->st);
   prof_end(ctx, pk, K_RHS_END, 3.0 * word_bytes(ctx));
   pk = prof_begin(ctx);
   launch_control(ctx->d_ctl, ctx->part_c, ctx->part_r, ctx->nb_rhs, ctx->st);
   prof_end(ctx, pk, K_CONTROL, 32.0 * ctx->nb_rhs);
+}
+
+// step-control header of a device-controlled batch of at most nmax steps
+void prime_ctl(const fv_ctx *ctx, Ctl *h, int nmax) {
+  std::memset(h, 0, offsetof(Ctl, err_hist));
+  h->time = ctx->time;
+  h->dt = ctx->dt;
+  h->t_out = ctx->t_out;
+  h->stoptime = ctx->c.stoptime;
+  h->cfl = ctx->c.cfl;
+  h->dtlimit = ctx->dtlimit;
+  h->maxerrold = ctx->maxerrold;
+  h->beta = ctx->c.beta;
+  h->order = ctx->c.order;
+  h->nsteps = nmax;
+}
+
+// replays the host scalars of the h->step steps the device completed, exactly
+// as the host path would have computed them (fosite.f90:367-464,
+// timedisc_base.f90:649-750); the state alternated x0 -> x1 -> x0 ...
+int replay_device_steps(fv_ctx *ctx, const Ctl *h, int x0, int x1, int *halted) {
+  const int k = h->step;
+  for (int s = 0; s < k; s++) {
+    const double dts = h->dt_hist[s];
+    if (dts < ctx->dtmin && ctx->dtcause != DTCAUSE_FILEIO) ctx->dtmin = dts;
+    const double dtnew = adjust_timestep(ctx, h->err_hist[s], dts);
+    const double t1 = ctx->time + dts;
+    ctx->iter++;
+    if (dtnew < ctx->dtlimit) return fail(ctx, "device step control missed a dtlimit break", -4);
+    ctx->dt = t1 - ctx->time;
+    ctx->time = t1;
+    ctx->dtold = dtnew;
+    ctx->dtcause = 0;
+    ctx->cvar = ctx->cold = (s % 2 == 0) ? x1 : x0;
+    ctx->k0cur = 1 - ctx->k0cur;
+  }
+  if (k > 0) {  // CalcTimestep of the last accepted step (before output clipping)
+    ctx->invdt = h->invdt;
+    ctx->dt = calc_timestep(ctx);
+  }
+  *halted = h->reason != H_BATCH;
+  ctx->batch_steps += k;
+  ctx->batch_halts += *halted;
+  return 0;
 }
 
 int get_graph(fv_ctx *ctx, int x0, int x1, int tb, int rb, int k0, hipGraphExec_t *out) {
@@ -536,17 +602,7 @@ int run_batch(fv_ctx *ctx, int nmax, int *done_steps, int *halted) {
   const int rb = pick_free(ctx, x0, x1, tb);
   const int k0 = ctx->k0cur;
   Ctl *h = ctx->h_ctl;
-  std::memset(h, 0, offsetof(Ctl, err_hist));
-  h->time = ctx->time;
-  h->dt = ctx->dt;
-  h->t_out = ctx->t_out;
-  h->stoptime = ctx->c.stoptime;
-  h->cfl = ctx->c.cfl;
-  h->dtlimit = ctx->dtlimit;
-  h->maxerrold = ctx->maxerrold;
-  h->beta = ctx->c.beta;
-  h->order = ctx->c.order;
-  h->nsteps = nmax;
+  prime_ctl(ctx, h, nmax);
   hipGraphExec_t exec = nullptr;
   if (!ctx->prof) {
     int rc = get_graph(ctx, x0, x1, tb, rb, k0, &exec);
@@ -563,31 +619,9 @@ int run_batch(fv_ctx *ctx, int nmax, int *done_steps, int *halted) {
   HIPCHK(hipMemcpyAsync(h, ctx->d_ctl, sizeof(Ctl), hipMemcpyDeviceToHost, ctx->st));
   HIPCHK(hipStreamSynchronize(ctx->st));
   prof_collect(ctx);
-  const int k = h->step;
-  // replay the host scalars of the k accepted steps exactly as the host path
-  // would have computed them (fosite.f90:367-464, timedisc_base.f90:649-750)
-  for (int s = 0; s < k; s++) {
-    const double dts = h->dt_hist[s];
-    if (dts < ctx->dtmin && ctx->dtcause != DTCAUSE_FILEIO) ctx->dtmin = dts;
-    const double dtnew = adjust_timestep(ctx, h->err_hist[s], dts);
-    const double t1 = ctx->time + dts;
-    ctx->iter++;
-    if (dtnew < ctx->dtlimit) return fail(ctx, "device step control missed a dtlimit break", -4);
-    ctx->dt = t1 - ctx->time;
-    ctx->time = t1;
-    ctx->dtold = dtnew;
-    ctx->dtcause = 0;
-    ctx->cvar = ctx->cold = (s % 2 == 0) ? x1 : x0;
-    ctx->k0cur = 1 - ctx->k0cur;
-  }
-  if (k > 0) {  // CalcTimestep of the last accepted step (before output clipping)
-    ctx->invdt = h->invdt;
-    ctx->dt = calc_timestep(ctx);
-  }
-  *halted = h->reason != H_BATCH;
-  ctx->batch_steps += k;
-  ctx->batch_halts += *halted;
-  *done_steps = k;
+  int rc = replay_device_steps(ctx, h, x0, x1, halted);
+  if (rc) return rc;
+  *done_steps = h->step;
   return 0;
 }
 
@@ -611,7 +645,7 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
   if (c.geometry != CARTESIAN && c.geometry != LOGCYLINDRICAL) return bail("geometry must be CARTESIAN (1) or LOGCYLINDRICAL (31)");
   if (c.inum < 4 || c.jnum < 4) return bail("inum and jnum must be >= 4 (2-D meshes only)");
   if (c.recvars != 2) return bail("only CONSERVATIVE reconstruction (variables = 2) is supported");
-  if (c.dtype != 0) return bail("only dtype 0 (FP64) is supported");
+  if (c.dtype != 0 && c.dtype != 1) return bail("dtype must be 0 (FP64) or 1 (FP32 precision form)");
   if (c.fargo != 0 && c.fargo != 2) return bail("fargo must be 0 or 2");
   if (c.fargo == 2 && (c.physics != EULER_ISOTHERM || c.geometry != LOGCYLINDRICAL))
     return bail("FARGO requires isothermal physics on a logcylindrical mesh");
@@ -622,8 +656,7 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
   for (int d = 0; d < 4; d++) {
     const int b = c.bc[d];
     if (b != NO_GRADIENTS && b != PERIODIC && b != ABSORBING) return bail("boundary must be NO_GRADIENTS, PERIODIC or ABSORBING");
-    if (b == ABSORBING && (d >= 2 || c.physics != EULER_ISOTHERM))
-      return bail("ABSORBING boundaries are supported in x for isothermal physics");
+    if (b == ABSORBING && d >= 2) return bail("ABSORBING boundaries are supported in x");
   }
   // boundary_generic.f90: a periodic boundary needs a periodic opposite
   if ((c.bc[0] == PERIODIC) != (c.bc[1] == PERIODIC) || (c.bc[2] == PERIODIC) != (c.bc[3] == PERIODIC))
@@ -654,7 +687,7 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
   }
   if (const char *e = std::getenv("FV_HOST_CONTROL")) ctx->fast = std::atoi(e) == 0;
   // device resources
-  const size_t sbytes = sizeof(double) * ctx->V * ctx->FS;
+  const size_t sbytes = elem_bytes(ctx) * ctx->V * ctx->FS;
   auto dalloc = [&](double **p, size_t b) { return hipMalloc((void **)p, b) == hipSuccess; };
   bool ok = hipStreamCreate(&ctx->st) == hipSuccess && hipEventCreate(&ctx->e0) == hipSuccess &&
             hipEventCreate(&ctx->e1) == hipSuccess;
@@ -670,11 +703,15 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
   g.limiter = c.limiter; g.theta = c.theta;
   for (int d = 0; d < 4; d++) g.bc[d] = c.bc[d];
   g.fargo = c.fargo == 2; g.gravity = c.gravity != 0; g.curvilinear = c.geometry != CARTESIAN;
+  g.dtype = c.dtype;
   g.tile_x = 64;
   if (const char *e = std::getenv("FV_RHS_TILE")) g.tile_x = std::atoi(e) == 32 ? 32 : 64;
   ctx->nb_rhs = rhs_blocks(g);
-  ok = ok && dalloc(&ctx->part_r, sizeof(double) * 2 * ctx->nb_rhs) &&
-       dalloc(&ctx->part_c, sizeof(double) * 2 * ctx->nb_rhs) && dalloc(&ctx->d_red, sizeof(double) * 2);
+  Geom g32 = g;  // partials sized for the finest tiling, which batches may choose
+  g32.tile_x = 32;
+  const int nb_part = std::max(ctx->nb_rhs, rhs_blocks(g32));
+  ok = ok && dalloc(&ctx->part_r, sizeof(double) * 2 * nb_part) &&
+       dalloc(&ctx->part_c, sizeof(double) * 2 * nb_part) && dalloc(&ctx->d_red, sizeof(double) * 2);
   ok = ok && hipMalloc((void **)&ctx->d_ctl, sizeof(Ctl)) == hipSuccess && hipMemset(ctx->d_ctl, 0, sizeof(Ctl)) == hipSuccess;
   ok = ok && hipHostMalloc((void **)&ctx->h_red, sizeof(double) * 2) == hipSuccess;
   ok = ok && hipHostMalloc((void **)&ctx->h_ctl, sizeof(Ctl)) == hipSuccess;
@@ -699,6 +736,7 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
 
 extern "C" int fv_destroy(fv_ctx *ctx) {
   if (!ctx) return 0;
+  if (ctx->batch) (void)fv_batch_remove(ctx->batch, ctx);
   if (ctx->st) (void)hipStreamSynchronize(ctx->st);
   for (auto &e : ctx->graphs) (void)hipGraphExecDestroy(e.exec);
   for (auto &b : ctx->buf) if (b) (void)hipFree(b);
@@ -764,9 +802,16 @@ extern "C" int fv_set_pvar(fv_ctx *ctx, const double *pvar) {
     if (V == 4) cv[3 * N + k] = pvar[3 * N + k] / (g - 1.) + 0.5 * rho * (u * u + v * v);
   }
   ctx->cvar = ctx->cold = 0;
+  const size_t es = elem_bytes(ctx);
+  std::vector<float> cf;
+  const void *src = cv.data();
+  if (ctx->c.dtype == 1) {  // precision form: conserved variables from FP64, stored rounded
+    cf.assign(cv.begin(), cv.end());
+    src = cf.data();
+  }
   for (int l = 0; l < V; l++)
-    HIPCHK(hipMemcpy2D(ctx->buf[0] + l * ctx->FS, sizeof(double) * ctx->P, cv.data() + l * N,
-                       sizeof(double) * ctx->NI, sizeof(double) * ctx->NI, ctx->NJ, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy2D(field(ctx, ctx->buf[0], l), es * ctx->P, static_cast<const char *>(src) + l * N * es,
+                       es * ctx->NI, es * ctx->NI, ctx->NJ, hipMemcpyHostToDevice));
   ctx->fl_pvar = ctx->fl_cvar = 0;
   ctx->have_state = true;
   return 0;
@@ -851,10 +896,14 @@ extern "C" int fv_get_state(fv_ctx *ctx, double *cvar, double *pvar) {
   const int V = ctx->V;
   const size_t N = (size_t)ctx->NI * ctx->NJ;
   std::vector<double> cv(V * N);
+  std::vector<float> cf(ctx->c.dtype == 1 ? V * N : 0);
+  const size_t es = elem_bytes(ctx);
+  char *dst = ctx->c.dtype == 1 ? reinterpret_cast<char *>(cf.data()) : reinterpret_cast<char *>(cv.data());
   HIPCHK(hipStreamSynchronize(ctx->st));
   for (int l = 0; l < V; l++)
-    HIPCHK(hipMemcpy2D(cv.data() + l * N, sizeof(double) * ctx->NI, ctx->buf[ctx->cvar] + l * ctx->FS,
-                       sizeof(double) * ctx->P, sizeof(double) * ctx->NI, ctx->NJ, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy2D(dst + l * N * es, es * ctx->NI, field(ctx, ctx->buf[ctx->cvar], l), es * ctx->P,
+                       es * ctx->NI, ctx->NJ, hipMemcpyDeviceToHost));
+  if (ctx->c.dtype == 1) std::copy(cf.begin(), cf.end(), cv.begin());
   if (cvar) std::memcpy(cvar, cv.data(), sizeof(double) * V * N);
   if (pvar) {
     const double g = ctx->c.gamma;
@@ -902,3 +951,440 @@ extern "C" int fv_kernel_stats(const fv_ctx *ctx, int kernel, int *launches, dou
 }
 
 extern "C" const char *fv_last_error(const fv_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+// ================================================================ batches
+// Many contexts advanced together.  A round of device-controlled steps
+// launches every kernel of a step once for all members: the per-member
+// arguments of each launch sit in tables in device memory (rewritten before
+// every round, so a captured graph stays valid while members' buffer roles
+// change), and block b of a launch runs the tile tiles[b] of member
+// tile_disk(tiles[b]).  Each member has its own Ctl; a member that reaches
+// its step count or a step the device cannot decide halts on its own, its
+// blocks then return at once, and the host replays or redoes its steps with
+// the single-context code above.
+
+namespace {
+constexpr int NCAND = 4;     // RHS tilings: 64 row-major, 64 XCD-chunked, 32 row-major, 32 XCD-chunked
+constexpr int XCDS = 8;      // MI355X: workgroup b of a launch is dispatched to XCD b mod 8
+constexpr int TUNE_REPS = 2;
+int cand_tx(int c) { return c < 2 ? 64 : 32; }
+}  // namespace
+
+struct fv_batch {
+  std::string err;
+  fv_batch_opts opt{};
+  std::vector<fv_ctx *> sims;
+  int V = 0, dtype = 0, m = 0, limiter = 0;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr, t0 = nullptr, t1 = nullptr;
+  int cap = 0;                       // members the tables hold
+  Ctl *d_ctl = nullptr, *h_ctl = nullptr;
+  RhsEntry *d_rhs = nullptr, *h_rhs = nullptr;          // [2 parities][m launches][cap]
+  RemapEntry *d_remap = nullptr, *h_remap = nullptr;    // [2][cap]
+  ControlEntry *d_cent = nullptr, *h_cent = nullptr;    // [NCAND][cap]
+  unsigned *d_tiles = nullptr;
+  size_t tiles_cap = 0;
+  int cand_off[NCAND] = {}, cand_n[NCAND] = {}, remap_off = 0, remap_n = 0;
+  bool dirty = true;
+  int cand = 0, tuned = 0, trial = 0;
+  double tune_ms[NCAND] = {};
+  struct G { int K, cand; hipGraphExec_t exec; };
+  std::vector<G> graphs;
+  struct Roles { int x0, x1, tb, rb, k0; };
+  std::vector<Roles> roles;
+  long dev_steps = 0, host_steps = 0, halts = 0, rounds = 0;
+  double last_ms = 0;
+};
+
+#define BCHK(x)                                                                \
+  do {                                                                         \
+    hipError_t e_ = (x);                                                       \
+    if (e_ != hipSuccess) {                                                    \
+      b->err = std::string(#x) + ": " + hipGetErrorString(e_);                 \
+      return -3;                                                               \
+    }                                                                          \
+  } while (0)
+
+namespace {
+
+int bfail(fv_batch *b, const std::string &m, int code = -1) {
+  b->err = m;
+  return code;
+}
+
+void drop_graphs(fv_batch *b) {
+  for (auto &g : b->graphs) (void)hipGraphExecDestroy(g.exec);
+  b->graphs.clear();
+}
+
+void free_tables(fv_batch *b) {
+  if (b->d_ctl) (void)hipFree(b->d_ctl);
+  if (b->d_rhs) (void)hipFree(b->d_rhs);
+  if (b->d_remap) (void)hipFree(b->d_remap);
+  if (b->d_cent) (void)hipFree(b->d_cent);
+  if (b->h_ctl) (void)hipHostFree(b->h_ctl);
+  if (b->h_rhs) (void)hipHostFree(b->h_rhs);
+  if (b->h_remap) (void)hipHostFree(b->h_remap);
+  if (b->h_cent) (void)hipHostFree(b->h_cent);
+  b->d_ctl = b->h_ctl = nullptr;
+  b->d_rhs = b->h_rhs = nullptr;
+  b->d_remap = b->h_remap = nullptr;
+  b->d_cent = b->h_cent = nullptr;
+  b->cap = 0;
+}
+
+// k_rhs tile list of tiling c over all members
+std::vector<unsigned> rhs_tiles(const fv_batch *b, int c) {
+  const int tx = cand_tx(c);
+  std::vector<unsigned> lin;
+  for (size_t d = 0; d < b->sims.size(); d++) {
+    const fv_ctx *x = b->sims[d];
+    const int nbx = (x->c.inum + tx - 1) / tx, nby = (x->c.jnum + TY_TILE - 1) / TY_TILE;
+    for (int by = 0; by < nby; by++)
+      for (int bx = 0; bx < nbx; bx++) lin.push_back(tile_code((int)d, bx, by));
+  }
+  if (!(c & 1)) return lin;
+  // XCD-chunked: tile t goes to XCD t / Q, at that XCD's slot t % Q
+  const size_t N = lin.size(), Q = (N + XCDS - 1) / XCDS;
+  std::vector<unsigned> out(Q * XCDS, NO_TILE);
+  for (size_t t = 0; t < N; t++) out[(t % Q) * XCDS + t / Q] = lin[t];
+  return out;
+}
+
+// tables and tile lists for the current members; graphs are recaptured
+int rebuild(fv_batch *b) {
+  drop_graphs(b);
+  const int n = (int)b->sims.size();
+  if (n > b->cap) {
+    free_tables(b);
+    const int cap = std::max(n, 8);
+    const int m = b->m;
+    BCHK(hipMalloc((void **)&b->d_ctl, sizeof(Ctl) * cap));
+    BCHK(hipMalloc((void **)&b->d_rhs, sizeof(RhsEntry) * 2 * m * cap));
+    BCHK(hipMalloc((void **)&b->d_remap, sizeof(RemapEntry) * 2 * cap));
+    BCHK(hipMalloc((void **)&b->d_cent, sizeof(ControlEntry) * NCAND * cap));
+    BCHK(hipHostMalloc((void **)&b->h_ctl, sizeof(Ctl) * cap));
+    BCHK(hipHostMalloc((void **)&b->h_rhs, sizeof(RhsEntry) * 2 * m * cap));
+    BCHK(hipHostMalloc((void **)&b->h_remap, sizeof(RemapEntry) * 2 * cap));
+    BCHK(hipHostMalloc((void **)&b->h_cent, sizeof(ControlEntry) * NCAND * cap));
+    BCHK(hipMemset(b->d_ctl, 0, sizeof(Ctl) * cap));
+    b->cap = cap;
+  }
+  std::vector<unsigned> all;
+  for (int c = 0; c < NCAND; c++) {
+    std::vector<unsigned> t = rhs_tiles(b, c);
+    b->cand_off[c] = (int)all.size();
+    b->cand_n[c] = (int)t.size();
+    all.insert(all.end(), t.begin(), t.end());
+  }
+  b->remap_off = (int)all.size();
+  for (int d = 0; d < n; d++) {
+    const fv_ctx *x = b->sims[d];
+    if (x->c.fargo != 2) continue;
+    const int nbx = (x->c.inum + 63) / 64, nby = (x->c.jnum + 3) / 4;
+    for (int by = 0; by < nby; by++)
+      for (int bx = 0; bx < nbx; bx++) all.push_back(tile_code(d, bx, by));
+  }
+  b->remap_n = (int)all.size() - b->remap_off;
+  if (all.size() > b->tiles_cap) {
+    if (b->d_tiles) (void)hipFree(b->d_tiles);
+    b->d_tiles = nullptr;
+    b->tiles_cap = 0;
+    BCHK(hipMalloc((void **)&b->d_tiles, sizeof(unsigned) * all.size()));
+    b->tiles_cap = all.size();
+  }
+  if (!all.empty()) BCHK(hipMemcpy(b->d_tiles, all.data(), sizeof(unsigned) * all.size(), hipMemcpyHostToDevice));
+  b->dirty = false;
+  return 0;
+}
+
+// the launches of one device step of parity p under tiling c
+void enqueue_batch_step(fv_batch *b, int p, int c) {
+  const int m = b->m, cap = b->cap, n = (int)b->sims.size();
+  const unsigned *tl = b->d_tiles + b->cand_off[c];
+  for (int q = 0; q < m - 1; q++)
+    launch_rhs_multi(b->d_rhs + (size_t)(p * m + q) * cap, tl, b->cand_n[c], b->V, b->dtype, b->limiter,
+                     cand_tx(c), b->st);
+  if (b->remap_n)
+    launch_remap_multi(b->d_remap + (size_t)p * cap, b->d_tiles + b->remap_off, b->remap_n, b->V, b->dtype, b->st);
+  launch_rhs_multi(b->d_rhs + (size_t)(p * m + m - 1) * cap, tl, b->cand_n[c], b->V, b->dtype, b->limiter,
+                   cand_tx(c), b->st);
+  launch_control_multi(b->d_cent + (size_t)c * cap, n, b->st);
+}
+
+int batch_graph(fv_batch *b, int K, int c, hipGraphExec_t *out) {
+  for (auto &g : b->graphs)
+    if (g.K == K && g.cand == c) {
+      *out = g.exec;
+      return 0;
+    }
+  hipGraph_t graph;
+  BCHK(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal));
+  for (int s = 0; s < K; s++) enqueue_batch_step(b, s % 2, c);
+  BCHK(hipStreamEndCapture(b->st, &graph));
+  hipGraphExec_t exec;
+  hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  BCHK(e);
+  b->graphs.push_back({K, c, exec});
+  *out = exec;
+  return 0;
+}
+
+// one round: want[d] device steps of member d (0 = not in this round), K =
+// steps launched (>= every want), tiling c; timed rounds are issued eagerly
+// between events and return their device time in *ms.  done[d] = steps the
+// device completed, halted[d] = 1 if member d stopped on a step the host must redo.
+int batch_round(fv_batch *b, const std::vector<int> &want, int K, int c, bool timed, double *ms,
+                std::vector<int> &done, std::vector<int> &halted) {
+  const int n = (int)b->sims.size(), m = b->m, cap = b->cap;
+  b->roles.resize(n);
+  for (int d = 0; d < n; d++) {
+    fv_ctx *x = b->sims[d];
+    Ctl *h = &b->h_ctl[d];
+    Ctl *dc = b->d_ctl + d;
+    if (want[d] > 0) {
+      prime_ctl(x, h, want[d]);
+    } else {
+      std::memset(h, 0, offsetof(Ctl, err_hist));
+      h->halt = 1;
+    }
+    fv_batch::Roles &r = b->roles[d];
+    r.x0 = x->cvar;
+    r.x1 = pick_free(x, r.x0, -1, -1);
+    r.tb = pick_free(x, r.x0, r.x1, -1);
+    r.rb = pick_free(x, r.x0, r.x1, r.tb);
+    r.k0 = x->k0cur;
+    for (int p = 0; p < 2; p++) {
+      FastStep f;
+      fast_step_args(x, dc, p ? r.x1 : r.x0, p ? r.x0 : r.x1, r.tb, r.rb, p ? 1 - r.k0 : r.k0,
+                     p ? r.k0 : 1 - r.k0, f);
+      for (int q = 0; q < m - 1; q++) b->h_rhs[(size_t)(p * m + q) * cap + d] = RhsEntry{x->g, f.st[q]};
+      b->h_rhs[(size_t)(p * m + m - 1) * cap + d] = RhsEntry{x->g, f.end};
+      RemapEntry re;
+      std::memset(&re, 0, sizeof(re));
+      re.g = x->g;
+      re.a = f.remap ? f.rm : RemapArgs{nullptr, nullptr, 0.0, dc};
+      b->h_remap[(size_t)p * cap + d] = re;
+    }
+    for (int cc = 0; cc < NCAND; cc++) {
+      const int tx = cand_tx(cc);
+      const int nb = ((x->c.inum + tx - 1) / tx) * ((x->c.jnum + TY_TILE - 1) / TY_TILE);
+      b->h_cent[(size_t)cc * cap + d] = ControlEntry{dc, x->part_c, x->part_r, nb};
+    }
+  }
+  BCHK(hipMemcpy2DAsync(b->d_ctl, sizeof(Ctl), b->h_ctl, sizeof(Ctl), offsetof(Ctl, err_hist), n,
+                        hipMemcpyHostToDevice, b->st));
+  BCHK(hipMemcpyAsync(b->d_rhs, b->h_rhs, sizeof(RhsEntry) * 2 * m * cap, hipMemcpyHostToDevice, b->st));
+  BCHK(hipMemcpyAsync(b->d_remap, b->h_remap, sizeof(RemapEntry) * 2 * cap, hipMemcpyHostToDevice, b->st));
+  BCHK(hipMemcpyAsync(b->d_cent, b->h_cent, sizeof(ControlEntry) * NCAND * cap, hipMemcpyHostToDevice, b->st));
+  if (timed) {
+    BCHK(hipEventRecord(b->t0, b->st));
+    for (int s = 0; s < K; s++) enqueue_batch_step(b, s % 2, c);
+    BCHK(hipGetLastError());
+    BCHK(hipEventRecord(b->t1, b->st));
+  } else {
+    hipGraphExec_t exec;
+    int rc = batch_graph(b, K, c, &exec);
+    if (rc) return rc;
+    BCHK(hipGraphLaunch(exec, b->st));
+  }
+  BCHK(hipMemcpyAsync(b->h_ctl, b->d_ctl, sizeof(Ctl) * n, hipMemcpyDeviceToHost, b->st));
+  BCHK(hipStreamSynchronize(b->st));
+  b->rounds++;
+  if (timed) {
+    float f = 0;
+    BCHK(hipEventElapsedTime(&f, b->t0, b->t1));
+    *ms = f;
+  }
+  done.assign(n, 0);
+  halted.assign(n, 0);
+  for (int d = 0; d < n; d++) {
+    if (want[d] <= 0) continue;
+    fv_ctx *x = b->sims[d];
+    int hl = 0;
+    int rc = replay_device_steps(x, &b->h_ctl[d], b->roles[d].x0, b->roles[d].x1, &hl);
+    if (rc) return bfail(b, "member " + std::to_string(d) + ": " + x->err, rc);
+    done[d] = b->h_ctl[d].step;
+    halted[d] = hl;
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int fv_batch_create(fv_batch **out, const fv_batch_opts *opts) {
+  if (!out) return -1;
+  *out = nullptr;
+  fv_batch *b = new (std::nothrow) fv_batch();
+  if (!b) return -1;
+  if (opts) b->opt = *opts;
+  if (b->opt.tiling < 0 || b->opt.tiling >= NCAND || b->opt.max_graph_steps < 0 || b->opt.max_graph_steps > CTL_HIST) {
+    delete b;
+    std::fprintf(stderr, "fv_batch_create: tiling must be 0..3 and max_graph_steps 0..%d\n", CTL_HIST);
+    return -1;
+  }
+  if (b->opt.max_graph_steps == 0) b->opt.max_graph_steps = CTL_HIST;
+  b->cand = b->opt.tiling;
+  if (hipStreamCreate(&b->st) != hipSuccess || hipEventCreate(&b->e0) != hipSuccess ||
+      hipEventCreate(&b->e1) != hipSuccess || hipEventCreate(&b->t0) != hipSuccess ||
+      hipEventCreate(&b->t1) != hipSuccess) {
+    fv_batch_destroy(b);
+    std::fprintf(stderr, "fv_batch_create: stream or event creation failed\n");
+    return -3;
+  }
+  *out = b;
+  return 0;
+}
+
+extern "C" int fv_batch_add(fv_batch *b, fv_ctx *x) {
+  if (!b || !x) return -1;
+  if (x->batch) return bfail(b, "context already belongs to a batch");
+  if ((int)b->sims.size() >= MAX_BATCH) return bfail(b, "batch is full (" + std::to_string(MAX_BATCH) + " members)");
+  const int nbx = (x->c.inum + 31) / 32, nby = (x->c.jnum + TY_TILE - 1) / TY_TILE, rby = (x->c.jnum + 3) / 4;
+  if (nbx >= (1 << TILE_BX_BITS) || nby >= (1 << TILE_BY_BITS) || rby >= (1 << TILE_BY_BITS))
+    return bfail(b, "mesh too large for a batch tile code");
+  if (b->sims.empty()) {
+    b->V = x->V;
+    b->dtype = x->c.dtype;
+    b->m = x->tab.m;
+    b->limiter = x->c.limiter;
+  } else {
+    if (x->V != b->V || x->c.dtype != b->dtype || x->tab.m != b->m ||
+        x->c.method != b->sims[0]->c.method || x->c.order != b->sims[0]->c.order)
+      return bfail(b, "batch members must agree in physics, dtype and time integrator");
+    if (x->c.limiter != b->limiter) b->limiter = 0;  // run-time limiter switch
+  }
+  b->sims.push_back(x);
+  x->batch = b;
+  b->dirty = true;
+  return 0;
+}
+
+extern "C" int fv_batch_remove(fv_batch *b, fv_ctx *x) {
+  if (!b || !x) return -1;
+  auto it = std::find(b->sims.begin(), b->sims.end(), x);
+  if (it == b->sims.end()) return bfail(b, "context is not a member of this batch");
+  (void)hipStreamSynchronize(b->st);
+  b->sims.erase(it);
+  x->batch = nullptr;
+  b->dirty = true;
+  if (!b->sims.empty()) {
+    b->limiter = b->sims[0]->c.limiter;
+    for (auto *y : b->sims)
+      if (y->c.limiter != b->limiter) b->limiter = 0;
+  }
+  return 0;
+}
+
+extern "C" int fv_batch_run(fv_batch *b, int nsteps, int *taken) {
+  if (!b || nsteps < 0) return -1;
+  const int n = (int)b->sims.size();
+  std::vector<int> rem(n, nsteps), tk(n, 0), fin(n, 0), want(n), done, halted;
+  int rc = 0;
+  if (n == 0) {
+    b->last_ms = 0;
+    return 0;
+  }
+  if (b->dirty && (rc = rebuild(b))) return rc;
+  BCHK(hipEventRecord(b->e0, b->st));
+  auto host_step = [&](int d) -> int {  // one Step on the host path
+    int dn = 0;
+    const int r = fv_step(b->sims[d], &dn);
+    if (r) return bfail(b, "member " + std::to_string(d) + ": " + b->sims[d]->err, r);
+    if (dn) {
+      fin[d] = 1;
+    } else {
+      rem[d]--;
+      tk[d]++;
+      b->host_steps++;
+    }
+    return 0;
+  };
+  for (;;) {
+    // members the device path cannot take yet step on the host path
+    for (int d = 0; d < n; d++)
+      while (rem[d] > 0 && !fin[d] && !fast_eligible(b->sims[d]))
+        if ((rc = host_step(d))) return rc;
+    int kmax = 0, kmin = CTL_HIST;
+    for (int d = 0; d < n; d++) {
+      want[d] = (rem[d] > 0 && !fin[d]) ? std::min(rem[d], b->opt.max_graph_steps) : 0;
+      if (want[d] > 0) {
+        kmax = std::max(kmax, want[d]);
+        kmin = std::min(kmin, want[d]);
+        BCHK(hipStreamSynchronize(b->sims[d]->st));  // host-path work on the member's stream is done
+      }
+    }
+    if (kmax == 0) break;
+    const bool tuning = b->opt.autotune && !b->tuned && kmin >= 2;
+    int K = kmax, c = b->cand;
+    double ms = 0;
+    if (tuning) {
+      K = 2;
+      c = b->trial % NCAND;
+      for (int d = 0; d < n; d++) want[d] = std::min(want[d], 2);
+    }
+    if ((rc = batch_round(b, want, K, c, tuning, &ms, done, halted))) return rc;
+    if (tuning) {
+      const double per = ms / K;
+      if (b->tune_ms[c] == 0 || per < b->tune_ms[c]) b->tune_ms[c] = per;
+      if (++b->trial == NCAND * TUNE_REPS) {
+        int best = 0;
+        for (int q = 1; q < NCAND; q++)
+          if (b->tune_ms[q] < b->tune_ms[best]) best = q;
+        b->cand = best;
+        b->tuned = 1;
+      }
+    }
+    for (int d = 0; d < n; d++) {
+      if (want[d] <= 0) continue;
+      rem[d] -= done[d];
+      tk[d] += done[d];
+      b->dev_steps += done[d];
+      if (halted[d]) {
+        b->halts++;
+        if (rem[d] > 0 && (rc = host_step(d))) return rc;  // the step the device left
+      }
+    }
+  }
+  BCHK(hipEventRecord(b->e1, b->st));
+  BCHK(hipEventSynchronize(b->e1));
+  float ms = 0;
+  BCHK(hipEventElapsedTime(&ms, b->e0, b->e1));
+  b->last_ms = ms;
+  if (taken)
+    for (int d = 0; d < n; d++) taken[d] = tk[d];
+  return 0;
+}
+
+extern "C" int fv_batch_get_info(const fv_batch *b, fv_batch_info *o) {
+  if (!b || !o) return -1;
+  o->members = (int)b->sims.size();
+  o->tiling = b->cand;
+  o->tuned = b->tuned;
+  for (int c = 0; c < NCAND; c++) o->tune_ms[c] = b->tune_ms[c];
+  o->device_steps = b->dev_steps;
+  o->host_steps = b->host_steps;
+  o->halts = b->halts;
+  o->rounds = b->rounds;
+  o->last_run_ms = b->last_ms;
+  return 0;
+}
+
+extern "C" int fv_batch_destroy(fv_batch *b) {
+  if (!b) return 0;
+  if (b->st) (void)hipStreamSynchronize(b->st);
+  for (auto *x : b->sims) x->batch = nullptr;
+  drop_graphs(b);
+  free_tables(b);
+  if (b->d_tiles) (void)hipFree(b->d_tiles);
+  if (b->e0) (void)hipEventDestroy(b->e0);
+  if (b->e1) (void)hipEventDestroy(b->e1);
+  if (b->t0) (void)hipEventDestroy(b->t0);
+  if (b->t1) (void)hipEventDestroy(b->t1);
+  if (b->st) (void)hipStreamDestroy(b->st);
+  delete b;
+  return 0;
+}
+
+extern "C" const char *fv_batch_last_error(const fv_batch *b) { return b ? b->err.c_str() : "null batch"; }

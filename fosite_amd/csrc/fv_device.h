@@ -35,6 +35,7 @@ struct Geom {
   int bc[4];
   int fargo, gravity, curvilinear;
   int tile_x;           // k_rhs tile width in cells: 64 or 32
+  int dtype;            // stored state: 0 FP64 (plain drop-in), 1 FP32 (precision form)
   const double *ring;   // [R_COUNT][NI]
   const double *col;    // [C_COUNT][NJ]
 };
@@ -60,44 +61,85 @@ struct Ctl {
 // the final combination and the embedded error estimate
 // (timedisc_rkfehlberg.f90:199-266, timedisc_base.f90:1012-1053) in place of
 // storing the stage's rhs.
-struct StageArgs {
-  const double *Y;
-  const double *k[MAXK];
+template <typename T>
+struct StageArgsT {
+  const T *Y;
+  const T *k[MAXK];
   double a[MAXK];       // Butcher coefficients of this stage
   int nk;
   double dt;            // time step (host-controlled)
   Ctl *ctl;             // device-controlled: dt and halting come from here
-  double *kout;         // rhs (interior cells); unused with ynew
-  double *yout;         // transformed state incl. ghost cells, or null
+  T *kout;              // rhs (interior cells); unused with ynew
+  T *yout;              // transformed state incl. ghost cells, or null
   int add_bg, sub_bg;   // FARGO frame change at entry / before the fluxes
   int pmin_clamp;       // Euler CHECK_PMIN round trip (timedisc_base.f90:1150-1164)
   int want_cfl, want_check;
   double *part;         // per-block partials: [2*b] max 1/dt, [2*b+1] check flags
   // fused final combination (last stage only)
-  double *ynew;
+  T *ynew;
   double bh[MAXK], bl[MAXK];
   double tol_rel, tol_abs[4];
   double *part_c;       // per-block partials: [2*b] max error, [2*b+1] = 0
 };
+using StageArgs = StageArgsT<double>;
 
 // FargoAdvectionY on the combined state (timedisc_base.f90:1935-2069):
 // comoving-frame CenterBoundary for the y-ghost rows, then per ring the
 // integer shift and the minmod-limited fractional transport.
-struct RemapArgs {
-  const double *Y;
-  double *out;          // remapped interior cells
+template <typename T>
+struct RemapArgsT {
+  const T *Y;
+  T *out;               // remapped interior cells
   double dt_step;       // this%dt (host-controlled)
   Ctl *ctl;             // device-controlled: dt_step = (time+dt)-time, halting
 };
+using RemapArgs = RemapArgsT<double>;
 
 enum Flag { F_INVALID = 1, F_RHOMIN = 2, F_PMIN = 4 };
 
-// kernel launchers (fv_kernels.hip)
+// Batches of independent meshes: one launch per kernel of the step covers
+// every mesh.  Each launch reads its per-mesh arguments from a table in device
+// memory and maps block b to a tile through a tile list of 32-bit codes
+// (mesh index, tile column, tile row).
+struct RhsEntry {
+  Geom g;
+  StageArgs a;
+};
+struct RemapEntry {
+  Geom g;
+  RemapArgs a;
+};
+struct ControlEntry {
+  Ctl *ctl;
+  const double *part_c, *part_r;
+  int nb;
+};
+constexpr int TILE_DISK_BITS = 9, TILE_BX_BITS = 11, TILE_BY_BITS = 12;
+constexpr int MAX_BATCH = 1 << TILE_DISK_BITS;
+constexpr unsigned NO_TILE = 0xFFFFFFFFu;  // empty slot of a tile list
+__host__ __device__ inline unsigned tile_code(int disk, int bx, int by) {
+  return (unsigned)disk | ((unsigned)bx << TILE_DISK_BITS) | ((unsigned)by << (TILE_DISK_BITS + TILE_BX_BITS));
+}
+__host__ __device__ inline int tile_disk(unsigned e) { return (int)(e & ((1u << TILE_DISK_BITS) - 1)); }
+__host__ __device__ inline int tile_bx(unsigned e) { return (int)((e >> TILE_DISK_BITS) & ((1u << TILE_BX_BITS) - 1)); }
+__host__ __device__ inline int tile_by(unsigned e) { return (int)(e >> (TILE_DISK_BITS + TILE_BX_BITS)); }
+
+// kernel launchers (fv_kernels.hip).  The host passes state buffers through
+// the FP64-typed argument structs; for g.dtype == 1 they hold FP32 arrays.
 int rhs_blocks(const Geom &g);
 void launch_rhs(const Geom &g, const StageArgs &a, int V, hipStream_t s);
 void launch_remap(const Geom &g, const RemapArgs &a, int V, hipStream_t s);
-void launch_frame(const Geom &g, const double *src, double *dst, int sign, int V, hipStream_t s);
+void launch_frame(const Geom &g, const void *src, void *dst, int sign, int V, hipStream_t s);
 void launch_reduce(const double *part, int n, double *out, hipStream_t s);
 void launch_control(Ctl *ctl, const double *part_c, const double *part_r, int nb, hipStream_t s);
+// batch launchers: ntiles blocks over the tile list; TX = tile width of the list
+#ifndef FV_RHS_ROWS
+#define FV_RHS_ROWS 8
+#endif
+constexpr int TY_TILE = FV_RHS_ROWS;  // k_rhs tile rows
+void launch_rhs_multi(const RhsEntry *tab, const unsigned *tiles, int ntiles, int V, int dtype, int limiter,
+                      int tx, hipStream_t s);
+void launch_remap_multi(const RemapEntry *tab, const unsigned *tiles, int ntiles, int V, int dtype, hipStream_t s);
+void launch_control_multi(const ControlEntry *tab, int n, hipStream_t s);
 
 }  // namespace fv

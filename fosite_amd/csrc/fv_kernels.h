@@ -1,0 +1,1083 @@
+// fv_kernels.h -- CDNA4 (gfx950) kernels of fosite's unsplit finite-volume
+// step (templates; instantiated by fv_inst_*.hip, dispatched by fv_kernels.hip).  Every kernel is a template on the real type T of the stored state:
+//
+//   T = double  the plain drop-in.  Every floating-point expression keeps the
+//               reference's operation order and the file is compiled with
+//               -ffp-contract=off, so IEEE +,-,*,/ and sqrt give the
+//               reference's bits.
+//   T = float   the precision form: reconstruction, fluxes, sources and the
+//               stage arithmetic in FP32 (fast division and square root),
+//               the RK stage sums and the final combination with compensated
+//               (two-sum / two-product) accumulation, the embedded-error norm
+//               in FP64; CFL partials are reduced in FP64 by the host path or
+//               k_control.
+//
+// Citations are file:line in the fosite source tree.
+//
+// k_rhs (one launch per right-hand-side evaluation) is fully fused:
+//   * the input state of every loaded cell is formed on the fly: the RK stage
+//     state Y - sum a*dt*k (timedisc_rkfehlberg.f90:277-302);
+//   * ghost cells are evaluated on the fly from interior cells with the
+//     boundary rules (boundary_generic.f90:287-517, boundary_nogradients.f90,
+//     boundary_periodic.f90, boundary_absorbing.f90), so no boundary pass and
+//     no intermediate state array exist;
+//   * FARGO frame round trip, Euler PMIN round trip, linear reconstruction,
+//     face conversions, wave speeds, physical and KT fluxes, geometric,
+//     gravity and FARGO sources, rhs assembly, CFL and data checks follow.
+//   * in the last stage of a step, the final RK combination and the embedded
+//     error norm (pointwise) replace the store of the stage's rhs.
+// k_remap: FARGO azimuthal shift of the combined state (y direction only).
+// k_control: one block; in device-controlled batches it reduces the partials
+// of the last stage and of the end-of-step k_rhs, runs the accept test and
+// the CFL update.  Kernel boundaries order everything: no grid-wide fences.
+#pragma once
+#include <float.h>
+#include <math.h>
+
+#include <cstring>
+
+#include "fv_device.h"
+
+namespace fv {
+
+// ------------------------------------------------------------ arithmetic
+template <typename T>
+struct Real;
+template <>
+struct Real<double> {
+  static constexpr double tiny = DBL_MIN, huge = DBL_MAX;
+};
+template <>
+struct Real<float> {
+  static constexpr float tiny = FLT_MIN, huge = FLT_MAX;
+};
+
+template <typename T>
+__device__ __forceinline__ T fmin2(T a, T b) { return (b < a) ? b : a; }
+template <typename T>
+__device__ __forceinline__ T fmax2(T a, T b) { return (b > a) ? b : a; }
+template <typename T>
+__device__ __forceinline__ T fsign(T a, T b) { return copysign(fabs(a), b); }
+template <typename T>
+__device__ __forceinline__ bool bad(T x) { return (x != x) || (x > Real<T>::huge); }
+// division, square root, log, exp: IEEE in FP64; hardware-rate in FP32
+__device__ __forceinline__ double dv(double a, double b) { return a / b; }
+__device__ __forceinline__ float dv(float a, float b) { return __fdividef(a, b); }
+__device__ __forceinline__ double sq(double a) { return sqrt(a); }
+__device__ __forceinline__ float sq(float a) { return __builtin_amdgcn_sqrtf(a); }
+__device__ __forceinline__ double lg(double a) { return log(a); }
+__device__ __forceinline__ float lg(float a) { return __logf(a); }
+__device__ __forceinline__ double ex(double a) { return exp(a); }
+__device__ __forceinline__ float ex(float a) { return __expf(a); }
+
+// error-free transformations for the compensated FP32 accumulation
+// (Knuth's two-sum; two-product with a fused multiply-add)
+__device__ __forceinline__ void two_sum(float a, float b, float &s, float &e) {
+  s = a + b;
+  const float bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+// hi + lo = y - sum_q c_q k_q, with c_q = (chi_q + clo_q) given in two parts
+struct Acc {
+  float hi, lo;
+  __device__ __forceinline__ explicit Acc(float y) : hi(y), lo(0.f) {}
+  __device__ __forceinline__ void sub(float ch, float cl, float k) {
+    const float p = ch * k, pe = fmaf(ch, k, -p);  // ch*k = p + pe exactly
+    float s, e;
+    two_sum(hi, -p, s, e);
+    hi = s;
+    lo = lo + (e - pe - cl * k);
+  }
+  __device__ __forceinline__ float value() const { return hi + lo; }
+  __device__ __forceinline__ double wide() const { return (double)hi + (double)lo; }
+};
+
+// per-ring arrays (FP64 in memory): uniform base per id, 32-bit lane offset
+__device__ __forceinline__ double ring(const Geom &g, int id, int i) { return (g.ring + id * g.NI)[(unsigned)i]; }
+template <typename T>
+__device__ __forceinline__ T rg(const Geom &g, int id, int i) { return (T)ring(g, id, i); }
+__device__ __forceinline__ long cidx(const Geom &g, int i, int j) { return (long)j * g.P + i; }
+
+// ---------------------------------------------------------------- physics
+// Cons2Prim2d / Prim2Cons2d (physics_eulerisotherm.f90:2835-2881,
+// physics_euler.f90:3056-3104)
+template <typename T, int V>
+__device__ __forceinline__ void c2p(const T *u, T *p, T gamma) {
+  T inv = dv(T(1), u[0]);
+  p[0] = u[0];
+  p[1] = u[1] * inv;
+  p[2] = u[2] * inv;
+  if (V == 4) p[3] = (gamma - T(1)) * (u[3] - T(0.5) * inv * (u[1] * u[1] + u[2] * u[2]));
+}
+template <typename T, int V>
+__device__ __forceinline__ void p2c(const T *p, T *u, T gamma) {
+  u[0] = p[0];
+  u[1] = p[0] * p[1];
+  u[2] = p[0] * p[2];
+  if (V == 4) u[3] = dv(p[3], gamma - T(1)) + T(0.5) * p[0] * (p[1] * p[1] + p[2] * p[2]);
+}
+// physics_euler.f90:2682-2689
+template <typename T>
+__device__ __forceinline__ T sound_speed(T gamma, T rho, T pr) {
+  return sq(fmax2(T(2) * Real<T>::tiny, dv(gamma * pr, rho)));
+}
+
+// Absorbing boundary in x: characteristic variables from the two edge cells
+// (primitive states pL at the lower, pR at the higher index), incoming waves
+// set to zero, extrapolation into the ghost cells (boundary_absorbing.f90:
+// 114-150).  Isothermal: physics_eulerisotherm.f90:1764-1849, 2027-2085,
+// 2895-2969.  Euler: physics_euler.f90:743-846, 1072-1129 with
+// SetEigenValues, SetCharVars and SetBoundaryData (1d/2d).  i1 is the edge
+// cell, iR the higher-index cell of the pair.
+template <typename C, int V>
+__device__ __forceinline__ void absorbing_math(const Geom &g, int east, int i1, int iR, C gam, const C *pL,
+                                               const C *pR, C *p) {
+  const C delta = east ? C(1) : C(-1);
+  C p1[V];  // element-wise select keeps both states in registers
+#pragma unroll
+  for (int l = 0; l < V; l++) p1[l] = east ? pR[l] : pL[l];
+  if constexpr (V == 4) {
+    const C rho1 = p1[0], u1 = p1[1], v1 = p1[2], P1 = p1[3];
+    const C cs = sound_speed(gam, rho1, P1);
+    const C lam0 = u1 - cs, lam1 = u1, lam3 = u1 + cs;
+    const C dlnP = lg(dv(pR[3], pL[3]));
+    const C gammadu_cs = dv(C(2) * gam * (pR[1] - pL[1]), lam3 - lam0);
+    C x0 = dlnP - gammadu_cs, x1 = dlnP - gam * lg(dv(pR[0], pL[0])), x2 = pR[2] - pL[2];
+    C x3 = dlnP + gammadu_cs;
+    if (east) {
+      if (lam0 <= C(0)) x0 = C(0);
+      if (lam1 <= C(0)) x1 = C(0), x2 = C(0);
+      if (lam3 <= C(0)) x3 = C(0);
+    } else {
+      if (lam0 >= C(0)) x0 = C(0);
+      if (lam1 >= C(0)) x1 = C(0), x2 = C(0);
+      if (lam3 >= C(0)) x3 = C(0);
+    }
+    const C dlp = C(0.5) * (x3 + x0);
+    const C rho2 = rho1 * ex(dv(delta, gam) * (dlp - x1));
+    const C P2 = P1 * ex(delta * dlp);
+    const C cs2 = sound_speed(gam, rho1 + rho2, P1 + P2);
+    p[0] = rho2;
+    p[1] = u1 + dv(C(0.5) * delta * cs2, gam) * (x3 - x0);
+    p[2] = v1 + delta * x2;
+    p[3] = P2;
+  } else {
+    const C cs = rg<C>(g, R_CS, i1), u1 = p1[1];
+    const C lam0 = u1 - cs, lam1 = u1, lam2 = u1 + cs;
+    const C csf = rg<C>(g, R_CSW, iR);
+    const C dlnrho = lg(dv(pR[0], pL[0]));
+    const C ducs = dv(pR[1] - pL[1], csf);
+    C x0 = dlnrho - ducs, x1 = pR[2] - pL[2], x2 = dlnrho + ducs;
+    if (east) {
+      if (lam0 <= C(0)) x0 = C(0);
+      if (lam1 <= C(0)) x1 = C(0);
+      if (lam2 <= C(0)) x2 = C(0);
+    } else {
+      if (lam0 >= C(0)) x0 = C(0);
+      if (lam1 >= C(0)) x1 = C(0);
+      if (lam2 >= C(0)) x2 = C(0);
+    }
+    const C cs2 = rg<C>(g, east ? R_CSE : R_CSW, i1);
+    p[0] = p1[0] * ex(delta * C(0.5) * (x2 + x0));
+    p[1] = u1 + delta * C(0.5) * cs2 * (x2 - x0);
+    p[2] = p1[2] + delta * x1;
+  }
+}
+
+// ------------------------------------------------------------ input state
+template <typename T, int V>
+struct In {
+  const Geom &g;
+  const StageArgsT<T> &a;
+  double dt;
+  T gam;
+  int nk;
+
+  __forceinline__ __device__ In(const Geom &g_, const StageArgsT<T> &a_) : g(g_), a(a_) {
+    dt = a.ctl ? a.ctl->dt : a.dt;  // device-controlled: this step's dt
+    nk = a.nk;
+    gam = (T)g.gamma;
+  }
+
+  // s = Y - sum_q (a_q*dt) k_q for one cell from loaded values, in the
+  // reference's order (FP64) or compensated (FP32).  The coefficients are
+  // formed at their use (identical products), so no per-stage register stays
+  // live across the kernel.
+  __forceinline__ __device__ void stage_sum(const T *y, const T (*kv)[V], T *s) const {
+    if constexpr (sizeof(T) == 8) {
+#pragma unroll
+      for (int l = 0; l < V; l++) s[l] = y[l];
+#pragma unroll
+      for (int q = 0; q < MAXK; q++)
+        if (q < nk) {
+          const T cq = (T)(a.a[q] * dt);
+#pragma unroll
+          for (int l = 0; l < V; l++) s[l] = s[l] - cq * kv[q][l];
+        }
+    } else {
+#pragma unroll
+      for (int l = 0; l < V; l++) {
+        Acc acc((float)y[l]);
+#pragma unroll
+        for (int q = 0; q < MAXK; q++)
+          if (q < nk) {
+            const double cq = a.a[q] * dt;
+            const float ch = (float)cq;
+            acc.sub(ch, (float)(cq - (double)ch), (float)kv[q][l]);
+          }
+        s[l] = (T)acc.value();
+      }
+    }
+  }
+
+  // input state at an interior cell
+  __forceinline__ __device__ void interior(int i, int j, T *s) const {
+    const long ix = cidx(g, i, j);
+    // all loads first (uniform branches only), then the stage sum
+    T y[V], kv[MAXK][V];
+#pragma unroll
+    for (int l = 0; l < V; l++) y[l] = a.Y[l * g.FS + ix];
+#pragma unroll
+    for (int q = 0; q < MAXK; q++)
+      if (q < nk) {
+#pragma unroll
+        for (int l = 0; l < V; l++) kv[q][l] = a.k[q][l * g.FS + ix];
+      }
+    stage_sum(y, kv, s);
+  }
+
+  // stage states of two interior cells (p0, p1 say which are wanted): all
+  // loads of both cells are issued before any arithmetic
+  __forceinline__ __device__ void interior2(bool p0, int ia, int ja, T *s0, bool p1, int ib, int jb, T *s1) const {
+    const unsigned x0 = (unsigned)ja * (unsigned)g.P + (unsigned)ia;
+    const unsigned x1 = (unsigned)jb * (unsigned)g.P + (unsigned)ib;
+    T y0[V], y1[V], k0[MAXK][V], k1[MAXK][V];
+    if (p0) {
+#pragma unroll
+      for (int l = 0; l < V; l++) y0[l] = (a.Y + l * g.FS)[x0];
+#pragma unroll
+      for (int q = 0; q < MAXK; q++)
+        if (q < nk) {
+#pragma unroll
+          for (int l = 0; l < V; l++) k0[q][l] = (a.k[q] + l * g.FS)[x0];
+        }
+    }
+    if (p1) {
+#pragma unroll
+      for (int l = 0; l < V; l++) y1[l] = (a.Y + l * g.FS)[x1];
+#pragma unroll
+      for (int q = 0; q < MAXK; q++)
+        if (q < nk) {
+#pragma unroll
+          for (int l = 0; l < V; l++) k1[q][l] = (a.k[q] + l * g.FS)[x1];
+        }
+    }
+    stage_sum(y0, k0, s0);
+    stage_sum(y1, k1, s1);
+  }
+
+  // ComputeRHS transform (timedisc_base.f90:1098-1208): ur = cvar after the
+  // frame change, CenterBoundary and PMIN (real frame), pr = pvar at that
+  // point, uc = cvar handed to the fluxes
+  __forceinline__ __device__ void transform(int i, const T *s, T *ur, T *pr, T *uc) const {
+#pragma unroll
+    for (int l = 0; l < V; l++) ur[l] = s[l];
+    if (a.add_bg) ur[2] = ur[2] + ur[0] * rg<T>(g, R_W, i);
+    c2p<T, V>(ur, pr, gam);
+    if (V == 4 && a.pmin_clamp) {
+      pr[3] = fmax2(pr[3], (T)g.pmin);
+      p2c<T, V>(pr, ur, gam);
+    }
+#pragma unroll
+    for (int l = 0; l < V; l++) uc[l] = ur[l];
+    if constexpr (sizeof(T) == 4) {
+      // FP32: m' + rho*w - rho*w would cancel to a few bits of the residual
+      // momentum m'; the stored residual is what the subtraction means
+      if (a.sub_bg) uc[2] = a.add_bg ? s[2] : ur[2] - ur[0] * rg<T>(g, R_W, i);
+    } else {
+      if (a.sub_bg) uc[2] = ur[2] - ur[0] * rg<T>(g, R_W, i);
+    }
+  }
+
+  // pvar of an interior cell right after CenterBoundary's conversion
+  __forceinline__ __device__ void prim_interior(int i, int j, T *p) const {
+    T s[V];
+    interior(i, j, s);
+    if (a.add_bg) s[2] = s[2] + s[0] * rg<T>(g, R_W, i);
+    c2p<T, V>(s, p, gam);
+  }
+
+  // absorbing boundary, x direction (absorbing_math) at row j
+  __forceinline__ __device__ void absorbing_x(int east, int j, T *p) const {
+    const int imin = GH, imax = GH + g.inum - 1;
+    const int i1 = east ? imax : imin, i2 = east ? imax - 1 : imin + 1;
+    const int iL = i1 < i2 ? i1 : i2, iR = i1 < i2 ? i2 : i1;
+    T pL[V], pR[V];
+    prim_interior(iL, j, pL);
+    prim_interior(iR, j, pR);
+    absorbing_math<T, V>(g, east, i1, iR, gam, pL, pR, p);
+  }
+
+  // x-ghost (side, layer 1..2) at interior row j
+  __forceinline__ __device__ void ghost_we(int east, int gl, int j, T *p) const {
+    const int imin = GH, imax = GH + g.inum - 1, bc = east ? g.bc[1] : g.bc[0];
+    if (bc == NO_GRADIENTS)
+      prim_interior(east ? imax : imin, j, p);
+    else if (bc == PERIODIC)
+      prim_interior(east ? imin + gl - 1 : imax - gl + 1, j, p);
+    else
+      absorbing_x(east, j, p);
+  }
+  // y-ghost (side, layer) at interior column i
+  __forceinline__ __device__ void ghost_sn(int north, int gl, int i, T *p) const {
+    const int jmin = GH, jmax = GH + g.jnum - 1, bc = north ? g.bc[3] : g.bc[2];
+    if (bc == NO_GRADIENTS)
+      prim_interior(i, north ? jmax : jmin, p);
+    else
+      prim_interior(i, north ? jmin + gl - 1 : jmax - gl + 1, p);
+  }
+
+  // pvar of any ghost cell after CenterBoundary: boundary rules in the order
+  // W, E, S, N, then SetCornerEdges (boundary_generic.f90:434-517)
+  __forceinline__ __device__ void ghost_prim(int i, int j, T *p) const {
+    const int imin = GH, imax = GH + g.inum - 1, jmin = GH, jmax = GH + g.jnum - 1;
+    const bool xg = i < imin || i > imax, yg = j < jmin || j > jmax;
+    if (xg && !yg) {
+      const int east = i > imax;
+      ghost_we(east, east ? i - imax : imin - i, j, p);
+    } else if (yg && !xg) {
+      const int north = j > jmax;
+      ghost_sn(north, north ? j - jmax : jmin - j, i, p);
+    } else {
+      const int east = i > imax, north = j > jmax;
+      const int ga = east ? i - imax : imin - i, gb = north ? j - jmax : jmin - j;
+      const int ir = east ? imax : imin, jr = north ? jmax : jmin;
+      if (ga == 2 && gb == 1) {
+        ghost_we(east, 2, jr, p);
+      } else if (ga == 1 && gb == 2) {
+        ghost_sn(north, 2, ir, p);
+      } else {
+        T px[V], py[V];
+        ghost_we(east, ga, jr, px);
+        ghost_sn(north, gb, ir, py);
+#pragma unroll
+        for (int l = 0; l < V; l++) p[l] = T(0.5) * (px[l] + py[l]);
+      }
+    }
+  }
+
+  // final ghost state: PMIN round trip, prim->cons, data check, frame change
+  __forceinline__ __device__ void ghost_final(int i, int j, T *u, T *p, int &flags) const {
+    ghost_prim(i, j, p);
+    if (V == 4 && a.pmin_clamp) p[3] = fmax2(p[3], (T)g.pmin);
+    p2c<T, V>(p, u, gam);
+    if (a.want_check) {
+#pragma unroll
+      for (int l = 0; l < V; l++)
+        if (bad(u[l]) || bad(p[l])) flags |= F_INVALID;
+    }
+    if (a.sub_bg) {
+      const T w = rg<T>(g, R_W, i);
+      p[2] = p[2] - w;
+      u[2] = u[2] - u[0] * w;
+    }
+  }
+};
+
+// CFL contribution of one cell (timedisc_base.f90:869-872 with
+// physics_eulerisotherm.f90:530-552); u, v = pvar after the frame change back
+template <typename T>
+__device__ __forceinline__ T cell_invdt(const Geom &g, int i, T u, T v, T cs) {
+  const T mnx = fmin2(T(0), u - cs), mxx = fmax2(T(0), u + cs);
+  const T mny = fmin2(T(0), v - cs), mxy = fmax2(T(0), v + cs);
+  return dv(fmax2(mxx, -mnx), rg<T>(g, R_DLX, i)) + dv(fmax2(mxy, -mny), rg<T>(g, R_DLY, i));
+}
+
+// ------------------------------------------------------------ reductions
+__device__ __forceinline__ double wave_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax2(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ __forceinline__ int wave_or(int v) {
+  for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, 64);
+  return v;
+}
+// block-wide max / or; every thread gets the result
+static __device__ double blk_max(double v, double *red) {
+  v = wave_max(v);
+  const int t = threadIdx.x + threadIdx.y * blockDim.x, nw = (blockDim.x * blockDim.y + 63) / 64;
+  __syncthreads();
+  if ((t & 63) == 0) red[t >> 6] = v;
+  __syncthreads();
+  double r = -DBL_MAX;
+  for (int k = 0; k < nw; k++) r = fmax2(r, red[k]);
+  return r;
+}
+static __device__ int blk_or(int v, int *red) {
+  v = wave_or(v);
+  const int t = threadIdx.x + threadIdx.y * blockDim.x, nw = (blockDim.x * blockDim.y + 63) / 64;
+  __syncthreads();
+  if ((t & 63) == 0) red[t >> 6] = v;
+  __syncthreads();
+  int r = 0;
+  for (int k = 0; k < nw; k++) r |= red[k];
+  return r;
+}
+// ----------------------------------------------------------- reconstruction
+// limiters, reconstruction_linear.f90:645-733
+// LIM = 0: limiter chosen at run time from g.limiter
+template <typename T, int LIM>
+__device__ __forceinline__ T slope(const Geom &g, T um, T u0, T up) {
+  const T a1 = u0 - um, a2 = up - u0;
+  const T one = T(1), zero = T(0), tiny = Real<T>::tiny;
+  switch (LIM ? LIM : g.limiter) {
+    case MINMOD:
+      return (fsign(one, a1) * fsign(one, a2) > zero) ? fsign(fmin2(fabs(a1), fabs(a2)), a1) : zero;
+    case MONOCENT: {
+      const T th = (T)g.theta;
+      const T b1 = th * (u0 - um), b2 = th * (up - u0), b3 = T(0.5) * (up - um);
+      if ((fsign(one, b1) * fsign(one, b2) > zero) && (fsign(one, b2) * fsign(one, b3) > zero))
+        return fsign(fmin2(fmin2(fabs(b1), fabs(b2)), fabs(b3)), b1);
+      return zero;
+    }
+    case SWEBY:
+    case SUPERBEE: {
+      const T q = (LIM ? LIM : g.limiter) == SUPERBEE ? T(2) : (T)g.theta;
+      return (fsign(one, a1) * fsign(one, a2) > zero)
+                 ? fsign(fmax2(fmin2(q * fabs(a1), fabs(a2)), fmin2(fabs(a1), q * fabs(a2))), a1)
+                 : zero;
+    }
+    case OSPRE:
+      return dv(T(1.5) * a1 * a2 * (a1 + a2), a1 * (a1 + T(0.5) * a2) + a2 * (a2 + T(0.5) * a1) + tiny);
+    case VANLEER: {
+      const T b1 = fabs(a1), b2 = fabs(a2);
+      return dv(a1 * b2 + a2 * b1, b1 + b2 + tiny);
+    }
+    default:
+      return T(0.5) * (a1 + a2);
+  }
+}
+
+// KT flux across one interface from the two face states (fluxes_kt.f90:126-176,
+// physics_eulerisotherm.f90:555-686, 835-930, 2742-2817; physics_euler.f90:492-600)
+// d = 0: x-interface (normal momentum index 1), d = 1: y-interface (index 2).
+template <typename T, int V>
+__device__ __forceinline__ void kt_flux(const Geom &g, int d, const T *UL, const T *UR, T csL, T csR, T dA, T *F) {
+  const int un = 1 + d, ut = 2 - d;
+  const T gam = (T)g.gamma, zero = T(0);
+  T pl[V], pr[V];
+  c2p<T, V>(UL, pl, gam);
+  c2p<T, V>(UR, pr, gam);
+  if (V == 4) {
+    csL = sound_speed(gam, pl[0], pl[3]);
+    csR = sound_speed(gam, pr[0], pr[3]);
+  }
+  const T vL = pl[un], vR = pr[un];
+  const T amin = fmin2(fmin2(zero, fmin2(zero, vR - csR)), fmin2(zero, vL - csL));
+  const T amax = fmax2(fmax2(zero, fmax2(zero, vR + csR)), fmax2(zero, vL + csL));
+  T fL[V], fR[V];
+  fL[0] = pl[0] * vL;
+  fR[0] = pr[0] * vR;
+  if (V == 3) {
+    fL[un] = UL[un] * vL + pl[0] * csL * csL;
+    fR[un] = UR[un] * vR + pr[0] * csR * csR;
+  } else {
+    fL[un] = UL[un] * vL + pl[3];
+    fR[un] = UR[un] * vR + pr[3];
+    fL[3] = (UL[3] + pl[3]) * vL;
+    fR[3] = (UR[3] + pr[3]) * vR;
+  }
+  fL[ut] = UL[ut] * vL;
+  fR[ut] = UR[ut] * vR;
+  const T fac = dv(dA, amax - amin);
+#pragma unroll
+  for (int l = 0; l < V; l++) F[l] = fac * (amax * fL[l] - amin * fR[l] + amin * amax * (UR[l] - UL[l]));
+}
+
+// ------------------------------------------------------------ step control
+// PI controller (timedisc_base.f90:759-814), used on the device only to stop
+// a batch when the step could fall under dtlimit; the host redoes such steps
+static __device__ double pi_dtnew(const Ctl *c, double maxerr, double dtold) {
+  double dttmp = maxerr > 0. ? 0.9 * dtold * exp(-log(maxerr) / c->order) : 4. * dtold;
+  double dtnew = c->maxerrold > 0. ? dttmp * exp(-c->beta * log(c->maxerrold / maxerr)) : dttmp;
+  if (maxerr < 1.0 && 4. * dtold < dtnew) dtnew = 4. * dtold;
+  return dtnew;
+}
+static __device__ void halt(Ctl *c, int reason) {
+  c->halt = 1;
+  c->reason = reason;
+}
+// after SolveODE: accept when the embedded error is finite and < 1
+// (timedisc_base.f90:686-713, AcceptSolution 911-959)
+static __device__ void control_accept(Ctl *c, double err) {
+  const int s = c->step;
+  c->err_hist[s] = err;
+  c->dt_hist[s] = c->dt;
+  if (!(err < 1.0)) return halt(c, H_ERR);  // NaN, Inf or a rejection
+  if (pi_dtnew(c, err, c->dt) < c->dtlimit * (1.0 + 1.0e-6)) return halt(c, H_DTLIMIT);
+  c->maxerrold = err;
+  const double t1 = c->time + c->dt;
+  c->dt_step = t1 - c->time;
+  c->time = t1;
+}
+// after the end-of-step RHS: data checks, output test, next CFL step,
+// stop-time test and output-time clipping (fosite.f90:383-462,
+// io/fileio_base.f90:604-620)
+static __device__ void control_next(Ctl *c, double invdt, int flags) {
+  if (flags) return halt(c, H_FLAGS);
+  const double t = c->time;
+  if (fabs(c->t_out - t) <= 1.0E-5 * c->t_out) return halt(c, H_OUTPUT);
+  double dt = c->cfl / invdt;
+  if (fabs(c->stoptime - t) <= 1.0E-05 * c->stoptime) return halt(c, H_STOP);
+  if ((t + dt) / c->t_out > 1.0)
+    dt = c->t_out - t;
+  else if ((t + 1.5 * dt) / c->t_out > 1.0)
+    dt = 0.5 * (c->t_out - t);
+  c->invdt = invdt;
+  c->dt = dt;
+  c->step = c->step + 1;
+  if (c->step >= c->nsteps) halt(c, H_BATCH);
+}
+
+// -------------------------------------------------------------- fused RHS
+// Tile: TX x TY_RHS output cells (TX = 64 or 32), one thread each.  The
+// footprint (2-cell halo, corners excluded) is staged in LDS as the state the
+// fluxes see; threads with an extra job cover the halo cells and the faces
+// and fluxes one cell beyond the tile.  Phases, separated by barriers:
+//   1 stage state + transform of the own and one extra footprint cell (ghost
+//     cells on boundary tiles), sources, data checks and CFL of the own cell
+//   2 limited slopes and face states: own cell; rows -1 and TY; columns -1, TX
+//   3 KT interface fluxes: own east and north interfaces; west of column 0,
+//     south of row 0
+//   4 rhs assembly, or the final RK combination in the last stage; stores
+// Faces live in their own LDS region; the fluxes reuse the footprint region,
+// which is dead after phase 2, so three barriers separate the four phases.
+// Output-aligned tiles keep every lane of the flux phases busy and divide
+// power-of-two meshes exactly; at most 128 VGPRs, so that 4 waves share
+// each SIMD.
+#ifndef FV_RHS_WAVES_PER_SIMD
+#define FV_RHS_WAVES_PER_SIMD 4
+#endif
+constexpr int TY_RHS = TY_TILE;
+
+// one TX x TY tile (bx, by) of a mesh tiled nbx x nby
+template <typename T, int V, int LIM, int TX>
+__device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, int bx, int by, int nbx,
+                                         int nby) {
+  constexpr int TY = TY_RHS, SX = TX + 4, SY = TY + 4;
+  constexpr int N_UC = V * SY * SX;
+  constexpr int XW = V * TY * (TX + 1), XE = V * TY, YS = V * (TY + 1) * TX, YN = V * TX;
+  constexpr int N_FACE = XW + XE + YS + YN;
+  constexpr int FX = V * TY * (TX + 1), FY = V * (TY + 1) * TX;
+  constexpr int N_FLUX = FX + FY;
+  // region A: footprint state (phases 1-2), then fluxes (phases 3-4);
+  // region B: face states (phases 2-3)
+  constexpr int NA = N_UC > N_FLUX ? N_UC : N_FLUX;
+  __shared__ T lds[NA + N_FACE];
+  __shared__ double redd[TY];
+  __shared__ int redi[TY];
+  // CFL term and check flags of the ghost cells a thread evaluated (kept in
+  // LDS rather than registers across the interior path)
+  __shared__ T gcfl[TX * TY];
+  __shared__ int gflg[TX * TY];
+  // footprint state, tile-relative row r in [-2, TY+1], column c in [-2, TX+1]
+  auto UC = [&](int l, int r, int c) -> T & { return lds[(l * SY + r + 2) * SX + c + 2]; };
+  // faces: W of columns 0..TX, E of column -1, S of rows 0..TY, N of row -1
+  T *fb = lds + NA;
+  auto FW = [&](int l, int r, int c) -> T & { return fb[(l * TY + r) * (TX + 1) + c]; };
+  auto FE1 = [&](int l, int r) -> T & { return fb[XW + l * TY + r]; };
+  auto FS = [&](int l, int r, int c) -> T & { return fb[XW + XE + (l * (TY + 1) + r) * TX + c]; };
+  auto FN1 = [&](int l, int c) -> T & { return fb[XW + XE + YS + l * TX + c]; };
+  // fluxes: x interface c-1/2 (c = 0..TX) of row r; y interface r-1/2 (r = 0..TY) of column c
+  auto FLX = [&](int l, int r, int c) -> T & { return lds[(l * TY + r) * (TX + 1) + c]; };
+  auto FLY = [&](int l, int r, int c) -> T & { return lds[FX + (l * (TY + 1) + r) * TX + c]; };
+
+  if (a.ctl && a.ctl->halt) return;
+  const In<T, V> in(g, a);
+  const int tx = threadIdx.x, ty = threadIdx.y, t = tx + TX * ty;
+  const int i0 = GH + bx * TX, j0 = GH + by * TY;
+  const int imin = GH, jmin = GH, imax = GH + g.inum - 1, jmax = GH + g.jnum - 1;
+  const int i = i0 + tx, j = j0 + ty;
+  const bool out = i <= imax && j <= jmax;  // own cell is an interior cell
+  const T zero = T(0);
+
+  // ---- phase 1: footprint state into LDS
+  // extra footprint cell: rows -2,-1,TY,TY+1 of the own column (t < 4*TX),
+  // then columns -2,-1,TX,TX+1 of rows 0..TY-1 (next 4*TY threads)
+  int er = 0, ec = 0;
+  bool has_e = false;
+  if (t < 4 * TX) {
+    er = ty < 2 ? ty - 2 : TY + ty - 2;
+    ec = tx;
+    has_e = true;
+  } else if (t < 4 * TX + 4 * TY) {
+    const int k = t - 4 * TX, q = k / TY;
+    ec = q < 2 ? q - 2 : TX + q - 2;
+    er = k % TY;
+    has_e = true;
+  }
+  const int ei = i0 + ec, ej = j0 + er;
+  const bool e_in = has_e && ei >= imin && ei <= imax && ej >= jmin && ej <= jmax;
+  // ghost cells or cells beyond the mesh (boundary tiles only) first, so that
+  // no value of the interior path is live across this register-hungry code.
+  // Every non-corner ghost cell of the mesh lies in some tile's footprint, so
+  // the ghost part of the data checks and of the CFL reduction (ghosts
+  // included, timedisc_base.f90:869-872) and the ghost cells of the new state
+  // are produced here; a cell seen by two tiles contributes identical values.
+  const bool own_g = !out, e_g = has_e && !e_in;
+  if (own_g || e_g) {
+    T ginv = -Real<T>::huge;
+    int gf = 0;
+#pragma unroll
+    for (int p = 0; p < 2; p++) {
+      if (!(p ? e_g : own_g)) continue;
+      const int ii = p ? ei : i, jj = p ? ej : j;
+      const bool xin = ii >= imin && ii <= imax, yin = jj >= jmin && jj <= jmax;
+      T u[V];
+      if ((xin || yin) && ii >= 0 && jj >= 0 && ii < g.NI && jj < g.NJ) {
+        T pp[V];
+        in.ghost_final(ii, jj, u, pp, gf);
+        if (a.want_cfl) {
+          const T cs = (V == 4) ? sound_speed(in.gam, pp[0], pp[3]) : rg<T>(g, R_CS, ii);
+          ginv = fmax2(ginv, cell_invdt(g, ii, pp[1], pp[2], cs));
+        }
+        if (a.yout) {
+          const unsigned ix = (unsigned)jj * (unsigned)g.P + (unsigned)ii;
+#pragma unroll
+          for (int l = 0; l < V; l++) (a.yout + l * g.FS)[ix] = u[l];
+        }
+      } else {
+#pragma unroll
+        for (int l = 0; l < V; l++) u[l] = zero;  // corner or beyond the mesh: never needed
+      }
+#pragma unroll
+      for (int l = 0; l < V; l++) UC(l, p ? er : ty, p ? ec : tx) = u[l];
+    }
+    gcfl[t] = ginv;
+    gflg[t] = gf;
+  }
+
+  T ur[V], pr[V];
+  T vres = zero;  // FP32: azimuthal velocity of the stored state (residual in the FARGO frame)
+  {
+    T so[V], se[V], uo[V], ue[V], re[V], pe[V];
+    in.interior2(out, i, j, so, e_in, ei, ej, se);
+    if (out) {
+      in.transform(i, so, ur, pr, uo);
+      if constexpr (sizeof(T) == 4) vres = dv(so[2], so[0]);
+#pragma unroll
+      for (int l = 0; l < V; l++) UC(l, ty, tx) = uo[l];
+    }
+    if (e_in) {
+      in.transform(ei, se, re, pe, ue);
+#pragma unroll
+      for (int l = 0; l < V; l++) UC(l, er, ec) = ue[l];
+    }
+  }
+  // sources, data checks and CFL of the own interior cell
+  T geo[V], ext[V];
+  T invdt = -Real<T>::huge;
+  int flags = 0;
+#pragma unroll
+  for (int l = 0; l < V; l++) geo[l] = ext[l] = zero;
+  if (out) {
+    // geometrical sources (physics_eulerisotherm.f90:689-832, elementals 3069-3088;
+    // physics_euler.f90:1706-1846); zero on cartesian meshes
+    if (g.curvilinear) {
+      const T cxyx = rg<T>(g, R_CXYX, i), cyxy = rg<T>(g, R_CYXY, i);
+      const T czxz = rg<T>(g, R_CZXZ, i), czyz = rg<T>(g, R_CZYZ, i);
+      const T vx = pr[1], vy = pr[2], vz = zero, mx = ur[1], my = ur[2], mz = zero;
+      T P;
+      if (V == 4) {
+        P = pr[3];
+      } else {
+        const T cs = rg<T>(g, R_CS, i);
+        P = pr[0] * (cs * cs);
+      }
+      if constexpr (sizeof(T) == 4) {
+        // FP32: the radial momentum source is the small difference of the
+        // centrifugal force and gravity (relative size ~(cs/vK)^2 on a disk);
+        // both are formed in FP64 from the stored state, with the rotating
+        // frame velocity added exactly, and only their sum is rounded
+        const double vyd = a.add_bg ? (double)vres + ring(g, R_W, i) : (double)vy;
+        const double rho = (double)ur[0], myd = rho * vyd;
+        const double cyd = ring(g, R_CYXY, i);
+        double s1 = myd * (cyd * vyd - ring(g, R_CXYX, i) * (double)vx) + (cyd + ring(g, R_CZXZ, i)) * (double)P;
+        if (g.gravity) s1 = s1 + rho * ring(g, R_ACCX, i);
+        geo[1] = (T)s1;
+      } else {
+        geo[1] = my * (cyxy * vy - cxyx * vx) + mz * (czxz * vz - zero * vx) + (cyxy + czxz) * P;
+      }
+      geo[2] = mz * (czyz * vz - zero * vy) + mx * (cxyx * vx - cyxy * vy) + (cxyx + czyz) * P;
+    }
+    // FARGO source (physics_eulerisotherm.f90:1531-1586)
+    if (g.fargo)
+      geo[2] = geo[2] - dv(ur[1] * T(0.5) * (rg<T>(g, R_W, i + 1) - rg<T>(g, R_W, i - 1)), rg<T>(g, R_DLX, i));
+    // point-mass gravity through the source list (sources_generic.f90:220-261,
+    // sources_gravity.f90:164-199, physics_eulerisotherm.f90:1201-1226,
+    // physics_euler.f90:1849-1885)
+    if (g.gravity) {
+      const T ax = rg<T>(g, R_ACCX, i), ay = rg<T>(g, R_ACCY, i);
+      // FP32 curvilinear: already part of the FP64 radial balance above
+      if (sizeof(T) == 8 || !g.curvilinear) ext[1] = zero + ur[0] * ax;
+      ext[2] = zero + ur[0] * ay;
+      if (V == 4) ext[3] = zero + (ur[1] * ax + ur[2] * ay);
+    }
+    // CheckData (timedisc_base.f90:1666-1742) and CFL (timedisc_base.f90:818-909)
+    if (a.want_check) {
+      if (pr[0] < (T)g.rhomin) flags |= F_RHOMIN;
+      if (V == 4 && pr[3] < (T)g.pmin) flags |= F_PMIN;
+#pragma unroll
+      for (int l = 0; l < V; l++)
+        if (bad(pr[l]) || bad(ur[l])) flags |= F_INVALID;
+    }
+    if (a.want_cfl) {
+      const T vy = a.sub_bg ? pr[2] - rg<T>(g, R_W, i) : pr[2];
+      const T cs = (V == 4) ? sound_speed(in.gam, pr[0], pr[3]) : rg<T>(g, R_CS, i);
+      invdt = cell_invdt(g, i, pr[1], vy, cs);
+    }
+  }
+  __syncthreads();
+
+  // ---- phase 2: limited slopes and face states (reconstruction_linear.f90)
+  const int ir = i < g.NI - 1 ? i : g.NI - 2;  // ring index, kept inside the mesh arrays
+  const int jr = j < g.NJ ? j : g.NJ - 1;
+  const T invdx = (T)g.invdx, invdy = (T)g.invdy;
+  T uc[V], fe[V], fn[V];
+  {
+    const T dw = rg<T>(g, R_RDXW, ir), de = rg<T>(g, R_RDXE, ir);
+    const T ds = (T)g.col[C_RDYS * g.NJ + jr], dn = (T)g.col[C_RDYN * g.NJ + jr];
+#pragma unroll
+    for (int l = 0; l < V; l++) {
+      uc[l] = UC(l, ty, tx);
+      const T sx = invdx * slope<T, LIM>(g, UC(l, ty, tx - 1), uc[l], UC(l, ty, tx + 1));
+      FW(l, ty, tx) = uc[l] + sx * dw;
+      fe[l] = uc[l] + sx * de;
+      const T sy = invdy * slope<T, LIM>(g, UC(l, ty - 1, tx), uc[l], UC(l, ty + 1, tx));
+      FS(l, ty, tx) = uc[l] + sy * ds;
+      fn[l] = uc[l] + sy * dn;
+    }
+  }
+  // extra faces: N of row -1 and S of row TY (t < 2*TX); E of column -1 and
+  // W of column TX (next 2*TY threads)
+  if (t < 2 * TX) {
+    const int xk = t / TX;  // 0: row -1 (N face), 1: row TY (S face)
+    const int xr = xk ? TY : -1, xc = tx;
+    const int jj = j0 + xr, jc = jj < g.NJ ? jj : g.NJ - 1;
+    const T off = (T)(xk ? g.col[C_RDYS * g.NJ + jc] : g.col[C_RDYN * g.NJ + jc]);
+#pragma unroll
+    for (int l = 0; l < V; l++) {
+      const T c0 = UC(l, xr, xc);
+      const T f = c0 + invdy * slope<T, LIM>(g, UC(l, xr - 1, xc), c0, UC(l, xr + 1, xc)) * off;
+      if (xk)
+        FS(l, TY, xc) = f;
+      else
+        FN1(l, xc) = f;
+    }
+  } else if (t < 2 * TX + 2 * TY) {
+    const int k = t - 2 * TX;
+    const int xk = k / TY;  // 0: column -1 (E face), 1: column TX (W face)
+    const int xr = k % TY, xc = xk ? TX : -1;
+    const int ii = i0 + xc, ic = ii < g.NI ? ii : g.NI - 1;
+    const T off = xk ? rg<T>(g, R_RDXW, ic) : rg<T>(g, R_RDXE, ic);
+#pragma unroll
+    for (int l = 0; l < V; l++) {
+      const T c0 = UC(l, xr, xc);
+      const T f = c0 + invdx * slope<T, LIM>(g, UC(l, xr, xc - 1), c0, UC(l, xr, xc + 1)) * off;
+      if (xk)
+        FW(l, xr, TX) = f;
+      else
+        FE1(l, xr) = f;
+    }
+  }
+  __syncthreads();
+
+  // ---- phase 3: KT interface fluxes (fluxes_kt.f90:126-176)
+  T xf[V], yf[V];
+  {
+    T r2[V], n2[V];
+#pragma unroll
+    for (int l = 0; l < V; l++) {
+      r2[l] = FW(l, ty, tx + 1);
+      n2[l] = FS(l, ty + 1, tx);
+    }
+    kt_flux<T, V>(g, 0, fe, r2, rg<T>(g, R_CSE, ir), rg<T>(g, R_CSW, ir + 1), rg<T>(g, R_DAXW, ir + 1), xf);
+    kt_flux<T, V>(g, 1, fn, n2, rg<T>(g, R_CSN, ir), rg<T>(g, R_CSS, ir), rg<T>(g, R_DAYS, ir), yf);
+#pragma unroll
+    for (int l = 0; l < V; l++) {
+      FLX(l, ty, tx + 1) = xf[l];
+      FLY(l, ty + 1, tx) = yf[l];
+    }
+  }
+  // extra fluxes: south interface of row 0 (t < TX), west interface of
+  // column 0 (threads 2*TX .. 2*TX+TY-1)
+  if (t < TX) {
+    T s1[V], n1[V], ef[V];
+#pragma unroll
+    for (int l = 0; l < V; l++) {
+      n1[l] = FN1(l, tx);
+      s1[l] = FS(l, 0, tx);
+    }
+    kt_flux<T, V>(g, 1, n1, s1, rg<T>(g, R_CSN, ir), rg<T>(g, R_CSS, ir), rg<T>(g, R_DAYS, ir), ef);
+#pragma unroll
+    for (int l = 0; l < V; l++) FLY(l, 0, tx) = ef[l];
+  } else if (t >= 2 * TX && t < 2 * TX + TY) {
+    const int fr = t - 2 * TX;
+    T e1[V], w1[V], ef[V];
+#pragma unroll
+    for (int l = 0; l < V; l++) {
+      e1[l] = FE1(l, fr);
+      w1[l] = FW(l, fr, 0);
+    }
+    kt_flux<T, V>(g, 0, e1, w1, rg<T>(g, R_CSE, i0 - 1), rg<T>(g, R_CSW, i0), rg<T>(g, R_DAXW, i0), ef);
+#pragma unroll
+    for (int l = 0; l < V; l++) FLX(l, fr, 0) = ef[l];
+  }
+  __syncthreads();
+
+  // ---- phase 4: rhs assembly (timedisc_base.f90:1254-1271)
+  double err = -DBL_MAX;
+  if (out) {
+    const T dydzdV = rg<T>(g, R_DYDZDV, i), dzdxdV = rg<T>(g, R_DZDXDV, i);
+    const T dxdydV = rg<T>(g, R_DXDYDV, i);
+    const unsigned ix = (unsigned)j * (unsigned)g.P + (unsigned)i;
+    T rhs[V];
+#pragma unroll
+    for (int l = 0; l < V; l++) {
+      const T xw = FLX(l, ty, tx), ys = FLY(l, ty, tx);
+      rhs[l] = dydzdV * (xf[l] - xw) + dzdxdV * (yf[l] - ys) + dxdydV * (zero - zero) - geo[l] - ext[l];
+      if (a.yout) (a.yout + l * g.FS)[ix] = uc[l];
+    }
+    if (a.ynew) {
+      // final combination with this stage's rhs as k_s, and the error of the
+      // embedded solution (FP64: same operation order as the reference;
+      // FP32: compensated sums, error norm in FP64)
+      T y[V], kq[MAXK][V];
+#pragma unroll
+      for (int l = 0; l < V; l++) y[l] = (a.Y + l * g.FS)[ix];
+#pragma unroll
+      for (int q = 0; q < MAXK; q++)
+        if (q < in.nk) {
+#pragma unroll
+          for (int l = 0; l < V; l++) kq[q][l] = (a.k[q] + l * g.FS)[ix];
+        }
+#pragma unroll
+      for (int l = 0; l < V; l++) {
+        // k_1 .. k_{s-1} from memory, then this stage's rhs as k_s (every
+        // coefficient index is a compile-time constant)
+        double yh, yl;
+        if constexpr (sizeof(T) == 8) {
+          T h = y[l], w = y[l];
+#pragma unroll
+          for (int q = 0; q < MAXK; q++)
+            if (q <= in.nk) {
+              const T kk = q < in.nk ? kq[q][l] : rhs[l];
+              w = w - (T)(a.bl[q] * in.dt) * kk;
+              h = h - (T)(a.bh[q] * in.dt) * kk;
+            }
+          (a.ynew + l * g.FS)[ix] = h;
+          yh = (double)h;
+          yl = (double)w;
+        } else {
+          Acc h((float)y[l]), w((float)y[l]);
+#pragma unroll
+          for (int q = 0; q < MAXK; q++)
+            if (q <= in.nk) {
+              const float kk = (float)(q < in.nk ? kq[q][l] : rhs[l]);
+              const double bh = a.bh[q] * in.dt, bl = a.bl[q] * in.dt;
+              const float bhh = (float)bh, blh = (float)bl;
+              h.sub(bhh, (float)(bh - (double)bhh), kk);
+              w.sub(blh, (float)(bl - (double)blh), kk);
+            }
+          (a.ynew + l * g.FS)[ix] = (T)h.value();
+          yh = h.wide();
+          yl = w.wide();
+        }
+        err = fmax2(err, fabs(yh - yl) / (a.tol_rel * fabs(yh) + a.tol_abs[l]));
+      }
+    } else {
+#pragma unroll
+      for (int l = 0; l < V; l++) (a.kout + l * g.FS)[ix] = rhs[l];
+    }
+  }
+  if (a.ynew) {
+    const double m = blk_max(err, redd);
+    if (t == 0) {
+      const int b = bx + by * nbx;
+      a.part_c[2 * b] = m;
+      a.part_c[2 * b + 1] = 0.0;
+    }
+  }
+
+  if (!(a.want_cfl || a.want_check)) return;
+  if (own_g || e_g) {  // ghost cells evaluated in phase 1
+    invdt = fmax2(invdt, gcfl[t]);
+    flags |= gflg[t];
+  }
+  // ---- the corner ghost cells (in no footprint) of the corner tiles:
+  // data checks, CFL and the new state (SetCornerEdges, boundary_generic.f90:434-517)
+  if (t < 16) {
+    const bool west = bx == 0, east = bx == nbx - 1;
+    const bool south = by == 0, north = by == nby - 1;
+    const int c = t / 4, ga = 1 + (t & 1), gb = 1 + ((t >> 1) & 1);
+    const int ce = c & 1, cn = c >> 1;
+    if ((ce ? east : west) && (cn ? north : south)) {
+      const int gi = ce ? imax + ga : imin - ga, gj = cn ? jmax + gb : jmin - gb;
+      T u[V], p[V];
+      in.ghost_final(gi, gj, u, p, flags);
+      if (a.want_cfl) {
+        const T cs = (V == 4) ? sound_speed(in.gam, p[0], p[3]) : rg<T>(g, R_CS, gi);
+        invdt = fmax2(invdt, cell_invdt(g, gi, p[1], p[2], cs));
+      }
+      if (a.yout) {
+        const unsigned ix = (unsigned)gj * (unsigned)g.P + (unsigned)gi;
+#pragma unroll
+        for (int l = 0; l < V; l++) (a.yout + l * g.FS)[ix] = u[l];
+      }
+    }
+  }
+  // CFL partials in FP64 (the maximum of FP32 values is exact)
+  const double m = blk_max(invdt == -Real<T>::huge ? -DBL_MAX : (double)invdt, redd);
+  const int f = blk_or(flags, redi);
+  if (t == 0) {
+    const int b = bx + by * nbx;
+    a.part[2 * b] = m;
+    a.part[2 * b + 1] = (double)f;
+  }
+}
+
+template <typename T, int V, int LIM, int TX>
+__global__ __launch_bounds__(TX *TY_RHS, FV_RHS_WAVES_PER_SIMD) void k_rhs(Geom g, StageArgsT<T> a) {
+  rhs_tile<T, V, LIM, TX>(g, a, blockIdx.x, blockIdx.y, gridDim.x, gridDim.y);
+}
+
+// Many meshes in one launch: block b runs the tile tiles[b] (TileCode) of
+// the mesh whose Geom and arguments of this launch are tab[disk].
+template <typename T, int V, int LIM, int TX>
+__global__ __launch_bounds__(TX *TY_RHS, FV_RHS_WAVES_PER_SIMD) void k_rhs_multi(
+    const RhsEntry *__restrict__ tab, const unsigned *__restrict__ tiles) {
+  const unsigned e = tiles[blockIdx.x];
+  if (e == NO_TILE) return;
+  const RhsEntry &E = tab[tile_disk(e)];
+  const StageArgsT<T> &a = reinterpret_cast<const StageArgsT<T> &>(E.a);
+  const int nbx = (E.g.inum + TX - 1) / TX, nby = (E.g.jnum + TY_RHS - 1) / TY_RHS;
+  rhs_tile<T, V, LIM, TX>(E.g, a, tile_bx(e), tile_by(e), nbx, nby);
+}
+
+// ------------------------------------------------------------ FARGO remap
+// FargoAdvectionY (timedisc_base.f90:1935-2069) of one interior cell from
+// the combined state: per ring the shift d = w*dt/dly split into its nearest
+// integer and a fraction, minmod-limited linear transport by the fraction,
+// then CSHIFT(., -shift) over JMIN:JMAX.  Rows outside JMIN:JMAX come from
+// the comoving-frame CenterBoundary (prim round trip, no frame change).
+template <typename T, int V>
+__device__ __forceinline__ void ycell(const Geom &g, const T *Y, int i, int jj, T *u) {
+  const int jmin = GH, jmax = GH + g.jnum - 1;
+  int js = jj;
+  bool ghost = false;
+  if (jj < jmin) {
+    ghost = true;
+    js = g.bc[2] == PERIODIC ? jj + g.jnum : jmin;
+  } else if (jj > jmax) {
+    ghost = true;
+    js = g.bc[3] == PERIODIC ? jj - g.jnum : jmax;
+  }
+  const long ix = cidx(g, i, js);
+#pragma unroll
+  for (int l = 0; l < V; l++) u[l] = Y[l * g.FS + ix];
+  if (ghost) {
+    T p[V];
+    c2p<T, V>(u, p, (T)g.gamma);
+    p2c<T, V>(p, u, (T)g.gamma);
+  }
+}
+
+// one 64 x 4 block (bx, by) of the remap
+template <typename T, int V>
+__device__ __forceinline__ void remap_block(const Geom &g, const RemapArgsT<T> &a, int bx, int by) {
+  if (a.ctl && a.ctl->halt) return;
+  double dts = a.dt_step;
+  if (a.ctl) {  // this%dt as AcceptSolution will set it
+    const double t = a.ctl->time, dt = a.ctl->dt;
+    dts = (t + dt) - t;
+  }
+  const int i = GH + bx * 64 + threadIdx.x, j = GH + by * 4 + threadIdx.y;
+  if (i >= GH + g.inum || j >= GH + g.jnum) return;
+  const int jmin = GH, J = g.jnum;
+  double dd = ring(g, R_W, i) * dts / ring(g, R_DLY, i);
+  const double shift = round(dd);
+  const T d = (T)(dd - shift);
+  // source row: jmin + ((j - jmin - shift) mod J); fmod is exact
+  int jj = (j - jmin) - (int)fmod(shift, (double)J);
+  if (jj < 0) jj += J;
+  if (jj >= J) jj -= J;
+  const int js = jmin + jj;
+  T v[5][V];
+#pragma unroll
+  for (int q = 0; q < 5; q++) ycell<T, V>(g, a.Y, i, js - 2 + q, v[q]);
+  const long ix = cidx(g, i, j);
+  const T one = T(1), zero = T(0), half = T(0.5);
+#pragma unroll
+  for (int l = 0; l < V; l++) {
+    T dq[4], dql[3];
+#pragma unroll
+    for (int q = 0; q < 4; q++) dq[q] = v[q + 1][l] - v[q][l];
+#pragma unroll
+    for (int q = 0; q < 3; q++)
+      dql[q] = (fsign(one, dq[q]) * fsign(one, dq[q + 1]) > zero) ? fsign(fmin2(fabs(dq[q]), fabs(dq[q + 1])), dq[q])
+                                                                 : zero;
+    T f0, f1;  // flux(js-1), flux(js)
+    if (d > zero) {
+      f0 = v[1][l] + half * dql[0] * (one - d);
+      f1 = v[2][l] + half * dql[1] * (one - d);
+    } else {
+      f0 = v[2][l] - half * dql[1] * (one + d);
+      f1 = v[3][l] - half * dql[2] * (one + d);
+    }
+    a.out[l * g.FS + ix] = v[2][l] - d * (f1 - f0);
+  }
+}
+
+template <typename T, int V>
+__global__ __launch_bounds__(256) void k_remap(Geom g, RemapArgsT<T> a) {
+  remap_block<T, V>(g, a, blockIdx.x, blockIdx.y);
+}
+template <typename T, int V>
+__global__ __launch_bounds__(256) void k_remap_multi(const RemapEntry *__restrict__ tab,
+                                                     const unsigned *__restrict__ tiles) {
+  const unsigned e = tiles[blockIdx.x];
+  if (e == NO_TILE) return;
+  const RemapEntry &E = tab[tile_disk(e)];
+  remap_block<T, V>(E.g, reinterpret_cast<const RemapArgsT<T> &>(E.a), tile_bx(e), tile_by(e));
+}
+
+// FARGO frame change on all cells (physics_eulerisotherm.f90:1263-1413):
+// sign +1 adds rho*w to the azimuthal momentum, -1 subtracts it
+template <typename T, int V>
+__global__ __launch_bounds__(256) void k_frame(Geom g, const T *src, T *dst, int sign) {
+  const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+  if (i >= g.NI || j >= g.NJ) return;
+  const long ix = cidx(g, i, j);
+  const T w = rg<T>(g, R_W, i);
+#pragma unroll
+  for (int l = 0; l < V; l++) {
+    T v = src[l * g.FS + ix];
+    if (l == 2) v = sign > 0 ? v + src[ix] * w : v - src[ix] * w;
+    dst[l * g.FS + ix] = v;
+  }
+}
+
+// typed launchers, explicitly instantiated in fv_inst_*.hip (one translation
+// unit per element type and kernel family, compiled in parallel)
+template <typename T>
+void rhs_single(const Geom &g, const StageArgsT<T> &a, int V, hipStream_t s);
+template <typename T>
+void remap_single(const Geom &g, const RemapArgsT<T> &a, int V, hipStream_t s);
+template <typename T>
+void frame_single(const Geom &g, const void *src, void *dst, int sign, int V, hipStream_t s);
+template <typename T>
+void rhs_multi(const RhsEntry *tab, const unsigned *tiles, int n, int V, int limiter, int tx, hipStream_t s);
+template <typename T>
+void remap_multi(const RemapEntry *tab, const unsigned *tiles, int n, int V, hipStream_t s);
+
+}  // namespace fv

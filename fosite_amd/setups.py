@@ -114,7 +114,8 @@ def kepler_disk_config(ni=512, nj=1024, steps=1000, cs_ratio=0.05):
 
 
 def kepler_disk_initial(cfg):
-    """Sigma ~ r^-1 (1 + 0.01 U(-1,1)), v_r = 0, v_phi = sqrt(GM/r),
+    """Sigma ~ r^slope (1 + 0.01 U(-1,1)) (slope: cfg "sigma_slope", default
+    -1), v_r = 0, v_phi = sqrt(GM/r),
     cs = cs_ratio * sqrt(GM/r) at centres and faces, w(i) = sqrt(GM/r_i)."""
     xc, yc, dx, _ = coords(cfg)
     gp, GM = cfg["gparam"], cfg["GN"] * cfg["mass"]
@@ -123,7 +124,9 @@ def kepler_disk_initial(cfg):
     rw = gp * _exp(xc + (-(0.5 * dx)))
     re = gp * _exp(xc + 0.5 * dx)
     u = splitmix64_uniform(cfg.get("seed", 20261202), NI * NJ).reshape(NJ, NI)
-    sigma = (1.0 / r)[None, :] * (1.0 + 0.01 * (2.0 * u - 1.0))
+    slope = cfg.get("sigma_slope", -1.0)
+    prof = 1.0 / r if slope == -1.0 else r ** slope
+    sigma = prof[None, :] * (1.0 + 0.01 * (2.0 * u - 1.0))
     vk = np.sqrt(GM / r)
     pv = np.zeros((3, NJ, NI))
     pv[0] = sigma
@@ -138,7 +141,65 @@ def kepler_disk_initial(cfg):
     return dict(pvar=pv, bccsound=cs, fcsound=fcs, w=vk.copy())
 
 
+# ----------------------------------------------------------------- config 3
+
+def euler_disk_config(ni=1024, nj=2048, steps=500, h=0.05, dtype="fp64"):
+    """Euler (gamma 1.4) Keplerian disk on a logcylindrical grid, r in [1,10],
+    Dormand-Prince with rtol 1e-4, point-mass gravity, absorbing radial and
+    periodic azimuthal boundaries, no FARGO (the precision-form workload)."""
+    return full(dict(physics=EULER, gamma=1.4, geometry=LOGCYLINDRICAL, inum=ni, jnum=nj,
+                     xmin=math.log(1.0), xmax=math.log(10.0), ymin=0.0, ymax=2 * math.pi,
+                     gparam=1.0, bc=[ABSORBING, ABSORBING, PERIODIC, PERIODIC],
+                     limiter=VANLEER, recvars=CONSERVATIVE, method=DORMAND_PRINCE, order=5,
+                     tol_rel=1e-4, cfl=0.4, stoptime=1.0e3, fargo=0, gravity=1, mass=1.0, GN=1.0,
+                     steps=steps, h=h, seed=20261203, dtype=dtype, problem="euler_disk"))
+
+
+def standard_normal(seed, n):
+    """N(0,1) samples by the Box-Muller transform of SplitMix64 uniforms
+    (pair k uses uniforms 2k, 2k+1)."""
+    u = splitmix64_uniform(seed, 2 * n)
+    u1, u2 = u[0::2], u[1::2]
+    return np.sqrt(-2.0 * np.log1p(-u1)) * np.cos(2.0 * math.pi * u2)
+
+
+def euler_disk_initial(cfg):
+    """Sigma = r^-1.5 exp(0.01 N(0,1)), v_r = 0, v_phi = sqrt(GM/r),
+    p = Sigma (h v_phi)^2 at the cell centres."""
+    xc, _, _, _ = coords(cfg)
+    GM = cfg["GN"] * cfg["mass"]
+    NI, NJ = len(xc), cfg["jnum"] + 2 * GNUM
+    r = cfg["gparam"] * _exp(xc)
+    z = standard_normal(cfg.get("seed", 20261203), NI * NJ).reshape(NJ, NI)
+    sigma = (r ** -1.5)[None, :] * np.exp(0.01 * z)
+    vk = np.sqrt(GM / r)
+    pv = np.zeros((4, NJ, NI))
+    pv[0] = sigma
+    pv[2] = vk[None, :]
+    pv[3] = sigma * ((cfg.get("h", 0.05) * vk) ** 2)[None, :]
+    return dict(pvar=pv)
+
+
+# ----------------------------------------------------------------- config 4
+
+def batch_disks(n=64, seed=20261204, sizes=(128, 256, 384, 512), dtype="fp32", steps=500):
+    """n independent isothermal Keplerian disks of uneven sizes (the batched
+    workload): radial cells drawn from sizes, azimuthal cells = 2x radial,
+    sound-speed ratio U(0.03, 0.08), surface-density noise seeded per disk,
+    FARGO on, SSPRK3."""
+    u = splitmix64_uniform(seed, 2 * n)
+    out = []
+    for k in range(n):
+        ni = sizes[min(int(u[2 * k] * len(sizes)), len(sizes) - 1)]
+        cfg = kepler_disk_config(ni=ni, nj=2 * ni, steps=steps, cs_ratio=0.03 + 0.05 * float(u[2 * k + 1]))
+        cfg.update(seed=seed + 1 + k, dtype=dtype)
+        out.append(cfg)
+    return out
+
+
 def initial(cfg):
+    if cfg.get("problem") == "euler_disk":
+        return euler_disk_initial(cfg)
     if cfg["physics"] == EULER:
         return riemann2d_initial(cfg)
     return kepler_disk_initial(cfg)

@@ -45,7 +45,10 @@ typedef struct fv_config {
   int gravity;      /* 1: point mass at the origin, Newtonian, no accretion
                        (sources/gravity_pointmass.f90:132-301 with outbound = 0) */
   double mass, GN;  /* point mass and gravitational constant (GEOMETRICAL units: GN = 1) */
-  int dtype;        /* 0 = FP64 (the plain drop-in) */
+  int dtype;        /* 0 = FP64, the plain drop-in (bit-exact with the reference where no
+                       libm call is on the path); 1 = FP32 precision form: FP32 state and
+                       arithmetic, compensated RK accumulation, FP64 error norm and CFL
+                       reduction (no reference counterpart) */
 } fv_config;
 
 typedef struct fv_scalars {
@@ -116,6 +119,51 @@ const char *fv_last_error(const fv_ctx *ctx);
  * per word, interior cells). */
 int fv_profile(fv_ctx *ctx, int enable);
 int fv_kernel_stats(const fv_ctx *ctx, int kernel, int *launches, double *ms, double *alg_bytes);
+
+/* ------------------------------------------------------------------ batches
+ * No reference counterpart: the reference runs one simulation per process.
+ * A batch advances many independent contexts together.  Every kernel of a
+ * device-controlled step is ONE launch over the tiles of all members, and
+ * the steps of a round are one HIP graph launch.  Each member keeps its own
+ * mesh, time step control and host scalars; results are bit-identical to
+ * running the member alone with fv_run.  Members must agree in physics,
+ * dtype and time integrator (method, order); limiters, boundaries, FARGO and
+ * sizes may differ.  A context belongs to at most one batch; fv_destroy
+ * removes it from its batch.
+ *
+ * Tilings of the RHS kernels: 0 = 64-wide tiles in row-major order, 1 =
+ * 64-wide tiles in XCD-chunked order (block b of the launch runs on XCD
+ * b mod 8; each XCD gets a contiguous run of tiles, so neighbouring tiles
+ * share that XCD's L2), 2 and 3 = the same with 32-wide tiles.  Every tiling
+ * gives the same bits.  With autotune, the first device steps of
+ * fv_batch_run are timed under each tiling in turn (2 steps per trial, 2
+ * trials per tiling), and the fastest is kept. */
+typedef struct fv_batch fv_batch;
+typedef struct fv_batch_opts {
+  int autotune;        /* 1: choose the tiling by timing (see above) */
+  int tiling;          /* tiling without autotune (0..3) */
+  int max_graph_steps; /* device steps per graph launch, 1..512 (0 = 512) */
+} fv_batch_opts;
+typedef struct fv_batch_info {
+  int members;
+  int tiling;          /* tiling in use */
+  int tuned;           /* 1 once the autotuner has chosen */
+  double tune_ms[4];   /* best time per device step of each tiling in the trials (0 = not timed) */
+  long device_steps;   /* member steps taken on the device-controlled path */
+  long host_steps;     /* member steps taken on the host path (first steps, halts) */
+  long halts;          /* device rounds a member left for the host path */
+  long rounds;         /* graph or trial launches */
+  double last_run_ms;  /* wall time of the last fv_batch_run (HIP events, batch stream) */
+} fv_batch_info;
+int fv_batch_create(fv_batch **b, const fv_batch_opts *opts); /* empty batch (opts may be NULL) */
+int fv_batch_add(fv_batch *b, fv_ctx *ctx);
+int fv_batch_remove(fv_batch *b, fv_ctx *ctx);
+/* nsteps Steps (as fv_run) of every member; taken[k] = steps member k took
+ * (members in the order they were added; fewer if its run ended). */
+int fv_batch_run(fv_batch *b, int nsteps, int *taken);
+int fv_batch_get_info(const fv_batch *b, fv_batch_info *out);
+int fv_batch_destroy(fv_batch *b);
+const char *fv_batch_last_error(const fv_batch *b);
 
 #ifdef __cplusplus
 }

@@ -71,6 +71,7 @@ def _load():
         L.fvr_step.argtypes = [ctypes.c_void_p]
         L.fvr_get.argtypes = [ctypes.c_void_p, P, P, ctypes.POINTER(_Scalars)]
         L.fvr_compute_rhs.argtypes = [ctypes.c_void_p, P]
+        L.fvr_round_state_f32.argtypes = [ctypes.c_void_p]
         _lib = L
     return _lib
 
@@ -163,3 +164,8 @@ class Oracle:
         out = np.empty((self.V, self.NJ, self.NI))
         self.L.fvr_compute_rhs(self._h, _ptr(out))
         return out
+
+    def round_state_f32(self):
+        """Round the stored state, saved state and stored rhs to FP32 (the
+        storage floor of the FP32 form; FP64 arithmetic is kept)."""
+        self.L.fvr_round_state_f32(self._h)

@@ -267,6 +267,45 @@ static void absorbing_x_iso(fvr *s, double *p, int side) {
   }
 }
 
+/* absorbing boundary, Euler: boundary_absorbing.f90:114-150 with
+ * physics_euler.f90:743-846 (CalculateCharSystemX: eigenvalues at i1 from
+ * SetEigenValues1d/2d, characteristic variables from the cells iL, iR by
+ * SetCharVars1d/2d) and 1072-1129 (CalculateBoundaryDataX with
+ * SetBoundaryData1d/2d).  side 0 = west, 1 = east. */
+static void absorbing_x_euler(fvr *s, double *p, int side) {
+  int N = s->N, imin = G, imax = G + s->c.inum - 1;
+  int i1 = side ? imax : imin, i2n = side ? imax - 1 : imin + 1; /* char system */
+  int iL = i1 < i2n ? i1 : i2n, iR = i1 < i2n ? i2n : i1;
+  int ig = side ? imax + 1 : imin - 1, delta = ig - i1;
+  const double gam = s->c.gamma;
+  for (int j = G; j < G + s->c.jnum; j++) {
+    int k1 = IDX(s, i1, j), kL = IDX(s, iL, j), kR = IDX(s, iR, j), kg = IDX(s, ig, j);
+    double rho1 = p[k1], u1 = p[1 * N + k1], v1 = p[2 * N + k1], P1 = p[3 * N + k1];
+    double cs = sound_speed(gam, rho1, P1);
+    double lam[4] = {u1 - cs, u1, u1, u1 + cs};
+    double dlnP = log(p[3 * N + kR] / p[3 * N + kL]);
+    double gammadu_cs = 2 * gam * (p[1 * N + kR] - p[1 * N + kL]) / (lam[3] - lam[0]);
+    double xv[4] = {dlnP - gammadu_cs, dlnP - gam * log(p[kR] / p[kL]), p[2 * N + kR] - p[2 * N + kL],
+                    dlnP + gammadu_cs};
+    for (int l = 0; l < 4; l++)
+      if (side ? (lam[l] <= 0.0) : (lam[l] >= 0.0)) xv[l] = 0.0;
+    {
+      double dlp = 0.5 * (xv[3] + xv[0]);
+      double rho2 = rho1 * exp((double)delta / gam * (dlp - xv[1]));
+      double P2 = P1 * exp(delta * dlp);
+      double cs2 = sound_speed(gam, rho1 + rho2, P1 + P2);
+      p[kg] = rho2;
+      p[1 * N + kg] = u1 + 0.5 * delta * cs2 / gam * (xv[3] - xv[0]);
+      p[2 * N + kg] = v1 + delta * xv[2];
+      p[3 * N + kg] = P2;
+    }
+    for (int gi = 2; gi <= G; gi++) {
+      int kk = IDX(s, side ? imax + gi : imin - gi, j);
+      for (int l = 0; l < s->V; l++) p[l * N + kk] = p[l * N + kg];
+    }
+  }
+}
+
 /* boundary_nogradients.f90:76-129, boundary_periodic.f90:73-116 */
 static void set_boundary(fvr *s, double *p, int dir) {
   int N = s->N, V = s->V, imin = G, imax = G + s->c.inum - 1, jmin = G, jmax = G + s->c.jnum - 1;
@@ -296,8 +335,11 @@ static void set_boundary(fvr *s, double *p, int dir) {
       }
     }
   } else if (bc == ABSORBING) {
-    /* only the x-direction for isothermal physics is needed by the configs */
-    absorbing_x_iso(s, p, dir == E_);
+    /* the x-direction is what the configurations need */
+    if (s->V == 4)
+      absorbing_x_euler(s, p, dir == E_);
+    else
+      absorbing_x_iso(s, p, dir == E_);
   }
 }
 
@@ -904,7 +946,7 @@ fvr *fvr_create(const fvr_cfg *cfg) {
   for (int d = 0; d < 4; d++) {
     int b = cfg->bc[d];
     if (b != NO_GRADIENTS && b != PERIODIC && b != ABSORBING) return NULL;
-    if (b == ABSORBING && (d >= 2 || cfg->physics != EULER_ISOTHERM)) return NULL;
+    if (b == ABSORBING && d >= 2) return NULL;
   }
   s = (fvr *)calloc(1, sizeof(fvr));
   s->c = *cfg;
@@ -1045,4 +1087,19 @@ void fvr_compute_rhs(fvr *s, double *rhs_out) {
   compute_rhs(s, &p, &c, CHECK_NOTHING, rhs_out);
   free(p.d);
   free(c.d);
+}
+
+/* Test aid with no reference counterpart: rounds every array the FP32 form
+ * stores between steps (the state, the saved state and the stored
+ * first-stage rhs) to IEEE single precision and recomputes the primitive
+ * variables.  Steps run in FP64 as before, so a run that calls this after
+ * every step shows the accuracy floor that FP32 storage alone sets. */
+void fvr_round_state_f32(fvr *s) {
+  const size_t n = (size_t)s->V * s->N;
+  for (size_t k = 0; k < n; k++) {
+    s->cvar.d[k] = (double)(float)s->cvar.d[k];
+    s->cold.d[k] = (double)(float)s->cold.d[k];
+    s->coeff[0][k] = (double)(float)s->coeff[0][k];
+  }
+  for (int k = 0; k < s->N; k++) cons2prim_pt(s, s->pvar.d, s->cvar.d, s->N, k);
 }

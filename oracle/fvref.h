@@ -63,6 +63,8 @@ void fvr_get(const fvr *s, double *cvar, double *pvar, fvr_scalars *sc);
 /* one ComputeRHS of the current (cvar,pvar) pair without touching the step
  * state's history; rhs out (all cells) -- used for unit-level parity */
 void fvr_compute_rhs(fvr *s, double *rhs_out);
+/* test aid: FP32 storage emulation (see fvref.c) */
+void fvr_round_state_f32(fvr *s);
 
 #ifdef __cplusplus
 }

@@ -44,6 +44,9 @@ CASES = {
     "disk_no_fargo": (dict(D, fargo=0), 8, (8,)),
     "disk_nograd_radial": (dict(D, bc=[S.NO_GRADIENTS, S.NO_GRADIENTS, S.PERIODIC, S.PERIODIC]), 8, (8,)),
     "disk_cfl_0.9": (dict(D, cfl=0.9), 12, (12,)),
+    "euler_disk_absorbing_dp5_24x48": (S.euler_disk_config(ni=24, nj=48), 12, (1, 12)),
+    "euler_disk_absorbing_ssprk3": (dict(S.euler_disk_config(ni=16, nj=32), method=S.SSPRK, order=3,
+                                         tol_rel=0.01), 10, (10,)),
 }
 MESHES = {"cartesian_16": E, "logcyl_16x32": D}
 SCALARS = ("iter", "n_adj", "brk", "time", "dt", "dtold", "maxerrold")

@@ -16,6 +16,24 @@ def rel_l1(a, b):
     return np.array(out)
 
 
+def field_errors(a, b):
+    """Relative L1 over interior cells of density, momentum (the vector:
+    sum of |dm_x| + |dm_y| over sum of |m_x| + |m_y|) and energy (when
+    present), plus each momentum component on its own."""
+    I = (slice(2, -2), slice(2, -2))
+    d = np.abs(a - b)
+
+    def r(num, den):
+        return float(num / den) if den > 0 else float(num)
+    out = {"density": r(d[0][I].sum(), np.abs(b[0][I]).sum()),
+           "momentum": r(d[1][I].sum() + d[2][I].sum(), np.abs(b[1][I]).sum() + np.abs(b[2][I]).sum()),
+           "x_momentum": r(d[1][I].sum(), np.abs(b[1][I]).sum()),
+           "y_momentum": r(d[2][I].sum(), np.abs(b[2][I]).sum())}
+    if a.shape[0] == 4:
+        out["energy"] = r(d[3][I].sum(), np.abs(b[3][I]).sum())
+    return out
+
+
 def run_pair(cfg, nsteps, every=1):
     """Yields (step, gpu_cvar, gpu_scalars, oracle_cvar, oracle_scalars)."""
     import fosite_amd

@@ -60,7 +60,20 @@ def test_create_rejects_unsupported_configuration(what):
     cfg = dict(S.riemann2d_config(n=16), **BAD[what])
     c = fosite_amd.FvConfig.from_dict(cfg)
     if what == "dtype":
-        c.dtype = 1
+        c.dtype = 2
     h = ctypes.c_void_p()
     assert L.fv_create(ctypes.byref(h), ctypes.byref(c)) == -1
     assert not h.value
+
+
+def test_batch_create_rejects_bad_options():
+    """Batch options are checked before any device resource is created."""
+    L = fosite_amd.load_library()
+    h = ctypes.c_void_p()
+    for o in ((0, 4, 0), (0, -1, 0), (1, 0, 513), (0, 0, -2)):
+        assert L.fv_batch_create(ctypes.byref(h), ctypes.byref(fosite_amd.FvBatchOpts(*o))) == -1, o
+        assert not h.value
+    assert L.fv_batch_run(None, 1, None) == -1
+    assert L.fv_batch_add(None, None) == -1
+    assert L.fv_batch_remove(None, None) == -1
+    assert L.fv_batch_destroy(None) == 0

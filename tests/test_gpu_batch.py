@@ -1,0 +1,159 @@
+"""Batches (fv_batch_*): contexts advanced together, every kernel of a step one
+launch over all members, must give exactly the bits of each context run alone
+with fv_run, under every tiling, across halts (rejected steps, stop times),
+membership changes and both dtypes.  Runs alone are themselves checked
+against the oracle in test_gpu_parity.py; one member is also checked against
+the oracle directly here."""
+import numpy as np
+import pytest
+
+import fosite_amd
+import oracle
+from fosite_amd import setups as S
+from tests.parity import rel_l1
+
+I = (slice(None), slice(2, -2), slice(2, -2))
+
+
+def make(cfg):
+    init = S.initial(cfg)
+    sim = fosite_amd.Fosite(cfg)
+    sim.set_initial(init["pvar"], init.get("bccsound"), init.get("fcsound"), init.get("w"))
+    return sim
+
+
+def alone(cfg, chunks):
+    sim = make(cfg)
+    sim.first_step()
+    taken = sum(sim.run(n) for n in chunks)
+    cv, _ = sim.state()
+    sc = sim.scalars()
+    sim.close()
+    return taken, cv, sc
+
+
+def members(dtype="fp64"):
+    D = S.kepler_disk_config
+    out = [
+        dict(D(ni=40, nj=64), seed=11),
+        dict(D(ni=72, nj=144, cs_ratio=0.03), seed=12),
+        dict(D(ni=56, nj=80), fargo=0, seed=13),
+        dict(D(ni=33, nj=70, cs_ratio=0.08), limiter=S.MINMOD, seed=14),
+        dict(D(ni=100, nj=90), bc=[S.NO_GRADIENTS, S.NO_GRADIENTS, S.PERIODIC, S.PERIODIC], seed=15),
+        dict(D(ni=48, nj=64), cfl=0.9, seed=16),  # rejected steps: halts redone on the host path
+    ]
+    return [dict(c, dtype=dtype) for c in out]
+
+
+def run_batch(cfgs, chunks, **opts):
+    sims = [make(c) for c in cfgs]
+    for s in sims:
+        s.first_step()
+    b = fosite_amd.Batch(sims, **opts)
+    taken = [0] * len(sims)
+    for n in chunks:
+        taken = [t + k for t, k in zip(taken, b.run(n))]
+    res = [(taken[k], sims[k].state()[0], sims[k].scalars()) for k in range(len(sims))]
+    info = b.info()
+    b.close()
+    for s in sims:
+        s.close()
+    return res, info
+
+
+def check_same(cfgs, res, chunks):
+    for k, cfg in enumerate(cfgs):
+        t0, cv0, sc0 = alone(cfg, chunks)
+        t1, cv1, sc1 = res[k]
+        assert t1 == t0, (k, t1, t0)
+        assert sc1 == sc0, (k, sc1, sc0)
+        assert np.array_equal(cv1[I], cv0[I]), (k, np.abs(cv1[I] - cv0[I]).max())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype", ["fp64", "fp32"])
+@pytest.mark.parametrize("tiling", [0, 1, 2, 3])
+def test_gpu_batch_equals_members_alone(dtype, tiling):
+    cfgs = members(dtype)
+    chunks = [25, 31]
+    print("batch", dtype, "tiling", tiling, [(c["inum"], c["jnum"]) for c in cfgs], flush=True)
+    res, info = run_batch(cfgs, chunks, autotune=False, tiling=tiling)
+    print("batch info", info, flush=True)
+    assert info["tiling"] == tiling and info["device_steps"] > 0 and info["halts"] > 0
+    check_same(cfgs, res, chunks)
+
+
+@pytest.mark.gpu
+def test_gpu_batch_autotune_and_short_graphs():
+    """The autotuner times all four tilings on real steps (no work is lost);
+    graphs of at most 7 steps give the same bits."""
+    cfgs = members("fp32")[:4]
+    chunks = [40, 13]
+    res, info = run_batch(cfgs, chunks, autotune=True, max_graph_steps=7)
+    print("autotune info", info, flush=True)
+    assert info["tuned"] == 1 and all(t > 0 for t in info["tune_ms"])
+    assert info["tiling"] == int(np.argmin(info["tune_ms"]))
+    check_same(cfgs, res, chunks)
+
+
+@pytest.mark.gpu
+def test_gpu_batch_member_reaching_its_stop_time():
+    cfgs = members("fp64")[:3]
+    cfgs[1] = dict(cfgs[1], stoptime=0.3)  # ends inside the run
+    chunks = [60]
+    res, _ = run_batch(cfgs, chunks, autotune=False, tiling=1)
+    assert res[1][0] < 60
+    check_same(cfgs, res, chunks)
+
+
+@pytest.mark.gpu
+def test_gpu_batch_euler_dp5_members():
+    """Euler members with the embedded-error controller and absorbing
+    boundaries (device log/exp): same bits as alone."""
+    cfgs = [S.euler_disk_config(ni=40, nj=72), dict(S.euler_disk_config(ni=64, nj=48), seed=5),
+            dict(S.euler_disk_config(ni=24, nj=96), cfl=0.9, seed=6)]
+    chunks = [20, 20]
+    res, info = run_batch(cfgs, chunks, autotune=False, tiling=3)
+    print("euler batch info", info, flush=True)
+    check_same(cfgs, res, chunks)
+
+
+@pytest.mark.gpu
+def test_gpu_batch_membership_changes():
+    cfgs = members("fp64")[:4]
+    sims = [make(c) for c in cfgs]
+    for s in sims:
+        s.first_step()
+    b = fosite_amd.Batch(sims[:3], autotune=False, tiling=0)
+    b.run(20)                 # members 0, 1, 2
+    b.remove(sims[1])
+    sims[1].run(7)            # alone, outside the batch
+    b.add(sims[3])
+    b.run(15)                 # members 0, 2, 3
+    b.add(sims[1])
+    b.run(10)                 # members 0, 2, 3, 1
+    b.close()
+    plan = {0: [20, 15, 10], 1: [20, 7, 10], 2: [20, 15, 10], 3: [15, 10]}
+    for k, cfg in enumerate(cfgs):
+        t0, cv0, sc0 = alone(cfg, plan[k])
+        cv1, _ = sims[k].state()
+        assert sims[k].scalars() == sc0, k
+        assert np.array_equal(cv1[I], cv0[I]), k
+        sims[k].close()
+
+
+@pytest.mark.gpu
+def test_gpu_batch_member_matches_oracle():
+    cfgs = members("fp64")[:3]
+    res, _ = run_batch(cfgs, [30], autotune=False, tiling=1)
+    cfg = cfgs[0]
+    init = S.initial(cfg)
+    o = oracle.Oracle(cfg)
+    o.set_initial(init["pvar"], init.get("bccsound"), init.get("fcsound"), init.get("w"))
+    o.first_step()
+    for _ in range(30):
+        o.step()
+    rc, _, rs = o.state()
+    _, cv, sc = res[0]
+    assert sc["iter"] == rs["iter"] and sc["n_adj"] == rs["n_adj"]
+    assert (rel_l1(cv, rc) <= 1e-12).all(), rel_l1(cv, rc)

@@ -32,6 +32,9 @@ CASES = {
     "disk_no_fargo": (dict(D, fargo=0), 30, 1e-12),
     "disk_cfl_0.9_rejects": (dict(D, cfl=0.9), 20, 1e-12),
     "disk_ragged_100x90": (dict(D, inum=100, jnum=90), 20, 1e-12),
+    "euler_disk_absorbing_dp5": (S.euler_disk_config(ni=40, nj=72), 20, 1e-12),
+    "euler_disk_absorbing_ssprk3_ragged": (dict(S.euler_disk_config(ni=70, nj=50), method=S.SSPRK, order=3,
+                                                tol_rel=0.01), 20, 1e-12),
 }
 
 
@@ -49,8 +52,11 @@ def test_gpu_matches_oracle(name):
             I = (slice(None), slice(2, -2), slice(2, -2))
             assert np.array_equal(cv[I], rc[I]), (n, np.abs(cv[I] - rc[I]).max())
         else:
-            assert abs(sc["time"] - rs["time"]) <= tol * max(1.0, abs(rs["time"])), (n, sc, rs)
-            assert abs(sc["dt"] - rs["dt"]) <= tol * abs(rs["dt"]), (n, sc, rs)
+            # dt of an embedded-error controller follows the error estimate,
+            # which magnifies last-bit state differences (see the batched test)
+            stol = 1e-10 if cfg["method"] in (S.DORMAND_PRINCE, S.RK_FEHLBERG) else tol
+            assert abs(sc["time"] - rs["time"]) <= stol * max(1.0, abs(rs["time"])), (n, sc, rs)
+            assert abs(sc["dt"] - rs["dt"]) <= stol * abs(rs["dt"]), (n, sc, rs)
             err = rel_l1(cv, rc)
             assert (err <= tol).all(), (n, err)
     print("case", name, "max relative L1 over all steps: %.3e" % worst)
@@ -81,6 +87,7 @@ BATCH_CASES = {
     "disk_fargo_absorbing": (D, 90, 40, 1e-12),
     "disk_cfl_0.9_rejects": (dict(D, cfl=0.9), 60, 30, 1e-12),
     "disk_ragged_100x90": (dict(D, inum=100, jnum=90), 40, 17, 1e-12),
+    "euler_disk_absorbing_dp5": (S.euler_disk_config(ni=40, nj=72), 40, 15, 1e-12),
 }
 
 
@@ -91,6 +98,7 @@ def test_gpu_batched_run_matches_oracle(name):
     print("batched case", name, cfg["inum"], cfg["jnum"], nsteps, chunk, flush=True)
     I = (slice(None), slice(2, -2), slice(2, -2))
     ended = False
+    worst = {}
     for n, cv, sc, rc, rs, ended in run_batched(cfg, nsteps, chunk):
         for key in ("iter", "n_adj", "brk"):
             assert sc[key] == rs[key], (n, key, sc, rs)
@@ -100,8 +108,21 @@ def test_gpu_batched_run_matches_oracle(name):
             assert np.array_equal(cv[I], rc[I]), (n, np.abs(cv[I] - rc[I]).max())
         else:
             for key in ("time", "dt", "dtold", "maxerrold"):
-                assert abs(sc[key] - rs[key]) <= tol * max(abs(rs[key]), 1e-300), (n, key, sc, rs)
-            assert (rel_l1(cv, rc) <= tol).all(), (n, rel_l1(cv, rc))
+                d = abs(sc[key] - rs[key]) / max(abs(rs[key]), 1e-300)
+                worst[key] = max(worst.get(key, 0.0), d)
+            worst["state"] = max(worst.get("state", 0.0), float(rel_l1(cv, rc).max()))
+    if tol != 0.0:
+        print("worst relative deviations", worst, flush=True)
+        # The conserved fields are held to the drop-in bound.  With an
+        # embedded-error controller, maxerrold is the difference of two
+        # solutions that agree to about tol_rel*maxerr, so last-bit
+        # differences of the state reach it magnified by ~1/(tol_rel*maxerr)
+        # (1e5-1e6 here); dt takes its 1/order power and time sums dt.
+        adaptive = cfg["method"] in (S.DORMAND_PRINCE, S.RK_FEHLBERG)
+        bounds = {"state": tol, "time": 1e-10 if adaptive else tol, "dt": 1e-10 if adaptive else tol,
+                  "dtold": 1e-10 if adaptive else tol, "maxerrold": 1e-8 if adaptive else tol}
+        bad = {k: (worst[k], bounds[k]) for k in worst if worst[k] > bounds[k]}
+        assert not bad, bad
     if name == "euler_stop_inside_batch":
         assert ended
 
