@@ -12,7 +12,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 # kernel templates are instantiated per element type and kernel family in
 # separate translation units, compiled in parallel
 SRC = [os.path.join(HERE, "csrc", f) for f in (
-    "fv_inst_f64_single.hip", "fv_inst_f64_multi.hip", "fv_inst_f32_single.hip", "fv_inst_f32_multi.hip",
+    "fv_inst_f64_single_s3.hip", "fv_inst_f64_single_s7.hip", "fv_inst_f32_single_s3.hip",
+    "fv_inst_f32_single_s7.hip", "fv_inst_f64_multi_s3.hip", "fv_inst_f64_multi_s7.hip",
+    "fv_inst_f32_multi_s3.hip", "fv_inst_f32_multi_s7.hip",
     "fv_kernels.hip", "fv_api.cpp", "fv_mesh.cpp")]
 OUT = os.path.join(HERE, "libfosite_fv.so")
 OBJ = os.path.join(HERE, "_build")

@@ -704,6 +704,7 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
   for (int d = 0; d < 4; d++) g.bc[d] = c.bc[d];
   g.fargo = c.fargo == 2; g.gravity = c.gravity != 0; g.curvilinear = c.geometry != CARTESIAN;
   g.dtype = c.dtype;
+  g.stages = ctx->tab.m;
   g.tile_x = 64;
   if (const char *e = std::getenv("FV_RHS_TILE")) g.tile_x = std::atoi(e) == 32 ? 32 : 64;
   ctx->nb_rhs = rhs_blocks(g);
@@ -1104,11 +1105,11 @@ void enqueue_batch_step(fv_batch *b, int p, int c) {
   const unsigned *tl = b->d_tiles + b->cand_off[c];
   for (int q = 0; q < m - 1; q++)
     launch_rhs_multi(b->d_rhs + (size_t)(p * m + q) * cap, tl, b->cand_n[c], b->V, b->dtype, b->limiter,
-                     cand_tx(c), b->st);
+                     cand_tx(c), b->m, b->st);
   if (b->remap_n)
     launch_remap_multi(b->d_remap + (size_t)p * cap, b->d_tiles + b->remap_off, b->remap_n, b->V, b->dtype, b->st);
   launch_rhs_multi(b->d_rhs + (size_t)(p * m + m - 1) * cap, tl, b->cand_n[c], b->V, b->dtype, b->limiter,
-                   cand_tx(c), b->st);
+                   cand_tx(c), b->m, b->st);
   launch_control_multi(b->d_cent + (size_t)c * cap, n, b->st);
 }
 

@@ -35,6 +35,7 @@ struct Geom {
   int bc[4];
   int fargo, gravity, curvilinear;
   int tile_x;           // k_rhs tile width in cells: 64 or 32
+  int stages;           // RK stages of the tableau (picks the kernels' coefficient slots)
   int dtype;            // stored state: 0 FP64 (plain drop-in), 1 FP32 (precision form)
   const double *ring;   // [R_COUNT][NI]
   const double *col;    // [C_COUNT][NJ]
@@ -138,7 +139,7 @@ void launch_control(Ctl *ctl, const double *part_c, const double *part_r, int nb
 #endif
 constexpr int TY_TILE = FV_RHS_ROWS;  // k_rhs tile rows
 void launch_rhs_multi(const RhsEntry *tab, const unsigned *tiles, int ntiles, int V, int dtype, int limiter,
-                      int tx, hipStream_t s);
+                      int tx, int stages, hipStream_t s);
 void launch_remap_multi(const RemapEntry *tab, const unsigned *tiles, int ntiles, int V, int dtype, hipStream_t s);
 void launch_control_multi(const ControlEntry *tab, int n, hipStream_t s);
 

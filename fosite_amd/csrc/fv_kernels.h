@@ -185,41 +185,46 @@ __device__ __forceinline__ void absorbing_math(const Geom &g, int east, int i1, 
 }
 
 // ------------------------------------------------------------ input state
-template <typename T, int V>
+// KM: coefficient slots of the tableau (3 for up to three stages, else MAXK);
+// loops over stages are unrolled to KM with run-time guards on nk
+template <typename T, int V, int KM>
 struct In {
   const Geom &g;
   const StageArgsT<T> &a;
   double dt;
   T gam;
   int nk;
+  double c[sizeof(T) == 8 ? KM : 1];  // FP64: a(m,q)*dt, formed once
 
   __forceinline__ __device__ In(const Geom &g_, const StageArgsT<T> &a_) : g(g_), a(a_) {
     dt = a.ctl ? a.ctl->dt : a.dt;  // device-controlled: this step's dt
     nk = a.nk;
     gam = (T)g.gamma;
+    if constexpr (sizeof(T) == 8) {
+#pragma unroll
+      for (int q = 0; q < KM; q++) c[q] = a.a[q] * dt;
+    }
   }
 
   // s = Y - sum_q (a_q*dt) k_q for one cell from loaded values, in the
-  // reference's order (FP64) or compensated (FP32).  The coefficients are
-  // formed at their use (identical products), so no per-stage register stays
-  // live across the kernel.
+  // reference's order (FP64) or compensated (FP32; the coefficients are then
+  // split into hi/lo parts at their use, so no pair stays live).
   __forceinline__ __device__ void stage_sum(const T *y, const T (*kv)[V], T *s) const {
     if constexpr (sizeof(T) == 8) {
 #pragma unroll
       for (int l = 0; l < V; l++) s[l] = y[l];
 #pragma unroll
-      for (int q = 0; q < MAXK; q++)
+      for (int q = 0; q < KM; q++)
         if (q < nk) {
-          const T cq = (T)(a.a[q] * dt);
 #pragma unroll
-          for (int l = 0; l < V; l++) s[l] = s[l] - cq * kv[q][l];
+          for (int l = 0; l < V; l++) s[l] = s[l] - c[q] * kv[q][l];
         }
     } else {
 #pragma unroll
       for (int l = 0; l < V; l++) {
         Acc acc((float)y[l]);
 #pragma unroll
-        for (int q = 0; q < MAXK; q++)
+        for (int q = 0; q < KM; q++)
           if (q < nk) {
             const double cq = a.a[q] * dt;
             const float ch = (float)cq;
@@ -234,11 +239,11 @@ struct In {
   __forceinline__ __device__ void interior(int i, int j, T *s) const {
     const long ix = cidx(g, i, j);
     // all loads first (uniform branches only), then the stage sum
-    T y[V], kv[MAXK][V];
+    T y[V], kv[KM][V];
 #pragma unroll
     for (int l = 0; l < V; l++) y[l] = a.Y[l * g.FS + ix];
 #pragma unroll
-    for (int q = 0; q < MAXK; q++)
+    for (int q = 0; q < KM; q++)
       if (q < nk) {
 #pragma unroll
         for (int l = 0; l < V; l++) kv[q][l] = a.k[q][l * g.FS + ix];
@@ -251,12 +256,12 @@ struct In {
   __forceinline__ __device__ void interior2(bool p0, int ia, int ja, T *s0, bool p1, int ib, int jb, T *s1) const {
     const unsigned x0 = (unsigned)ja * (unsigned)g.P + (unsigned)ia;
     const unsigned x1 = (unsigned)jb * (unsigned)g.P + (unsigned)ib;
-    T y0[V], y1[V], k0[MAXK][V], k1[MAXK][V];
+    T y0[V], y1[V], k0[KM][V], k1[KM][V];
     if (p0) {
 #pragma unroll
       for (int l = 0; l < V; l++) y0[l] = (a.Y + l * g.FS)[x0];
 #pragma unroll
-      for (int q = 0; q < MAXK; q++)
+      for (int q = 0; q < KM; q++)
         if (q < nk) {
 #pragma unroll
           for (int l = 0; l < V; l++) k0[q][l] = (a.k[q] + l * g.FS)[x0];
@@ -266,7 +271,7 @@ struct In {
 #pragma unroll
       for (int l = 0; l < V; l++) y1[l] = (a.Y + l * g.FS)[x1];
 #pragma unroll
-      for (int q = 0; q < MAXK; q++)
+      for (int q = 0; q < KM; q++)
         if (q < nk) {
 #pragma unroll
           for (int l = 0; l < V; l++) k1[q][l] = (a.k[q] + l * g.FS)[x1];
@@ -561,7 +566,7 @@ static __device__ void control_next(Ctl *c, double invdt, int flags) {
 constexpr int TY_RHS = TY_TILE;
 
 // one TX x TY tile (bx, by) of a mesh tiled nbx x nby
-template <typename T, int V, int LIM, int TX>
+template <typename T, int V, int LIM, int TX, int KM>
 __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, int bx, int by, int nbx,
                                          int nby) {
   constexpr int TY = TY_RHS, SX = TX + 4, SY = TY + 4;
@@ -593,7 +598,7 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
   auto FLY = [&](int l, int r, int c) -> T & { return lds[FX + (l * (TY + 1) + r) * TX + c]; };
 
   if (a.ctl && a.ctl->halt) return;
-  const In<T, V> in(g, a);
+  const In<T, V, KM> in(g, a);
   const int tx = threadIdx.x, ty = threadIdx.y, t = tx + TX * ty;
   const int i0 = GH + bx * TX, j0 = GH + by * TY;
   const int imin = GH, jmin = GH, imax = GH + g.inum - 1, jmax = GH + g.jnum - 1;
@@ -852,11 +857,11 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
       // final combination with this stage's rhs as k_s, and the error of the
       // embedded solution (FP64: same operation order as the reference;
       // FP32: compensated sums, error norm in FP64)
-      T y[V], kq[MAXK][V];
+      T y[V], kq[KM][V];
 #pragma unroll
       for (int l = 0; l < V; l++) y[l] = (a.Y + l * g.FS)[ix];
 #pragma unroll
-      for (int q = 0; q < MAXK; q++)
+      for (int q = 0; q < KM; q++)
         if (q < in.nk) {
 #pragma unroll
           for (int l = 0; l < V; l++) kq[q][l] = (a.k[q] + l * g.FS)[ix];
@@ -869,7 +874,7 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
         if constexpr (sizeof(T) == 8) {
           T h = y[l], w = y[l];
 #pragma unroll
-          for (int q = 0; q < MAXK; q++)
+          for (int q = 0; q < KM; q++)
             if (q <= in.nk) {
               const T kk = q < in.nk ? kq[q][l] : rhs[l];
               w = w - (T)(a.bl[q] * in.dt) * kk;
@@ -881,7 +886,7 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
         } else {
           Acc h((float)y[l]), w((float)y[l]);
 #pragma unroll
-          for (int q = 0; q < MAXK; q++)
+          for (int q = 0; q < KM; q++)
             if (q <= in.nk) {
               const float kk = (float)(q < in.nk ? kq[q][l] : rhs[l]);
               const double bh = a.bh[q] * in.dt, bl = a.bl[q] * in.dt;
@@ -946,14 +951,14 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
   }
 }
 
-template <typename T, int V, int LIM, int TX>
+template <typename T, int V, int LIM, int TX, int KM>
 __global__ __launch_bounds__(TX *TY_RHS, FV_RHS_WAVES_PER_SIMD) void k_rhs(Geom g, StageArgsT<T> a) {
-  rhs_tile<T, V, LIM, TX>(g, a, blockIdx.x, blockIdx.y, gridDim.x, gridDim.y);
+  rhs_tile<T, V, LIM, TX, KM>(g, a, blockIdx.x, blockIdx.y, gridDim.x, gridDim.y);
 }
 
 // Many meshes in one launch: block b runs the tile tiles[b] (TileCode) of
 // the mesh whose Geom and arguments of this launch are tab[disk].
-template <typename T, int V, int LIM, int TX>
+template <typename T, int V, int LIM, int TX, int KM>
 __global__ __launch_bounds__(TX *TY_RHS, FV_RHS_WAVES_PER_SIMD) void k_rhs_multi(
     const RhsEntry *__restrict__ tab, const unsigned *__restrict__ tiles) {
   const unsigned e = tiles[blockIdx.x];
@@ -961,7 +966,7 @@ __global__ __launch_bounds__(TX *TY_RHS, FV_RHS_WAVES_PER_SIMD) void k_rhs_multi
   const RhsEntry &E = tab[tile_disk(e)];
   const StageArgsT<T> &a = reinterpret_cast<const StageArgsT<T> &>(E.a);
   const int nbx = (E.g.inum + TX - 1) / TX, nby = (E.g.jnum + TY_RHS - 1) / TY_RHS;
-  rhs_tile<T, V, LIM, TX>(E.g, a, tile_bx(e), tile_by(e), nbx, nby);
+  rhs_tile<T, V, LIM, TX, KM>(E.g, a, tile_bx(e), tile_by(e), nbx, nby);
 }
 
 // ------------------------------------------------------------ FARGO remap
@@ -1069,14 +1074,17 @@ __global__ __launch_bounds__(256) void k_frame(Geom g, const T *src, T *dst, int
 
 // typed launchers, explicitly instantiated in fv_inst_*.hip (one translation
 // unit per element type and kernel family, compiled in parallel)
-template <typename T>
+template <typename T, int KM>
 void rhs_single(const Geom &g, const StageArgsT<T> &a, int V, hipStream_t s);
 template <typename T>
 void remap_single(const Geom &g, const RemapArgsT<T> &a, int V, hipStream_t s);
 template <typename T>
 void frame_single(const Geom &g, const void *src, void *dst, int sign, int V, hipStream_t s);
-template <typename T>
+template <typename T, int KM>
 void rhs_multi(const RhsEntry *tab, const unsigned *tiles, int n, int V, int limiter, int tx, hipStream_t s);
+// coefficient slots the kernels are compiled for, by tableau size
+constexpr int KM_SMALL = 3;
+inline int km_for(int stages) { return stages <= KM_SMALL ? KM_SMALL : MAXK; }
 template <typename T>
 void remap_multi(const RemapEntry *tab, const unsigned *tiles, int n, int V, hipStream_t s);
 

@@ -69,9 +69,11 @@ void launch_rhs(const Geom &g, const StageArgs &a, int V, hipStream_t s) {
   if (g.dtype == 1) {
     StageArgsT<float> af;
     std::memcpy(&af, &a, sizeof(af));
-    rhs_single<float>(g, af, V, s);
+    if (km_for(g.stages) == KM_SMALL) rhs_single<float, KM_SMALL>(g, af, V, s);
+    else rhs_single<float, MAXK>(g, af, V, s);
   } else {
-    rhs_single<double>(g, a, V, s);
+    if (km_for(g.stages) == KM_SMALL) rhs_single<double, KM_SMALL>(g, a, V, s);
+    else rhs_single<double, MAXK>(g, a, V, s);
   }
 }
 void launch_remap(const Geom &g, const RemapArgs &a, int V, hipStream_t s) {
@@ -93,11 +95,15 @@ void launch_control(Ctl *ctl, const double *part_c, const double *part_r, int nb
   hipLaunchKernelGGL(k_control, dim3(1), dim3(1024), 0, s, ctl, part_c, part_r, nb);
 }
 void launch_rhs_multi(const RhsEntry *tab, const unsigned *tiles, int ntiles, int V, int dtype, int limiter,
-                      int tx, hipStream_t s) {
-  if (dtype == 1)
-    rhs_multi<float>(tab, tiles, ntiles, V, limiter, tx, s);
-  else
-    rhs_multi<double>(tab, tiles, ntiles, V, limiter, tx, s);
+                      int tx, int stages, hipStream_t s) {
+  const bool small = km_for(stages) == KM_SMALL;
+  if (dtype == 1) {
+    if (small) rhs_multi<float, KM_SMALL>(tab, tiles, ntiles, V, limiter, tx, s);
+    else rhs_multi<float, MAXK>(tab, tiles, ntiles, V, limiter, tx, s);
+  } else {
+    if (small) rhs_multi<double, KM_SMALL>(tab, tiles, ntiles, V, limiter, tx, s);
+    else rhs_multi<double, MAXK>(tab, tiles, ntiles, V, limiter, tx, s);
+  }
 }
 void launch_remap_multi(const RemapEntry *tab, const unsigned *tiles, int ntiles, int V, int dtype, hipStream_t s) {
   if (dtype == 1)

@@ -5,29 +5,29 @@
 
 namespace fv {
 
-template <typename T, int V, int LIM, int TX>
+template <typename T, int V, int LIM, int TX, int KM>
 static void rhs_lim(const Geom &g, const StageArgsT<T> &a, hipStream_t s) {
   dim3 grid((g.inum + TX - 1) / TX, (g.jnum + TY_RHS - 1) / TY_RHS), block(TX, TY_RHS);
-  hipLaunchKernelGGL((k_rhs<T, V, LIM, TX>), grid, block, 0, s, g, a);
+  hipLaunchKernelGGL((k_rhs<T, V, LIM, TX, KM>), grid, block, 0, s, g, a);
 }
-template <typename T, int V, int TX>
+template <typename T, int V, int TX, int KM>
 static void rhs_v(const Geom &g, const StageArgsT<T> &a, hipStream_t s) {
   // compile-time limiters for the common choices, run-time switch otherwise
   if (g.limiter == VANLEER)
-    rhs_lim<T, V, VANLEER, TX>(g, a, s);
+    rhs_lim<T, V, VANLEER, TX, KM>(g, a, s);
   else if (g.limiter == MINMOD)
-    rhs_lim<T, V, MINMOD, TX>(g, a, s);
+    rhs_lim<T, V, MINMOD, TX, KM>(g, a, s);
   else
-    rhs_lim<T, V, 0, TX>(g, a, s);
+    rhs_lim<T, V, 0, TX, KM>(g, a, s);
 }
-template <typename T>
+template <typename T, int KM>
 void rhs_single(const Geom &g, const StageArgsT<T> &a, int V, hipStream_t s) {
   if (V == 3) {
-    if (g.tile_x == 32) rhs_v<T, 3, 32>(g, a, s);
-    else rhs_v<T, 3, 64>(g, a, s);
+    if (g.tile_x == 32) rhs_v<T, 3, 32, KM>(g, a, s);
+    else rhs_v<T, 3, 64, KM>(g, a, s);
   } else {
-    if (g.tile_x == 32) rhs_v<T, 4, 32>(g, a, s);
-    else rhs_v<T, 4, 64>(g, a, s);
+    if (g.tile_x == 32) rhs_v<T, 4, 32, KM>(g, a, s);
+    else rhs_v<T, 4, 64, KM>(g, a, s);
   }
 }
 template <typename T>
@@ -51,22 +51,22 @@ void frame_single(const Geom &g, const void *src, void *dst, int sign, int V, hi
 
 // batches: van Leer compiled in, every other limiter (and mixed limiters) by
 // the run-time switch
-template <typename T, int V, int TX>
+template <typename T, int V, int TX, int KM>
 static void rhs_multi_v(const RhsEntry *tab, const unsigned *tiles, int n, int limiter, hipStream_t s) {
   const dim3 block(TX, TY_RHS);
   if (limiter == VANLEER)
-    hipLaunchKernelGGL((k_rhs_multi<T, V, VANLEER, TX>), dim3(n), block, 0, s, tab, tiles);
+    hipLaunchKernelGGL((k_rhs_multi<T, V, VANLEER, TX, KM>), dim3(n), block, 0, s, tab, tiles);
   else
-    hipLaunchKernelGGL((k_rhs_multi<T, V, 0, TX>), dim3(n), block, 0, s, tab, tiles);
+    hipLaunchKernelGGL((k_rhs_multi<T, V, 0, TX, KM>), dim3(n), block, 0, s, tab, tiles);
 }
-template <typename T>
+template <typename T, int KM>
 void rhs_multi(const RhsEntry *tab, const unsigned *tiles, int n, int V, int limiter, int tx, hipStream_t s) {
   if (V == 3) {
-    if (tx == 32) rhs_multi_v<T, 3, 32>(tab, tiles, n, limiter, s);
-    else rhs_multi_v<T, 3, 64>(tab, tiles, n, limiter, s);
+    if (tx == 32) rhs_multi_v<T, 3, 32, KM>(tab, tiles, n, limiter, s);
+    else rhs_multi_v<T, 3, 64, KM>(tab, tiles, n, limiter, s);
   } else {
-    if (tx == 32) rhs_multi_v<T, 4, 32>(tab, tiles, n, limiter, s);
-    else rhs_multi_v<T, 4, 64>(tab, tiles, n, limiter, s);
+    if (tx == 32) rhs_multi_v<T, 4, 32, KM>(tab, tiles, n, limiter, s);
+    else rhs_multi_v<T, 4, 64, KM>(tab, tiles, n, limiter, s);
   }
 }
 template <typename T>
