@@ -190,6 +190,53 @@ def batched_form(n=64, steps=500, alone_steps=50):
             out["fp64_alone"]["cell_updates_per_s"]}
 
 
+def serving_form(n=300, rate=20.0, nmin=64, nmax=256, orbits=10.0, budget_gb=64.0, round_steps=32):
+    """The serving loop: n disk runs arriving as a Poisson stream (mean `rate`
+    per second), radial cells uniform in [nmin, nmax] (azimuthal 2x), FP32:FP64
+    3:1, surface-density slope U(-1.5,-0.5), each integrated for `orbits` inner
+    orbits; admitted by device-memory budget, continuous batching.  Latency of
+    a run = completion - arrival (server clock); throughput = all cell updates
+    / (last completion - first arrival)."""
+    import time
+    import numpy as np
+    import fosite_amd
+    from fosite_amd import setups
+    reqs = setups.serving_requests(n=n, rate=rate, nmin=nmin, nmax=nmax, orbits=orbits)
+    srv = fosite_amd.Server(budget_gb * 1e9, round_steps=round_steps, autotune=True)
+    c0 = srv.stats()["clock"]
+    t0 = time.perf_counter()
+    ids, k, done = [], 0, 0
+    while done < n:
+        now = time.perf_counter() - t0
+        while k < n and reqs[k][0] <= now:
+            cfg = reqs[k][1]
+            ids.append(srv.submit(cfg, setups.initial(cfg)))
+            k += 1
+        st = srv.stats()
+        if st["running"] == 0 and st["queued"] == 0 and k < n:
+            time.sleep(max(0.0, min(0.01, reqs[k][0] - (time.perf_counter() - t0))))
+            continue
+        srv.step()
+        done = srv.stats()["completed"] + srv.stats()["failed"]
+    st = srv.stats()
+    lat, upd, t_end = [], 0.0, 0.0
+    for rid, (arr, cfg) in zip(ids, reqs):
+        info = srv.poll(rid)
+        lat.append(info["t_done"] - (c0 + arr))
+        upd += cfg["inum"] * cfg["jnum"] * (info["steps"] + 1)
+        t_end = max(t_end, info["t_done"])
+    lat = np.array(lat)
+    wall = t_end - (c0 + reqs[0][0])
+    srv.close()
+    return {"workload": "%d disk runs, Poisson arrivals %.0f/s, radial cells U[%d,%d] (azimuthal 2x), FP32:FP64 "
+                        "3:1, slope U(-1.5,-0.5), %g inner orbits, FARGO, SSPRK3" % (n, rate, nmin, nmax, orbits),
+            "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
+            "mean_latency_s": float(lat.mean()), "cell_updates_per_s": upd / wall, "wall_s": wall,
+            "completed": st["completed"], "failed": st["failed"], "peak_running": st["peak_running"],
+            "peak_device_gb": st["peak_device_bytes"] / 1e9, "budget_gb": budget_gb, "rounds": st["rounds"],
+            "gpu_busy_share": st["busy_ms"] * 1e-3 / wall}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -198,8 +245,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--profile-steps", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--forms", action="store_true",
-                    help="also measure the precision form (Euler disk 1024x2048, DP5)")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="skip the precision, batched and serving forms")
     ap.add_argument("--form-steps", type=int, default=500)
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -262,9 +309,10 @@ def main():
         "kernels": kernels,
     }
     sim.close()
-    if a.forms:
+    if not a.no_forms:
         out["precision_form"] = precision_form(1024, 2048, a.form_steps)
         out["batched_form"] = batched_form()
+        out["serving_form"] = serving_form()
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg, init, a.cpu_steps)
     print(json.dumps(out))

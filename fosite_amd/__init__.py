@@ -19,7 +19,7 @@ from . import setups
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.environ.get("FOSITE_FV_LIB") or os.path.join(HERE, "libfosite_fv.so")
 
-__all__ = ["Fosite", "Batch", "FvConfig", "mesh_array", "setups", "load_library", "LIBPATH"]
+__all__ = ["Fosite", "Batch", "Server", "device_bytes", "FvConfig", "mesh_array", "setups", "load_library", "LIBPATH"]
 
 
 class FvConfig(ctypes.Structure):
@@ -67,7 +67,25 @@ EXPORTS = ["fv_mesh_array", "fv_create", "fv_destroy", "fv_dims", "fv_set_sound_
            "fv_set_background_velocity", "fv_set_pvar", "fv_first_step", "fv_step", "fv_run",
            "fv_get_state", "fv_get_scalars", "fv_last_run_ms", "fv_last_error", "fv_profile",
            "fv_kernel_stats", "fv_batch_create", "fv_batch_add", "fv_batch_remove", "fv_batch_run",
-           "fv_batch_get_info", "fv_batch_destroy", "fv_batch_last_error"]
+           "fv_batch_get_info", "fv_batch_destroy", "fv_batch_last_error", "fv_device_bytes",
+           "fv_server_create", "fv_server_submit", "fv_server_step", "fv_server_poll", "fv_server_result",
+           "fv_server_get_stats", "fv_server_destroy", "fv_server_last_error"]
+
+
+class FvServerOpts(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_double), ("round_steps", ctypes.c_int), ("autotune", ctypes.c_int)]
+
+
+class FvRequestInfo(ctypes.Structure):
+    _fields_ = [("state", ctypes.c_int), ("steps", ctypes.c_long), ("t_submit", ctypes.c_double),
+                ("t_admit", ctypes.c_double), ("t_done", ctypes.c_double), ("device_bytes", ctypes.c_double)]
+
+
+class FvServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_int), ("completed", ctypes.c_int), ("failed", ctypes.c_int),
+                ("running", ctypes.c_int), ("queued", ctypes.c_int), ("peak_running", ctypes.c_int),
+                ("device_bytes", ctypes.c_double), ("peak_device_bytes", ctypes.c_double),
+                ("rounds", ctypes.c_long), ("busy_ms", ctypes.c_double), ("clock", ctypes.c_double)]
 
 
 class FvBatchOpts(ctypes.Structure):
@@ -124,6 +142,16 @@ def load_library():
     L.fv_batch_destroy.argtypes = [V]
     L.fv_batch_last_error.argtypes = [V]
     L.fv_batch_last_error.restype = ctypes.c_char_p
+    L.fv_device_bytes.argtypes = [ctypes.POINTER(FvConfig), P]
+    L.fv_server_create.argtypes = [ctypes.POINTER(V), ctypes.POINTER(FvServerOpts)]
+    L.fv_server_submit.argtypes = [V, ctypes.POINTER(FvConfig), P, P, P, P, ctypes.POINTER(ctypes.c_int)]
+    L.fv_server_step.argtypes = [V, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    L.fv_server_poll.argtypes = [V, ctypes.c_int, ctypes.POINTER(FvRequestInfo)]
+    L.fv_server_result.argtypes = [V, ctypes.c_int, P, ctypes.POINTER(FvScalars)]
+    L.fv_server_get_stats.argtypes = [V, ctypes.POINTER(FvServerStats)]
+    L.fv_server_destroy.argtypes = [V]
+    L.fv_server_last_error.argtypes = [V]
+    L.fv_server_last_error.restype = ctypes.c_char_p
     _lib = L
     return L
 
@@ -286,6 +314,86 @@ class Batch:
             self.L.fv_batch_destroy(self._h)
             self._h = None
             self.members = []
+
+    def __del__(self):
+        self.close()
+
+
+def device_bytes(cfg):
+    """Device memory a run with this configuration holds (fv_device_bytes)."""
+    L = load_library()
+    b = ctypes.c_double()
+    if L.fv_device_bytes(ctypes.byref(FvConfig.from_dict(cfg)), ctypes.byref(b)):
+        raise ValueError("unsupported configuration")
+    return b.value
+
+
+class Server:
+    """Continuous batching of runs to their stop times (fv_server_*): runs are
+    admitted in arrival order while they fit the device memory budget, stay
+    resident, and advance in the FP32 and FP64 batches, round by round."""
+
+    STATES = {0: "queued", 1: "running", 2: "done", 3: "failed", 4: "taken"}
+
+    def __init__(self, hbm_budget_bytes, round_steps=64, autotune=True):
+        self.L = load_library()
+        o = FvServerOpts(float(hbm_budget_bytes), int(round_steps), int(bool(autotune)))
+        h = ctypes.c_void_p()
+        rc = self.L.fv_server_create(ctypes.byref(h), ctypes.byref(o))
+        if rc:
+            raise RuntimeError("fv_server_create failed (%d); see stderr" % rc)
+        self._h = h
+        self.shapes = {}
+
+    def _chk(self, rc, what):
+        if rc:
+            raise RuntimeError("%s failed (%d): %s" % (what, rc, self.L.fv_server_last_error(self._h).decode()))
+
+    def submit(self, cfg, init):
+        """Queue a run of cfg (to cfg["stoptime"]) from initial data init (as
+        returned by setups.initial); returns the request id."""
+        cfg = setups.full(cfg)
+        c = FvConfig.from_dict(cfg)
+        arrs = []
+        for key in ("pvar", "bccsound", "fcsound", "w"):
+            a = init.get(key)
+            if key == "w" and cfg.get("fargo", 0) != 2:
+                a = None
+            arrs.append(None if a is None else np.ascontiguousarray(a, dtype=np.float64))
+        ptrs = [None if a is None else _ptr(a) for a in arrs]
+        i = ctypes.c_int()
+        self._chk(self.L.fv_server_submit(self._h, ctypes.byref(c), *ptrs, ctypes.byref(i)), "fv_server_submit")
+        self.shapes[i.value] = (4 if cfg["physics"] == setups.EULER else 3, cfg["jnum"] + 4, cfg["inum"] + 4)
+        return i.value
+
+    def step(self):
+        """One scheduling round; returns (running, queued)."""
+        r, q = ctypes.c_int(), ctypes.c_int()
+        self._chk(self.L.fv_server_step(self._h, ctypes.byref(r), ctypes.byref(q)), "fv_server_step")
+        return r.value, q.value
+
+    def poll(self, rid):
+        o = FvRequestInfo()
+        self._chk(self.L.fv_server_poll(self._h, int(rid), ctypes.byref(o)), "fv_server_poll")
+        d = {k: getattr(o, k) for k, _ in FvRequestInfo._fields_}
+        d["state"] = self.STATES[d["state"]]
+        return d
+
+    def result(self, rid):
+        cv = np.empty(self.shapes[rid])
+        s = FvScalars()
+        self._chk(self.L.fv_server_result(self._h, int(rid), _ptr(cv), ctypes.byref(s)), "fv_server_result")
+        return cv, {k: getattr(s, k) for k, _ in FvScalars._fields_}
+
+    def stats(self):
+        o = FvServerStats()
+        self._chk(self.L.fv_server_get_stats(self._h, ctypes.byref(o)), "fv_server_get_stats")
+        return {k: getattr(o, k) for k, _ in FvServerStats._fields_}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.L.fv_server_destroy(self._h)
+            self._h = None
 
     def __del__(self):
         self.close()

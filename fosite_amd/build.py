@@ -15,7 +15,7 @@ SRC = [os.path.join(HERE, "csrc", f) for f in (
     "fv_inst_f64_single_s3.hip", "fv_inst_f64_single_s7.hip", "fv_inst_f32_single_s3.hip",
     "fv_inst_f32_single_s7.hip", "fv_inst_f64_multi_s3.hip", "fv_inst_f64_multi_s7.hip",
     "fv_inst_f32_multi_s3.hip", "fv_inst_f32_multi_s7.hip",
-    "fv_kernels.hip", "fv_api.cpp", "fv_mesh.cpp")]
+    "fv_kernels.hip", "fv_api.cpp", "fv_mesh.cpp", "fv_server.cpp")]
 OUT = os.path.join(HERE, "libfosite_fv.so")
 OBJ = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -106,6 +106,7 @@ struct fv_ctx {
   Mesh mesh;
   std::vector<double> ring_h, col_h;
   double *d_ring = nullptr, *d_col = nullptr;
+  float *d_ringf = nullptr, *d_colf = nullptr;  // FP32 copies (dtype 1)
   double *buf[NBUF] = {};
   int cvar = 0, cold = 0;
   double *k[MAXK] = {};     // stage rhs k_2..k_s (k[0] unused)
@@ -446,8 +447,20 @@ int integration_step(fv_ctx *ctx) {
 
 int upload_geom(fv_ctx *ctx) {
   const int NI = ctx->NI, NJ = ctx->NJ;
-  HIPCHK(hipMemcpy(ctx->d_ring, ctx->ring_h.data(), sizeof(double) * R_COUNT * NI, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(ctx->d_col, ctx->col_h.data(), sizeof(double) * C_COUNT * NJ, hipMemcpyHostToDevice));
+  // copies on the context's own (non-blocking) stream: no null-stream
+  // synchronisation with work other contexts have in flight
+  HIPCHK(hipMemcpyAsync(ctx->d_ring, ctx->ring_h.data(), sizeof(double) * R_COUNT * NI, hipMemcpyHostToDevice,
+                        ctx->st));
+  HIPCHK(hipMemcpyAsync(ctx->d_col, ctx->col_h.data(), sizeof(double) * C_COUNT * NJ, hipMemcpyHostToDevice,
+                        ctx->st));
+  std::vector<float> rf, cf;
+  if (ctx->d_ringf) {
+    rf.assign(ctx->ring_h.begin(), ctx->ring_h.end());
+    cf.assign(ctx->col_h.begin(), ctx->col_h.end());
+    HIPCHK(hipMemcpyAsync(ctx->d_ringf, rf.data(), sizeof(float) * R_COUNT * NI, hipMemcpyHostToDevice, ctx->st));
+    HIPCHK(hipMemcpyAsync(ctx->d_colf, cf.data(), sizeof(float) * C_COUNT * NJ, hipMemcpyHostToDevice, ctx->st));
+  }
+  HIPCHK(hipStreamSynchronize(ctx->st));
   return 0;
 }
 
@@ -528,11 +541,19 @@ void enqueue_fast_step(fv_ctx *ctx, int xin, int xout, int tb, int rb, int k0in,
   prof_end(ctx, pk, K_CONTROL, 32.0 * ctx->nb_rhs);
 }
 
-// step-control header of a device-controlled batch of at most nmax steps
+// step-control header of a device-controlled batch of at most nmax steps.
+// ctx->dt is the CFL step of the last CalcTimestep; the first device step
+// takes it after the output-time clipping the host applies at the start of
+// every Step (io/fileio_base.f90:604-620), as k_control does for the others.
 void prime_ctl(const fv_ctx *ctx, Ctl *h, int nmax) {
   std::memset(h, 0, offsetof(Ctl, err_hist));
+  double dt = ctx->dt;
+  if ((ctx->time + dt) / ctx->t_out > 1.0)
+    dt = ctx->t_out - ctx->time;
+  else if ((ctx->time + 1.5 * dt) / ctx->t_out > 1.0)
+    dt = 0.5 * (ctx->t_out - ctx->time);
   h->time = ctx->time;
-  h->dt = ctx->dt;
+  h->dt = dt;
   h->t_out = ctx->t_out;
   h->stoptime = ctx->c.stoptime;
   h->cfl = ctx->c.cfl;
@@ -689,12 +710,19 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
   // device resources
   const size_t sbytes = elem_bytes(ctx) * ctx->V * ctx->FS;
   auto dalloc = [&](double **p, size_t b) { return hipMalloc((void **)p, b) == hipSuccess; };
-  bool ok = hipStreamCreate(&ctx->st) == hipSuccess && hipEventCreate(&ctx->e0) == hipSuccess &&
+  bool ok = hipStreamCreateWithFlags(&ctx->st, hipStreamNonBlocking) == hipSuccess &&
+            hipEventCreate(&ctx->e0) == hipSuccess &&
             hipEventCreate(&ctx->e1) == hipSuccess;
-  for (int q = 0; q < NBUF && ok; q++) ok = dalloc(&ctx->buf[q], sbytes) && hipMemset(ctx->buf[q], 0, sbytes) == hipSuccess;
-  for (int q = 1; q < ctx->tab.m && ok; q++) ok = dalloc(&ctx->k[q], sbytes) && hipMemset(ctx->k[q], 0, sbytes) == hipSuccess;
-  for (int q = 0; q < 2 && ok; q++) ok = dalloc(&ctx->k0[q], sbytes) && hipMemset(ctx->k0[q], 0, sbytes) == hipSuccess;
+  auto zalloc = [&](double **p) {
+    return dalloc(p, sbytes) && hipMemsetAsync(*p, 0, sbytes, ctx->st) == hipSuccess;
+  };
+  for (int q = 0; q < NBUF && ok; q++) ok = zalloc(&ctx->buf[q]);
+  for (int q = 1; q < ctx->tab.m && ok; q++) ok = zalloc(&ctx->k[q]);
+  for (int q = 0; q < 2 && ok; q++) ok = zalloc(&ctx->k0[q]);
   ok = ok && dalloc(&ctx->d_ring, sizeof(double) * R_COUNT * ctx->NI) && dalloc(&ctx->d_col, sizeof(double) * C_COUNT * ctx->NJ);
+  if (c.dtype == 1)
+    ok = ok && hipMalloc((void **)&ctx->d_ringf, sizeof(float) * R_COUNT * ctx->NI) == hipSuccess &&
+         hipMalloc((void **)&ctx->d_colf, sizeof(float) * C_COUNT * ctx->NJ) == hipSuccess;
   Geom &g = ctx->g;
   g.NI = ctx->NI; g.NJ = ctx->NJ; g.P = ctx->P; g.FS = ctx->FS;
   g.inum = c.inum; g.jnum = c.jnum;
@@ -713,7 +741,8 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
   const int nb_part = std::max(ctx->nb_rhs, rhs_blocks(g32));
   ok = ok && dalloc(&ctx->part_r, sizeof(double) * 2 * nb_part) &&
        dalloc(&ctx->part_c, sizeof(double) * 2 * nb_part) && dalloc(&ctx->d_red, sizeof(double) * 2);
-  ok = ok && hipMalloc((void **)&ctx->d_ctl, sizeof(Ctl)) == hipSuccess && hipMemset(ctx->d_ctl, 0, sizeof(Ctl)) == hipSuccess;
+  ok = ok && hipMalloc((void **)&ctx->d_ctl, sizeof(Ctl)) == hipSuccess &&
+       hipMemsetAsync(ctx->d_ctl, 0, sizeof(Ctl), ctx->st) == hipSuccess && hipStreamSynchronize(ctx->st) == hipSuccess;
   ok = ok && hipHostMalloc((void **)&ctx->h_red, sizeof(double) * 2) == hipSuccess;
   ok = ok && hipHostMalloc((void **)&ctx->h_ctl, sizeof(Ctl)) == hipSuccess;
   if (!ok) {
@@ -723,6 +752,8 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
   }
   g.ring = ctx->d_ring;
   g.col = ctx->d_col;
+  g.ringf = ctx->d_ringf;
+  g.colf = ctx->d_colf;
   if (upload_geom(ctx)) { fv_destroy(ctx); return -3; }
   // timedisc_base.f90:315-352 defaults
   ctx->time = 0.0;
@@ -749,6 +780,8 @@ extern "C" int fv_destroy(fv_ctx *ctx) {
   if (ctx->d_ctl) (void)hipFree(ctx->d_ctl);
   if (ctx->d_ring) (void)hipFree(ctx->d_ring);
   if (ctx->d_col) (void)hipFree(ctx->d_col);
+  if (ctx->d_ringf) (void)hipFree(ctx->d_ringf);
+  if (ctx->d_colf) (void)hipFree(ctx->d_colf);
   if (ctx->h_red) (void)hipHostFree(ctx->h_red);
   if (ctx->h_ctl) (void)hipHostFree(ctx->h_ctl);
   for (auto e : ctx->evpool) (void)hipEventDestroy(e);
@@ -756,6 +789,24 @@ extern "C" int fv_destroy(fv_ctx *ctx) {
   if (ctx->e1) (void)hipEventDestroy(ctx->e1);
   if (ctx->st) (void)hipStreamDestroy(ctx->st);
   delete ctx;
+  return 0;
+}
+
+extern "C" int fv_device_bytes(const fv_config *c, double *bytes) {
+  if (!c || !bytes) return -1;
+  Tableau t;
+  if (c->inum < 4 || c->jnum < 4 || (c->dtype != 0 && c->dtype != 1) || !set_tableau(c->method, c->order, t))
+    return -1;
+  const double NI = c->inum + 4.0, NJ = c->jnum + 4.0, P = std::ceil(NI / 16.0) * 16.0;
+  const double V = c->physics == EULER ? 4.0 : 3.0, es = c->dtype == 1 ? 4.0 : 8.0;
+  const double state = es * V * NJ * P;  // one state buffer, ghost cells and row padding included
+  const double nb32 = std::ceil(c->inum / 32.0) * std::ceil(c->jnum / (double)TY_TILE);
+  double b = state * (NBUF + (t.m - 1) + 2);          // states, stage rhs, coeff(1) pair
+  b += 2.0 * 2.0 * 8.0 * nb32;                         // CFL and error partials
+  b += 8.0 * (R_COUNT * NI + C_COUNT * NJ);             // metric arrays
+  if (c->dtype == 1) b += 4.0 * (R_COUNT * NI + C_COUNT * NJ);
+  b += sizeof(Ctl) + 16.0;
+  *bytes = b;
   return 0;
 }
 
@@ -811,8 +862,9 @@ extern "C" int fv_set_pvar(fv_ctx *ctx, const double *pvar) {
     src = cf.data();
   }
   for (int l = 0; l < V; l++)
-    HIPCHK(hipMemcpy2D(field(ctx, ctx->buf[0], l), es * ctx->P, static_cast<const char *>(src) + l * N * es,
-                       es * ctx->NI, es * ctx->NI, ctx->NJ, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy2DAsync(field(ctx, ctx->buf[0], l), es * ctx->P, static_cast<const char *>(src) + l * N * es,
+                            es * ctx->NI, es * ctx->NI, ctx->NJ, hipMemcpyHostToDevice, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
   ctx->fl_pvar = ctx->fl_cvar = 0;
   ctx->have_state = true;
   return 0;
@@ -902,8 +954,9 @@ extern "C" int fv_get_state(fv_ctx *ctx, double *cvar, double *pvar) {
   char *dst = ctx->c.dtype == 1 ? reinterpret_cast<char *>(cf.data()) : reinterpret_cast<char *>(cv.data());
   HIPCHK(hipStreamSynchronize(ctx->st));
   for (int l = 0; l < V; l++)
-    HIPCHK(hipMemcpy2D(dst + l * N * es, es * ctx->NI, field(ctx, ctx->buf[ctx->cvar], l), es * ctx->P,
-                       es * ctx->NI, ctx->NJ, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy2DAsync(dst + l * N * es, es * ctx->NI, field(ctx, ctx->buf[ctx->cvar], l), es * ctx->P,
+                            es * ctx->NI, ctx->NJ, hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
   if (ctx->c.dtype == 1) std::copy(cf.begin(), cf.end(), cv.begin());
   if (cvar) std::memcpy(cvar, cv.data(), sizeof(double) * V * N);
   if (pvar) {
@@ -1068,7 +1121,7 @@ int rebuild(fv_batch *b) {
     BCHK(hipHostMalloc((void **)&b->h_rhs, sizeof(RhsEntry) * 2 * m * cap));
     BCHK(hipHostMalloc((void **)&b->h_remap, sizeof(RemapEntry) * 2 * cap));
     BCHK(hipHostMalloc((void **)&b->h_cent, sizeof(ControlEntry) * NCAND * cap));
-    BCHK(hipMemset(b->d_ctl, 0, sizeof(Ctl) * cap));
+    BCHK(hipMemsetAsync(b->d_ctl, 0, sizeof(Ctl) * cap, b->st));
     b->cap = cap;
   }
   std::vector<unsigned> all;
@@ -1094,7 +1147,9 @@ int rebuild(fv_batch *b) {
     BCHK(hipMalloc((void **)&b->d_tiles, sizeof(unsigned) * all.size()));
     b->tiles_cap = all.size();
   }
-  if (!all.empty()) BCHK(hipMemcpy(b->d_tiles, all.data(), sizeof(unsigned) * all.size(), hipMemcpyHostToDevice));
+  if (!all.empty())
+    BCHK(hipMemcpyAsync(b->d_tiles, all.data(), sizeof(unsigned) * all.size(), hipMemcpyHostToDevice, b->st));
+  BCHK(hipStreamSynchronize(b->st));
   b->dirty = false;
   return 0;
 }
@@ -1227,7 +1282,7 @@ extern "C" int fv_batch_create(fv_batch **out, const fv_batch_opts *opts) {
   }
   if (b->opt.max_graph_steps == 0) b->opt.max_graph_steps = CTL_HIST;
   b->cand = b->opt.tiling;
-  if (hipStreamCreate(&b->st) != hipSuccess || hipEventCreate(&b->e0) != hipSuccess ||
+  if (hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&b->e0) != hipSuccess ||
       hipEventCreate(&b->e1) != hipSuccess || hipEventCreate(&b->t0) != hipSuccess ||
       hipEventCreate(&b->t1) != hipSuccess) {
     fv_batch_destroy(b);

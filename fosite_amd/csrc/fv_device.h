@@ -39,6 +39,8 @@ struct Geom {
   int dtype;            // stored state: 0 FP64 (plain drop-in), 1 FP32 (precision form)
   const double *ring;   // [R_COUNT][NI]
   const double *col;    // [C_COUNT][NJ]
+  const float *ringf;   // FP32 copies read by the FP32 form (dtype 1)
+  const float *colf;
 };
 
 // Device-side step control (one per context).  In a device-controlled batch

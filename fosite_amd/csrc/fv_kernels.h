@@ -6,7 +6,8 @@
 //               -ffp-contract=off, so IEEE +,-,*,/ and sqrt give the
 //               reference's bits.
 //   T = float   the precision form: reconstruction, fluxes, sources and the
-//               stage arithmetic in FP32 (fast division and square root),
+//               stage arithmetic in FP32 (hardware reciprocal and square
+//               root, FP32 copies of the metric arrays),
 //               the RK stage sums and the final combination with compensated
 //               (two-sum / two-product) accumulation, the embedded-error norm
 //               in FP64; CFL partials are reduced in FP64 by the host path or
@@ -62,7 +63,7 @@ template <typename T>
 __device__ __forceinline__ bool bad(T x) { return (x != x) || (x > Real<T>::huge); }
 // division, square root, log, exp: IEEE in FP64; hardware-rate in FP32
 __device__ __forceinline__ double dv(double a, double b) { return a / b; }
-__device__ __forceinline__ float dv(float a, float b) { return __fdividef(a, b); }
+__device__ __forceinline__ float dv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 __device__ __forceinline__ double sq(double a) { return sqrt(a); }
 __device__ __forceinline__ float sq(float a) { return __builtin_amdgcn_sqrtf(a); }
 __device__ __forceinline__ double lg(double a) { return log(a); }
@@ -92,10 +93,23 @@ struct Acc {
   __device__ __forceinline__ double wide() const { return (double)hi + (double)lo; }
 };
 
-// per-ring arrays (FP64 in memory): uniform base per id, 32-bit lane offset
+// per-ring and per-column metric arrays: uniform base per id, 32-bit lane
+// offset; the FP32 form reads FP32 copies
 __device__ __forceinline__ double ring(const Geom &g, int id, int i) { return (g.ring + id * g.NI)[(unsigned)i]; }
 template <typename T>
-__device__ __forceinline__ T rg(const Geom &g, int id, int i) { return (T)ring(g, id, i); }
+__device__ __forceinline__ T rg(const Geom &g, int id, int i) {
+  if constexpr (sizeof(T) == 4)
+    return (g.ringf + id * g.NI)[(unsigned)i];
+  else
+    return ring(g, id, i);
+}
+template <typename T>
+__device__ __forceinline__ T cl(const Geom &g, int id, int j) {
+  if constexpr (sizeof(T) == 4)
+    return (g.colf + id * g.NJ)[(unsigned)j];
+  else
+    return (g.col + id * g.NJ)[(unsigned)j];
+}
 __device__ __forceinline__ long cidx(const Geom &g, int i, int j) { return (long)j * g.P + i; }
 
 // ---------------------------------------------------------------- physics
@@ -194,21 +208,28 @@ struct In {
   double dt;
   T gam;
   int nk;
-  double c[sizeof(T) == 8 ? KM : 1];  // FP64: a(m,q)*dt, formed once
+  // stage coefficients a(m,q)*dt, formed once: FP64, or FP32 hi/lo pairs
+  double c[sizeof(T) == 8 ? KM : 1];
+  float ch[sizeof(T) == 4 ? KM : 1], clo[sizeof(T) == 4 ? KM : 1];
 
   __forceinline__ __device__ In(const Geom &g_, const StageArgsT<T> &a_) : g(g_), a(a_) {
     dt = a.ctl ? a.ctl->dt : a.dt;  // device-controlled: this step's dt
     nk = a.nk;
     gam = (T)g.gamma;
-    if constexpr (sizeof(T) == 8) {
 #pragma unroll
-      for (int q = 0; q < KM; q++) c[q] = a.a[q] * dt;
+    for (int q = 0; q < KM; q++) {
+      const double cq = a.a[q] * dt;
+      if constexpr (sizeof(T) == 8) {
+        c[q] = cq;
+      } else {
+        ch[q] = (float)cq;
+        clo[q] = (float)(cq - (double)ch[q]);
+      }
     }
   }
 
   // s = Y - sum_q (a_q*dt) k_q for one cell from loaded values, in the
-  // reference's order (FP64) or compensated (FP32; the coefficients are then
-  // split into hi/lo parts at their use, so no pair stays live).
+  // reference's order (FP64) or compensated (FP32).
   __forceinline__ __device__ void stage_sum(const T *y, const T (*kv)[V], T *s) const {
     if constexpr (sizeof(T) == 8) {
 #pragma unroll
@@ -225,11 +246,7 @@ struct In {
         Acc acc((float)y[l]);
 #pragma unroll
         for (int q = 0; q < KM; q++)
-          if (q < nk) {
-            const double cq = a.a[q] * dt;
-            const float ch = (float)cq;
-            acc.sub(ch, (float)(cq - (double)ch), (float)kv[q][l]);
-          }
+          if (q < nk) acc.sub(ch[q], clo[q], (float)kv[q][l]);
         s[l] = (T)acc.value();
       }
     }
@@ -751,7 +768,7 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
   T uc[V], fe[V], fn[V];
   {
     const T dw = rg<T>(g, R_RDXW, ir), de = rg<T>(g, R_RDXE, ir);
-    const T ds = (T)g.col[C_RDYS * g.NJ + jr], dn = (T)g.col[C_RDYN * g.NJ + jr];
+    const T ds = cl<T>(g, C_RDYS, jr), dn = cl<T>(g, C_RDYN, jr);
 #pragma unroll
     for (int l = 0; l < V; l++) {
       uc[l] = UC(l, ty, tx);
@@ -769,7 +786,7 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
     const int xk = t / TX;  // 0: row -1 (N face), 1: row TY (S face)
     const int xr = xk ? TY : -1, xc = tx;
     const int jj = j0 + xr, jc = jj < g.NJ ? jj : g.NJ - 1;
-    const T off = (T)(xk ? g.col[C_RDYS * g.NJ + jc] : g.col[C_RDYN * g.NJ + jc]);
+    const T off = xk ? cl<T>(g, C_RDYS, jc) : cl<T>(g, C_RDYN, jc);
 #pragma unroll
     for (int l = 0; l < V; l++) {
       const T c0 = UC(l, xr, xc);

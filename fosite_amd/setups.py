@@ -197,6 +197,28 @@ def batch_disks(n=64, seed=20261204, sizes=(128, 256, 384, 512), dtype="fp32", s
     return out
 
 
+# ----------------------------------------------------------------- config 5
+
+def serving_requests(n=2000, rate=20.0, seed=20261205, nmin=64, nmax=1024, orbits=10.0, fp32_share=0.75,
+                     cs_ratio=0.05):
+    """Disk runs for the serving loop: Poisson arrivals (mean rate per second),
+    radial cells uniform in [nmin, nmax] (azimuthal = 2x), FP32 with
+    probability fp32_share else FP64, surface-density slope U(-1.5, -0.5),
+    1% seeded noise, FARGO, SSPRK3, integrated for `orbits` orbits of the
+    inner edge (r = 1).  Returns [(arrival_s, cfg), ...] in arrival order."""
+    u = splitmix64_uniform(seed, 4 * n)
+    t, out = 0.0, []
+    for k in range(n):
+        t += -math.log1p(-float(u[4 * k])) / rate
+        ni = nmin + min(int(u[4 * k + 1] * (nmax - nmin + 1)), nmax - nmin)
+        cfg = kepler_disk_config(ni=ni, nj=2 * ni, cs_ratio=cs_ratio)
+        cfg.update(stoptime=orbits * 2.0 * math.pi, seed=seed + 1 + k,
+                   dtype="fp32" if u[4 * k + 2] < fp32_share else "fp64",
+                   sigma_slope=-1.5 + float(u[4 * k + 3]))
+        out.append((t, cfg))
+    return out
+
+
 def initial(cfg):
     if cfg.get("problem") == "euler_disk":
         return euler_disk_initial(cfg)

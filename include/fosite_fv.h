@@ -73,6 +73,9 @@ int fv_create(fv_ctx **ctx, const fv_config *cfg);
 int fv_destroy(fv_ctx *ctx);
 /* ghost-inclusive cell counts and number of conservative fields */
 int fv_dims(const fv_ctx *ctx, int *ni, int *nj, int *vnum);
+/* Device memory a context with this configuration allocates, in bytes
+ * (no reference counterpart; used for admission by memory budget). */
+int fv_device_bytes(const fv_config *cfg, double *bytes);
 
 /* physics_eulerisotherm%SetSoundSpeeds_center / _faces
  * (physics/physics_eulerisotherm.f90:288-309); isothermal physics only.
@@ -164,6 +167,50 @@ int fv_batch_run(fv_batch *b, int nsteps, int *taken);
 int fv_batch_get_info(const fv_batch *b, fv_batch_info *out);
 int fv_batch_destroy(fv_batch *b);
 const char *fv_batch_last_error(const fv_batch *b);
+
+/* ------------------------------------------------------------------ serving
+ * No reference counterpart.  A server runs many simulations to their stop
+ * times by continuous batching: requests queue on the host and are admitted
+ * in arrival order while their device footprint (fv_device_bytes) fits the
+ * memory budget; admitted runs stay on the device and join the batch of
+ * their dtype.  fv_server_step is one scheduling round: both batches advance
+ * by up to round_steps steps (concurrently, from two host threads), queued
+ * requests that fit are prepared meanwhile (context, initial data, first
+ * step), then finished runs are copied back and released and the prepared
+ * ones join.  A served result is bit-identical to the same run made alone.
+ * Times are seconds on the server's steady clock (fv_server_get_stats). */
+typedef struct fv_server fv_server;
+typedef struct fv_server_opts {
+  double hbm_budget_bytes; /* device memory all admitted runs together may hold */
+  int round_steps;         /* steps per batch round, 1..512 */
+  int autotune;            /* batches choose their tiling by timing */
+} fv_server_opts;
+typedef struct fv_request_info {
+  int state;               /* 0 queued, 1 running, 2 done, 3 failed, 4 result taken */
+  long steps;              /* steps taken in batch rounds (the first step is extra) */
+  double t_submit, t_admit, t_done, device_bytes;
+} fv_request_info;
+typedef struct fv_server_stats {
+  int submitted, completed, failed, running, queued, peak_running;
+  double device_bytes, peak_device_bytes;
+  long rounds;
+  double busy_ms;          /* summed wall time of the batch rounds */
+  double clock;            /* current server time, s */
+} fv_server_stats;
+int fv_server_create(fv_server **s, const fv_server_opts *opts);
+/* queue one run: configuration (its stoptime ends the run) and initial data
+ * as for fv_set_pvar / fv_set_sound_speeds / fv_set_background_velocity
+ * (bccsound, fcsound, w may be NULL); *id numbers requests from 0. */
+int fv_server_submit(fv_server *s, const fv_config *cfg, const double *pvar, const double *bccsound,
+                     const double *fcsound, const double *w, int *id);
+int fv_server_step(fv_server *s, int *running, int *queued);
+int fv_server_poll(const fv_server *s, int id, fv_request_info *out);
+/* final conservative state (all cells) and scalars of a finished run; the
+ * result is released afterwards */
+int fv_server_result(fv_server *s, int id, double *cvar, fv_scalars *sc);
+int fv_server_get_stats(const fv_server *s, fv_server_stats *out);
+int fv_server_destroy(fv_server *s);
+const char *fv_server_last_error(const fv_server *s);
 
 #ifdef __cplusplus
 }

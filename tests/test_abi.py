@@ -77,3 +77,26 @@ def test_batch_create_rejects_bad_options():
     assert L.fv_batch_add(None, None) == -1
     assert L.fv_batch_remove(None, None) == -1
     assert L.fv_batch_destroy(None) == 0
+
+
+def test_server_create_rejects_bad_options():
+    L = fosite_amd.load_library()
+    h = ctypes.c_void_p()
+    for o in ((0.0, 16, 0), (1e9, 0, 0), (1e9, 513, 1)):
+        assert L.fv_server_create(ctypes.byref(h), ctypes.byref(fosite_amd.FvServerOpts(*o))) == -1, o
+        assert not h.value
+    assert L.fv_server_step(None, None, None) == -1
+    assert L.fv_server_destroy(None) == 0
+
+
+def test_device_bytes_accounts_for_states_and_dtype():
+    """Footprint: 4 state buffers + (stages-1) stage rhs + 2 coeff(1) buffers,
+    ghost-inclusive and row-padded, plus small per-run arrays."""
+    cfg = S.kepler_disk_config(ni=512, nj=1024)
+    b64 = fosite_amd.device_bytes(cfg)
+    b32 = fosite_amd.device_bytes(dict(cfg, dtype="fp32"))
+    state64 = 8 * 3 * (1024 + 4) * 528  # NJ x padded NI
+    assert 8 * state64 <= b64 <= 8 * state64 * 1.01   # SSPRK3: 4 + 2 + 2 buffers
+    assert 0.49 < b32 / b64 < 0.52
+    dp5 = fosite_amd.device_bytes(S.euler_disk_config(ni=512, nj=1024))
+    assert dp5 > 12 * 8 * 4 * 1028 * 528              # DP5: 4 + 6 + 2 buffers of 4 fields

@@ -157,3 +157,44 @@ def test_gpu_batch_member_matches_oracle():
     _, cv, sc = res[0]
     assert sc["iter"] == rs["iter"] and sc["n_adj"] == rs["n_adj"]
     assert (rel_l1(cv, rc) <= 1e-12).all(), rel_l1(cv, rc)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("rs", [1, 2, 3, 4])
+def test_gpu_short_rounds_across_the_output_time(rs):
+    """Rounds of rs device steps that end just before an output-clipped step:
+    the next round must start with the clipped step (as the host path does at
+    the start of every Step).  Batched members, and a single context run in
+    chunks of rs steps, both against the oracle run to the same stop time."""
+    cfgs = [dict(S.kepler_disk_config(ni=24, nj=48), stoptime=1.7, seed=21),
+            dict(S.kepler_disk_config(ni=40, nj=64, cs_ratio=0.04), stoptime=2.3, seed=22)]
+    sims = [make(c) for c in cfgs]
+    for s in sims:
+        s.first_step()
+    b = fosite_amd.Batch(sims, autotune=False, tiling=1, max_graph_steps=rs)
+    for _ in range(400):
+        if sum(b.run(rs)) == 0:
+            break
+    b.close()
+    for cfg, sim in zip(cfgs, sims):
+        init = S.initial(cfg)
+        o = oracle.Oracle(cfg)
+        o.set_initial(init["pvar"], init.get("bccsound"), init.get("fcsound"), init.get("w"))
+        o.first_step()
+        while not o.step():
+            pass
+        rc, _, rs_ = o.state()
+        cv, _ = sim.state()
+        sc = sim.scalars()
+        assert sc["iter"] == rs_["iter"] and sc["n_adj"] == rs_["n_adj"] and sc["brk"] == rs_["brk"], (sc, rs_)
+        assert abs(sc["time"] - rs_["time"]) <= 1e-12 * rs_["time"]
+        assert (rel_l1(cv, rc) <= 1e-12).all(), rel_l1(cv, rc)
+        # one context alone, device batches of rs steps
+        one = make(cfg)
+        one.first_step()
+        while one.run(rs) == rs:
+            pass
+        cv1, _ = one.state()
+        assert one.scalars() == sc and np.array_equal(cv1[I], cv[I])
+        one.close()
+        sim.close()
