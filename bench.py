@@ -248,11 +248,18 @@ def main():
     ap.add_argument("--no-forms", action="store_true",
                     help="skip the precision, batched and serving forms")
     ap.add_argument("--form-steps", type=int, default=500)
+    ap.add_argument("--only-form", choices=["precision", "batched", "serving"],
+                    help="run one form alone and print its object (for profiling)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU; the FV step shards by radial slabs "
               "(see DESIGN.md) but no multi-GPU run is part of it", file=sys.stderr)
         return 2
+    if a.only_form:
+        f = {"precision": lambda: precision_form(1024, 2048, a.form_steps), "batched": batched_form,
+             "serving": serving_form}[a.only_form]
+        print(json.dumps(f()))
+        return 0
 
     import fosite_amd
     from fosite_amd import setups
@@ -301,7 +308,7 @@ def main():
                    "sim_time_reached": sc["time"], "rejected_steps": sc["n_adj"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_BYTES_PER_LAUNCH,
-                     "kernel": "fv::k_rhs<3,BY,VANLEER> on a stage state (fused stage sum, ghost cells, RHS)",
+                     "kernel": "fv::k_rhs<double,3,VANLEER,64,3> on a stage state (fused stage sum, ghost cells, RHS)",
                      "launches": n_rhs, "mean_launch_us": 1e3 * ms_rhs / max(n_rhs, 1),
                      "alg_bytes_per_launch": bytes_rhs / max(n_rhs, 1),
                      "timing": "HIP events around each launch on the simulation stream, "
