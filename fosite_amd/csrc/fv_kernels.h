@@ -6,12 +6,11 @@
 //               -ffp-contract=off, so IEEE +,-,*,/ and sqrt give the
 //               reference's bits.
 //   T = float   the precision form: reconstruction, fluxes, sources and the
-//               stage arithmetic in FP32 (hardware reciprocal and square
-//               root, FP32 copies of the metric arrays),
-//               the RK stage sums and the final combination with compensated
-//               (two-sum / two-product) accumulation, the embedded-error norm
-//               in FP64; CFL partials are reduced in FP64 by the host path or
-//               k_control.
+//               stage states in FP32 (hardware reciprocal and square root,
+//               FP32 copies of the metric arrays); the new state of a step
+//               accumulated with compensated (two-sum / two-product)
+//               arithmetic; the embedded-error norm in FP64; CFL partials
+//               reduced in FP64 by the host path or k_control.
 //
 // Citations are file:line in the fosite source tree.
 //
@@ -208,9 +207,9 @@ struct In {
   double dt;
   T gam;
   int nk;
-  // stage coefficients a(m,q)*dt, formed once: FP64, or FP32 hi/lo pairs
+  // stage coefficients a(m,q)*dt, formed once
   double c[sizeof(T) == 8 ? KM : 1];
-  float ch[sizeof(T) == 4 ? KM : 1], clo[sizeof(T) == 4 ? KM : 1];
+  float ch[sizeof(T) == 4 ? KM : 1];
 
   __forceinline__ __device__ In(const Geom &g_, const StageArgsT<T> &a_) : g(g_), a(a_) {
     dt = a.ctl ? a.ctl->dt : a.dt;  // device-controlled: this step's dt
@@ -219,17 +218,15 @@ struct In {
 #pragma unroll
     for (int q = 0; q < KM; q++) {
       const double cq = a.a[q] * dt;
-      if constexpr (sizeof(T) == 8) {
+      if constexpr (sizeof(T) == 8)
         c[q] = cq;
-      } else {
+      else
         ch[q] = (float)cq;
-        clo[q] = (float)(cq - (double)ch[q]);
-      }
     }
   }
 
   // s = Y - sum_q (a_q*dt) k_q for one cell from loaded values, in the
-  // reference's order (FP64) or compensated (FP32).
+  // reference's order (FP64) or with FP32 fused multiply-adds.
   __forceinline__ __device__ void stage_sum(const T *y, const T (*kv)[V], T *s) const {
     if constexpr (sizeof(T) == 8) {
 #pragma unroll
@@ -241,13 +238,14 @@ struct In {
           for (int l = 0; l < V; l++) s[l] = s[l] - c[q] * kv[q][l];
         }
     } else {
+      // the stage state only feeds an FP32 RHS evaluation: plain FP32 sums
 #pragma unroll
       for (int l = 0; l < V; l++) {
-        Acc acc((float)y[l]);
+        float acc = (float)y[l];
 #pragma unroll
         for (int q = 0; q < KM; q++)
-          if (q < nk) acc.sub(ch[q], clo[q], (float)kv[q][l]);
-        s[l] = (T)acc.value();
+          if (q < nk) acc = fmaf(-ch[q], (float)kv[q][l], acc);
+        s[l] = (T)acc;
       }
     }
   }
@@ -560,6 +558,21 @@ static __device__ void control_next(Ctl *c, double invdt, int flags) {
   c->step = c->step + 1;
   if (c->step >= c->nsteps) halt(c, H_BATCH);
 }
+
+// ----------------------------------------------------------- tile orders
+// Workgroup b of a launch is dispatched to XCD b mod 8.  A 1-D grid of
+// 8*Q blocks (Q = ceil(tiles/8)) whose block b runs tile (b mod 8)*Q + b/8
+// gives every XCD a contiguous, row-major run of tiles, so tiles that share
+// halo rows share that XCD's L2.  Returns false for the padding blocks.
+constexpr int XCDS = 8;
+__device__ __forceinline__ bool xcd_tile(int b, int nbx, int nby, int &bx, int &by) {
+  const int n = nbx * nby, q = (n + XCDS - 1) / XCDS, t = (b % XCDS) * q + b / XCDS;
+  if (t >= n) return false;
+  bx = t % nbx;
+  by = t / nbx;
+  return true;
+}
+inline int xcd_grid(int nbx, int nby) { return ((nbx * nby + XCDS - 1) / XCDS) * XCDS; }
 
 // -------------------------------------------------------------- fused RHS
 // Tile: TX x TY_RHS output cells (TX = 64 or 32), one thread each.  The
@@ -883,11 +896,22 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
 #pragma unroll
           for (int l = 0; l < V; l++) kq[q][l] = (a.k[q] + l * g.FS)[ix];
         }
+      // FP32: weights b_h*dt as hi/lo pairs, and (b_h - b_l)*dt for the error
+      float bhh[KM], bhl[KM], bde[KM];
+      if constexpr (sizeof(T) == 4) {
+#pragma unroll
+        for (int q = 0; q < KM; q++) {
+          const double bh = a.bh[q] * in.dt;
+          bhh[q] = (float)bh;
+          bhl[q] = (float)(bh - (double)bhh[q]);
+          bde[q] = (float)((a.bh[q] - a.bl[q]) * in.dt);
+        }
+      }
 #pragma unroll
       for (int l = 0; l < V; l++) {
         // k_1 .. k_{s-1} from memory, then this stage's rhs as k_s (every
         // coefficient index is a compile-time constant)
-        double yh, yl;
+        double yh, dif;
         if constexpr (sizeof(T) == 8) {
           T h = y[l], w = y[l];
 #pragma unroll
@@ -899,23 +923,24 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
             }
           (a.ynew + l * g.FS)[ix] = h;
           yh = (double)h;
-          yl = (double)w;
+          dif = yh - (double)w;
         } else {
-          Acc h((float)y[l]), w((float)y[l]);
+          // compensated accumulation of the new state; the difference of the
+          // two embedded solutions summed directly (no cancellation)
+          Acc h((float)y[l]);
+          float e = 0.f;
 #pragma unroll
           for (int q = 0; q < KM; q++)
             if (q <= in.nk) {
               const float kk = (float)(q < in.nk ? kq[q][l] : rhs[l]);
-              const double bh = a.bh[q] * in.dt, bl = a.bl[q] * in.dt;
-              const float bhh = (float)bh, blh = (float)bl;
-              h.sub(bhh, (float)(bh - (double)bhh), kk);
-              w.sub(blh, (float)(bl - (double)blh), kk);
+              h.sub(bhh[q], bhl[q], kk);
+              e = fmaf(bde[q], kk, e);
             }
           (a.ynew + l * g.FS)[ix] = (T)h.value();
           yh = h.wide();
-          yl = w.wide();
+          dif = (double)e;
         }
-        err = fmax2(err, fabs(yh - yl) / (a.tol_rel * fabs(yh) + a.tol_abs[l]));
+        err = fmax2(err, fabs(dif) / (a.tol_rel * fabs(yh) + a.tol_abs[l]));
       }
     } else {
 #pragma unroll
@@ -970,7 +995,10 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
 
 template <typename T, int V, int LIM, int TX, int KM>
 __global__ __launch_bounds__(TX *TY_RHS, FV_RHS_WAVES_PER_SIMD) void k_rhs(Geom g, StageArgsT<T> a) {
-  rhs_tile<T, V, LIM, TX, KM>(g, a, blockIdx.x, blockIdx.y, gridDim.x, gridDim.y);
+  const int nbx = (g.inum + TX - 1) / TX, nby = (g.jnum + TY_RHS - 1) / TY_RHS;
+  int bx, by;
+  if (!xcd_tile(blockIdx.x, nbx, nby, bx, by)) return;
+  rhs_tile<T, V, LIM, TX, KM>(g, a, bx, by, nbx, nby);
 }
 
 // Many meshes in one launch: block b runs the tile tiles[b] (TileCode) of
@@ -1062,7 +1090,9 @@ __device__ __forceinline__ void remap_block(const Geom &g, const RemapArgsT<T> &
 
 template <typename T, int V>
 __global__ __launch_bounds__(256) void k_remap(Geom g, RemapArgsT<T> a) {
-  remap_block<T, V>(g, a, blockIdx.x, blockIdx.y);
+  int bx, by;
+  if (!xcd_tile(blockIdx.x, (g.inum + 63) / 64, (g.jnum + 3) / 4, bx, by)) return;
+  remap_block<T, V>(g, a, bx, by);
 }
 template <typename T, int V>
 __global__ __launch_bounds__(256) void k_remap_multi(const RemapEntry *__restrict__ tab,

@@ -7,7 +7,7 @@ namespace fv {
 
 template <typename T, int V, int LIM, int TX, int KM>
 static void rhs_lim(const Geom &g, const StageArgsT<T> &a, hipStream_t s) {
-  dim3 grid((g.inum + TX - 1) / TX, (g.jnum + TY_RHS - 1) / TY_RHS), block(TX, TY_RHS);
+  const dim3 grid(xcd_grid((g.inum + TX - 1) / TX, (g.jnum + TY_RHS - 1) / TY_RHS)), block(TX, TY_RHS);
   hipLaunchKernelGGL((k_rhs<T, V, LIM, TX, KM>), grid, block, 0, s, g, a);
 }
 template <typename T, int V, int TX, int KM>
@@ -32,7 +32,7 @@ void rhs_single(const Geom &g, const StageArgsT<T> &a, int V, hipStream_t s) {
 }
 template <typename T>
 void remap_single(const Geom &g, const RemapArgsT<T> &a, int V, hipStream_t s) {
-  dim3 grid((g.inum + 63) / 64, (g.jnum + 3) / 4), block(64, 4);
+  const dim3 grid(xcd_grid((g.inum + 63) / 64, (g.jnum + 3) / 4)), block(64, 4);
   if (V == 3)
     hipLaunchKernelGGL((k_remap<T, 3>), grid, block, 0, s, g, a);
   else
