@@ -65,7 +65,8 @@ class FvScalars(ctypes.Structure):
 # every entry point of include/fosite_fv.h
 EXPORTS = ["fv_mesh_array", "fv_create", "fv_destroy", "fv_dims", "fv_set_sound_speeds",
            "fv_set_background_velocity", "fv_set_pvar", "fv_first_step", "fv_step", "fv_run",
-           "fv_get_state", "fv_get_scalars", "fv_last_run_ms", "fv_last_error", "fv_profile",
+           "fv_get_state", "fv_get_scalars", "fv_get_boundary_flux", "fv_get_boundary_flux_total",
+           "fv_last_run_ms", "fv_last_error", "fv_profile",
            "fv_kernel_stats", "fv_batch_create", "fv_batch_add", "fv_batch_remove", "fv_batch_run",
            "fv_batch_get_info", "fv_batch_destroy", "fv_batch_last_error", "fv_device_bytes",
            "fv_server_create", "fv_server_submit", "fv_server_step", "fv_server_poll", "fv_server_result",
@@ -127,6 +128,8 @@ def load_library():
     L.fv_run.argtypes = [V, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     L.fv_get_state.argtypes = [V, P, P]
     L.fv_get_scalars.argtypes = [V, ctypes.POINTER(FvScalars)]
+    L.fv_get_boundary_flux.argtypes = [V, P, P]
+    L.fv_get_boundary_flux_total.argtypes = [V, ctypes.c_int, P]
     L.fv_last_run_ms.argtypes = [V]
     L.fv_last_run_ms.restype = ctypes.c_double
     L.fv_last_error.argtypes = [V]
@@ -253,6 +256,25 @@ class Fosite:
         pv = np.empty((self.V, self.NJ, self.NI))
         self._chk(self.L.fv_get_state(self._h, _ptr(cv), _ptr(pv)), "fv_get_state")
         return cv, pv
+
+    def boundary_flux(self):
+        """Fluxes%bxflux / byflux (fluxes_base.f90:142-143): bx[l, side, j]
+        (V x 2 x NJ) and by[l, side, i] (V x 2 x NI), side 0 west/south,
+        1 east/north."""
+        bx = np.empty((self.V, 2, self.NJ))
+        by = np.empty((self.V, 2, self.NI))
+        self._chk(self.L.fv_get_boundary_flux(self._h, _ptr(bx), _ptr(by)), "fv_get_boundary_flux")
+        return bx, by
+
+    WEST, EAST, SOUTH, NORTH = 1, 2, 3, 4
+
+    def boundary_flux_total(self, direction):
+        """Fluxes%GetBoundaryFlux (fluxes_base.f90:220-300): VNUM sums over
+        one boundary; direction WEST=1, EAST=2, SOUTH=3, NORTH=4."""
+        out = np.empty(self.V)
+        self._chk(self.L.fv_get_boundary_flux_total(self._h, int(direction), _ptr(out)),
+                  "fv_get_boundary_flux_total")
+        return out
 
     def scalars(self):
         s = FvScalars()

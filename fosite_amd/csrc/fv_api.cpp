@@ -112,8 +112,13 @@ struct fv_ctx {
   double *k[MAXK] = {};     // stage rhs k_2..k_s (k[0] unused)
   double *k0[2] = {};       // coeff(1) = Timedisc%rhs, double-buffered for rollback
   int k0cur = 0;
+  // boundary fluxes Fluxes%bxflux/byflux (fluxes_base.f90:142-143): bf[bcur]
+  // holds the accepted values (bxfold), the last stage of a step writes the
+  // other copy; AcceptSolution is a role swap, RejectSolution keeps bcur
+  double *bf[2] = {};
+  int bcur = 0;
   Ctl *d_ctl = nullptr, *h_ctl = nullptr;
-  struct GraphEntry { int x0, x1, t, k0, nsteps; hipGraphExec_t exec; };
+  struct GraphEntry { int x0, x1, t, k0, b0, nsteps; hipGraphExec_t exec; };
   std::vector<GraphEntry> graphs;
   bool fast = true;         // device-controlled batches allowed
   int batch_steps = 0, batch_halts = 0;
@@ -220,6 +225,7 @@ StageArgs base_args(fv_ctx *ctx) {
   StageArgs a;
   std::memset(&a, 0, sizeof(a));
   a.part = ctx->part_r;
+  a.bflux = 1;
   return a;
 }
 
@@ -284,7 +290,8 @@ int rhs_main(fv_ctx *ctx, int checkbm, int remap) {
 
 // arguments of stage m (1..s-1) on the state y with coeff(1) in k0[k0],
 // the last stage (m = s-1) fused with the final combination into buffer out
-StageArgs stage_args(fv_ctx *ctx, int m, int y, int k0, int out) {
+// and the boundary-flux accumulation bf[bin] -> bf[1 - bin]
+StageArgs stage_args(fv_ctx *ctx, int m, int y, int k0, int out, int bin) {
   StageArgs a = base_args(ctx);
   a.Y = ctx->buf[y];
   a.nk = m;
@@ -301,6 +308,8 @@ StageArgs stage_args(fv_ctx *ctx, int m, int y, int k0, int out) {
     a.tol_rel = ctx->c.tol_rel;
     for (int l = 0; l < 4; l++) a.tol_abs[l] = ctx->c.tol_abs[l];
     a.part_c = ctx->part_c;
+    a.bf_in = ctx->bf[bin];
+    a.bf_out = ctx->bf[1 - bin];
   } else {
     a.kout = ctx->k[m];
   }
@@ -317,7 +326,7 @@ void launch_stage(fv_ctx *ctx, const StageArgs &a, int m) {
 int rhs_stage(fv_ctx *ctx, int m, double dt, int out) {
   int add, sub;
   rhs_flags(ctx, ctx->fl_ptmp, ctx->fl_ctmp, add, sub);
-  StageArgs a = stage_args(ctx, m, ctx->cvar, ctx->k0cur, out);
+  StageArgs a = stage_args(ctx, m, ctx->cvar, ctx->k0cur, out, ctx->bcur);
   a.dt = dt;
   a.add_bg = add;
   a.sub_bg = sub;
@@ -349,8 +358,8 @@ double adjust_timestep(fv_ctx *ctx, double maxerr, double dtold) {
   return dtnew;
 }
 
-// timedisc_rkfehlberg.f90:170-268 (boundary-flux accounting omitted: it only
-// feeds diagnostics and accretion, neither enabled)
+// timedisc_rkfehlberg.f90:170-268; the boundary fluxes of the step go to
+// bf[1 - bcur] (the last stage accumulates them)
 int solve_ode(fv_ctx *ctx, double *dt, double *err) {
   const int outb = pick_free(ctx, ctx->cvar, ctx->cold, -1);
   for (int m = 1; m < ctx->tab.m; m++) {  // the last stage writes the combination
@@ -420,6 +429,7 @@ int integration_step(fv_ctx *ctx) {
       if (dt < ctx->dtmin) ctx->dtmin = dt;
     } else {  // AcceptSolution (timedisc_base.f90:911-959)
       time = time + dtold;
+      ctx->bcur = 1 - ctx->bcur;  // bxfold = bxflux
       if (time < ctx->time + ctx->dt) {
         if ((rc = rhs_main(ctx, CHECK_ALL, 0))) return rc;
         ctx->cold = ctx->cvar;
@@ -498,10 +508,11 @@ struct FastStep {
   RemapArgs rm;
   StageArgs end;       // end-of-step ComputeRHS: CFL, data checks, new state with ghosts
 };
-void fast_step_args(fv_ctx *ctx, Ctl *ctl, int xin, int xout, int tb, int rb, int k0in, int k0out, FastStep &f) {
+void fast_step_args(fv_ctx *ctx, Ctl *ctl, int xin, int xout, int tb, int rb, int k0in, int k0out, int bin,
+                    FastStep &f) {
   const int steady = ctx->c.fargo == 2 ? 1 : 0;
   for (int m = 1; m < ctx->tab.m; m++) {
-    StageArgs a = stage_args(ctx, m, xin, k0in, tb);
+    StageArgs a = stage_args(ctx, m, xin, k0in, tb, bin);
     a.ctl = ctl;
     a.add_bg = a.sub_bg = steady;
     f.st[m - 1] = a;
@@ -524,9 +535,9 @@ void fast_step_args(fv_ctx *ctx, Ctl *ctl, int xin, int xout, int tb, int rb, in
   f.end = a;
 }
 
-void enqueue_fast_step(fv_ctx *ctx, int xin, int xout, int tb, int rb, int k0in, int k0out) {
+void enqueue_fast_step(fv_ctx *ctx, int xin, int xout, int tb, int rb, int k0in, int k0out, int bin) {
   FastStep f;
-  fast_step_args(ctx, ctx->d_ctl, xin, xout, tb, rb, k0in, k0out, f);
+  fast_step_args(ctx, ctx->d_ctl, xin, xout, tb, rb, k0in, k0out, bin, f);
   for (int m = 1; m < ctx->tab.m; m++) launch_stage(ctx, f.st[m - 1], m);
   if (f.remap) {
     long pk = prof_begin(ctx);
@@ -582,6 +593,7 @@ int replay_device_steps(fv_ctx *ctx, const Ctl *h, int x0, int x1, int *halted) 
     ctx->dtcause = 0;
     ctx->cvar = ctx->cold = (s % 2 == 0) ? x1 : x0;
     ctx->k0cur = 1 - ctx->k0cur;
+    ctx->bcur = 1 - ctx->bcur;
   }
   if (k > 0) {  // CalcTimestep of the last accepted step (before output clipping)
     ctx->invdt = h->invdt;
@@ -593,21 +605,21 @@ int replay_device_steps(fv_ctx *ctx, const Ctl *h, int x0, int x1, int *halted) 
   return 0;
 }
 
-int get_graph(fv_ctx *ctx, int x0, int x1, int tb, int rb, int k0, hipGraphExec_t *out) {
+int get_graph(fv_ctx *ctx, int x0, int x1, int tb, int rb, int k0, int b0, hipGraphExec_t *out) {
   for (auto &e : ctx->graphs)
-    if (e.x0 == x0 && e.x1 == x1 && e.t == tb && e.k0 == k0) {
+    if (e.x0 == x0 && e.x1 == x1 && e.t == tb && e.k0 == k0 && e.b0 == b0) {
       *out = e.exec;
       return 0;
     }
   hipGraph_t graph;
   HIPCHK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
-  enqueue_fast_step(ctx, x0, x1, tb, rb, k0, 1 - k0);
-  enqueue_fast_step(ctx, x1, x0, tb, rb, 1 - k0, k0);
+  enqueue_fast_step(ctx, x0, x1, tb, rb, k0, 1 - k0, b0);
+  enqueue_fast_step(ctx, x1, x0, tb, rb, 1 - k0, k0, 1 - b0);
   HIPCHK(hipStreamEndCapture(ctx->st, &graph));
   hipGraphExec_t exec;
   HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
   (void)hipGraphDestroy(graph);
-  ctx->graphs.push_back({x0, x1, tb, k0, 2, exec});
+  ctx->graphs.push_back({x0, x1, tb, k0, b0, 2, exec});
   *out = exec;
   return 0;
 }
@@ -621,12 +633,12 @@ int run_batch(fv_ctx *ctx, int nmax, int *done_steps, int *halted) {
   const int x1 = pick_free(ctx, x0, -1, -1);
   const int tb = pick_free(ctx, x0, x1, -1);
   const int rb = pick_free(ctx, x0, x1, tb);
-  const int k0 = ctx->k0cur;
+  const int k0 = ctx->k0cur, b0 = ctx->bcur;
   Ctl *h = ctx->h_ctl;
   prime_ctl(ctx, h, nmax);
   hipGraphExec_t exec = nullptr;
   if (!ctx->prof) {
-    int rc = get_graph(ctx, x0, x1, tb, rb, k0, &exec);
+    int rc = get_graph(ctx, x0, x1, tb, rb, k0, b0, &exec);
     if (rc) return rc;
   }
   HIPCHK(hipMemcpyAsync(ctx->d_ctl, h, offsetof(Ctl, err_hist), hipMemcpyHostToDevice, ctx->st));
@@ -634,7 +646,8 @@ int run_batch(fv_ctx *ctx, int nmax, int *done_steps, int *halted) {
     for (int s = 0; s < nmax; s += 2) HIPCHK(hipGraphLaunch(exec, ctx->st));
   } else {  // timed: the same launches, issued one by one between events
     for (int s = 0; s < nmax; s++)
-      enqueue_fast_step(ctx, s % 2 ? x1 : x0, s % 2 ? x0 : x1, tb, rb, s % 2 ? 1 - k0 : k0, s % 2 ? k0 : 1 - k0);
+      enqueue_fast_step(ctx, s % 2 ? x1 : x0, s % 2 ? x0 : x1, tb, rb, s % 2 ? 1 - k0 : k0, s % 2 ? k0 : 1 - k0,
+                        s % 2 ? 1 - b0 : b0);
     HIPCHK(hipGetLastError());
   }
   HIPCHK(hipMemcpyAsync(h, ctx->d_ctl, sizeof(Ctl), hipMemcpyDeviceToHost, ctx->st));
@@ -719,6 +732,9 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
   for (int q = 0; q < NBUF && ok; q++) ok = zalloc(&ctx->buf[q]);
   for (int q = 1; q < ctx->tab.m && ok; q++) ok = zalloc(&ctx->k[q]);
   for (int q = 0; q < 2 && ok; q++) ok = zalloc(&ctx->k0[q]);
+  const size_t bfbytes = sizeof(double) * 2 * ctx->V * (ctx->NJ + ctx->NI);
+  for (int q = 0; q < 2 && ok; q++)  // InitFluxes: zero (fluxes_base.f90:209-214)
+    ok = dalloc(&ctx->bf[q], bfbytes) && hipMemsetAsync(ctx->bf[q], 0, bfbytes, ctx->st) == hipSuccess;
   ok = ok && dalloc(&ctx->d_ring, sizeof(double) * R_COUNT * ctx->NI) && dalloc(&ctx->d_col, sizeof(double) * C_COUNT * ctx->NJ);
   if (c.dtype == 1)
     ok = ok && hipMalloc((void **)&ctx->d_ringf, sizeof(float) * R_COUNT * ctx->NI) == hipSuccess &&
@@ -730,6 +746,8 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
   g.gamma = c.gamma; g.pmin = DBL_MIN; g.rhomin = DBL_MIN;
   g.limiter = c.limiter; g.theta = c.theta;
   for (int d = 0; d < 4; d++) g.bc[d] = c.bc[d];
+  g.bdydz = ctx->mesh.dy * ctx->mesh.dz;
+  g.bdzdx = ctx->mesh.dz * ctx->mesh.dx;
   g.fargo = c.fargo == 2; g.gravity = c.gravity != 0; g.curvilinear = c.geometry != CARTESIAN;
   g.dtype = c.dtype;
   g.stages = ctx->tab.m;
@@ -774,6 +792,7 @@ extern "C" int fv_destroy(fv_ctx *ctx) {
   for (auto &b : ctx->buf) if (b) (void)hipFree(b);
   for (auto &b : ctx->k) if (b) (void)hipFree(b);
   for (auto &b : ctx->k0) if (b) (void)hipFree(b);
+  for (auto &b : ctx->bf) if (b) (void)hipFree(b);
   if (ctx->part_r) (void)hipFree(ctx->part_r);
   if (ctx->part_c) (void)hipFree(ctx->part_c);
   if (ctx->d_red) (void)hipFree(ctx->d_red);
@@ -802,6 +821,7 @@ extern "C" int fv_device_bytes(const fv_config *c, double *bytes) {
   const double state = es * V * NJ * P;  // one state buffer, ghost cells and row padding included
   const double nb32 = std::ceil(c->inum / 32.0) * std::ceil(c->jnum / (double)TY_TILE);
   double b = state * (NBUF + (t.m - 1) + 2);          // states, stage rhs, coeff(1) pair
+  b += 2.0 * 8.0 * 2.0 * V * (NI + NJ);                // boundary fluxes, double-buffered
   b += 2.0 * 2.0 * 8.0 * nb32;                         // CFL and error partials
   b += 8.0 * (R_COUNT * NI + C_COUNT * NJ);             // metric arrays
   if (c->dtype == 1) b += 4.0 * (R_COUNT * NI + C_COUNT * NJ);
@@ -972,6 +992,34 @@ extern "C" int fv_get_state(fv_ctx *ctx, double *cvar, double *pvar) {
   return 0;
 }
 
+extern "C" int fv_get_boundary_flux(fv_ctx *ctx, double *bx, double *by) {
+  if (!ctx) return -1;
+  const size_t nx = (size_t)2 * ctx->V * ctx->NJ, ny = (size_t)2 * ctx->V * ctx->NI;
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  // the device layout is the reference's (ghost entries stay zero)
+  if (bx) HIPCHK(hipMemcpyAsync(bx, ctx->bf[ctx->bcur], sizeof(double) * nx, hipMemcpyDeviceToHost, ctx->st));
+  if (by) HIPCHK(hipMemcpyAsync(by, ctx->bf[ctx->bcur] + nx, sizeof(double) * ny, hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return 0;
+}
+
+extern "C" int fv_get_boundary_flux_total(fv_ctx *ctx, int direction, double *bflux) {
+  if (!ctx || !bflux || direction < 1 || direction > 4) return fail(ctx, "fv_get_boundary_flux_total: direction 1..4");
+  const int V = ctx->V, NI = ctx->NI, NJ = ctx->NJ;
+  std::vector<double> bx((size_t)2 * V * NJ), by((size_t)2 * V * NI);
+  int rc = fv_get_boundary_flux(ctx, bx.data(), by.data());
+  if (rc) return rc;
+  const bool xd = direction <= 2;
+  const int side = (direction - 1) & 1, n = xd ? NJ : NI, lo = 2, hi = (xd ? ctx->c.jnum : ctx->c.inum) + 1;
+  for (int l = 0; l < V; l++) {  // SUM over JMIN:JMAX (IMIN:IMAX), in index order
+    const double *a = (xd ? bx.data() : by.data()) + (size_t)(l * 2 + side) * n;
+    double acc = 0.0;
+    for (int k = lo; k <= hi; k++) acc = acc + a[k];
+    bflux[l] = acc;
+  }
+  return 0;
+}
+
 extern "C" int fv_get_scalars(const fv_ctx *ctx, fv_scalars *o) {
   if (!ctx || !o) return -1;
   o->iter = ctx->iter;
@@ -1044,7 +1092,7 @@ struct fv_batch {
   double tune_ms[NCAND] = {};
   struct G { int K, cand; hipGraphExec_t exec; };
   std::vector<G> graphs;
-  struct Roles { int x0, x1, tb, rb, k0; };
+  struct Roles { int x0, x1, tb, rb, k0, b0; };
   std::vector<Roles> roles;
   long dev_steps = 0, host_steps = 0, halts = 0, rounds = 0;
   double last_ms = 0;
@@ -1211,10 +1259,11 @@ int batch_round(fv_batch *b, const std::vector<int> &want, int K, int c, bool ti
     r.tb = pick_free(x, r.x0, r.x1, -1);
     r.rb = pick_free(x, r.x0, r.x1, r.tb);
     r.k0 = x->k0cur;
+    r.b0 = x->bcur;
     for (int p = 0; p < 2; p++) {
       FastStep f;
       fast_step_args(x, dc, p ? r.x1 : r.x0, p ? r.x0 : r.x1, r.tb, r.rb, p ? 1 - r.k0 : r.k0,
-                     p ? r.k0 : 1 - r.k0, f);
+                     p ? r.k0 : 1 - r.k0, p ? 1 - r.b0 : r.b0, f);
       for (int q = 0; q < m - 1; q++) b->h_rhs[(size_t)(p * m + q) * cap + d] = RhsEntry{x->g, f.st[q]};
       b->h_rhs[(size_t)(p * m + m - 1) * cap + d] = RhsEntry{x->g, f.end};
       RemapEntry re;

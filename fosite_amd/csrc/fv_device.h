@@ -33,6 +33,7 @@ struct Geom {
   int limiter;
   double theta;
   int bc[4];
+  double bdydz, bdzdx;  // Mesh%dy*Mesh%dz, Mesh%dz*Mesh%dx: boundary-flux weights
   int fargo, gravity, curvilinear;
   int tile_x;           // k_rhs tile width in cells: 64 or 32
   int stages;           // RK stages of the tableau (picks the kernels' coefficient slots)
@@ -83,6 +84,14 @@ struct StageArgsT {
   double bh[MAXK], bl[MAXK];
   double tol_rel, tol_abs[4];
   double *part_c;       // per-block partials: [2*b] max error, [2*b+1] = 0
+  // boundary fluxes (timedisc_base.f90:1224-1252, timedisc_rkfehlberg.f90:216-259):
+  // with bflux set, a stage stores the boundary rows of its rhs at the ghost
+  // positions next to the boundary cells of kout; the last stage accumulates
+  // bf_out = bf_in - sum_q dt*b_high(q)*k_q over those rows.  Layout of
+  // bf_in/bf_out: x part [V][2 sides][NJ], then y part [V][2 sides][NI].
+  int bflux;
+  const double *bf_in;
+  double *bf_out;
 };
 using StageArgs = StageArgsT<double>;
 

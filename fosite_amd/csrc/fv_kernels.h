@@ -883,7 +883,53 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
       rhs[l] = dydzdV * (xf[l] - xw) + dzdxdV * (yf[l] - ys) + dxdydV * (zero - zero) - geo[l] - ext[l];
       if (a.yout) (a.yout + l * g.FS)[ix] = uc[l];
     }
+    // boundary rows: the fluxes through the mesh boundary at the boundary
+    // cells' outer interfaces, Mesh%dy*Mesh%dz*xflux (W), -Mesh%dy*Mesh%dz*xflux
+    // (E) and likewise in y (timedisc_base.f90:1224-1252), stored at the ghost
+    // position next to the cell; the last stage accumulates them into the
+    // boundary fluxes (timedisc_rkfehlberg.f90:216-259).  bxs/bys: the x and
+    // y side of a boundary cell (0 W/S, 1 E/N, -1 none).
+    const int bxs = !a.bflux ? -1 : i == imin ? 0 : i == imax ? 1 : -1;
+    const int bys = !a.bflux ? -1 : j == jmin ? 0 : j == jmax ? 1 : -1;
+    const unsigned gxx = (unsigned)j * (unsigned)g.P + (unsigned)(bxs == 0 ? imin - 1 : imax + 1);
+    const unsigned gxy = (unsigned)(bys == 0 ? jmin - 1 : jmax + 1) * (unsigned)g.P + (unsigned)i;
+    const long bix = (long)bxs * g.NJ + j, biy = 2L * V * g.NJ + (long)bys * g.NI + i;
+    T kbx[V], kby[V];
+    if (bxs >= 0 || bys >= 0) {
+      const T ax = (T)g.bdydz, ay = (T)g.bdzdx;
+#pragma unroll
+      for (int l = 0; l < V; l++) {
+        kbx[l] = bxs == 0 ? ax * FLX(l, ty, tx) : -(ax * xf[l]);
+        kby[l] = bys == 0 ? ay * FLY(l, ty, tx) : -(ay * yf[l]);
+        if (!a.ynew) {
+          if (bxs >= 0) (a.kout + l * g.FS)[gxx] = kbx[l];
+          if (bys >= 0) (a.kout + l * g.FS)[gxy] = kby[l];
+        }
+      }
+    }
     if (a.ynew) {
+      // boundary-flux inputs of the accumulation, issued with the loads of
+      // the final combination: Fluxes%bxfold and the boundary rows of k_1..k_{s-1}
+      double bxi[V], byi[V];
+      T kbxq[KM][V], kbyq[KM][V];
+      if (bxs >= 0) {
+#pragma unroll
+        for (int l = 0; l < V; l++) {
+          bxi[l] = a.bf_in[bix + 2L * l * g.NJ];
+#pragma unroll
+          for (int q = 0; q < KM; q++)
+            if (q < in.nk) kbxq[q][l] = (a.k[q] + l * g.FS)[gxx];
+        }
+      }
+      if (bys >= 0) {
+#pragma unroll
+        for (int l = 0; l < V; l++) {
+          byi[l] = a.bf_in[biy + 2L * l * g.NI];
+#pragma unroll
+          for (int q = 0; q < KM; q++)
+            if (q < in.nk) kbyq[q][l] = (a.k[q] + l * g.FS)[gxy];
+        }
+      }
       // final combination with this stage's rhs as k_s, and the error of the
       // embedded solution (FP64: same operation order as the reference;
       // FP32: compensated sums, error norm in FP64)
@@ -941,6 +987,24 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
           dif = (double)e;
         }
         err = fmax2(err, fabs(dif) / (a.tol_rel * fabs(yh) + a.tol_abs[l]));
+      }
+      // bxflux = bxfold - sum_q dt*b_high(q)*k_q at the boundary rows
+#pragma unroll
+      for (int l = 0; l < V; l++) {
+        if (bxs >= 0) {
+          double acc = bxi[l];
+#pragma unroll
+          for (int q = 0; q < KM; q++)
+            if (q <= in.nk) acc = acc - (in.dt * a.bh[q]) * (double)(q < in.nk ? kbxq[q][l] : kbx[l]);
+          a.bf_out[bix + 2L * l * g.NJ] = acc;
+        }
+        if (bys >= 0) {
+          double acc = byi[l];
+#pragma unroll
+          for (int q = 0; q < KM; q++)
+            if (q <= in.nk) acc = acc - (in.dt * a.bh[q]) * (double)(q < in.nk ? kbyq[q][l] : kby[l]);
+          a.bf_out[biy + 2L * l * g.NI] = acc;
+        }
       }
     } else {
 #pragma unroll

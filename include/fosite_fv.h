@@ -104,6 +104,19 @@ int fv_run(fv_ctx *ctx, int nsteps, int *taken);
  * variables Convert2Primitive(cvar) (either pointer may be NULL). */
 int fv_get_state(fv_ctx *ctx, double *cvar, double *pvar);
 int fv_get_scalars(const fv_ctx *ctx, fv_scalars *out);
+/* Fluxes%bxflux and Fluxes%byflux (fluxes/fluxes_base.f90:142-143): the
+ * fluxes through the mesh boundaries accumulated over every accepted step,
+ * sum_m -dt*b_high(m)*rhs_m at the boundary rows
+ * (timedisc/timedisc_rkfehlberg.f90:216-259, timedisc/timedisc_base.f90:
+ * 1224-1252, 937-957, 981-1000), in the reference's layout:
+ * bx(JGMIN:JGMAX,1,2,VNUM) = NJ*2*VNUM values, by(1,IGMIN:IGMAX,2,VNUM) =
+ * NI*2*VNUM values, side 1 west/south, side 2 east/north; ghost entries are
+ * zero.  Either pointer may be NULL. */
+int fv_get_boundary_flux(fv_ctx *ctx, double *bx, double *by);
+/* Fluxes%GetBoundaryFlux (fluxes/fluxes_base.f90:220-300, serial build):
+ * direction 1 WEST, 2 EAST, 3 SOUTH, 4 NORTH (mesh/mesh_base.f90:102-105);
+ * bflux[VNUM] = SUM over the boundary's interior range, in index order. */
+int fv_get_boundary_flux_total(fv_ctx *ctx, int direction, double *bflux);
 /* wall time of the device work issued by the last fv_run, in ms
  * (HIP events on the context's stream) */
 double fv_last_run_ms(const fv_ctx *ctx);

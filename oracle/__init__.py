@@ -72,6 +72,7 @@ def _load():
         L.fvr_get.argtypes = [ctypes.c_void_p, P, P, ctypes.POINTER(_Scalars)]
         L.fvr_compute_rhs.argtypes = [ctypes.c_void_p, P]
         L.fvr_round_state_f32.argtypes = [ctypes.c_void_p]
+        L.fvr_get_bflux.argtypes = [ctypes.c_void_p, P, P]
         _lib = L
     return _lib
 
@@ -159,6 +160,14 @@ class Oracle:
         sc = _Scalars()
         self.L.fvr_get(self._h, _ptr(cv), _ptr(pv), ctypes.byref(sc))
         return cv, pv, {k: getattr(sc, k) for k, _ in _Scalars._fields_}
+
+    def bflux(self):
+        """Accumulated boundary fluxes (Fluxes%bxflux, byflux): arrays
+        bx[l, side, j] (V x 2 x NJ) and by[l, side, i] (V x 2 x NI)."""
+        bx = np.empty((self.V, 2, self.NJ))
+        by = np.empty((self.V, 2, self.NI))
+        self.L.fvr_get_bflux(self._h, _ptr(bx), _ptr(by))
+        return bx, by
 
     def rhs(self):
         out = np.empty((self.V, self.NJ, self.NI))

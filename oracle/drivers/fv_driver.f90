@@ -40,7 +40,7 @@ PROGRAM fv_driver
   TYPE(Dict_TYP), POINTER    :: config, mesh, phys, bnd, flx, tdisc, dfile, &
                                 src, pmass
   CHARACTER(LEN=512)         :: cfgname
-  INTEGER                    :: u, n, vnum, ncell
+  INTEGER                    :: u, ub, n, vnum, ncell
   INTEGER(KIND=8)            :: c0, c1, crate, ctot
   LOGICAL                    :: brk
   REAL, ALLOCATABLE          :: pv(:), cs(:), fcs(:), w(:)
@@ -118,6 +118,8 @@ PROGRAM fv_driver
   ! ---- integration ----
   OPEN(NEWUNIT=u, FILE=TRIM(k_prefix) // "_steps.bin", ACCESS="stream", &
        FORM="unformatted", STATUS="replace")
+  OPEN(NEWUNIT=ub, FILE=TRIM(k_prefix) // "_bflux.bin", ACCESS="stream", &
+       FORM="unformatted", STATUS="replace")
   CALL Sim%FirstStep()
   CALL DumpState(u, 0)
   ctot = 0
@@ -131,6 +133,7 @@ PROGRAM fv_driver
     IF (brk .OR. Sim%Timedisc%break) EXIT
   END DO
   CLOSE(u)
+  CLOSE(ub)
   ! wall time of the Step() calls only (setup, FirstStep and dumps excluded)
   WRITE(*,'(A,I0,A,ES24.16)') "FV_DRIVER_STEP_SECONDS ", MIN(n, k_nsteps), " ", &
        REAL(ctot, 8) / REAL(crate, 8)
@@ -156,6 +159,13 @@ CONTAINS
     WRITE(u) ibuf, rbuf
     WRITE(u) Sim%Timedisc%cvar%data1d
     WRITE(u) Sim%Timedisc%pvar%data1d
+    ! boundary fluxes (fluxes_base.f90:142-143) and their sums per boundary
+    ! (GetBoundaryFlux, fluxes_base.f90:220-365), into the side file
+    WRITE(ub) Sim%Fluxes%bxflux, Sim%Fluxes%byflux
+    WRITE(ub) Sim%Fluxes%GetBoundaryFlux(Sim%Mesh, Sim%Physics, WEST), &
+              Sim%Fluxes%GetBoundaryFlux(Sim%Mesh, Sim%Physics, EAST), &
+              Sim%Fluxes%GetBoundaryFlux(Sim%Mesh, Sim%Physics, SOUTH), &
+              Sim%Fluxes%GetBoundaryFlux(Sim%Mesh, Sim%Physics, NORTH)
   END SUBROUTINE DumpState
 
   !> metric arrays the GPU context is built from (all cells incl. ghosts)

@@ -60,6 +60,9 @@ struct fvr {
   /* timedisc */
   sv pvar, cvar, ptmp, ctmp, cold;
   double *geo_src, *src, *sterm, **coeff, *cerr, *xflux, *yflux;
+  /* boundary fluxes (fluxes_base.f90:142-143): bx[(l*2+side)*NJ + j],
+   * by[(l*2+side)*NI + i]; bxo/byo are bxfold/byfold */
+  double *bx, *by, *bxo, *byo;
   double *a, *b_high, *b_low, *cc; /* Butcher tableau, a row-major m x m */
   /* fluxes */
   double *prim, *cons, *pfluxes, *minwav, *maxwav, *tmp, *tmp1, *slopes;
@@ -755,8 +758,29 @@ static double adjust_timestep(fvr *s, double maxerr, double dtold) {
   return dtnew;
 }
 
-/* timedisc_rkfehlberg.f90:170-268 (boundary-flux bookkeeping omitted: it
- * feeds diagnostics and accretion only, neither of which is enabled) */
+/* boundary-flux accumulation of one SolveODE (timedisc_rkfehlberg.f90:216-259):
+ * the boundary rows of every stage's rhs (compute_rhs stores the boundary
+ * fluxes there, timedisc_base.f90:1224-1252), weighted by -dt*b_high(m) */
+static void accumulate_bflux(fvr *s, double dt) {
+  const int N = s->N, NI = s->NI, NJ = s->NJ;
+  const int imin = G, imax = G + s->c.inum - 1, jmin = G, jmax = G + s->c.jnum - 1;
+  for (int m = 0; m < s->m; m++)
+    for (int l = 0; l < s->V; l++) {
+      const double *k = s->coeff[m] + (size_t)l * N;
+      double *bw = s->bx + (size_t)(2 * l) * NJ, *be = bw + NJ;
+      double *bs = s->by + (size_t)(2 * l) * NI, *bn = bs + NI;
+      for (int j = jmin; j <= jmax; j++) {
+        bw[j] = bw[j] - dt * s->b_high[m] * k[IDX(s, imin - 1, j)];
+        be[j] = be[j] - dt * s->b_high[m] * k[IDX(s, imax + 1, j)];
+      }
+      for (int i = imin; i <= imax; i++) {
+        bs[i] = bs[i] - dt * s->b_high[m] * k[IDX(s, i, jmin - 1)];
+        bn[i] = bn[i] - dt * s->b_high[m] * k[IDX(s, i, jmax + 1)];
+      }
+    }
+}
+
+/* timedisc_rkfehlberg.f90:170-268 */
 static void solve_ode(fvr *s, double time, double *dt, double *err) {
   size_t n = (size_t)s->V * s->N;
   (void)time;
@@ -771,6 +795,7 @@ static void solve_ode(fvr *s, double time, double *dt, double *err) {
       s->ctmp.d[k] = s->ctmp.d[k] - s->b_low[m] * *dt * s->coeff[m][k];
       s->cvar.d[k] = s->cvar.d[k] - s->b_high[m] * *dt * s->coeff[m][k];
     }
+  accumulate_bflux(s, *dt);
   if (s->c.tol_rel <= 1.0) {
     *err = compute_error(s);
     *dt = adjust_timestep(s, *err, *dt);
@@ -832,6 +857,8 @@ static void reject_solution(fvr *s, double time, double dt) {
   (void)time;
   copy_sv(s, &s->cvar, &s->cold);
   compute_rhs(s, &s->pvar, &s->cvar, CHECK_NOTHING, s->coeff[0]);
+  memcpy(s->bx, s->bxo, sizeof(double) * 2 * s->V * s->NJ);
+  memcpy(s->by, s->byo, sizeof(double) * 2 * s->V * s->NI);
   s->n_adj++;
   s->dtcause = DTCAUSE_ERRADJ;
   if (dt < s->dtmin) s->dtmin = dt;
@@ -844,6 +871,8 @@ static void accept_solution(fvr *s, double *time, double dt) {
     compute_rhs(s, &s->pvar, &s->cvar, CHECK_ALL, s->coeff[0]);
     copy_sv(s, &s->cold, &s->cvar);
   }
+  memcpy(s->bxo, s->bx, sizeof(double) * 2 * s->V * s->NJ);
+  memcpy(s->byo, s->by, sizeof(double) * 2 * s->V * s->NI);
   s->iter++;
 }
 
@@ -967,6 +996,8 @@ fvr *fvr_create(const fvr_cfg *cfg) {
     s->ctmp.d = zalloc(V * N); s->cold.d = zalloc(V * N);
     s->geo_src = zalloc(V * N); s->src = zalloc(V * N); s->sterm = zalloc(V * N); s->cerr = zalloc(V * N);
     s->xflux = zalloc(V * N); s->yflux = zalloc(V * N);
+    s->bx = zalloc(2 * V * s->NJ); s->by = zalloc(2 * V * s->NI);
+    s->bxo = zalloc(2 * V * s->NJ); s->byo = zalloc(2 * V * s->NI);
     s->coeff = (double **)calloc(s->m, sizeof(double *));
     for (int k = 0; k < s->m; k++) s->coeff[k] = zalloc(V * N);
     s->prim = zalloc(4 * V * N); s->cons = zalloc(4 * V * N); s->pfluxes = zalloc(4 * V * N);
@@ -998,6 +1029,7 @@ void fvr_destroy(fvr *s) {
   free(s->bccsound); free(s->fcsound); free(s->w);
   free(s->pvar.d); free(s->cvar.d); free(s->ptmp.d); free(s->ctmp.d); free(s->cold.d);
   free(s->geo_src); free(s->src); free(s->sterm); free(s->cerr); free(s->xflux); free(s->yflux);
+  free(s->bx); free(s->by); free(s->bxo); free(s->byo);
   for (int k = 0; k < s->m; k++) free(s->coeff[k]);
   free(s->coeff);
   free(s->prim); free(s->cons); free(s->pfluxes); free(s->minwav); free(s->maxwav);
@@ -1075,6 +1107,11 @@ void fvr_get(const fvr *s, double *cvar, double *pvar, fvr_scalars *sc) {
     sc->iter = s->iter; sc->n_adj = s->n_adj; sc->brk = s->brk; sc->dtcause = s->dtcause;
     sc->time = s->time; sc->dt = s->dt; sc->dtold = s->dtold; sc->maxerrold = s->maxerrold;
   }
+}
+
+void fvr_get_bflux(const fvr *s, double *bx, double *by) {
+  if (bx) memcpy(bx, s->bx, sizeof(double) * 2 * s->V * s->NJ);
+  if (by) memcpy(by, s->by, sizeof(double) * 2 * s->V * s->NI);
 }
 
 void fvr_compute_rhs(fvr *s, double *rhs_out) {

@@ -60,6 +60,10 @@ int fvr_first_step(fvr *s);
 /* fosite%Step: returns 1 if the run ended (stop time or break) */
 int fvr_step(fvr *s);
 void fvr_get(const fvr *s, double *cvar, double *pvar, fvr_scalars *sc);
+/* accumulated boundary fluxes Fluxes%bxflux / byflux in the reference's
+ * layout (fluxes_base.f90:142-143): bx(JGMIN:JGMAX,1,2,VNUM) = NJ*2*V values,
+ * by(1,IGMIN:IGMAX,2,VNUM) = NI*2*V values (side 1 west/south, 2 east/north) */
+void fvr_get_bflux(const fvr *s, double *bx, double *by);
 /* one ComputeRHS of the current (cvar,pvar) pair without touching the step
  * state's history; rhs out (all cells) -- used for unit-level parity */
 void fvr_compute_rhs(fvr *s, double *rhs_out);

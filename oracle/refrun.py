@@ -96,6 +96,15 @@ def run(cfg, init, nsteps, dump_every=1, workdir=None):
         recs.append(dict(step=int(ib[0]), iter=int(ib[1]), n_adj=int(ib[2]), brk=int(ib[3]),
                          time=rb[0], dt=rb[1], dtold=rb[2], maxerrold=rb[3], cvar=cv, pvar=pv))
         off += rec
+    # boundary fluxes of every record: bxflux(JG,1,2,V), byflux(1,IG,2,V),
+    # then GetBoundaryFlux of W, E, S, N (V values each)
+    braw = np.fromfile(prefix + "_bflux.bin", dtype="<f8")
+    brec = 2 * V * NJ + 2 * V * NI + 4 * V
+    for n, r in enumerate(recs):
+        b = braw[n * brec:(n + 1) * brec]
+        r["bx"] = b[:2 * V * NJ].reshape(V, 2, NJ).copy()
+        r["by"] = b[2 * V * NJ:2 * V * (NJ + NI)].reshape(V, 2, NI).copy()
+        r["bsum"] = b[2 * V * (NJ + NI):].reshape(4, V).copy()
     mesh["step_seconds"] = step_seconds
     if workdir is None:
         shutil.rmtree(tmp, ignore_errors=True)

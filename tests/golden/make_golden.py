@@ -3,7 +3,8 @@
 Runs oracle/_ref/fv_driver -- the reference's own Fortran sources compiled
 by oracle/build_ref.py, driven through FirstStep + Step() (fosite.f90:258-464)
 -- on small seeded configurations and stores, per case, the configuration,
-the step scalars after every step and the conservative state after chosen
+the step scalars after every step, the conservative state and the
+accumulated boundary fluxes (with their GetBoundaryFlux sums) after chosen
 steps, plus the reference's mesh arrays for one Cartesian and one
 logcylindrical mesh.  Output: tests/golden/fv_golden.npz (data only).
 
@@ -67,6 +68,10 @@ def main():
             out[name + "/" + k] = np.array([r[k] for r in recs])
         for s in keep:
             out["%s/cvar_%d" % (name, s)] = recs[s]["cvar"]
+            # boundary fluxes bxflux(JG,1,2,V) / byflux(1,IG,2,V) as [l][side][j|i]
+            out["%s/bx_%d" % (name, s)] = recs[s]["bx"]
+            out["%s/by_%d" % (name, s)] = recs[s]["by"]
+            out["%s/bsum_%d" % (name, s)] = recs[s]["bsum"]
         print(name, "n_adj", recs[-1]["n_adj"], "time %.6e" % recs[-1]["time"])
     for name, cfg in MESHES.items():
         mesh, _ = refrun.run(cfg, S.initial(cfg), 0, dump_every=1)

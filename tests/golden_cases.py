@@ -32,6 +32,18 @@ def case(name):
     return cfg, steps, scalars, states
 
 
+def bflux(name):
+    """{step: (bx [V,2,NJ], by [V,2,NI], sums [4,V] of W,E,S,N)} of the
+    steps whose state is stored."""
+    d = data()
+    out = {}
+    for k in d:
+        if k.startswith(name + "/bx_"):
+            s = int(k.split("_")[-1])
+            out[s] = (d[k], d["%s/by_%d" % (name, s)], d["%s/bsum_%d" % (name, s)])
+    return out
+
+
 def meshes():
     return sorted({k.split("/")[0][5:] for k in data() if k.startswith("mesh_")})
 

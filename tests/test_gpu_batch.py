@@ -28,8 +28,9 @@ def alone(cfg, chunks):
     taken = sum(sim.run(n) for n in chunks)
     cv, _ = sim.state()
     sc = sim.scalars()
+    bf = sim.boundary_flux()
     sim.close()
-    return taken, cv, sc
+    return taken, cv, sc, bf
 
 
 def members(dtype="fp64"):
@@ -53,7 +54,7 @@ def run_batch(cfgs, chunks, **opts):
     taken = [0] * len(sims)
     for n in chunks:
         taken = [t + k for t, k in zip(taken, b.run(n))]
-    res = [(taken[k], sims[k].state()[0], sims[k].scalars()) for k in range(len(sims))]
+    res = [(taken[k], sims[k].state()[0], sims[k].scalars(), sims[k].boundary_flux()) for k in range(len(sims))]
     info = b.info()
     b.close()
     for s in sims:
@@ -63,11 +64,13 @@ def run_batch(cfgs, chunks, **opts):
 
 def check_same(cfgs, res, chunks):
     for k, cfg in enumerate(cfgs):
-        t0, cv0, sc0 = alone(cfg, chunks)
-        t1, cv1, sc1 = res[k]
+        t0, cv0, sc0, bf0 = alone(cfg, chunks)
+        t1, cv1, sc1, bf1 = res[k]
         assert t1 == t0, (k, t1, t0)
         assert sc1 == sc0, (k, sc1, sc0)
         assert np.array_equal(cv1[I], cv0[I]), (k, np.abs(cv1[I] - cv0[I]).max())
+        # boundary fluxes (Fluxes%bxflux/byflux) accumulated by the batch
+        assert np.array_equal(bf1[0], bf0[0]) and np.array_equal(bf1[1], bf0[1]), k
 
 
 @pytest.mark.gpu
@@ -135,10 +138,12 @@ def test_gpu_batch_membership_changes():
     b.close()
     plan = {0: [20, 15, 10], 1: [20, 7, 10], 2: [20, 15, 10], 3: [15, 10]}
     for k, cfg in enumerate(cfgs):
-        t0, cv0, sc0 = alone(cfg, plan[k])
+        t0, cv0, sc0, bf0 = alone(cfg, plan[k])
         cv1, _ = sims[k].state()
+        bf1 = sims[k].boundary_flux()
         assert sims[k].scalars() == sc0, k
         assert np.array_equal(cv1[I], cv0[I]), k
+        assert np.array_equal(bf1[0], bf0[0]) and np.array_equal(bf1[1], bf0[1]), k
         sims[k].close()
 
 
@@ -154,9 +159,12 @@ def test_gpu_batch_member_matches_oracle():
     for _ in range(30):
         o.step()
     rc, _, rs = o.state()
-    _, cv, sc = res[0]
+    _, cv, sc, bf = res[0]
     assert sc["iter"] == rs["iter"] and sc["n_adj"] == rs["n_adj"]
     assert (rel_l1(cv, rc) <= 1e-12).all(), rel_l1(cv, rc)
+    rbx, rby = o.bflux()
+    for got, ref in ((bf[0], rbx), (bf[1], rby)):
+        assert np.abs(got - ref).sum() <= 1e-12 * np.abs(ref).sum()
 
 
 @pytest.mark.gpu

@@ -60,6 +60,56 @@ def test_oracle_matches_reference_fixture(name):
         assert np.array_equal(got[s][I], ref[I]), (name, s, np.abs(got[s][I] - ref[I]).max())
 
 
+@pytest.mark.parametrize("name", [n for n in G.cases() if G.bflux(n)])
+def test_oracle_boundary_flux_matches_reference_fixture(name):
+    """Fluxes%bxflux / byflux after the stored steps, bit for bit, and the
+    reference's GetBoundaryFlux sums equal index-order sums of those arrays."""
+    cfg, steps, _, _ = G.case(name)
+    ref = G.bflux(name)
+    init = S.initial(cfg)
+    o = oracle.Oracle(cfg)
+    o.set_initial(init["pvar"], init.get("bccsound"), init.get("fcsound"), init.get("w"))
+    o.first_step()
+    for n in range(1, steps + 1):
+        o.step()
+        if n in ref:
+            bx, by = o.bflux()
+            rbx, rby, rsum = ref[n]
+            assert np.array_equal(bx, rbx), (name, n, np.abs(bx - rbx).max())
+            assert np.array_equal(by, rby), (name, n, np.abs(by - rby).max())
+            for d, (arr, m) in enumerate(((rbx[:, 0], cfg["jnum"]), (rbx[:, 1], cfg["jnum"]),
+                                          (rby[:, 0], cfg["inum"]), (rby[:, 1], cfg["inum"]))):
+                for l in range(arr.shape[0]):
+                    acc = 0.0
+                    for k in range(2, m + 2):
+                        acc = acc + arr[l, k]
+                    assert acc == rsum[d, l], (name, n, d, l)
+
+
+def test_oracle_boundary_flux_balances_the_interior_totals():
+    """Cartesian mesh, no sources: the change of sum(U)*dx*dy over the
+    interior equals minus the sum of all accumulated boundary fluxes (each
+    side weighted by its face length), to round-off -- a wrong sign, side or
+    stage weight breaks it."""
+    cfg = dict(S.riemann2d_config(n=24), inum=24, jnum=20, ymax=20.0 / 24.0)
+    init = S.initial(cfg)
+    o = oracle.Oracle(cfg)
+    o.set_initial(init["pvar"])
+    o.first_step()
+    cv0, _, _ = o.state()
+    for _ in range(15):
+        o.step()
+    cv, _, sc = o.state()
+    bx, by = o.bflux()
+    dx, dy = (cfg["xmax"] - cfg["xmin"]) / cfg["inum"], (cfg["ymax"] - cfg["ymin"]) / cfg["jnum"]
+    I = (slice(None), slice(2, -2), slice(2, -2))
+    change = (cv[I].sum(axis=(1, 2)) - cv0[I].sum(axis=(1, 2))) * dx * dy
+    out = bx.sum(axis=(1, 2)) * 1.0 + by.sum(axis=(1, 2)) * 1.0
+    scale = np.abs(cv0[I]).sum(axis=(1, 2)) * dx * dy
+    assert sc["n_adj"] >= 0 and np.abs(out).max() > 1e-6, out
+    assert np.all(np.abs(change + out) <= 1e-12 * scale), (change, out)
+
+
 @pytest.mark.parametrize("name", G.meshes())
 def test_oracle_mesh_matches_reference_fixture(name):
     cfg, arrays = G.mesh(name)
