@@ -66,6 +66,7 @@ class FvScalars(ctypes.Structure):
 EXPORTS = ["fv_mesh_array", "fv_create", "fv_destroy", "fv_dims", "fv_set_sound_speeds",
            "fv_set_background_velocity", "fv_set_pvar", "fv_first_step", "fv_step", "fv_run",
            "fv_get_state", "fv_get_scalars", "fv_get_boundary_flux", "fv_get_boundary_flux_total",
+           "fv_get_background_velocity",
            "fv_last_run_ms", "fv_last_error", "fv_profile",
            "fv_kernel_stats", "fv_batch_create", "fv_batch_add", "fv_batch_remove", "fv_batch_run",
            "fv_batch_get_info", "fv_batch_destroy", "fv_batch_last_error", "fv_device_bytes",
@@ -129,6 +130,7 @@ def load_library():
     L.fv_get_state.argtypes = [V, P, P]
     L.fv_get_scalars.argtypes = [V, ctypes.POINTER(FvScalars)]
     L.fv_get_boundary_flux.argtypes = [V, P, P]
+    L.fv_get_background_velocity.argtypes = [V, P]
     L.fv_get_boundary_flux_total.argtypes = [V, ctypes.c_int, P]
     L.fv_last_run_ms.argtypes = [V]
     L.fv_last_run_ms.restype = ctypes.c_double
@@ -256,6 +258,12 @@ class Fosite:
         pv = np.empty((self.V, self.NJ, self.NI))
         self._chk(self.L.fv_get_state(self._h, _ptr(cv), _ptr(pv)), "fv_get_state")
         return cv, pv
+
+    def background_velocity(self):
+        """Timedisc%w: FARGO background velocity per x-ring (NI values)."""
+        w = np.empty(self.NI)
+        self._chk(self.L.fv_get_background_velocity(self._h, _ptr(w)), "fv_get_background_velocity")
+        return w
 
     def boundary_flux(self):
         """Fluxes%bxflux / byflux (fluxes_base.f90:142-143): bx[l, side, j]

@@ -117,6 +117,9 @@ struct fv_ctx {
   // other copy; AcceptSolution is a role swap, RejectSolution keeps bcur
   double *bf[2] = {};
   int bcur = 0;
+  // FARGO type 1: azimuthal velocities the last ComputeRHS left for the next
+  // CalcBackgroundVelocity ([NJ][P], FP64)
+  double *vbuf = nullptr;
   Ctl *d_ctl = nullptr, *h_ctl = nullptr;
   struct GraphEntry { int x0, x1, t, k0, b0, nsteps; hipGraphExec_t exec; };
   std::vector<GraphEntry> graphs;
@@ -233,7 +236,7 @@ StageArgs base_args(fv_ctx *ctx) {
 // returns add/sub and updates the flags of (p, c) like the reference does
 void rhs_flags(const fv_ctx *ctx, int &fp, int &fc, int &add, int &sub) {
   add = sub = 0;
-  if (ctx->c.fargo != 2) return;
+  if (!ctx->c.fargo) return;
   if (fc && fp) {
     add = 1;
     fc = fp = 0;
@@ -274,6 +277,7 @@ int rhs_main(fv_ctx *ctx, int checkbm, int remap) {
   a.pmin_clamp = pmin;
   a.want_cfl = 1;
   a.want_check = checkbm != CHECK_NOTHING;
+  a.vout = ctx->vbuf;
   long pk = prof_begin(ctx);
   launch_rhs(ctx->g, a, ctx->V, ctx->st);
   prof_end(ctx, pk, K_RHS_END, 3.0 * word_bytes(ctx));
@@ -406,12 +410,24 @@ int sub_bg_main(fv_ctx *ctx) {  // physics_eulerisotherm.f90:1373-1413
   ctx->fl_cvar = ctx->fl_pvar = 1;
   return rc;
 }
+// FARGO type 1 CalcBackgroundVelocity (timedisc_base.f90:2243-2326): back to
+// the real frame with the old w, then w from the azimuthal velocities the
+// last ComputeRHS produced (kept in vbuf: the pvar that AddBackgroundVelocityY
+// restores is (v - w) + w of that evaluation)
+int calc_background_velocity(fv_ctx *ctx) {
+  int rc = add_bg_main(ctx);
+  if (rc) return rc;
+  launch_wsum(ctx->g, ctx->vbuf, ctx->d_ring, ctx->d_ringf, ctx->st);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
 
 // timedisc_base.f90:649-750
 int integration_step(fv_ctx *ctx) {
   int rc;
   double err = 0.0, dtold, dt, time;
-  if (ctx->c.fargo == 2 && (rc = sub_bg_main(ctx))) return rc;
+  if (ctx->c.fargo == 1 && (rc = calc_background_velocity(ctx))) return rc;
+  if (ctx->c.fargo && (rc = sub_bg_main(ctx))) return rc;
   time = ctx->time;
   dt = ctx->dt;
   if (dt < ctx->dtmin && ctx->dtcause != DTCAUSE_FILEIO) ctx->dtmin = dt;
@@ -443,7 +459,7 @@ int integration_step(fv_ctx *ctx) {
     ctx->dt = time - ctx->time;
     ctx->time = time;
     ctx->dtold = dt;
-    const int remap = ctx->c.fargo == 2;
+    const int remap = ctx->c.fargo != 0;
     // FARGO: CenterBoundary in the comoving frame, FargoAdvectionY and
     // Convert2Primitive (timedisc_base.f90:722-741) are folded into the
     // end-of-step RHS evaluation, which reads the remapped state on the fly
@@ -492,6 +508,7 @@ bool ring_from(fv_ctx *ctx, int id, const double *cells) {
 // batch and are redone by the host path above.
 bool fast_eligible(const fv_ctx *ctx) {
   if (!ctx->fast || ctx->iter < 1 || ctx->brk || ctx->cvar != ctx->cold) return false;
+  if (ctx->c.fargo == 1) return false;  // w changes every step: host-controlled path
   if (!(ctx->c.tol_rel < 1.0)) return false;  // AdjustTimestep active
   // the stop test of the next Step() is the host's (fosite.f90:370-373)
   if (std::fabs(ctx->c.stoptime - ctx->time) <= 1.0E-05 * ctx->c.stoptime) return false;
@@ -680,8 +697,8 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
   if (c.inum < 4 || c.jnum < 4) return bail("inum and jnum must be >= 4 (2-D meshes only)");
   if (c.recvars != 2) return bail("only CONSERVATIVE reconstruction (variables = 2) is supported");
   if (c.dtype != 0 && c.dtype != 1) return bail("dtype must be 0 (FP64) or 1 (FP32 precision form)");
-  if (c.fargo != 0 && c.fargo != 2) return bail("fargo must be 0 or 2");
-  if (c.fargo == 2 && (c.physics != EULER_ISOTHERM || c.geometry != LOGCYLINDRICAL))
+  if (c.fargo < 0 || c.fargo > 2) return bail("fargo must be 0, 1 or 2");
+  if (c.fargo && (c.physics != EULER_ISOTHERM || c.geometry != LOGCYLINDRICAL))
     return bail("FARGO requires isothermal physics on a logcylindrical mesh");
   switch (c.limiter) {
     case MINMOD: case MONOCENT: case SWEBY: case SUPERBEE: case OSPRE: case VANLEER: case NOLIMIT: break;
@@ -709,7 +726,7 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
   const struct { int r, m; } rm[] = {{R_DAXW, M_DAXW}, {R_DAYS, M_DAYS}, {R_DYDZDV, M_DYDZDV}, {R_DZDXDV, M_DZDXDV},
                                      {R_DXDYDV, M_DXDYDV}, {R_DLX, M_DLX}, {R_DLY, M_DLY}, {R_CXYX, M_CXYX},
                                      {R_CYXY, M_CYXY}, {R_CZXZ, M_CZXZ}, {R_CZYZ, M_CZYZ}, {R_RDXW, M_RDXW},
-                                     {R_RDXE, M_RDXE}, {R_ACCX, M_ACCX}, {R_ACCY, M_ACCY}};
+                                     {R_RDXE, M_RDXE}, {R_ACCX, M_ACCX}, {R_ACCY, M_ACCY}, {R_WFAC, M_WFAC}};
   for (auto &p : rm)
     if (!ring_from(ctx, p.r, ctx->mesh.a[p.m].data())) return bail("mesh metric depends on j");
   for (int j = 0; j < ctx->NJ; j++) {
@@ -733,6 +750,7 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
   for (int q = 1; q < ctx->tab.m && ok; q++) ok = zalloc(&ctx->k[q]);
   for (int q = 0; q < 2 && ok; q++) ok = zalloc(&ctx->k0[q]);
   const size_t bfbytes = sizeof(double) * 2 * ctx->V * (ctx->NJ + ctx->NI);
+  if (c.fargo == 1) ok = ok && dalloc(&ctx->vbuf, sizeof(double) * ctx->FS);
   for (int q = 0; q < 2 && ok; q++)  // InitFluxes: zero (fluxes_base.f90:209-214)
     ok = dalloc(&ctx->bf[q], bfbytes) && hipMemsetAsync(ctx->bf[q], 0, bfbytes, ctx->st) == hipSuccess;
   ok = ok && dalloc(&ctx->d_ring, sizeof(double) * R_COUNT * ctx->NI) && dalloc(&ctx->d_col, sizeof(double) * C_COUNT * ctx->NJ);
@@ -748,7 +766,7 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
   for (int d = 0; d < 4; d++) g.bc[d] = c.bc[d];
   g.bdydz = ctx->mesh.dy * ctx->mesh.dz;
   g.bdzdx = ctx->mesh.dz * ctx->mesh.dx;
-  g.fargo = c.fargo == 2; g.gravity = c.gravity != 0; g.curvilinear = c.geometry != CARTESIAN;
+  g.fargo = c.fargo != 0; g.gravity = c.gravity != 0; g.curvilinear = c.geometry != CARTESIAN;
   g.dtype = c.dtype;
   g.stages = ctx->tab.m;
   g.tile_x = 64;
@@ -793,6 +811,7 @@ extern "C" int fv_destroy(fv_ctx *ctx) {
   for (auto &b : ctx->k) if (b) (void)hipFree(b);
   for (auto &b : ctx->k0) if (b) (void)hipFree(b);
   for (auto &b : ctx->bf) if (b) (void)hipFree(b);
+  if (ctx->vbuf) (void)hipFree(ctx->vbuf);
   if (ctx->part_r) (void)hipFree(ctx->part_r);
   if (ctx->part_c) (void)hipFree(ctx->part_c);
   if (ctx->d_red) (void)hipFree(ctx->d_red);
@@ -822,6 +841,7 @@ extern "C" int fv_device_bytes(const fv_config *c, double *bytes) {
   const double nb32 = std::ceil(c->inum / 32.0) * std::ceil(c->jnum / (double)TY_TILE);
   double b = state * (NBUF + (t.m - 1) + 2);          // states, stage rhs, coeff(1) pair
   b += 2.0 * 8.0 * 2.0 * V * (NI + NJ);                // boundary fluxes, double-buffered
+  if (c->fargo == 1) b += 8.0 * NJ * P;                // FARGO type 1 velocities
   b += 2.0 * 2.0 * 8.0 * nb32;                         // CFL and error partials
   b += 8.0 * (R_COUNT * NI + C_COUNT * NJ);             // metric arrays
   if (c->dtype == 1) b += 4.0 * (R_COUNT * NI + C_COUNT * NJ);
@@ -897,13 +917,28 @@ extern "C" int fv_first_step(fv_ctx *ctx, int *brk) {
   if (ctx->c.fargo == 2 && !ctx->have_w) return fail(ctx, "fargo = 2 needs the background velocity");
   if (ctx->iter > -1) return fail(ctx, "FirstStep should only be called once");
   ctx->iter = 0;
-  int rc = rhs_main(ctx, CHECK_ALL, 0);
+  int rc;
+  if (ctx->c.fargo == 1) {
+    // CenterBoundary, then CalcBackgroundVelocity (fosite.f90:305-311): one
+    // evaluation with w = 0 on the initial state only for its velocities
+    // (the frame flags are clear, so the frame change is a no-op)
+    StageArgs a = base_args(ctx);
+    a.Y = ctx->buf[ctx->cvar];
+    a.kout = ctx->k0[1 - ctx->k0cur];
+    a.sub_bg = 1;
+    a.bflux = 0;
+    a.vout = ctx->vbuf;
+    launch_rhs(ctx->g, a, ctx->V, ctx->st);
+    HIPCHK(hipGetLastError());
+    if ((rc = calc_background_velocity(ctx))) return rc;
+  }
+  rc = rhs_main(ctx, CHECK_ALL, 0);
   if (rc) return rc;
   ctx->dt = calc_timestep(ctx);
   ctx->cold = ctx->cvar;
   ctx->fl_cold = ctx->fl_cvar;
   // initial WriteDataset moves the state to the real frame (io/fileio_base.f90:523-533)
-  if (ctx->time == 0.0 && ctx->c.fargo == 2 && (rc = add_bg_main(ctx))) return rc;
+  if (ctx->time == 0.0 && ctx->c.fargo && (rc = add_bg_main(ctx))) return rc;
   if (brk) *brk = ctx->brk;
   return 0;
 }
@@ -927,7 +962,7 @@ extern "C" int fv_step(fv_ctx *ctx, int *done) {
   }
   if ((rc = integration_step(ctx))) return rc;
   if (std::fabs(ctx->t_out - ctx->time) <= 1.0E-5 * ctx->t_out || ctx->brk) {
-    if (ctx->c.fargo == 2 && (rc = add_bg_main(ctx))) return rc;  // WriteDataset
+    if (ctx->c.fargo && (rc = add_bg_main(ctx))) return rc;  // WriteDataset
     if (ctx->brk) {
       if (done) *done = 1;
       return 0;
@@ -989,6 +1024,15 @@ extern "C" int fv_get_state(fv_ctx *ctx, double *cvar, double *pvar) {
       if (V == 4) pvar[3 * N + k] = (g - 1.) * (cv[3 * N + k] - 0.5 * inv * (mu * mu + mv * mv));
     }
   }
+  return 0;
+}
+
+extern "C" int fv_get_background_velocity(fv_ctx *ctx, double *w) {
+  if (!ctx || !w) return -1;
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  HIPCHK(hipMemcpyAsync(w, ctx->d_ring + (size_t)R_W * ctx->NI, sizeof(double) * ctx->NI, hipMemcpyDeviceToHost,
+                        ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
   return 0;
 }
 

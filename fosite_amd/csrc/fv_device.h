@@ -21,7 +21,7 @@ enum Limiter { MINMOD = 1, MONOCENT = 2, SWEBY = 3, SUPERBEE = 4, OSPRE = 5, VAN
 // except the y reconstruction offsets, which depend on j only.
 enum RingId {
   R_DAXW, R_DAYS, R_DYDZDV, R_DZDXDV, R_DXDYDV, R_DLX, R_DLY, R_CXYX, R_CYXY, R_CZXZ, R_CZYZ,
-  R_RDXW, R_RDXE, R_ACCX, R_ACCY, R_CS, R_CSW, R_CSE, R_CSS, R_CSN, R_W, R_COUNT
+  R_RDXW, R_RDXE, R_ACCX, R_ACCY, R_CS, R_CSW, R_CSE, R_CSS, R_CSN, R_W, R_WFAC, R_COUNT
 };
 enum ColId { C_RDYS, C_RDYN, C_COUNT };
 
@@ -92,6 +92,10 @@ struct StageArgsT {
   int bflux;
   const double *bf_in;
   double *bf_out;
+  // FARGO type 1: the azimuthal velocity of the real frame as the next
+  // CalcBackgroundVelocity sees it, (v - w) + w (timedisc_base.f90:2243-2326),
+  // for interior rows of every ring (ghost rings included); [NJ][P], or null
+  double *vout;
 };
 using StageArgs = StageArgsT<double>;
 
@@ -144,6 +148,9 @@ void launch_remap(const Geom &g, const RemapArgs &a, int V, hipStream_t s);
 void launch_frame(const Geom &g, const void *src, void *dst, int sign, int V, hipStream_t s);
 void launch_reduce(const double *part, int n, double *out, hipStream_t s);
 void launch_control(Ctl *ctl, const double *part_c, const double *part_r, int nb, hipStream_t s);
+// FARGO type 1: w(i) = SUM_j v(i,j)*wfactor(i) / JNUM per ring into the ring
+// arrays (FP64 and, for the precision form, FP32)
+void launch_wsum(const Geom &g, const double *v, double *ring, float *ringf, hipStream_t s);
 // batch launchers: ntiles blocks over the tile list; TX = tile width of the list
 #ifndef FV_RHS_ROWS
 #define FV_RHS_ROWS 8

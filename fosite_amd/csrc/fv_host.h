@@ -13,7 +13,7 @@ enum MeshId {
   M_DAXW, M_DAXE, M_DAYS, M_DAYN, M_DYDZDV, M_DZDXDV, M_DXDYDV, M_DLX, M_DLY,
   M_CXYX, M_CYXY, M_CZXZ, M_CZYZ, M_RDXW, M_RDXE, M_RDYS, M_RDYN, M_RAD,
   M_RADW, M_RADE, M_RADS, M_RADN, M_POSX, M_POSY, M_BCX, M_BCY, M_ACCX, M_ACCY,
-  M_COUNT
+  M_WFAC, M_COUNT
 };
 
 struct Mesh {

@@ -681,6 +681,10 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
 #pragma unroll
           for (int l = 0; l < V; l++) (a.yout + l * g.FS)[ix] = u[l];
         }
+        if (a.vout && yin) {  // x-ghost ring of an interior row: (v - w) + w
+          const T w = rg<T>(g, R_W, ii), vc = a.sub_bg ? pp[2] : pp[2] - w;
+          a.vout[(unsigned)jj * (unsigned)g.P + (unsigned)ii] = (double)(vc + w);
+        }
       } else {
 #pragma unroll
         for (int l = 0; l < V; l++) u[l] = zero;  // corner or beyond the mesh: never needed
@@ -700,6 +704,10 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
     if (out) {
       in.transform(i, so, ur, pr, uo);
       if constexpr (sizeof(T) == 4) vres = dv(so[2], so[0]);
+      if (a.vout) {  // pvar after SubtractBackgroundVelocityY, then + w again
+        const T w = rg<T>(g, R_W, i);
+        a.vout[(unsigned)j * (unsigned)g.P + (unsigned)i] = (double)((pr[2] - w) + w);
+      }
 #pragma unroll
       for (int l = 0; l < V; l++) UC(l, ty, tx) = uo[l];
     }

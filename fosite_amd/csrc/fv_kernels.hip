@@ -56,6 +56,42 @@ __global__ __launch_bounds__(256) void k_reduce(const double *part, int n, doubl
   }
 }
 
+// FARGO type 1 background velocity (timedisc_base.f90:2243-2326, direction
+// 2): w(i) = SUM(v(i,JMIN:JMAX)*wfactor(i)) / JNUM for every ring, the sum in
+// index order.  A block covers 64 rings; waves 1-3 stage the rows of the
+// next chunk in LDS while wave 0 adds the current one sequentially.
+constexpr int WS_RINGS = 64, WS_ROWS = 64;
+__global__ __launch_bounds__(256) void k_wsum(Geom g, const double *__restrict__ v, double *ring, float *ringf) {
+  __shared__ double buf[2][WS_ROWS][WS_RINGS];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int i = blockIdx.x * WS_RINGS + lane, jmin = GH, J = g.jnum;
+  const int nchunk = (J + WS_ROWS - 1) / WS_ROWS;
+  auto stage = [&](int c) {  // waves 1..3: rows of chunk c into buf[c & 1]
+    for (int r = wv - 1; r < WS_ROWS; r += 3) {
+      const int j = c * WS_ROWS + r;
+      buf[c & 1][r][lane] = (i < g.NI && j < J) ? v[(long)(jmin + j) * g.P + i] : 0.0;
+    }
+  };
+  if (wv > 0) stage(0);
+  __syncthreads();
+  double acc = 0.0;
+  const double f = i < g.NI ? g.ring[(long)R_WFAC * g.NI + i] : 0.0;
+  for (int c = 0; c < nchunk; c++) {
+    if (wv > 0) {
+      if (c + 1 < nchunk) stage(c + 1);
+    } else {
+      const int rows = J - c * WS_ROWS < WS_ROWS ? J - c * WS_ROWS : WS_ROWS;
+      for (int r = 0; r < rows; r++) acc = acc + buf[c & 1][r][lane] * f;
+    }
+    __syncthreads();
+  }
+  if (wv == 0 && i < g.NI) {
+    const double w = acc / (double)J;
+    ring[(long)R_W * g.NI + i] = w;
+    if (ringf) ringf[(long)R_W * g.NI + i] = (float)w;
+  }
+}
+
 // -------------------------------------------------------------- launchers
 // The host fills StageArgs / RemapArgs with untyped buffer addresses; the
 // FP32 form reads the same fields as their float counterparts (identical
@@ -113,6 +149,9 @@ void launch_remap_multi(const RemapEntry *tab, const unsigned *tiles, int ntiles
 }
 void launch_control_multi(const ControlEntry *tab, int n, hipStream_t s) {
   hipLaunchKernelGGL(k_control_multi, dim3(n), dim3(256), 0, s, tab);
+}
+void launch_wsum(const Geom &g, const double *v, double *ring, float *ringf, hipStream_t s) {
+  hipLaunchKernelGGL(k_wsum, dim3((g.NI + WS_RINGS - 1) / WS_RINGS), dim3(256), 0, s, g, v, ring, ringf);
 }
 void launch_reduce(const double *part, int n, double *out, hipStream_t s) {
   hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, s, part, n, out);

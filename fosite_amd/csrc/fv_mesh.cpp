@@ -7,6 +7,7 @@
 // gravity list sum sources/sources_gravity.f90:226).  Operation order follows
 // the reference so that the values are bit-identical; the device only ever
 // sees the per-ring (i) and per-column (j) slices of these arrays.
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <vector>
@@ -62,6 +63,10 @@ int mesh_build(const fv_config &c, Mesh &m) {
       m.a[M_CXYX][k] = 0.5 * (hzN + hzS) * (hxN - hxS) * m.a[M_DZDXDV][k];
       m.a[M_CZXZ][k] = 0.5 * (hyE + hyW) * (hzE - hzW) * m.a[M_DYDZDV][k];
       m.a[M_CZYZ][k] = 0.5 * (hxN + hxS) * (hzN - hzS) * m.a[M_DZDXDV][k];
+      // FARGO type 1 wfactor (timedisc_base.f90:428-447, direction 2): the
+      // smallest hy over the W, E, bottom and top faces (bottom/top at the
+      // centre's x) over hy at the bary-centre
+      m.a[M_WFAC][k] = std::min(std::min(hyW, hyE), std::min(hxS, hxS)) / hyc;
       m.a[M_DLX][k] = hxc * m.dx;
       m.a[M_DLY][k] = hyc * m.dy;
       // reconstruction offsets faces - bary centre (reconstruction_linear.f90:156-177)

@@ -12,7 +12,8 @@ A configuration is a plain dict whose keys follow the reference's Dict keys
                                                  reconstruction_base.f90:117)
   method, order, cfl, tol_rel, tol_abs, stoptime, dtlimit, beta
                                                 (timedisc_base.f90:187-191, 315-352)
-  fargo     0 off | 2 user-supplied background velocity (mesh_base.f90:372-423)
+  fargo     0 off | 1 per-ring mean velocity, every step | 2 user-supplied
+            background velocity (mesh_base.f90:372-423)
   gravity, mass, GN                             (gravity_pointmass.f90:132-301)
 
 Initial data is returned as primitive variables on all cells (ghosts

@@ -41,7 +41,9 @@ typedef struct fv_config {
   int method, order;/* 2 RK_FEHLBERG, 4 DORMAND_PRINCE, 5 SSPRK (timedisc/timedisc_base.f90:187-191) */
   double cfl, tol_rel, tol_abs[4], stoptime, dtlimit, beta; /* timedisc_base.f90:315-352;
                        dtlimit <= 0 selects the default EPSILON*stoptime */
-  int fargo;        /* 0 off, 2 user-supplied background velocity (mesh/mesh_base.f90:372-423) */
+  int fargo;        /* 0 off, 1 per-ring mean azimuthal velocity recomputed every step
+                     (timedisc/timedisc_base.f90:2243-2326; host-controlled steps),
+                     2 user-supplied background velocity (mesh/mesh_base.f90:372-423) */
   int gravity;      /* 1: point mass at the origin, Newtonian, no accretion
                        (sources/gravity_pointmass.f90:132-301 with outbound = 0) */
   double mass, GN;  /* point mass and gravitational constant (GEOMETRICAL units: GN = 1) */
@@ -61,7 +63,8 @@ typedef struct fv_scalars {
  * Ids: 0 dAxdydz(W) 1 dAxdydz(E) 2 dAydzdx(S) 3 dAydzdx(N) 4 dydzdV 5 dzdxdV
  * 6 dxdydV 7 dlx 8 dly 9 cxyx 10 cyxy 11 czxz 12 czyz 13..16 reconstruction
  * offsets W,E,S,N 17 radius(bcenter) 18..21 radius(faces W,E,S,N)
- * 22 posvec_x 23 posvec_y 24 bcenter x 25 bcenter y 26 accel_x 27 accel_y.
+ * 22 posvec_x 23 posvec_y 24 bcenter x 25 bcenter y 26 accel_x 27 accel_y
+ * 28 FARGO type-1 wfactor (timedisc/timedisc_base.f90:428-447).
  * Replaces mesh/mesh_midpoint.f90:73-146 (InitMesh_midpoint) and
  * sources/gravity_pointmass.f90:263-282 (static acceleration). */
 int fv_mesh_array(const fv_config *cfg, int id, double *out);
@@ -104,6 +107,12 @@ int fv_run(fv_ctx *ctx, int nsteps, int *taken);
  * variables Convert2Primitive(cvar) (either pointer may be NULL). */
 int fv_get_state(fv_ctx *ctx, double *cvar, double *pvar);
 int fv_get_scalars(const fv_ctx *ctx, fv_scalars *out);
+/* Timedisc%w, the FARGO background velocity per x-ring (NI values, ghost
+ * rings included; timedisc/timedisc_base.f90:389,420): the value set by
+ * fv_set_background_velocity (fargo 2), or the one CalcBackgroundVelocity
+ * computed at the start of the last step (fargo 1,
+ * timedisc/timedisc_base.f90:2243-2326). */
+int fv_get_background_velocity(fv_ctx *ctx, double *w);
 /* Fluxes%bxflux and Fluxes%byflux (fluxes/fluxes_base.f90:142-143): the
  * fluxes through the mesh boundaries accumulated over every accepted step,
  * sum_m -dt*b_high(m)*rhs_m at the boundary rows

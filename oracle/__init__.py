@@ -73,6 +73,7 @@ def _load():
         L.fvr_compute_rhs.argtypes = [ctypes.c_void_p, P]
         L.fvr_round_state_f32.argtypes = [ctypes.c_void_p]
         L.fvr_get_bflux.argtypes = [ctypes.c_void_p, P, P]
+        L.fvr_get_background_velocity.argtypes = [ctypes.c_void_p, P]
         _lib = L
     return _lib
 
@@ -160,6 +161,12 @@ class Oracle:
         sc = _Scalars()
         self.L.fvr_get(self._h, _ptr(cv), _ptr(pv), ctypes.byref(sc))
         return cv, pv, {k: getattr(sc, k) for k, _ in _Scalars._fields_}
+
+    def background_velocity(self):
+        """Timedisc%w (one value per x-ring, ghost rings included)."""
+        w = np.empty(self.NI)
+        self.L.fvr_get_background_velocity(self._h, _ptr(w))
+        return w
 
     def bflux(self):
         """Accumulated boundary fluxes (Fluxes%bxflux, byflux): arrays

@@ -40,7 +40,7 @@ PROGRAM fv_driver
   TYPE(Dict_TYP), POINTER    :: config, mesh, phys, bnd, flx, tdisc, dfile, &
                                 src, pmass
   CHARACTER(LEN=512)         :: cfgname
-  INTEGER                    :: u, ub, n, vnum, ncell
+  INTEGER                    :: u, ub, uw, n, vnum, ncell
   INTEGER(KIND=8)            :: c0, c1, crate, ctot
   LOGICAL                    :: brk
   REAL, ALLOCATABLE          :: pv(:), cs(:), fcs(:), w(:)
@@ -120,6 +120,8 @@ PROGRAM fv_driver
        FORM="unformatted", STATUS="replace")
   OPEN(NEWUNIT=ub, FILE=TRIM(k_prefix) // "_bflux.bin", ACCESS="stream", &
        FORM="unformatted", STATUS="replace")
+  OPEN(NEWUNIT=uw, FILE=TRIM(k_prefix) // "_w.bin", ACCESS="stream", &
+       FORM="unformatted", STATUS="replace")
   CALL Sim%FirstStep()
   CALL DumpState(u, 0)
   ctot = 0
@@ -134,6 +136,7 @@ PROGRAM fv_driver
   END DO
   CLOSE(u)
   CLOSE(ub)
+  CLOSE(uw)
   ! wall time of the Step() calls only (setup, FirstStep and dumps excluded)
   WRITE(*,'(A,I0,A,ES24.16)') "FV_DRIVER_STEP_SECONDS ", MIN(n, k_nsteps), " ", &
        REAL(ctot, 8) / REAL(crate, 8)
@@ -166,6 +169,8 @@ CONTAINS
               Sim%Fluxes%GetBoundaryFlux(Sim%Mesh, Sim%Physics, EAST), &
               Sim%Fluxes%GetBoundaryFlux(Sim%Mesh, Sim%Physics, SOUTH), &
               Sim%Fluxes%GetBoundaryFlux(Sim%Mesh, Sim%Physics, NORTH)
+    ! FARGO background velocity Timedisc%w(IGMIN:IGMAX,1)
+    IF (k_fargo.GT.0) WRITE(uw) Sim%Timedisc%w(:,1)
   END SUBROUTINE DumpState
 
   !> metric arrays the GPU context is built from (all cells incl. ghosts)

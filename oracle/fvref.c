@@ -57,6 +57,7 @@ struct fvr {
   /* physics */
   double *bccsound, *fcsound; /* fcsound [face][cell] */
   double *w;                  /* fargo background velocity per x-ring */
+  double *wfac;               /* FARGO type 1: geometric factor per x-ring */
   /* timedisc */
   sv pvar, cvar, ptmp, ctmp, cold;
   double *geo_src, *src, *sterm, **coeff, *cerr, *xflux, *yflux;
@@ -807,6 +808,20 @@ static void copy_sv(fvr *s, sv *dst, const sv *src) {
   dst->flag = src->flag;
 }
 
+/* FARGO type 1 (timedisc_base.f90:2243-2326, direction 2): back to the real
+ * frame with the old w, then per x-ring (ghost rings included) the mean of
+ * the azimuthal velocity over JMIN:JMAX weighted by wfactor; SUM in index
+ * order, as the reference build evaluates it */
+static void calc_background_velocity(fvr *s) {
+  const int N = s->N, jmin = G, jmax = G + s->c.jnum - 1;
+  add_bg_velocity(s, &s->pvar, &s->cvar);
+  for (int i = 0; i < s->NI; i++) {
+    double wi = 0.0;
+    for (int j = jmin; j <= jmax; j++) wi = wi + s->pvar.d[2 * N + IDX(s, i, j)] * s->wfac[i];
+    s->w[i] = wi / (double)s->c.jnum;
+  }
+}
+
 /* timedisc_base.f90:1935-2069 (2-D, direction 2) */
 static void fargo_advection_y(fvr *s) {
   int N = s->N, NI = s->NI, jmin = G, jmax = G + s->c.jnum - 1;
@@ -879,6 +894,7 @@ static void accept_solution(fvr *s, double *time, double dt) {
 /* timedisc_base.f90:649-750 */
 static void integration_step(fvr *s) {
   double err = 0.0, dtold, dt, time;
+  if (s->c.fargo == 1) calc_background_velocity(s); /* timedisc_base.f90:664-665 */
   if (s->c.fargo == 1 || s->c.fargo == 2) sub_bg_velocity(s, &s->pvar, &s->cvar);
   time = s->time;
   dt = s->dt;
@@ -900,7 +916,7 @@ static void integration_step(fvr *s) {
     s->dt = time - s->time;
     s->time = time;
     s->dtold = dt;
-    if (s->c.fargo == 2) {
+    if (s->c.fargo) {
       center_boundary(s, &s->pvar, &s->cvar);
       fargo_advection_y(s);
       for (int k = 0; k < s->N; k++) cons2prim_pt(s, s->pvar.d, s->cvar.d, s->N, k);
@@ -970,7 +986,7 @@ fvr *fvr_create(const fvr_cfg *cfg) {
   if (cfg->physics != EULER && cfg->physics != EULER_ISOTHERM) return NULL;
   if (cfg->geometry != CARTESIAN && cfg->geometry != LOGCYLINDRICAL) return NULL;
   if (cfg->inum < 4 || cfg->jnum < 4) return NULL;
-  if (cfg->fargo != 0 && cfg->fargo != 2) return NULL;
+  if (cfg->fargo < 0 || cfg->fargo > 2) return NULL;
   if (cfg->fargo && cfg->physics != EULER_ISOTHERM) return NULL;
   for (int d = 0; d < 4; d++) {
     int b = cfg->bc[d];
@@ -991,7 +1007,7 @@ fvr *fvr_create(const fvr_cfg *cfg) {
     s->cxyx = zalloc(N); s->cyxy = zalloc(N); s->czxz = zalloc(N); s->czyz = zalloc(N);
     for (int f = 0; f < 4; f++) { s->rdx[f] = zalloc(N); s->radf[f] = zalloc(N); }
     s->rad = zalloc(N); s->posx = zalloc(N); s->posy = zalloc(N); s->bcx = zalloc(N); s->bcy = zalloc(N);
-    s->bccsound = zalloc(N); s->fcsound = zalloc(4 * N); s->w = zalloc(s->NI);
+    s->bccsound = zalloc(N); s->fcsound = zalloc(4 * N); s->w = zalloc(s->NI); s->wfac = zalloc(s->NI);
     s->pvar.d = zalloc(V * N); s->cvar.d = zalloc(V * N); s->ptmp.d = zalloc(V * N);
     s->ctmp.d = zalloc(V * N); s->cold.d = zalloc(V * N);
     s->geo_src = zalloc(V * N); s->src = zalloc(V * N); s->sterm = zalloc(V * N); s->cerr = zalloc(V * N);
@@ -1005,6 +1021,16 @@ fvr *fvr_create(const fvr_cfg *cfg) {
     s->slopes = zalloc(2 * V * N);
   }
   mesh_build(s);
+  /* FARGO type 1 wfactor (timedisc_base.f90:428-447, direction 2): the
+   * smallest hy over the W, E, bottom and top faces of the ring over hy at
+   * the bary-centre; the bottom/top faces sit at the centre's x (mesh_dz = 0) */
+  for (int i = 0; i < s->NI; i++) {
+    const double mdx = s->dx, xc = s->c.xmin + ((double)(2 * (i - 1) - 1) * 0.5) * mdx;
+    const double xw = xc + (-(0.5 * mdx)), xe = xc + 0.5 * mdx, xbt = xc + 0.0;
+    double hw = hscale(s, xw), he = hscale(s, xe), hbt = hscale(s, xbt), hc = hscale(s, xc);
+    if (s->c.geometry == CARTESIAN) hw = he = hbt = hc = 1.0;
+    s->wfac[i] = fmin2(fmin2(hw, he), fmin2(hbt, hbt)) / hc;
+  }
   /* timedisc_base.f90:224-352 defaults */
   s->time = 0.0;
   s->dt = cfg->stoptime;
@@ -1026,7 +1052,7 @@ void fvr_destroy(fvr *s) {
   free(s->cxyx); free(s->cyxy); free(s->czxz); free(s->czyz);
   for (int f = 0; f < 4; f++) { free(s->rdx[f]); free(s->radf[f]); }
   free(s->rad); free(s->posx); free(s->posy); free(s->bcx); free(s->bcy);
-  free(s->bccsound); free(s->fcsound); free(s->w);
+  free(s->bccsound); free(s->fcsound); free(s->w); free(s->wfac);
   free(s->pvar.d); free(s->cvar.d); free(s->ptmp.d); free(s->ctmp.d); free(s->cold.d);
   free(s->geo_src); free(s->src); free(s->sterm); free(s->cerr); free(s->xflux); free(s->yflux);
   free(s->bx); free(s->by); free(s->bxo); free(s->byo);
@@ -1067,6 +1093,10 @@ void fvr_set_pvar(fvr *s, const double *pvar) {
 /* fosite.f90:258-348 */
 int fvr_first_step(fvr *s) {
   s->iter = 0;
+  if (s->c.fargo == 1) { /* fosite.f90:305-311 */
+    center_boundary(s, &s->pvar, &s->cvar);
+    calc_background_velocity(s);
+  }
   compute_rhs(s, &s->pvar, &s->cvar, CHECK_ALL, s->coeff[0]);
   s->dt = calc_timestep(s);
   copy_sv(s, &s->cold, &s->cvar);
@@ -1108,6 +1138,8 @@ void fvr_get(const fvr *s, double *cvar, double *pvar, fvr_scalars *sc) {
     sc->time = s->time; sc->dt = s->dt; sc->dtold = s->dtold; sc->maxerrold = s->maxerrold;
   }
 }
+
+void fvr_get_background_velocity(const fvr *s, double *w) { memcpy(w, s->w, sizeof(double) * s->NI); }
 
 void fvr_get_bflux(const fvr *s, double *bx, double *by) {
   if (bx) memcpy(bx, s->bx, sizeof(double) * 2 * s->V * s->NJ);

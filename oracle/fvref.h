@@ -30,7 +30,7 @@ typedef struct fvr_cfg {
   int recvars;        /* 1 PRIMITIVE, 2 CONSERVATIVE */
   int method, order;  /* 2 RK_FEHLBERG, 3 CASH_KARP, 4 DORMAND_PRINCE, 5 SSPRK */
   double cfl, tol_rel, tol_abs[4], stoptime, dtlimit, beta;
-  int fargo;          /* 0 off, 2 user-supplied background velocity */
+  int fargo;          /* 0 off, 1 mean azimuthal velocity per ring (dynamic), 2 user-supplied */
   int gravity;        /* 1: central point mass at the origin, no accretion */
   double mass, GN;
 } fvr_cfg;
@@ -60,6 +60,8 @@ int fvr_first_step(fvr *s);
 /* fosite%Step: returns 1 if the run ended (stop time or break) */
 int fvr_step(fvr *s);
 void fvr_get(const fvr *s, double *cvar, double *pvar, fvr_scalars *sc);
+/* Timedisc%w: FARGO background velocity per x-ring (NI values) */
+void fvr_get_background_velocity(const fvr *s, double *w);
 /* accumulated boundary fluxes Fluxes%bxflux / byflux in the reference's
  * layout (fluxes_base.f90:142-143): bx(JGMIN:JGMAX,1,2,VNUM) = NJ*2*V values,
  * by(1,IGMIN:IGMAX,2,VNUM) = NI*2*V values (side 1 west/south, 2 east/north) */

@@ -105,6 +105,10 @@ def run(cfg, init, nsteps, dump_every=1, workdir=None):
         r["bx"] = b[:2 * V * NJ].reshape(V, 2, NJ).copy()
         r["by"] = b[2 * V * NJ:2 * V * (NJ + NI)].reshape(V, 2, NI).copy()
         r["bsum"] = b[2 * V * (NJ + NI):].reshape(4, V).copy()
+    if cfg.get("fargo", 0):
+        wraw = np.fromfile(prefix + "_w.bin", dtype="<f8")
+        for n, r in enumerate(recs):
+            r["w"] = wraw[n * NI:(n + 1) * NI].copy()
     mesh["step_seconds"] = step_seconds
     if workdir is None:
         shutil.rmtree(tmp, ignore_errors=True)

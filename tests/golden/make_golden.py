@@ -48,6 +48,9 @@ CASES = {
     "euler_disk_absorbing_dp5_24x48": (S.euler_disk_config(ni=24, nj=48), 12, (1, 12)),
     "euler_disk_absorbing_ssprk3": (dict(S.euler_disk_config(ni=16, nj=32), method=S.SSPRK, order=3,
                                          tol_rel=0.01), 10, (10,)),
+    # FARGO type 1: background velocity recomputed every step
+    "disk_fargo1_absorbing": (dict(D, fargo=1), 10, (1, 10)),
+    "disk_fargo1_cfl_0.9_rejects": (dict(D, fargo=1, cfl=0.9), 12, (12,)),
 }
 MESHES = {"cartesian_16": E, "logcyl_16x32": D}
 SCALARS = ("iter", "n_adj", "brk", "time", "dt", "dtold", "maxerrold")
@@ -72,6 +75,8 @@ def main():
             out["%s/bx_%d" % (name, s)] = recs[s]["bx"]
             out["%s/by_%d" % (name, s)] = recs[s]["by"]
             out["%s/bsum_%d" % (name, s)] = recs[s]["bsum"]
+            if cfg.get("fargo", 0):  # Timedisc%w(IGMIN:IGMAX)
+                out["%s/w_%d" % (name, s)] = recs[s]["w"]
         print(name, "n_adj", recs[-1]["n_adj"], "time %.6e" % recs[-1]["time"])
     for name, cfg in MESHES.items():
         mesh, _ = refrun.run(cfg, S.initial(cfg), 0, dump_every=1)

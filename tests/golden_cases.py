@@ -32,6 +32,12 @@ def case(name):
     return cfg, steps, scalars, states
 
 
+def background_velocity(name):
+    """{step: Timedisc%w} of the stored steps of a FARGO case"""
+    d = data()
+    return {int(k.split("_")[-1]): d[k] for k in d if k.startswith(name + "/w_")}
+
+
 def bflux(name):
     """{step: (bx [V,2,NJ], by [V,2,NI], sums [4,V] of W,E,S,N)} of the
     steps whose state is stored."""

@@ -60,6 +60,31 @@ def test_oracle_matches_reference_fixture(name):
         assert np.array_equal(got[s][I], ref[I]), (name, s, np.abs(got[s][I] - ref[I]).max())
 
 
+@pytest.mark.parametrize("name", [n for n in G.cases() if G.background_velocity(n)])
+def test_oracle_background_velocity_matches_reference_fixture(name):
+    """Timedisc%w after the stored steps, bit for bit (FARGO type 1 computes
+    it every step from the mean azimuthal velocity of each ring)."""
+    cfg, steps, _, _ = G.case(name)
+    ref = G.background_velocity(name)
+    init = S.initial(cfg)
+    o = oracle.Oracle(cfg)
+    o.set_initial(init["pvar"], init.get("bccsound"), init.get("fcsound"), init.get("w"))
+    o.first_step()
+    for n in range(1, steps + 1):
+        o.step()
+        if n in ref:
+            assert np.array_equal(o.background_velocity(), ref[n]), (name, n)
+
+
+def test_fargo_type1_fixture_differs_from_type2():
+    """The type-1 fixtures exercise a background velocity that is not the
+    initial Keplerian one (so a type-2 run would not reproduce them)."""
+    w = G.background_velocity("disk_fargo1_absorbing")[10]
+    cfg = G.case("disk_fargo1_absorbing")[0]
+    w2 = S.initial(dict(cfg, fargo=2))["w"]
+    assert np.abs(w - w2).max() > 1e-6
+
+
 @pytest.mark.parametrize("name", [n for n in G.cases() if G.bflux(n)])
 def test_oracle_boundary_flux_matches_reference_fixture(name):
     """Fluxes%bxflux / byflux after the stored steps, bit for bit, and the
