@@ -117,6 +117,7 @@ struct fv_ctx {
   // other copy; AcceptSolution is a role swap, RejectSolution keeps bcur
   double *bf[2] = {};
   int bcur = 0;
+  double *bks = nullptr;  // boundary rows of the last stage's rhs
   // FARGO type 1: azimuthal velocities the last ComputeRHS left for the next
   // CalcBackgroundVelocity ([NJ][P], FP64)
   double *vbuf = nullptr;
@@ -294,8 +295,7 @@ int rhs_main(fv_ctx *ctx, int checkbm, int remap) {
 
 // arguments of stage m (1..s-1) on the state y with coeff(1) in k0[k0],
 // the last stage (m = s-1) fused with the final combination into buffer out
-// and the boundary-flux accumulation bf[bin] -> bf[1 - bin]
-StageArgs stage_args(fv_ctx *ctx, int m, int y, int k0, int out, int bin) {
+StageArgs stage_args(fv_ctx *ctx, int m, int y, int k0, int out) {
   StageArgs a = base_args(ctx);
   a.Y = ctx->buf[y];
   a.nk = m;
@@ -312,13 +312,35 @@ StageArgs stage_args(fv_ctx *ctx, int m, int y, int k0, int out, int bin) {
     a.tol_rel = ctx->c.tol_rel;
     for (int l = 0; l < 4; l++) a.tol_abs[l] = ctx->c.tol_abs[l];
     a.part_c = ctx->part_c;
-    a.bf_in = ctx->bf[bin];
-    a.bf_out = ctx->bf[1 - bin];
+    a.bks = ctx->bks;
   } else {
     a.kout = ctx->k[m];
   }
   return a;
 }
+// boundary-flux accumulation of a step with coeff(1) in k0[k0]: bf[bin] ->
+// bf[1 - bin] (dt: host-controlled steps; device rounds read Ctl::dt)
+BfluxArgs bflux_args(const fv_ctx *ctx, int k0, int bin, double dt) {
+  BfluxArgs b;
+  std::memset(&b, 0, sizeof(b));
+  for (int q = 0; q < ctx->tab.m - 1; q++) b.k[q] = q ? ctx->k[q] : ctx->k0[k0];
+  for (int q = 0; q < ctx->tab.m; q++) b.bh[q] = ctx->tab.bh[q];
+  b.ks = ctx->bks;
+  b.in = ctx->bf[bin];
+  b.out = ctx->bf[1 - bin];
+  b.dt = dt;
+  b.nk = ctx->tab.m - 1;
+  b.V = ctx->V;
+  b.NI = ctx->NI;
+  b.NJ = ctx->NJ;
+  b.P = ctx->P;
+  b.inum = ctx->c.inum;
+  b.jnum = ctx->c.jnum;
+  b.dtype = ctx->c.dtype;
+  b.FS = ctx->FS;
+  return b;
+}
+
 void launch_stage(fv_ctx *ctx, const StageArgs &a, int m) {
   const bool last = m == ctx->tab.m - 1;
   long pk = prof_begin(ctx);
@@ -330,7 +352,7 @@ void launch_stage(fv_ctx *ctx, const StageArgs &a, int m) {
 int rhs_stage(fv_ctx *ctx, int m, double dt, int out) {
   int add, sub;
   rhs_flags(ctx, ctx->fl_ptmp, ctx->fl_ctmp, add, sub);
-  StageArgs a = stage_args(ctx, m, ctx->cvar, ctx->k0cur, out, ctx->bcur);
+  StageArgs a = stage_args(ctx, m, ctx->cvar, ctx->k0cur, out);
   a.dt = dt;
   a.add_bg = add;
   a.sub_bg = sub;
@@ -370,6 +392,8 @@ int solve_ode(fv_ctx *ctx, double *dt, double *err) {
     int rc = rhs_stage(ctx, m, *dt, outb);
     if (rc) return rc;
   }
+  launch_bflux(bflux_args(ctx, ctx->k0cur, ctx->bcur, *dt), ctx->st);
+  HIPCHK(hipGetLastError());
   ctx->fl_ctmp = ctx->fl_cvar;  // ctmp = cvar
   ctx->cvar = outb;
   double e;
@@ -529,7 +553,7 @@ void fast_step_args(fv_ctx *ctx, Ctl *ctl, int xin, int xout, int tb, int rb, in
                     FastStep &f) {
   const int steady = ctx->c.fargo == 2 ? 1 : 0;
   for (int m = 1; m < ctx->tab.m; m++) {
-    StageArgs a = stage_args(ctx, m, xin, k0in, tb, bin);
+    StageArgs a = stage_args(ctx, m, xin, k0in, tb);
     a.ctl = ctl;
     a.add_bg = a.sub_bg = steady;
     f.st[m - 1] = a;
@@ -549,6 +573,7 @@ void fast_step_args(fv_ctx *ctx, Ctl *ctl, int xin, int xout, int tb, int rb, in
   a.add_bg = a.sub_bg = steady;
   a.pmin_clamp = ctx->V == 4 ? 1 : 0;
   a.want_cfl = a.want_check = 1;
+  a.bfa = bflux_args(ctx, k0in, bin, 0.0);  // the step's boundary fluxes
   f.end = a;
 }
 
@@ -751,6 +776,7 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
   for (int q = 0; q < 2 && ok; q++) ok = zalloc(&ctx->k0[q]);
   const size_t bfbytes = sizeof(double) * 2 * ctx->V * (ctx->NJ + ctx->NI);
   if (c.fargo == 1) ok = ok && dalloc(&ctx->vbuf, sizeof(double) * ctx->FS);
+  ok = ok && dalloc(&ctx->bks, sizeof(double) * 2 * ctx->V * (ctx->NJ + ctx->NI));
   for (int q = 0; q < 2 && ok; q++)  // InitFluxes: zero (fluxes_base.f90:209-214)
     ok = dalloc(&ctx->bf[q], bfbytes) && hipMemsetAsync(ctx->bf[q], 0, bfbytes, ctx->st) == hipSuccess;
   ok = ok && dalloc(&ctx->d_ring, sizeof(double) * R_COUNT * ctx->NI) && dalloc(&ctx->d_col, sizeof(double) * C_COUNT * ctx->NJ);
@@ -812,6 +838,7 @@ extern "C" int fv_destroy(fv_ctx *ctx) {
   for (auto &b : ctx->k0) if (b) (void)hipFree(b);
   for (auto &b : ctx->bf) if (b) (void)hipFree(b);
   if (ctx->vbuf) (void)hipFree(ctx->vbuf);
+  if (ctx->bks) (void)hipFree(ctx->bks);
   if (ctx->part_r) (void)hipFree(ctx->part_r);
   if (ctx->part_c) (void)hipFree(ctx->part_c);
   if (ctx->d_red) (void)hipFree(ctx->d_red);
@@ -840,7 +867,7 @@ extern "C" int fv_device_bytes(const fv_config *c, double *bytes) {
   const double state = es * V * NJ * P;  // one state buffer, ghost cells and row padding included
   const double nb32 = std::ceil(c->inum / 32.0) * std::ceil(c->jnum / (double)TY_TILE);
   double b = state * (NBUF + (t.m - 1) + 2);          // states, stage rhs, coeff(1) pair
-  b += 2.0 * 8.0 * 2.0 * V * (NI + NJ);                // boundary fluxes, double-buffered
+  b += 3.0 * 8.0 * 2.0 * V * (NI + NJ);                // boundary fluxes (two copies), last-stage rows
   if (c->fargo == 1) b += 8.0 * NJ * P;                // FARGO type 1 velocities
   b += 2.0 * 2.0 * 8.0 * nb32;                         // CFL and error partials
   b += 8.0 * (R_COUNT * NI + C_COUNT * NJ);             // metric arrays
@@ -1227,7 +1254,7 @@ int rebuild(fv_batch *b) {
   for (int d = 0; d < n; d++) {
     const fv_ctx *x = b->sims[d];
     if (x->c.fargo != 2) continue;
-    const int nbx = (x->c.inum + 63) / 64, nby = (x->c.jnum + 3) / 4;
+    const int nbx = (x->c.inum + 63) / 64, nby = remap_rows_blocks(x->c.jnum);
     for (int by = 0; by < nby; by++)
       for (int bx = 0; bx < nbx; bx++) all.push_back(tile_code(d, bx, by));
   }
@@ -1390,7 +1417,7 @@ extern "C" int fv_batch_add(fv_batch *b, fv_ctx *x) {
   if (!b || !x) return -1;
   if (x->batch) return bfail(b, "context already belongs to a batch");
   if ((int)b->sims.size() >= MAX_BATCH) return bfail(b, "batch is full (" + std::to_string(MAX_BATCH) + " members)");
-  const int nbx = (x->c.inum + 31) / 32, nby = (x->c.jnum + TY_TILE - 1) / TY_TILE, rby = (x->c.jnum + 3) / 4;
+  const int nbx = (x->c.inum + 31) / 32, nby = (x->c.jnum + TY_TILE - 1) / TY_TILE, rby = remap_rows_blocks(x->c.jnum);
   if (nbx >= (1 << TILE_BX_BITS) || nby >= (1 << TILE_BY_BITS) || rby >= (1 << TILE_BY_BITS))
     return bfail(b, "mesh too large for a batch tile code");
   if (b->sims.empty()) {

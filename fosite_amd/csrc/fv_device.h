@@ -59,6 +59,20 @@ struct Ctl {
   double err_hist[CTL_HIST], dt_hist[CTL_HIST];
 };
 
+// Boundary-flux accumulation of one step (timedisc_rkfehlberg.f90:216-259):
+// out = in - sum_q dt*b_high(q)*k_q over the boundary rows, k_1..k_{s-1} from
+// the ghost positions of the stage rhs arrays (element type by dtype), k_s
+// from ks.  in/out/ks use the boundary-flux layout of StageArgsT::bks.
+struct BfluxArgs {
+  const void *k[MAXK];
+  const double *ks, *in;
+  double *out;
+  double bh[MAXK];
+  double dt;            // host-controlled step; device rounds take Ctl::dt
+  int nk, V, NI, NJ, P, inum, jnum, dtype;
+  long FS;
+};
+
 // One right-hand-side evaluation (timedisc_base.f90:1073-1271) on the RK
 // stage state S = Y - sum_q (a[q]*dt) * k[q] (timedisc_rkfehlberg.f90:277-302).
 // With ynew set, the launch is the last stage of the step and also performs
@@ -84,14 +98,16 @@ struct StageArgsT {
   double bh[MAXK], bl[MAXK];
   double tol_rel, tol_abs[4];
   double *part_c;       // per-block partials: [2*b] max error, [2*b+1] = 0
-  // boundary fluxes (timedisc_base.f90:1224-1252, timedisc_rkfehlberg.f90:216-259):
-  // with bflux set, a stage stores the boundary rows of its rhs at the ghost
-  // positions next to the boundary cells of kout; the last stage accumulates
-  // bf_out = bf_in - sum_q dt*b_high(q)*k_q over those rows.  Layout of
-  // bf_in/bf_out: x part [V][2 sides][NJ], then y part [V][2 sides][NI].
+  // boundary fluxes (timedisc_base.f90:1224-1252): with bflux set, a stage
+  // stores the boundary rows of its rhs at the ghost positions next to the
+  // boundary cells of kout; the last stage (which stores no rhs) writes them
+  // to bks instead, in the boundary-flux layout: x part [V][2 sides][NJ],
+  // then y part [V][2 sides][NI]
   int bflux;
-  const double *bf_in;
-  double *bf_out;
+  double *bks;
+  // device-controlled end-of-step evaluation: the step's boundary-flux
+  // accumulation (bfa.out set), by the launch's first threads after their tiles
+  BfluxArgs bfa;
   // FARGO type 1: the azimuthal velocity of the real frame as the next
   // CalcBackgroundVelocity sees it, (v - w) + w (timedisc_base.f90:2243-2326),
   // for interior rows of every ring (ghost rings included); [NJ][P], or null
@@ -148,6 +164,7 @@ void launch_remap(const Geom &g, const RemapArgs &a, int V, hipStream_t s);
 void launch_frame(const Geom &g, const void *src, void *dst, int sign, int V, hipStream_t s);
 void launch_reduce(const double *part, int n, double *out, hipStream_t s);
 void launch_control(Ctl *ctl, const double *part_c, const double *part_r, int nb, hipStream_t s);
+void launch_bflux(const BfluxArgs &bf, hipStream_t s);
 // FARGO type 1: w(i) = SUM_j v(i,j)*wfactor(i) / JNUM per ring into the ring
 // arrays (FP64 and, for the precision form, FP32)
 void launch_wsum(const Geom &g, const double *v, double *ring, float *ringf, hipStream_t s);
@@ -156,6 +173,12 @@ void launch_wsum(const Geom &g, const double *v, double *ring, float *ringf, hip
 #define FV_RHS_ROWS 8
 #endif
 constexpr int TY_TILE = FV_RHS_ROWS;  // k_rhs tile rows
+// FARGO remap: rows per thread; a 64 x 4 block covers 64 rings x 4*REMAP_ROWS rows
+#ifndef FV_REMAP_ROWS
+#define FV_REMAP_ROWS 2
+#endif
+constexpr int REMAP_ROWS = FV_REMAP_ROWS;
+__host__ __device__ inline int remap_rows_blocks(int jnum) { return (jnum + 4 * REMAP_ROWS - 1) / (4 * REMAP_ROWS); }
 void launch_rhs_multi(const RhsEntry *tab, const unsigned *tiles, int ntiles, int V, int dtype, int limiter,
                       int tx, int stages, hipStream_t s);
 void launch_remap_multi(const RemapEntry *tab, const unsigned *tiles, int ntiles, int V, int dtype, hipStream_t s);

@@ -433,6 +433,26 @@ static __device__ double blk_max(double v, double *red) {
   for (int k = 0; k < nw; k++) r = fmax2(r, red[k]);
   return r;
 }
+// block-wide max of v and OR of f with one barrier; the results are valid in
+// thread 0 only, and a block may use its scratch (red, redi) once
+static __device__ void blk_max_or0(double &v, int &f, double *red, int *redi) {
+  v = wave_max(v);
+  f = wave_or(f);
+  const int t = threadIdx.x + threadIdx.y * blockDim.x, nw = (blockDim.x * blockDim.y + 63) / 64;
+  if ((t & 63) == 0) {
+    red[t >> 6] = v;
+    redi[t >> 6] = f;
+  }
+  __syncthreads();
+  if (t == 0) {  // from -DBL_MAX in wave order, as blk_max
+    v = -DBL_MAX;
+    f = 0;
+    for (int k = 0; k < nw; k++) {
+      v = fmax2(v, red[k]);
+      f |= redi[k];
+    }
+  }
+}
 static __device__ int blk_or(int v, int *red) {
   v = wave_or(v);
   const int t = threadIdx.x + threadIdx.y * blockDim.x, nw = (blockDim.x * blockDim.y + 63) / 64;
@@ -557,6 +577,43 @@ static __device__ void control_next(Ctl *c, double invdt, int flags) {
   c->dt = dt;
   c->step = c->step + 1;
   if (c->step >= c->nsteps) halt(c, H_BATCH);
+}
+
+// ------------------------------------------------ boundary-flux accounting
+// bxflux(j,1:2,l) and byflux(i,1:2,l) of one step (timedisc_rkfehlberg.f90:
+// 216-259): threads t, t+nt, ... of the caller each take one boundary row
+__device__ __forceinline__ void bflux_accumulate(const BfluxArgs &b, double dt, int t, int nt) {
+  const int nx = b.V * 2 * b.jnum, n = nx + b.V * 2 * b.inum;
+  for (int e = t; e < n; e += nt) {
+    int l, side, gx;
+    long bi;
+    if (e < nx) {  // west / east: ghost column IMIN-1 / IMAX+1 of row j
+      l = e / (2 * b.jnum);
+      side = (e / b.jnum) & 1;
+      const int j = GH + e % b.jnum;
+      bi = (long)(l * 2 + side) * b.NJ + j;
+      gx = j * b.P + (side ? GH + b.inum : GH - 1);
+    } else {       // south / north: ghost row JMIN-1 / JMAX+1 of column i
+      const int f = e - nx;
+      l = f / (2 * b.inum);
+      side = (f / b.inum) & 1;
+      const int i = GH + f % b.inum;
+      bi = 2L * b.V * b.NJ + (long)(l * 2 + side) * b.NI + i;
+      gx = (side ? GH + b.jnum : GH - 1) * b.P + i;
+    }
+    double kq[MAXK];
+    const long kx = (long)l * b.FS + gx;
+#pragma unroll
+    for (int q = 0; q < MAXK; q++)  // all loads first
+      if (q < b.nk)
+        kq[q] = b.dtype == 1 ? (double)static_cast<const float *>(b.k[q])[kx] : static_cast<const double *>(b.k[q])[kx];
+    double acc = b.in[bi];
+    const double ks = b.ks[bi];
+#pragma unroll
+    for (int q = 0; q < MAXK; q++)
+      if (q < b.nk) acc = acc - (dt * b.bh[q]) * kq[q];
+    b.out[bi] = acc - (dt * b.bh[b.nk]) * ks;
+  }
 }
 
 // ----------------------------------------------------------- tile orders
@@ -894,9 +951,8 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
     // boundary rows: the fluxes through the mesh boundary at the boundary
     // cells' outer interfaces, Mesh%dy*Mesh%dz*xflux (W), -Mesh%dy*Mesh%dz*xflux
     // (E) and likewise in y (timedisc_base.f90:1224-1252), stored at the ghost
-    // position next to the cell; the last stage accumulates them into the
-    // boundary fluxes (timedisc_rkfehlberg.f90:216-259).  bxs/bys: the x and
-    // y side of a boundary cell (0 W/S, 1 E/N, -1 none).
+    // position next to the cell (the last stage: into bks).  bxs/bys: the x
+    // and y side of a boundary cell (0 W/S, 1 E/N, -1 none).
     const int bxs = !a.bflux ? -1 : i == imin ? 0 : i == imax ? 1 : -1;
     const int bys = !a.bflux ? -1 : j == jmin ? 0 : j == jmax ? 1 : -1;
     const unsigned gxx = (unsigned)j * (unsigned)g.P + (unsigned)(bxs == 0 ? imin - 1 : imax + 1);
@@ -916,28 +972,6 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
       }
     }
     if (a.ynew) {
-      // boundary-flux inputs of the accumulation, issued with the loads of
-      // the final combination: Fluxes%bxfold and the boundary rows of k_1..k_{s-1}
-      double bxi[V], byi[V];
-      T kbxq[KM][V], kbyq[KM][V];
-      if (bxs >= 0) {
-#pragma unroll
-        for (int l = 0; l < V; l++) {
-          bxi[l] = a.bf_in[bix + 2L * l * g.NJ];
-#pragma unroll
-          for (int q = 0; q < KM; q++)
-            if (q < in.nk) kbxq[q][l] = (a.k[q] + l * g.FS)[gxx];
-        }
-      }
-      if (bys >= 0) {
-#pragma unroll
-        for (int l = 0; l < V; l++) {
-          byi[l] = a.bf_in[biy + 2L * l * g.NI];
-#pragma unroll
-          for (int q = 0; q < KM; q++)
-            if (q < in.nk) kbyq[q][l] = (a.k[q] + l * g.FS)[gxy];
-        }
-      }
       // final combination with this stage's rhs as k_s, and the error of the
       // embedded solution (FP64: same operation order as the reference;
       // FP32: compensated sums, error norm in FP64)
@@ -996,23 +1030,11 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
         }
         err = fmax2(err, fabs(dif) / (a.tol_rel * fabs(yh) + a.tol_abs[l]));
       }
-      // bxflux = bxfold - sum_q dt*b_high(q)*k_q at the boundary rows
+      // this stage's boundary rows for the boundary-flux accumulation
 #pragma unroll
       for (int l = 0; l < V; l++) {
-        if (bxs >= 0) {
-          double acc = bxi[l];
-#pragma unroll
-          for (int q = 0; q < KM; q++)
-            if (q <= in.nk) acc = acc - (in.dt * a.bh[q]) * (double)(q < in.nk ? kbxq[q][l] : kbx[l]);
-          a.bf_out[bix + 2L * l * g.NJ] = acc;
-        }
-        if (bys >= 0) {
-          double acc = byi[l];
-#pragma unroll
-          for (int q = 0; q < KM; q++)
-            if (q <= in.nk) acc = acc - (in.dt * a.bh[q]) * (double)(q < in.nk ? kbyq[q][l] : kby[l]);
-          a.bf_out[biy + 2L * l * g.NI] = acc;
-        }
+        if (bxs >= 0) a.bks[bix + 2L * l * g.NJ] = (double)kbx[l];
+        if (bys >= 0) a.bks[biy + 2L * l * g.NI] = (double)kby[l];
       }
     } else {
 #pragma unroll
@@ -1020,7 +1042,9 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
     }
   }
   if (a.ynew) {
-    const double m = blk_max(err, redd);
+    double m = err;
+    int unused = 0;
+    blk_max_or0(m, unused, redd, redi);
     if (t == 0) {
       const int b = bx + by * nbx;
       a.part_c[2 * b] = m;
@@ -1056,12 +1080,20 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
     }
   }
   // CFL partials in FP64 (the maximum of FP32 values is exact)
-  const double m = blk_max(invdt == -Real<T>::huge ? -DBL_MAX : (double)invdt, redd);
-  const int f = blk_or(flags, redi);
+  double m = invdt == -Real<T>::huge ? -DBL_MAX : (double)invdt;
+  int f = flags;
+  if (a.ynew) __syncthreads();  // the reduction scratch is in use above
+  blk_max_or0(m, f, redd, redi);
   if (t == 0) {
     const int b = bx + by * nbx;
     a.part[2 * b] = m;
     a.part[2 * b + 1] = (double)f;
+  }
+  // the step's boundary-flux accumulation (device-controlled steps), after
+  // this tile's work so that it adds no register pressure to it
+  if (a.bfa.out) {
+    const int nt = TX * TY;
+    bflux_accumulate(a.bfa, in.dt, (bx + by * nbx) * nt + t, nbx * nby * nt);
   }
 }
 
@@ -1114,7 +1146,11 @@ __device__ __forceinline__ void ycell(const Geom &g, const T *Y, int i, int jj, 
   }
 }
 
-// one 64 x 4 block (bx, by) of the remap
+// one block (bx, by) of the remap: 64 rings x REMAP_ROWS*4 rows.  A thread
+// takes REMAP_ROWS consecutive rows of its ring; while their source rows do
+// not wrap around the ring, they share one window of REMAP_ROWS+4 source
+// cells (each cell's arithmetic is unchanged), otherwise every row loads its
+// own five.
 template <typename T, int V>
 __device__ __forceinline__ void remap_block(const Geom &g, const RemapArgsT<T> &a, int bx, int by) {
   if (a.ctl && a.ctl->halt) return;
@@ -1123,47 +1159,66 @@ __device__ __forceinline__ void remap_block(const Geom &g, const RemapArgsT<T> &
     const double t = a.ctl->time, dt = a.ctl->dt;
     dts = (t + dt) - t;
   }
-  const int i = GH + bx * 64 + threadIdx.x, j = GH + by * 4 + threadIdx.y;
-  if (i >= GH + g.inum || j >= GH + g.jnum) return;
+  constexpr int RR = REMAP_ROWS;
+  const int i = GH + bx * 64 + threadIdx.x, jb = (by * 4 + threadIdx.y) * RR;
+  if (i >= GH + g.inum || jb >= g.jnum) return;
   const int jmin = GH, J = g.jnum;
   double dd = ring(g, R_W, i) * dts / ring(g, R_DLY, i);
   const double shift = round(dd);
   const T d = (T)(dd - shift);
-  // source row: jmin + ((j - jmin - shift) mod J); fmod is exact
-  int jj = (j - jmin) - (int)fmod(shift, (double)J);
-  if (jj < 0) jj += J;
-  if (jj >= J) jj -= J;
-  const int js = jmin + jj;
-  T v[5][V];
-#pragma unroll
-  for (int q = 0; q < 5; q++) ycell<T, V>(g, a.Y, i, js - 2 + q, v[q]);
-  const long ix = cidx(g, i, j);
+  // source row of the first row: jmin + ((jb - shift) mod J); fmod is exact
+  int j0 = jb - (int)fmod(shift, (double)J);
+  if (j0 < 0) j0 += J;
+  if (j0 >= J) j0 -= J;
+  const int nr = J - jb < RR ? J - jb : RR;
   const T one = T(1), zero = T(0), half = T(0.5);
+  // the limited transport of one row from its five source cells
+  auto row = [&](const T (*v)[V], int j) {
+    const long ix = cidx(g, i, j);
 #pragma unroll
-  for (int l = 0; l < V; l++) {
-    T dq[4], dql[3];
+    for (int l = 0; l < V; l++) {
+      T dq[4], dql[3];
 #pragma unroll
-    for (int q = 0; q < 4; q++) dq[q] = v[q + 1][l] - v[q][l];
+      for (int q = 0; q < 4; q++) dq[q] = v[q + 1][l] - v[q][l];
 #pragma unroll
-    for (int q = 0; q < 3; q++)
-      dql[q] = (fsign(one, dq[q]) * fsign(one, dq[q + 1]) > zero) ? fsign(fmin2(fabs(dq[q]), fabs(dq[q + 1])), dq[q])
-                                                                 : zero;
-    T f0, f1;  // flux(js-1), flux(js)
-    if (d > zero) {
-      f0 = v[1][l] + half * dql[0] * (one - d);
-      f1 = v[2][l] + half * dql[1] * (one - d);
-    } else {
-      f0 = v[2][l] - half * dql[1] * (one + d);
-      f1 = v[3][l] - half * dql[2] * (one + d);
+      for (int q = 0; q < 3; q++)
+        dql[q] = (fsign(one, dq[q]) * fsign(one, dq[q + 1]) > zero) ? fsign(fmin2(fabs(dq[q]), fabs(dq[q + 1])), dq[q])
+                                                                   : zero;
+      T f0, f1;  // flux(js-1), flux(js)
+      if (d > zero) {
+        f0 = v[1][l] + half * dql[0] * (one - d);
+        f1 = v[2][l] + half * dql[1] * (one - d);
+      } else {
+        f0 = v[2][l] - half * dql[1] * (one + d);
+        f1 = v[3][l] - half * dql[2] * (one + d);
+      }
+      a.out[l * g.FS + ix] = v[2][l] - d * (f1 - f0);
     }
-    a.out[l * g.FS + ix] = v[2][l] - d * (f1 - f0);
+  };
+  if (j0 + nr <= J) {  // no wrap: one window
+    T v[RR + 4][V];
+#pragma unroll
+    for (int q = 0; q < RR + 4; q++)
+      if (q < nr + 4) ycell<T, V>(g, a.Y, i, jmin + j0 - 2 + q, v[q]);
+#pragma unroll
+    for (int r = 0; r < RR; r++)
+      if (r < nr) row(v + r, jmin + jb + r);
+  } else {
+    for (int r = 0; r < nr; r++) {
+      int jj = j0 + r;
+      if (jj >= J) jj -= J;
+      T v[5][V];
+#pragma unroll
+      for (int q = 0; q < 5; q++) ycell<T, V>(g, a.Y, i, jmin + jj - 2 + q, v[q]);
+      row(v, jmin + jb + r);
+    }
   }
 }
 
 template <typename T, int V>
 __global__ __launch_bounds__(256) void k_remap(Geom g, RemapArgsT<T> a) {
   int bx, by;
-  if (!xcd_tile(blockIdx.x, (g.inum + 63) / 64, (g.jnum + 3) / 4, bx, by)) return;
+  if (!xcd_tile(blockIdx.x, (g.inum + 63) / 64, remap_rows_blocks(g.jnum), bx, by)) return;
   remap_block<T, V>(g, a, bx, by);
 }
 template <typename T, int V>

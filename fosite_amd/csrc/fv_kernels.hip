@@ -11,8 +11,8 @@ namespace fv {
 // the step from the partials of the last stage, accept test, then CFL and
 // data checks from the end-of-step k_rhs partials and the next time step.
 __device__ __forceinline__ void control_block(Ctl *ctl, const double *part_c, const double *part_r, int nb) {
-  __shared__ double redd[16];
-  __shared__ int redi[16];
+  __shared__ double rede[4], redm[4];
+  __shared__ int redi[4];
   if (ctl->halt) return;
   double e = -DBL_MAX, m = -DBL_MAX;
   int f = 0;
@@ -21,16 +21,35 @@ __device__ __forceinline__ void control_block(Ctl *ctl, const double *part_c, co
     m = fmax2(m, part_r[2 * b]);
     f |= (int)part_r[2 * b + 1];
   }
-  e = blk_max(e, redd);
-  m = blk_max(m, redd);
-  f = blk_or(f, redi);
-  if (threadIdx.x == 0) {
+  // one barrier: wave results to LDS, thread 0 combines
+  e = wave_max(e);
+  m = wave_max(m);
+  f = wave_or(f);
+  const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    rede[w] = e;
+    redm[w] = m;
+    redi[w] = f;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {  // from -DBL_MAX in wave order, as blk_max
+    e = m = -DBL_MAX;
+    f = 0;
+    for (int k = 0; k < nw; k++) {
+      e = fmax2(e, rede[k]);
+      m = fmax2(m, redm[k]);
+      f |= redi[k];
+    }
     control_accept(ctl, e);
     if (!ctl->halt) control_next(ctl, m, f);
   }
 }
-__global__ __launch_bounds__(1024) void k_control(Ctl *ctl, const double *part_c, const double *part_r, int nb) {
+__global__ __launch_bounds__(256) void k_control(Ctl *ctl, const double *part_c, const double *part_r, int nb) {
   control_block(ctl, part_c, part_r, nb);
+}
+// host-controlled steps: the boundary-flux accumulation of a step
+__global__ __launch_bounds__(256) void k_bflux(BfluxArgs bf) {
+  bflux_accumulate(bf, bf.dt, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 }
 // one block per mesh of a batch
 __global__ __launch_bounds__(256) void k_control_multi(const ControlEntry *__restrict__ tab) {
@@ -128,7 +147,7 @@ void launch_frame(const Geom &g, const void *src, void *dst, int sign, int V, hi
     frame_single<double>(g, src, dst, sign, V, s);
 }
 void launch_control(Ctl *ctl, const double *part_c, const double *part_r, int nb, hipStream_t s) {
-  hipLaunchKernelGGL(k_control, dim3(1), dim3(1024), 0, s, ctl, part_c, part_r, nb);
+  hipLaunchKernelGGL(k_control, dim3(1), dim3(256), 0, s, ctl, part_c, part_r, nb);
 }
 void launch_rhs_multi(const RhsEntry *tab, const unsigned *tiles, int ntiles, int V, int dtype, int limiter,
                       int tx, int stages, hipStream_t s) {
@@ -152,6 +171,10 @@ void launch_control_multi(const ControlEntry *tab, int n, hipStream_t s) {
 }
 void launch_wsum(const Geom &g, const double *v, double *ring, float *ringf, hipStream_t s) {
   hipLaunchKernelGGL(k_wsum, dim3((g.NI + WS_RINGS - 1) / WS_RINGS), dim3(256), 0, s, g, v, ring, ringf);
+}
+void launch_bflux(const BfluxArgs &bf, hipStream_t s) {
+  const int n = bf.V * 2 * (bf.inum + bf.jnum);
+  hipLaunchKernelGGL(k_bflux, dim3((n + 255) / 256), dim3(256), 0, s, bf);
 }
 void launch_reduce(const double *part, int n, double *out, hipStream_t s) {
   hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, s, part, n, out);

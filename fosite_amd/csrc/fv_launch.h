@@ -32,7 +32,7 @@ void rhs_single(const Geom &g, const StageArgsT<T> &a, int V, hipStream_t s) {
 }
 template <typename T>
 void remap_single(const Geom &g, const RemapArgsT<T> &a, int V, hipStream_t s) {
-  const dim3 grid(xcd_grid((g.inum + 63) / 64, (g.jnum + 3) / 4)), block(64, 4);
+  const dim3 grid(xcd_grid((g.inum + 63) / 64, remap_rows_blocks(g.jnum))), block(64, 4);
   if (V == 3)
     hipLaunchKernelGGL((k_remap<T, 3>), grid, block, 0, s, g, a);
   else
