@@ -1277,13 +1277,13 @@ int rebuild(fv_batch *b) {
 void enqueue_batch_step(fv_batch *b, int p, int c) {
   const int m = b->m, cap = b->cap, n = (int)b->sims.size();
   const unsigned *tl = b->d_tiles + b->cand_off[c];
-  for (int q = 0; q < m - 1; q++)
+  for (int q = 0; q < m - 1; q++)  // stages 2..s, the last fused with the combination
     launch_rhs_multi(b->d_rhs + (size_t)(p * m + q) * cap, tl, b->cand_n[c], b->V, b->dtype, b->limiter,
-                     cand_tx(c), b->m, b->st);
+                     cand_tx(c), b->m, q == m - 2 ? 1 : 0, b->st);
   if (b->remap_n)
     launch_remap_multi(b->d_remap + (size_t)p * cap, b->d_tiles + b->remap_off, b->remap_n, b->V, b->dtype, b->st);
   launch_rhs_multi(b->d_rhs + (size_t)(p * m + m - 1) * cap, tl, b->cand_n[c], b->V, b->dtype, b->limiter,
-                   cand_tx(c), b->m, b->st);
+                   cand_tx(c), b->m, 2, b->st);
   launch_control_multi(b->d_cent + (size_t)c * cap, n, b->st);
 }
 

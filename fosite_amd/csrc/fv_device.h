@@ -179,8 +179,9 @@ constexpr int TY_TILE = FV_RHS_ROWS;  // k_rhs tile rows
 #endif
 constexpr int REMAP_ROWS = FV_REMAP_ROWS;
 __host__ __device__ inline int remap_rows_blocks(int jnum) { return (jnum + 4 * REMAP_ROWS - 1) / (4 * REMAP_ROWS); }
+// mode: 0 RK stage rhs, 1 last stage with the final combination, 2 end-of-step evaluation
 void launch_rhs_multi(const RhsEntry *tab, const unsigned *tiles, int ntiles, int V, int dtype, int limiter,
-                      int tx, int stages, hipStream_t s);
+                      int tx, int stages, int mode, hipStream_t s);
 void launch_remap_multi(const RemapEntry *tab, const unsigned *tiles, int ntiles, int V, int dtype, hipStream_t s);
 void launch_control_multi(const ControlEntry *tab, int n, hipStream_t s);
 

@@ -2,6 +2,6 @@
 #include "fv_launch.h"
 
 namespace fv {
-template void rhs_multi<float, KM_SMALL>(const RhsEntry *, const unsigned *, int, int, int, int, hipStream_t);
+template void rhs_multi<float, KM_SMALL>(const RhsEntry *, const unsigned *, int, int, int, int, int, hipStream_t);
 template void remap_multi<float>(const RemapEntry *, const unsigned *, int, int, hipStream_t);
 }  // namespace fv

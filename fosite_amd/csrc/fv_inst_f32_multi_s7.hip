@@ -2,5 +2,5 @@
 #include "fv_launch.h"
 
 namespace fv {
-template void rhs_multi<float, MAXK>(const RhsEntry *, const unsigned *, int, int, int, int, hipStream_t);
+template void rhs_multi<float, MAXK>(const RhsEntry *, const unsigned *, int, int, int, int, int, hipStream_t);
 }  // namespace fv

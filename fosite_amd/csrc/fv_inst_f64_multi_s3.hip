@@ -2,6 +2,6 @@
 #include "fv_launch.h"
 
 namespace fv {
-template void rhs_multi<double, KM_SMALL>(const RhsEntry *, const unsigned *, int, int, int, int, hipStream_t);
+template void rhs_multi<double, KM_SMALL>(const RhsEntry *, const unsigned *, int, int, int, int, int, hipStream_t);
 template void remap_multi<double>(const RemapEntry *, const unsigned *, int, int, hipStream_t);
 }  // namespace fv

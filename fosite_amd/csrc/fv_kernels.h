@@ -207,11 +207,13 @@ struct In {
   double dt;
   T gam;
   int nk;
-  // stage coefficients a(m,q)*dt, formed once
-  double c[sizeof(T) == 8 ? KM : 1];
-  float ch[sizeof(T) == 4 ? KM : 1];
+  bool chk;  // CheckData on the ghost cells
+  // stage coefficients a(m,q)*dt, formed once (KM = 0: no stage sum)
+  static constexpr int KA = KM > 0 ? KM : 1;
+  double c[sizeof(T) == 8 ? KA : 1];
+  float ch[sizeof(T) == 4 ? KA : 1];
 
-  __forceinline__ __device__ In(const Geom &g_, const StageArgsT<T> &a_) : g(g_), a(a_) {
+  __forceinline__ __device__ In(const Geom &g_, const StageArgsT<T> &a_, bool chk_) : g(g_), a(a_), chk(chk_) {
     dt = a.ctl ? a.ctl->dt : a.dt;  // device-controlled: this step's dt
     nk = a.nk;
     gam = (T)g.gamma;
@@ -228,6 +230,11 @@ struct In {
   // s = Y - sum_q (a_q*dt) k_q for one cell from loaded values, in the
   // reference's order (FP64) or with FP32 fused multiply-adds.
   __forceinline__ __device__ void stage_sum(const T *y, const T (*kv)[V], T *s) const {
+    if constexpr (KM == 0) {
+#pragma unroll
+      for (int l = 0; l < V; l++) s[l] = y[l];
+      return;
+    }
     if constexpr (sizeof(T) == 8) {
 #pragma unroll
       for (int l = 0; l < V; l++) s[l] = y[l];
@@ -254,7 +261,7 @@ struct In {
   __forceinline__ __device__ void interior(int i, int j, T *s) const {
     const long ix = cidx(g, i, j);
     // all loads first (uniform branches only), then the stage sum
-    T y[V], kv[KM][V];
+    T y[V], kv[KA][V];
 #pragma unroll
     for (int l = 0; l < V; l++) y[l] = a.Y[l * g.FS + ix];
 #pragma unroll
@@ -271,7 +278,7 @@ struct In {
   __forceinline__ __device__ void interior2(bool p0, int ia, int ja, T *s0, bool p1, int ib, int jb, T *s1) const {
     const unsigned x0 = (unsigned)ja * (unsigned)g.P + (unsigned)ia;
     const unsigned x1 = (unsigned)jb * (unsigned)g.P + (unsigned)ib;
-    T y0[V], y1[V], k0[KM][V], k1[KM][V];
+    T y0[V], y1[V], k0[KA][V], k1[KA][V];
     if (p0) {
 #pragma unroll
       for (int l = 0; l < V; l++) y0[l] = (a.Y + l * g.FS)[x0];
@@ -391,7 +398,7 @@ struct In {
     ghost_prim(i, j, p);
     if (V == 4 && a.pmin_clamp) p[3] = fmax2(p[3], (T)g.pmin);
     p2c<T, V>(p, u, gam);
-    if (a.want_check) {
+    if (chk) {
 #pragma unroll
       for (int l = 0; l < V; l++)
         if (bad(u[l]) || bad(p[l])) flags |= F_INVALID;
@@ -651,11 +658,25 @@ inline int xcd_grid(int nbx, int nby) { return ((nbx * nby + XCDS - 1) / XCDS) *
 #define FV_RHS_WAVES_PER_SIMD 4
 #endif
 constexpr int TY_RHS = TY_TILE;
+// launch modes of the RHS kernel
+enum RhsMode { M_STAGE = 0, M_LAST = 1, M_END = 2 };
+template <typename T>
+inline int rhs_mode(const StageArgsT<T> &a) { return a.ynew ? M_LAST : (a.nk == 0 ? M_END : M_STAGE); }
 
 // one TX x TY tile (bx, by) of a mesh tiled nbx x nby
-template <typename T, int V, int LIM, int TX, int KM>
+template <typename T, int V, int LIM, int TX, int KM, int MODE>
 __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, int bx, int by, int nbx,
                                          int nby) {
+  // what this launch does, fixed at compile time (the run-time flags of the
+  // arguments still apply within it): RK stage rhs, last stage with the
+  // final combination, or the end-of-step evaluation (no stage sum; CFL,
+  // data checks, new state, FARGO type-1 velocities, boundary fluxes)
+  constexpr bool kEnd = MODE == M_END, kLast = MODE == M_LAST;
+  constexpr int KE = kEnd ? 0 : KM;
+  const bool want_cfl = kEnd && a.want_cfl, want_check = kEnd && a.want_check;
+  T *const yout = kEnd ? a.yout : nullptr;
+  double *const vout = kEnd ? a.vout : nullptr;
+  T *const ynew = kLast ? a.ynew : nullptr;
   constexpr int TY = TY_RHS, SX = TX + 4, SY = TY + 4;
   constexpr int N_UC = V * SY * SX;
   constexpr int XW = V * TY * (TX + 1), XE = V * TY, YS = V * (TY + 1) * TX, YN = V * TX;
@@ -685,7 +706,7 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
   auto FLY = [&](int l, int r, int c) -> T & { return lds[FX + (l * (TY + 1) + r) * TX + c]; };
 
   if (a.ctl && a.ctl->halt) return;
-  const In<T, V, KM> in(g, a);
+  const In<T, V, KE> in(g, a, want_check);
   const int tx = threadIdx.x, ty = threadIdx.y, t = tx + TX * ty;
   const int i0 = GH + bx * TX, j0 = GH + by * TY;
   const int imin = GH, jmin = GH, imax = GH + g.inum - 1, jmax = GH + g.jnum - 1;
@@ -729,18 +750,18 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
       if ((xin || yin) && ii >= 0 && jj >= 0 && ii < g.NI && jj < g.NJ) {
         T pp[V];
         in.ghost_final(ii, jj, u, pp, gf);
-        if (a.want_cfl) {
+        if (want_cfl) {
           const T cs = (V == 4) ? sound_speed(in.gam, pp[0], pp[3]) : rg<T>(g, R_CS, ii);
           ginv = fmax2(ginv, cell_invdt(g, ii, pp[1], pp[2], cs));
         }
-        if (a.yout) {
+        if (yout) {
           const unsigned ix = (unsigned)jj * (unsigned)g.P + (unsigned)ii;
 #pragma unroll
-          for (int l = 0; l < V; l++) (a.yout + l * g.FS)[ix] = u[l];
+          for (int l = 0; l < V; l++) (yout + l * g.FS)[ix] = u[l];
         }
-        if (a.vout && yin) {  // x-ghost ring of an interior row: (v - w) + w
+        if (vout && yin) {  // x-ghost ring of an interior row: (v - w) + w
           const T w = rg<T>(g, R_W, ii), vc = a.sub_bg ? pp[2] : pp[2] - w;
-          a.vout[(unsigned)jj * (unsigned)g.P + (unsigned)ii] = (double)(vc + w);
+          vout[(unsigned)jj * (unsigned)g.P + (unsigned)ii] = (double)(vc + w);
         }
       } else {
 #pragma unroll
@@ -761,9 +782,9 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
     if (out) {
       in.transform(i, so, ur, pr, uo);
       if constexpr (sizeof(T) == 4) vres = dv(so[2], so[0]);
-      if (a.vout) {  // pvar after SubtractBackgroundVelocityY, then + w again
+      if (vout) {  // pvar after SubtractBackgroundVelocityY, then + w again
         const T w = rg<T>(g, R_W, i);
-        a.vout[(unsigned)j * (unsigned)g.P + (unsigned)i] = (double)((pr[2] - w) + w);
+        vout[(unsigned)j * (unsigned)g.P + (unsigned)i] = (double)((pr[2] - w) + w);
       }
 #pragma unroll
       for (int l = 0; l < V; l++) UC(l, ty, tx) = uo[l];
@@ -824,14 +845,14 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
       if (V == 4) ext[3] = zero + (ur[1] * ax + ur[2] * ay);
     }
     // CheckData (timedisc_base.f90:1666-1742) and CFL (timedisc_base.f90:818-909)
-    if (a.want_check) {
+    if (want_check) {
       if (pr[0] < (T)g.rhomin) flags |= F_RHOMIN;
       if (V == 4 && pr[3] < (T)g.pmin) flags |= F_PMIN;
 #pragma unroll
       for (int l = 0; l < V; l++)
         if (bad(pr[l]) || bad(ur[l])) flags |= F_INVALID;
     }
-    if (a.want_cfl) {
+    if (want_cfl) {
       const T vy = a.sub_bg ? pr[2] - rg<T>(g, R_W, i) : pr[2];
       const T cs = (V == 4) ? sound_speed(in.gam, pr[0], pr[3]) : rg<T>(g, R_CS, i);
       invdt = cell_invdt(g, i, pr[1], vy, cs);
@@ -946,7 +967,7 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
     for (int l = 0; l < V; l++) {
       const T xw = FLX(l, ty, tx), ys = FLY(l, ty, tx);
       rhs[l] = dydzdV * (xf[l] - xw) + dzdxdV * (yf[l] - ys) + dxdydV * (zero - zero) - geo[l] - ext[l];
-      if (a.yout) (a.yout + l * g.FS)[ix] = uc[l];
+      if (yout) (yout + l * g.FS)[ix] = uc[l];
     }
     // boundary rows: the fluxes through the mesh boundary at the boundary
     // cells' outer interfaces, Mesh%dy*Mesh%dz*xflux (W), -Mesh%dy*Mesh%dz*xflux
@@ -965,13 +986,13 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
       for (int l = 0; l < V; l++) {
         kbx[l] = bxs == 0 ? ax * FLX(l, ty, tx) : -(ax * xf[l]);
         kby[l] = bys == 0 ? ay * FLY(l, ty, tx) : -(ay * yf[l]);
-        if (!a.ynew) {
+        if (!ynew) {
           if (bxs >= 0) (a.kout + l * g.FS)[gxx] = kbx[l];
           if (bys >= 0) (a.kout + l * g.FS)[gxy] = kby[l];
         }
       }
     }
-    if (a.ynew) {
+    if (ynew) {
       // final combination with this stage's rhs as k_s, and the error of the
       // embedded solution (FP64: same operation order as the reference;
       // FP32: compensated sums, error norm in FP64)
@@ -1009,7 +1030,7 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
               w = w - (T)(a.bl[q] * in.dt) * kk;
               h = h - (T)(a.bh[q] * in.dt) * kk;
             }
-          (a.ynew + l * g.FS)[ix] = h;
+          (ynew + l * g.FS)[ix] = h;
           yh = (double)h;
           dif = yh - (double)w;
         } else {
@@ -1024,7 +1045,7 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
               h.sub(bhh[q], bhl[q], kk);
               e = fmaf(bde[q], kk, e);
             }
-          (a.ynew + l * g.FS)[ix] = (T)h.value();
+          (ynew + l * g.FS)[ix] = (T)h.value();
           yh = h.wide();
           dif = (double)e;
         }
@@ -1041,7 +1062,7 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
       for (int l = 0; l < V; l++) (a.kout + l * g.FS)[ix] = rhs[l];
     }
   }
-  if (a.ynew) {
+  if (ynew) {
     double m = err;
     int unused = 0;
     blk_max_or0(m, unused, redd, redi);
@@ -1052,7 +1073,7 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
     }
   }
 
-  if (!(a.want_cfl || a.want_check)) return;
+  if (!(want_cfl || want_check)) return;
   if (own_g || e_g) {  // ghost cells evaluated in phase 1
     invdt = fmax2(invdt, gcfl[t]);
     flags |= gflg[t];
@@ -1068,21 +1089,21 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
       const int gi = ce ? imax + ga : imin - ga, gj = cn ? jmax + gb : jmin - gb;
       T u[V], p[V];
       in.ghost_final(gi, gj, u, p, flags);
-      if (a.want_cfl) {
+      if (want_cfl) {
         const T cs = (V == 4) ? sound_speed(in.gam, p[0], p[3]) : rg<T>(g, R_CS, gi);
         invdt = fmax2(invdt, cell_invdt(g, gi, p[1], p[2], cs));
       }
-      if (a.yout) {
+      if (yout) {
         const unsigned ix = (unsigned)gj * (unsigned)g.P + (unsigned)gi;
 #pragma unroll
-        for (int l = 0; l < V; l++) (a.yout + l * g.FS)[ix] = u[l];
+        for (int l = 0; l < V; l++) (yout + l * g.FS)[ix] = u[l];
       }
     }
   }
   // CFL partials in FP64 (the maximum of FP32 values is exact)
   double m = invdt == -Real<T>::huge ? -DBL_MAX : (double)invdt;
   int f = flags;
-  if (a.ynew) __syncthreads();  // the reduction scratch is in use above
+  if (ynew) __syncthreads();  // the reduction scratch is in use above
   blk_max_or0(m, f, redd, redi);
   if (t == 0) {
     const int b = bx + by * nbx;
@@ -1091,23 +1112,23 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
   }
   // the step's boundary-flux accumulation (device-controlled steps), after
   // this tile's work so that it adds no register pressure to it
-  if (a.bfa.out) {
+  if (kEnd && a.bfa.out) {
     const int nt = TX * TY;
     bflux_accumulate(a.bfa, in.dt, (bx + by * nbx) * nt + t, nbx * nby * nt);
   }
 }
 
-template <typename T, int V, int LIM, int TX, int KM>
+template <typename T, int V, int LIM, int TX, int KM, int MODE>
 __global__ __launch_bounds__(TX *TY_RHS, FV_RHS_WAVES_PER_SIMD) void k_rhs(Geom g, StageArgsT<T> a) {
   const int nbx = (g.inum + TX - 1) / TX, nby = (g.jnum + TY_RHS - 1) / TY_RHS;
   int bx, by;
   if (!xcd_tile(blockIdx.x, nbx, nby, bx, by)) return;
-  rhs_tile<T, V, LIM, TX, KM>(g, a, bx, by, nbx, nby);
+  rhs_tile<T, V, LIM, TX, KM, MODE>(g, a, bx, by, nbx, nby);
 }
 
 // Many meshes in one launch: block b runs the tile tiles[b] (TileCode) of
 // the mesh whose Geom and arguments of this launch are tab[disk].
-template <typename T, int V, int LIM, int TX, int KM>
+template <typename T, int V, int LIM, int TX, int KM, int MODE>
 __global__ __launch_bounds__(TX *TY_RHS, FV_RHS_WAVES_PER_SIMD) void k_rhs_multi(
     const RhsEntry *__restrict__ tab, const unsigned *__restrict__ tiles) {
   const unsigned e = tiles[blockIdx.x];
@@ -1115,7 +1136,7 @@ __global__ __launch_bounds__(TX *TY_RHS, FV_RHS_WAVES_PER_SIMD) void k_rhs_multi
   const RhsEntry &E = tab[tile_disk(e)];
   const StageArgsT<T> &a = reinterpret_cast<const StageArgsT<T> &>(E.a);
   const int nbx = (E.g.inum + TX - 1) / TX, nby = (E.g.jnum + TY_RHS - 1) / TY_RHS;
-  rhs_tile<T, V, LIM, TX, KM>(E.g, a, tile_bx(e), tile_by(e), nbx, nby);
+  rhs_tile<T, V, LIM, TX, KM, MODE>(E.g, a, tile_bx(e), tile_by(e), nbx, nby);
 }
 
 // ------------------------------------------------------------ FARGO remap
@@ -1255,7 +1276,7 @@ void remap_single(const Geom &g, const RemapArgsT<T> &a, int V, hipStream_t s);
 template <typename T>
 void frame_single(const Geom &g, const void *src, void *dst, int sign, int V, hipStream_t s);
 template <typename T, int KM>
-void rhs_multi(const RhsEntry *tab, const unsigned *tiles, int n, int V, int limiter, int tx, hipStream_t s);
+void rhs_multi(const RhsEntry *tab, const unsigned *tiles, int n, int V, int limiter, int tx, int mode, hipStream_t s);
 // coefficient slots the kernels are compiled for, by tableau size
 constexpr int KM_SMALL = 3;
 inline int km_for(int stages) { return stages <= KM_SMALL ? KM_SMALL : MAXK; }

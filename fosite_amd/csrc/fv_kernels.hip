@@ -150,14 +150,14 @@ void launch_control(Ctl *ctl, const double *part_c, const double *part_r, int nb
   hipLaunchKernelGGL(k_control, dim3(1), dim3(256), 0, s, ctl, part_c, part_r, nb);
 }
 void launch_rhs_multi(const RhsEntry *tab, const unsigned *tiles, int ntiles, int V, int dtype, int limiter,
-                      int tx, int stages, hipStream_t s) {
+                      int tx, int stages, int mode, hipStream_t s) {
   const bool small = km_for(stages) == KM_SMALL;
   if (dtype == 1) {
-    if (small) rhs_multi<float, KM_SMALL>(tab, tiles, ntiles, V, limiter, tx, s);
-    else rhs_multi<float, MAXK>(tab, tiles, ntiles, V, limiter, tx, s);
+    if (small) rhs_multi<float, KM_SMALL>(tab, tiles, ntiles, V, limiter, tx, mode, s);
+    else rhs_multi<float, MAXK>(tab, tiles, ntiles, V, limiter, tx, mode, s);
   } else {
-    if (small) rhs_multi<double, KM_SMALL>(tab, tiles, ntiles, V, limiter, tx, s);
-    else rhs_multi<double, MAXK>(tab, tiles, ntiles, V, limiter, tx, s);
+    if (small) rhs_multi<double, KM_SMALL>(tab, tiles, ntiles, V, limiter, tx, mode, s);
+    else rhs_multi<double, MAXK>(tab, tiles, ntiles, V, limiter, tx, mode, s);
   }
 }
 void launch_remap_multi(const RemapEntry *tab, const unsigned *tiles, int ntiles, int V, int dtype, hipStream_t s) {

@@ -8,7 +8,11 @@ namespace fv {
 template <typename T, int V, int LIM, int TX, int KM>
 static void rhs_lim(const Geom &g, const StageArgsT<T> &a, hipStream_t s) {
   const dim3 grid(xcd_grid((g.inum + TX - 1) / TX, (g.jnum + TY_RHS - 1) / TY_RHS)), block(TX, TY_RHS);
-  hipLaunchKernelGGL((k_rhs<T, V, LIM, TX, KM>), grid, block, 0, s, g, a);
+  switch (rhs_mode(a)) {
+    case M_STAGE: hipLaunchKernelGGL((k_rhs<T, V, LIM, TX, KM, M_STAGE>), grid, block, 0, s, g, a); break;
+    case M_LAST: hipLaunchKernelGGL((k_rhs<T, V, LIM, TX, KM, M_LAST>), grid, block, 0, s, g, a); break;
+    default: hipLaunchKernelGGL((k_rhs<T, V, LIM, TX, KM, M_END>), grid, block, 0, s, g, a); break;
+  }
 }
 template <typename T, int V, int TX, int KM>
 static void rhs_v(const Geom &g, const StageArgsT<T> &a, hipStream_t s) {
@@ -51,22 +55,30 @@ void frame_single(const Geom &g, const void *src, void *dst, int sign, int V, hi
 
 // batches: van Leer compiled in, every other limiter (and mixed limiters) by
 // the run-time switch
-template <typename T, int V, int TX, int KM>
-static void rhs_multi_v(const RhsEntry *tab, const unsigned *tiles, int n, int limiter, hipStream_t s) {
+template <typename T, int V, int LIM, int TX, int KM>
+static void rhs_multi_lim(const RhsEntry *tab, const unsigned *tiles, int n, int mode, hipStream_t s) {
   const dim3 block(TX, TY_RHS);
+  switch (mode) {
+    case M_STAGE: hipLaunchKernelGGL((k_rhs_multi<T, V, LIM, TX, KM, M_STAGE>), dim3(n), block, 0, s, tab, tiles); break;
+    case M_LAST: hipLaunchKernelGGL((k_rhs_multi<T, V, LIM, TX, KM, M_LAST>), dim3(n), block, 0, s, tab, tiles); break;
+    default: hipLaunchKernelGGL((k_rhs_multi<T, V, LIM, TX, KM, M_END>), dim3(n), block, 0, s, tab, tiles); break;
+  }
+}
+template <typename T, int V, int TX, int KM>
+static void rhs_multi_v(const RhsEntry *tab, const unsigned *tiles, int n, int limiter, int mode, hipStream_t s) {
   if (limiter == VANLEER)
-    hipLaunchKernelGGL((k_rhs_multi<T, V, VANLEER, TX, KM>), dim3(n), block, 0, s, tab, tiles);
+    rhs_multi_lim<T, V, VANLEER, TX, KM>(tab, tiles, n, mode, s);
   else
-    hipLaunchKernelGGL((k_rhs_multi<T, V, 0, TX, KM>), dim3(n), block, 0, s, tab, tiles);
+    rhs_multi_lim<T, V, 0, TX, KM>(tab, tiles, n, mode, s);
 }
 template <typename T, int KM>
-void rhs_multi(const RhsEntry *tab, const unsigned *tiles, int n, int V, int limiter, int tx, hipStream_t s) {
+void rhs_multi(const RhsEntry *tab, const unsigned *tiles, int n, int V, int limiter, int tx, int mode, hipStream_t s) {
   if (V == 3) {
-    if (tx == 32) rhs_multi_v<T, 3, 32, KM>(tab, tiles, n, limiter, s);
-    else rhs_multi_v<T, 3, 64, KM>(tab, tiles, n, limiter, s);
+    if (tx == 32) rhs_multi_v<T, 3, 32, KM>(tab, tiles, n, limiter, mode, s);
+    else rhs_multi_v<T, 3, 64, KM>(tab, tiles, n, limiter, mode, s);
   } else {
-    if (tx == 32) rhs_multi_v<T, 4, 32, KM>(tab, tiles, n, limiter, s);
-    else rhs_multi_v<T, 4, 64, KM>(tab, tiles, n, limiter, s);
+    if (tx == 32) rhs_multi_v<T, 4, 32, KM>(tab, tiles, n, limiter, mode, s);
+    else rhs_multi_v<T, 4, 64, KM>(tab, tiles, n, limiter, mode, s);
   }
 }
 template <typename T>
