@@ -37,7 +37,8 @@ MODULE fosite_fv_c
   PUBLIC :: fv_mesh_array, fv_create, fv_destroy, fv_dims, fv_set_sound_speeds, &
        fv_set_background_velocity, fv_set_pvar, fv_first_step, fv_step, fv_run, &
        fv_get_state, fv_get_scalars, fv_last_run_ms, fv_last_error, fv_profile, &
-       fv_kernel_stats, c_string
+       fv_kernel_stats, fv_get_background_velocity, fv_get_boundary_flux, &
+       fv_get_boundary_flux_total, c_string
 
   INTERFACE
     INTEGER(c_int) FUNCTION fv_mesh_array(cfg, id, out) BIND(C, NAME="fv_mesh_array")
@@ -100,6 +101,22 @@ MODULE fosite_fv_c
       TYPE(c_ptr), VALUE :: ctx
       TYPE(fv_scalars), INTENT(OUT) :: sc
     END FUNCTION fv_get_scalars
+    INTEGER(c_int) FUNCTION fv_get_background_velocity(ctx, w) BIND(C, NAME="fv_get_background_velocity")
+      IMPORT :: c_int, c_ptr, c_double
+      TYPE(c_ptr), VALUE :: ctx
+      REAL(c_double), INTENT(OUT) :: w(*)
+    END FUNCTION fv_get_background_velocity
+    INTEGER(c_int) FUNCTION fv_get_boundary_flux(ctx, bx, by) BIND(C, NAME="fv_get_boundary_flux")
+      IMPORT :: c_int, c_ptr
+      TYPE(c_ptr), VALUE :: ctx, bx, by
+    END FUNCTION fv_get_boundary_flux
+    INTEGER(c_int) FUNCTION fv_get_boundary_flux_total(ctx, direction, bflux) &
+         BIND(C, NAME="fv_get_boundary_flux_total")
+      IMPORT :: c_int, c_ptr, c_double
+      TYPE(c_ptr), VALUE :: ctx
+      INTEGER(c_int), VALUE :: direction
+      REAL(c_double), INTENT(OUT) :: bflux(*)
+    END FUNCTION fv_get_boundary_flux_total
     REAL(c_double) FUNCTION fv_last_run_ms(ctx) BIND(C, NAME="fv_last_run_ms")
       IMPORT :: c_double, c_ptr
       TYPE(c_ptr), VALUE :: ctx

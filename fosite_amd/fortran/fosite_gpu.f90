@@ -86,6 +86,9 @@ MODULE fosite_gpu_mod
     PROCEDURE :: Step
     PROCEDURE :: Run
     PROCEDURE :: GetState
+    PROCEDURE :: GetBackgroundVelocity
+    PROCEDURE :: GetBoundaryFluxes
+    PROCEDURE :: GetBoundaryFlux
     PROCEDURE :: MeshArray
     PROCEDURE :: LastRunMs
     PROCEDURE :: Finalize
@@ -231,6 +234,37 @@ CONTAINS
     IF (PRESENT(pvar)) pp = C_LOC(pvar)
     CALL Check(this, fv_get_state(this%ctx, pc, pp), "GetState")
   END SUBROUTINE GetState
+
+  !> Timedisc%w, one value per x-ring IGMIN..IGMAX (FARGO type 1: the value
+  !! CalcBackgroundVelocity computed at the start of the last step)
+  SUBROUTINE GetBackgroundVelocity(this, w)
+    CLASS(fosite_gpu), INTENT(INOUT) :: this
+    REAL(DP), INTENT(OUT), CONTIGUOUS :: w(:)
+    CALL Check(this, fv_get_background_velocity(this%ctx, w), "GetBackgroundVelocity")
+  END SUBROUTINE GetBackgroundVelocity
+
+  !> Fluxes%bxflux(JGMIN:JGMAX,1,2,VNUM) and Fluxes%byflux(1,IGMIN:IGMAX,2,VNUM)
+  !! (fluxes_base.f90:142-143)
+  SUBROUTINE GetBoundaryFluxes(this, bxflux, byflux)
+    CLASS(fosite_gpu), INTENT(INOUT) :: this
+    REAL(DP), INTENT(OUT), CONTIGUOUS, TARGET, OPTIONAL :: bxflux(:,:,:,:), byflux(:,:,:,:)
+    TYPE(c_ptr) :: px, py
+    px = c_null_ptr
+    py = c_null_ptr
+    IF (PRESENT(bxflux)) px = C_LOC(bxflux)
+    IF (PRESENT(byflux)) py = C_LOC(byflux)
+    CALL Check(this, fv_get_boundary_flux(this%ctx, px, py), "GetBoundaryFluxes")
+  END SUBROUTINE GetBoundaryFluxes
+
+  !> Fluxes%GetBoundaryFlux (fluxes_base.f90:220-300): the VNUM totals through
+  !! one boundary, direction WEST=1, EAST=2, SOUTH=3, NORTH=4
+  FUNCTION GetBoundaryFlux(this, direction) RESULT(bflux)
+    CLASS(fosite_gpu), INTENT(INOUT) :: this
+    INTEGER, INTENT(IN) :: direction
+    REAL(DP) :: bflux(4)
+    bflux(:) = 0.0_DP
+    CALL Check(this, fv_get_boundary_flux_total(this%ctx, INT(direction, c_int), bflux), "GetBoundaryFlux")
+  END FUNCTION GetBoundaryFlux
 
   !> one metric array of the configured mesh (ids as fv_mesh_array)
   SUBROUTINE MeshArray(this, id, a)

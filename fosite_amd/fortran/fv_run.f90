@@ -38,9 +38,10 @@ PROGRAM fv_run
 
   TYPE(fosite_gpu) :: Sim
   CHARACTER(LEN=512) :: cfgname
-  INTEGER :: u, o, n, k
+  INTEGER :: u, o, ob, n, k
   LOGICAL :: done
   REAL(DP), ALLOCATABLE :: pv(:,:,:), cs(:,:), fcs(:,:,:), w(:), cv(:,:,:), pvo(:,:,:)
+  REAL(DP), ALLOCATABLE :: bx(:,:,:,:), by(:,:,:,:)
 
   CALL GET_COMMAND_ARGUMENT(1, cfgname)
   OPEN(NEWUNIT=u, FILE=TRIM(cfgname), STATUS="old")
@@ -99,6 +100,11 @@ PROGRAM fv_run
 
   OPEN(NEWUNIT=o, FILE=TRIM(k_prefix) // "_steps.bin", ACCESS="stream", FORM="unformatted", &
        STATUS="replace")
+  OPEN(NEWUNIT=ob, FILE=TRIM(k_prefix) // "_bflux.bin", ACCESS="stream", FORM="unformatted", &
+       STATUS="replace")
+  ASSOCIATE(M => Sim%Mesh, V => Sim%Physics%VNUM)
+    ALLOCATE(bx(M%JGMIN:M%JGMAX, 1, 2, V), by(1, M%IGMIN:M%IGMAX, 2, V))
+  END ASSOCIATE
   CALL Sim%FirstStep()
   CALL Dump(0)
   n = 0
@@ -116,6 +122,7 @@ PROGRAM fv_run
     END IF
   END DO
   CLOSE(o)
+  CLOSE(ob)
   WRITE(*,'(A,I0,A,ES24.16)') "FV_RUN_STEPS ", n, " TIME ", Sim%Timedisc%time
   CALL Sim%Finalize()
 
@@ -131,6 +138,11 @@ CONTAINS
     WRITE(o) ibuf, rbuf
     WRITE(o) cv
     WRITE(o) pvo
+    ! boundary fluxes and their totals per boundary (WEST, EAST, SOUTH, NORTH)
+    CALL Sim%GetBoundaryFluxes(bx, by)
+    WRITE(ob) bx, by
+    WRITE(ob) Sim%GetBoundaryFlux(1), Sim%GetBoundaryFlux(2), Sim%GetBoundaryFlux(3), &
+              Sim%GetBoundaryFlux(4)
   END SUBROUTINE Dump
 
 END PROGRAM fv_run

@@ -36,6 +36,7 @@ def run(cfg, nsteps, tmp, batch=0, dump_every=1, timeout=300):
     r = subprocess.run([FV_RUN, path], cwd=tmp, capture_output=True, text=True, timeout=timeout)
     recs = []
     out = os.path.join(tmp, "run_steps.bin")
+    bout = os.path.join(tmp, "run_bflux.bin")
     if r.returncode == 0 and os.path.exists(out):
         NI, NJ = cfg["inum"] + 4, cfg["jnum"] + 4
         V = 4 if cfg["physics"] == setups.EULER else 3
@@ -49,4 +50,13 @@ def run(cfg, nsteps, tmp, batch=0, dump_every=1, timeout=300):
             recs.append(dict(step=int(ib[0]), iter=int(ib[1]), n_adj=int(ib[2]), brk=int(ib[3]),
                              time=rb[0], dt=rb[1], dtold=rb[2], maxerrold=rb[3], cvar=cv))
             off += rec
+        # boundary fluxes of every record (GetBoundaryFluxes, GetBoundaryFlux x4)
+        if os.path.exists(bout):
+            braw = np.fromfile(bout, dtype="<f8")
+            brec = 2 * V * NJ + 2 * V * NI + 4 * 4
+            for n, d in enumerate(recs):
+                b = braw[n * brec:(n + 1) * brec]
+                d["bx"] = b[:2 * V * NJ].reshape(V, 2, NJ)
+                d["by"] = b[2 * V * NJ:2 * V * (NJ + NI)].reshape(V, 2, NI)
+                d["bsum"] = b[2 * V * (NJ + NI):].reshape(4, 4)[:, :V]
     return r, recs

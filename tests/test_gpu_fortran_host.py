@@ -13,15 +13,16 @@ I = (slice(None), slice(2, -2), slice(2, -2))
 
 
 def oracle_states(cfg, steps):
+    """{step: (cvar, pvar, scalars, (bx, by))}"""
     init = S.initial(cfg)
     o = oracle.Oracle(cfg)
     o.set_initial(init["pvar"], init.get("bccsound"), init.get("fcsound"), init.get("w"))
     o.first_step()
-    out = {0: o.state()}
+    out = {0: o.state() + (o.bflux(),)}
     for n in range(1, max(steps) + 1):
         o.step()
         if n in steps:
-            out[n] = o.state()
+            out[n] = o.state() + (o.bflux(),)
     return out
 
 
@@ -38,11 +39,22 @@ def test_fortran_host_matches_oracle(tmp_path, name, cfg, nsteps, batch, tol):
     assert recs and recs[-1]["step"] == nsteps
     ref = oracle_states(cfg, {rec["step"] for rec in recs})
     for rec in recs:
-        rc, _, rs = ref[rec["step"]]
+        rc, _, rs, (rbx, rby) = ref[rec["step"]]
         assert rec["iter"] == rs["iter"] and rec["n_adj"] == rs["n_adj"], (rec["step"], rs)
         if tol == 0.0:
             assert rec["time"] == rs["time"] and rec["dt"] == rs["dt"]
             assert np.array_equal(rec["cvar"][I], rc[I]), rec["step"]
+            assert np.array_equal(rec["bx"], rbx) and np.array_equal(rec["by"], rby), rec["step"]
         else:
             assert abs(rec["time"] - rs["time"]) <= tol * rs["time"]
             assert (rel_l1(rec["cvar"], rc) <= tol).all(), rel_l1(rec["cvar"], rc)
+            for g, r in ((rec["bx"], rbx), (rec["by"], rby)):
+                assert np.abs(g - r).sum() <= tol * max(np.abs(r).sum(), 1e-300), rec["step"]
+        # GetBoundaryFlux through the Fortran object API: index-order sums
+        for d, (arr, m) in enumerate(((rec["bx"][:, 0], cfg["jnum"]), (rec["bx"][:, 1], cfg["jnum"]),
+                                      (rec["by"][:, 0], cfg["inum"]), (rec["by"][:, 1], cfg["inum"]))):
+            for l in range(arr.shape[0]):
+                acc = 0.0
+                for k in range(2, m + 2):
+                    acc = acc + arr[l, k]
+                assert rec["bsum"][d, l] == acc, (rec["step"], d, l)
