@@ -554,14 +554,23 @@ static __device__ void halt(Ctl *c, int reason) {
   c->halt = 1;
   c->reason = reason;
 }
+// A lower bound of pi_dtnew for an accepted step (0 <= err < 1) without
+// exp/log: err^(-1/order) >= 1, and (maxerrold/err)^(-beta) >= min(1,
+// err/maxerrold) for 0 <= beta <= 1; other beta take the exact form
+static __device__ double pi_dtnew_lower(const Ctl *c, double err, double dtold) {
+  if (!(c->beta >= 0.0 && c->beta <= 1.0 && c->order > 0.0)) return pi_dtnew(c, err, dtold);
+  const double f = (c->maxerrold > 0. && err < c->maxerrold) ? err / c->maxerrold : 1.0;
+  return 0.9 * dtold * f;
+}
 // after SolveODE: accept when the embedded error is finite and < 1
-// (timedisc_base.f90:686-713, AcceptSolution 911-959)
+// (timedisc_base.f90:686-713, AcceptSolution 911-959); halt when the next
+// step could fall under dtlimit (the host redoes such a step exactly)
 static __device__ void control_accept(Ctl *c, double err) {
   const int s = c->step;
   c->err_hist[s] = err;
   c->dt_hist[s] = c->dt;
   if (!(err < 1.0)) return halt(c, H_ERR);  // NaN, Inf or a rejection
-  if (pi_dtnew(c, err, c->dt) < c->dtlimit * (1.0 + 1.0e-6)) return halt(c, H_DTLIMIT);
+  if (pi_dtnew_lower(c, err, c->dt) < c->dtlimit * (1.0 + 1.0e-6)) return halt(c, H_DTLIMIT);
   c->maxerrold = err;
   const double t1 = c->time + c->dt;
   c->dt_step = t1 - c->time;

@@ -16,6 +16,7 @@ __device__ __forceinline__ void control_block(Ctl *ctl, const double *part_c, co
   if (ctl->halt) return;
   double e = -DBL_MAX, m = -DBL_MAX;
   int f = 0;
+#pragma unroll 4
   for (int b = threadIdx.x; b < nb; b += blockDim.x) {
     e = fmax2(e, part_c[2 * b]);
     m = fmax2(m, part_r[2 * b]);
