@@ -25,8 +25,8 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the stage-2 RHS kernel on this workload: rocprofv3
 # FETCH_SIZE x2.0 + WRITE_SIZE x1.0 (factors from a 1 GiB copy kernel),
-# 15702 KiB and 13056 KiB per dispatch (profiles/k_rhs_disk512x1024_hbm_traffic.md)
-TRAFFIC_BYTES_PER_LAUNCH = 45.5e6
+# 13547 KiB and 12411 KiB per dispatch (profiles/k_rhs_disk512x1024_hbm_traffic.md)
+TRAFFIC_BYTES_PER_LAUNCH = (2.0 * 13547 + 12411) * 1024
 METRIC = "cell-updates/s & HBM GB/s vs plain FP64 at stated err; p50/p99 run latency"
 
 
@@ -308,7 +308,8 @@ def main():
                    "sim_time_reached": sc["time"], "rejected_steps": sc["n_adj"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_BYTES_PER_LAUNCH,
-                     "kernel": "fv::k_rhs<double,3,VANLEER,64,3> on a stage state (fused stage sum, ghost cells, RHS)",
+                     "kernel": "fv::k_rhs<double,3,VANLEER,64,3,M_STAGE> on a stage state (fused stage sum, "
+                               "ghost cells, RHS)",
                      "launches": n_rhs, "mean_launch_us": 1e3 * ms_rhs / max(n_rhs, 1),
                      "alg_bytes_per_launch": bytes_rhs / max(n_rhs, 1),
                      "timing": "HIP events around each launch on the simulation stream, "
