@@ -1075,7 +1075,8 @@ extern "C" int fv_get_boundary_flux(fv_ctx *ctx, double *bx, double *by) {
 }
 
 extern "C" int fv_get_boundary_flux_total(fv_ctx *ctx, int direction, double *bflux) {
-  if (!ctx || !bflux || direction < 1 || direction > 4) return fail(ctx, "fv_get_boundary_flux_total: direction 1..4");
+  if (!ctx) return -1;
+  if (!bflux || direction < 1 || direction > 4) return fail(ctx, "fv_get_boundary_flux_total: direction 1..4");
   const int V = ctx->V, NI = ctx->NI, NJ = ctx->NJ;
   std::vector<double> bx((size_t)2 * V * NJ), by((size_t)2 * V * NI);
   int rc = fv_get_boundary_flux(ctx, bx.data(), by.data());

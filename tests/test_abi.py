@@ -100,3 +100,14 @@ def test_device_bytes_accounts_for_states_and_dtype():
     assert 0.49 < b32 / b64 < 0.52
     dp5 = fosite_amd.device_bytes(S.euler_disk_config(ni=512, nj=1024))
     assert dp5 > 12 * 8 * 4 * 1028 * 528              # DP5: 4 + 6 + 2 buffers of 4 fields
+
+
+def test_read_back_entry_points_reject_a_null_context():
+    """The boundary-flux and background-velocity read-backs fail cleanly
+    (no device call) on a null context or a bad direction."""
+    import ctypes
+    L = fosite_amd.load_library()
+    buf = (ctypes.c_double * 16)()
+    assert L.fv_get_boundary_flux(None, buf, buf) == -1
+    assert L.fv_get_boundary_flux_total(None, 1, buf) == -1
+    assert L.fv_get_background_velocity(None, buf) == -1
