@@ -234,7 +234,22 @@ def serving_form(n=300, rate=20.0, nmin=64, nmax=256, orbits=10.0, budget_gb=64.
             "mean_latency_s": float(lat.mean()), "cell_updates_per_s": upd / wall, "wall_s": wall,
             "completed": st["completed"], "failed": st["failed"], "peak_running": st["peak_running"],
             "peak_device_gb": st["peak_device_bytes"] / 1e9, "budget_gb": budget_gb, "rounds": st["rounds"],
-            "gpu_busy_share": st["busy_ms"] * 1e-3 / wall}
+            "gpu_busy_share": st["busy_ms"] * 1e-3 / wall, "cell_updates": upd,
+            "arrival_span_s": reqs[-1][0] - reqs[0][0]}
+
+
+def serving_form_full(n=40):
+    """The serving stream at the stated grid sizes, radial cells U[64,1024]
+    (azimuthal 2x), 20 requests/s: the offered load (all cell updates of the
+    n runs / their arrival span) is several times what one GPU serves, so the
+    queue grows and the latencies measure the backlog.  A bounded prefix of n
+    requests; the object adds the offered load and the share of it served."""
+    out = serving_form(n=n, nmin=64, nmax=1024)
+    span = out["arrival_span_s"]
+    out["offered_cell_updates_per_s"] = out["cell_updates"] / span if span > 0 else None
+    out["served_share"] = (out["cell_updates_per_s"] / out["offered_cell_updates_per_s"]
+                           if span > 0 else None)
+    return out
 
 
 def main():
@@ -248,7 +263,7 @@ def main():
     ap.add_argument("--no-forms", action="store_true",
                     help="skip the precision, batched and serving forms")
     ap.add_argument("--form-steps", type=int, default=500)
-    ap.add_argument("--only-form", choices=["precision", "batched", "serving"],
+    ap.add_argument("--only-form", choices=["precision", "batched", "serving", "serving_full"],
                     help="run one form alone and print its object (for profiling)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -257,7 +272,7 @@ def main():
         return 2
     if a.only_form:
         f = {"precision": lambda: precision_form(1024, 2048, a.form_steps), "batched": batched_form,
-             "serving": serving_form}[a.only_form]
+             "serving": serving_form, "serving_full": serving_form_full}[a.only_form]
         print(json.dumps(f()))
         return 0
 
@@ -328,6 +343,7 @@ def main():
         out["precision_form"] = precision_form(1024, 2048, a.form_steps)
         out["batched_form"] = batched_form()
         out["serving_form"] = serving_form()
+        out["serving_form_full_sizes"] = serving_form_full()
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg, init, a.cpu_steps)
     print(json.dumps(out))
