@@ -40,7 +40,8 @@ class _Cfg(ctypes.Structure):
                 ("tol_abs", ctypes.c_double * 4), ("stoptime", ctypes.c_double),
                 ("dtlimit", ctypes.c_double), ("beta", ctypes.c_double),
                 ("fargo", ctypes.c_int), ("gravity", ctypes.c_int),
-                ("mass", ctypes.c_double), ("GN", ctypes.c_double)]
+                ("mass", ctypes.c_double), ("GN", ctypes.c_double),
+                ("outbound", ctypes.c_int), ("acclimit", ctypes.c_double)]
 
 
 class _Scalars(ctypes.Structure):
@@ -74,6 +75,7 @@ def _load():
         L.fvr_round_state_f32.argtypes = [ctypes.c_void_p]
         L.fvr_get_bflux.argtypes = [ctypes.c_void_p, P, P]
         L.fvr_get_background_velocity.argtypes = [ctypes.c_void_p, P]
+        L.fvr_get_pointmass.argtypes = [ctypes.c_void_p, P]
         _lib = L
     return _lib
 
@@ -118,6 +120,8 @@ class Oracle:
         c.gravity = cfg.get("gravity", 0)
         c.mass = cfg.get("mass", 1.0)
         c.GN = cfg.get("GN", 1.0)
+        c.outbound = cfg.get("outbound", 0)
+        c.acclimit = cfg.get("acclimit", -1.0)
         self._h = L.fvr_create(ctypes.byref(c))
         if not self._h:
             raise ValueError("oracle: unsupported configuration")
@@ -175,6 +179,12 @@ class Oracle:
         by = np.empty((self.V, 2, self.NI))
         self.L.fvr_get_bflux(self._h, _ptr(bx), _ptr(by))
         return bx, by
+
+    def pointmass(self):
+        """Point-mass state: array [mass, accrate, massloss, mdot]."""
+        out = np.empty(4)
+        self.L.fvr_get_pointmass(self._h, _ptr(out))
+        return out
 
     def rhs(self):
         out = np.empty((self.V, self.NJ, self.NI))

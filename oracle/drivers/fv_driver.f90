@@ -17,6 +17,9 @@
 PROGRAM fv_driver
   USE fosite_mod
   USE physics_eulerisotherm_mod, ONLY : physics_eulerisotherm
+  USE sources_base_mod, ONLY : sources_base
+  USE sources_gravity_mod, ONLY : sources_gravity
+  USE gravity_pointmass_mod, ONLY : gravity_pointmass
   IMPLICIT NONE
   !--------------------------------------------------------------------------!
   INTEGER :: k_physics = 17, k_geometry = 1, k_inum = 32, k_jnum = 32
@@ -28,19 +31,20 @@ PROGRAM fv_driver
   REAL    :: k_cfl = 0.4, k_tolrel = 0.01, k_tolabs(4) = 0.001, k_stoptime = 1.0
   REAL    :: k_dtlimit = -1.0, k_beta = 0.08
   INTEGER :: k_fargo = 0, k_gravity = 0, k_units = 0
-  REAL    :: k_mass = 1.0
+  REAL    :: k_mass = 1.0, k_acclimit = -1.0
+  INTEGER :: k_outbound = 0
   INTEGER :: k_nsteps = 10, k_dumpevery = 1
   CHARACTER(LEN=512) :: k_infile = "", k_prefix = "out"
   NAMELIST /fvcfg/ k_physics, k_geometry, k_inum, k_jnum, k_gamma, k_xmin, k_xmax, k_ymin, k_ymax, &
        k_gparam, k_bcw, k_bce, k_bcs, k_bcn, k_limiter, k_recvars, k_theta, k_method, k_order,   &
        k_cfl, k_tolrel, k_tolabs, k_stoptime, k_dtlimit, k_beta, k_fargo, k_gravity, k_units,   &
-       k_mass, k_nsteps, k_dumpevery, k_infile, k_prefix
+       k_mass, k_outbound, k_acclimit, k_nsteps, k_dumpevery, k_infile, k_prefix
   !--------------------------------------------------------------------------!
   CLASS(fosite), ALLOCATABLE :: Sim
   TYPE(Dict_TYP), POINTER    :: config, mesh, phys, bnd, flx, tdisc, dfile, &
                                 src, pmass
   CHARACTER(LEN=512)         :: cfgname
-  INTEGER                    :: u, ub, uw, n, vnum, ncell
+  INTEGER                    :: u, ub, uw, ug, n, vnum, ncell
   INTEGER(KIND=8)            :: c0, c1, crate, ctot
   LOGICAL                    :: brk
   REAL, ALLOCATABLE          :: pv(:), cs(:), fcs(:), w(:)
@@ -83,7 +87,7 @@ PROGRAM fv_driver
        "fluxes" / flx, "timedisc" / tdisc, "datafile" / dfile)
   IF (k_gravity.GT.0) THEN
     pmass => Dict("gtype" / POINTMASS, "potential" / NEWTON, "mass" / k_mass, &
-         "outbound" / 0)
+         "outbound" / k_outbound, "acclimit" / k_acclimit)
     src => Dict("grav/stype" / GRAVITY, "grav/pmass" / pmass)
     CALL SetAttr(config, "sources", src)
   END IF
@@ -122,6 +126,8 @@ PROGRAM fv_driver
        FORM="unformatted", STATUS="replace")
   OPEN(NEWUNIT=uw, FILE=TRIM(k_prefix) // "_w.bin", ACCESS="stream", &
        FORM="unformatted", STATUS="replace")
+  OPEN(NEWUNIT=ug, FILE=TRIM(k_prefix) // "_grav.bin", ACCESS="stream", &
+       FORM="unformatted", STATUS="replace")
   CALL Sim%FirstStep()
   CALL DumpState(u, 0)
   ctot = 0
@@ -137,6 +143,7 @@ PROGRAM fv_driver
   CLOSE(u)
   CLOSE(ub)
   CLOSE(uw)
+  CLOSE(ug)
   ! wall time of the Step() calls only (setup, FirstStep and dumps excluded)
   WRITE(*,'(A,I0,A,ES24.16)') "FV_DRIVER_STEP_SECONDS ", MIN(n, k_nsteps), " ", &
        REAL(ctot, 8) / REAL(crate, 8)
@@ -171,7 +178,27 @@ CONTAINS
               Sim%Fluxes%GetBoundaryFlux(Sim%Mesh, Sim%Physics, NORTH)
     ! FARGO background velocity Timedisc%w(IGMIN:IGMAX,1)
     IF (k_fargo.GT.0) WRITE(uw) Sim%Timedisc%w(:,1)
+    IF (k_gravity.GT.0) CALL DumpPointMass(ug)
   END SUBROUTINE DumpState
+
+  !> point-mass state (gravity_pointmass.f90:79-85): mass, accrate, massloss,
+  !! mdot, then the acceleration accel(:,:,:,1:2) of the point mass (all cells)
+  SUBROUTINE DumpPointMass(u)
+    INTEGER, INTENT(IN) :: u
+    CLASS(sources_base), POINTER :: sp
+    sp => Sim%Sources%next
+    DO WHILE (ASSOCIATED(sp))
+      SELECT TYPE(sg => sp)
+      TYPE IS(sources_gravity)
+        SELECT TYPE(gp => sg%glist)
+        TYPE IS(gravity_pointmass)
+          WRITE(u) gp%mass, gp%accrate, gp%massloss, gp%mdot
+          WRITE(u) gp%accel%data4d(:,:,:,1), gp%accel%data4d(:,:,:,2)
+        END SELECT
+      END SELECT
+      sp => sp%next
+    END DO
+  END SUBROUTINE DumpPointMass
 
   !> metric arrays the GPU context is built from (all cells incl. ghosts)
   SUBROUTINE DumpMesh(fname)

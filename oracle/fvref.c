@@ -11,7 +11,7 @@
  * geometry, Euler and isothermal Euler physics, KT fluxes with linear
  * reconstruction (all non-PP limiters), embedded RK (RKF 3/5, Cash-Karp,
  * Dormand-Prince, SSPRK 3/5) with error control and reject path, no-gradient,
- * periodic and absorbing boundaries, point-mass gravity without accretion,
+ * periodic and absorbing boundaries, point-mass gravity with accretion,
  * FARGO with user-supplied background velocity (type 2).
  */
 #include "fvref.h"
@@ -54,6 +54,7 @@ struct fvr {
   double *dAx[2], *dAy[2], *dydzdV, *dzdxdV, *dxdydV, *dlx, *dly;
   double *cxyx, *cyxy, *czxz, *czyz, *rdx[4], *rad, *radf[4], *posx, *posy, *bcx, *bcy;
   double *accel[2];
+  double pm_mass, pm_accrate, pm_massloss, pm_mdot; /* gravity_pointmass.f90:79-85 */
   /* physics */
   double *bccsound, *fcsound; /* fcsound [face][cell] */
   double *w;                  /* fargo background velocity per x-ring */
@@ -429,11 +430,50 @@ static void geometrical_sources(fvr *s, const double *p, const double *c, double
         for (int l = 0; l < V; l++) st[l * N + IDX(s, i, j)] = 0.0;
 }
 
+/* Fluxes%GetBoundaryFlux (fluxes_base.f90:220-300, serial build) of field l:
+ * SUM over the boundary's interior range in index order */
+static double boundary_flux_sum(const fvr *s, int direction, int l) {
+  const int xd = direction <= 2, side = (direction - 1) & 1;
+  const double *a = xd ? s->bx + (size_t)(2 * l + side) * s->NJ : s->by + (size_t)(2 * l + side) * s->NI;
+  const int hi = G + (xd ? s->c.jnum : s->c.inum) - 1;
+  double acc = 0.0;
+  for (int k = G; k <= hi; k++) acc = acc + a[k];
+  return acc;
+}
+
+/* UpdateGravity_single with accretion (gravity_pointmass.f90:383-428; the
+ * switch-on phase is off): the mass that crossed the outflow boundary since
+ * the last evaluation is added to the point mass and the acceleration is
+ * rescaled by new/old mass.  s->accel holds sources_gravity's sum
+ * 0 + accel_pointmass (sources_gravity.f90:213-226); scaling it gives the same
+ * values as scaling the point mass's own array and adding it to zero. */
+static void update_gravity(fvr *s, double dt) {
+  if (!s->c.gravity || !s->c.outbound) return;
+  const double bflux = boundary_flux_sum(s, s->c.outbound, 0);
+  const double oldmass = s->pm_mass;
+  double dmass = bflux - s->pm_mdot;
+  if (s->c.acclimit >= 0.) {
+    const double dmasslim = fmin2(dmass, s->c.acclimit * dt * s->pm_mass);
+    s->pm_massloss = s->pm_massloss + dmass - dmasslim;
+    dmass = dmasslim;
+  }
+  s->pm_mass = s->pm_mass + dmass;
+  if (dt > 0.0) s->pm_accrate = dmass / dt;
+  const double massfac = s->pm_mass / oldmass;
+  for (int k = 0; k < s->N; k++) {
+    s->accel[0][k] = s->accel[0][k] * massfac;
+    s->accel[1][k] = s->accel[1][k] * massfac;
+  }
+  s->pm_mdot = bflux;
+}
+
 /* sources_generic.f90:220-261, sources_gravity.f90:164-199,
- * physics_eulerisotherm.f90:1201-1226, physics_euler.f90:1849-1885 */
-static void external_sources(fvr *s, const double *c, double *st) {
+ * physics_eulerisotherm.f90:1201-1226, physics_euler.f90:1849-1885;
+ * dt < 0: no UpdateGravity (test aid) */
+static void external_sources(fvr *s, const double *c, double *st, double dt) {
   int N = s->N, V = s->V;
   if (!s->c.gravity) return;
+  if (dt >= 0.0) update_gravity(s, dt);
   for (int k = 0; k < V * N; k++) st[k] = 0.0;
   for (int k = 0; k < N; k++) {
     s->sterm[k] = 0.0;
@@ -649,7 +689,9 @@ static void check_data(fvr *s, const double *p, const double *c, int bm) {
 }
 
 /* timedisc_base.f90:1073-1271 (rhstype 0, 2-D) */
-static void compute_rhs(fvr *s, sv *p, sv *c, int checkbm, double *rhs) {
+/* dt: ComputeRHS's own dt argument (timedisc_base.f90:1073), read by the
+ * point-mass accretion update */
+static void compute_rhs(fvr *s, sv *p, sv *c, int checkbm, double *rhs, double dt) {
   int N = s->N, V = s->V, imin = G, imax = G + s->c.inum - 1, jmin = G, jmax = G + s->c.jnum - 1;
   int fargo = s->c.fargo == 1 || s->c.fargo == 2;
   if (fargo) add_bg_velocity(s, p, c);
@@ -662,7 +704,7 @@ static void compute_rhs(fvr *s, sv *p, sv *c, int checkbm, double *rhs) {
   if (V == 4) update_bccsound(s, s->pvar.d);
   if (checkbm != CHECK_NOTHING) check_data(s, p->d, c->d, checkbm);
   geometrical_sources(s, p->d, c->d, s->geo_src);
-  external_sources(s, c->d, s->src);
+  external_sources(s, c->d, s->src, dt);
   if (fargo) {
     if (s->c.geometry == CARTESIAN) memset(s->geo_src, 0, sizeof(double) * V * N);
     add_fargo_sources(s, c->d, s->geo_src);
@@ -787,7 +829,7 @@ static void solve_ode(fvr *s, double time, double *dt, double *err) {
   (void)time;
   for (int m = 1; m < s->m; m++) {
     compute_cvar(s, m, *dt);
-    compute_rhs(s, &s->ptmp, &s->ctmp, CHECK_NOTHING, s->coeff[m]);
+    compute_rhs(s, &s->ptmp, &s->ctmp, CHECK_NOTHING, s->coeff[m], *dt);
   }
   memcpy(s->ctmp.d, s->cvar.d, sizeof(double) * n);
   s->ctmp.flag = s->cvar.flag;
@@ -871,7 +913,7 @@ static void fargo_advection_y(fvr *s) {
 static void reject_solution(fvr *s, double time, double dt) {
   (void)time;
   copy_sv(s, &s->cvar, &s->cold);
-  compute_rhs(s, &s->pvar, &s->cvar, CHECK_NOTHING, s->coeff[0]);
+  compute_rhs(s, &s->pvar, &s->cvar, CHECK_NOTHING, s->coeff[0], dt);
   memcpy(s->bx, s->bxo, sizeof(double) * 2 * s->V * s->NJ);
   memcpy(s->by, s->byo, sizeof(double) * 2 * s->V * s->NI);
   s->n_adj++;
@@ -883,7 +925,7 @@ static void reject_solution(fvr *s, double time, double dt) {
 static void accept_solution(fvr *s, double *time, double dt) {
   *time = *time + dt;
   if (*time < s->time + s->dt) {
-    compute_rhs(s, &s->pvar, &s->cvar, CHECK_ALL, s->coeff[0]);
+    compute_rhs(s, &s->pvar, &s->cvar, CHECK_ALL, s->coeff[0], dt);
     copy_sv(s, &s->cold, &s->cvar);
   }
   memcpy(s->bxo, s->bx, sizeof(double) * 2 * s->V * s->NJ);
@@ -921,9 +963,9 @@ static void integration_step(fvr *s) {
       fargo_advection_y(s);
       for (int k = 0; k < s->N; k++) cons2prim_pt(s, s->pvar.d, s->cvar.d, s->N, k);
       s->pvar.flag = s->cvar.flag;
-      compute_rhs(s, &s->pvar, &s->cvar, CHECK_ALL, s->coeff[0]);
+      compute_rhs(s, &s->pvar, &s->cvar, CHECK_ALL, s->coeff[0], 0.0);
     } else {
-      compute_rhs(s, &s->pvar, &s->cvar, CHECK_ALL, s->coeff[0]);
+      compute_rhs(s, &s->pvar, &s->cvar, CHECK_ALL, s->coeff[0], dtold);
     }
     copy_sv(s, &s->cold, &s->cvar);
   }
@@ -993,8 +1035,10 @@ fvr *fvr_create(const fvr_cfg *cfg) {
     if (b != NO_GRADIENTS && b != PERIODIC && b != ABSORBING) return NULL;
     if (b == ABSORBING && d >= 2) return NULL;
   }
+  if (cfg->outbound < 0 || cfg->outbound > 4) return NULL;
   s = (fvr *)calloc(1, sizeof(fvr));
   s->c = *cfg;
+  s->pm_mass = cfg->mass; /* mdot, massloss, accrate start at 0 (gravity_pointmass.f90:206-209) */
   s->V = cfg->physics == EULER ? 4 : 3;
   s->NI = cfg->inum + 2 * G;
   s->NJ = cfg->jnum + 2 * G;
@@ -1097,7 +1141,7 @@ int fvr_first_step(fvr *s) {
     center_boundary(s, &s->pvar, &s->cvar);
     calc_background_velocity(s);
   }
-  compute_rhs(s, &s->pvar, &s->cvar, CHECK_ALL, s->coeff[0]);
+  compute_rhs(s, &s->pvar, &s->cvar, CHECK_ALL, s->coeff[0], 0.0);
   s->dt = calc_timestep(s);
   copy_sv(s, &s->cold, &s->cvar);
   /* initial WriteDataset: fileio_base.f90:523-533 moves the state to the
@@ -1146,6 +1190,13 @@ void fvr_get_bflux(const fvr *s, double *bx, double *by) {
   if (by) memcpy(by, s->by, sizeof(double) * 2 * s->V * s->NI);
 }
 
+void fvr_get_pointmass(const fvr *s, double *out) {
+  out[0] = s->pm_mass;
+  out[1] = s->pm_accrate;
+  out[2] = s->pm_massloss;
+  out[3] = s->pm_mdot;
+}
+
 void fvr_compute_rhs(fvr *s, double *rhs_out) {
   size_t n = (size_t)s->V * s->N;
   sv p = {(double *)malloc(sizeof(double) * n), s->pvar.flag};
@@ -1153,7 +1204,7 @@ void fvr_compute_rhs(fvr *s, double *rhs_out) {
   memcpy(p.d, s->pvar.d, sizeof(double) * n);
   memcpy(c.d, s->cvar.d, sizeof(double) * n);
   memset(rhs_out, 0, sizeof(double) * n);
-  compute_rhs(s, &p, &c, CHECK_NOTHING, rhs_out);
+  compute_rhs(s, &p, &c, CHECK_NOTHING, rhs_out, -1.0);
   free(p.d);
   free(c.d);
 }

@@ -31,8 +31,11 @@ typedef struct fvr_cfg {
   int method, order;  /* 2 RK_FEHLBERG, 3 CASH_KARP, 4 DORMAND_PRINCE, 5 SSPRK */
   double cfl, tol_rel, tol_abs[4], stoptime, dtlimit, beta;
   int fargo;          /* 0 off, 1 mean azimuthal velocity per ring (dynamic), 2 user-supplied */
-  int gravity;        /* 1: central point mass at the origin, no accretion */
+  int gravity;        /* 1: central point mass at the origin */
   double mass, GN;
+  int outbound;       /* accretion through boundary 1 W, 2 E, 3 S, 4 N; 0 off
+                         (gravity_pointmass.f90:285-299) */
+  double acclimit;    /* accretion limit per unit mass and time; < 0 off */
 } fvr_cfg;
 
 typedef struct fvr_scalars {
@@ -66,6 +69,9 @@ void fvr_get_background_velocity(const fvr *s, double *w);
  * layout (fluxes_base.f90:142-143): bx(JGMIN:JGMAX,1,2,VNUM) = NJ*2*V values,
  * by(1,IGMIN:IGMAX,2,VNUM) = NI*2*V values (side 1 west/south, 2 east/north) */
 void fvr_get_bflux(const fvr *s, double *bx, double *by);
+/* point mass (gravity_pointmass.f90:79-85): out[0..3] = mass, accrate,
+ * massloss, mdot */
+void fvr_get_pointmass(const fvr *s, double *out);
 /* one ComputeRHS of the current (cvar,pvar) pair without touching the step
  * state's history; rhs out (all cells) -- used for unit-level parity */
 void fvr_compute_rhs(fvr *s, double *rhs_out);

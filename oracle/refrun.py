@@ -32,7 +32,8 @@ def _nml(cfg, nsteps, dump_every, infile, prefix):
                 k_stoptime=cfg["stoptime"], k_dtlimit=cfg.get("dtlimit", -1.0),
                 k_beta=cfg.get("beta", 0.08), k_fargo=cfg.get("fargo", 0),
                 k_gravity=cfg.get("gravity", 0), k_units=3 if cfg.get("gravity", 0) else 0,
-                k_mass=cfg.get("mass", 1.0), k_nsteps=nsteps, k_dumpevery=dump_every)
+                k_mass=cfg.get("mass", 1.0), k_outbound=cfg.get("outbound", 0),
+                k_acclimit=float(cfg.get("acclimit", -1.0)), k_nsteps=nsteps, k_dumpevery=dump_every)
     lines = ["&fvcfg"]
     for k, v in vals.items():
         lines.append(" %s = %s" % (k, repr(float(v)) if isinstance(v, float) else v))
@@ -109,6 +110,15 @@ def run(cfg, init, nsteps, dump_every=1, workdir=None):
         wraw = np.fromfile(prefix + "_w.bin", dtype="<f8")
         for n, r in enumerate(recs):
             r["w"] = wraw[n * NI:(n + 1) * NI].copy()
+    if cfg.get("gravity", 0):
+        # point mass of every record: mass, accrate, massloss, mdot, then
+        # accel(:,:,:,1:2) (gravity_pointmass.f90:79-85)
+        graw = np.fromfile(prefix + "_grav.bin", dtype="<f8")
+        grec = 4 + 2 * N
+        for n, r in enumerate(recs):
+            g = graw[n * grec:(n + 1) * grec]
+            r["pointmass"] = g[:4].copy()
+            r["accel"] = g[4:].reshape(2, NJ, NI).copy()
     mesh["step_seconds"] = step_seconds
     if workdir is None:
         shutil.rmtree(tmp, ignore_errors=True)

@@ -5,7 +5,8 @@ by oracle/build_ref.py, driven through FirstStep + Step() (fosite.f90:258-464)
 -- on small seeded configurations and stores, per case, the configuration,
 the step scalars after every step, the conservative state and the
 accumulated boundary fluxes (with their GetBoundaryFlux sums) after chosen
-steps, plus the reference's mesh arrays for one Cartesian and one
+steps (and, with accretion, the point mass per step and its acceleration
+after the chosen steps), plus the reference's mesh arrays for one Cartesian and one
 logcylindrical mesh.  Output: tests/golden/fv_golden.npz (data only).
 
 Usage (needs the reference checkout and its build):
@@ -51,6 +52,13 @@ CASES = {
     # FARGO type 1: background velocity recomputed every step
     "disk_fargo1_absorbing": (dict(D, fargo=1), 10, (1, 10)),
     "disk_fargo1_cfl_0.9_rejects": (dict(D, fargo=1, cfl=0.9), 12, (12,)),
+    # point-mass accretion through the western boundary (outbound = WEST)
+    "disk_accretion_west": (dict(D, outbound=1), 10, (1, 10)),
+    "disk_accretion_no_fargo_acclimit": (dict(D, outbound=1, fargo=0, acclimit=0.001), 8, (8,)),
+    "disk_accretion_fargo1": (dict(D, outbound=1, fargo=1), 8, (8,)),
+    "disk_accretion_cfl_0.9_rejects": (dict(D, outbound=1, cfl=0.9), 12, (12,)),
+    "disk_accretion_east_no_fargo": (dict(D, outbound=2, fargo=0), 8, (8,)),
+    "euler_disk_accretion": (dict(S.euler_disk_config(ni=16, nj=32), outbound=1), 8, (8,)),
 }
 MESHES = {"cartesian_16": E, "logcyl_16x32": D}
 SCALARS = ("iter", "n_adj", "brk", "time", "dt", "dtold", "maxerrold")
@@ -69,6 +77,8 @@ def main():
         out[name + "/steps"] = np.array([steps])
         for k in SCALARS:
             out[name + "/" + k] = np.array([r[k] for r in recs])
+        if cfg.get("outbound", 0):  # point mass: mass, accrate, massloss, mdot per step
+            out[name + "/pointmass"] = np.array([r["pointmass"] for r in recs])
         for s in keep:
             out["%s/cvar_%d" % (name, s)] = recs[s]["cvar"]
             # boundary fluxes bxflux(JG,1,2,V) / byflux(1,IG,2,V) as [l][side][j|i]
@@ -77,6 +87,8 @@ def main():
             out["%s/bsum_%d" % (name, s)] = recs[s]["bsum"]
             if cfg.get("fargo", 0):  # Timedisc%w(IGMIN:IGMAX)
                 out["%s/w_%d" % (name, s)] = recs[s]["w"]
+            if cfg.get("outbound", 0):  # accel(:,:,:,1:2) of the point mass
+                out["%s/accel_%d" % (name, s)] = recs[s]["accel"]
         print(name, "n_adj", recs[-1]["n_adj"], "time %.6e" % recs[-1]["time"])
     for name, cfg in MESHES.items():
         mesh, _ = refrun.run(cfg, S.initial(cfg), 0, dump_every=1)

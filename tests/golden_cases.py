@@ -50,6 +50,14 @@ def bflux(name):
     return out
 
 
+def pointmass(name):
+    """(per-step [mass, accrate, massloss, mdot] array, {step: accel [2,NJ,NI]})
+    of an accretion case"""
+    d = data()
+    acc = {int(k.split("_")[-1]): d[k] for k in d if k.startswith(name + "/accel_")}
+    return d[name + "/pointmass"], acc
+
+
 def meshes():
     return sorted({k.split("/")[0][5:] for k in data() if k.startswith("mesh_")})
 

@@ -135,6 +135,69 @@ def test_oracle_boundary_flux_balances_the_interior_totals():
     assert np.all(np.abs(change + out) <= 1e-12 * scale), (change, out)
 
 
+ACCRETION = [n for n in G.cases() if G.case(n)[0].get("outbound", 0)]
+
+
+@pytest.mark.parametrize("name", ACCRETION)
+def test_oracle_point_mass_matches_reference_fixture(name):
+    """Point-mass accretion (gravity_pointmass.f90:383-428): mass, accrate,
+    massloss and mdot after every step, and the rescaled acceleration after
+    the stored steps, bit for bit.  The cases cover FARGO types 0/1/2 (the
+    end-of-step evaluation passes dt = 0 with FARGO, so accrate stays 0
+    there), rejected steps (whose RejectSolution evaluation sees the
+    rejected boundary fluxes), the accretion limit, the eastern boundary and
+    Euler physics."""
+    cfg, steps, _, _ = G.case(name)
+    pm_ref, acc_ref = G.pointmass(name)
+    init = S.initial(cfg)
+    o = oracle.Oracle(cfg)
+    o.set_initial(init["pvar"], init.get("bccsound"), init.get("fcsound"), init.get("w"))
+    o.first_step()
+    assert np.array_equal(o.pointmass(), pm_ref[0]), name
+    for n in range(1, steps + 1):
+        o.step()
+        assert np.array_equal(o.pointmass(), pm_ref[n]), (name, n, o.pointmass(), pm_ref[n])
+        if n in acc_ref:
+            assert np.array_equal(o.mesh("accel_x"), acc_ref[n][0]), (name, n)
+            assert np.array_equal(o.mesh("accel_y"), acc_ref[n][1]), (name, n)
+
+
+def test_accretion_fixtures_cover_the_update_branches():
+    pm = {n: G.pointmass(n)[0] for n in ACCRETION}
+    assert any(p[-1, 1] != 0.0 for p in pm.values())            # accrate set (no FARGO)
+    assert any(p[-1, 2] != 0.0 for p in pm.values())            # accretion limit hit
+    assert any(p[-1, 0] < 1.0 for p in pm.values())             # mass can fall
+    assert max(G.case(n)[2]["n_adj"][-1] for n in ACCRETION) > 5  # rejected steps
+
+
+@pytest.mark.parametrize("outbound,fargo,acclimit", [(1, 2, -1.0), (1, 0, 0.001), (2, 0, -1.0), (1, 1, -1.0)])
+def test_accretion_mass_budget_and_acceleration_scaling(outbound, fargo, acclimit):
+    """What the accretion update must satisfy whatever its evaluation order:
+    mdot is the GetBoundaryFlux density sum of the current boundary fluxes;
+    the mass increments telescope, so mass + massloss - mass0 = mdot to
+    round-off; every evaluation scales the acceleration by new/old mass, so
+    accel = accel0 * mass/mass0 to round-off.  A wrong boundary, side, sign
+    or a dropped rescale fails one of them."""
+    cfg = dict(S.kepler_disk_config(ni=16, nj=32), outbound=outbound, fargo=fargo, acclimit=acclimit)
+    init = S.initial(cfg)
+    o = oracle.Oracle(cfg)
+    o.set_initial(init["pvar"], init.get("bccsound"), init.get("fcsound"), init.get("w"))
+    o.first_step()
+    ax0 = o.mesh("accel_x").copy()
+    for _ in range(8):
+        o.step()
+    mass, _, massloss, mdot = o.pointmass()
+    bx, _ = o.bflux()
+    acc = 0.0
+    for k in range(2, cfg["jnum"] + 2):
+        acc = acc + bx[0, outbound - 1, k]
+    assert mdot == acc and abs(mdot) > 1e-5
+    assert abs(mass + massloss - 1.0 - mdot) <= 1e-13
+    ax = o.mesh("accel_x")
+    nz = ax0 != 0
+    assert np.abs(ax[nz] / ax0[nz] - mass / 1.0).max() <= 1e-13
+
+
 @pytest.mark.parametrize("name", G.meshes())
 def test_oracle_mesh_matches_reference_fixture(name):
     cfg, arrays = G.mesh(name)
