@@ -35,7 +35,8 @@ class FvConfig(ctypes.Structure):
                 ("tol_abs", ctypes.c_double * 4), ("stoptime", ctypes.c_double),
                 ("dtlimit", ctypes.c_double), ("beta", ctypes.c_double),
                 ("fargo", ctypes.c_int), ("gravity", ctypes.c_int),
-                ("mass", ctypes.c_double), ("GN", ctypes.c_double), ("dtype", ctypes.c_int)]
+                ("mass", ctypes.c_double), ("GN", ctypes.c_double), ("dtype", ctypes.c_int),
+                ("outbound", ctypes.c_int), ("acclimit", ctypes.c_double)]
 
     @classmethod
     def from_dict(cls, cfg):
@@ -66,7 +67,7 @@ class FvScalars(ctypes.Structure):
 EXPORTS = ["fv_mesh_array", "fv_create", "fv_destroy", "fv_dims", "fv_set_sound_speeds",
            "fv_set_background_velocity", "fv_set_pvar", "fv_first_step", "fv_step", "fv_run",
            "fv_get_state", "fv_get_scalars", "fv_get_boundary_flux", "fv_get_boundary_flux_total",
-           "fv_get_background_velocity",
+           "fv_get_background_velocity", "fv_get_point_mass",
            "fv_last_run_ms", "fv_last_error", "fv_profile",
            "fv_kernel_stats", "fv_batch_create", "fv_batch_add", "fv_batch_remove", "fv_batch_run",
            "fv_batch_get_info", "fv_batch_destroy", "fv_batch_last_error", "fv_device_bytes",
@@ -132,6 +133,7 @@ def load_library():
     L.fv_get_boundary_flux.argtypes = [V, P, P]
     L.fv_get_background_velocity.argtypes = [V, P]
     L.fv_get_boundary_flux_total.argtypes = [V, ctypes.c_int, P]
+    L.fv_get_point_mass.argtypes = [V, P]
     L.fv_last_run_ms.argtypes = [V]
     L.fv_last_run_ms.restype = ctypes.c_double
     L.fv_last_error.argtypes = [V]
@@ -282,6 +284,14 @@ class Fosite:
         out = np.empty(self.V)
         self._chk(self.L.fv_get_boundary_flux_total(self._h, int(direction), _ptr(out)),
                   "fv_get_boundary_flux_total")
+        return out
+
+    def point_mass(self):
+        """The point mass (gravity_pointmass.f90:79-85): array [mass, accrate,
+        massloss, mdot]; with accretion (config key ``outbound``) updated by
+        every ComputeRHS as UpdateGravity_single does (383-428)."""
+        out = np.empty(4)
+        self._chk(self.L.fv_get_point_mass(self._h, _ptr(out)), "fv_get_point_mass")
         return out
 
     def scalars(self):

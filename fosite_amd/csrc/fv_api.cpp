@@ -118,11 +118,17 @@ struct fv_ctx {
   double *bf[2] = {};
   int bcur = 0;
   double *bks = nullptr;  // boundary rows of the last stage's rhs
+  // point-mass accretion (c.outbound): state buffers [PM_HDR + 2*NI] and their
+  // FP32 rings; every UpdateGravity reads grav[gcur], writes the other and
+  // makes it current
+  double *grav[2] = {};
+  float *gravf[2] = {};
+  int gcur = 0;
   // FARGO type 1: azimuthal velocities the last ComputeRHS left for the next
   // CalcBackgroundVelocity ([NJ][P], FP64)
   double *vbuf = nullptr;
   Ctl *d_ctl = nullptr, *h_ctl = nullptr;
-  struct GraphEntry { int x0, x1, t, k0, b0, nsteps; hipGraphExec_t exec; };
+  struct GraphEntry { int x0, x1, t, k0, b0, g0, nsteps; hipGraphExec_t exec; };
   std::vector<GraphEntry> graphs;
   bool fast = true;         // device-controlled batches allowed
   int batch_steps = 0, batch_halts = 0;
@@ -225,12 +231,54 @@ int reduce(fv_ctx *ctx, const double *part, int n, double *mx, int *flags) {
   return 0;
 }
 
+// gravitational acceleration rings of an evaluation: point-mass state q
+// (accretion), else the static rings
+void set_acc(const fv_ctx *ctx, StageArgs &a, int q) {
+  if (ctx->grav[0]) {
+    a.acc = ctx->grav[q] + PM_HDR;
+    a.accf = ctx->gravf[q];
+  } else {
+    a.acc = ctx->d_ring + (size_t)R_ACCX * ctx->NI;
+    a.accf = ctx->d_ringf ? ctx->d_ringf + (size_t)R_ACCX * ctx->NI : nullptr;
+  }
+}
+
 StageArgs base_args(fv_ctx *ctx) {
   StageArgs a;
   std::memset(&a, 0, sizeof(a));
   a.part = ctx->part_r;
   a.bflux = 1;
+  set_acc(ctx, a, ctx->gcur);
   return a;
+}
+
+// UpdateGravity_single of one ComputeRHS (gravity_pointmass.f90:383-428):
+// point-mass state gin -> 1 - gin, with the boundary fluxes bf[b] and the
+// call's dt argument
+AccreteArgs accrete_args(const fv_ctx *ctx, int gin, int b, double dt) {
+  AccreteArgs p;
+  std::memset(&p, 0, sizeof(p));
+  const int ob = ctx->c.outbound, xd = ob <= 2, side = (ob - 1) & 1;
+  p.gin = ctx->grav[gin];
+  p.gout = ctx->grav[1 - gin];
+  p.gfout = ctx->gravf[1 - gin];
+  // density rows: x part [V][2 sides][NJ], then y part [V][2 sides][NI]
+  p.bf = ctx->bf[b] + (xd ? (size_t)side * ctx->NJ : 2 * (size_t)ctx->V * ctx->NJ + (size_t)side * ctx->NI);
+  p.lo = GH;
+  p.hi = GH + (xd ? ctx->c.jnum : ctx->c.inum) - 1;
+  p.NI = ctx->NI;
+  p.acclimit = ctx->c.acclimit;
+  p.dt = dt;
+  return p;
+}
+// the UpdateGravity that ComputeRHS performs first; the reference runs it at
+// every evaluation, here on the host-controlled path as well
+int accrete(fv_ctx *ctx, int b, double dt) {
+  if (!ctx->grav[0]) return 0;
+  launch_accrete(accrete_args(ctx, ctx->gcur, b, dt), ctx->st);
+  HIPCHK(hipGetLastError());
+  ctx->gcur = 1 - ctx->gcur;
+  return 0;
 }
 
 // FARGO frame flags of one ComputeRHS call (timedisc_base.f90:1098-1208):
@@ -350,6 +398,7 @@ void launch_stage(fv_ctx *ctx, const StageArgs &a, int m) {
 
 // ComputeRHS on the stage state (ptmp, ctmp), ctmp = cvar - sum a*dt*k
 int rhs_stage(fv_ctx *ctx, int m, double dt, int out) {
+  if (int rc = accrete(ctx, ctx->bcur, dt)) return rc;
   int add, sub;
   rhs_flags(ctx, ctx->fl_ptmp, ctx->fl_ctmp, add, sub);
   StageArgs a = stage_args(ctx, m, ctx->cvar, ctx->k0cur, out);
@@ -449,7 +498,7 @@ int calc_background_velocity(fv_ctx *ctx) {
 // timedisc_base.f90:649-750
 int integration_step(fv_ctx *ctx) {
   int rc;
-  double err = 0.0, dtold, dt, time;
+  double err = 0.0, dtold = ctx->dt, dt, time;
   if (ctx->c.fargo == 1 && (rc = calc_background_velocity(ctx))) return rc;
   if (ctx->c.fargo && (rc = sub_bg_main(ctx))) return rc;
   time = ctx->time;
@@ -463,6 +512,9 @@ int integration_step(fv_ctx *ctx) {
     } else if (err >= 1.0) {  // RejectSolution (timedisc_base.f90:961-1009)
       ctx->cvar = ctx->cold;
       ctx->fl_cvar = ctx->fl_cold;
+      // its ComputeRHS runs before bxflux = bxfold: the point mass sees the
+      // rejected step's boundary fluxes
+      if ((rc = accrete(ctx, 1 - ctx->bcur, dt))) return rc;
       if ((rc = rhs_main(ctx, CHECK_NOTHING, 0))) return rc;
       ctx->n_adj++;
       ctx->dtcause = DTCAUSE_ERRADJ;
@@ -471,6 +523,7 @@ int integration_step(fv_ctx *ctx) {
       time = time + dtold;
       ctx->bcur = 1 - ctx->bcur;  // bxfold = bxflux
       if (time < ctx->time + ctx->dt) {
+        if ((rc = accrete(ctx, ctx->bcur, dtold))) return rc;
         if ((rc = rhs_main(ctx, CHECK_ALL, 0))) return rc;
         ctx->cold = ctx->cvar;
         ctx->fl_cold = ctx->fl_cvar;
@@ -488,6 +541,8 @@ int integration_step(fv_ctx *ctx) {
     // Convert2Primitive (timedisc_base.f90:722-741) are folded into the
     // end-of-step RHS evaluation, which reads the remapped state on the fly
     if (remap) ctx->fl_pvar = ctx->fl_cvar;
+    // ComputeRHS(..., this%time, 0. with FARGO, else dtold, ...) (timedisc_base.f90:742-746)
+    if ((rc = accrete(ctx, ctx->bcur, remap ? 0.0 : dtold))) return rc;
     if ((rc = rhs_main(ctx, CHECK_ALL, remap))) return rc;
     ctx->cold = ctx->cvar;
     ctx->fl_cold = ctx->fl_cvar;
@@ -547,16 +602,32 @@ struct FastStep {
   StageArgs st[MAXK];  // stages 2..s (index m-1); the last one fused with the combination
   bool remap;
   RemapArgs rm;
+  bool accrete;        // point-mass accretion: boundary fluxes, then UpdateGravity
+  BfluxArgs bfl;
+  AccreteArgs acc;
   StageArgs end;       // end-of-step ComputeRHS: CFL, data checks, new state with ghosts
 };
 void fast_step_args(fv_ctx *ctx, Ctl *ctl, int xin, int xout, int tb, int rb, int k0in, int k0out, int bin,
-                    FastStep &f) {
+                    int gin, FastStep &f) {
   const int steady = ctx->c.fargo == 2 ? 1 : 0;
   for (int m = 1; m < ctx->tab.m; m++) {
     StageArgs a = stage_args(ctx, m, xin, k0in, tb);
     a.ctl = ctl;
     a.add_bg = a.sub_bg = steady;
+    set_acc(ctx, a, gin);
     f.st[m - 1] = a;
+  }
+  // accretion: the stage evaluations see unchanged boundary fluxes (their
+  // UpdateGravity only sets accrate = 0); the end-of-step evaluation sees the
+  // step's, so they are accumulated before it
+  f.accrete = ctx->grav[0] != nullptr;
+  if (f.accrete) {
+    f.bfl = bflux_args(ctx, k0in, bin, 0.0);
+    f.bfl.ctl = ctl;
+    f.acc = accrete_args(ctx, gin, 1 - bin, 0.0);
+    f.acc.dt_ctl = ctx->c.fargo ? 0 : 1;
+    f.acc.zero_accrate = 1;
+    f.acc.ctl = ctl;
   }
   int y = tb;
   f.remap = ctx->c.fargo == 2;
@@ -573,18 +644,25 @@ void fast_step_args(fv_ctx *ctx, Ctl *ctl, int xin, int xout, int tb, int rb, in
   a.add_bg = a.sub_bg = steady;
   a.pmin_clamp = ctx->V == 4 ? 1 : 0;
   a.want_cfl = a.want_check = 1;
-  a.bfa = bflux_args(ctx, k0in, bin, 0.0);  // the step's boundary fluxes
+  if (f.accrete)
+    set_acc(ctx, a, 1 - gin);
+  else
+    a.bfa = bflux_args(ctx, k0in, bin, 0.0);  // the step's boundary fluxes
   f.end = a;
 }
 
-void enqueue_fast_step(fv_ctx *ctx, int xin, int xout, int tb, int rb, int k0in, int k0out, int bin) {
+void enqueue_fast_step(fv_ctx *ctx, int xin, int xout, int tb, int rb, int k0in, int k0out, int bin, int gin) {
   FastStep f;
-  fast_step_args(ctx, ctx->d_ctl, xin, xout, tb, rb, k0in, k0out, bin, f);
+  fast_step_args(ctx, ctx->d_ctl, xin, xout, tb, rb, k0in, k0out, bin, gin, f);
   for (int m = 1; m < ctx->tab.m; m++) launch_stage(ctx, f.st[m - 1], m);
   if (f.remap) {
     long pk = prof_begin(ctx);
     launch_remap(ctx->g, f.rm, ctx->V, ctx->st);
     prof_end(ctx, pk, K_REMAP, 2.0 * word_bytes(ctx));
+  }
+  if (f.accrete) {
+    launch_bflux(f.bfl, ctx->st);
+    launch_accrete(f.acc, ctx->st);
   }
   long pk = prof_begin(ctx);
   launch_rhs(ctx->g, f.end, ctx->V, ctx->st);
@@ -636,6 +714,7 @@ int replay_device_steps(fv_ctx *ctx, const Ctl *h, int x0, int x1, int *halted) 
     ctx->cvar = ctx->cold = (s % 2 == 0) ? x1 : x0;
     ctx->k0cur = 1 - ctx->k0cur;
     ctx->bcur = 1 - ctx->bcur;
+    if (ctx->grav[0]) ctx->gcur = 1 - ctx->gcur;
   }
   if (k > 0) {  // CalcTimestep of the last accepted step (before output clipping)
     ctx->invdt = h->invdt;
@@ -647,21 +726,22 @@ int replay_device_steps(fv_ctx *ctx, const Ctl *h, int x0, int x1, int *halted) 
   return 0;
 }
 
-int get_graph(fv_ctx *ctx, int x0, int x1, int tb, int rb, int k0, int b0, hipGraphExec_t *out) {
+int get_graph(fv_ctx *ctx, int x0, int x1, int tb, int rb, int k0, int b0, int g0, hipGraphExec_t *out) {
   for (auto &e : ctx->graphs)
-    if (e.x0 == x0 && e.x1 == x1 && e.t == tb && e.k0 == k0 && e.b0 == b0) {
+    if (e.x0 == x0 && e.x1 == x1 && e.t == tb && e.k0 == k0 && e.b0 == b0 && e.g0 == g0) {
       *out = e.exec;
       return 0;
     }
   hipGraph_t graph;
   HIPCHK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
-  enqueue_fast_step(ctx, x0, x1, tb, rb, k0, 1 - k0, b0);
-  enqueue_fast_step(ctx, x1, x0, tb, rb, 1 - k0, k0, 1 - b0);
+  const int g1 = ctx->grav[0] ? 1 - g0 : g0;
+  enqueue_fast_step(ctx, x0, x1, tb, rb, k0, 1 - k0, b0, g0);
+  enqueue_fast_step(ctx, x1, x0, tb, rb, 1 - k0, k0, 1 - b0, g1);
   HIPCHK(hipStreamEndCapture(ctx->st, &graph));
   hipGraphExec_t exec;
   HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
   (void)hipGraphDestroy(graph);
-  ctx->graphs.push_back({x0, x1, tb, k0, b0, 2, exec});
+  ctx->graphs.push_back({x0, x1, tb, k0, b0, g0, 2, exec});
   *out = exec;
   return 0;
 }
@@ -675,12 +755,12 @@ int run_batch(fv_ctx *ctx, int nmax, int *done_steps, int *halted) {
   const int x1 = pick_free(ctx, x0, -1, -1);
   const int tb = pick_free(ctx, x0, x1, -1);
   const int rb = pick_free(ctx, x0, x1, tb);
-  const int k0 = ctx->k0cur, b0 = ctx->bcur;
+  const int k0 = ctx->k0cur, b0 = ctx->bcur, g0 = ctx->gcur, g1 = ctx->grav[0] ? 1 - g0 : g0;
   Ctl *h = ctx->h_ctl;
   prime_ctl(ctx, h, nmax);
   hipGraphExec_t exec = nullptr;
   if (!ctx->prof) {
-    int rc = get_graph(ctx, x0, x1, tb, rb, k0, b0, &exec);
+    int rc = get_graph(ctx, x0, x1, tb, rb, k0, b0, g0, &exec);
     if (rc) return rc;
   }
   HIPCHK(hipMemcpyAsync(ctx->d_ctl, h, offsetof(Ctl, err_hist), hipMemcpyHostToDevice, ctx->st));
@@ -689,7 +769,7 @@ int run_batch(fv_ctx *ctx, int nmax, int *done_steps, int *halted) {
   } else {  // timed: the same launches, issued one by one between events
     for (int s = 0; s < nmax; s++)
       enqueue_fast_step(ctx, s % 2 ? x1 : x0, s % 2 ? x0 : x1, tb, rb, s % 2 ? 1 - k0 : k0, s % 2 ? k0 : 1 - k0,
-                        s % 2 ? 1 - b0 : b0);
+                        s % 2 ? 1 - b0 : b0, s % 2 ? g1 : g0);
     HIPCHK(hipGetLastError());
   }
   HIPCHK(hipMemcpyAsync(h, ctx->d_ctl, sizeof(Ctl), hipMemcpyDeviceToHost, ctx->st));
@@ -738,6 +818,8 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
   if ((c.bc[0] == PERIODIC) != (c.bc[1] == PERIODIC) || (c.bc[2] == PERIODIC) != (c.bc[3] == PERIODIC))
     return bail("opposite boundary should be periodic");
   if (c.gravity && c.geometry != LOGCYLINDRICAL) return bail("point-mass gravity requires a logcylindrical mesh");
+  if (c.outbound < 0 || c.outbound > 4) return bail("outbound must be 0 (no accretion) or a boundary 1..4");
+  if (c.outbound && !c.gravity) return bail("accretion (outbound) needs the point mass (gravity = 1)");
   if (!set_tableau(c.method, c.order, ctx->tab)) return bail("unsupported time integrator (method/order)");
   if (mesh_build(c, ctx->mesh)) return bail("mesh construction failed");
   ctx->V = c.physics == EULER ? 4 : 3;
@@ -807,6 +889,10 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
        hipMemsetAsync(ctx->d_ctl, 0, sizeof(Ctl), ctx->st) == hipSuccess && hipStreamSynchronize(ctx->st) == hipSuccess;
   ok = ok && hipHostMalloc((void **)&ctx->h_red, sizeof(double) * 2) == hipSuccess;
   ok = ok && hipHostMalloc((void **)&ctx->h_ctl, sizeof(Ctl)) == hipSuccess;
+  for (int q = 0; q < 2 && ok && c.outbound; q++) {
+    ok = dalloc(&ctx->grav[q], sizeof(double) * (PM_HDR + 2 * ctx->NI));
+    if (ok && c.dtype == 1) ok = hipMalloc((void **)&ctx->gravf[q], sizeof(float) * 2 * ctx->NI) == hipSuccess;
+  }
   if (!ok) {
     fv_destroy(ctx);
     std::fprintf(stderr, "fv_create: device allocation failed\n");
@@ -817,6 +903,19 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
   g.ringf = ctx->d_ringf;
   g.colf = ctx->d_colf;
   if (upload_geom(ctx)) { fv_destroy(ctx); return -3; }
+  if (c.outbound) {  // InitGravity_pointmass: mass; mdot, massloss, accrate = 0 (gravity_pointmass.f90:206-209)
+    std::vector<double> gh(PM_HDR + 2 * (size_t)ctx->NI, 0.0);
+    gh[PM_MASS] = c.mass;
+    std::memcpy(&gh[PM_HDR], &ctx->ring_h[(size_t)R_ACCX * ctx->NI], sizeof(double) * 2 * ctx->NI);
+    std::vector<float> gf(gh.begin() + PM_HDR, gh.end());
+    bool up = true;
+    for (int q = 0; q < 2; q++) {
+      up = up && hipMemcpy(ctx->grav[q], gh.data(), sizeof(double) * gh.size(), hipMemcpyHostToDevice) == hipSuccess;
+      if (ctx->gravf[q])
+        up = up && hipMemcpy(ctx->gravf[q], gf.data(), sizeof(float) * gf.size(), hipMemcpyHostToDevice) == hipSuccess;
+    }
+    if (!up) { fv_destroy(ctx); return -3; }
+  }
   // timedisc_base.f90:315-352 defaults
   ctx->time = 0.0;
   ctx->dt = c.stoptime;
@@ -837,6 +936,8 @@ extern "C" int fv_destroy(fv_ctx *ctx) {
   for (auto &b : ctx->k) if (b) (void)hipFree(b);
   for (auto &b : ctx->k0) if (b) (void)hipFree(b);
   for (auto &b : ctx->bf) if (b) (void)hipFree(b);
+  for (auto &b : ctx->grav) if (b) (void)hipFree(b);
+  for (auto &b : ctx->gravf) if (b) (void)hipFree(b);
   if (ctx->vbuf) (void)hipFree(ctx->vbuf);
   if (ctx->bks) (void)hipFree(ctx->bks);
   if (ctx->part_r) (void)hipFree(ctx->part_r);
@@ -869,6 +970,7 @@ extern "C" int fv_device_bytes(const fv_config *c, double *bytes) {
   double b = state * (NBUF + (t.m - 1) + 2);          // states, stage rhs, coeff(1) pair
   b += 3.0 * 8.0 * 2.0 * V * (NI + NJ);                // boundary fluxes (two copies), last-stage rows
   if (c->fargo == 1) b += 8.0 * NJ * P;                // FARGO type 1 velocities
+  if (c->outbound) b += 2.0 * (8.0 * (PM_HDR + 2.0 * NI) + (c->dtype == 1 ? 4.0 * 2.0 * NI : 0.0));
   b += 2.0 * 2.0 * 8.0 * nb32;                         // CFL and error partials
   b += 8.0 * (R_COUNT * NI + C_COUNT * NJ);             // metric arrays
   if (c->dtype == 1) b += 4.0 * (R_COUNT * NI + C_COUNT * NJ);
@@ -959,6 +1061,7 @@ extern "C" int fv_first_step(fv_ctx *ctx, int *brk) {
     HIPCHK(hipGetLastError());
     if ((rc = calc_background_velocity(ctx))) return rc;
   }
+  if ((rc = accrete(ctx, ctx->bcur, 0.0))) return rc;  // ComputeRHS(..., 0.0, ...) (fosite.f90:315-317)
   rc = rhs_main(ctx, CHECK_ALL, 0);
   if (rc) return rc;
   ctx->dt = calc_timestep(ctx);
@@ -1070,6 +1173,20 @@ extern "C" int fv_get_boundary_flux(fv_ctx *ctx, double *bx, double *by) {
   // the device layout is the reference's (ghost entries stay zero)
   if (bx) HIPCHK(hipMemcpyAsync(bx, ctx->bf[ctx->bcur], sizeof(double) * nx, hipMemcpyDeviceToHost, ctx->st));
   if (by) HIPCHK(hipMemcpyAsync(by, ctx->bf[ctx->bcur] + nx, sizeof(double) * ny, hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  return 0;
+}
+
+extern "C" int fv_get_point_mass(fv_ctx *ctx, double *out) {
+  if (!ctx) return -1;
+  if (!out) return fail(ctx, "fv_get_point_mass: null output");
+  if (!ctx->c.gravity) return fail(ctx, "fv_get_point_mass: no point mass (gravity = 0)");
+  if (!ctx->grav[0]) {
+    out[PM_MASS] = ctx->c.mass;
+    out[PM_ACCRATE] = out[PM_MASSLOSS] = out[PM_MDOT] = 0.0;
+    return 0;
+  }
+  HIPCHK(hipMemcpyAsync(out, ctx->grav[ctx->gcur], sizeof(double) * PM_HDR, hipMemcpyDeviceToHost, ctx->st));
   HIPCHK(hipStreamSynchronize(ctx->st));
   return 0;
 }
@@ -1335,7 +1452,7 @@ int batch_round(fv_batch *b, const std::vector<int> &want, int K, int c, bool ti
     for (int p = 0; p < 2; p++) {
       FastStep f;
       fast_step_args(x, dc, p ? r.x1 : r.x0, p ? r.x0 : r.x1, r.tb, r.rb, p ? 1 - r.k0 : r.k0,
-                     p ? r.k0 : 1 - r.k0, p ? 1 - r.b0 : r.b0, f);
+                     p ? r.k0 : 1 - r.k0, p ? 1 - r.b0 : r.b0, x->gcur, f);  // members have no accretion
       for (int q = 0; q < m - 1; q++) b->h_rhs[(size_t)(p * m + q) * cap + d] = RhsEntry{x->g, f.st[q]};
       b->h_rhs[(size_t)(p * m + m - 1) * cap + d] = RhsEntry{x->g, f.end};
       RemapEntry re;
@@ -1417,6 +1534,7 @@ extern "C" int fv_batch_create(fv_batch **out, const fv_batch_opts *opts) {
 extern "C" int fv_batch_add(fv_batch *b, fv_ctx *x) {
   if (!b || !x) return -1;
   if (x->batch) return bfail(b, "context already belongs to a batch");
+  if (x->grav[0]) return bfail(b, "point-mass accretion couples a boundary reduction into every step: not batched");
   if ((int)b->sims.size() >= MAX_BATCH) return bfail(b, "batch is full (" + std::to_string(MAX_BATCH) + " members)");
   const int nbx = (x->c.inum + 31) / 32, nby = (x->c.jnum + TY_TILE - 1) / TY_TILE, rby = remap_rows_blocks(x->c.jnum);
   if (nbx >= (1 << TILE_BX_BITS) || nby >= (1 << TILE_BY_BITS) || rby >= (1 << TILE_BY_BITS))

@@ -71,6 +71,26 @@ struct BfluxArgs {
   double dt;            // host-controlled step; device rounds take Ctl::dt
   int nk, V, NI, NJ, P, inum, jnum, dtype;
   long FS;
+  Ctl *ctl;             // a device-controlled k_bflux launch: dt and halting from here
+};
+
+// Point-mass accretion (gravity_pointmass.f90:79-85, 383-428).  A state
+// buffer holds mass, accrate, massloss, mdot, then the acceleration rings
+// x and y ([2][NI], sources_gravity's 0 + accel_pointmass); the context keeps
+// two and every update reads one and writes the other.
+enum PointMassSlot { PM_MASS, PM_ACCRATE, PM_MASSLOSS, PM_MDOT, PM_HDR };
+struct AccreteArgs {
+  const double *gin;
+  double *gout;
+  float *gfout;         // FP32 copies of the output rings (dtype 1), or null
+  const double *bf;     // density row of the outflow boundary's boundary fluxes
+  int lo, hi;           // its interior range (GetBoundaryFlux sums lo..hi in order)
+  int NI;
+  double acclimit;
+  double dt;            // ComputeRHS's dt argument (host-controlled)
+  int dt_ctl;           // device rounds: dt = Ctl::dt
+  int zero_accrate;     // device rounds: the stage evaluations before (dmass = 0, dt > 0) left accrate = 0
+  Ctl *ctl;             // device rounds: halting
 };
 
 // One right-hand-side evaluation (timedisc_base.f90:1073-1271) on the RK
@@ -112,6 +132,10 @@ struct StageArgsT {
   // CalcBackgroundVelocity sees it, (v - w) + w (timedisc_base.f90:2243-2326),
   // for interior rows of every ring (ghost rings included); [NJ][P], or null
   double *vout;
+  // gravitational acceleration of this evaluation, rings x then y ([2][NI]):
+  // the static rings, or the current point-mass state with accretion
+  const double *acc;
+  const float *accf;    // FP32 copies (dtype 1)
 };
 using StageArgs = StageArgsT<double>;
 
@@ -165,6 +189,7 @@ void launch_frame(const Geom &g, const void *src, void *dst, int sign, int V, hi
 void launch_reduce(const double *part, int n, double *out, hipStream_t s);
 void launch_control(Ctl *ctl, const double *part_c, const double *part_r, int nb, hipStream_t s);
 void launch_bflux(const BfluxArgs &bf, hipStream_t s);
+void launch_accrete(const AccreteArgs &p, hipStream_t s);
 // FARGO type 1: w(i) = SUM_j v(i,j)*wfactor(i) / JNUM per ring into the ring
 // arrays (FP64 and, for the precision form, FP32)
 void launch_wsum(const Geom &g, const double *v, double *ring, float *ringf, hipStream_t s);

@@ -833,7 +833,7 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
         const double rho = (double)ur[0], myd = rho * vyd;
         const double cyd = ring(g, R_CYXY, i);
         double s1 = myd * (cyd * vyd - ring(g, R_CXYX, i) * (double)vx) + (cyd + ring(g, R_CZXZ, i)) * (double)P;
-        if (g.gravity) s1 = s1 + rho * ring(g, R_ACCX, i);
+        if (g.gravity) s1 = s1 + rho * a.acc[i];
         geo[1] = (T)s1;
       } else {
         geo[1] = my * (cyxy * vy - cxyx * vx) + mz * (czxz * vz - zero * vx) + (cyxy + czxz) * P;
@@ -847,7 +847,8 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
     // sources_gravity.f90:164-199, physics_eulerisotherm.f90:1201-1226,
     // physics_euler.f90:1849-1885)
     if (g.gravity) {
-      const T ax = rg<T>(g, R_ACCX, i), ay = rg<T>(g, R_ACCY, i);
+      const T *acc = sizeof(T) == 4 ? reinterpret_cast<const T *>(a.accf) : reinterpret_cast<const T *>(a.acc);
+      const T ax = acc[(unsigned)i], ay = acc[(unsigned)(g.NI + i)];
       // FP32 curvilinear: already part of the FP64 radial balance above
       if (sizeof(T) == 8 || !g.curvilinear) ext[1] = zero + ur[0] * ax;
       ext[2] = zero + ur[0] * ay;

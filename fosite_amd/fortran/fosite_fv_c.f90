@@ -27,6 +27,8 @@ MODULE fosite_fv_c
     INTEGER(c_int) :: fargo = 0, gravity = 0
     REAL(c_double) :: mass = 1, GN = 1
     INTEGER(c_int) :: dtype = 0
+    INTEGER(c_int) :: outbound = 0
+    REAL(c_double) :: acclimit = -1
   END TYPE fv_config
 
   TYPE, BIND(C), PUBLIC :: fv_scalars
@@ -38,7 +40,7 @@ MODULE fosite_fv_c
        fv_set_background_velocity, fv_set_pvar, fv_first_step, fv_step, fv_run, &
        fv_get_state, fv_get_scalars, fv_last_run_ms, fv_last_error, fv_profile, &
        fv_kernel_stats, fv_get_background_velocity, fv_get_boundary_flux, &
-       fv_get_boundary_flux_total, c_string
+       fv_get_boundary_flux_total, fv_get_point_mass, c_string
 
   INTERFACE
     INTEGER(c_int) FUNCTION fv_mesh_array(cfg, id, out) BIND(C, NAME="fv_mesh_array")
@@ -117,6 +119,11 @@ MODULE fosite_fv_c
       INTEGER(c_int), VALUE :: direction
       REAL(c_double), INTENT(OUT) :: bflux(*)
     END FUNCTION fv_get_boundary_flux_total
+    INTEGER(c_int) FUNCTION fv_get_point_mass(ctx, pm) BIND(C, NAME="fv_get_point_mass")
+      IMPORT :: c_int, c_ptr, c_double
+      TYPE(c_ptr), VALUE :: ctx
+      REAL(c_double), INTENT(OUT) :: pm(*)
+    END FUNCTION fv_get_point_mass
     REAL(c_double) FUNCTION fv_last_run_ms(ctx) BIND(C, NAME="fv_last_run_ms")
       IMPORT :: c_double, c_ptr
       TYPE(c_ptr), VALUE :: ctx

@@ -52,10 +52,14 @@ MODULE fosite_gpu_mod
                southern = NO_GRADIENTS, northern = NO_GRADIENTS
   END TYPE boundary_gpu
 
-  !> point mass at the origin, Newtonian, no accretion (outbound = 0)
+  !> point mass at the origin, Newtonian; accretion through boundary
+  !! outbound (1..4, 0 off) with limit acclimit (< 0 off)
+  !! (gravity_pointmass.f90:185-195, 285-299)
   TYPE, PUBLIC :: sources_gpu
     LOGICAL  :: pointmass = .FALSE.
     REAL(DP) :: mass = 1, GN = 1
+    INTEGER  :: outbound = 0
+    REAL(DP) :: acclimit = -1
   END TYPE sources_gpu
 
   TYPE, PUBLIC :: timedisc_gpu
@@ -89,6 +93,7 @@ MODULE fosite_gpu_mod
     PROCEDURE :: GetBackgroundVelocity
     PROCEDURE :: GetBoundaryFluxes
     PROCEDURE :: GetBoundaryFlux
+    PROCEDURE :: GetPointMass
     PROCEDURE :: MeshArray
     PROCEDURE :: LastRunMs
     PROCEDURE :: Finalize
@@ -126,6 +131,8 @@ CONTAINS
     c%gravity = MERGE(1, 0, this%Sources%pointmass)
     c%mass = this%Sources%mass
     c%GN = this%Sources%GN
+    c%outbound = this%Sources%outbound
+    c%acclimit = this%Sources%acclimit
     c%dtype = 0
   END FUNCTION Config
 
@@ -265,6 +272,13 @@ CONTAINS
     bflux(:) = 0.0_DP
     CALL Check(this, fv_get_boundary_flux_total(this%ctx, INT(direction, c_int), bflux), "GetBoundaryFlux")
   END FUNCTION GetBoundaryFlux
+
+  !> the point mass: mass, accrate, massloss, mdot (gravity_pointmass.f90:79-85)
+  FUNCTION GetPointMass(this) RESULT(pm)
+    CLASS(fosite_gpu), INTENT(INOUT) :: this
+    REAL(DP) :: pm(4)
+    CALL Check(this, fv_get_point_mass(this%ctx, pm), "GetPointMass")
+  END FUNCTION GetPointMass
 
   !> one metric array of the configured mesh (ids as fv_mesh_array)
   SUBROUTINE MeshArray(this, id, a)

@@ -28,13 +28,14 @@ PROGRAM fv_run
   REAL(DP) :: k_cfl = 0.4_DP, k_tolrel = 0.01_DP, k_tolabs(4) = 0.001_DP, k_stoptime = 1
   REAL(DP) :: k_dtlimit = -1, k_beta = 0.08_DP
   INTEGER  :: k_fargo = 0, k_gravity = 0, k_units = 0
-  REAL(DP) :: k_mass = 1
+  REAL(DP) :: k_mass = 1, k_acclimit = -1
+  INTEGER  :: k_outbound = 0
   INTEGER  :: k_nsteps = 10, k_dumpevery = 1, k_batch = 0
   CHARACTER(LEN=512) :: k_infile = "", k_prefix = "out"
   NAMELIST /fvcfg/ k_physics, k_geometry, k_inum, k_jnum, k_gamma, k_xmin, k_xmax, k_ymin, k_ymax, &
        k_gparam, k_bcw, k_bce, k_bcs, k_bcn, k_limiter, k_recvars, k_theta, k_method, k_order,   &
        k_cfl, k_tolrel, k_tolabs, k_stoptime, k_dtlimit, k_beta, k_fargo, k_gravity, k_units,   &
-       k_mass, k_nsteps, k_dumpevery, k_batch, k_infile, k_prefix
+       k_mass, k_outbound, k_acclimit, k_nsteps, k_dumpevery, k_batch, k_infile, k_prefix
 
   TYPE(fosite_gpu) :: Sim
   CHARACTER(LEN=512) :: cfgname
@@ -67,6 +68,8 @@ PROGRAM fv_run
   Sim%Boundary%northern = k_bcn
   Sim%Sources%pointmass = k_gravity.GT.0
   Sim%Sources%mass = k_mass
+  Sim%Sources%outbound = k_outbound
+  Sim%Sources%acclimit = k_acclimit
   Sim%Timedisc%method = k_method
   Sim%Timedisc%order = k_order
   Sim%Timedisc%cfl = k_cfl
@@ -143,6 +146,8 @@ CONTAINS
     WRITE(ob) bx, by
     WRITE(ob) Sim%GetBoundaryFlux(1), Sim%GetBoundaryFlux(2), Sim%GetBoundaryFlux(3), &
               Sim%GetBoundaryFlux(4)
+    ! point mass: mass, accrate, massloss, mdot
+    IF (k_gravity.GT.0) WRITE(ob) Sim%GetPointMass()
   END SUBROUTINE Dump
 
 END PROGRAM fv_run

@@ -35,7 +35,10 @@ GNUM = 2
 
 DEFAULTS = dict(gamma=1.4, gparam=1.0, limiter=MINMOD, theta=1.0, recvars=CONSERVATIVE,
                 method=SSPRK, order=5, cfl=0.4, tol_rel=0.01, tol_abs=[0.001] * 4,
-                dtlimit=-1.0, beta=0.08, fargo=0, gravity=0, mass=1.0, GN=1.0)
+                dtlimit=-1.0, beta=0.08, fargo=0, gravity=0, mass=1.0, GN=1.0,
+                # point-mass accretion off unless named: the reference's default
+                # on cylindrical meshes is WEST (gravity_pointmass.f90:285-299)
+                outbound=0, acclimit=-1.0)
 
 
 def full(cfg):

@@ -44,13 +44,19 @@ typedef struct fv_config {
   int fargo;        /* 0 off, 1 per-ring mean azimuthal velocity recomputed every step
                      (timedisc/timedisc_base.f90:2243-2326; host-controlled steps),
                      2 user-supplied background velocity (mesh/mesh_base.f90:372-423) */
-  int gravity;      /* 1: point mass at the origin, Newtonian, no accretion
-                       (sources/gravity_pointmass.f90:132-301 with outbound = 0) */
+  int gravity;      /* 1: point mass at the origin, Newtonian
+                       (sources/gravity_pointmass.f90:132-301) */
   double mass, GN;  /* point mass and gravitational constant (GEOMETRICAL units: GN = 1) */
   int dtype;        /* 0 = FP64, the plain drop-in (bit-exact with the reference where no
                        libm call is on the path); 1 = FP32 precision form: FP32 state and
                        arithmetic, compensated RK accumulation, FP64 error norm and CFL
                        reduction (no reference counterpart) */
+  int outbound;     /* point-mass accretion through boundary 1 WEST, 2 EAST, 3 SOUTH, 4 NORTH,
+                       0 off (sources/gravity_pointmass.f90:285-299, 383-428).  Zero-initialised
+                       means off; the reference's own default on cylindrical meshes is WEST,
+                       so a setup that relies on it sets 1 here. */
+  double acclimit;  /* accretion limit per unit mass and time, < 0 off
+                       (sources/gravity_pointmass.f90:190, 412-416) */
 } fv_config;
 
 typedef struct fv_scalars {
@@ -126,6 +132,12 @@ int fv_get_boundary_flux(fv_ctx *ctx, double *bx, double *by);
  * direction 1 WEST, 2 EAST, 3 SOUTH, 4 NORTH (mesh/mesh_base.f90:102-105);
  * bflux[VNUM] = SUM over the boundary's interior range, in index order. */
 int fv_get_boundary_flux_total(fv_ctx *ctx, int direction, double *bflux);
+/* The point mass (sources/gravity_pointmass.f90:79-85, output attributes
+ * 192-195): out[0] mass, out[1] accrate, out[2] massloss, out[3] mdot (the
+ * GetBoundaryFlux density sum its last update saw).  With accretion the
+ * values follow UpdateGravity_single (gravity_pointmass.f90:383-428), run at
+ * every ComputeRHS; without it they stay at (mass, 0, 0, 0). */
+int fv_get_point_mass(fv_ctx *ctx, double *out);
 /* wall time of the device work issued by the last fv_run, in ms
  * (HIP events on the context's stream) */
 double fv_last_run_ms(const fv_ctx *ctx);

@@ -53,10 +53,13 @@ def run(cfg, nsteps, tmp, batch=0, dump_every=1, timeout=300):
         # boundary fluxes of every record (GetBoundaryFluxes, GetBoundaryFlux x4)
         if os.path.exists(bout):
             braw = np.fromfile(bout, dtype="<f8")
-            brec = 2 * V * NJ + 2 * V * NI + 4 * 4
+            grav = 4 if cfg.get("gravity", 0) else 0  # GetPointMass after the totals
+            brec = 2 * V * NJ + 2 * V * NI + 4 * 4 + grav
             for n, d in enumerate(recs):
                 b = braw[n * brec:(n + 1) * brec]
                 d["bx"] = b[:2 * V * NJ].reshape(V, 2, NJ)
                 d["by"] = b[2 * V * NJ:2 * V * (NJ + NI)].reshape(V, 2, NI)
-                d["bsum"] = b[2 * V * (NJ + NI):].reshape(4, 4)[:, :V]
+                d["bsum"] = b[2 * V * (NJ + NI):2 * V * (NJ + NI) + 16].reshape(4, 4)[:, :V]
+                if grav:
+                    d["pointmass"] = b[-4:].copy()
     return r, recs
