@@ -247,7 +247,7 @@ class Fosite:
     def profile(self, enable=True):
         self._chk(self.L.fv_profile(self._h, int(enable)), "fv_profile")
 
-    KERNELS = {"rhs": 0, "rhs_end": 1, "rhs_combine": 2, "frame": 3, "control": 4, "remap": 5}
+    KERNELS = {"rhs": 0, "rhs_end": 1, "rhs_combine": 2, "frame": 3, "control": 4, "remap": 5, "accrete": 6}
 
     def kernel_stats(self, kernel):
         n, ms, b = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()

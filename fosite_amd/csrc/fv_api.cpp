@@ -29,7 +29,7 @@ namespace {
 constexpr int NBUF = 4;
 // kernel classes of the timing statistics (fv_kernel_stats)
 constexpr int K_RHS = 0, K_RHS_END = 1, K_RHS_COMBINE = 2, K_FRAME = 3, K_CONTROL = 4, K_REMAP = 5,
-              K_CLASSES = 6;
+              K_ACCRETE = 6, K_CLASSES = 7;
 // timedisc_base.f90:193-203
 constexpr int CHECK_NOTHING = 0, CHECK_ALL = ~0, CHECK_CSOUND = 1, CHECK_PMIN = 2;
 constexpr int DTCAUSE_ERRADJ = -1, DTCAUSE_SMALLERR = -2, DTCAUSE_FILEIO = -4;
@@ -660,9 +660,11 @@ void enqueue_fast_step(fv_ctx *ctx, int xin, int xout, int tb, int rb, int k0in,
     launch_remap(ctx->g, f.rm, ctx->V, ctx->st);
     prof_end(ctx, pk, K_REMAP, 2.0 * word_bytes(ctx));
   }
-  if (f.accrete) {
+  if (f.accrete) {  // the step's boundary fluxes, then UpdateGravity
+    long pk = prof_begin(ctx);
     launch_bflux(f.bfl, ctx->st);
     launch_accrete(f.acc, ctx->st);
+    prof_end(ctx, pk, K_ACCRETE, 0.0);
   }
   long pk = prof_begin(ctx);
   launch_rhs(ctx->g, f.end, ctx->V, ctx->st);

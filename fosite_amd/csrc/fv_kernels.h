@@ -596,39 +596,115 @@ static __device__ void control_next(Ctl *c, double invdt, int flags) {
 }
 
 // ------------------------------------------------ boundary-flux accounting
-// bxflux(j,1:2,l) and byflux(i,1:2,l) of one step (timedisc_rkfehlberg.f90:
-// 216-259): threads t, t+nt, ... of the caller each take one boundary row
+// Element e of the boundary rows (west/east rows of every field first, then
+// south/north): its new bxflux(j,side,l) / byflux(i,side,l) after one step
+// (timedisc_rkfehlberg.f90:216-259), and in *bi its index in the layout.
+__device__ __forceinline__ double bflux_elem(const BfluxArgs &b, double dt, int e, long *bi) {
+  const int nx = b.V * 2 * b.jnum;
+  int l, side, gx;
+  if (e < nx) {  // west / east: ghost column IMIN-1 / IMAX+1 of row j
+    l = e / (2 * b.jnum);
+    side = (e / b.jnum) & 1;
+    const int j = GH + e % b.jnum;
+    *bi = (long)(l * 2 + side) * b.NJ + j;
+    gx = j * b.P + (side ? GH + b.inum : GH - 1);
+  } else {       // south / north: ghost row JMIN-1 / JMAX+1 of column i
+    const int f = e - nx;
+    l = f / (2 * b.inum);
+    side = (f / b.inum) & 1;
+    const int i = GH + f % b.inum;
+    *bi = 2L * b.V * b.NJ + (long)(l * 2 + side) * b.NI + i;
+    gx = (side ? GH + b.jnum : GH - 1) * b.P + i;
+  }
+  double kq[MAXK];
+  const long kx = (long)l * b.FS + gx;
+#pragma unroll
+  for (int q = 0; q < MAXK; q++)  // all loads first
+    if (q < b.nk)
+      kq[q] = b.dtype == 1 ? (double)static_cast<const float *>(b.k[q])[kx] : static_cast<const double *>(b.k[q])[kx];
+  double acc = b.in[*bi];
+  const double ks = b.ks[*bi];
+#pragma unroll
+  for (int q = 0; q < MAXK; q++)
+    if (q < b.nk) acc = acc - (dt * b.bh[q]) * kq[q];
+  return acc - (dt * b.bh[b.nk]) * ks;
+}
+// the whole accumulation: threads t, t+nt, ... of the caller each take one
+// boundary row element
 __device__ __forceinline__ void bflux_accumulate(const BfluxArgs &b, double dt, int t, int nt) {
-  const int nx = b.V * 2 * b.jnum, n = nx + b.V * 2 * b.inum;
+  const int n = b.V * 2 * (b.jnum + b.inum);
   for (int e = t; e < n; e += nt) {
-    int l, side, gx;
     long bi;
-    if (e < nx) {  // west / east: ghost column IMIN-1 / IMAX+1 of row j
-      l = e / (2 * b.jnum);
-      side = (e / b.jnum) & 1;
-      const int j = GH + e % b.jnum;
-      bi = (long)(l * 2 + side) * b.NJ + j;
-      gx = j * b.P + (side ? GH + b.inum : GH - 1);
-    } else {       // south / north: ghost row JMIN-1 / JMAX+1 of column i
-      const int f = e - nx;
-      l = f / (2 * b.inum);
-      side = (f / b.inum) & 1;
-      const int i = GH + f % b.inum;
-      bi = 2L * b.V * b.NJ + (long)(l * 2 + side) * b.NI + i;
-      gx = (side ? GH + b.jnum : GH - 1) * b.P + i;
-    }
-    double kq[MAXK];
-    const long kx = (long)l * b.FS + gx;
+    const double v = bflux_elem(b, dt, e, &bi);
+    b.out[bi] = v;
+  }
+}
+
+// ----------------------------------------------------- point-mass accretion
+// sum0 + term(0) + ... + term(n-1) in index order: one FP64 chain.  Full
+// chunks of PF terms are loaded before their adds (the adds then wait on the
+// loads in issue order, not on each load), with no per-term bounds test;
+// the remainder follows term by term.  n must be wave-uniform.
+template <int PF, class F>
+__device__ __forceinline__ double ordered_sum(double sum, int n, F term) {
+  int k = 0;
+  for (; k + PF <= n; k += PF) {
+    double t[PF];
 #pragma unroll
-    for (int q = 0; q < MAXK; q++)  // all loads first
-      if (q < b.nk)
-        kq[q] = b.dtype == 1 ? (double)static_cast<const float *>(b.k[q])[kx] : static_cast<const double *>(b.k[q])[kx];
-    double acc = b.in[bi];
-    const double ks = b.ks[bi];
+    for (int q = 0; q < PF; q++) t[q] = term(k + q);
 #pragma unroll
-    for (int q = 0; q < MAXK; q++)
-      if (q < b.nk) acc = acc - (dt * b.bh[q]) * kq[q];
-    b.out[bi] = acc - (dt * b.bh[b.nk]) * ks;
+    for (int q = 0; q < PF; q++) sum = sum + t[q];
+  }
+  for (; k < n; k++) sum = sum + term(k);
+  return sum;
+}
+
+// SUM(row(0:n-1)) in index order, the density sum of GetBoundaryFlux
+// (fluxes_base.f90:220-300), by all 256 threads of a workgroup: they stage
+// the row in LDS (2048 values at a time, one memory latency for the meshes
+// in use), then wave 0 adds it, every lane on the same broadcast values.
+// The result is valid in wave 0.
+constexpr int ACC_CHUNK = 2048;
+__device__ __forceinline__ double ordered_row_sum(const double *row, int n, int tid) {
+  __shared__ double srow[ACC_CHUNK];
+  double sum = 0.0;
+  for (int c0 = 0; c0 < n; c0 += ACC_CHUNK) {
+    const int m = min(ACC_CHUNK, n - c0);
+    for (int k = tid; k < m; k += 256) srow[k] = row[c0 + k];
+    __syncthreads();
+    if (tid < 64) sum = ordered_sum<16>(sum, __builtin_amdgcn_readfirstlane(m), [&](int k) { return srow[k]; });
+    __syncthreads();
+  }
+  return sum;
+}
+
+// UpdateGravity_single with accretion (gravity_pointmass.f90:383-428; no
+// switch-on phase) given the boundary's density sum: header of gin -> gout,
+// acceleration rings rescaled by new/old mass (lanes lane, lane+nl, ...)
+__device__ __forceinline__ void accrete_update(const AccreteArgs &p, double sum, int lane, int nl) {
+  const double oldmass = p.gin[PM_MASS];
+  const double dt = p.dt_ctl ? p.ctl->dt : p.dt;
+  double accrate = p.zero_accrate ? 0.0 : p.gin[PM_ACCRATE], massloss = p.gin[PM_MASSLOSS];
+  double dmass = sum - p.gin[PM_MDOT];
+  if (p.acclimit >= 0.) {
+    const double lim = p.acclimit * dt * oldmass;
+    const double dmasslim = (lim < dmass) ? lim : dmass;  // MIN(dmass, lim)
+    massloss = massloss + dmass - dmasslim;
+    dmass = dmasslim;
+  }
+  const double mass = oldmass + dmass;
+  if (dt > 0.0) accrate = dmass / dt;
+  const double f = mass / oldmass;
+  if (lane == 0) {
+    p.gout[PM_MASS] = mass;
+    p.gout[PM_ACCRATE] = accrate;
+    p.gout[PM_MASSLOSS] = massloss;
+    p.gout[PM_MDOT] = sum;
+  }
+  for (int k = lane; k < 2 * p.NI; k += nl) {
+    const double x = p.gin[PM_HDR + k] * f;
+    p.gout[PM_HDR + k] = x;
+    if (p.gfout) p.gfout[k] = (float)x;
   }
 }
 

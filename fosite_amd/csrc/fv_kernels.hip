@@ -54,53 +54,15 @@ __global__ __launch_bounds__(256) void k_bflux(BfluxArgs bf) {
   bflux_accumulate(bf, bf.ctl ? bf.ctl->dt : bf.dt, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 }
 
-// UpdateGravity_single with accretion (gravity_pointmass.f90:383-428; no
-// switch-on phase), one workgroup: the density sum of GetBoundaryFlux over
-// the outflow boundary in index order (fluxes_base.f90:220-300), the mass
-// update, and the acceleration rings rescaled by new/old mass, gin -> gout.
-// The row is staged through LDS in chunks so that one lane can add it in
-// order at LDS latency.
-constexpr int ACC_CHUNK = 2048;
+// UpdateGravity_single of one ComputeRHS (gravity_pointmass.f90:383-428; no
+// switch-on phase), one workgroup: GetBoundaryFlux's density sum of the
+// outflow boundary in index order, then the point-mass update.  A strictly
+// ordered chain of FP64 adds: about 12 us for 1024 terms on MI355X.
 __global__ __launch_bounds__(256) void k_accrete(AccreteArgs p) {
   if (p.ctl && p.ctl->halt) return;
-  __shared__ double row[ACC_CHUNK];
-  __shared__ double fac;
-  const int t = threadIdx.x;
-  double sum = 0.0;
-  for (int c0 = p.lo; c0 <= p.hi; c0 += ACC_CHUNK) {
-    const int n = min(ACC_CHUNK, p.hi - c0 + 1);
-    for (int k = t; k < n; k += 256) row[k] = p.bf[c0 + k];
-    __syncthreads();
-    if (t == 0)
-      for (int k = 0; k < n; k++) sum = sum + row[k];
-    __syncthreads();
-  }
-  if (t == 0) {
-    const double oldmass = p.gin[PM_MASS];
-    const double dt = p.dt_ctl ? p.ctl->dt : p.dt;
-    double accrate = p.zero_accrate ? 0.0 : p.gin[PM_ACCRATE], massloss = p.gin[PM_MASSLOSS];
-    double dmass = sum - p.gin[PM_MDOT];
-    if (p.acclimit >= 0.) {
-      const double lim = p.acclimit * dt * oldmass;
-      const double dmasslim = (lim < dmass) ? lim : dmass;  // MIN(dmass, lim)
-      massloss = massloss + dmass - dmasslim;
-      dmass = dmasslim;
-    }
-    const double mass = oldmass + dmass;
-    if (dt > 0.0) accrate = dmass / dt;
-    p.gout[PM_MASS] = mass;
-    p.gout[PM_ACCRATE] = accrate;
-    p.gout[PM_MASSLOSS] = massloss;
-    p.gout[PM_MDOT] = sum;
-    fac = mass / oldmass;
-  }
-  __syncthreads();
-  const double f = fac;
-  for (int k = t; k < 2 * p.NI; k += 256) {
-    const double v = p.gin[PM_HDR + k] * f;
-    p.gout[PM_HDR + k] = v;
-    if (p.gfout) p.gfout[k] = (float)v;
-  }
+  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+  const double sum = ordered_row_sum(p.bf + p.lo, p.hi - p.lo + 1, tid);
+  if (tid < 64) accrete_update(p, sum, tid, 64);
 }
 // one block per mesh of a batch
 __global__ __launch_bounds__(256) void k_control_multi(const ControlEntry *__restrict__ tab) {
@@ -228,7 +190,7 @@ void launch_bflux(const BfluxArgs &bf, hipStream_t s) {
   hipLaunchKernelGGL(k_bflux, dim3((n + 255) / 256), dim3(256), 0, s, bf);
 }
 void launch_accrete(const AccreteArgs &p, hipStream_t s) {
-  hipLaunchKernelGGL(k_accrete, dim3(1), dim3(256), 0, s, p);
+  hipLaunchKernelGGL(k_accrete, dim3(1), dim3(64, 4), 0, s, p);
 }
 void launch_reduce(const double *part, int n, double *out, hipStream_t s) {
   hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, s, part, n, out);

@@ -150,7 +150,9 @@ const char *fv_last_error(const fv_ctx *ctx);
  * state, 1 end-of-step RHS (CFL, data checks, next step's first slope),
  * 2 last RK stage fused with the final combination and embedded error,
  * 3 FARGO frame change, 4 step control of the device-controlled batches (one
- * block), 5 FARGO azimuthal remap.  fv_kernel_stats
+ * block), 5 FARGO azimuthal remap, 6 accretion update of a device-controlled
+ * step (boundary-flux accumulation and UpdateGravity, two launches; no
+ * algorithmic bytes counted).  fv_kernel_stats
  * returns the number of launches, their summed device time in ms and their
  * algorithmic bytes (state words each launch must read and write, 8 bytes
  * per word, interior cells). */
