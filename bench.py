@@ -238,6 +238,36 @@ def serving_form(n=300, rate=20.0, nmin=64, nmax=256, orbits=10.0, budget_gb=64.
             "arrival_span_s": reqs[-1][0] - reqs[0][0]}
 
 
+def accretion_variant(steps=300):
+    """The headline disk with point-mass accretion through the western
+    boundary (outbound = 1): device-controlled steps with the accretion update
+    (boundary-flux accumulation, the ordered density sum, rescaled gravity)
+    before every end-of-step evaluation.  ms per step with and without it, and
+    the update's launch time from the library's HIP-event profiling."""
+    import fosite_amd
+    from fosite_amd import setups
+    out = {"workload": "isothermal Keplerian disk, logcylindrical 512x1024, FP64, SSPRK3, FARGO; "
+                       "accretion through the western boundary"}
+    for name, extra in (("no_accretion", {}), ("accretion", {"outbound": 1})):
+        cfg = dict(setups.kepler_disk_config(ni=512, nj=1024), **extra)
+        init = setups.initial(cfg)
+        sim = fosite_amd.Fosite(cfg)
+        sim.set_initial(init["pvar"], init["bccsound"], init["fcsound"], init["w"])
+        sim.first_step()
+        sim.run(20)
+        taken = sim.run(steps)
+        out[name + "_ms_per_step"] = sim.last_run_ms() / max(taken, 1)
+        if extra:
+            sim.profile(True)
+            sim.run(50)
+            n, ms, _ = sim.kernel_stats("accrete")
+            sim.profile(False)
+            out["accretion_update_us"] = 1e3 * ms / max(n, 1)
+            out["point_mass"] = [float(x) for x in sim.point_mass()]
+        sim.close()
+    return out
+
+
 def serving_form_full(n=40):
     """The serving stream at the stated grid sizes, radial cells U[64,1024]
     (azimuthal 2x), 20 requests/s: the offered load (all cell updates of the
@@ -263,7 +293,7 @@ def main():
     ap.add_argument("--no-forms", action="store_true",
                     help="skip the precision, batched and serving forms")
     ap.add_argument("--form-steps", type=int, default=500)
-    ap.add_argument("--only-form", choices=["precision", "batched", "serving", "serving_full"],
+    ap.add_argument("--only-form", choices=["precision", "batched", "serving", "serving_full", "accretion"],
                     help="run one form alone and print its object (for profiling)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -272,7 +302,8 @@ def main():
         return 2
     if a.only_form:
         f = {"precision": lambda: precision_form(1024, 2048, a.form_steps), "batched": batched_form,
-             "serving": serving_form, "serving_full": serving_form_full}[a.only_form]
+             "serving": serving_form, "serving_full": serving_form_full,
+             "accretion": accretion_variant}[a.only_form]
         print(json.dumps(f()))
         return 0
 
@@ -344,6 +375,7 @@ def main():
         out["batched_form"] = batched_form()
         out["serving_form"] = serving_form()
         out["serving_form_full_sizes"] = serving_form_full()
+        out["accretion_variant"] = accretion_variant()
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg, init, a.cpu_steps)
     print(json.dumps(out))

@@ -150,3 +150,8 @@ def test_gpu_point_mass_without_accretion_and_batches_refuse_accretion():
     with pytest.raises(RuntimeError, match="point mass"):
         nograv.point_mass()
     nograv.close()
+    srv = fosite_amd.Server(1e9)
+    acfg = dict(cfg, outbound=1)
+    with pytest.raises(RuntimeError, match="accretion"):
+        srv.submit(acfg, S.initial(acfg))
+    srv.close()
