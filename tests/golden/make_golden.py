@@ -59,6 +59,14 @@ CASES = {
     "disk_accretion_cfl_0.9_rejects": (dict(D, outbound=1, cfl=0.9), 12, (12,)),
     "disk_accretion_east_no_fargo": (dict(D, outbound=2, fargo=0), 8, (8,)),
     "euler_disk_accretion": (dict(S.euler_disk_config(ni=16, nj=32), outbound=1), 8, (8,)),
+    # PRIMITIVE reconstruction (fluxes_base.f90:378-381)
+    "euler_prim_vanleer": (dict(E, recvars=S.PRIMITIVE), 10, (1, 10)),
+    "euler_prim_minmod_dp5": (dict(E, recvars=S.PRIMITIVE, limiter=S.MINMOD, method=S.DORMAND_PRINCE, order=5,
+                                   tol_rel=1e-4), 8, (8,)),
+    "disk_prim": (dict(D, recvars=S.PRIMITIVE), 10, (10,)),
+    "disk_prim_nograd_fargo1": (dict(D, recvars=S.PRIMITIVE, fargo=1,
+                                     bc=[S.NO_GRADIENTS, S.NO_GRADIENTS, S.PERIODIC, S.PERIODIC]), 8, (8,)),
+    "euler_disk_prim": (dict(S.euler_disk_config(ni=16, nj=32), recvars=S.PRIMITIVE), 8, (8,)),
 }
 MESHES = {"cartesian_16": E, "logcyl_16x32": D}
 SCALARS = ("iter", "n_adj", "brk", "time", "dt", "dtold", "maxerrold")
