@@ -802,7 +802,7 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
   if (c.physics != EULER_ISOTHERM && c.physics != EULER) return bail("physics must be EULER_ISOTHERM (16) or EULER (17)");
   if (c.geometry != CARTESIAN && c.geometry != LOGCYLINDRICAL) return bail("geometry must be CARTESIAN (1) or LOGCYLINDRICAL (31)");
   if (c.inum < 4 || c.jnum < 4) return bail("inum and jnum must be >= 4 (2-D meshes only)");
-  if (c.recvars != 2) return bail("only CONSERVATIVE reconstruction (variables = 2) is supported");
+  if (c.recvars != 1 && c.recvars != 2) return bail("reconstruction variables must be 1 PRIMITIVE or 2 CONSERVATIVE");
   if (c.dtype != 0 && c.dtype != 1) return bail("dtype must be 0 (FP64) or 1 (FP32 precision form)");
   if (c.fargo < 0 || c.fargo > 2) return bail("fargo must be 0, 1 or 2");
   if (c.fargo && (c.physics != EULER_ISOTHERM || c.geometry != LOGCYLINDRICAL))
@@ -873,6 +873,7 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
   g.invdx = ctx->mesh.invdx; g.invdy = ctx->mesh.invdy;
   g.gamma = c.gamma; g.pmin = DBL_MIN; g.rhomin = DBL_MIN;
   g.limiter = c.limiter; g.theta = c.theta;
+  g.recon_prim = c.recvars == 1;
   for (int d = 0; d < 4; d++) g.bc[d] = c.bc[d];
   g.bdydz = ctx->mesh.dy * ctx->mesh.dz;
   g.bdzdx = ctx->mesh.dz * ctx->mesh.dx;
@@ -1300,6 +1301,10 @@ struct fv_batch {
 
 namespace {
 
+// limiter a batch launch compiles in for this member: 0 (the run-time
+// switch) for PRIMITIVE reconstruction, which only that instantiation carries
+int launch_limiter(const fv_ctx *x) { return x->c.recvars == 1 ? 0 : x->c.limiter; }
+
 int bfail(fv_batch *b, const std::string &m, int code = -1) {
   b->err = m;
   return code;
@@ -1545,12 +1550,12 @@ extern "C" int fv_batch_add(fv_batch *b, fv_ctx *x) {
     b->V = x->V;
     b->dtype = x->c.dtype;
     b->m = x->tab.m;
-    b->limiter = x->c.limiter;
+    b->limiter = launch_limiter(x);
   } else {
     if (x->V != b->V || x->c.dtype != b->dtype || x->tab.m != b->m ||
         x->c.method != b->sims[0]->c.method || x->c.order != b->sims[0]->c.order)
       return bfail(b, "batch members must agree in physics, dtype and time integrator");
-    if (x->c.limiter != b->limiter) b->limiter = 0;  // run-time limiter switch
+    if (launch_limiter(x) != b->limiter) b->limiter = 0;  // run-time limiter switch
   }
   b->sims.push_back(x);
   x->batch = b;
@@ -1567,9 +1572,9 @@ extern "C" int fv_batch_remove(fv_batch *b, fv_ctx *x) {
   x->batch = nullptr;
   b->dirty = true;
   if (!b->sims.empty()) {
-    b->limiter = b->sims[0]->c.limiter;
+    b->limiter = launch_limiter(b->sims[0]);
     for (auto *y : b->sims)
-      if (y->c.limiter != b->limiter) b->limiter = 0;
+      if (launch_limiter(y) != b->limiter) b->limiter = 0;
   }
   return 0;
 }

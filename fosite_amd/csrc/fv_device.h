@@ -32,6 +32,7 @@ struct Geom {
   double invdx, invdy, gamma, pmin, rhomin;
   int limiter;
   double theta;
+  int recon_prim;       // 1: PRIMITIVE reconstruction (reconstruction_base.f90:117, fluxes_base.f90:378-381)
   int bc[4];
   double bdydz, bdzdx;  // Mesh%dy*Mesh%dz, Mesh%dz*Mesh%dx: boundary-flux weights
   int fargo, gravity, curvilinear;

@@ -326,6 +326,21 @@ struct In {
     }
   }
 
+  // PRIMITIVE reconstruction: the pvar handed to the reconstruction, after
+  // SubtractBackgroundVelocityY (physics_eulerisotherm.f90:1373-1413) of the
+  // real-frame pvar p (as transform returns it); s is the stage state
+  __forceinline__ __device__ void prim_comoving(int i, const T *s, const T *p, T *po) const {
+#pragma unroll
+    for (int l = 0; l < V; l++) po[l] = p[l];
+    if constexpr (sizeof(T) == 4) {
+      // FP32: (v' + w) - w would keep a few bits of the residual velocity v';
+      // the stored residual is what the subtraction means
+      if (a.sub_bg) po[2] = a.add_bg ? dv(s[2], s[0]) : p[2] - rg<T>(g, R_W, i);
+    } else {
+      if (a.sub_bg) po[2] = p[2] - rg<T>(g, R_W, i);
+    }
+  }
+
   // pvar of an interior cell right after CenterBoundary's conversion
   __forceinline__ __device__ void prim_interior(int i, int j, T *p) const {
     T s[V];
@@ -508,13 +523,12 @@ __device__ __forceinline__ T slope(const Geom &g, T um, T u0, T up) {
 // KT flux across one interface from the two face states (fluxes_kt.f90:126-176,
 // physics_eulerisotherm.f90:555-686, 835-930, 2742-2817; physics_euler.f90:492-600)
 // d = 0: x-interface (normal momentum index 1), d = 1: y-interface (index 2).
+// kt_core takes both representations of the two face states.
 template <typename T, int V>
-__device__ __forceinline__ void kt_flux(const Geom &g, int d, const T *UL, const T *UR, T csL, T csR, T dA, T *F) {
+__device__ __forceinline__ void kt_core(const Geom &g, int d, const T *UL, const T *UR, const T *pl, const T *pr, T csL,
+                                        T csR, T dA, T *F) {
   const int un = 1 + d, ut = 2 - d;
   const T gam = (T)g.gamma, zero = T(0);
-  T pl[V], pr[V];
-  c2p<T, V>(UL, pl, gam);
-  c2p<T, V>(UR, pr, gam);
   if (V == 4) {
     csL = sound_speed(gam, pl[0], pl[3]);
     csR = sound_speed(gam, pr[0], pr[3]);
@@ -539,6 +553,26 @@ __device__ __forceinline__ void kt_flux(const Geom &g, int d, const T *UL, const
   const T fac = dv(dA, amax - amin);
 #pragma unroll
   for (int l = 0; l < V; l++) F[l] = fac * (amax * fL[l] - amin * fR[l] + amin * amax * (UR[l] - UL[l]));
+}
+// conservative reconstruction: the face states are conservative, the
+// primitive ones follow (Physics%Convert2Primitive, fluxes_base.f90:382-383)
+template <typename T, int V>
+__device__ __forceinline__ void kt_flux(const Geom &g, int d, const T *UL, const T *UR, T csL, T csR, T dA, T *F) {
+  const T gam = (T)g.gamma;
+  T pl[V], pr[V];
+  c2p<T, V>(UL, pl, gam);
+  c2p<T, V>(UR, pr, gam);
+  kt_core<T, V>(g, d, UL, UR, pl, pr, csL, csR, dA, F);
+}
+// PRIMITIVE reconstruction: the face states are primitive, the conservative
+// ones follow (Physics%Convert2Conservative, fluxes_base.f90:378-381)
+template <typename T, int V>
+__device__ __forceinline__ void kt_flux_p(const Geom &g, int d, const T *PL, const T *PR, T csL, T csR, T dA, T *F) {
+  const T gam = (T)g.gamma;
+  T ul[V], ur[V];
+  p2c<T, V>(PL, ul, gam);
+  p2c<T, V>(PR, ur, gam);
+  kt_core<T, V>(g, d, ul, ur, PL, PR, csL, csR, dA, F);
 }
 
 // ------------------------------------------------------------ step control
@@ -791,6 +825,9 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
   auto FLY = [&](int l, int r, int c) -> T & { return lds[FX + (l * (TY + 1) + r) * TX + c]; };
 
   if (a.ctl && a.ctl->halt) return;
+  // PRIMITIVE reconstruction: the footprint and the face states hold pvar
+  // (only the run-time-limiter instantiation carries this path)
+  const bool prim_rec = LIM == 0 && g.recon_prim;
   const In<T, V, KE> in(g, a, want_check);
   const int tx = threadIdx.x, ty = threadIdx.y, t = tx + TX * ty;
   const int i0 = GH + bx * TX, j0 = GH + by * TY;
@@ -831,9 +868,8 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
       if (!(p ? e_g : own_g)) continue;
       const int ii = p ? ei : i, jj = p ? ej : j;
       const bool xin = ii >= imin && ii <= imax, yin = jj >= jmin && jj <= jmax;
-      T u[V];
+      T u[V], pp[V];
       if ((xin || yin) && ii >= 0 && jj >= 0 && ii < g.NI && jj < g.NJ) {
-        T pp[V];
         in.ghost_final(ii, jj, u, pp, gf);
         if (want_cfl) {
           const T cs = (V == 4) ? sound_speed(in.gam, pp[0], pp[3]) : rg<T>(g, R_CS, ii);
@@ -850,10 +886,10 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
         }
       } else {
 #pragma unroll
-        for (int l = 0; l < V; l++) u[l] = zero;  // corner or beyond the mesh: never needed
+        for (int l = 0; l < V; l++) u[l] = pp[l] = zero;  // corner or beyond the mesh: never needed
       }
 #pragma unroll
-      for (int l = 0; l < V; l++) UC(l, p ? er : ty, p ? ec : tx) = u[l];
+      for (int l = 0; l < V; l++) UC(l, p ? er : ty, p ? ec : tx) = prim_rec ? pp[l] : u[l];
     }
     gcfl[t] = ginv;
     gflg[t] = gf;
@@ -871,11 +907,13 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
         const T w = rg<T>(g, R_W, i);
         vout[(unsigned)j * (unsigned)g.P + (unsigned)i] = (double)((pr[2] - w) + w);
       }
+      if (prim_rec) in.prim_comoving(i, so, pr, uo);
 #pragma unroll
       for (int l = 0; l < V; l++) UC(l, ty, tx) = uo[l];
     }
     if (e_in) {
       in.transform(ei, se, re, pe, ue);
+      if (prim_rec) in.prim_comoving(ei, se, pe, ue);
 #pragma unroll
       for (int l = 0; l < V; l++) UC(l, er, ec) = ue[l];
     }
@@ -1000,6 +1038,12 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
   __syncthreads();
 
   // ---- phase 3: KT interface fluxes (fluxes_kt.f90:126-176)
+  auto kt = [&](int d, const T *L, const T *R, T csL, T csR, T dA, T *F) {
+    if (prim_rec)
+      kt_flux_p<T, V>(g, d, L, R, csL, csR, dA, F);
+    else
+      kt_flux<T, V>(g, d, L, R, csL, csR, dA, F);
+  };
   T xf[V], yf[V];
   {
     T r2[V], n2[V];
@@ -1008,8 +1052,8 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
       r2[l] = FW(l, ty, tx + 1);
       n2[l] = FS(l, ty + 1, tx);
     }
-    kt_flux<T, V>(g, 0, fe, r2, rg<T>(g, R_CSE, ir), rg<T>(g, R_CSW, ir + 1), rg<T>(g, R_DAXW, ir + 1), xf);
-    kt_flux<T, V>(g, 1, fn, n2, rg<T>(g, R_CSN, ir), rg<T>(g, R_CSS, ir), rg<T>(g, R_DAYS, ir), yf);
+    kt(0, fe, r2, rg<T>(g, R_CSE, ir), rg<T>(g, R_CSW, ir + 1), rg<T>(g, R_DAXW, ir + 1), xf);
+    kt(1, fn, n2, rg<T>(g, R_CSN, ir), rg<T>(g, R_CSS, ir), rg<T>(g, R_DAYS, ir), yf);
 #pragma unroll
     for (int l = 0; l < V; l++) {
       FLX(l, ty, tx + 1) = xf[l];
@@ -1025,7 +1069,7 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
       n1[l] = FN1(l, tx);
       s1[l] = FS(l, 0, tx);
     }
-    kt_flux<T, V>(g, 1, n1, s1, rg<T>(g, R_CSN, ir), rg<T>(g, R_CSS, ir), rg<T>(g, R_DAYS, ir), ef);
+    kt(1, n1, s1, rg<T>(g, R_CSN, ir), rg<T>(g, R_CSS, ir), rg<T>(g, R_DAYS, ir), ef);
 #pragma unroll
     for (int l = 0; l < V; l++) FLY(l, 0, tx) = ef[l];
   } else if (t >= 2 * TX && t < 2 * TX + TY) {
@@ -1036,7 +1080,7 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
       e1[l] = FE1(l, fr);
       w1[l] = FW(l, fr, 0);
     }
-    kt_flux<T, V>(g, 0, e1, w1, rg<T>(g, R_CSE, i0 - 1), rg<T>(g, R_CSW, i0), rg<T>(g, R_DAXW, i0), ef);
+    kt(0, e1, w1, rg<T>(g, R_CSE, i0 - 1), rg<T>(g, R_CSW, i0), rg<T>(g, R_DAXW, i0), ef);
 #pragma unroll
     for (int l = 0; l < V; l++) FLX(l, fr, 0) = ef[l];
   }

@@ -17,7 +17,10 @@ static void rhs_lim(const Geom &g, const StageArgsT<T> &a, hipStream_t s) {
 template <typename T, int V, int TX, int KM>
 static void rhs_v(const Geom &g, const StageArgsT<T> &a, hipStream_t s) {
   // compile-time limiters for the common choices, run-time switch otherwise
-  if (g.limiter == VANLEER)
+  // (and for PRIMITIVE reconstruction, which only that instantiation carries)
+  if (g.recon_prim)
+    rhs_lim<T, V, 0, TX, KM>(g, a, s);
+  else if (g.limiter == VANLEER)
     rhs_lim<T, V, VANLEER, TX, KM>(g, a, s);
   else if (g.limiter == MINMOD)
     rhs_lim<T, V, MINMOD, TX, KM>(g, a, s);

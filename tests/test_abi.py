@@ -43,7 +43,9 @@ def test_host_mesh_matches_reference_fixture(name):
 BAD = {
     "physics": dict(physics=99),
     "geometry": dict(geometry=3),
-    "primitive reconstruction": dict(recvars=S.PRIMITIVE),
+    "reconstruction variables": dict(recvars=3),
+    "outflow boundary": dict(outbound=5),
+    "accretion without a point mass": dict(outbound=1),
     "limiter": dict(limiter=6),
     "unpaired periodic": dict(bc=[S.PERIODIC, S.NO_GRADIENTS, S.NO_GRADIENTS, S.NO_GRADIENTS]),
     "absorbing in y": dict(bc=[S.NO_GRADIENTS, S.NO_GRADIENTS, S.ABSORBING, S.NO_GRADIENTS]),

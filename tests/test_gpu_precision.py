@@ -99,6 +99,7 @@ CASES = {
     "kepler_disk_ragged_no_fargo": (dict(S.kepler_disk_config(ni=70, nj=90), fargo=0), 40),
     "euler_disk_absorbing_dp5": (S.euler_disk_config(ni=48, nj=96), 40),
     "riemann2d_euler": (S.riemann2d_config(n=64), 40),
+    "kepler_disk_primitive_reconstruction": (dict(S.kepler_disk_config(ni=64, nj=128), recvars=S.PRIMITIVE), 60),
 }
 
 
