@@ -897,6 +897,7 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
 
   T ur[V], pr[V];
   T vres = zero;  // FP32: azimuthal velocity of the stored state (residual in the FARGO frame)
+  T ucons[V];     // PRIMITIVE reconstruction: the own cell's cvar for the new state (the footprint holds pvar)
   {
     T so[V], se[V], uo[V], ue[V], re[V], pe[V];
     in.interior2(out, i, j, so, e_in, ei, ej, se);
@@ -907,7 +908,11 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
         const T w = rg<T>(g, R_W, i);
         vout[(unsigned)j * (unsigned)g.P + (unsigned)i] = (double)((pr[2] - w) + w);
       }
-      if (prim_rec) in.prim_comoving(i, so, pr, uo);
+      if (prim_rec) {
+#pragma unroll
+        for (int l = 0; l < V; l++) ucons[l] = uo[l];
+        in.prim_comoving(i, so, pr, uo);
+      }
 #pragma unroll
       for (int l = 0; l < V; l++) UC(l, ty, tx) = uo[l];
     }
@@ -1097,7 +1102,7 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
     for (int l = 0; l < V; l++) {
       const T xw = FLX(l, ty, tx), ys = FLY(l, ty, tx);
       rhs[l] = dydzdV * (xf[l] - xw) + dzdxdV * (yf[l] - ys) + dxdydV * (zero - zero) - geo[l] - ext[l];
-      if (yout) (yout + l * g.FS)[ix] = uc[l];
+      if (yout) (yout + l * g.FS)[ix] = prim_rec ? ucons[l] : uc[l];
     }
     // boundary rows: the fluxes through the mesh boundary at the boundary
     // cells' outer interfaces, Mesh%dy*Mesh%dz*xflux (W), -Mesh%dy*Mesh%dz*xflux
