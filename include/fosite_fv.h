@@ -37,7 +37,8 @@ typedef struct fv_config {
   int limiter;      /* 1 minmod, 2 mc, 3 sweby, 4 superbee, 5 ospre, 7 van leer, 8 no limit
                        (fluxes/reconstruction_linear.f90:71-78) */
   double theta;     /* limiter parameter for mc and sweby */
-  int recvars;      /* 2 CONSERVATIVE (1 PRIMITIVE is rejected) (fluxes/reconstruction_base.f90:117) */
+  int recvars;      /* 1 PRIMITIVE, 2 CONSERVATIVE (fluxes/reconstruction_base.f90:117,
+                       fluxes/fluxes_base.f90:378-384) */
   int method, order;/* 2 RK_FEHLBERG, 4 DORMAND_PRINCE, 5 SSPRK (timedisc/timedisc_base.f90:187-191) */
   double cfl, tol_rel, tol_abs[4], stoptime, dtlimit, beta; /* timedisc_base.f90:315-352;
                        dtlimit <= 0 selects the default EPSILON*stoptime */
