@@ -125,10 +125,16 @@ struct fv_ctx {
   float *gravf[2] = {};
   int gcur = 0;
   // FARGO type 1: azimuthal velocities the last ComputeRHS left for the next
-  // CalcBackgroundVelocity ([NJ][P], FP64)
-  double *vbuf = nullptr;
+  // CalcBackgroundVelocity ([NJ][P], FP64) in vbufs[vcur]; w changes every
+  // step, so the per-ring arrays exist twice and differ only in the w row:
+  // rings(wcur) is current.  A device step reads rings(wcur) and vbufs[vcur]
+  // and writes the other ones, so a halted step leaves the host's intact.
+  double *vbufs[2] = {};
+  int vcur = 0, wcur = 0;
+  double *d_ring1 = nullptr;
+  float *d_ringf1 = nullptr;
   Ctl *d_ctl = nullptr, *h_ctl = nullptr;
-  struct GraphEntry { int x0, x1, t, k0, b0, g0, nsteps; hipGraphExec_t exec; };
+  struct GraphEntry { int x0, x1, t, k0, b0, g0, w0, v0, nsteps; hipGraphExec_t exec; };
   std::vector<GraphEntry> graphs;
   bool fast = true;         // device-controlled batches allowed
   int batch_steps = 0, batch_halts = 0;
@@ -166,6 +172,17 @@ struct fv_ctx {
       return -3;                                                               \
     }                                                                          \
   } while (0)
+
+namespace {
+// the per-ring arrays with w of parity q (FARGO type 1 keeps two)
+double *ring_tab(const fv_ctx *ctx, int q) { return q ? ctx->d_ring1 : ctx->d_ring; }
+float *ringf_tab(const fv_ctx *ctx, int q) { return q ? ctx->d_ringf1 : ctx->d_ringf; }
+void set_wcur(fv_ctx *ctx, int q) {
+  ctx->wcur = q;
+  ctx->g.ring = ring_tab(ctx, q);
+  ctx->g.ringf = ringf_tab(ctx, q);
+}
+}  // namespace
 
 namespace {
 
@@ -326,7 +343,7 @@ int rhs_main(fv_ctx *ctx, int checkbm, int remap) {
   a.pmin_clamp = pmin;
   a.want_cfl = 1;
   a.want_check = checkbm != CHECK_NOTHING;
-  a.vout = ctx->vbuf;
+  a.vout = ctx->vbufs[ctx->vcur];
   long pk = prof_begin(ctx);
   launch_rhs(ctx->g, a, ctx->V, ctx->st);
   prof_end(ctx, pk, K_RHS_END, 3.0 * word_bytes(ctx));
@@ -389,10 +406,10 @@ BfluxArgs bflux_args(const fv_ctx *ctx, int k0, int bin, double dt) {
   return b;
 }
 
-void launch_stage(fv_ctx *ctx, const StageArgs &a, int m) {
+void launch_stage(fv_ctx *ctx, const Geom &g, const StageArgs &a, int m) {
   const bool last = m == ctx->tab.m - 1;
   long pk = prof_begin(ctx);
-  launch_rhs(ctx->g, a, ctx->V, ctx->st);
+  launch_rhs(g, a, ctx->V, ctx->st);
   prof_end(ctx, pk, last ? K_RHS_COMBINE : K_RHS, (last ? m + 1.0 : m + 2.0) * word_bytes(ctx));
 }
 
@@ -405,7 +422,7 @@ int rhs_stage(fv_ctx *ctx, int m, double dt, int out) {
   a.dt = dt;
   a.add_bg = add;
   a.sub_bg = sub;
-  launch_stage(ctx, a, m);
+  launch_stage(ctx, ctx->g, a, m);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -490,8 +507,9 @@ int sub_bg_main(fv_ctx *ctx) {  // physics_eulerisotherm.f90:1373-1413
 int calc_background_velocity(fv_ctx *ctx) {
   int rc = add_bg_main(ctx);
   if (rc) return rc;
-  launch_wsum(ctx->g, ctx->vbuf, ctx->d_ring, ctx->d_ringf, ctx->st);
+  launch_wsum(ctx->g, ctx->vbufs[ctx->vcur], ring_tab(ctx, 1 - ctx->wcur), ringf_tab(ctx, 1 - ctx->wcur), ctx->st);
   HIPCHK(hipGetLastError());
+  set_wcur(ctx, 1 - ctx->wcur);
   return 0;
 }
 
@@ -554,15 +572,20 @@ int upload_geom(fv_ctx *ctx) {
   const int NI = ctx->NI, NJ = ctx->NJ;
   // copies on the context's own (non-blocking) stream: no null-stream
   // synchronisation with work other contexts have in flight
-  HIPCHK(hipMemcpyAsync(ctx->d_ring, ctx->ring_h.data(), sizeof(double) * R_COUNT * NI, hipMemcpyHostToDevice,
-                        ctx->st));
+  for (int q = 0; q < 2; q++)
+    if (ring_tab(ctx, q))
+      HIPCHK(hipMemcpyAsync(ring_tab(ctx, q), ctx->ring_h.data(), sizeof(double) * R_COUNT * NI,
+                            hipMemcpyHostToDevice, ctx->st));
   HIPCHK(hipMemcpyAsync(ctx->d_col, ctx->col_h.data(), sizeof(double) * C_COUNT * NJ, hipMemcpyHostToDevice,
                         ctx->st));
   std::vector<float> rf, cf;
   if (ctx->d_ringf) {
     rf.assign(ctx->ring_h.begin(), ctx->ring_h.end());
     cf.assign(ctx->col_h.begin(), ctx->col_h.end());
-    HIPCHK(hipMemcpyAsync(ctx->d_ringf, rf.data(), sizeof(float) * R_COUNT * NI, hipMemcpyHostToDevice, ctx->st));
+    for (int q = 0; q < 2; q++)
+      if (ringf_tab(ctx, q))
+        HIPCHK(hipMemcpyAsync(ringf_tab(ctx, q), rf.data(), sizeof(float) * R_COUNT * NI, hipMemcpyHostToDevice,
+                              ctx->st));
     HIPCHK(hipMemcpyAsync(ctx->d_colf, cf.data(), sizeof(float) * C_COUNT * NJ, hipMemcpyHostToDevice, ctx->st));
   }
   HIPCHK(hipStreamSynchronize(ctx->st));
@@ -585,13 +608,15 @@ bool ring_from(fv_ctx *ctx, int id, const double *cells) {
 // RHS and CFL update, and a 2-step HIP graph (buffers alternate between two
 // state arrays) is launched repeatedly.  Steps that leave this path halt the
 // batch and are redone by the host path above.
-bool fast_eligible(const fv_ctx *ctx) {
+// (batch: members of a batch, whose shared launches have no FARGO type-1
+// step, take host-controlled steps for it)
+bool fast_eligible(const fv_ctx *ctx, bool batch = false) {
   if (!ctx->fast || ctx->iter < 1 || ctx->brk || ctx->cvar != ctx->cold) return false;
-  if (ctx->c.fargo == 1) return false;  // w changes every step: host-controlled path
+  if (batch && ctx->c.fargo == 1) return false;
   if (!(ctx->c.tol_rel < 1.0)) return false;  // AdjustTimestep active
   // the stop test of the next Step() is the host's (fosite.f90:370-373)
   if (std::fabs(ctx->c.stoptime - ctx->time) <= 1.0E-05 * ctx->c.stoptime) return false;
-  const int f = ctx->c.fargo == 2 ? 1 : 0;
+  const int f = ctx->c.fargo ? 1 : 0;
   return ctx->fl_cvar == f && ctx->fl_pvar == f && ctx->fl_ctmp == f && ctx->fl_ptmp == f &&
          ctx->fl_cold == f;
 }
@@ -599,6 +624,17 @@ bool fast_eligible(const fv_ctx *ctx) {
 // The launches of one device-controlled step: input state x_in/k0_in ->
 // x_out/k0_out (tb: combined state, rb: remapped state), step control in ctl
 struct FastStep {
+  Geom g;              // geometry of the step's evaluations (FARGO type 1: with the new w)
+  // FARGO type 1, CalcBackgroundVelocity at the start of the step: x_in to the
+  // real frame with the old w (into rb), w from vin into the other ring
+  // arrays, back to the comoving frame with it (into x_out, the stages' input)
+  bool cbv;
+  Geom gold;
+  const double *vin;
+  double *wring;
+  float *wringf;
+  const void *fsrc;
+  void *fmid, *fdst;
   StageArgs st[MAXK];  // stages 2..s (index m-1); the last one fused with the combination
   bool remap;
   RemapArgs rm;
@@ -608,10 +644,27 @@ struct FastStep {
   StageArgs end;       // end-of-step ComputeRHS: CFL, data checks, new state with ghosts
 };
 void fast_step_args(fv_ctx *ctx, Ctl *ctl, int xin, int xout, int tb, int rb, int k0in, int k0out, int bin,
-                    int gin, FastStep &f) {
-  const int steady = ctx->c.fargo == 2 ? 1 : 0;
+                    int gin, int win, int vin, FastStep &f) {
+  const int steady = ctx->c.fargo ? 1 : 0;
+  f.g = ctx->g;
+  f.cbv = ctx->c.fargo == 1;
+  int ystage = xin;
+  if (f.cbv) {
+    f.gold = ctx->g;
+    f.gold.ring = ring_tab(ctx, win);
+    f.gold.ringf = ringf_tab(ctx, win);
+    f.g.ring = ring_tab(ctx, 1 - win);
+    f.g.ringf = ringf_tab(ctx, 1 - win);
+    f.vin = ctx->vbufs[vin];
+    f.wring = ring_tab(ctx, 1 - win);
+    f.wringf = ringf_tab(ctx, 1 - win);
+    f.fsrc = ctx->buf[xin];
+    f.fmid = ctx->buf[rb];
+    f.fdst = ctx->buf[xout];
+    ystage = xout;  // free until the end-of-step evaluation writes it
+  }
   for (int m = 1; m < ctx->tab.m; m++) {
-    StageArgs a = stage_args(ctx, m, xin, k0in, tb);
+    StageArgs a = stage_args(ctx, m, ystage, k0in, tb);
     a.ctl = ctl;
     a.add_bg = a.sub_bg = steady;
     set_acc(ctx, a, gin);
@@ -630,7 +683,7 @@ void fast_step_args(fv_ctx *ctx, Ctl *ctl, int xin, int xout, int tb, int rb, in
     f.acc.ctl = ctl;
   }
   int y = tb;
-  f.remap = ctx->c.fargo == 2;
+  f.remap = ctx->c.fargo != 0;
   if (f.remap) {
     f.rm = RemapArgs{ctx->buf[tb], ctx->buf[rb], 0.0, ctl};
     y = rb;
@@ -648,16 +701,25 @@ void fast_step_args(fv_ctx *ctx, Ctl *ctl, int xin, int xout, int tb, int rb, in
     set_acc(ctx, a, 1 - gin);
   else
     a.bfa = bflux_args(ctx, k0in, bin, 0.0);  // the step's boundary fluxes
+  if (f.cbv) a.vout = ctx->vbufs[1 - vin];  // for the next step's CalcBackgroundVelocity
   f.end = a;
 }
 
-void enqueue_fast_step(fv_ctx *ctx, int xin, int xout, int tb, int rb, int k0in, int k0out, int bin, int gin) {
+void enqueue_fast_step(fv_ctx *ctx, int xin, int xout, int tb, int rb, int k0in, int k0out, int bin, int gin,
+                       int win, int vin) {
   FastStep f;
-  fast_step_args(ctx, ctx->d_ctl, xin, xout, tb, rb, k0in, k0out, bin, gin, f);
-  for (int m = 1; m < ctx->tab.m; m++) launch_stage(ctx, f.st[m - 1], m);
+  fast_step_args(ctx, ctx->d_ctl, xin, xout, tb, rb, k0in, k0out, bin, gin, win, vin, f);
+  if (f.cbv) {
+    long pk = prof_begin(ctx);
+    launch_frame(f.gold, f.fsrc, f.fmid, +1, ctx->V, ctx->st, ctx->d_ctl);
+    launch_wsum(f.gold, f.vin, f.wring, f.wringf, ctx->st, ctx->d_ctl);
+    launch_frame(f.g, f.fmid, f.fdst, -1, ctx->V, ctx->st, ctx->d_ctl);
+    prof_end(ctx, pk, K_FRAME, 4.0 * word_bytes(ctx));
+  }
+  for (int m = 1; m < ctx->tab.m; m++) launch_stage(ctx, f.g, f.st[m - 1], m);
   if (f.remap) {
     long pk = prof_begin(ctx);
-    launch_remap(ctx->g, f.rm, ctx->V, ctx->st);
+    launch_remap(f.g, f.rm, ctx->V, ctx->st);
     prof_end(ctx, pk, K_REMAP, 2.0 * word_bytes(ctx));
   }
   if (f.accrete) {  // the step's boundary fluxes, then UpdateGravity
@@ -667,7 +729,7 @@ void enqueue_fast_step(fv_ctx *ctx, int xin, int xout, int tb, int rb, int k0in,
     prof_end(ctx, pk, K_ACCRETE, 0.0);
   }
   long pk = prof_begin(ctx);
-  launch_rhs(ctx->g, f.end, ctx->V, ctx->st);
+  launch_rhs(f.g, f.end, ctx->V, ctx->st);
   prof_end(ctx, pk, K_RHS_END, 3.0 * word_bytes(ctx));
   pk = prof_begin(ctx);
   launch_control(ctx->d_ctl, ctx->part_c, ctx->part_r, ctx->nb_rhs, ctx->st);
@@ -717,6 +779,10 @@ int replay_device_steps(fv_ctx *ctx, const Ctl *h, int x0, int x1, int *halted) 
     ctx->k0cur = 1 - ctx->k0cur;
     ctx->bcur = 1 - ctx->bcur;
     if (ctx->grav[0]) ctx->gcur = 1 - ctx->gcur;
+    if (ctx->c.fargo == 1) {
+      set_wcur(ctx, 1 - ctx->wcur);
+      ctx->vcur = 1 - ctx->vcur;
+    }
   }
   if (k > 0) {  // CalcTimestep of the last accepted step (before output clipping)
     ctx->invdt = h->invdt;
@@ -728,22 +794,24 @@ int replay_device_steps(fv_ctx *ctx, const Ctl *h, int x0, int x1, int *halted) 
   return 0;
 }
 
-int get_graph(fv_ctx *ctx, int x0, int x1, int tb, int rb, int k0, int b0, int g0, hipGraphExec_t *out) {
+int get_graph(fv_ctx *ctx, int x0, int x1, int tb, int rb, int k0, int b0, int g0, int w0, int v0,
+              hipGraphExec_t *out) {
   for (auto &e : ctx->graphs)
-    if (e.x0 == x0 && e.x1 == x1 && e.t == tb && e.k0 == k0 && e.b0 == b0 && e.g0 == g0) {
+    if (e.x0 == x0 && e.x1 == x1 && e.t == tb && e.k0 == k0 && e.b0 == b0 && e.g0 == g0 && e.w0 == w0 &&
+        e.v0 == v0) {
       *out = e.exec;
       return 0;
     }
   hipGraph_t graph;
   HIPCHK(hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
-  const int g1 = ctx->grav[0] ? 1 - g0 : g0;
-  enqueue_fast_step(ctx, x0, x1, tb, rb, k0, 1 - k0, b0, g0);
-  enqueue_fast_step(ctx, x1, x0, tb, rb, 1 - k0, k0, 1 - b0, g1);
+  const int g1 = ctx->grav[0] ? 1 - g0 : g0, t1 = ctx->c.fargo == 1;
+  enqueue_fast_step(ctx, x0, x1, tb, rb, k0, 1 - k0, b0, g0, w0, v0);
+  enqueue_fast_step(ctx, x1, x0, tb, rb, 1 - k0, k0, 1 - b0, g1, t1 ? 1 - w0 : w0, t1 ? 1 - v0 : v0);
   HIPCHK(hipStreamEndCapture(ctx->st, &graph));
   hipGraphExec_t exec;
   HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
   (void)hipGraphDestroy(graph);
-  ctx->graphs.push_back({x0, x1, tb, k0, b0, g0, 2, exec});
+  ctx->graphs.push_back({x0, x1, tb, k0, b0, g0, w0, v0, 2, exec});
   *out = exec;
   return 0;
 }
@@ -758,11 +826,12 @@ int run_batch(fv_ctx *ctx, int nmax, int *done_steps, int *halted) {
   const int tb = pick_free(ctx, x0, x1, -1);
   const int rb = pick_free(ctx, x0, x1, tb);
   const int k0 = ctx->k0cur, b0 = ctx->bcur, g0 = ctx->gcur, g1 = ctx->grav[0] ? 1 - g0 : g0;
+  const int w0 = ctx->wcur, v0 = ctx->vcur, t1 = ctx->c.fargo == 1;
   Ctl *h = ctx->h_ctl;
   prime_ctl(ctx, h, nmax);
   hipGraphExec_t exec = nullptr;
   if (!ctx->prof) {
-    int rc = get_graph(ctx, x0, x1, tb, rb, k0, b0, g0, &exec);
+    int rc = get_graph(ctx, x0, x1, tb, rb, k0, b0, g0, w0, v0, &exec);
     if (rc) return rc;
   }
   HIPCHK(hipMemcpyAsync(ctx->d_ctl, h, offsetof(Ctl, err_hist), hipMemcpyHostToDevice, ctx->st));
@@ -771,7 +840,8 @@ int run_batch(fv_ctx *ctx, int nmax, int *done_steps, int *halted) {
   } else {  // timed: the same launches, issued one by one between events
     for (int s = 0; s < nmax; s++)
       enqueue_fast_step(ctx, s % 2 ? x1 : x0, s % 2 ? x0 : x1, tb, rb, s % 2 ? 1 - k0 : k0, s % 2 ? k0 : 1 - k0,
-                        s % 2 ? 1 - b0 : b0, s % 2 ? g1 : g0);
+                        s % 2 ? 1 - b0 : b0, s % 2 ? g1 : g0, (s % 2 && t1) ? 1 - w0 : w0,
+                        (s % 2 && t1) ? 1 - v0 : v0);
     HIPCHK(hipGetLastError());
   }
   HIPCHK(hipMemcpyAsync(h, ctx->d_ctl, sizeof(Ctl), hipMemcpyDeviceToHost, ctx->st));
@@ -859,7 +929,11 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
   for (int q = 1; q < ctx->tab.m && ok; q++) ok = zalloc(&ctx->k[q]);
   for (int q = 0; q < 2 && ok; q++) ok = zalloc(&ctx->k0[q]);
   const size_t bfbytes = sizeof(double) * 2 * ctx->V * (ctx->NJ + ctx->NI);
-  if (c.fargo == 1) ok = ok && dalloc(&ctx->vbuf, sizeof(double) * ctx->FS);
+  for (int q = 0; q < 2 && c.fargo == 1; q++) ok = ok && dalloc(&ctx->vbufs[q], sizeof(double) * ctx->FS);
+  if (c.fargo == 1) {
+    ok = ok && dalloc(&ctx->d_ring1, sizeof(double) * R_COUNT * ctx->NI);
+    if (c.dtype == 1) ok = ok && hipMalloc((void **)&ctx->d_ringf1, sizeof(float) * R_COUNT * ctx->NI) == hipSuccess;
+  }
   ok = ok && dalloc(&ctx->bks, sizeof(double) * 2 * ctx->V * (ctx->NJ + ctx->NI));
   for (int q = 0; q < 2 && ok; q++)  // InitFluxes: zero (fluxes_base.f90:209-214)
     ok = dalloc(&ctx->bf[q], bfbytes) && hipMemsetAsync(ctx->bf[q], 0, bfbytes, ctx->st) == hipSuccess;
@@ -941,7 +1015,9 @@ extern "C" int fv_destroy(fv_ctx *ctx) {
   for (auto &b : ctx->bf) if (b) (void)hipFree(b);
   for (auto &b : ctx->grav) if (b) (void)hipFree(b);
   for (auto &b : ctx->gravf) if (b) (void)hipFree(b);
-  if (ctx->vbuf) (void)hipFree(ctx->vbuf);
+  for (auto &b : ctx->vbufs) if (b) (void)hipFree(b);
+  if (ctx->d_ring1) (void)hipFree(ctx->d_ring1);
+  if (ctx->d_ringf1) (void)hipFree(ctx->d_ringf1);
   if (ctx->bks) (void)hipFree(ctx->bks);
   if (ctx->part_r) (void)hipFree(ctx->part_r);
   if (ctx->part_c) (void)hipFree(ctx->part_c);
@@ -972,7 +1048,8 @@ extern "C" int fv_device_bytes(const fv_config *c, double *bytes) {
   const double nb32 = std::ceil(c->inum / 32.0) * std::ceil(c->jnum / (double)TY_TILE);
   double b = state * (NBUF + (t.m - 1) + 2);          // states, stage rhs, coeff(1) pair
   b += 3.0 * 8.0 * 2.0 * V * (NI + NJ);                // boundary fluxes (two copies), last-stage rows
-  if (c->fargo == 1) b += 8.0 * NJ * P;                // FARGO type 1 velocities
+  if (c->fargo == 1)                                    // FARGO type 1: velocities, second ring arrays
+    b += 2.0 * 8.0 * NJ * P + (8.0 + (c->dtype == 1 ? 4.0 : 0.0)) * R_COUNT * NI;
   if (c->outbound) b += 2.0 * (8.0 * (PM_HDR + 2.0 * NI) + (c->dtype == 1 ? 4.0 * 2.0 * NI : 0.0));
   b += 2.0 * 2.0 * 8.0 * nb32;                         // CFL and error partials
   b += 8.0 * (R_COUNT * NI + C_COUNT * NJ);             // metric arrays
@@ -1059,7 +1136,7 @@ extern "C" int fv_first_step(fv_ctx *ctx, int *brk) {
     a.kout = ctx->k0[1 - ctx->k0cur];
     a.sub_bg = 1;
     a.bflux = 0;
-    a.vout = ctx->vbuf;
+    a.vout = ctx->vbufs[ctx->vcur];
     launch_rhs(ctx->g, a, ctx->V, ctx->st);
     HIPCHK(hipGetLastError());
     if ((rc = calc_background_velocity(ctx))) return rc;
@@ -1163,7 +1240,7 @@ extern "C" int fv_get_state(fv_ctx *ctx, double *cvar, double *pvar) {
 extern "C" int fv_get_background_velocity(fv_ctx *ctx, double *w) {
   if (!ctx || !w) return -1;
   HIPCHK(hipStreamSynchronize(ctx->st));
-  HIPCHK(hipMemcpyAsync(w, ctx->d_ring + (size_t)R_W * ctx->NI, sizeof(double) * ctx->NI, hipMemcpyDeviceToHost,
+  HIPCHK(hipMemcpyAsync(w, ring_tab(ctx, ctx->wcur) + (size_t)R_W * ctx->NI, sizeof(double) * ctx->NI, hipMemcpyDeviceToHost,
                         ctx->st));
   HIPCHK(hipStreamSynchronize(ctx->st));
   return 0;
@@ -1459,7 +1536,7 @@ int batch_round(fv_batch *b, const std::vector<int> &want, int K, int c, bool ti
     for (int p = 0; p < 2; p++) {
       FastStep f;
       fast_step_args(x, dc, p ? r.x1 : r.x0, p ? r.x0 : r.x1, r.tb, r.rb, p ? 1 - r.k0 : r.k0,
-                     p ? r.k0 : 1 - r.k0, p ? 1 - r.b0 : r.b0, x->gcur, f);  // members have no accretion
+                     p ? r.k0 : 1 - r.k0, p ? 1 - r.b0 : r.b0, x->gcur, x->wcur, x->vcur, f);  // no accretion, no type 1
       for (int q = 0; q < m - 1; q++) b->h_rhs[(size_t)(p * m + q) * cap + d] = RhsEntry{x->g, f.st[q]};
       b->h_rhs[(size_t)(p * m + m - 1) * cap + d] = RhsEntry{x->g, f.end};
       RemapEntry re;
@@ -1606,7 +1683,7 @@ extern "C" int fv_batch_run(fv_batch *b, int nsteps, int *taken) {
   for (;;) {
     // members the device path cannot take yet step on the host path
     for (int d = 0; d < n; d++)
-      while (rem[d] > 0 && !fin[d] && !fast_eligible(b->sims[d]))
+      while (rem[d] > 0 && !fin[d] && !fast_eligible(b->sims[d], true))
         if ((rc = host_step(d))) return rc;
     int kmax = 0, kmin = CTL_HIST;
     for (int d = 0; d < n; d++) {

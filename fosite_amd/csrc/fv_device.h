@@ -186,14 +186,16 @@ __host__ __device__ inline int tile_by(unsigned e) { return (int)(e >> (TILE_DIS
 int rhs_blocks(const Geom &g);
 void launch_rhs(const Geom &g, const StageArgs &a, int V, hipStream_t s);
 void launch_remap(const Geom &g, const RemapArgs &a, int V, hipStream_t s);
-void launch_frame(const Geom &g, const void *src, void *dst, int sign, int V, hipStream_t s);
+void launch_frame(const Geom &g, const void *src, void *dst, int sign, int V, hipStream_t s,
+                  const Ctl *ctl = nullptr);
 void launch_reduce(const double *part, int n, double *out, hipStream_t s);
 void launch_control(Ctl *ctl, const double *part_c, const double *part_r, int nb, hipStream_t s);
 void launch_bflux(const BfluxArgs &bf, hipStream_t s);
 void launch_accrete(const AccreteArgs &p, hipStream_t s);
 // FARGO type 1: w(i) = SUM_j v(i,j)*wfactor(i) / JNUM per ring into the ring
 // arrays (FP64 and, for the precision form, FP32)
-void launch_wsum(const Geom &g, const double *v, double *ring, float *ringf, hipStream_t s);
+void launch_wsum(const Geom &g, const double *v, double *ring, float *ringf, hipStream_t s,
+                 const Ctl *ctl = nullptr);
 // batch launchers: ntiles blocks over the tile list; TX = tile width of the list
 #ifndef FV_RHS_ROWS
 #define FV_RHS_ROWS 8

@@ -4,5 +4,5 @@
 namespace fv {
 template void rhs_single<double, KM_SMALL>(const Geom &, const StageArgsT<double> &, int, hipStream_t);
 template void remap_single<double>(const Geom &, const RemapArgsT<double> &, int, hipStream_t);
-template void frame_single<double>(const Geom &, const void *, void *, int, int, hipStream_t);
+template void frame_single<double>(const Geom &, const void *, void *, int, int, const Ctl *, hipStream_t);
 }  // namespace fv

@@ -1387,9 +1387,11 @@ __global__ __launch_bounds__(256) void k_remap_multi(const RemapEntry *__restric
 }
 
 // FARGO frame change on all cells (physics_eulerisotherm.f90:1263-1413):
-// sign +1 adds rho*w to the azimuthal momentum, -1 subtracts it
+// sign +1 adds rho*w to the azimuthal momentum, -1 subtracts it (ctl: a
+// device-controlled step, skipped once the round has halted)
 template <typename T, int V>
-__global__ __launch_bounds__(256) void k_frame(Geom g, const T *src, T *dst, int sign) {
+__global__ __launch_bounds__(256) void k_frame(Geom g, const T *src, T *dst, int sign, const Ctl *ctl) {
+  if (ctl && ctl->halt) return;
   const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
   if (i >= g.NI || j >= g.NJ) return;
   const long ix = cidx(g, i, j);
@@ -1409,7 +1411,7 @@ void rhs_single(const Geom &g, const StageArgsT<T> &a, int V, hipStream_t s);
 template <typename T>
 void remap_single(const Geom &g, const RemapArgsT<T> &a, int V, hipStream_t s);
 template <typename T>
-void frame_single(const Geom &g, const void *src, void *dst, int sign, int V, hipStream_t s);
+void frame_single(const Geom &g, const void *src, void *dst, int sign, int V, const Ctl *ctl, hipStream_t s);
 template <typename T, int KM>
 void rhs_multi(const RhsEntry *tab, const unsigned *tiles, int n, int V, int limiter, int tx, int mode, hipStream_t s);
 // coefficient slots the kernels are compiled for, by tableau size

@@ -93,8 +93,10 @@ __global__ __launch_bounds__(256) void k_reduce(const double *part, int n, doubl
 // index order.  A block covers 64 rings; waves 1-3 stage the rows of the
 // next chunk in LDS while wave 0 adds the current one sequentially.
 constexpr int WS_RINGS = 64, WS_ROWS = 64;
-__global__ __launch_bounds__(256) void k_wsum(Geom g, const double *__restrict__ v, double *ring, float *ringf) {
+__global__ __launch_bounds__(256) void k_wsum(Geom g, const double *__restrict__ v, double *ring, float *ringf,
+                                              const Ctl *ctl) {
   __shared__ double buf[2][WS_ROWS][WS_RINGS];
+  if (ctl && ctl->halt) return;  // a device-controlled step after the round halted
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int i = blockIdx.x * WS_RINGS + lane, jmin = GH, J = g.jnum;
   const int nchunk = (J + WS_ROWS - 1) / WS_ROWS;
@@ -153,11 +155,11 @@ void launch_remap(const Geom &g, const RemapArgs &a, int V, hipStream_t s) {
     remap_single<double>(g, a, V, s);
   }
 }
-void launch_frame(const Geom &g, const void *src, void *dst, int sign, int V, hipStream_t s) {
+void launch_frame(const Geom &g, const void *src, void *dst, int sign, int V, hipStream_t s, const Ctl *ctl) {
   if (g.dtype == 1)
-    frame_single<float>(g, src, dst, sign, V, s);
+    frame_single<float>(g, src, dst, sign, V, ctl, s);
   else
-    frame_single<double>(g, src, dst, sign, V, s);
+    frame_single<double>(g, src, dst, sign, V, ctl, s);
 }
 void launch_control(Ctl *ctl, const double *part_c, const double *part_r, int nb, hipStream_t s) {
   hipLaunchKernelGGL(k_control, dim3(1), dim3(256), 0, s, ctl, part_c, part_r, nb);
@@ -182,8 +184,8 @@ void launch_remap_multi(const RemapEntry *tab, const unsigned *tiles, int ntiles
 void launch_control_multi(const ControlEntry *tab, int n, hipStream_t s) {
   hipLaunchKernelGGL(k_control_multi, dim3(n), dim3(256), 0, s, tab);
 }
-void launch_wsum(const Geom &g, const double *v, double *ring, float *ringf, hipStream_t s) {
-  hipLaunchKernelGGL(k_wsum, dim3((g.NI + WS_RINGS - 1) / WS_RINGS), dim3(256), 0, s, g, v, ring, ringf);
+void launch_wsum(const Geom &g, const double *v, double *ring, float *ringf, hipStream_t s, const Ctl *ctl) {
+  hipLaunchKernelGGL(k_wsum, dim3((g.NI + WS_RINGS - 1) / WS_RINGS), dim3(256), 0, s, g, v, ring, ringf, ctl);
 }
 void launch_bflux(const BfluxArgs &bf, hipStream_t s) {
   const int n = bf.V * 2 * (bf.inum + bf.jnum);

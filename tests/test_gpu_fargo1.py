@@ -73,13 +73,47 @@ def test_gpu_fargo1_matches_oracle(name):
 
 
 @pytest.mark.gpu
-def test_gpu_fargo1_run_uses_host_steps():
-    """Fosite.run() on a type-1 context takes host-controlled steps (w changes
-    every step) with the same bits as step()."""
-    cfg = CASES["nograd_radial"][0]
+@pytest.mark.parametrize("name", ["nograd_radial", "nograd_cfl_0.9_rejects", "absorbing"])
+def test_gpu_fargo1_device_rounds_match_oracle(name):
+    """Fosite.run() on a type-1 context: device-controlled steps with
+    CalcBackgroundVelocity in the step graph (frame change, ordered ring sums
+    into the other ring arrays, frame change back).  Rounds of several
+    lengths; a rejected step halts the round and the host redoes it from the
+    start-of-step w and velocities."""
+    cfg, _, tol = CASES[name]
+    print("fargo1 run()", name, flush=True)
     sim, ref = pair(cfg)
-    assert sim.run(12) == 12
-    for _ in range(12):
-        ref.step()
-    check(sim, ref, 0.0, "run")
+    done = 0
+    for chunk in (1, 2, 7, 12, 8):
+        done += sim.run(chunk)
+        for _ in range(chunk):
+            ref.step()
+        sc = check(sim, ref, tol, (name, done))
+    assert done == 30
+    if "rejects" in name:
+        assert sc["n_adj"] > 0
     sim.close()
+
+
+@pytest.mark.gpu
+def test_gpu_fargo1_batch_member_matches_run_alone():
+    """In a batch a type-1 member takes host-controlled steps (the batch's
+    shared launches have no CalcBackgroundVelocity); its bits equal those of
+    the same context run alone in device rounds."""
+    cfg = dict(CASES["nograd_radial"][0], seed=31)
+    other = dict(S.kepler_disk_config(ni=48, nj=64), seed=32)
+    alone, _ = pair(cfg)
+    alone.run(15)
+    ref_cv = alone.state()[0]
+    alone.close()
+    a, _ = pair(cfg)
+    init = S.initial(other)  # a FARGO type-2 member beside it
+    b = fosite_amd.Fosite(other)
+    b.set_initial(init["pvar"], init["bccsound"], init["fcsound"], init["w"])
+    b.first_step()
+    batch = fosite_amd.Batch([a, b])
+    assert batch.run(15) == [15, 15]
+    assert np.array_equal(a.state()[0], ref_cv)
+    batch.close()
+    a.close()
+    b.close()
