@@ -625,16 +625,17 @@ bool fast_eligible(const fv_ctx *ctx, bool batch = false) {
 // x_out/k0_out (tb: combined state, rb: remapped state), step control in ctl
 struct FastStep {
   Geom g;              // geometry of the step's evaluations (FARGO type 1: with the new w)
-  // FARGO type 1, CalcBackgroundVelocity at the start of the step: x_in to the
-  // real frame with the old w (into rb), w from vin into the other ring
-  // arrays, back to the comoving frame with it (into x_out, the stages' input)
+  // FARGO type 1, CalcBackgroundVelocity at the start of the step: w from vin
+  // into the other ring arrays, then x_in to the real frame with the old w
+  // and back to the comoving frame with the new one, in one pass (into x_out,
+  // the stages' input)
   bool cbv;
   Geom gold;
   const double *vin;
   double *wring;
   float *wringf;
-  const void *fsrc;
-  void *fmid, *fdst;
+  const void *fsrc, *wnew;
+  void *fdst;
   StageArgs st[MAXK];  // stages 2..s (index m-1); the last one fused with the combination
   bool remap;
   RemapArgs rm;
@@ -659,8 +660,9 @@ void fast_step_args(fv_ctx *ctx, Ctl *ctl, int xin, int xout, int tb, int rb, in
     f.wring = ring_tab(ctx, 1 - win);
     f.wringf = ringf_tab(ctx, 1 - win);
     f.fsrc = ctx->buf[xin];
-    f.fmid = ctx->buf[rb];
     f.fdst = ctx->buf[xout];
+    f.wnew = ctx->c.dtype == 1 ? (const void *)(f.wringf + (size_t)R_W * ctx->NI)
+                               : (const void *)(f.wring + (size_t)R_W * ctx->NI);
     ystage = xout;  // free until the end-of-step evaluation writes it
   }
   for (int m = 1; m < ctx->tab.m; m++) {
@@ -711,10 +713,9 @@ void enqueue_fast_step(fv_ctx *ctx, int xin, int xout, int tb, int rb, int k0in,
   fast_step_args(ctx, ctx->d_ctl, xin, xout, tb, rb, k0in, k0out, bin, gin, win, vin, f);
   if (f.cbv) {
     long pk = prof_begin(ctx);
-    launch_frame(f.gold, f.fsrc, f.fmid, +1, ctx->V, ctx->st, ctx->d_ctl);
     launch_wsum(f.gold, f.vin, f.wring, f.wringf, ctx->st, ctx->d_ctl);
-    launch_frame(f.g, f.fmid, f.fdst, -1, ctx->V, ctx->st, ctx->d_ctl);
-    prof_end(ctx, pk, K_FRAME, 4.0 * word_bytes(ctx));
+    launch_frame(f.gold, f.fsrc, f.fdst, +1, ctx->V, ctx->st, ctx->d_ctl, f.wnew);
+    prof_end(ctx, pk, K_FRAME, 2.0 * word_bytes(ctx));
   }
   for (int m = 1; m < ctx->tab.m; m++) launch_stage(ctx, f.g, f.st[m - 1], m);
   if (f.remap) {

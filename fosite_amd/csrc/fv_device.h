@@ -186,8 +186,10 @@ __host__ __device__ inline int tile_by(unsigned e) { return (int)(e >> (TILE_DIS
 int rhs_blocks(const Geom &g);
 void launch_rhs(const Geom &g, const StageArgs &a, int V, hipStream_t s);
 void launch_remap(const Geom &g, const RemapArgs &a, int V, hipStream_t s);
+// (w2: the w row, of the state's element type, of a second frame change
+// applied in the same pass)
 void launch_frame(const Geom &g, const void *src, void *dst, int sign, int V, hipStream_t s,
-                  const Ctl *ctl = nullptr);
+                  const Ctl *ctl = nullptr, const void *w2 = nullptr);
 void launch_reduce(const double *part, int n, double *out, hipStream_t s);
 void launch_control(Ctl *ctl, const double *part_c, const double *part_r, int nb, hipStream_t s);
 void launch_bflux(const BfluxArgs &bf, hipStream_t s);

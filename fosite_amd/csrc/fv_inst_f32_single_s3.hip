@@ -4,5 +4,6 @@
 namespace fv {
 template void rhs_single<float, KM_SMALL>(const Geom &, const StageArgsT<float> &, int, hipStream_t);
 template void remap_single<float>(const Geom &, const RemapArgsT<float> &, int, hipStream_t);
-template void frame_single<float>(const Geom &, const void *, void *, int, int, const Ctl *, hipStream_t);
+template void frame_single<float>(const Geom &, const void *, void *, int, int, const Ctl *, const void *,
+                                   hipStream_t);
 }  // namespace fv

@@ -4,5 +4,6 @@
 namespace fv {
 template void rhs_single<double, KM_SMALL>(const Geom &, const StageArgsT<double> &, int, hipStream_t);
 template void remap_single<double>(const Geom &, const RemapArgsT<double> &, int, hipStream_t);
-template void frame_single<double>(const Geom &, const void *, void *, int, int, const Ctl *, hipStream_t);
+template void frame_single<double>(const Geom &, const void *, void *, int, int, const Ctl *, const void *,
+                                   hipStream_t);
 }  // namespace fv

@@ -1387,10 +1387,12 @@ __global__ __launch_bounds__(256) void k_remap_multi(const RemapEntry *__restric
 }
 
 // FARGO frame change on all cells (physics_eulerisotherm.f90:1263-1413):
-// sign +1 adds rho*w to the azimuthal momentum, -1 subtracts it (ctl: a
-// device-controlled step, skipped once the round has halted)
+// sign +1 adds rho*w to the azimuthal momentum, -1 subtracts it.  With w2
+// (the w row of other ring arrays) both changes in one pass: + rho*w, then
+// - rho*w2, rounded as the two passes round.  ctl: a device-controlled step,
+// skipped once the round has halted.
 template <typename T, int V>
-__global__ __launch_bounds__(256) void k_frame(Geom g, const T *src, T *dst, int sign, const Ctl *ctl) {
+__global__ __launch_bounds__(256) void k_frame(Geom g, const T *src, T *dst, int sign, const Ctl *ctl, const T *w2) {
   if (ctl && ctl->halt) return;
   const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
   if (i >= g.NI || j >= g.NJ) return;
@@ -1399,7 +1401,10 @@ __global__ __launch_bounds__(256) void k_frame(Geom g, const T *src, T *dst, int
 #pragma unroll
   for (int l = 0; l < V; l++) {
     T v = src[l * g.FS + ix];
-    if (l == 2) v = sign > 0 ? v + src[ix] * w : v - src[ix] * w;
+    if (l == 2) {
+      v = sign > 0 ? v + src[ix] * w : v - src[ix] * w;
+      if (w2) v = v - src[ix] * w2[i];
+    }
     dst[l * g.FS + ix] = v;
   }
 }
@@ -1411,7 +1416,8 @@ void rhs_single(const Geom &g, const StageArgsT<T> &a, int V, hipStream_t s);
 template <typename T>
 void remap_single(const Geom &g, const RemapArgsT<T> &a, int V, hipStream_t s);
 template <typename T>
-void frame_single(const Geom &g, const void *src, void *dst, int sign, int V, const Ctl *ctl, hipStream_t s);
+void frame_single(const Geom &g, const void *src, void *dst, int sign, int V, const Ctl *ctl, const void *w2,
+                  hipStream_t s);
 template <typename T, int KM>
 void rhs_multi(const RhsEntry *tab, const unsigned *tiles, int n, int V, int limiter, int tx, int mode, hipStream_t s);
 // coefficient slots the kernels are compiled for, by tableau size

@@ -100,10 +100,20 @@ __global__ __launch_bounds__(256) void k_wsum(Geom g, const double *__restrict__
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int i = blockIdx.x * WS_RINGS + lane, jmin = GH, J = g.jnum;
   const int nchunk = (J + WS_ROWS - 1) / WS_ROWS;
-  auto stage = [&](int c) {  // waves 1..3: rows of chunk c into buf[c & 1]
-    for (int r = wv - 1; r < WS_ROWS; r += 3) {
-      const int j = c * WS_ROWS + r;
-      buf[c & 1][r][lane] = (i < g.NI && j < J) ? v[(long)(jmin + j) * g.P + i] : 0.0;
+  // waves 1..3: rows wv-1, wv+2, ... of chunk c into buf[c & 1], every load
+  // issued before the first store (one memory latency per chunk)
+  constexpr int RPW = (WS_ROWS + 2) / 3;
+  auto stage = [&](int c) {
+    double t[RPW];
+#pragma unroll
+    for (int q = 0; q < RPW; q++) {
+      const int r = wv - 1 + 3 * q, j = c * WS_ROWS + r;
+      t[q] = (r < WS_ROWS && i < g.NI && j < J) ? v[(long)(jmin + j) * g.P + i] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < RPW; q++) {
+      const int r = wv - 1 + 3 * q;
+      if (r < WS_ROWS) buf[c & 1][r][lane] = t[q];
     }
   };
   if (wv > 0) stage(0);
@@ -155,11 +165,12 @@ void launch_remap(const Geom &g, const RemapArgs &a, int V, hipStream_t s) {
     remap_single<double>(g, a, V, s);
   }
 }
-void launch_frame(const Geom &g, const void *src, void *dst, int sign, int V, hipStream_t s, const Ctl *ctl) {
+void launch_frame(const Geom &g, const void *src, void *dst, int sign, int V, hipStream_t s, const Ctl *ctl,
+                  const void *w2) {
   if (g.dtype == 1)
-    frame_single<float>(g, src, dst, sign, V, ctl, s);
+    frame_single<float>(g, src, dst, sign, V, ctl, w2, s);
   else
-    frame_single<double>(g, src, dst, sign, V, ctl, s);
+    frame_single<double>(g, src, dst, sign, V, ctl, w2, s);
 }
 void launch_control(Ctl *ctl, const double *part_c, const double *part_r, int nb, hipStream_t s) {
   hipLaunchKernelGGL(k_control, dim3(1), dim3(256), 0, s, ctl, part_c, part_r, nb);

@@ -46,14 +46,15 @@ void remap_single(const Geom &g, const RemapArgsT<T> &a, int V, hipStream_t s) {
     hipLaunchKernelGGL((k_remap<T, 4>), grid, block, 0, s, g, a);
 }
 template <typename T>
-void frame_single(const Geom &g, const void *src, void *dst, int sign, int V, const Ctl *ctl, hipStream_t s) {
+void frame_single(const Geom &g, const void *src, void *dst, int sign, int V, const Ctl *ctl, const void *w2,
+                  hipStream_t s) {
   dim3 grid((g.NI + 63) / 64, (g.NJ + 3) / 4), block(64, 4);
   const T *sp = static_cast<const T *>(src);
   T *dp = static_cast<T *>(dst);
   if (V == 3)
-    hipLaunchKernelGGL((k_frame<T, 3>), grid, block, 0, s, g, sp, dp, sign, ctl);
+    hipLaunchKernelGGL((k_frame<T, 3>), grid, block, 0, s, g, sp, dp, sign, ctl, static_cast<const T *>(w2));
   else
-    hipLaunchKernelGGL((k_frame<T, 4>), grid, block, 0, s, g, sp, dp, sign, ctl);
+    hipLaunchKernelGGL((k_frame<T, 4>), grid, block, 0, s, g, sp, dp, sign, ctl, static_cast<const T *>(w2));
 }
 
 // batches: van Leer compiled in, every other limiter (and mixed limiters) by
