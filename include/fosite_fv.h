@@ -43,7 +43,7 @@ typedef struct fv_config {
   double cfl, tol_rel, tol_abs[4], stoptime, dtlimit, beta; /* timedisc_base.f90:315-352;
                        dtlimit <= 0 selects the default EPSILON*stoptime */
   int fargo;        /* 0 off, 1 per-ring mean azimuthal velocity recomputed every step
-                     (timedisc/timedisc_base.f90:2243-2326; host-controlled steps),
+                     (timedisc/timedisc_base.f90:2243-2326),
                      2 user-supplied background velocity (mesh/mesh_base.f90:372-423) */
   int gravity;      /* 1: point mass at the origin, Newtonian
                        (sources/gravity_pointmass.f90:132-301) */
