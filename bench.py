@@ -268,6 +268,26 @@ def accretion_variant(steps=300):
     return out
 
 
+def fargo1_variant(steps=300):
+    """The headline disk with FARGO type 1 (the background velocity
+    recomputed from the mean azimuthal velocity of every ring at each step)
+    instead of the user-supplied one: ms per step of both, device-controlled."""
+    import fosite_amd
+    from fosite_amd import setups
+    out = {"workload": "isothermal Keplerian disk, logcylindrical 512x1024, FP64, SSPRK3; FARGO type 1 vs type 2"}
+    for name, fargo in (("type2", 2), ("type1", 1)):
+        cfg = dict(setups.kepler_disk_config(ni=512, nj=1024), fargo=fargo)
+        init = setups.initial(cfg)
+        sim = fosite_amd.Fosite(cfg)
+        sim.set_initial(init["pvar"], init["bccsound"], init["fcsound"], init["w"] if fargo == 2 else None)
+        sim.first_step()
+        sim.run(20)
+        taken = sim.run(steps)
+        out[name + "_ms_per_step"] = sim.last_run_ms() / max(taken, 1)
+        sim.close()
+    return out
+
+
 def serving_form_full(n=40):
     """The serving stream at the stated grid sizes, radial cells U[64,1024]
     (azimuthal 2x), 20 requests/s: the offered load (all cell updates of the
@@ -293,7 +313,8 @@ def main():
     ap.add_argument("--no-forms", action="store_true",
                     help="skip the precision, batched and serving forms")
     ap.add_argument("--form-steps", type=int, default=500)
-    ap.add_argument("--only-form", choices=["precision", "batched", "serving", "serving_full", "accretion"],
+    ap.add_argument("--only-form", choices=["precision", "batched", "serving", "serving_full", "accretion",
+                                             "fargo1"],
                     help="run one form alone and print its object (for profiling)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -303,7 +324,7 @@ def main():
     if a.only_form:
         f = {"precision": lambda: precision_form(1024, 2048, a.form_steps), "batched": batched_form,
              "serving": serving_form, "serving_full": serving_form_full,
-             "accretion": accretion_variant}[a.only_form]
+             "accretion": accretion_variant, "fargo1": fargo1_variant}[a.only_form]
         print(json.dumps(f()))
         return 0
 
@@ -376,6 +397,7 @@ def main():
         out["serving_form"] = serving_form()
         out["serving_form_full_sizes"] = serving_form_full()
         out["accretion_variant"] = accretion_variant()
+        out["fargo1_variant"] = fargo1_variant()
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg, init, a.cpu_steps)
     print(json.dumps(out))
