@@ -286,6 +286,15 @@ class Fosite:
                   "fv_get_boundary_flux_total")
         return out
 
+    def write_dataset(self, path):
+        """The current state as a dataset in fosite's binary format
+        (io/fileio_binary.f90; see fosite_amd.datafile): mesh coordinates,
+        time, primitive variables, point mass, configuration."""
+        from . import datafile
+        _, pv = self.state()
+        pm = self.point_mass() if self.cfg.get("gravity", 0) else None
+        datafile.write(path, datafile.dataset_entries(self.cfg, self.scalars()["time"], pv, pm))
+
     def point_mass(self):
         """The point mass (gravity_pointmass.f90:79-85): array [mass, accrate,
         massloss, mdot]; with accretion (config key ``outbound``) updated by

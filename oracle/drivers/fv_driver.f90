@@ -33,12 +33,13 @@ PROGRAM fv_driver
   INTEGER :: k_fargo = 0, k_gravity = 0, k_units = 0
   REAL    :: k_mass = 1.0, k_acclimit = -1.0
   INTEGER :: k_outbound = 0
+  INTEGER :: k_binary = 0
   INTEGER :: k_nsteps = 10, k_dumpevery = 1
   CHARACTER(LEN=512) :: k_infile = "", k_prefix = "out"
   NAMELIST /fvcfg/ k_physics, k_geometry, k_inum, k_jnum, k_gamma, k_xmin, k_xmax, k_ymin, k_ymax, &
        k_gparam, k_bcw, k_bce, k_bcs, k_bcn, k_limiter, k_recvars, k_theta, k_method, k_order,   &
        k_cfl, k_tolrel, k_tolabs, k_stoptime, k_dtlimit, k_beta, k_fargo, k_gravity, k_units,   &
-       k_mass, k_outbound, k_acclimit, k_nsteps, k_dumpevery, k_infile, k_prefix
+       k_mass, k_outbound, k_acclimit, k_binary, k_nsteps, k_dumpevery, k_infile, k_prefix
   !--------------------------------------------------------------------------!
   CLASS(fosite), ALLOCATABLE :: Sim
   TYPE(Dict_TYP), POINTER    :: config, mesh, phys, bnd, flx, tdisc, dfile, &
@@ -81,8 +82,14 @@ PROGRAM fv_driver
     CALL SetAttr(tdisc, "tol_abs", k_tolabs(1:3))
   END IF
   IF (k_dtlimit.GT.0.0) CALL SetAttr(tdisc, "dtlimit", k_dtlimit)
-  dfile => Dict("fileformat" / GNUPLOT, "filename" / (TRIM(k_prefix) // "_data"), &
-       "count" / 1)
+  ! k_binary = 1: the reference's own binary datasets (io/fileio_binary.f90)
+  IF (k_binary.EQ.1) THEN
+    dfile => Dict("fileformat" / BINARY, "filename" / (TRIM(k_prefix) // "_data"), &
+         "count" / 1)
+  ELSE
+    dfile => Dict("fileformat" / GNUPLOT, "filename" / (TRIM(k_prefix) // "_data"), &
+         "count" / 1)
+  END IF
   config => Dict("mesh" / mesh, "physics" / phys, "boundary" / bnd, &
        "fluxes" / flx, "timedisc" / tdisc, "datafile" / dfile)
   IF (k_gravity.GT.0) THEN

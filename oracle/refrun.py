@@ -33,7 +33,8 @@ def _nml(cfg, nsteps, dump_every, infile, prefix):
                 k_beta=cfg.get("beta", 0.08), k_fargo=cfg.get("fargo", 0),
                 k_gravity=cfg.get("gravity", 0), k_units=3 if cfg.get("gravity", 0) else 0,
                 k_mass=cfg.get("mass", 1.0), k_outbound=cfg.get("outbound", 0),
-                k_acclimit=float(cfg.get("acclimit", -1.0)), k_nsteps=nsteps, k_dumpevery=dump_every)
+                k_acclimit=float(cfg.get("acclimit", -1.0)), k_binary=int(cfg.get("binary_output", 0)),
+                k_nsteps=nsteps, k_dumpevery=dump_every)
     lines = ["&fvcfg"]
     for k, v in vals.items():
         lines.append(" %s = %s" % (k, repr(float(v)) if isinstance(v, float) else v))
@@ -46,7 +47,9 @@ def _nml(cfg, nsteps, dump_every, infile, prefix):
 
 def run(cfg, init, nsteps, dump_every=1, workdir=None):
     """Returns (mesh dict, list of step records {step, iter, n_adj, brk, time,
-    dt, dtold, maxerrold, cvar, pvar})."""
+    dt, dtold, maxerrold, cvar, pvar}).  With cfg["binary_output"] = 1 the
+    reference also writes its binary datasets (run_data_NNNN.bin, at time 0
+    and at the stop time) into workdir."""
     NI, NJ = cfg["inum"] + 4, cfg["jnum"] + 4
     N = NI * NJ
     V = 4 if cfg["physics"] == 17 else 3

@@ -23,6 +23,8 @@ def write_inputs(cfg, nsteps, tmp, batch=0, dump_every=1):
         if cfg.get("fargo", 0) == 2:
             f.write(np.ascontiguousarray(init["w"], dtype="<f8").tobytes())
     nml = refrun._nml(cfg, nsteps, dump_every, infile, os.path.join(tmp, "run"))
+    # the reference driver's dataset switch is not a key of this program
+    nml = "\n".join(l for l in nml.split("\n") if not l.strip().startswith("k_binary"))
     nml = nml.replace("\n/\n", "\n k_batch = %d\n/\n" % batch)
     path = os.path.join(tmp, "cfg.nml")
     with open(path, "w") as f:
