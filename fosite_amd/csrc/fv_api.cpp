@@ -118,6 +118,11 @@ struct fv_ctx {
   double *bf[2] = {};
   int bcur = 0;
   double *bks = nullptr;  // boundary rows of the last stage's rhs
+  // Timedisc%pvar as the last host-path ComputeRHS left it (pvalid; its FARGO
+  // frame flag in pflag); device rounds do not write it
+  double *pbuf = nullptr;  // element type by dtype, like the state buffers
+  bool pvalid = false;
+  int pflag = 0;
   // point-mass accretion (c.outbound): state buffers [PM_HDR + 2*NI] and their
   // FP32 rings; every UpdateGravity reads grav[gcur], writes the other and
   // makes it current
@@ -344,10 +349,13 @@ int rhs_main(fv_ctx *ctx, int checkbm, int remap) {
   a.want_cfl = 1;
   a.want_check = checkbm != CHECK_NOTHING;
   a.vout = ctx->vbufs[ctx->vcur];
+  a.pout = ctx->pbuf;
   long pk = prof_begin(ctx);
   launch_rhs(ctx->g, a, ctx->V, ctx->st);
   prof_end(ctx, pk, K_RHS_END, 3.0 * word_bytes(ctx));
   HIPCHK(hipGetLastError());
+  ctx->pvalid = true;
+  ctx->pflag = ctx->fl_pvar;
   ctx->cvar = outb;
   int rc = reduce(ctx, ctx->part_r, ctx->nb_rhs, &ctx->invdt, &ctx->flags);
   if (rc) return rc;
@@ -765,6 +773,7 @@ void prime_ctl(const fv_ctx *ctx, Ctl *h, int nmax) {
 // timedisc_base.f90:649-750); the state alternated x0 -> x1 -> x0 ...
 int replay_device_steps(fv_ctx *ctx, const Ctl *h, int x0, int x1, int *halted) {
   const int k = h->step;
+  if (k > 0) ctx->pvalid = false;  // device steps leave no pvar
   for (int s = 0; s < k; s++) {
     const double dts = h->dt_hist[s];
     if (dts < ctx->dtmin && ctx->dtcause != DTCAUSE_FILEIO) ctx->dtmin = dts;
@@ -927,6 +936,7 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
     return dalloc(p, sbytes) && hipMemsetAsync(*p, 0, sbytes, ctx->st) == hipSuccess;
   };
   for (int q = 0; q < NBUF && ok; q++) ok = zalloc(&ctx->buf[q]);
+  ok = ok && zalloc(&ctx->pbuf);
   for (int q = 1; q < ctx->tab.m && ok; q++) ok = zalloc(&ctx->k[q]);
   for (int q = 0; q < 2 && ok; q++) ok = zalloc(&ctx->k0[q]);
   const size_t bfbytes = sizeof(double) * 2 * ctx->V * (ctx->NJ + ctx->NI);
@@ -1017,6 +1027,7 @@ extern "C" int fv_destroy(fv_ctx *ctx) {
   for (auto &b : ctx->grav) if (b) (void)hipFree(b);
   for (auto &b : ctx->gravf) if (b) (void)hipFree(b);
   for (auto &b : ctx->vbufs) if (b) (void)hipFree(b);
+  if (ctx->pbuf) (void)hipFree(ctx->pbuf);
   if (ctx->d_ring1) (void)hipFree(ctx->d_ring1);
   if (ctx->d_ringf1) (void)hipFree(ctx->d_ringf1);
   if (ctx->bks) (void)hipFree(ctx->bks);
@@ -1047,7 +1058,7 @@ extern "C" int fv_device_bytes(const fv_config *c, double *bytes) {
   const double V = c->physics == EULER ? 4.0 : 3.0, es = c->dtype == 1 ? 4.0 : 8.0;
   const double state = es * V * NJ * P;  // one state buffer, ghost cells and row padding included
   const double nb32 = std::ceil(c->inum / 32.0) * std::ceil(c->jnum / (double)TY_TILE);
-  double b = state * (NBUF + (t.m - 1) + 2);          // states, stage rhs, coeff(1) pair
+  double b = state * (NBUF + (t.m - 1) + 2 + 1);      // states, stage rhs, coeff(1) pair, pvar
   b += 3.0 * 8.0 * 2.0 * V * (NI + NJ);                // boundary fluxes (two copies), last-stage rows
   if (c->fargo == 1)                                    // FARGO type 1: velocities, second ring arrays
     b += 2.0 * 8.0 * NJ * P + (8.0 + (c->dtype == 1 ? 4.0 : 0.0)) * R_COUNT * NI;
@@ -1117,6 +1128,7 @@ extern "C" int fv_set_pvar(fv_ctx *ctx, const double *pvar) {
   HIPCHK(hipStreamSynchronize(ctx->st));
   ctx->fl_pvar = ctx->fl_cvar = 0;
   ctx->have_state = true;
+  ctx->pvalid = false;
   return 0;
 }
 
@@ -1225,7 +1237,28 @@ extern "C" int fv_get_state(fv_ctx *ctx, double *cvar, double *pvar) {
   HIPCHK(hipStreamSynchronize(ctx->st));
   if (ctx->c.dtype == 1) std::copy(cf.begin(), cf.end(), cv.begin());
   if (cvar) std::memcpy(cvar, cv.data(), sizeof(double) * V * N);
-  if (pvar) {
+  if (pvar && ctx->pvalid) {
+    // Timedisc%pvar of the last ComputeRHS, in the frame of the state: after
+    // an output-time AddBackgroundVelocityY (physics_eulerisotherm.f90:
+    // 1263-1303) the same + w as the reference applies to its pvar
+    std::vector<double> pv(V * N);
+    std::vector<float> pf(ctx->c.dtype == 1 ? V * N : 0);
+    char *pd = ctx->c.dtype == 1 ? reinterpret_cast<char *>(pf.data()) : reinterpret_cast<char *>(pv.data());
+    for (int l = 0; l < V; l++)
+      HIPCHK(hipMemcpy2DAsync(pd + l * N * es, es * ctx->NI, field(ctx, ctx->pbuf, l), es * ctx->P, es * ctx->NI,
+                              ctx->NJ, hipMemcpyDeviceToHost, ctx->st));
+    std::vector<double> w(ctx->NI, 0.0);
+    const bool addw = ctx->c.fargo && ctx->pflag && !ctx->fl_pvar;
+    if (addw)
+      HIPCHK(hipMemcpyAsync(w.data(), ring_tab(ctx, ctx->wcur) + (size_t)R_W * ctx->NI, sizeof(double) * ctx->NI,
+                            hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(hipStreamSynchronize(ctx->st));
+    if (ctx->c.dtype == 1) std::copy(pf.begin(), pf.end(), pv.begin());
+    if (addw)
+      for (int j = 0; j < ctx->NJ; j++)
+        for (int i = 0; i < ctx->NI; i++) pv[2 * N + (size_t)j * ctx->NI + i] += w[i];
+    std::memcpy(pvar, pv.data(), sizeof(double) * V * N);
+  } else if (pvar) {  // before FirstStep: Convert2Primitive of the state
     const double g = ctx->c.gamma;
     for (size_t k = 0; k < N; k++) {  // Cons2Prim2d
       const double rho = cv[k], inv = 1. / rho, mu = cv[N + k], mv = cv[2 * N + k];

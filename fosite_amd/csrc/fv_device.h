@@ -133,6 +133,10 @@ struct StageArgsT {
   // CalcBackgroundVelocity sees it, (v - w) + w (timedisc_base.f90:2243-2326),
   // for interior rows of every ring (ghost rings included); [NJ][P], or null
   double *vout;
+  // end-of-step evaluation: Timedisc%pvar as ComputeRHS leaves it (after the
+  // PMIN clamp and the frame change; ghost cells from CenterBoundary), all
+  // cells, or null
+  T *pout;
   // gravitational acceleration of this evaluation, rings x then y ([2][NI]):
   // the static rings, or the current point-mass state with accretion
   const double *acc;

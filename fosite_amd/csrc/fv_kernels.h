@@ -794,6 +794,7 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
   constexpr int KE = kEnd ? 0 : KM;
   const bool want_cfl = kEnd && a.want_cfl, want_check = kEnd && a.want_check;
   T *const yout = kEnd ? a.yout : nullptr;
+  T *const pout = kEnd ? a.pout : nullptr;
   double *const vout = kEnd ? a.vout : nullptr;
   T *const ynew = kLast ? a.ynew : nullptr;
   constexpr int TY = TY_RHS, SX = TX + 4, SY = TY + 4;
@@ -880,6 +881,11 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
 #pragma unroll
           for (int l = 0; l < V; l++) (yout + l * g.FS)[ix] = u[l];
         }
+        if (pout) {
+          const unsigned ix = (unsigned)jj * (unsigned)g.P + (unsigned)ii;
+#pragma unroll
+          for (int l = 0; l < V; l++) (pout + l * g.FS)[ix] = pp[l];
+        }
         if (vout && yin) {  // x-ghost ring of an interior row: (v - w) + w
           const T w = rg<T>(g, R_W, ii), vc = a.sub_bg ? pp[2] : pp[2] - w;
           vout[(unsigned)jj * (unsigned)g.P + (unsigned)ii] = (double)(vc + w);
@@ -907,6 +913,11 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
       if (vout) {  // pvar after SubtractBackgroundVelocityY, then + w again
         const T w = rg<T>(g, R_W, i);
         vout[(unsigned)j * (unsigned)g.P + (unsigned)i] = (double)((pr[2] - w) + w);
+      }
+      if (pout) {  // pvar after SubtractBackgroundVelocityY
+        const unsigned ix = (unsigned)j * (unsigned)g.P + (unsigned)i;
+#pragma unroll
+        for (int l = 0; l < V; l++) (pout + l * g.FS)[ix] = (l == 2 && a.sub_bg) ? pr[2] - rg<T>(g, R_W, i) : pr[l];
       }
       if (prim_rec) {
 #pragma unroll
@@ -1232,6 +1243,11 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
         const unsigned ix = (unsigned)gj * (unsigned)g.P + (unsigned)gi;
 #pragma unroll
         for (int l = 0; l < V; l++) (yout + l * g.FS)[ix] = u[l];
+      }
+      if (pout) {
+        const unsigned ix = (unsigned)gj * (unsigned)g.P + (unsigned)gi;
+#pragma unroll
+        for (int l = 0; l < V; l++) (pout + l * g.FS)[ix] = p[l];
       }
     }
   }

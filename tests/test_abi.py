@@ -92,16 +92,17 @@ def test_server_create_rejects_bad_options():
 
 
 def test_device_bytes_accounts_for_states_and_dtype():
-    """Footprint: 4 state buffers + (stages-1) stage rhs + 2 coeff(1) buffers,
-    ghost-inclusive and row-padded, plus small per-run arrays."""
+    """Footprint: 4 state buffers + (stages-1) stage rhs + 2 coeff(1) buffers
+    + the pvar readout buffer, ghost-inclusive and row-padded, plus small
+    per-run arrays."""
     cfg = S.kepler_disk_config(ni=512, nj=1024)
     b64 = fosite_amd.device_bytes(cfg)
     b32 = fosite_amd.device_bytes(dict(cfg, dtype="fp32"))
     state64 = 8 * 3 * (1024 + 4) * 528  # NJ x padded NI
-    assert 8 * state64 <= b64 <= 8 * state64 * 1.01   # SSPRK3: 4 + 2 + 2 buffers
+    assert 9 * state64 <= b64 <= 9 * state64 * 1.01   # SSPRK3: 4 + 2 + 2 + 1 buffers
     assert 0.49 < b32 / b64 < 0.52
     dp5 = fosite_amd.device_bytes(S.euler_disk_config(ni=512, nj=1024))
-    assert dp5 > 12 * 8 * 4 * 1028 * 528              # DP5: 4 + 6 + 2 buffers of 4 fields
+    assert dp5 > 13 * 8 * 4 * 1028 * 528              # DP5: 4 + 6 + 2 + 1 buffers of 4 fields
 
 
 def test_read_back_entry_points_reject_a_null_context():
