@@ -110,8 +110,12 @@ int fv_step(fv_ctx *ctx, int *done);
 /* nsteps x fv_step; *taken = steps executed before the run ended */
 int fv_run(fv_ctx *ctx, int nsteps, int *taken);
 
-/* Device -> host copy of Timedisc%cvar (all cells) and of the primitive
- * variables Convert2Primitive(cvar) (either pointer may be NULL). */
+/* Device -> host copy of Timedisc%cvar and Timedisc%pvar, all cells (either
+ * pointer may be NULL).  pvar is the one the last host-controlled ComputeRHS
+ * left (timedisc_base.f90:1098-1164: after the PMIN clamp, ghost cells from
+ * CenterBoundary), in the state's FARGO frame; every output step is such a
+ * step.  After device-controlled steps, or before FirstStep, pvar is
+ * Convert2Primitive(cvar). */
 int fv_get_state(fv_ctx *ctx, double *cvar, double *pvar);
 int fv_get_scalars(const fv_ctx *ctx, fv_scalars *out);
 /* Timedisc%w, the FARGO background velocity per x-ring (NI values, ghost
