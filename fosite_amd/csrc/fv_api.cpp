@@ -196,7 +196,7 @@ int fail(fv_ctx *ctx, const std::string &m, int code = -1) {
   return code;
 }
 
-int pick_free(const fv_ctx *ctx, int a, int b, int c) {
+int pick_free(const fv_ctx *, int a, int b, int c) {
   for (int q = 0; q < NBUF; q++)
     if (q != a && q != b && q != c) return q;
   return -1;

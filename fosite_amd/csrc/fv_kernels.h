@@ -457,7 +457,7 @@ static __device__ double blk_max(double v, double *red) {
 }
 // block-wide max of v and OR of f with one barrier; the results are valid in
 // thread 0 only, and a block may use its scratch (red, redi) once
-static __device__ void blk_max_or0(double &v, int &f, double *red, int *redi) {
+static __device__ inline void blk_max_or0(double &v, int &f, double *red, int *redi) {
   v = wave_max(v);
   f = wave_or(f);
   const int t = threadIdx.x + threadIdx.y * blockDim.x, nw = (blockDim.x * blockDim.y + 63) / 64;
