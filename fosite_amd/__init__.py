@@ -286,14 +286,19 @@ class Fosite:
                   "fv_get_boundary_flux_total")
         return out
 
-    def write_dataset(self, path):
+    def write_dataset(self, path, filename=None):
         """The current state as a dataset in fosite's binary format
         (io/fileio_binary.f90; see fosite_amd.datafile): mesh coordinates,
-        time, primitive variables, point mass, configuration."""
+        time, primitive variables, point mass, configuration tree.  filename
+        is stored as /config/datafile/filename; by default it is path without
+        its extension and step number, as fosite names its files
+        (<filename>_<step 4 digits>.bin)."""
         from . import datafile
+        if filename is None:
+            filename = datafile.filename_of(path)
         _, pv = self.state()
         pm = self.point_mass() if self.cfg.get("gravity", 0) else None
-        datafile.write(path, datafile.dataset_entries(self.cfg, self.scalars()["time"], pv, pm))
+        datafile.write(path, datafile.dataset_entries(self.cfg, self.scalars()["time"], pv, pm, filename))
 
     def point_mass(self):
         """The point mass (gravity_pointmass.f90:79-85): array [mass, accrate,

@@ -10,22 +10,23 @@ Layout of a file (fileio_binary.f90:36-49, WriteHeader 215-247, WriteKey
             5)], data
 
 Types are common_dict.f90:93-105 (0 a node of the key tree, 1 int, 2 real,
-5 1-D real, 7 3-D real, 8 4-D real).  Arrays are stored as Fortran stores
-them: first index fastest.  The writer emits, in the reference's order and
-under its keys, the mesh coordinates (mesh_base.f90:835-862), the time and
+3 character, 5 1-D real, 7 3-D real, 8 4-D real).  Arrays are stored as
+Fortran stores them: first index fastest.  The writer emits, in the
+reference's order and under its keys, the mesh coordinates (mesh_base.f90:835-862), the time and
 the primitive variables of the Timedisc (timedisc_base.f90:566-580), the
-point mass (gravity_pointmass.f90:152-195) and the configuration values that
-a context holds (the reference also stores every default it filled in;
-those are not repeated here).
+point mass (gravity_pointmass.f90:152-195) and the configuration tree: the
+values a setup passes and every default that fosite's Init routines fill in.
 """
 import math
+import re
 import struct
 
 import numpy as np
 
 from . import setups
 
-NODE, INT, REAL, REAL_1D, REAL_3D, REAL_4D = 0, 1, 2, 5, 7, 8
+NODE, INT, REAL, CHAR, REAL_1D, REAL_3D, REAL_4D = 0, 1, 2, 3, 5, 7, 8
+MAX_CHAR_LEN = 128  # common_dict.f90:92; strings are blank-padded to it
 _NDIMS = {6: 3, 7: 3, 8: 4, 12: 5}
 
 
@@ -37,7 +38,8 @@ def _record(key, typ, payload, dims=None):
 
 def write(path, entries):
     """entries: (key, value) pairs in file order; value None is a node, int an
-    integer, float a real, a 1-D array a REAL_ONED, an array with .ndim 3 or 4
+    integer, float a real, str a blank-padded character value of MAX_CHAR_LEN
+    bytes, a 1-D array a REAL_ONED, an array with .ndim 3 or 4
     (Fortran index order, i.e. numpy shape reversed) a 3-D / 4-D real array."""
     out = [b"FOSITE" + b"II" + bytes([0]) + b"%2d%2d" % (8, 4)]
     for key, v in entries:
@@ -49,6 +51,11 @@ def write(path, entries):
             out.append(_record(key, INT, struct.pack("<i", int(v))))
         elif isinstance(v, (float, np.floating)):
             out.append(_record(key, REAL, struct.pack("<d", float(v))))
+        elif isinstance(v, str):
+            b = v.encode()
+            if len(b) > MAX_CHAR_LEN:
+                raise ValueError("string value of %s longer than %d bytes" % (key, MAX_CHAR_LEN))
+            out.append(_record(key, CHAR, b.ljust(MAX_CHAR_LEN, b" ")))
         else:
             a = np.ascontiguousarray(v, dtype="<f8")
             if a.ndim == 1:
@@ -94,6 +101,13 @@ def read(path):
     return out
 
 
+def filename_of(path):
+    """The /config/datafile/filename value for an output file at path: path
+    without its extension and step number, fosite's files being named
+    <filename>_<step, 4 digits>.bin (fileio_base.f90:803-812)."""
+    return re.sub(r"(_\d{4})?\.bin$", "", path)
+
+
 def _cartesian(cfg, x, y, z):
     """Convert2Cartesian_coords (geometry_logcylindrical.f90:344-356; the
     cartesian geometry is the identity)"""
@@ -137,10 +151,12 @@ PVAR_NAMES = {setups.EULER_ISOTHERM: ("density", "xvelocity", "yvelocity"),
               setups.EULER: ("density", "xvelocity", "yvelocity", "pressure")}
 
 
-def dataset_entries(cfg, time, pvar, point_mass=None):
+def dataset_entries(cfg, time, pvar, point_mass=None, filename=""):
     """The entries of one output: pvar with ghost cells ([V][NJ][NI], the
-    reference's marray layout), the time, and the point mass ([mass,
-    accrate, massloss, mdot]) when the configuration has one."""
+    reference's marray layout), the time, the point mass ([mass, accrate,
+    massloss, mdot]) when the configuration has one, and the configuration
+    tree; filename is the value of /config/datafile/filename (the output
+    file's name without its step number and extension)."""
     cfg = setups.full(cfg)
     e = mesh_entries(cfg)
     e += [("/timedisc", None), ("/timedisc/time", float(time))]
@@ -150,31 +166,119 @@ def dataset_entries(cfg, time, pvar, point_mass=None):
         e += [("/sources", None), ("/sources/grav", None), ("/sources/grav/pmass", None),
               ("/sources/grav/pmass/mass", float(point_mass[0])),
               ("/sources/grav/pmass/accrate", float(point_mass[1]))]
-    e += config_entries(cfg)
+    e += config_entries(cfg, filename)
     return e
 
 
-def config_entries(cfg):
-    """/config: the Dict values a context is built from, under the reference's
-    keys (e.g. tests/riemann2d.f90:586-670)."""
-    e = [("/config", None), ("/config/mesh", None), ("/config/mesh/meshtype", 1),
-         ("/config/mesh/geometry", int(cfg["geometry"])), ("/config/mesh/inum", int(cfg["inum"])),
-         ("/config/mesh/jnum", int(cfg["jnum"])), ("/config/mesh/knum", 1),
-         ("/config/mesh/xmin", float(cfg["xmin"])), ("/config/mesh/xmax", float(cfg["xmax"])),
-         ("/config/mesh/ymin", float(cfg["ymin"])), ("/config/mesh/ymax", float(cfg["ymax"])),
-         ("/config/mesh/zmin", 0.0), ("/config/mesh/zmax", 0.0), ("/config/mesh/gparam", float(cfg["gparam"]))]
-    if cfg.get("fargo", 0):
-        e += [("/config/mesh/fargo", None), ("/config/mesh/fargo/method", int(cfg["fargo"]))]
-    e += [("/config/physics", None), ("/config/physics/problem", int(cfg["physics"]))]
-    if cfg["physics"] == setups.EULER:
-        e.append(("/config/physics/gamma", float(cfg["gamma"])))
-    e += [("/config/boundary", None)]
-    for d, name in enumerate(("western", "eastern", "southern", "northern")):
-        e.append(("/config/boundary/" + name, int(cfg["bc"][d])))
-    e += [("/config/fluxes", None), ("/config/fluxes/variables", int(cfg["recvars"])),
-          ("/config/fluxes/limiter", int(cfg["limiter"])), ("/config/fluxes/theta", float(cfg["theta"])),
-          ("/config/timedisc", None), ("/config/timedisc/method", int(cfg["method"])),
-          ("/config/timedisc/order", int(cfg["order"])), ("/config/timedisc/cfl", float(cfg["cfl"])),
-          ("/config/timedisc/stoptime", float(cfg["stoptime"])), ("/config/timedisc/tol_rel", float(cfg["tol_rel"])),
-          ("/config/timedisc/beta", float(cfg["beta"]))]
+CVAR_NAMES = {setups.EULER_ISOTHERM: ("density", "xmomentum", "ymomentum"),
+              setups.EULER: ("density", "xmomentum", "ymomentum", "energy")}
+VERSION = "0.8.3"  # CMakeLists.txt:18
+BINARY, LINEAR, KT, GRAVITY, POINTMASS, NEWTON = 1, 2, 1, 1, 1, 1
+SI, GEOMETRICAL = 1, 3  # physics units (physics_base.f90:554)
+CHECK_ALL = -1
+
+
+def _tree(prefix, pairs):
+    """(key, value) pairs under prefix; a key "a/b" is preceded by its node
+    "a" the first time "a" occurs, as SetAttr creates it."""
+    out, seen = [], set()
+    for k, v in pairs:
+        parts = k.split("/")
+        for d in range(1, len(parts)):
+            node = "/".join(parts[:d])
+            if node not in seen:
+                seen.add(node)
+                out.append((prefix + node, None))
+        seen.add(k)
+        out.append((prefix + k, v))
+    return out
+
+
+def config_entries(cfg, filename=""):
+    """/config: the configuration Dict of a run, in the order its keys enter
+    it -- first the values a setup program passes (as tests/golden's driver
+    builds them, like tests/riemann2d.f90:586-670), then each default that an
+    Init routine fills in with GetAttr, in call order:
+      mesh       geometry_cartesian.f90:91 (dz), mesh_base.f90:324-378
+                 (shearingbox, omega, Qshear, rotation_center, fargo/method),
+                 mesh_base.f90:836-920 (output/*)
+      physics    physics_base.f90:554-590 (units, mu, advanced_wave_speeds,
+                 softening, output/*), physics_eulerisotherm.f90:176 (cs)
+      fluxes     fluxes_base.f90 output/*
+      timedisc   timedisc_base.f90:295-359 (starttime .. rhomin),
+                 SetOutput 534-646 (output/*), timedisc_rkfehlberg.f90:121
+      datafile   fileio_base.f90:420-470
+      sources    gravity_pointmass.f90, sources_gravity.f90
+      version    fosite.f90:250
+    The values are the ones fosite 0.8.3 writes (checked key by key against
+    datasets the reference wrote)."""
+    cfg = setups.full(cfg)
+    isot = cfg["physics"] == setups.EULER_ISOTHERM
+    V = 3 if isot else 4
+    fargo = int(cfg.get("fargo", 0))
+    stop = float(cfg["stoptime"])
+    mesh = [("meshtype", 1), ("geometry", int(cfg["geometry"])), ("inum", int(cfg["inum"])),
+            ("jnum", int(cfg["jnum"])), ("knum", 1),
+            ("xmin", float(cfg["xmin"])), ("xmax", float(cfg["xmax"])),
+            ("ymin", float(cfg["ymin"])), ("ymax", float(cfg["ymax"])),
+            ("zmin", 0.0), ("zmax", 0.0), ("gparam", float(cfg["gparam"]))]
+    if fargo:
+        mesh.append(("fargo/method", fargo))
+    if cfg["geometry"] == setups.CARTESIAN:
+        mesh.append(("dz", 1.0))
+    mesh += [("shearingbox", 0), ("omega", 0.0), ("Qshear", 1.5),
+             ("rotation_center", np.zeros(3))]
+    if not fargo:
+        mesh.append(("fargo/method", 0))
+    mesh += [("output/" + k, v) for k, v in (
+        ("corners", 0), ("grid", 1), ("bary_curv", 1), ("bary", 1), ("rotation", 0), ("dl", 0),
+        ("bh", 0), ("commutator", 0), ("volume", 0), ("dA", 0), ("radius", 0),
+        ("position_vector", 0))]
+    phys = [("problem", int(cfg["physics"]))]
+    if not isot:
+        phys.append(("gamma", float(cfg["gamma"])))
+    phys += [("units", GEOMETRICAL if cfg.get("gravity", 0) else SI), ("mu", 0.029),
+             ("advanced_wave_speeds", 0), ("softening", 1.0)]
+    if isot:
+        phys.append(("cs", 0.0))
+    phys += [("output/bccsound", 0), ("output/fcsound", 0)]
+    bnd = [(name, int(cfg["bc"][d])) for d, name in
+           enumerate(("western", "eastern", "southern", "northern"))]
+    bnd += [("bottomer", setups.NO_GRADIENTS), ("topper", setups.NO_GRADIENTS)]
+    flx = [("order", LINEAR), ("fluxtype", KT), ("variables", int(cfg["recvars"])),
+           ("limiter", int(cfg["limiter"])), ("theta", float(cfg["theta"]))]
+    flx += [("output/" + k, 0) for k in ("pstates", "cstates", "pfluxes", "wave_speeds", "slopes")]
+    dtlimit = float(cfg["dtlimit"]) if cfg.get("dtlimit", -1.0) > 0 else np.finfo(np.float64).eps * stop
+    td = [("method", int(cfg["method"])), ("order", int(cfg["order"])), ("cfl", float(cfg["cfl"])),
+          ("stoptime", stop), ("tol_rel", float(cfg["tol_rel"])), ("beta", float(cfg["beta"])),
+          ("maxiter", 100000000), ("tol_abs", np.array([float(x) for x in cfg["tol_abs"]][:V])),
+          ("starttime", 0.0), ("dtlimit", float(dtlimit)), ("dtmax", 5.0), ("rhstype", 0),
+          ("always_update_bccsound", 1), ("checkdata", CHECK_ALL)]
+    tiny = float(np.finfo(np.float64).tiny)
+    td += [("pmin", tiny), ("tmin", tiny), ("rhomin", tiny)]
+    pn, cn = PVAR_NAMES[cfg["physics"]], CVAR_NAMES[cfg["physics"]]
+    td += [("output/" + pn[0], 1), ("output/error", 0), ("output/solution", 0), ("output/time", 1)]
+    for l in range(V):
+        if l:
+            td.append(("output/" + pn[l], 1))
+        td.append(("output/" + (cn[l] + "_cvar" if cn[l] == pn[l] else cn[l]), 0))
+        if not l:
+            td += [("output/" + k, 0) for k in ("geometrical_sources", "external_sources", "rhs", "fluxes")]
+    td += [("output/bflux", 0), ("ShowButcherTableau", 0)]
+    dfile = [("fileformat", BINARY), ("filename", filename), ("count", 1), ("filepath", ""),
+             ("filecycles", 2), ("unit", 11), ("dtwall", 3600), ("stoptime", stop),
+             ("walltime", float(np.finfo(np.float64).max)), ("step", 0), ("cartcoords", 1)]
+    e = [("/config", None)]
+    e += _tree("/config/", [("mesh/" + k, v) for k, v in mesh] + [("physics/" + k, v) for k, v in phys]
+               + [("boundary/" + k, v) for k, v in bnd] + [("fluxes/" + k, v) for k, v in flx]
+               + [("timedisc/" + k, v) for k, v in td] + [("datafile/" + k, v) for k, v in dfile])
+    if cfg.get("gravity", 0):
+        pm = [("gtype", POINTMASS), ("potential", NEWTON), ("mass", float(cfg["mass"])),
+              ("outbound", int(cfg.get("outbound", 0))), ("acclimit", float(cfg.get("acclimit", -1.0))),
+              ("output/accel", 0), ("output/position", 0), ("output/potential", 0),
+              ("x", 0.0), ("y", 0.0), ("z", 0.0), ("softening", 0.0), ("switchon", -1.0)]
+        grav = [("stype", GRAVITY)] + [("pmass/" + k, v) for k, v in pm]
+        grav += [("update_disk_height", 0), ("energy", 1), ("output/height", 0), ("output/accel", 0)]
+        e += _tree("/config/", [("sources/grav/" + k, v) for k, v in grav])
+    e.append(("/config/version", VERSION))
     return e

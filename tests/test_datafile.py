@@ -34,9 +34,14 @@ def oracle_output(cfg, nsteps):
     return sc["time"], pv, (o.pointmass() if cfg.get("gravity", 0) else None)
 
 
+def ref_filename(ref):
+    """/config/datafile/filename of a reference dataset (the path prefix the
+    reference run was given), as a string."""
+    return ref["/config/datafile/filename"][1].decode().rstrip(" ")
+
+
 def compare(ours, ref, tol=0.0):
-    keys = list(ours)
-    assert keys == [k for k in ref if k in ours]  # same keys, in the reference's order
+    assert list(ours) == list(ref)  # every key, in the reference's order
     for k, (t, v) in ours.items():
         rt, rv = ref[k]
         assert t == rt, (k, t, rt)
@@ -55,12 +60,33 @@ def test_dataset_of_the_oracle_state_equals_the_reference_dataset(name, tmp_path
     cfg, nsteps = CASES[name]
     time, pv, pm = oracle_output(cfg, nsteps)
     path = str(tmp_path / "out.bin")
-    DF.write(path, DF.dataset_entries(cfg, time, pv, pm))
     ref = reference(name)
+    DF.write(path, DF.dataset_entries(cfg, time, pv, pm, ref_filename(ref)))
     ours = DF.read(path)
     assert ours["/timedisc/time"][1] == cfg["stoptime"]
     compare(ours, ref)
-    assert len(ours) >= 40
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_configuration_tree_depends_on_the_configuration(name):
+    """Values that follow the configuration (not fixed defaults) move with
+    it: dtlimit is EPSILON*stoptime unless given (timedisc_base.f90:308),
+    the stop times, tol_abs has VNUM entries, the geometry's and FARGO's
+    keys."""
+    cfg, _ = CASES[name]
+    base = dict(DF.config_entries(S.full(cfg)))
+    alt = dict(DF.config_entries(S.full(dict(cfg, stoptime=2.5))))
+    assert alt["/config/timedisc/dtlimit"] == np.finfo(np.float64).eps * 2.5
+    assert alt["/config/timedisc/stoptime"] == alt["/config/datafile/stoptime"] == 2.5
+    given = dict(DF.config_entries(S.full(dict(cfg, dtlimit=1e-9))))
+    assert given["/config/timedisc/dtlimit"] == 1e-9
+    assert len(base["/config/timedisc/tol_abs"]) == (3 if cfg["physics"] == S.EULER_ISOTHERM else 4)
+    assert ("/config/mesh/dz" in base) == (cfg["geometry"] == S.CARTESIAN)
+    keys = [k for k, _ in DF.config_entries(S.full(dict(cfg, fargo=1)))]
+    assert keys.index("/config/mesh/fargo/method") < keys.index("/config/mesh/shearingbox")
+    keys = [k for k, _ in DF.config_entries(S.full(dict(cfg, fargo=0)))]
+    assert keys.index("/config/mesh/fargo/method") > keys.index("/config/mesh/rotation_center")
+    assert ("/config/sources/grav/pmass/mass" in base) == bool(cfg.get("gravity", 0))
 
 
 def test_reader_round_trip_and_header(tmp_path):
@@ -76,3 +102,9 @@ def test_reader_round_trip_and_header(tmp_path):
     assert np.array_equal(d["/n/v"][1], [1.0, 2.0])
     assert d["/n/a"][0] == DF.REAL_3D and np.array_equal(d["/n/a"][1], a3)
     assert d["/n/b"][0] == DF.REAL_4D and np.array_equal(d["/n/b"][1], a4)
+    assert DF.filename_of("/tmp/x/run_data_0001.bin") == "/tmp/x/run_data"
+    assert DF.filename_of("out.bin") == "out" and DF.filename_of("out") == "out"
+    DF.write(path, [("/s", "run_data")])
+    assert DF.read(path)["/s"] == (DF.CHAR, b"run_data".ljust(DF.MAX_CHAR_LEN))
+    with pytest.raises(ValueError):
+        DF.write(path, [("/s", "x" * (DF.MAX_CHAR_LEN + 1))])

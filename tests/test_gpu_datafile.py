@@ -1,6 +1,7 @@
 """Datasets written from a GPU run (Fosite.write_dataset) against the ones
 the reference wrote for the same runs (tests/golden/ref_dataset_*.bin).
-Mesh, time, point mass and configuration entries are bit-exact, and so are
+Every key of the reference's file is written, in its order; mesh, time,
+point mass and the configuration tree are bit-exact, and so are
 the primitive variables -- Timedisc%pvar as the last ComputeRHS left it,
 after Euler's PMIN clamp (timedisc_base.f90:1150-1164) -- except where the
 disk's absorbing boundaries (LOG/EXP on the path) bound them to relative L1
@@ -12,7 +13,7 @@ import fosite_amd
 from fosite_amd import datafile as DF
 from fosite_amd import setups as S
 from tests.dataset_cases import CASES
-from tests.test_datafile import compare, reference
+from tests.test_datafile import compare, ref_filename, reference
 
 
 @pytest.mark.gpu
@@ -29,10 +30,11 @@ def test_gpu_dataset_equals_the_reference_dataset(name, tmp_path):
         if sim.step():
             break
     path = str(tmp_path / "gpu.bin")
-    sim.write_dataset(path)
+    ref = reference(name)
+    sim.write_dataset(path, ref_filename(ref))
     sim.close()
     tol = 1e-12 if S.ABSORBING in cfg["bc"] else 0.0
-    compare(DF.read(path), reference(name), tol)
+    compare(DF.read(path), ref, tol)
 
 
 @pytest.mark.gpu
