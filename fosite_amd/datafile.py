@@ -154,7 +154,8 @@ PVAR_NAMES = {setups.EULER_ISOTHERM: ("density", "xvelocity", "yvelocity"),
 def dataset_entries(cfg, time, pvar, point_mass=None, filename=""):
     """The entries of one output: pvar with ghost cells ([V][NJ][NI], the
     reference's marray layout), the time, the point mass ([mass, accrate,
-    massloss, mdot]) when the configuration has one, and the configuration
+    massloss, mdot]: mass and accrate, and massloss when an accretion limit is
+    set) when the configuration has one, and the configuration
     tree; filename is the value of /config/datafile/filename (the output
     file's name without its step number and extension)."""
     cfg = setups.full(cfg)
@@ -166,6 +167,8 @@ def dataset_entries(cfg, time, pvar, point_mass=None, filename=""):
         e += [("/sources", None), ("/sources/grav", None), ("/sources/grav/pmass", None),
               ("/sources/grav/pmass/mass", float(point_mass[0])),
               ("/sources/grav/pmass/accrate", float(point_mass[1]))]
+        if cfg.get("acclimit", -1.0) > 0.0:  # gravity_pointmass.f90:193-194
+            e.append(("/sources/grav/pmass/massloss", float(point_mass[2])))
     e += config_entries(cfg, filename)
     return e
 
@@ -248,12 +251,15 @@ def config_entries(cfg, filename=""):
     flx = [("order", LINEAR), ("fluxtype", KT), ("variables", int(cfg["recvars"])),
            ("limiter", int(cfg["limiter"])), ("theta", float(cfg["theta"]))]
     flx += [("output/" + k, 0) for k in ("pstates", "cstates", "pfluxes", "wave_speeds", "slopes")]
-    dtlimit = float(cfg["dtlimit"]) if cfg.get("dtlimit", -1.0) > 0 else np.finfo(np.float64).eps * stop
     td = [("method", int(cfg["method"])), ("order", int(cfg["order"])), ("cfl", float(cfg["cfl"])),
           ("stoptime", stop), ("tol_rel", float(cfg["tol_rel"])), ("beta", float(cfg["beta"])),
-          ("maxiter", 100000000), ("tol_abs", np.array([float(x) for x in cfg["tol_abs"]][:V])),
-          ("starttime", 0.0), ("dtlimit", float(dtlimit)), ("dtmax", 5.0), ("rhstype", 0),
-          ("always_update_bccsound", 1), ("checkdata", CHECK_ALL)]
+          ("maxiter", 100000000), ("tol_abs", np.array([float(x) for x in cfg["tol_abs"]][:V]))]
+    # a given dtlimit is the setup's value; otherwise the default follows starttime
+    if cfg.get("dtlimit", -1.0) > 0:
+        td += [("dtlimit", float(cfg["dtlimit"])), ("starttime", 0.0)]
+    else:
+        td += [("starttime", 0.0), ("dtlimit", float(np.finfo(np.float64).eps * stop))]
+    td += [("dtmax", 5.0), ("rhstype", 0), ("always_update_bccsound", 1), ("checkdata", CHECK_ALL)]
     tiny = float(np.finfo(np.float64).tiny)
     td += [("pmin", tiny), ("tmin", tiny), ("rhomin", tiny)]
     pn, cn = PVAR_NAMES[cfg["physics"]], CVAR_NAMES[cfg["physics"]]

@@ -8,4 +8,15 @@ CASES = {
     "disk16x32": (dict(S.kepler_disk_config(ni=16, nj=32), stoptime=0.05), 3),
     # Euler, four-quadrant Riemann problem, no-gradient boundaries
     "riemann16": (dict(S.riemann2d_config(n=16), stoptime=0.02), 8),
+    # Euler disk: Dormand-Prince, primitive reconstruction, accretion through
+    # the western boundary with an accretion limit, no-gradient radial
+    # boundaries, no FARGO
+    "euler_disk16x32": (dict(S.euler_disk_config(ni=16, nj=32), stoptime=0.02,
+                             recvars=S.PRIMITIVE, outbound=1, acclimit=0.001,
+                             bc=[S.NO_GRADIENTS, S.NO_GRADIENTS, S.PERIODIC, S.PERIODIC]), 40),
+    # isothermal disk: FARGO type 1, a given dtlimit, accretion through the
+    # eastern boundary
+    "disk_fargo1_16x32": (dict(S.kepler_disk_config(ni=16, nj=32), stoptime=0.05, fargo=1,
+                               dtlimit=1e-10, outbound=2,
+                               bc=[S.NO_GRADIENTS, S.NO_GRADIENTS, S.PERIODIC, S.PERIODIC]), 20),
 }

@@ -5,7 +5,8 @@ and writes its output there.  Output: tests/golden/ref_dataset_<name>.bin
 (data only).
 
 Usage (needs the reference checkout and its build):
-    python oracle/build_ref.py && python tests/golden/make_datasets.py
+    python oracle/build_ref.py && python tests/golden/make_datasets.py [NAME ...]
+(all cases when no NAME is given)
 """
 import os
 import shutil
@@ -25,7 +26,8 @@ def main():
     if not refrun.available():
         sys.exit("reference build missing: run oracle/build_ref.py first")
     from fosite_amd import setups as S
-    for name, (cfg, nsteps) in CASES.items():
+    for name in sys.argv[1:] or list(CASES):
+        cfg, nsteps = CASES[name]
         tmp = tempfile.mkdtemp(prefix="fvds_")
         refrun.run(dict(cfg, binary_output=1), S.initial(cfg), nsteps, workdir=tmp)
         out = os.path.join(HERE, "ref_dataset_%s.bin" % name)

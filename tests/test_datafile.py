@@ -75,11 +75,13 @@ def test_configuration_tree_depends_on_the_configuration(name):
     keys."""
     cfg, _ = CASES[name]
     base = dict(DF.config_entries(S.full(cfg)))
-    alt = dict(DF.config_entries(S.full(dict(cfg, stoptime=2.5))))
+    alt = dict(DF.config_entries(S.full(dict(cfg, stoptime=2.5, dtlimit=-1.0))))
     assert alt["/config/timedisc/dtlimit"] == np.finfo(np.float64).eps * 2.5
     assert alt["/config/timedisc/stoptime"] == alt["/config/datafile/stoptime"] == 2.5
-    given = dict(DF.config_entries(S.full(dict(cfg, dtlimit=1e-9))))
-    assert given["/config/timedisc/dtlimit"] == 1e-9
+    given = DF.config_entries(S.full(dict(cfg, dtlimit=1e-9)))
+    assert dict(given)["/config/timedisc/dtlimit"] == 1e-9
+    keys = [k for k, _ in given]
+    assert keys.index("/config/timedisc/dtlimit") < keys.index("/config/timedisc/starttime")
     assert len(base["/config/timedisc/tol_abs"]) == (3 if cfg["physics"] == S.EULER_ISOTHERM else 4)
     assert ("/config/mesh/dz" in base) == (cfg["geometry"] == S.CARTESIAN)
     keys = [k for k, _ in DF.config_entries(S.full(dict(cfg, fargo=1)))]
