@@ -125,7 +125,9 @@ __global__ __launch_bounds__(256) void k_wsum(Geom g, const double *__restrict__
       if (c + 1 < nchunk) stage(c + 1);
     } else {
       const int rows = J - c * WS_ROWS < WS_ROWS ? J - c * WS_ROWS : WS_ROWS;
-      for (int r = 0; r < rows; r++) acc = acc + buf[c & 1][r][lane] * f;
+      // acc + v*f term by term in row order; the LDS reads and products of
+      // 16 rows are issued ahead of their adds
+      acc = ordered_sum<16>(acc, rows, [&](int r) { return buf[c & 1][r][lane] * f; });
     }
     __syncthreads();
   }
