@@ -90,30 +90,35 @@ __global__ __launch_bounds__(256) void k_reduce(const double *part, int n, doubl
 
 // FARGO type 1 background velocity (timedisc_base.f90:2243-2326, direction
 // 2): w(i) = SUM(v(i,JMIN:JMAX)*wfactor(i)) / JNUM for every ring, the sum in
-// index order.  A block covers 64 rings; waves 1-3 stage the rows of the
-// next chunk in LDS while wave 0 adds the current one sequentially.
-constexpr int WS_RINGS = 64, WS_ROWS = 64;
+// index order.  A block covers WS_RINGS = 16 rings; waves 1-3 stage the next
+// chunk of WS_ROWS rows in LDS (lane = ring + WS_RINGS * row group) while
+// wave 0 adds the current one sequentially, one chain per ring.  Few rings
+// per block and long chunks: more workgroups and fewer chunks, each chunk
+// costing one memory latency.
+constexpr int WS_RINGS = 16, WS_ROWS = 256, WS_RG = 64 / WS_RINGS;
 __global__ __launch_bounds__(256) void k_wsum(Geom g, const double *__restrict__ v, double *ring, float *ringf,
                                               const Ctl *ctl) {
   __shared__ double buf[2][WS_ROWS][WS_RINGS];
   if (ctl && ctl->halt) return;  // a device-controlled step after the round halted
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int i = blockIdx.x * WS_RINGS + lane, jmin = GH, J = g.jnum;
+  const int rl = lane % WS_RINGS, rg = lane / WS_RINGS;
+  const int i = blockIdx.x * WS_RINGS + rl, jmin = GH, J = g.jnum;
   const int nchunk = (J + WS_ROWS - 1) / WS_ROWS;
-  // waves 1..3: rows wv-1, wv+2, ... of chunk c into buf[c & 1], every load
-  // issued before the first store (one memory latency per chunk)
-  constexpr int RPW = (WS_ROWS + 2) / 3;
+  // waves 1..3 stage rows WS_RG*m + rg, m = wv-1, wv+2, ..., of chunk c into
+  // buf[c & 1] (each row exactly once), every load issued before the first
+  // store (one memory latency per chunk)
+  constexpr int RPW = (WS_ROWS / WS_RG + 2) / 3;
   auto stage = [&](int c) {
     double t[RPW];
 #pragma unroll
     for (int q = 0; q < RPW; q++) {
-      const int r = wv - 1 + 3 * q, j = c * WS_ROWS + r;
+      const int r = WS_RG * (wv - 1 + 3 * q) + rg, j = c * WS_ROWS + r;
       t[q] = (r < WS_ROWS && i < g.NI && j < J) ? v[(long)(jmin + j) * g.P + i] : 0.0;
     }
 #pragma unroll
     for (int q = 0; q < RPW; q++) {
-      const int r = wv - 1 + 3 * q;
-      if (r < WS_ROWS) buf[c & 1][r][lane] = t[q];
+      const int r = WS_RG * (wv - 1 + 3 * q) + rg;
+      if (r < WS_ROWS) buf[c & 1][r][rl] = t[q];
     }
   };
   if (wv > 0) stage(0);
@@ -125,13 +130,14 @@ __global__ __launch_bounds__(256) void k_wsum(Geom g, const double *__restrict__
       if (c + 1 < nchunk) stage(c + 1);
     } else {
       const int rows = J - c * WS_ROWS < WS_ROWS ? J - c * WS_ROWS : WS_ROWS;
-      // acc + v*f term by term in row order; the LDS reads and products of
-      // 16 rows are issued ahead of their adds
-      acc = ordered_sum<16>(acc, rows, [&](int r) { return buf[c & 1][r][lane] * f; });
+      // acc + v*f term by term in row order (lanes from WS_RINGS on repeat
+      // the first WS_RINGS);
+      // the LDS reads and products of 16 rows are issued ahead of their adds
+      acc = ordered_sum<16>(acc, rows, [&](int r) { return buf[c & 1][r][rl] * f; });
     }
     __syncthreads();
   }
-  if (wv == 0 && i < g.NI) {
+  if (wv == 0 && rg == 0 && i < g.NI) {
     const double w = acc / (double)J;
     ring[(long)R_W * g.NI + i] = w;
     if (ringf) ringf[(long)R_W * g.NI + i] = (float)w;
