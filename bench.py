@@ -7,7 +7,7 @@ CFL 0.3, van Leer, KT, point-mass gravity, FARGO, absorbing/periodic
 boundaries.  A "step" is one fosite Step() (one accepted time step of the
 whole grid).  Inputs are resident in HBM before the timed region.
 
-Prints one JSON line (see DESIGN.md for the roofline accounting):
+Prints one JSON line:
   value      cell-updates/s (interior cells x accepted steps / device time)
   roofline   fused RHS kernel: algorithmic bytes per launch / mean launch time
   cpu_baseline  the compiled reference (oracle/_ref/fv_driver), or the C
@@ -318,8 +318,8 @@ def main():
                     help="run one form alone and print its object (for profiling)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build measures one GPU; the FV step shards by radial slabs "
-              "(see DESIGN.md) but no multi-GPU run is part of it", file=sys.stderr)
+        print("bench.py: measures one GPU only; the FV step would shard by radial slabs "
+              "with ghost-ring exchange, but no multi-GPU run is implemented", file=sys.stderr)
         return 2
     if a.only_form:
         f = {"precision": lambda: precision_form(1024, 2048, a.form_steps), "batched": batched_form,
