@@ -36,7 +36,9 @@ class FvConfig(ctypes.Structure):
                 ("dtlimit", ctypes.c_double), ("beta", ctypes.c_double),
                 ("fargo", ctypes.c_int), ("gravity", ctypes.c_int),
                 ("mass", ctypes.c_double), ("GN", ctypes.c_double), ("dtype", ctypes.c_int),
-                ("outbound", ctypes.c_int), ("acclimit", ctypes.c_double)]
+                ("outbound", ctypes.c_int), ("acclimit", ctypes.c_double),
+                ("pmin", ctypes.c_double), ("rhomin", ctypes.c_double),
+                ("count", ctypes.c_int), ("starttime", ctypes.c_double)]
 
     @classmethod
     def from_dict(cls, cfg):
@@ -167,6 +169,30 @@ def _ptr(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
+def _sized(name, a, shape):
+    """a as a contiguous FP64 array of exactly shape.size values (any shape
+    with that many values is the same memory layout), else ValueError."""
+    x = np.ascontiguousarray(a, dtype=np.float64)
+    n = int(np.prod(shape))
+    if x.size != n:
+        raise ValueError("%s: %d values given, %d expected (shape %s)" % (name, x.size, n, shape))
+    return x
+
+
+def init_arrays(V, NJ, NI, bccsound=None, fcsound=None, w=None, pvar=None):
+    """The initial-data arrays of fv_set_sound_speeds, fv_set_background_velocity
+    and fv_set_pvar, checked against the sizes the C ABI reads: bccsound NJ*NI,
+    fcsound 4*NJ*NI (both or neither), w NI, pvar V*NJ*NI.  Returns
+    (bccsound, fcsound, w, pvar), None where not given."""
+    if (bccsound is None) != (fcsound is None):
+        raise ValueError("bccsound and fcsound go together")
+    b = None if bccsound is None else _sized("bccsound", bccsound, (NJ, NI))
+    f = None if fcsound is None else _sized("fcsound", fcsound, (4, NJ, NI))
+    ww = None if w is None else _sized("w", w, (NI,))
+    p = None if pvar is None else _sized("pvar", pvar, (V, NJ, NI))
+    return b, f, ww, p
+
+
 def mesh_array(cfg, name):
     """Host-side mesh metric array (no GPU needed), shape (NJ, NI)."""
     ids = ["dAx_w", "dAx_e", "dAy_s", "dAy_n", "dydzdV", "dzdxdV", "dxdydV", "dlx", "dly",
@@ -212,18 +238,17 @@ class Fosite:
         self.close()
 
     def set_initial(self, pvar, bccsound=None, fcsound=None, w=None):
-        """InitData: primitive state (V, NJ, NI), isothermal sound speeds,
-        FARGO background velocity (NI,)."""
-        if bccsound is not None:
-            b = np.ascontiguousarray(bccsound, dtype=np.float64)
-            f = np.ascontiguousarray(fcsound, dtype=np.float64)
+        """InitData: primitive state (V, NJ, NI), isothermal sound speeds
+        at the bary-centres (NJ, NI) and the faces (4, NJ, NI), FARGO
+        background velocity (NI,).  Arrays of another size raise ValueError
+        before anything reaches the library."""
+        b, f, ww, p = init_arrays(self.V, self.NJ, self.NI, bccsound, fcsound,
+                                  w if self.cfg.get("fargo", 0) == 2 else None, pvar)
+        if b is not None:
             self._chk(self.L.fv_set_sound_speeds(self._h, _ptr(b), _ptr(f)), "fv_set_sound_speeds")
-        if w is not None and self.cfg.get("fargo", 0) == 2:
-            ww = np.ascontiguousarray(w, dtype=np.float64)
+        if ww is not None:
             self._chk(self.L.fv_set_background_velocity(self._h, _ptr(ww)),
                       "fv_set_background_velocity")
-        p = np.ascontiguousarray(pvar, dtype=np.float64)
-        assert p.shape == (self.V, self.NJ, self.NI), p.shape
         self._chk(self.L.fv_set_pvar(self._h, _ptr(p)), "fv_set_pvar")
 
     def first_step(self):
@@ -408,16 +433,15 @@ class Server:
         returned by setups.initial); returns the request id."""
         cfg = setups.full(cfg)
         c = FvConfig.from_dict(cfg)
-        arrs = []
-        for key in ("pvar", "bccsound", "fcsound", "w"):
-            a = init.get(key)
-            if key == "w" and cfg.get("fargo", 0) != 2:
-                a = None
-            arrs.append(None if a is None else np.ascontiguousarray(a, dtype=np.float64))
-        ptrs = [None if a is None else _ptr(a) for a in arrs]
+        V, NJ, NI = 4 if cfg["physics"] == setups.EULER else 3, cfg["jnum"] + 4, cfg["inum"] + 4
+        if init.get("pvar") is None:
+            raise ValueError("pvar is required")
+        b, f, ww, p = init_arrays(V, NJ, NI, init.get("bccsound"), init.get("fcsound"),
+                                  init.get("w") if cfg.get("fargo", 0) == 2 else None, init["pvar"])
+        ptrs = [None if a is None else _ptr(a) for a in (p, b, f, ww)]
         i = ctypes.c_int()
         self._chk(self.L.fv_server_submit(self._h, ctypes.byref(c), *ptrs, ctypes.byref(i)), "fv_server_submit")
-        self.shapes[i.value] = (4 if cfg["physics"] == setups.EULER else 3, cfg["jnum"] + 4, cfg["inum"] + 4)
+        self.shapes[i.value] = (V, NJ, NI)
         return i.value
 
     def step(self):

@@ -41,7 +41,8 @@ struct Tableau {
 };
 
 // Butcher tableaux: timedisc_ssprk.f90:93-126, timedisc_rkfehlberg.f90:305-335,
-// timedisc_dormand_prince.f90:91-114 (same rational literals, FP64)
+// timedisc_dormand_prince.f90:91-114, timedisc_cashkarp.f90:101-113 (same
+// rational literals, FP64)
 bool set_tableau(int method, int order, Tableau &t) {
   std::memset(&t, 0, sizeof(t));
   if (method == SSPRK && order == 3) {
@@ -88,6 +89,18 @@ bool set_tableau(int method, int order, Tableau &t) {
     for (int i = 0; i < 7; i++) for (int j = 0; j < 7; j++) t.a[i][j] = a[i][j];
     double bh[] = {35. / 384., 0., 500. / 1113., 125. / 192., -2187. / 6784., 11. / 84., 0.};
     double bl[] = {5179. / 57600., 0., 7571. / 16695., 393. / 640., -92097. / 339200., 187. / 2100., 1. / 40.};
+    std::memcpy(t.bh, bh, sizeof(bh)); std::memcpy(t.bl, bl, sizeof(bl));
+  } else if (method == CASH_KARP && order == 5) {  // timedisc_cashkarp.f90:101-113
+    t.m = 6;
+    double a[6][6] = {{0, 0, 0, 0, 0, 0},
+                      {0.2, 0, 0, 0, 0, 0},
+                      {0.075, 0.225, 0, 0, 0, 0},
+                      {0.3, -0.9, 1.2, 0, 0, 0},
+                      {-11.0 / 54.0, 2.5, -70.0 / 27.0, 35.0 / 27.0, 0, 0},
+                      {1631.0 / 55296.0, 175.0 / 512.0, 575.0 / 13824.0, 44275.0 / 110592.0, 253.0 / 4096.0, 0}};
+    for (int i = 0; i < 6; i++) for (int j = 0; j < 6; j++) t.a[i][j] = a[i][j];
+    double bh[] = {37.0 / 378.0, 0.0, 250.0 / 621.0, 125.0 / 594.0, 0.0, 512.0 / 1771.0};
+    double bl[] = {2825.0 / 27648.0, 0.0, 18575.0 / 48384.0, 13525.0 / 55296.0, 277.0 / 14336.0, 0.25};
     std::memcpy(t.bh, bh, sizeof(bh)); std::memcpy(t.bl, bl, sizeof(bl));
   } else {
     return false;
@@ -154,6 +167,7 @@ struct fv_ctx {
   int fl_pvar = 0, fl_cvar = 0, fl_ptmp = 0, fl_ctmp = 0, fl_cold = 0;
   double time = 0, dt = 0, dtold = 0, dtmin = 0, maxerrold = 0, t_out = 0, dtlimit = 0;
   int iter = -1, n_adj = 0, brk = 0, dtcause = 0;
+  int count = 1;  // datafile outputs between the start and the stop time
   bool have_cs = false, have_w = false, have_state = false;
   double cs_min = 0;
   double last_ms = 0;
@@ -956,7 +970,10 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
   g.NI = ctx->NI; g.NJ = ctx->NJ; g.P = ctx->P; g.FS = ctx->FS;
   g.inum = c.inum; g.jnum = c.jnum;
   g.invdx = ctx->mesh.invdx; g.invdy = ctx->mesh.invdy;
-  g.gamma = c.gamma; g.pmin = DBL_MIN; g.rhomin = DBL_MIN;
+  // CheckData limits and the Euler pressure floor (timedisc_base.f90:353-359)
+  g.gamma = c.gamma;
+  g.pmin = c.pmin > 0 ? c.pmin : DBL_MIN;
+  g.rhomin = c.rhomin > 0 ? c.rhomin : DBL_MIN;
   g.limiter = c.limiter; g.theta = c.theta;
   g.recon_prim = c.recvars == 1;
   for (int d = 0; d < 4; d++) g.bc[d] = c.bc[d];
@@ -1004,12 +1021,16 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
     }
     if (!up) { fv_destroy(ctx); return -3; }
   }
-  // timedisc_base.f90:315-352 defaults
-  ctx->time = 0.0;
+  // timedisc_base.f90:295-352 defaults
+  ctx->time = c.starttime;
   ctx->dt = c.stoptime;
   ctx->dtold = ctx->dtmin = c.stoptime;
   ctx->dtlimit = c.dtlimit > 0 ? c.dtlimit : DBL_EPSILON * c.stoptime;
-  ctx->t_out = c.stoptime;  // one output at the stop time
+  // the output clock (io/fileio_base.f90:474-487, 591-600) starts at the
+  // start time; one IncTime happens before the first Step (in InitFileIO
+  // after a restart, after FirstStep's initial dataset otherwise)
+  ctx->count = c.count > 0 ? c.count : 1;
+  ctx->t_out = ctx->time + std::fabs(c.stoptime - ctx->time) / ctx->count;
   ctx->iter = -1;
   *out = ctx;
   return 0;
@@ -1190,7 +1211,7 @@ extern "C" int fv_step(fv_ctx *ctx, int *done) {
       if (done) *done = 1;
       return 0;
     }
-    ctx->t_out = ctx->t_out + std::fabs(ctx->c.stoptime - 0.0) / 1;
+    ctx->t_out = ctx->t_out + std::fabs(ctx->c.stoptime - ctx->c.starttime) / ctx->count;  // IncTime
     ctx->dtmin = ctx->c.stoptime;
     ctx->n_adj = 0;
   }

@@ -255,13 +255,18 @@ def config_entries(cfg, filename=""):
           ("stoptime", stop), ("tol_rel", float(cfg["tol_rel"])), ("beta", float(cfg["beta"])),
           ("maxiter", 100000000), ("tol_abs", np.array([float(x) for x in cfg["tol_abs"]][:V]))]
     # a given dtlimit is the setup's value; otherwise the default follows starttime
-    if cfg.get("dtlimit", -1.0) > 0:
-        td += [("dtlimit", float(cfg["dtlimit"])), ("starttime", 0.0)]
-    else:
-        td += [("starttime", 0.0), ("dtlimit", float(np.finfo(np.float64).eps * stop))]
-    td += [("dtmax", 5.0), ("rhstype", 0), ("always_update_bccsound", 1), ("checkdata", CHECK_ALL)]
+    # keys the setup gives come first (in the order a setup program sets
+    # them), the defaults InitTimedisc fills in follow in its call order
+    start = float(cfg.get("starttime", 0.0))
     tiny = float(np.finfo(np.float64).tiny)
-    td += [("pmin", tiny), ("tmin", tiny), ("rhomin", tiny)]
+    given = dict(dtlimit=float(cfg.get("dtlimit", -1.0)), pmin=float(cfg.get("pmin", -1.0)),
+                 rhomin=float(cfg.get("rhomin", -1.0)), starttime=start)
+    setup = {k: v for k, v in given.items() if (v != 0.0 if k == "starttime" else v > 0)}
+    td += list(setup.items())
+    defaults = [("starttime", 0.0), ("dtlimit", float(np.finfo(np.float64).eps * stop)), ("dtmax", 5.0),
+                ("rhstype", 0), ("always_update_bccsound", 1), ("checkdata", CHECK_ALL), ("pmin", tiny),
+                ("tmin", tiny), ("rhomin", tiny)]
+    td += [(k, v) for k, v in defaults if k not in setup]
     pn, cn = PVAR_NAMES[cfg["physics"]], CVAR_NAMES[cfg["physics"]]
     td += [("output/" + pn[0], 1), ("output/error", 0), ("output/solution", 0), ("output/time", 1)]
     for l in range(V):
@@ -271,7 +276,8 @@ def config_entries(cfg, filename=""):
         if not l:
             td += [("output/" + k, 0) for k in ("geometrical_sources", "external_sources", "rhs", "fluxes")]
     td += [("output/bflux", 0), ("ShowButcherTableau", 0)]
-    dfile = [("fileformat", BINARY), ("filename", filename), ("count", 1), ("filepath", ""),
+    count = int(cfg.get("count", 1))
+    dfile = [("fileformat", BINARY), ("filename", filename), ("count", count if count > 0 else 1), ("filepath", ""),
              ("filecycles", 2), ("unit", 11), ("dtwall", 3600), ("stoptime", stop),
              ("walltime", float(np.finfo(np.float64).max)), ("step", 0), ("cartcoords", 1)]
     e = [("/config", None)]

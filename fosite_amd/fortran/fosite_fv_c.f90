@@ -29,6 +29,9 @@ MODULE fosite_fv_c
     INTEGER(c_int) :: dtype = 0
     INTEGER(c_int) :: outbound = 0
     REAL(c_double) :: acclimit = -1
+    REAL(c_double) :: pmin = -1, rhomin = -1
+    INTEGER(c_int) :: count = 1
+    REAL(c_double) :: starttime = 0
   END TYPE fv_config
 
   TYPE, BIND(C), PUBLIC :: fv_scalars

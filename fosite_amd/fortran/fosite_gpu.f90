@@ -27,7 +27,7 @@ MODULE fosite_gpu_mod
   INTEGER, PARAMETER, PUBLIC :: MINMOD = 1, MONOCENT = 2, SWEBY = 3, SUPERBEE = 4, &
        OSPRE = 5, VANLEER = 7, NOLIMIT = 8
   INTEGER, PARAMETER, PUBLIC :: PRIMITIVE = 1, CONSERVATIVE = 2
-  INTEGER, PARAMETER, PUBLIC :: RK_FEHLBERG = 2, DORMAND_PRINCE = 4, SSPRK = 5
+  INTEGER, PARAMETER, PUBLIC :: RK_FEHLBERG = 2, CASH_KARP = 3, DORMAND_PRINCE = 4, SSPRK = 5
 
   TYPE, PUBLIC :: mesh_gpu
     INTEGER  :: geometry = CARTESIAN, inum = 32, jnum = 32
@@ -66,6 +66,10 @@ MODULE fosite_gpu_mod
     INTEGER  :: method = SSPRK, order = 5, fargo = 0
     REAL(DP) :: cfl = 0.4_DP, tol_rel = 0.01_DP, tol_abs(4) = 0.001_DP
     REAL(DP) :: stoptime = 1, dtlimit = -1, beta = 0.08_DP
+    !> CheckData limits (<= 0: TINY) and start time (timedisc_base.f90:295, 353-359)
+    REAL(DP) :: pmin = -1, rhomin = -1, starttime = 0
+    !> outputs between starttime and stoptime (fileio_base.f90:420-470)
+    INTEGER  :: count = 1
     !> scalars after the last call, as Timedisc holds them
     REAL(DP) :: time = 0, dt = 0, dtold = 0, maxerrold = 0
     INTEGER  :: n_adj = 0
@@ -133,6 +137,10 @@ CONTAINS
     c%GN = this%Sources%GN
     c%outbound = this%Sources%outbound
     c%acclimit = this%Sources%acclimit
+    c%pmin = this%Timedisc%pmin
+    c%rhomin = this%Timedisc%rhomin
+    c%count = this%Timedisc%count
+    c%starttime = this%Timedisc%starttime
     c%dtype = 0
   END FUNCTION Config
 

@@ -30,12 +30,14 @@ PROGRAM fv_run
   INTEGER  :: k_fargo = 0, k_gravity = 0, k_units = 0
   REAL(DP) :: k_mass = 1, k_acclimit = -1
   INTEGER  :: k_outbound = 0
-  INTEGER  :: k_nsteps = 10, k_dumpevery = 1, k_batch = 0
+  INTEGER  :: k_nsteps = 10, k_dumpevery = 1, k_batch = 0, k_count = 1
+  REAL(DP) :: k_pmin = -1, k_rhomin = -1, k_starttime = 0
   CHARACTER(LEN=512) :: k_infile = "", k_prefix = "out"
   NAMELIST /fvcfg/ k_physics, k_geometry, k_inum, k_jnum, k_gamma, k_xmin, k_xmax, k_ymin, k_ymax, &
        k_gparam, k_bcw, k_bce, k_bcs, k_bcn, k_limiter, k_recvars, k_theta, k_method, k_order,   &
        k_cfl, k_tolrel, k_tolabs, k_stoptime, k_dtlimit, k_beta, k_fargo, k_gravity, k_units,   &
-       k_mass, k_outbound, k_acclimit, k_nsteps, k_dumpevery, k_batch, k_infile, k_prefix
+       k_mass, k_outbound, k_acclimit, k_nsteps, k_dumpevery, k_batch, k_infile, k_prefix, &
+       k_count, k_pmin, k_rhomin, k_starttime
 
   TYPE(fosite_gpu) :: Sim
   CHARACTER(LEN=512) :: cfgname
@@ -79,6 +81,10 @@ PROGRAM fv_run
   Sim%Timedisc%dtlimit = k_dtlimit
   Sim%Timedisc%beta = k_beta
   Sim%Timedisc%fargo = k_fargo
+  Sim%Timedisc%pmin = k_pmin
+  Sim%Timedisc%rhomin = k_rhomin
+  Sim%Timedisc%starttime = k_starttime
+  Sim%Timedisc%count = k_count
   CALL Sim%Setup()
 
   ASSOCIATE(M => Sim%Mesh, V => Sim%Physics%VNUM)

@@ -15,6 +15,8 @@ A configuration is a plain dict whose keys follow the reference's Dict keys
   fargo     0 off | 1 per-ring mean velocity, every step | 2 user-supplied
             background velocity (mesh_base.f90:372-423)
   gravity, mass, GN                             (gravity_pointmass.f90:132-301)
+  pmin, rhomin, starttime                       (timedisc_base.f90:295, 353-359)
+  count     datafile outputs between starttime and stoptime (fileio_base.f90:420-470)
 
 Initial data is returned as primitive variables on all cells (ghosts
 included), the isothermal sound speeds at cell centres and faces, and the
@@ -38,7 +40,10 @@ DEFAULTS = dict(gamma=1.4, gparam=1.0, limiter=MINMOD, theta=1.0, recvars=CONSER
                 dtlimit=-1.0, beta=0.08, fargo=0, gravity=0, mass=1.0, GN=1.0,
                 # point-mass accretion off unless named: the reference's default
                 # on cylindrical meshes is WEST (gravity_pointmass.f90:285-299)
-                outbound=0, acclimit=-1.0)
+                outbound=0, acclimit=-1.0,
+                # CheckData limits (<= 0: TINY, timedisc_base.f90:353-359),
+                # datafile count, start time (timedisc_base.f90:295)
+                pmin=-1.0, rhomin=-1.0, count=1, starttime=0.0)
 
 
 def full(cfg):
@@ -223,9 +228,30 @@ def serving_requests(n=2000, rate=20.0, seed=20261205, nmin=64, nmax=1024, orbit
     return out
 
 
+def rarefaction_config(n=32, **kw):
+    """Two halves of the unit box moving apart (Euler): the centre empties
+    step by step, so the minimum density and pressure fall through any
+    chosen limit (the CheckData and pressure-floor workload)."""
+    return full(dict(riemann2d_config(n=n), problem="rarefaction", **kw))
+
+
+def rarefaction_initial(cfg, u0=2.0):
+    """rho = 1, p = 0.4, u = -u0 for x < 0.5 and +u0 beyond, v = 0.1 sin(2 pi y)."""
+    xc, yc, _, _ = coords(cfg)
+    X, Y = np.meshgrid(xc, yc)
+    pv = np.zeros((4,) + X.shape)
+    pv[0] = 1.0
+    pv[1] = np.where(X < 0.5, -u0, u0)
+    pv[2] = 0.1 * np.sin(2 * np.pi * Y)
+    pv[3] = 0.4
+    return dict(pvar=pv)
+
+
 def initial(cfg):
     if cfg.get("problem") == "euler_disk":
         return euler_disk_initial(cfg)
+    if cfg.get("problem") == "rarefaction":
+        return rarefaction_initial(cfg)
     if cfg["physics"] == EULER:
         return riemann2d_initial(cfg)
     return kepler_disk_initial(cfg)

@@ -39,7 +39,8 @@ typedef struct fv_config {
   double theta;     /* limiter parameter for mc and sweby */
   int recvars;      /* 1 PRIMITIVE, 2 CONSERVATIVE (fluxes/reconstruction_base.f90:117,
                        fluxes/fluxes_base.f90:378-384) */
-  int method, order;/* 2 RK_FEHLBERG, 4 DORMAND_PRINCE, 5 SSPRK (timedisc/timedisc_base.f90:187-191) */
+  int method, order;/* 2 RK_FEHLBERG (order 3, 5), 3 CASH_KARP (5), 4 DORMAND_PRINCE (5),
+                       5 SSPRK (3, 5) (timedisc/timedisc_base.f90:187-191) */
   double cfl, tol_rel, tol_abs[4], stoptime, dtlimit, beta; /* timedisc_base.f90:315-352;
                        dtlimit <= 0 selects the default EPSILON*stoptime */
   int fargo;        /* 0 off, 1 per-ring mean azimuthal velocity recomputed every step
@@ -58,6 +59,13 @@ typedef struct fv_config {
                        so a setup that relies on it sets 1 here. */
   double acclimit;  /* accretion limit per unit mass and time, < 0 off
                        (sources/gravity_pointmass.f90:190, 412-416) */
+  double pmin, rhomin; /* CheckData limits and the Euler pressure floor of ComputeRHS
+                       (timedisc/timedisc_base.f90:353-359, 1150-1164, 1666-1742);
+                       <= 0 selects the default TINY(1.0) */
+  int count;        /* datafile count: outputs at starttime + k*|stoptime - starttime|/count
+                       (io/fileio_base.f90:420-470, 487, 591-600); <= 0 selects 1 */
+  double starttime; /* Timedisc%time at setup (timedisc/timedisc_base.f90:295); a restart
+                       sets it to the dataset's time (examples/restart.f90:103-106) */
 } fv_config;
 
 typedef struct fv_scalars {

@@ -23,7 +23,7 @@ def build(force=False):
     src = os.path.join(HERE, "fvref.c")
     if not force and os.path.exists(LIB) and os.path.getmtime(LIB) >= os.path.getmtime(src):
         return LIB
-    subprocess.check_call(["gcc", "-O2", "-ffp-contract=off", "-fno-fast-math", "-fPIC",
+    subprocess.check_call(["gcc", "-O3", "-ffp-contract=off", "-fno-fast-math", "-fopenmp", "-fPIC",
                            "-shared", "-o", LIB, src, "-lm"])
     return LIB
 
@@ -41,7 +41,9 @@ class _Cfg(ctypes.Structure):
                 ("dtlimit", ctypes.c_double), ("beta", ctypes.c_double),
                 ("fargo", ctypes.c_int), ("gravity", ctypes.c_int),
                 ("mass", ctypes.c_double), ("GN", ctypes.c_double),
-                ("outbound", ctypes.c_int), ("acclimit", ctypes.c_double)]
+                ("outbound", ctypes.c_int), ("acclimit", ctypes.c_double),
+                ("pmin", ctypes.c_double), ("rhomin", ctypes.c_double),
+                ("count", ctypes.c_int), ("starttime", ctypes.c_double)]
 
 
 class _Scalars(ctypes.Structure):
@@ -122,6 +124,10 @@ class Oracle:
         c.GN = cfg.get("GN", 1.0)
         c.outbound = cfg.get("outbound", 0)
         c.acclimit = cfg.get("acclimit", -1.0)
+        c.pmin = cfg.get("pmin", -1.0)
+        c.rhomin = cfg.get("rhomin", -1.0)
+        c.count = cfg.get("count", 1)
+        c.starttime = cfg.get("starttime", 0.0)
         self._h = L.fvr_create(ctypes.byref(c))
         if not self._h:
             raise ValueError("oracle: unsupported configuration")

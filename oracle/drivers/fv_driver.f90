@@ -33,19 +33,21 @@ PROGRAM fv_driver
   INTEGER :: k_fargo = 0, k_gravity = 0, k_units = 0
   REAL    :: k_mass = 1.0, k_acclimit = -1.0
   INTEGER :: k_outbound = 0
-  INTEGER :: k_binary = 0
+  INTEGER :: k_binary = 0, k_count = 1
+  REAL    :: k_pmin = -1.0, k_rhomin = -1.0, k_starttime = 0.0
   INTEGER :: k_nsteps = 10, k_dumpevery = 1
   CHARACTER(LEN=512) :: k_infile = "", k_prefix = "out"
   NAMELIST /fvcfg/ k_physics, k_geometry, k_inum, k_jnum, k_gamma, k_xmin, k_xmax, k_ymin, k_ymax, &
        k_gparam, k_bcw, k_bce, k_bcs, k_bcn, k_limiter, k_recvars, k_theta, k_method, k_order,   &
        k_cfl, k_tolrel, k_tolabs, k_stoptime, k_dtlimit, k_beta, k_fargo, k_gravity, k_units,   &
-       k_mass, k_outbound, k_acclimit, k_binary, k_nsteps, k_dumpevery, k_infile, k_prefix
+       k_mass, k_outbound, k_acclimit, k_binary, k_nsteps, k_dumpevery, k_infile, k_prefix, &
+       k_count, k_pmin, k_rhomin, k_starttime
   !--------------------------------------------------------------------------!
   CLASS(fosite), ALLOCATABLE :: Sim
   TYPE(Dict_TYP), POINTER    :: config, mesh, phys, bnd, flx, tdisc, dfile, &
                                 src, pmass
   CHARACTER(LEN=512)         :: cfgname
-  INTEGER                    :: u, ub, uw, ug, n, vnum, ncell
+  INTEGER                    :: u, ub, uw, ug, us, n, vnum, ncell
   INTEGER(KIND=8)            :: c0, c1, crate, ctot
   LOGICAL                    :: brk
   REAL, ALLOCATABLE          :: pv(:), cs(:), fcs(:), w(:)
@@ -82,13 +84,16 @@ PROGRAM fv_driver
     CALL SetAttr(tdisc, "tol_abs", k_tolabs(1:3))
   END IF
   IF (k_dtlimit.GT.0.0) CALL SetAttr(tdisc, "dtlimit", k_dtlimit)
+  IF (k_pmin.GT.0.0) CALL SetAttr(tdisc, "pmin", k_pmin)
+  IF (k_rhomin.GT.0.0) CALL SetAttr(tdisc, "rhomin", k_rhomin)
+  IF (k_starttime.NE.0.0) CALL SetAttr(tdisc, "starttime", k_starttime)
   ! k_binary = 1: the reference's own binary datasets (io/fileio_binary.f90)
   IF (k_binary.EQ.1) THEN
     dfile => Dict("fileformat" / BINARY, "filename" / (TRIM(k_prefix) // "_data"), &
-         "count" / 1)
+         "count" / k_count)
   ELSE
     dfile => Dict("fileformat" / GNUPLOT, "filename" / (TRIM(k_prefix) // "_data"), &
-         "count" / 1)
+         "count" / k_count)
   END IF
   config => Dict("mesh" / mesh, "physics" / phys, "boundary" / bnd, &
        "fluxes" / flx, "timedisc" / tdisc, "datafile" / dfile)
@@ -135,14 +140,18 @@ PROGRAM fv_driver
        FORM="unformatted", STATUS="replace")
   OPEN(NEWUNIT=ug, FILE=TRIM(k_prefix) // "_grav.bin", ACCESS="stream", &
        FORM="unformatted", STATUS="replace")
+  OPEN(NEWUNIT=us, FILE=TRIM(k_prefix) // "_scal.bin", ACCESS="stream", &
+       FORM="unformatted", STATUS="replace")
   CALL Sim%FirstStep()
   CALL DumpState(u, 0)
+  CALL DumpScalars(us, 0)
   ctot = 0
   DO n = 1, k_nsteps
     CALL SYSTEM_CLOCK(c0, crate)
     brk = Sim%Step()
     CALL SYSTEM_CLOCK(c1)
     ctot = ctot + (c1 - c0)
+    CALL DumpScalars(us, n)
     IF (MOD(n, k_dumpevery).EQ.0 .OR. n.EQ.k_nsteps .OR. brk .OR. Sim%Timedisc%break) &
       CALL DumpState(u, n)
     IF (brk .OR. Sim%Timedisc%break) EXIT
@@ -151,6 +160,7 @@ PROGRAM fv_driver
   CLOSE(ub)
   CLOSE(uw)
   CLOSE(ug)
+  CLOSE(us)
   ! wall time of the Step() calls only (setup, FirstStep and dumps excluded)
   WRITE(*,'(A,I0,A,ES24.16)') "FV_DRIVER_STEP_SECONDS ", MIN(n, k_nsteps), " ", &
        REAL(ctot, 8) / REAL(crate, 8)
@@ -187,6 +197,23 @@ CONTAINS
     IF (k_fargo.GT.0) WRITE(uw) Sim%Timedisc%w(:,1)
     IF (k_gravity.GT.0) CALL DumpPointMass(ug)
   END SUBROUTINE DumpState
+
+  !> the scalars of every step (a small side file, also between the state
+  !! dumps): int32 step, iter, n_adj, break, float64 time, dt, dtold, maxerrold
+  SUBROUTINE DumpScalars(u, n)
+    INTEGER, INTENT(IN) :: u, n
+    INTEGER(KIND=4)     :: ibuf(4)
+    REAL(KIND=8)        :: rbuf(4)
+    ibuf(1) = n
+    ibuf(2) = Sim%iter
+    ibuf(3) = Sim%Timedisc%n_adj
+    ibuf(4) = MERGE(1, 0, Sim%Timedisc%break)
+    rbuf(1) = Sim%Timedisc%time
+    rbuf(2) = Sim%Timedisc%dt
+    rbuf(3) = Sim%Timedisc%dtold
+    rbuf(4) = Sim%Timedisc%maxerrold
+    WRITE(u) ibuf, rbuf
+  END SUBROUTINE DumpScalars
 
   !> point-mass state (gravity_pointmass.f90:79-85): mass, accrate, massloss,
   !! mdot, then the acceleration accel(:,:,:,1:2) of the point mass (all cells)

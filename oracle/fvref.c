@@ -9,10 +9,16 @@
  *
  * Scope: 2-D meshes (knum = 1), midpoint mesh, cartesian and logcylindrical
  * geometry, Euler and isothermal Euler physics, KT fluxes with linear
- * reconstruction (all non-PP limiters), embedded RK (RKF 3/5, Cash-Karp,
- * Dormand-Prince, SSPRK 3/5) with error control and reject path, no-gradient,
- * periodic and absorbing boundaries, point-mass gravity with accretion,
- * FARGO with user-supplied background velocity (type 2).
+ * reconstruction (all non-PP limiters) of conservative or primitive
+ * variables, embedded RK (RKF 3/5, Cash-Karp 5, Dormand-Prince 5, SSPRK 3/5)
+ * with error control and reject path, no-gradient, periodic and absorbing
+ * boundaries, point-mass gravity with accretion, FARGO with the per-ring mean
+ * (type 1) or a user-supplied (type 2) background velocity, CheckData breaks,
+ * output times (datafile count) and a start time.
+ *
+ * Every function is pinned by the golden fixtures of the compiled reference
+ * (tests/golden/fv_golden.npz, tests/test_oracle_golden.py) or by the
+ * invariants tested there.
  */
 #include "fvref.h"
 
@@ -70,10 +76,21 @@ struct fvr {
   double *prim, *cons, *pfluxes, *minwav, *maxwav, *tmp, *tmp1, *slopes;
   double time, dt, dtold, dtmin, maxerrold, pmin, rhomin;
   int dtcause, n_adj, iter, brk;
-  double t_out; int out_count; /* Datafile output time (count = 1) */
+  double t_out; int out_count; /* Datafile output time and count */
 };
 
 #define IDX(s, i, j) ((j) * (s)->NI + (i))
+
+/* Loops over cells whose iterations are independent run on OpenMP threads
+ * on meshes large enough to pay for it.  Every value is computed by the same
+ * expression as in a serial run, so the results do not depend on the number
+ * of threads; reductions (sums in index order, the min/max scans) stay serial
+ * or are per ring. */
+#ifdef _OPENMP
+#define PAR_FOR _Pragma("omp parallel for schedule(static) if (s->N >= 32768)")
+#else
+#define PAR_FOR
+#endif
 
 static double *zalloc(size_t n) { return (double *)calloc(n ? n : 1, sizeof(double)); }
 
@@ -199,10 +216,12 @@ static inline double sound_speed(double gamma, double rho, double p) {
 
 /* Convert2Primitive/Conservative on an i,j box of a cell-centred vector */
 static void c2p_box(const fvr *s, double *p, const double *c, int i1, int i2, int j1, int j2) {
+  PAR_FOR
   for (int j = j1; j <= j2; j++)
     for (int i = i1; i <= i2; i++) cons2prim_pt(s, p, c, s->N, IDX(s, i, j));
 }
 static void p2c_box(const fvr *s, double *c, const double *p, int i1, int i2, int j1, int j2) {
+  PAR_FOR
   for (int j = j1; j <= j2; j++)
     for (int i = i1; i <= i2; i++) prim2cons_pt(s, c, p, s->N, IDX(s, i, j));
 }
@@ -216,6 +235,7 @@ static void update_bccsound(fvr *s, const double *p) {
 static void add_bg_velocity(fvr *s, sv *p, sv *c) {
   if (!c->flag) return;
   if (!p->flag) return; /* "should not happen" branch, kept as in the reference */
+  PAR_FOR
   for (int j = 0; j < s->NJ; j++)
     for (int i = 0; i < s->NI; i++) {
       int k = IDX(s, i, j);
@@ -228,6 +248,7 @@ static void add_bg_velocity(fvr *s, sv *p, sv *c) {
 static void sub_bg_velocity(fvr *s, sv *p, sv *c) {
   if (c->flag) return;
   if (p->flag) return;
+  PAR_FOR
   for (int j = 0; j < s->NJ; j++)
     for (int i = 0; i < s->NI; i++) {
       int k = IDX(s, i, j);
@@ -411,6 +432,7 @@ static void center_boundary(fvr *s, sv *p, sv *c) {
 static void geometrical_sources(fvr *s, const double *p, const double *c, double *st) {
   int N = s->N, V = s->V;
   if (s->c.geometry == CARTESIAN) return;
+  PAR_FOR
   for (int k = 0; k < N; k++) {
     double vx = p[1 * N + k], vy = p[2 * N + k], vz = 0.0, mx = c[1 * N + k], my = c[2 * N + k],
            mz = 0.0;
@@ -424,6 +446,7 @@ static void geometrical_sources(fvr *s, const double *p, const double *c, double
     st[2 * N + k] = mz * (s->czyz[k] * vz - 0.0 * vy) + mx * (s->cxyx[k] * vx - s->cyxy[k] * vy) +
                     (s->cxyx[k] + s->czyz[k]) * P;
   }
+  PAR_FOR
   for (int j = 0; j < s->NJ; j++)
     for (int i = 0; i < s->NI; i++)
       if (i < G || i >= G + s->c.inum || j < G || j >= G + s->c.jnum)
@@ -460,6 +483,7 @@ static void update_gravity(fvr *s, double dt) {
   s->pm_mass = s->pm_mass + dmass;
   if (dt > 0.0) s->pm_accrate = dmass / dt;
   const double massfac = s->pm_mass / oldmass;
+  PAR_FOR
   for (int k = 0; k < s->N; k++) {
     s->accel[0][k] = s->accel[0][k] * massfac;
     s->accel[1][k] = s->accel[1][k] * massfac;
@@ -474,14 +498,18 @@ static void external_sources(fvr *s, const double *c, double *st, double dt) {
   int N = s->N, V = s->V;
   if (!s->c.gravity) return;
   if (dt >= 0.0) update_gravity(s, dt);
+  PAR_FOR
   for (int k = 0; k < V * N; k++) st[k] = 0.0;
+  PAR_FOR
   for (int k = 0; k < N; k++) {
     s->sterm[k] = 0.0;
     s->sterm[1 * N + k] = c[k] * s->accel[0][k];
     s->sterm[2 * N + k] = c[k] * s->accel[1][k];
     if (V == 4) s->sterm[3 * N + k] = c[1 * N + k] * s->accel[0][k] + c[2 * N + k] * s->accel[1][k];
   }
+  PAR_FOR
   for (int k = 0; k < V * N; k++) st[k] = st[k] + s->sterm[k];
+  PAR_FOR
   for (int j = 0; j < s->NJ; j++)
     for (int i = 0; i < s->NI; i++)
       if (i < G || i >= G + s->c.inum || j < G || j >= G + s->c.jnum)
@@ -491,6 +519,7 @@ static void external_sources(fvr *s, const double *c, double *st, double dt) {
 /* physics_eulerisotherm.f90:1531-1586 (2-D x-y momentum) */
 static void add_fargo_sources(fvr *s, const double *c, double *st) {
   int N = s->N;
+  PAR_FOR
   for (int j = G; j < G + s->c.jnum; j++)
     for (int i = G; i < G + s->c.inum; i++) {
       int k = IDX(s, i, j);
@@ -541,15 +570,18 @@ static void calculate_states(fvr *s, const double *rv, double *rs) {
   for (int l = 0; l < V; l++) {
     const double *u = rv + (size_t)l * N;
     double *sx = s->slopes + ((size_t)l * 2 + 0) * N, *sy = s->slopes + ((size_t)l * 2 + 1) * N;
+    PAR_FOR
     for (int j = 0; j < s->NJ; j++)
       for (int i = 1; i < NI - 1; i++) {
         int k = IDX(s, i, j);
         sx[k] = s->invdx * slope_lim(s, u[k - 1], u[k], u[k + 1]);
       }
+    PAR_FOR
     for (int k = NI; k < N - NI; k++) sy[k] = s->invdy * slope_lim(s, u[k - NI], u[k], u[k + NI]);
   }
   for (int l = 0; l < V; l++)
     for (int n = 0; n < 4; n++)
+      PAR_FOR
       for (int k = 0; k < N; k++)
         rs[((size_t)l * 4 + n) * N + k] =
             rv[(size_t)l * N + k] + s->slopes[((size_t)l * 2 + n / 2) * N + k] * s->rdx[n][k];
@@ -563,6 +595,7 @@ static void wave_speeds_faces(fvr *s) {
     const double *vE = s->prim + ((size_t)(1 + d) * 4 + 2 * d + 1) * N;
     const double *cW = s->fcsound + (size_t)(2 * d) * N, *cE = s->fcsound + (size_t)(2 * d + 1) * N;
     double *mn = s->minwav + (size_t)d * N, *mx = s->maxwav + (size_t)d * N;
+    PAR_FOR
     for (int k = 0; k < N; k++) {
       s->tmp[k] = fmin2(0., vW[k] - cW[k]);
       s->tmp1[k] = fmax2(0., vW[k] + cW[k]);
@@ -570,6 +603,7 @@ static void wave_speeds_faces(fvr *s) {
       mx[k] = fmax2(0., vE[k] + cE[k]);
     }
     if (d == 0) {
+      PAR_FOR
       for (int j = 0; j < s->NJ; j++)
         for (int i = G - 1; i <= G + s->c.inum - 1; i++) {
           int k = IDX(s, i, j);
@@ -577,6 +611,7 @@ static void wave_speeds_faces(fvr *s) {
           mx[k] = fmax2(fmax2(0.0, s->tmp1[k + 1]), mx[k]);
         }
     } else {
+      PAR_FOR
       for (int j = G - 1; j <= G + s->c.jnum - 1; j++)
         for (int i = 0; i < NI; i++) {
           int k = IDX(s, i, j);
@@ -594,6 +629,7 @@ static void physical_fluxes(fvr *s, int d) {
   int N = s->N, V = s->V;
   int un = 1 + d, ut = 2 - d; /* normal / tangential momentum index */
   for (int f = 2 * d; f <= 2 * d + 1; f++)
+    PAR_FOR
     for (int k = 0; k < N; k++) {
 #define PF(l) s->prim[((size_t)(l) * 4 + f) * N + k]
 #define CF(l) s->cons[((size_t)(l) * 4 + f) * N + k]
@@ -622,20 +658,24 @@ static void calculate_fluxes(fvr *s, const double *pv, const double *cv) {
   if (s->c.recvars == PRIMITIVE) {
     calculate_states(s, pv, s->prim);
     for (int f = 0; f < 4; f++)
+      PAR_FOR
       for (int k = 0; k < N; k++) prim2cons_pt(s, s->cons + (size_t)f * N, s->prim + (size_t)f * N, 4 * (size_t)N, k);
   } else {
     calculate_states(s, cv, s->cons);
     for (int f = 0; f < 4; f++)
+      PAR_FOR
       for (int k = 0; k < N; k++) cons2prim_pt(s, s->prim + (size_t)f * N, s->cons + (size_t)f * N, 4 * (size_t)N, k);
   }
   if (V == 4)
     for (int f = 0; f < 4; f++)
+      PAR_FOR
       for (int k = 0; k < N; k++)
         s->fcsound[(size_t)f * N + k] =
             sound_speed(s->c.gamma, s->prim[(size_t)f * N + k], s->prim[((size_t)3 * 4 + f) * N + k]);
   wave_speeds_faces(s);
   physical_fluxes(s, 0);
   for (int l = 0; l < V; l++)
+    PAR_FOR
     for (int j = 0; j < s->NJ; j++)
       for (int i = G - 1; i <= G + s->c.inum - 1; i++) {
         int k = IDX(s, i, j);
@@ -648,6 +688,7 @@ static void calculate_fluxes(fvr *s, const double *pv, const double *cv) {
       }
   physical_fluxes(s, 1);
   for (int l = 0; l < V; l++)
+    PAR_FOR
     for (int j = G - 1; j <= G + s->c.jnum - 1; j++)
       for (int i = 0; i < NI; i++) {
         int k = IDX(s, i, j);
@@ -697,7 +738,9 @@ static void compute_rhs(fvr *s, sv *p, sv *c, int checkbm, double *rhs, double d
   if (fargo) add_bg_velocity(s, p, c);
   center_boundary(s, p, c);
   if ((checkbm & CHECK_PMIN) && V == 4) {
+    PAR_FOR
     for (int k = 0; k < N; k++) p->d[3 * N + k] = fmax2(p->d[3 * N + k], s->pmin);
+    PAR_FOR
     for (int k = 0; k < N; k++) prim2cons_pt(s, c->d, p->d, N, k);
     c->flag = p->flag;
   }
@@ -725,6 +768,7 @@ static void compute_rhs(fvr *s, sv *p, sv *c, int checkbm, double *rhs, double d
   for (int l = 0; l < V; l++) {
     double *r = rhs + (size_t)l * N, *xf = s->xflux + (size_t)l * N, *yf = s->yflux + (size_t)l * N;
     const double *gs = s->geo_src + (size_t)l * N, *sr = s->src + (size_t)l * N;
+    PAR_FOR
     for (int j = jmin; j <= jmax; j++)
       for (int i = imin; i <= imax; i++) {
         int k = IDX(s, i, j);
@@ -756,8 +800,10 @@ static double calc_timestep(fvr *s) {
 static void compute_cvar(fvr *s, int m, double dt) {
   size_t n = (size_t)s->V * s->N;
   const double *a = s->a + (size_t)m * s->m;
+  PAR_FOR
   for (size_t k = 0; k < n; k++) s->ctmp.d[k] = s->cvar.d[k] - a[0] * dt * s->coeff[0][k];
   for (int mm = 1; mm < m; mm++)
+    PAR_FOR
     for (size_t k = 0; k < n; k++) s->ctmp.d[k] = s->ctmp.d[k] - a[mm] * dt * s->coeff[mm][k];
 }
 
@@ -767,6 +813,7 @@ static double compute_error(fvr *s) {
   double maxerr = 0.0;
   for (int l = 0; l < s->V; l++) {
     double e = -HUGE_;
+    PAR_FOR
     for (int k = 0; k < N; k++) {
       double yh = s->cvar.d[(size_t)l * N + k], yl = s->ctmp.d[(size_t)l * N + k];
       s->cerr[(size_t)l * N + k] = fabs(yh - yl) / (s->c.tol_rel * fabs(yh) + s->c.tol_abs[l]);
@@ -834,6 +881,7 @@ static void solve_ode(fvr *s, double time, double *dt, double *err) {
   memcpy(s->ctmp.d, s->cvar.d, sizeof(double) * n);
   s->ctmp.flag = s->cvar.flag;
   for (int m = 0; m < s->m; m++)
+    PAR_FOR
     for (size_t k = 0; k < n; k++) {
       s->ctmp.d[k] = s->ctmp.d[k] - s->b_low[m] * *dt * s->coeff[m][k];
       s->cvar.d[k] = s->cvar.d[k] - s->b_high[m] * *dt * s->coeff[m][k];
@@ -857,6 +905,7 @@ static void copy_sv(fvr *s, sv *dst, const sv *src) {
 static void calc_background_velocity(fvr *s) {
   const int N = s->N, jmin = G, jmax = G + s->c.jnum - 1;
   add_bg_velocity(s, &s->pvar, &s->cvar);
+  PAR_FOR
   for (int i = 0; i < s->NI; i++) {
     double wi = 0.0;
     for (int j = jmin; j <= jmax; j++) wi = wi + s->pvar.d[2 * N + IDX(s, i, j)] * s->wfac[i];
@@ -864,49 +913,47 @@ static void calc_background_velocity(fvr *s) {
   }
 }
 
-/* timedisc_base.f90:1935-2069 (2-D, direction 2) */
+/* timedisc_base.f90:1935-2069 (2-D, direction 2).  Every x-ring is
+ * independent: per ring the limited transport by the fractional shift over
+ * the whole column (dq, dql and flux indexed by j; dql(0) and the flux below
+ * JMIN-1 stay zero as in the full-array form), then CSHIFT(., -shift) over
+ * JMIN:JMAX. */
 static void fargo_advection_y(fvr *s) {
-  int N = s->N, NI = s->NI, jmin = G, jmax = G + s->c.jnum - 1;
-  double *delxy = zalloc(NI), *dq = zalloc(N), *dql = zalloc(N), *flux = zalloc(N);
-  long *shift = (long *)calloc(NI, sizeof(long));
-  double *buf = zalloc(s->c.jnum);
+  const int N = s->N, NI = s->NI, NJ = s->NJ, jmin = G, jmax = G + s->c.jnum - 1, J = s->c.jnum;
+  PAR_FOR
   for (int i = 0; i < NI; i++) {
-    delxy[i] = s->w[i] * s->dt / s->dly[IDX(s, i, jmin)];
-    shift[i] = lround(delxy[i]);
-    delxy[i] = delxy[i] - (double)shift[i];
-  }
-  for (int l = 0; l < s->V; l++) {
-    double *u = s->cvar.d + (size_t)l * N;
-    for (int i = 0; i < NI; i++) dq[IDX(s, i, 0)] = u[IDX(s, i, 1)] - u[IDX(s, i, 0)];
-    for (int j = 1; j <= s->NJ - 2; j++)
-      for (int i = 0; i < NI; i++) {
-        int k = IDX(s, i, j);
-        dq[k] = u[k + NI] - u[k];
-        if (fsign(1.0, dq[k - NI]) * fsign(1.0, dq[k]) > 0.)
-          dql[k] = fsign(fmin2(fabs(dq[k - NI]), fabs(dq[k])), dq[k - NI]);
-        else
-          dql[k] = 0.;
-      }
-    for (int j = jmin - 1; j <= jmax; j++)
-      for (int i = 0; i < NI; i++) {
-        int k = IDX(s, i, j);
-        if (delxy[i] > 0.)
-          flux[k] = u[k] + .5 * dql[k] * (1. - delxy[i]);
-        else
-          flux[k] = u[k + NI] - .5 * dql[k + NI] * (1. + delxy[i]);
-        u[k] = u[k] - delxy[i] * (flux[k] - flux[k - NI]);
-      }
-  }
-  /* CSHIFT(a, -shift) over JMIN:JMAX: result(j) = a(j - shift) cyclically */
-  for (int l = 0; l < s->V; l++)
-    for (int i = 0; i < NI; i++) {
+    double *dq = zalloc(NJ), *dql = zalloc(NJ), *flux = zalloc(NJ), *buf = zalloc(J);
+    double delxy = s->w[i] * s->dt / s->dly[IDX(s, i, jmin)];
+    const long shift = lround(delxy);
+    delxy = delxy - (double)shift;
+    for (int l = 0; l < s->V; l++) {
       double *u = s->cvar.d + (size_t)l * N;
-      int J = s->c.jnum;
-      long sh = ((-shift[i]) % J + J) % J;
+      dq[0] = u[IDX(s, i, 1)] - u[IDX(s, i, 0)];
+      for (int j = 1; j <= NJ - 2; j++) {
+        dq[j] = u[IDX(s, i, j + 1)] - u[IDX(s, i, j)];
+        if (fsign(1.0, dq[j - 1]) * fsign(1.0, dq[j]) > 0.)
+          dql[j] = fsign(fmin2(fabs(dq[j - 1]), fabs(dq[j])), dq[j - 1]);
+        else
+          dql[j] = 0.;
+      }
+      for (int j = jmin - 1; j <= jmax; j++) {
+        const int k = IDX(s, i, j);
+        if (delxy > 0.)
+          flux[j] = u[k] + .5 * dql[j] * (1. - delxy);
+        else
+          flux[j] = u[k + NI] - .5 * dql[j + 1] * (1. + delxy);
+        u[k] = u[k] - delxy * (flux[j] - flux[j - 1]);
+      }
+    }
+    /* CSHIFT(a, -shift) over JMIN:JMAX: result(j) = a(j - shift) cyclically */
+    const long sh = ((-shift) % J + J) % J;
+    for (int l = 0; l < s->V; l++) {
+      double *u = s->cvar.d + (size_t)l * N;
       for (int j = 0; j < J; j++) buf[j] = u[IDX(s, i, jmin + (int)((j + sh) % J))];
       for (int j = 0; j < J; j++) u[IDX(s, i, jmin + j)] = buf[j];
     }
-  free(delxy); free(dq); free(dql); free(flux); free(shift); free(buf);
+    free(dq); free(dql); free(flux); free(buf);
+  }
 }
 
 /* timedisc_base.f90:961-1009 */
@@ -961,6 +1008,7 @@ static void integration_step(fvr *s) {
     if (s->c.fargo) {
       center_boundary(s, &s->pvar, &s->cvar);
       fargo_advection_y(s);
+      PAR_FOR
       for (int k = 0; k < s->N; k++) cons2prim_pt(s, s->pvar.d, s->cvar.d, s->N, k);
       s->pvar.flag = s->cvar.flag;
       compute_rhs(s, &s->pvar, &s->cvar, CHECK_ALL, s->coeff[0], 0.0);
@@ -1005,12 +1053,23 @@ static int set_tableau(fvr *s) {
   static const double dp5h[] = {35. / 384., 0., 500. / 1113., 125. / 192., -2187. / 6784., 11. / 84., 0.};
   static const double dp5l[] = {5179. / 57600., 0., 7571. / 16695., 393. / 640., -92097. / 339200., 187. / 2100., 1. / 40.};
   static const double dp5c[] = {0., .2, 3. / 10., 4. / 5., 8. / 9., 1., 1.};
+  /* timedisc_cashkarp.f90:101-113 */
+  static const double ck5a[] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0,
+                                0.2, 0.0, 0.0, 0.0, 0.0, 0.0,
+                                0.075, 0.225, 0.0, 0.0, 0.0, 0.0,
+                                0.3, -0.9, 1.2, 0.0, 0.0, 0.0,
+                                -11.0 / 54.0, 2.5, -70.0 / 27.0, 35.0 / 27.0, 0.0, 0.0,
+                                1631.0 / 55296.0, 175.0 / 512.0, 575.0 / 13824.0, 44275.0 / 110592.0, 253.0 / 4096.0, 0.0};
+  static const double ck5h[] = {37.0 / 378.0, 0.0, 250.0 / 621.0, 125.0 / 594.0, 0.0, 512.0 / 1771.0};
+  static const double ck5l[] = {2825.0 / 27648.0, 0.0, 18575.0 / 48384.0, 13525.0 / 55296.0, 277.0 / 14336.0, 0.25};
+  static const double ck5c[] = {0.0, 0.2, 0.3, 0.6, 1.0, 7.0 / 8.0};
   const double *A, *H, *L, *C;
   if (s->c.method == SSPRK && o == 3) { m = 3; A = ssp3a; H = ssp3h; L = ssp3l; C = ssp3c; }
   else if (s->c.method == SSPRK && o == 5) { m = 5; A = ssp5a; H = ssp5h; L = ssp5l; C = ssp5c; }
   else if (s->c.method == RK_FEHLBERG && o == 3) { m = 3; A = rkf3a; H = rkf3h; L = rkf3l; C = rkf3c; }
   else if (s->c.method == RK_FEHLBERG && o == 5) { m = 6; A = rkf5a; H = rkf5h; L = rkf5l; C = rkf5c; }
   else if (s->c.method == DORMAND_PRINCE && o == 5) { m = 7; A = dp5a; H = dp5h; L = dp5l; C = dp5c; }
+  else if (s->c.method == CASH_KARP && o == 5) { m = 6; A = ck5a; H = ck5h; L = ck5l; C = ck5c; }
   else return -1;
   s->m = m;
   s->a = zalloc(m * m); s->b_high = zalloc(m); s->b_low = zalloc(m); s->cc = zalloc(m);
@@ -1075,17 +1134,22 @@ fvr *fvr_create(const fvr_cfg *cfg) {
     if (s->c.geometry == CARTESIAN) hw = he = hbt = hc = 1.0;
     s->wfac[i] = fmin2(fmin2(hw, he), fmin2(hbt, hbt)) / hc;
   }
-  /* timedisc_base.f90:224-352 defaults */
-  s->time = 0.0;
+  /* timedisc_base.f90:224-359 defaults */
+  s->time = cfg->starttime;
   s->dt = cfg->stoptime;
   s->dtold = s->dt;
   s->dtmin = s->dt;
   if (s->c.dtlimit <= 0.0) s->c.dtlimit = EPS_ * cfg->stoptime;
-  s->pmin = TINY_;
-  s->rhomin = TINY_;
+  s->pmin = cfg->pmin > 0.0 ? cfg->pmin : TINY_;
+  s->rhomin = cfg->rhomin > 0.0 ? cfg->rhomin : TINY_;
   s->maxerrold = 0.;
   s->iter = -1;
-  s->t_out = cfg->stoptime; /* Datafile count = 1 */
+  /* Datafile (fileio_base.f90:474-487, 591-600): the output clock starts at
+   * the start time and advances by |stoptime - starttime|/count; one IncTime
+   * happens before the first Step, in InitFileIO (start time > 0) or after
+   * the initial dataset of FirstStep (start time 0) */
+  s->out_count = cfg->count > 0 ? cfg->count : 1;
+  s->t_out = s->time + fabs(cfg->stoptime - s->time) / s->out_count;
   return s;
 }
 
@@ -1150,7 +1214,7 @@ int fvr_first_step(fvr *s) {
   return s->brk;
 }
 
-/* fosite.f90:367-464 (Datafile count = 1, output at stoptime) */
+/* fosite.f90:367-464 (outputs at the Datafile times) */
 int fvr_step(fvr *s) {
   if (s->iter == -1) fvr_first_step(s);
   if (fabs(s->c.stoptime - s->time) <= 1.0E-05 * s->c.stoptime) return 1;
@@ -1166,7 +1230,7 @@ int fvr_step(fvr *s) {
   if (fabs(s->t_out - s->time) <= 1.0E-5 * s->t_out || s->brk) {
     if (s->c.fargo) add_bg_velocity(s, &s->pvar, &s->cvar); /* WriteDataset */
     if (s->brk) return 1;
-    s->t_out = s->t_out + fabs(s->c.stoptime - 0.0) / 1;
+    s->t_out = s->t_out + fabs(s->c.stoptime - s->c.starttime) / s->out_count; /* IncTime */
     s->dtmin = s->c.stoptime;
     s->n_adj = 0;
   }

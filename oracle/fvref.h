@@ -36,6 +36,9 @@ typedef struct fvr_cfg {
   int outbound;       /* accretion through boundary 1 W, 2 E, 3 S, 4 N; 0 off
                          (gravity_pointmass.f90:285-299) */
   double acclimit;    /* accretion limit per unit mass and time; < 0 off */
+  double pmin, rhomin;/* CheckData limits, <= 0: TINY (timedisc_base.f90:353-359) */
+  int count;          /* datafile count, <= 0: 1 (fileio_base.f90:420-470) */
+  double starttime;   /* timedisc_base.f90:295 */
 } fvr_cfg;
 
 typedef struct fvr_scalars {

@@ -34,7 +34,9 @@ def _nml(cfg, nsteps, dump_every, infile, prefix):
                 k_gravity=cfg.get("gravity", 0), k_units=3 if cfg.get("gravity", 0) else 0,
                 k_mass=cfg.get("mass", 1.0), k_outbound=cfg.get("outbound", 0),
                 k_acclimit=float(cfg.get("acclimit", -1.0)), k_binary=int(cfg.get("binary_output", 0)),
-                k_nsteps=nsteps, k_dumpevery=dump_every)
+                k_nsteps=nsteps, k_dumpevery=dump_every, k_count=int(cfg.get("count", 1)),
+                k_pmin=float(cfg.get("pmin", -1.0)), k_rhomin=float(cfg.get("rhomin", -1.0)),
+                k_starttime=float(cfg.get("starttime", 0.0)))
     lines = ["&fvcfg"]
     for k, v in vals.items():
         lines.append(" %s = %s" % (k, repr(float(v)) if isinstance(v, float) else v))
@@ -122,6 +124,16 @@ def run(cfg, init, nsteps, dump_every=1, workdir=None):
             g = graw[n * grec:(n + 1) * grec]
             r["pointmass"] = g[:4].copy()
             r["accel"] = g[4:].reshape(2, NJ, NI).copy()
+    # scalars of every step (written also between state dumps)
+    spath = prefix + "_scal.bin"
+    mesh["scalars"] = []
+    if os.path.exists(spath):
+        sraw = open(spath, "rb").read()
+        for off in range(0, len(sraw) - 47, 48):
+            ib = np.frombuffer(sraw, "<i4", 4, off)
+            rb = np.frombuffer(sraw, "<f8", 4, off + 16)
+            mesh["scalars"].append(dict(step=int(ib[0]), iter=int(ib[1]), n_adj=int(ib[2]), brk=int(ib[3]),
+                                        time=rb[0], dt=rb[1], dtold=rb[2], maxerrold=rb[3]))
     mesh["step_seconds"] = step_seconds
     if workdir is None:
         shutil.rmtree(tmp, ignore_errors=True)

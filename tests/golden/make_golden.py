@@ -67,6 +67,19 @@ CASES = {
     "disk_prim_nograd_fargo1": (dict(D, recvars=S.PRIMITIVE, fargo=1,
                                      bc=[S.NO_GRADIENTS, S.NO_GRADIENTS, S.PERIODIC, S.PERIODIC]), 8, (8,)),
     "euler_disk_prim": (dict(S.euler_disk_config(ni=16, nj=32), recvars=S.PRIMITIVE), 8, (8,)),
+    # Cash-Karp order 5 (timedisc_cashkarp.f90:101-113)
+    "euler_cashkarp5": (dict(E, method=S.CASH_KARP, order=5), 6, (6,)),
+    # CheckData limits and the pressure floor (timedisc_base.f90:353-359,
+    # 1150-1164, 1666-1742): a density break at step 12, a floor that clips
+    "rarefaction_rhomin_break": (S.rarefaction_config(n=16, rhomin=0.1), 12, (12,)),
+    "rarefaction_pmin_floor": (S.rarefaction_config(n=16, pmin=0.03), 24, (24,)),
+    # dtlimit break after SolveODE (timedisc_base.f90:686-713)
+    "rarefaction_dtlimit_break": (S.rarefaction_config(n=16, dtlimit=0.0045), 3, (3,)),
+    # output times: count = 5 before the stop time (fileio_base.f90:591-620)
+    "euler_count5": (dict(E, stoptime=0.06, count=5), 10, (10,)),
+    # a start time, as a restart sets it (timedisc_base.f90:295,
+    # fileio_base.f90:474-487), with three outputs left
+    "disk_starttime_count3": (dict(D, starttime=2.0, stoptime=8.0, count=3, fargo=1), 12, (12,)),
 }
 MESHES = {"cartesian_16": E, "logcyl_16x32": D}
 SCALARS = ("iter", "n_adj", "brk", "time", "dt", "dtold", "maxerrold")

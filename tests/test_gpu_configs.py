@@ -1,0 +1,194 @@
+"""Parity at the stated workload sizes (the other GPU tests use small
+meshes):
+
+* the 512x1024 isothermal disk with FARGO, 1000 steps, advanced by
+  device-controlled rounds as the benchmark does: against the oracle at
+  every 250 steps (relative L1 <= 1e-12 per conserved field, the drop-in
+  bar; n_adj and break equal), and against the fingerprints of the compiled
+  reference's own run of the same workload (tests/golden/fingerprints.npz:
+  every step's time and dt, weighted projections of the state);
+* the 1024x2048 Euler disk with Dormand-Prince, 20 steps, FP64 against the
+  oracle and the reference's fingerprints; the FP32 precision form's error
+  against the FP64 oracle is printed;
+* the 64-disk batch (radial cells from {128, 256, 384, 512}), FP32 and FP64:
+  every member bit-identical to its run alone after 20 steps, and the FP32
+  members' error against the FP64 oracle on four members;
+* served runs drawn from the serving workload's full size range (64 to 1024
+  radial cells, FP32:FP64 3:1), shortened to 0.01 orbits: bit-identical to
+  the same runs alone.
+
+Absorbing radial boundaries evaluate LOG/EXP, so the disks are held to
+relative L1 <= 1e-12 rather than bit equality (see test_gpu_parity)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import fosite_amd
+import oracle
+from fosite_amd import setups as S
+from tests import fingerprint as F
+from tests.parity import field_errors, rel_l1
+
+I = (slice(None), slice(2, -2), slice(2, -2))
+FP = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "fingerprints.npz")
+
+
+def reference_fingerprints(name):
+    if not os.path.exists(FP):
+        return None
+    z = np.load(FP)
+    if name + "/steps" not in z.files:
+        return None
+    out = {k.split("/", 1)[1]: z[k] for k in z.files if k.startswith(name + "/")}
+    out["config"] = json.loads(bytes(out["config"]).decode())
+    return out
+
+
+def make(cfg, cls=fosite_amd.Fosite):
+    init = S.initial(cfg)
+    o = cls(cfg)
+    o.set_initial(init["pvar"], init.get("bccsound"), init.get("fcsound"), init.get("w"))
+    o.first_step()
+    return o
+
+
+def compare_fingerprint(ref, step, cv, tol, what):
+    if ref is None or step not in list(ref["dumped"]):
+        return None
+    fp = F.fingerprint(cv)
+    r = {"l1": ref["l1_%d" % step], "proj": ref["proj_%d" % step]}
+    worst, ok = F.compare(fp, r, tol)
+    assert ok, (what, step, worst)
+    return worst
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(900)
+def test_disk_512x1024_1000_steps():
+    cfg = S.kepler_disk_config(ni=512, nj=1024)
+    ref_fp = reference_fingerprints("disk512x1024_1000")
+    if ref_fp is not None:
+        assert ref_fp["config"]["inum"] == 512 and int(ref_fp["steps"][0]) == 1000
+    print("disk 512x1024: GPU rounds of 250 steps against the oracle", flush=True)
+    sim, ref = make(cfg), make(cfg, oracle.Oracle)
+    n = 0
+    while n < 1000:
+        taken = sim.run(250)
+        assert taken == 250
+        for _ in range(250):
+            assert not ref.step()
+        n += 250
+        cv, _ = sim.state()
+        rc, _, rs = ref.state()
+        sc = sim.scalars()
+        err = rel_l1(cv, rc)
+        assert sc["n_adj"] == rs["n_adj"] and sc["brk"] == rs["brk"] == 0 and sc["iter"] == rs["iter"], (n, sc, rs)
+        assert abs(sc["time"] - rs["time"]) <= 1e-12 * rs["time"], (n, sc, rs)
+        assert (err <= 1e-12).all(), (n, err)
+        fpw = compare_fingerprint(ref_fp, n, cv, 1e-12, "disk 512x1024")
+        if ref_fp is not None:  # the reference's own run: time and dt of this step
+            assert abs(sc["time"] - ref_fp["time"][n]) <= 1e-12 * ref_fp["time"][n], (n, sc["time"])
+            assert abs(sc["dt"] - ref_fp["dt"][n]) <= 1e-12 * ref_fp["dt"][n], (n, sc["dt"])
+            assert sc["n_adj"] == ref_fp["n_adj"][n]
+        print("step %d: relative L1 vs oracle %s, vs reference projections %s, time %.6f" %
+              (n, err, fpw, sc["time"]), flush=True)
+    sim.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(900)
+def test_euler_disk_1024x2048_20_steps():
+    cfg = S.euler_disk_config(ni=1024, nj=2048)
+    ref_fp = reference_fingerprints("euler_disk1024x2048_20")
+    print("euler disk 1024x2048 DP5: 20 steps", flush=True)
+    sim, ref = make(cfg), make(cfg, oracle.Oracle)
+    f32 = make(dict(cfg, dtype="fp32"))
+    for n in (10, 20):
+        taken = sim.run(10)
+        f32.run(10)
+        for _ in range(taken):
+            ref.step()
+        cv, _ = sim.state()
+        rc, _, rs = ref.state()
+        sc = sim.scalars()
+        err = rel_l1(cv, rc)
+        assert taken == 10 and sc["n_adj"] == rs["n_adj"] and sc["iter"] == rs["iter"], (n, sc, rs)
+        # the embedded-error controller magnifies last-bit differences of
+        # the boundary's LOG/EXP in dt (see test_gpu_parity)
+        assert abs(sc["time"] - rs["time"]) <= 1e-10 * rs["time"], (n, sc, rs)
+        assert (err <= 1e-12).all(), (n, err)
+        fpw = compare_fingerprint(ref_fp, n, cv, 1e-12, "euler disk")
+        e32 = field_errors(f32.state()[0], rc)
+        print("step %d: FP64 relative L1 vs oracle %s, vs reference projections %s; FP32 form vs FP64 oracle %s"
+              % (n, err, fpw, e32), flush=True)
+    sim.close()
+    f32.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("dtype", ["fp32", "fp64"])
+def test_batch_of_64_disks_equals_runs_alone(dtype):
+    cfgs = S.batch_disks(n=64, dtype=dtype)
+    print("batch of 64 disks", dtype, sorted(set(c["inum"] for c in cfgs)), flush=True)
+    sims = [make(c) for c in cfgs]
+    b = fosite_amd.Batch(sims, autotune=True)
+    taken = [0] * 64
+    for k in (8, 12):
+        taken = [t + x for t, x in zip(taken, b.run(k))]
+    info = b.info()
+    got = [(s.state()[0], s.scalars()) for s in sims]
+    b.close()
+    for s in sims:
+        s.close()
+    print("batch info", info, flush=True)
+    assert taken == [20] * 64
+    for k, c in enumerate(cfgs):
+        o = make(c)
+        assert o.run(20) == 20
+        cv, sc = o.state()[0], o.scalars()
+        o.close()
+        assert sc == got[k][1], (k, sc, got[k][1])
+        assert np.array_equal(cv[I], got[k][0][I]), k
+    if dtype == "fp32":  # the precision form's error on the smallest members
+        small = [k for k, c in enumerate(cfgs) if c["inum"] == 128][:4]
+        for k in small:
+            ref = make(dict(cfgs[k], dtype="fp64"), oracle.Oracle)
+            for _ in range(20):
+                ref.step()
+            e = field_errors(got[k][0], ref.state()[0])
+            print("member", k, "FP32 vs FP64 oracle after 20 steps", e, flush=True)
+            assert e["density"] <= 1e-5 and e["momentum"] <= 1e-5, (k, e)
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(900)
+def test_served_runs_across_the_full_size_range_equal_runs_alone():
+    reqs = [c for _, c in S.serving_requests(n=8, seed=20261206, orbits=0.01)]
+    print("served runs", [(c["inum"], c["dtype"]) for c in reqs], flush=True)
+    budget = sum(fosite_amd.device_bytes(c) for c in reqs) * 0.6  # some must queue
+    srv = fosite_amd.Server(budget, round_steps=16, autotune=True)
+    ids = [srv.submit(c, S.initial(c)) for c in reqs[:5]]
+    rounds = 0
+    while True:
+        srv.step()
+        rounds += 1
+        if rounds == 2:
+            ids += [srv.submit(c, S.initial(c)) for c in reqs[5:]]
+        st = srv.stats()
+        if rounds >= 2 and st["completed"] + st["failed"] == len(reqs):
+            break
+        assert rounds < 20000
+    assert st["failed"] == 0
+    for rid, c in zip(ids, reqs):
+        cv, sc = srv.result(rid)
+        o = make(c)
+        while o.run(256) == 256:
+            pass
+        cv0, sc0 = o.state()[0], o.scalars()
+        o.close()
+        assert sc == sc0, (rid, sc, sc0)
+        assert np.array_equal(cv[I], cv0[I]), rid
+    srv.close()

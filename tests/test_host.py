@@ -1,5 +1,6 @@
 """Host-side pieces: configuration marshalling and the seeded initial data."""
 import numpy as np
+import pytest
 
 import fosite_amd
 from fosite_amd import setups as S
@@ -41,3 +42,18 @@ def test_riemann_quadrants():
     assert tuple(pv[:, 10, 1]) == (0.5323, 1.206, 0.0, 0.3)  # NW
     assert tuple(pv[:, 1, 1]) == (0.138, 1.206, 1.206, 0.029)  # SW
     assert tuple(pv[:, 1, 10]) == (0.5323, 0.0, 1.206, 0.3)  # SE
+
+
+def test_initial_arrays_are_checked_against_the_abi_sizes():
+    V, NJ, NI = 3, 12, 10
+    pv = np.zeros((V, NJ, NI))
+    b, f, w, p = fosite_amd.init_arrays(V, NJ, NI, np.ones((NJ, NI)), np.ones((4, NJ, NI)), np.ones(NI), pv)
+    assert (b.size, f.size, w.size, p.size) == (NJ * NI, 4 * NJ * NI, NI, V * NJ * NI)
+    bad = [dict(bccsound=np.ones(NJ * NI - 1), fcsound=np.ones((4, NJ, NI))),
+           dict(bccsound=np.ones((NJ, NI)), fcsound=np.ones((3, NJ, NI))),
+           dict(bccsound=np.ones((NJ, NI))),
+           dict(w=np.ones(NI - 1)),
+           dict(pvar=np.zeros((V, NJ, NI - 1)))]
+    for kw in bad:
+        with pytest.raises(ValueError):
+            fosite_amd.init_arrays(V, NJ, NI, **kw)
