@@ -899,8 +899,8 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
   if (c.recvars != 1 && c.recvars != 2) return bail("reconstruction variables must be 1 PRIMITIVE or 2 CONSERVATIVE");
   if (c.dtype != 0 && c.dtype != 1) return bail("dtype must be 0 (FP64) or 1 (FP32 precision form)");
   if (c.fargo < 0 || c.fargo > 2) return bail("fargo must be 0, 1 or 2");
-  if (c.fargo && (c.physics != EULER_ISOTHERM || c.geometry != LOGCYLINDRICAL))
-    return bail("FARGO requires isothermal physics on a logcylindrical mesh");
+  if (c.fargo && c.physics == EULER && c.dtype == 1)
+    return bail("FARGO with Euler physics runs in FP64 only (dtype 0)");
   switch (c.limiter) {
     case MINMOD: case MONOCENT: case SWEBY: case SUPERBEE: case OSPRE: case VANLEER: case NOLIMIT: break;
     default: return bail("unknown limiter");

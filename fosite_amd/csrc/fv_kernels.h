@@ -197,6 +197,21 @@ __device__ __forceinline__ void absorbing_math(const Geom &g, int east, int i1, 
   }
 }
 
+// FARGO frame change of one conservative state (physics_eulerisotherm.f90:
+// 1263-1303, 1373-1413): + or - rho*w on the azimuthal momentum; Euler first
+// changes the energy by w*(m +- 0.5*rho*w) with the old momentum
+// (physics_euler.f90:2003-2047, 2159-2203)
+template <typename T, int V>
+__device__ __forceinline__ void frame_add(T *u, T w) {
+  if (V == 4) u[3] = u[3] + w * (u[2] + T(0.5) * u[0] * w);
+  u[2] = u[2] + u[0] * w;
+}
+template <typename T, int V>
+__device__ __forceinline__ void frame_sub(T *u, T w) {
+  if (V == 4) u[3] = u[3] - w * (u[2] - T(0.5) * u[0] * w);
+  u[2] = u[2] - u[0] * w;
+}
+
 // ------------------------------------------------------------ input state
 // KM: coefficient slots of the tableau (3 for up to three stages, else MAXK);
 // loops over stages are unrolled to KM with run-time guards on nk
@@ -309,7 +324,7 @@ struct In {
   __forceinline__ __device__ void transform(int i, const T *s, T *ur, T *pr, T *uc) const {
 #pragma unroll
     for (int l = 0; l < V; l++) ur[l] = s[l];
-    if (a.add_bg) ur[2] = ur[2] + ur[0] * rg<T>(g, R_W, i);
+    if (a.add_bg) frame_add<T, V>(ur, rg<T>(g, R_W, i));
     c2p<T, V>(ur, pr, gam);
     if (V == 4 && a.pmin_clamp) {
       pr[3] = fmax2(pr[3], (T)g.pmin);
@@ -318,11 +333,12 @@ struct In {
 #pragma unroll
     for (int l = 0; l < V; l++) uc[l] = ur[l];
     if constexpr (sizeof(T) == 4) {
-      // FP32: m' + rho*w - rho*w would cancel to a few bits of the residual
-      // momentum m'; the stored residual is what the subtraction means
+      // FP32 (isothermal only): m' + rho*w - rho*w would cancel to a few
+      // bits of the residual momentum m'; the stored residual is what the
+      // subtraction means
       if (a.sub_bg) uc[2] = a.add_bg ? s[2] : ur[2] - ur[0] * rg<T>(g, R_W, i);
     } else {
-      if (a.sub_bg) uc[2] = ur[2] - ur[0] * rg<T>(g, R_W, i);
+      if (a.sub_bg) frame_sub<T, V>(uc, rg<T>(g, R_W, i));
     }
   }
 
@@ -345,7 +361,7 @@ struct In {
   __forceinline__ __device__ void prim_interior(int i, int j, T *p) const {
     T s[V];
     interior(i, j, s);
-    if (a.add_bg) s[2] = s[2] + s[0] * rg<T>(g, R_W, i);
+    if (a.add_bg) frame_add<T, V>(s, rg<T>(g, R_W, i));
     c2p<T, V>(s, p, gam);
   }
 
@@ -421,7 +437,7 @@ struct In {
     if (a.sub_bg) {
       const T w = rg<T>(g, R_W, i);
       p[2] = p[2] - w;
-      u[2] = u[2] - u[0] * w;
+      frame_sub<T, V>(u, w);
     }
   }
 };
@@ -771,14 +787,25 @@ inline int xcd_grid(int nbx, int nby) { return ((nbx * nby + XCDS - 1) / XCDS) *
 // Faces live in their own LDS region; the fluxes reuse the footprint region,
 // which is dead after phase 2, so three barriers separate the four phases.
 // Output-aligned tiles keep every lane of the flux phases busy and divide
-// power-of-two meshes exactly; at most 128 VGPRs, so that 4 waves share
-// each SIMD.
-#ifndef FV_RHS_WAVES_PER_SIMD
-#define FV_RHS_WAVES_PER_SIMD 4
-#endif
+// power-of-two meshes exactly.
+//
+// Occupancy: a CU has 160 KB of LDS.  The FP64 isothermal stage tiles
+// (64 x 8, V = 3) need 54.4 KB, so three tiles share a CU -- 6 waves per
+// SIMD -- when their VGPRs stay within 80, which the three-slot stage and
+// last-stage kernels do without spilling (the seven-slot ones would spill);
+// the ghost-cell CFL and flag arrays only exist in the end-of-step mode,
+// which keeps two tiles per CU.
 constexpr int TY_RHS = TY_TILE;
 // launch modes of the RHS kernel
 enum RhsMode { M_STAGE = 0, M_LAST = 1, M_END = 2 };
+template <typename T, int V, int TX, int KM, int MODE>
+constexpr int rhs_waves_per_simd() {
+#ifdef FV_RHS_WAVES_PER_SIMD
+  return FV_RHS_WAVES_PER_SIMD;
+#else
+  return (sizeof(T) == 8 && V == 3 && TX == 64 && KM <= 3 && MODE != M_END) ? 6 : 4;
+#endif
+}
 template <typename T>
 inline int rhs_mode(const StageArgsT<T> &a) { return a.ynew ? M_LAST : (a.nk == 0 ? M_END : M_STAGE); }
 
@@ -811,8 +838,8 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
   __shared__ int redi[TY];
   // CFL term and check flags of the ghost cells a thread evaluated (kept in
   // LDS rather than registers across the interior path)
-  __shared__ T gcfl[TX * TY];
-  __shared__ int gflg[TX * TY];
+  __shared__ T gcfl[kEnd ? TX * TY : 1];
+  __shared__ int gflg[kEnd ? TX * TY : 1];
   // footprint state, tile-relative row r in [-2, TY+1], column c in [-2, TX+1]
   auto UC = [&](int l, int r, int c) -> T & { return lds[(l * SY + r + 2) * SX + c + 2]; };
   // faces: W of columns 0..TX, E of column -1, S of rows 0..TY, N of row -1
@@ -897,8 +924,10 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
 #pragma unroll
       for (int l = 0; l < V; l++) UC(l, p ? er : ty, p ? ec : tx) = prim_rec ? pp[l] : u[l];
     }
-    gcfl[t] = ginv;
-    gflg[t] = gf;
+    if constexpr (kEnd) {
+      gcfl[t] = ginv;
+      gflg[t] = gf;
+    }
   }
 
   T ur[V], pr[V];
@@ -970,9 +999,21 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
       }
       geo[2] = mz * (czyz * vz - zero * vy) + mx * (cxyx * vx - cyxy * vy) + (cxyx + czyz) * P;
     }
-    // FARGO source (physics_eulerisotherm.f90:1531-1586)
-    if (g.fargo)
-      geo[2] = geo[2] - dv(ur[1] * T(0.5) * (rg<T>(g, R_W, i + 1) - rg<T>(g, R_W, i - 1)), rg<T>(g, R_DLX, i));
+    // FARGO source (physics_eulerisotherm.f90:1531-1586); Euler also takes
+    // -v_y*(that + GetGeometricalSourceY(vz = P = mz = 0)) from the energy
+    // (physics_euler.f90:2368-2400, elemental physics_eulerisotherm.f90:
+    // 3080-3088), with the real-frame velocities
+    if (g.fargo) {
+      T tmp = dv(ur[1] * T(0.5) * (rg<T>(g, R_W, i + 1) - rg<T>(g, R_W, i - 1)), rg<T>(g, R_DLX, i));
+      geo[2] = geo[2] - tmp;
+      if constexpr (V == 4) {
+        const T vx = pr[1], vy = pr[2], mx = ur[1];
+        T cxyx = zero, cyxy = zero, czyz = zero;
+        if (g.curvilinear) cxyx = rg<T>(g, R_CXYX, i), cyxy = rg<T>(g, R_CYXY, i), czyz = rg<T>(g, R_CZYZ, i);
+        tmp = tmp + (zero * (czyz * zero - zero * vy) + mx * (cxyx * vx - cyxy * vy) + (cxyx + czyz) * zero);
+        geo[3] = geo[3] - vy * tmp;
+      }
+    }
     // point-mass gravity through the source list (sources_generic.f90:220-261,
     // sources_gravity.f90:164-199, physics_eulerisotherm.f90:1201-1226,
     // physics_euler.f90:1849-1885)
@@ -1221,8 +1262,10 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
 
   if (!(want_cfl || want_check)) return;
   if (own_g || e_g) {  // ghost cells evaluated in phase 1
-    invdt = fmax2(invdt, gcfl[t]);
-    flags |= gflg[t];
+    if constexpr (kEnd) {
+      invdt = fmax2(invdt, gcfl[t]);
+      flags |= gflg[t];
+    }
   }
   // ---- the corner ghost cells (in no footprint) of the corner tiles:
   // data checks, CFL and the new state (SetCornerEdges, boundary_generic.f90:434-517)
@@ -1270,7 +1313,7 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
 }
 
 template <typename T, int V, int LIM, int TX, int KM, int MODE>
-__global__ __launch_bounds__(TX *TY_RHS, FV_RHS_WAVES_PER_SIMD) void k_rhs(Geom g, StageArgsT<T> a) {
+__global__ __launch_bounds__(TX *TY_RHS, (rhs_waves_per_simd<T, V, TX, KM, MODE>())) void k_rhs(Geom g, StageArgsT<T> a) {
   const int nbx = (g.inum + TX - 1) / TX, nby = (g.jnum + TY_RHS - 1) / TY_RHS;
   int bx, by;
   if (!xcd_tile(blockIdx.x, nbx, nby, bx, by)) return;
@@ -1280,7 +1323,7 @@ __global__ __launch_bounds__(TX *TY_RHS, FV_RHS_WAVES_PER_SIMD) void k_rhs(Geom 
 // Many meshes in one launch: block b runs the tile tiles[b] (TileCode) of
 // the mesh whose Geom and arguments of this launch are tab[disk].
 template <typename T, int V, int LIM, int TX, int KM, int MODE>
-__global__ __launch_bounds__(TX *TY_RHS, FV_RHS_WAVES_PER_SIMD) void k_rhs_multi(
+__global__ __launch_bounds__(TX *TY_RHS, (rhs_waves_per_simd<T, V, TX, KM, MODE>())) void k_rhs_multi(
     const RhsEntry *__restrict__ tab, const unsigned *__restrict__ tiles) {
   const unsigned e = tiles[blockIdx.x];
   if (e == NO_TILE) return;
@@ -1414,15 +1457,16 @@ __global__ __launch_bounds__(256) void k_frame(Geom g, const T *src, T *dst, int
   if (i >= g.NI || j >= g.NJ) return;
   const long ix = cidx(g, i, j);
   const T w = rg<T>(g, R_W, i);
+  T u[V];
 #pragma unroll
-  for (int l = 0; l < V; l++) {
-    T v = src[l * g.FS + ix];
-    if (l == 2) {
-      v = sign > 0 ? v + src[ix] * w : v - src[ix] * w;
-      if (w2) v = v - src[ix] * w2[i];
-    }
-    dst[l * g.FS + ix] = v;
-  }
+  for (int l = 0; l < V; l++) u[l] = src[l * g.FS + ix];
+  if (sign > 0)
+    frame_add<T, V>(u, w);
+  else
+    frame_sub<T, V>(u, w);
+  if (w2) frame_sub<T, V>(u, w2[i]);
+#pragma unroll
+  for (int l = 0; l < V; l++) dst[l * g.FS + ix] = u[l];
 }
 
 // typed launchers, explicitly instantiated in fv_inst_*.hip (one translation

@@ -227,6 +227,8 @@ def config_entries(cfg, filename=""):
             ("zmin", 0.0), ("zmax", 0.0), ("gparam", float(cfg["gparam"]))]
     if fargo:
         mesh.append(("fargo/method", fargo))
+        if cfg["geometry"] == setups.CARTESIAN:  # along y (mesh_base.f90:390-404)
+            mesh.append(("fargo/direction", 2))
     if cfg["geometry"] == setups.CARTESIAN:
         mesh.append(("dz", 1.0))
     mesh += [("shearingbox", 0), ("omega", 0.0), ("Qshear", 1.5),

@@ -174,7 +174,8 @@ def standard_normal(seed, n):
 
 def euler_disk_initial(cfg):
     """Sigma = r^-1.5 exp(0.01 N(0,1)), v_r = 0, v_phi = sqrt(GM/r),
-    p = Sigma (h v_phi)^2 at the cell centres."""
+    p = Sigma (h v_phi)^2 at the cell centres; FARGO background velocity
+    w(i) = sqrt(GM/r_i)."""
     xc, _, _, _ = coords(cfg)
     GM = cfg["GN"] * cfg["mass"]
     NI, NJ = len(xc), cfg["jnum"] + 2 * GNUM
@@ -186,7 +187,7 @@ def euler_disk_initial(cfg):
     pv[0] = sigma
     pv[2] = vk[None, :]
     pv[3] = sigma * ((cfg.get("h", 0.05) * vk) ** 2)[None, :]
-    return dict(pvar=pv)
+    return dict(pvar=pv, w=vk.copy())
 
 
 # ----------------------------------------------------------------- config 4
@@ -247,7 +248,42 @@ def rarefaction_initial(cfg, u0=2.0):
     return dict(pvar=pv)
 
 
+def shear_box_config(n=32, physics=EULER, fargo=2, **kw):
+    """A periodic Cartesian box with a shear layer: a dense band |x| < 0.25
+    moving in +y through a lighter medium moving in -y, with a small
+    transverse perturbation (the Kelvin-Helmholtz setup of tests/KHI2D.f90,
+    here with FARGO along y and a piecewise background velocity)."""
+    cfg = dict(physics=physics, gamma=1.4, geometry=CARTESIAN, inum=n, jnum=n,
+               xmin=-0.5, xmax=0.5, ymin=-0.5, ymax=0.5, bc=[PERIODIC] * 4,
+               limiter=MONOCENT, theta=1.2, recvars=CONSERVATIVE, method=SSPRK, order=3,
+               cfl=0.4, stoptime=1.0, fargo=fargo, problem="shear_box")
+    cfg.update(kw)
+    return full(cfg)
+
+
+def shear_box_initial(cfg):
+    """rho = 2 in the band, 1 outside; v_y = +0.5 / -0.5; v_x = 0.01 sin(2 pi y)
+    sin(pi x); p = 2.5 (Euler) or cs = 1 (isothermal); w(i) = v_y of column i."""
+    xc, yc, _, _ = coords(cfg)
+    X, Y = np.meshgrid(xc, yc)
+    band = np.abs(X) < 0.25
+    V = 4 if cfg["physics"] == EULER else 3
+    pv = np.zeros((V,) + X.shape)
+    pv[0] = np.where(band, 2.0, 1.0)
+    pv[1] = 0.01 * np.sin(2 * math.pi * Y) * np.sin(math.pi * X)
+    pv[2] = np.where(band, 0.5, -0.5)
+    out = dict(pvar=pv, w=np.where(np.abs(xc) < 0.25, 0.5, -0.5))
+    if V == 4:
+        pv[3] = 2.5
+    else:
+        out["bccsound"] = np.ones(X.shape)
+        out["fcsound"] = np.ones((4,) + X.shape)
+    return out
+
+
 def initial(cfg):
+    if cfg.get("problem") == "shear_box":
+        return shear_box_initial(cfg)
     if cfg.get("problem") == "euler_disk":
         return euler_disk_initial(cfg)
     if cfg.get("problem") == "rarefaction":

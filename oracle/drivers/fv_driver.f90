@@ -65,6 +65,9 @@ PROGRAM fv_driver
        "xmin" / k_xmin, "xmax" / k_xmax, "ymin" / k_ymin, "ymax" / k_ymax, &
        "zmin" / 0.0, "zmax" / 0.0, "gparam" / k_gparam)
   IF (k_fargo.GT.0) CALL SetAttr(mesh, "fargo/method", k_fargo)
+  ! a Cartesian mesh has no azimuthal index: FARGO along y is named
+  ! (mesh_base.f90:390-404)
+  IF (k_fargo.GT.0 .AND. k_geometry.EQ.CARTESIAN) CALL SetAttr(mesh, "fargo/direction", 2)
   bnd => Dict("western" / k_bcw, "eastern" / k_bce, "southern" / k_bcs, &
        "northern" / k_bcn, "bottomer" / NO_GRADIENTS, "topper" / NO_GRADIENTS)
   IF (k_physics.EQ.EULER) THEN

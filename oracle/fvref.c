@@ -12,8 +12,9 @@
  * reconstruction (all non-PP limiters) of conservative or primitive
  * variables, embedded RK (RKF 3/5, Cash-Karp 5, Dormand-Prince 5, SSPRK 3/5)
  * with error control and reject path, no-gradient, periodic and absorbing
- * boundaries, point-mass gravity with accretion, FARGO with the per-ring mean
- * (type 1) or a user-supplied (type 2) background velocity, CheckData breaks,
+ * boundaries, point-mass gravity with accretion, FARGO along y for both
+ * physics and geometries with the per-ring mean (type 1) or a user-supplied
+ * (type 2) background velocity, CheckData breaks,
  * output times (datafile count) and a start time.
  *
  * Every function is pinned by the golden fixtures of the compiled reference
@@ -231,16 +232,21 @@ static void update_bccsound(fvr *s, const double *p) {
   for (int c = 0; c < s->N; c++) s->bccsound[c] = sound_speed(s->c.gamma, p[c], p[3 * s->N + c]);
 }
 
-/* FARGO frame change, physics_eulerisotherm.f90:1263-1303 / 1373-1413 */
+/* FARGO frame change, physics_eulerisotherm.f90:1263-1303 / 1373-1413;
+ * Euler physics_euler.f90:2003-2047 / 2159-2203: the energy first, from the
+ * old momentum */
 static void add_bg_velocity(fvr *s, sv *p, sv *c) {
   if (!c->flag) return;
   if (!p->flag) return; /* "should not happen" branch, kept as in the reference */
+  const int N = s->N;
   PAR_FOR
   for (int j = 0; j < s->NJ; j++)
     for (int i = 0; i < s->NI; i++) {
       int k = IDX(s, i, j);
-      p->d[2 * s->N + k] = p->d[2 * s->N + k] + s->w[i];
-      c->d[2 * s->N + k] = c->d[2 * s->N + k] + c->d[k] * s->w[i];
+      const double w = s->w[i];
+      if (s->V == 4) c->d[3 * N + k] = c->d[3 * N + k] + w * (c->d[2 * N + k] + 0.5 * c->d[k] * w);
+      p->d[2 * N + k] = p->d[2 * N + k] + w;
+      c->d[2 * N + k] = c->d[2 * N + k] + c->d[k] * w;
     }
   c->flag = 0;
   p->flag = 0;
@@ -248,12 +254,15 @@ static void add_bg_velocity(fvr *s, sv *p, sv *c) {
 static void sub_bg_velocity(fvr *s, sv *p, sv *c) {
   if (c->flag) return;
   if (p->flag) return;
+  const int N = s->N;
   PAR_FOR
   for (int j = 0; j < s->NJ; j++)
     for (int i = 0; i < s->NI; i++) {
       int k = IDX(s, i, j);
-      p->d[2 * s->N + k] = p->d[2 * s->N + k] - s->w[i];
-      c->d[2 * s->N + k] = c->d[2 * s->N + k] - c->d[k] * s->w[i];
+      const double w = s->w[i];
+      if (s->V == 4) c->d[3 * N + k] = c->d[3 * N + k] - w * (c->d[2 * N + k] - 0.5 * c->d[k] * w);
+      p->d[2 * N + k] = p->d[2 * N + k] - w;
+      c->d[2 * N + k] = c->d[2 * N + k] - c->d[k] * w;
     }
   c->flag = 1;
   p->flag = 1;
@@ -516,14 +525,24 @@ static void external_sources(fvr *s, const double *c, double *st, double dt) {
         for (int l = 0; l < V; l++) st[l * N + IDX(s, i, j)] = 0.0;
 }
 
-/* physics_eulerisotherm.f90:1531-1586 (2-D x-y momentum) */
-static void add_fargo_sources(fvr *s, const double *c, double *st) {
+/* physics_eulerisotherm.f90:1531-1586 (2-D x-y momentum); Euler
+ * physics_euler.f90:2368-2400: the same momentum source, then the energy
+ * source -v_y*(that + GetGeometricalSourceY with vz = P = mz = 0), the
+ * elemental of physics_eulerisotherm.f90:3080-3088; p is in the real frame */
+static void add_fargo_sources(fvr *s, const double *p, const double *c, double *st) {
   int N = s->N;
   PAR_FOR
   for (int j = G; j < G + s->c.jnum; j++)
     for (int i = G; i < G + s->c.inum; i++) {
       int k = IDX(s, i, j);
-      st[2 * N + k] = st[2 * N + k] - c[1 * N + k] * 0.5 * (s->w[i + 1] - s->w[i - 1]) / s->dlx[k];
+      double tmp = c[1 * N + k] * 0.5 * (s->w[i + 1] - s->w[i - 1]) / s->dlx[k];
+      st[2 * N + k] = st[2 * N + k] - tmp;
+      if (s->V == 4) {
+        const double vx = p[1 * N + k], vy = p[2 * N + k], mx = c[1 * N + k];
+        tmp = tmp + (0.0 * (s->czyz[k] * 0.0 - 0.0 * vy) + mx * (s->cxyx[k] * vx - s->cyxy[k] * vy) +
+                     (s->cxyx[k] + s->czyz[k]) * 0.0);
+        st[3 * N + k] = st[3 * N + k] - vy * tmp;
+      }
     }
 }
 
@@ -750,7 +769,7 @@ static void compute_rhs(fvr *s, sv *p, sv *c, int checkbm, double *rhs, double d
   external_sources(s, c->d, s->src, dt);
   if (fargo) {
     if (s->c.geometry == CARTESIAN) memset(s->geo_src, 0, sizeof(double) * V * N);
-    add_fargo_sources(s, c->d, s->geo_src);
+    add_fargo_sources(s, p->d, c->d, s->geo_src);
     sub_bg_velocity(s, p, c);
   }
   calculate_fluxes(s, p->d, c->d);
@@ -1088,7 +1107,6 @@ fvr *fvr_create(const fvr_cfg *cfg) {
   if (cfg->geometry != CARTESIAN && cfg->geometry != LOGCYLINDRICAL) return NULL;
   if (cfg->inum < 4 || cfg->jnum < 4) return NULL;
   if (cfg->fargo < 0 || cfg->fargo > 2) return NULL;
-  if (cfg->fargo && cfg->physics != EULER_ISOTHERM) return NULL;
   for (int d = 0; d < 4; d++) {
     int b = cfg->bc[d];
     if (b != NO_GRADIENTS && b != PERIODIC && b != ABSORBING) return NULL;

@@ -41,8 +41,15 @@ def fingerprint(cv, w=None):
 
 
 def compare(fp, ref, tol):
-    """Largest |proj - ref proj| / ref l1 over fields and weights, and whether
-    it is within tol; fields with zero l1 compare absolutely."""
-    den = np.where(ref["l1"] > 0, ref["l1"], 1.0)
-    worst = float(np.max(np.abs(fp["proj"] - ref["proj"]) / den[:, None]))
+    """Largest relative projection difference over the conserved fields of
+    the state vector -- density, momentum (the vector: both components'
+    differences over both components' L1) and, for Euler, energy -- and
+    whether it is within tol."""
+    V = len(ref["l1"])
+    groups = [[0], [1, 2]] + ([[3]] if V == 4 else [])
+    d = np.abs(fp["proj"] - ref["proj"])
+    worst = 0.0
+    for g in groups:
+        den = float(sum(ref["l1"][l] for l in g)) or 1.0
+        worst = max(worst, float(max(sum(d[l][q] for l in g) for q in range(NPROJ)) / den))
     return worst, worst <= tol

@@ -80,6 +80,13 @@ CASES = {
     # a start time, as a restart sets it (timedisc_base.f90:295,
     # fileio_base.f90:474-487), with three outputs left
     "disk_starttime_count3": (dict(D, starttime=2.0, stoptime=8.0, count=3, fargo=1), 12, (12,)),
+    # FARGO with Euler physics (energy terms, physics_euler.f90:2003-2047,
+    # 2159-2203, 2368-2400) and on Cartesian meshes
+    "shear_euler_fargo2": (S.shear_box_config(n=16), 10, (1, 10)),
+    "shear_euler_fargo1": (S.shear_box_config(n=16, fargo=1), 10, (10,)),
+    "shear_iso_fargo2": (S.shear_box_config(n=16, physics=S.EULER_ISOTHERM), 10, (10,)),
+    "euler_disk_fargo2": (dict(S.euler_disk_config(ni=16, nj=32), fargo=2), 8, (8,)),
+    "euler_disk_fargo1_prim": (dict(S.euler_disk_config(ni=16, nj=32), fargo=1, recvars=S.PRIMITIVE), 8, (8,)),
 }
 MESHES = {"cartesian_16": E, "logcyl_16x32": D}
 SCALARS = ("iter", "n_adj", "brk", "time", "dt", "dtold", "maxerrold")

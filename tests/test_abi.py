@@ -49,7 +49,7 @@ BAD = {
     "limiter": dict(limiter=6),
     "unpaired periodic": dict(bc=[S.PERIODIC, S.NO_GRADIENTS, S.NO_GRADIENTS, S.NO_GRADIENTS]),
     "absorbing in y": dict(bc=[S.NO_GRADIENTS, S.NO_GRADIENTS, S.ABSORBING, S.NO_GRADIENTS]),
-    "fargo on cartesian": dict(fargo=2),
+    "euler fargo in fp32": dict(fargo=2, dtype=1),
     "tableau": dict(method=S.SSPRK, order=4),
     "tiny mesh": dict(inum=3),
     "dtype": dict(dtype=1),
