@@ -41,15 +41,68 @@ def b_alg(V=3, w=8, s=3, fargo=True):
     return w * V * (sum(m + 1 for m in range(2, s)) + (s + 1) + (2 if fargo else 0) + 3)
 
 
-def cpu_baseline(cfg, init, steps):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def host_cores():
+    """CPUs this process may run on (the scheduler's affinity mask), capped
+    at 16, the host share of one GPU on the measuring machines."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_throughput(cfg, steps, procs):
+    """Throughput mode of the serial reference: `procs` independent runs of
+    the same workload on different seeds at once, one process each; the
+    aggregate is the sum of their cell-updates over the wall clock of the
+    slowest Step() loop."""
+    from concurrent.futures import ThreadPoolExecutor
+    from fosite_amd import setups
+    from oracle import refrun
+    ni, nj = cfg["inum"], cfg["jnum"]
+
+    def one(k):
+        c = dict(cfg, seed=cfg.get("seed", 20261202) + 1000 + k)
+        mesh, _ = refrun.run(c, setups.initial(c), steps, dump_every=10 ** 9)
+        return mesh["step_seconds"]
+
+    with ThreadPoolExecutor(procs) as ex:
+        res = list(ex.map(one, range(procs)))
+    total = sum(ni * nj * n for n, _ in res)
+    slowest = max(sec for _, sec in res)
+    return dict(value=total / slowest, unit="cell-updates/s", processes=procs,
+                per_process=[ni * nj * n / sec for n, sec in res],
+                sample="%d concurrent serial reference runs of %d Step() calls of the %dx%d disk, "
+                       "seeds differing; cell-updates of all / Step-loop wall clock of the slowest"
+                       % (procs, steps, ni, nj))
+
+
+def cpu_baseline(cfg, init, steps, throughput=True):
     from oracle import refrun
     ni, nj = cfg["inum"], cfg["jnum"]
     if refrun.available():
         mesh, _ = refrun.run(cfg, init, steps, dump_every=10 ** 9)
         n, sec = mesh["step_seconds"]
-        return dict(value=ni * nj * n / sec, unit="cell-updates/s", cores=1, kind="reference",
-                    sample="%d Step() calls of the %dx%d disk (serial reference build, "
-                           "amdflang -O3, wall clock of the Step loop)" % (n, ni, nj))
+        out = dict(value=ni * nj * n / sec, unit="cell-updates/s", cores=1, kind="reference",
+                   sample="%d Step() calls of the %dx%d disk (serial reference build, "
+                          "amdflang -O3, wall clock of the Step loop)" % (n, ni, nj),
+                   cpu_model=cpu_model(), host_cpus=os.cpu_count())
+        if throughput:
+            # the reference has no threading and no MPI library is present:
+            # its multi-core figure is independent runs side by side
+            out["throughput_mode"] = cpu_throughput(cfg, steps, host_cores())
+        return out
     import oracle
     o = oracle.Oracle(cfg)
     o.set_initial(init["pvar"], init.get("bccsound"), init.get("fcsound"), init.get("w"))
@@ -310,6 +363,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--profile-steps", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu-throughput", action="store_true",
+                    help="time the reference on one core only (no concurrent runs)")
     ap.add_argument("--no-forms", action="store_true",
                     help="skip the precision, batched and serving forms")
     ap.add_argument("--form-steps", type=int, default=500)
@@ -399,7 +454,7 @@ def main():
         out["accretion_variant"] = accretion_variant()
         out["fargo1_variant"] = fargo1_variant()
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(cfg, init, a.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(cfg, init, a.cpu_steps, throughput=not a.no_cpu_throughput)
     print(json.dumps(out))
     return 0
 

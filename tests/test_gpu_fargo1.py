@@ -23,6 +23,11 @@ CASES = {
                                                              S.PERIODIC]), 20, 0.0),
     "absorbing": (dict(D, fargo=1), 25, 1e-12),
     "absorbing_ragged_70x90": (dict(S.kepler_disk_config(ni=70, nj=90), fargo=1), 15, 1e-12),
+    # more than 256 rows and not a multiple of 256: the ring sums stage three
+    # chunks (double-buffered), the last one partial
+    "nograd_24x600": (dict(S.kepler_disk_config(ni=24, nj=600), fargo=1,
+                           bc=[S.NO_GRADIENTS, S.NO_GRADIENTS, S.PERIODIC, S.PERIODIC]), 12, 0.0),
+    "absorbing_24x600": (dict(S.kepler_disk_config(ni=24, nj=600), fargo=1), 12, 1e-12),
 }
 
 
@@ -73,7 +78,8 @@ def test_gpu_fargo1_matches_oracle(name):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["nograd_radial", "nograd_cfl_0.9_rejects", "absorbing"])
+@pytest.mark.parametrize("name", ["nograd_radial", "nograd_cfl_0.9_rejects", "absorbing", "nograd_24x600",
+                                  "absorbing_24x600"])
 def test_gpu_fargo1_device_rounds_match_oracle(name):
     """Fosite.run() on a type-1 context: device-controlled steps with
     CalcBackgroundVelocity in the step graph (frame change, ordered ring sums
