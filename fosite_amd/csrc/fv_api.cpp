@@ -500,6 +500,23 @@ double calc_timestep(fv_ctx *ctx) {
   return ctx->c.cfl / ctx->invdt;  // gravity: dt_src = HUGE (sources_c_accel.f90:125)
 }
 
+// CalcTimestep reads the wave speeds of Timedisc%pvar (timedisc_base.f90:
+// 844-858).  After an output, WriteDataset has moved pvar to the real frame
+// (v + w), so the next CFL step is that of the real-frame velocities, not the
+// comoving ones the end-of-step evaluation reduced: recompute invdt from the
+// stored pvar of that evaluation with w added.
+bool pvar_moved_to_real_frame(const fv_ctx *ctx) {
+  return ctx->c.fargo && ctx->pvalid && ctx->pflag && !ctx->fl_pvar;
+}
+int cfl_real_frame(fv_ctx *ctx) {
+  const long cells = (long)ctx->NI * ctx->NJ;
+  const int nb = (int)std::min<long>(ctx->nb_rhs, (cells + 255) / 256);
+  launch_cfl_pvar(ctx->g, ctx->pbuf, 1, ctx->part_r, nb, ctx->V, ctx->st);
+  HIPCHK(hipGetLastError());
+  int f = 0;
+  return reduce(ctx, ctx->part_r, nb, &ctx->invdt, &f);
+}
+
 // out-of-place frame change of cvar (Physics%Add/SubtractBackgroundVelocityY)
 int frame_cvar(fv_ctx *ctx, int sign) {
   const int outb = pick_free(ctx, ctx->cvar, ctx->cold, -1);
@@ -840,6 +857,19 @@ int get_graph(fv_ctx *ctx, int x0, int x1, int tb, int rb, int k0, int b0, int g
   return 0;
 }
 
+// FV_EAGER=1: device rounds issue their launches one by one instead of
+// replaying HIP graphs (same kernels, same arguments, same order).  For runs
+// under a profiler: two profiled runs whose process had destroyed one
+// context's or batch's graphs before launching another's crashed in host
+// code inside a HIP call (no GPU fault; the same runs complete unprofiled).
+bool eager_launches() {
+  static const bool e = [] {
+    const char *v = std::getenv("FV_EAGER");
+    return v && std::atoi(v) != 0;
+  }();
+  return e;
+}
+
 // runs up to nmax steps on the device; *done_steps = steps completed there,
 // *halted = 1 if the batch stopped on a step the host must redo
 int run_batch(fv_ctx *ctx, int nmax, int *done_steps, int *halted) {
@@ -854,14 +884,14 @@ int run_batch(fv_ctx *ctx, int nmax, int *done_steps, int *halted) {
   Ctl *h = ctx->h_ctl;
   prime_ctl(ctx, h, nmax);
   hipGraphExec_t exec = nullptr;
-  if (!ctx->prof) {
+  if (!ctx->prof && !eager_launches()) {
     int rc = get_graph(ctx, x0, x1, tb, rb, k0, b0, g0, w0, v0, &exec);
     if (rc) return rc;
   }
   HIPCHK(hipMemcpyAsync(ctx->d_ctl, h, offsetof(Ctl, err_hist), hipMemcpyHostToDevice, ctx->st));
   if (exec) {
     for (int s = 0; s < nmax; s += 2) HIPCHK(hipGraphLaunch(exec, ctx->st));
-  } else {  // timed: the same launches, issued one by one between events
+  } else {  // timed or eager: the same launches, issued one by one
     for (int s = 0; s < nmax; s++)
       enqueue_fast_step(ctx, s % 2 ? x1 : x0, s % 2 ? x0 : x1, tb, rb, s % 2 ? 1 - k0 : k0, s % 2 ? k0 : 1 - k0,
                         s % 2 ? 1 - b0 : b0, s % 2 ? g1 : g0, (s % 2 && t1) ? 1 - w0 : w0,
@@ -1153,7 +1183,7 @@ extern "C" int fv_set_pvar(fv_ctx *ctx, const double *pvar) {
   return 0;
 }
 
-extern "C" int fv_first_step(fv_ctx *ctx, int *brk) {
+extern "C" int fv_first_rhs(fv_ctx *ctx, int *brk) {
   if (!ctx) return -1;
   if (!ctx->have_state) return fail(ctx, "initial data not set");
   if (ctx->V == 3 && !ctx->have_cs) return fail(ctx, "isothermal physics needs sound speeds");
@@ -1181,10 +1211,43 @@ extern "C" int fv_first_step(fv_ctx *ctx, int *brk) {
   ctx->dt = calc_timestep(ctx);
   ctx->cold = ctx->cvar;
   ctx->fl_cold = ctx->fl_cvar;
-  // initial WriteDataset moves the state to the real frame (io/fileio_base.f90:523-533)
-  if (ctx->time == 0.0 && ctx->c.fargo && (rc = add_bg_main(ctx))) return rc;
   if (brk) *brk = ctx->brk;
   return 0;
+}
+
+extern "C" int fv_first_step(fv_ctx *ctx, int *brk) {
+  int rc = fv_first_rhs(ctx, brk);
+  if (rc) return rc;
+  // initial WriteDataset moves the state to the real frame (io/fileio_base.f90:523-533)
+  if (ctx->time == 0.0 && ctx->c.fargo && (rc = add_bg_main(ctx))) return rc;
+  return 0;
+}
+
+extern "C" int fv_integration_step(fv_ctx *ctx, double dt, int dtcause, int *brk) {
+  if (!ctx) return -1;
+  if (ctx->iter < 0) return fail(ctx, "fv_integration_step before the first step");
+  if (!(dt > 0.0)) return fail(ctx, "fv_integration_step: dt must be positive");
+  ctx->dt = dt;
+  ctx->dtcause = dtcause;
+  int rc = integration_step(ctx);
+  if (brk) *brk = ctx->brk;
+  return rc;
+}
+
+extern "C" int fv_calc_timestep(fv_ctx *ctx, double *dt, int *dtcause) {
+  if (!ctx) return -1;
+  if (ctx->iter < 0) return fail(ctx, "fv_calc_timestep before the first step");
+  int rc;
+  if (pvar_moved_to_real_frame(ctx) && (rc = cfl_real_frame(ctx))) return rc;
+  ctx->dt = calc_timestep(ctx);
+  if (dt) *dt = ctx->dt;
+  if (dtcause) *dtcause = ctx->dtcause;
+  return 0;
+}
+
+extern "C" int fv_add_background_velocity(fv_ctx *ctx) {
+  if (!ctx) return -1;
+  return ctx->c.fargo ? add_bg_main(ctx) : 0;
 }
 
 extern "C" int fv_step(fv_ctx *ctx, int *done) {
@@ -1211,6 +1274,7 @@ extern "C" int fv_step(fv_ctx *ctx, int *done) {
       if (done) *done = 1;
       return 0;
     }
+    if (pvar_moved_to_real_frame(ctx) && (rc = cfl_real_frame(ctx))) return rc;
     ctx->t_out = ctx->t_out + std::fabs(ctx->c.stoptime - ctx->c.starttime) / ctx->count;  // IncTime
     ctx->dtmin = ctx->c.stoptime;
     ctx->n_adj = 0;
@@ -1616,6 +1680,9 @@ int batch_round(fv_batch *b, const std::vector<int> &want, int K, int c, bool ti
     for (int s = 0; s < K; s++) enqueue_batch_step(b, s % 2, c);
     BCHK(hipGetLastError());
     BCHK(hipEventRecord(b->t1, b->st));
+  } else if (eager_launches()) {
+    for (int s = 0; s < K; s++) enqueue_batch_step(b, s % 2, c);
+    BCHK(hipGetLastError());
   } else {
     hipGraphExec_t exec;
     int rc = batch_graph(b, K, c, &exec);

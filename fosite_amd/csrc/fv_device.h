@@ -195,6 +195,8 @@ void launch_remap(const Geom &g, const RemapArgs &a, int V, hipStream_t s);
 void launch_frame(const Geom &g, const void *src, void *dst, int sign, int V, hipStream_t s,
                   const Ctl *ctl = nullptr, const void *w2 = nullptr);
 void launch_reduce(const double *part, int n, double *out, hipStream_t s);
+// CalcTimestep partials (max per block) from a stored pvar, v + w when add_w
+void launch_cfl_pvar(const Geom &g, const void *pv, int add_w, double *part, int nblocks, int V, hipStream_t s);
 void launch_control(Ctl *ctl, const double *part_c, const double *part_r, int nb, hipStream_t s);
 void launch_bflux(const BfluxArgs &bf, hipStream_t s);
 void launch_accrete(const AccreteArgs &p, hipStream_t s);

@@ -798,12 +798,30 @@ inline int xcd_grid(int nbx, int nby) { return ((nbx * nby + XCDS - 1) / XCDS) *
 constexpr int TY_RHS = TY_TILE;
 // launch modes of the RHS kernel
 enum RhsMode { M_STAGE = 0, M_LAST = 1, M_END = 2 };
+// LDS bytes of one tile (the layout rhs_tile declares)
+template <typename T, int V, int TX>
+constexpr int rhs_lds_words() {
+  constexpr int TY = TY_RHS, SX = TX + 4, SY = TY + 4;
+  constexpr int N_UC = V * SY * SX, N_FLUX = V * TY * (TX + 1) + V * (TY + 1) * TX;
+  constexpr int N_FACE = V * TY * (TX + 1) + V * TY + V * (TY + 1) * TX + V * TX;
+  return (N_UC > N_FLUX ? N_UC : N_FLUX) + N_FACE;
+}
+// LDS bytes of one tile, rounded to the 512-byte allocation granule
+template <typename T, int V, int TX>
+constexpr int rhs_lds_bytes() {
+  return ((int)sizeof(T) * rhs_lds_words<T, V, TX>() + 12 * TY_RHS + 511) / 512 * 512;
+}
+// waves per SIMD the launch bounds ask for: the FP64 stage kernels of up to
+// three stages fit 80 VGPRs without spilling, so they take as many waves as
+// the tiles' LDS lets share a CU (160 KB), at most 6; everything else 4
 template <typename T, int V, int TX, int KM, int MODE>
 constexpr int rhs_waves_per_simd() {
 #ifdef FV_RHS_WAVES_PER_SIMD
   return FV_RHS_WAVES_PER_SIMD;
 #else
-  return (sizeof(T) == 8 && V == 3 && TX == 64 && KM <= 3 && MODE != M_END) ? 6 : 4;
+  if (!(sizeof(T) == 8 && V == 3 && KM <= 3 && MODE != M_END)) return 4;
+  constexpr int w = (163840 / rhs_lds_bytes<T, V, TX>()) * (TX * TY_RHS / 64) / 4;
+  return w < 4 ? 4 : (w > 6 ? 6 : w);
 #endif
 }
 template <typename T>
@@ -825,6 +843,9 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
   double *const vout = kEnd ? a.vout : nullptr;
   T *const ynew = kLast ? a.ynew : nullptr;
   constexpr int TY = TY_RHS, SX = TX + 4, SY = TY + 4;
+  // the threads with an extra footprint cell cover rows -2, -1, TY, TY+1 and
+  // columns -2, -1, TX, TX+1
+  static_assert(TX * TY >= 4 * TX + 4 * TY, "tile too small for its halo jobs");
   constexpr int N_UC = V * SY * SX;
   constexpr int XW = V * TY * (TX + 1), XE = V * TY, YS = V * (TY + 1) * TX, YN = V * TX;
   constexpr int N_FACE = XW + XE + YS + YN;
@@ -833,7 +854,8 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
   // region A: footprint state (phases 1-2), then fluxes (phases 3-4);
   // region B: face states (phases 2-3)
   constexpr int NA = N_UC > N_FLUX ? N_UC : N_FLUX;
-  __shared__ T lds[NA + N_FACE];
+  __shared__ T lds[rhs_lds_words<T, V, TX>()];
+  static_assert(NA + N_FACE == rhs_lds_words<T, V, TX>(), "LDS layout");
   __shared__ double redd[TY];
   __shared__ int redi[TY];
   // CFL term and check flags of the ghost cells a thread evaluated (kept in

@@ -144,6 +144,43 @@ __global__ __launch_bounds__(256) void k_wsum(Geom g, const double *__restrict__
   }
 }
 
+// CalcTimestep on a stored pvar (timedisc_base.f90:818-909 with
+// CalcWaveSpeeds_center, physics_eulerisotherm.f90:530-552) for the step
+// after an output: WriteDataset has moved fosite's pvar to the real frame
+// (v + w, physics_eulerisotherm.f90:1263-1303), and the wave speeds are those
+// of that pvar.  pv is the comoving pvar of the last evaluation, all cells;
+// grid-stride over the cells, one max per block into part[2b].
+template <typename T, int V>
+__global__ __launch_bounds__(256) void k_cfl_pvar(Geom g, const T *__restrict__ pv, int add_w, double *part) {
+  __shared__ double red[4];
+  double m = -DBL_MAX;
+  const long n = (long)g.NI * g.NJ;
+  for (long k = (long)blockIdx.x * 256 + threadIdx.x; k < n; k += (long)gridDim.x * 256) {
+    const int i = (int)(k % g.NI), j = (int)(k / g.NI);
+    const unsigned ix = (unsigned)j * (unsigned)g.P + (unsigned)i;
+    const T u = (pv + 1 * g.FS)[ix];
+    T v = (pv + 2 * g.FS)[ix];
+    if (add_w) v = v + rg<T>(g, R_W, i);
+    const T cs = (V == 4) ? sound_speed((T)g.gamma, (pv + 0 * g.FS)[ix], (pv + 3 * g.FS)[ix]) : rg<T>(g, R_CS, i);
+    m = fmax2(m, (double)cell_invdt(g, i, u, v, cs));
+  }
+  m = blk_max(m, red);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = m;
+    part[2 * blockIdx.x + 1] = 0.0;
+  }
+}
+
+void launch_cfl_pvar(const Geom &g, const void *pv, int add_w, double *part, int nblocks, int V, hipStream_t s) {
+  if (g.dtype == 1) {
+    if (V == 3) hipLaunchKernelGGL((k_cfl_pvar<float, 3>), dim3(nblocks), dim3(256), 0, s, g, (const float *)pv, add_w, part);
+    else hipLaunchKernelGGL((k_cfl_pvar<float, 4>), dim3(nblocks), dim3(256), 0, s, g, (const float *)pv, add_w, part);
+  } else {
+    if (V == 3) hipLaunchKernelGGL((k_cfl_pvar<double, 3>), dim3(nblocks), dim3(256), 0, s, g, (const double *)pv, add_w, part);
+    else hipLaunchKernelGGL((k_cfl_pvar<double, 4>), dim3(nblocks), dim3(256), 0, s, g, (const double *)pv, add_w, part);
+  }
+}
+
 // -------------------------------------------------------------- launchers
 // The host fills StageArgs / RemapArgs with untyped buffer addresses; the
 // FP32 form reads the same fields as their float counterparts (identical

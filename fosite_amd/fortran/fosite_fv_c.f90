@@ -43,7 +43,8 @@ MODULE fosite_fv_c
        fv_set_background_velocity, fv_set_pvar, fv_first_step, fv_step, fv_run, &
        fv_get_state, fv_get_scalars, fv_last_run_ms, fv_last_error, fv_profile, &
        fv_kernel_stats, fv_get_background_velocity, fv_get_boundary_flux, &
-       fv_get_boundary_flux_total, fv_get_point_mass, c_string
+       fv_get_boundary_flux_total, fv_get_point_mass, c_string, &
+       fv_first_rhs, fv_integration_step, fv_calc_timestep, fv_add_background_velocity
 
   INTERFACE
     INTEGER(c_int) FUNCTION fv_mesh_array(cfg, id, out) BIND(C, NAME="fv_mesh_array")
@@ -97,6 +98,28 @@ MODULE fosite_fv_c
       INTEGER(c_int), VALUE :: nsteps
       INTEGER(c_int), INTENT(OUT) :: taken
     END FUNCTION fv_run
+    INTEGER(c_int) FUNCTION fv_first_rhs(ctx, brk) BIND(C, NAME="fv_first_rhs")
+      IMPORT :: c_int, c_ptr
+      TYPE(c_ptr), VALUE :: ctx
+      INTEGER(c_int), INTENT(OUT) :: brk
+    END FUNCTION fv_first_rhs
+    INTEGER(c_int) FUNCTION fv_integration_step(ctx, dt, dtcause, brk) BIND(C, NAME="fv_integration_step")
+      IMPORT :: c_int, c_ptr, c_double
+      TYPE(c_ptr), VALUE :: ctx
+      REAL(c_double), VALUE :: dt
+      INTEGER(c_int), VALUE :: dtcause
+      INTEGER(c_int), INTENT(OUT) :: brk
+    END FUNCTION fv_integration_step
+    INTEGER(c_int) FUNCTION fv_calc_timestep(ctx, dt, dtcause) BIND(C, NAME="fv_calc_timestep")
+      IMPORT :: c_int, c_ptr, c_double
+      TYPE(c_ptr), VALUE :: ctx
+      REAL(c_double), INTENT(OUT) :: dt
+      INTEGER(c_int), INTENT(OUT) :: dtcause
+    END FUNCTION fv_calc_timestep
+    INTEGER(c_int) FUNCTION fv_add_background_velocity(ctx) BIND(C, NAME="fv_add_background_velocity")
+      IMPORT :: c_int, c_ptr
+      TYPE(c_ptr), VALUE :: ctx
+    END FUNCTION fv_add_background_velocity
     INTEGER(c_int) FUNCTION fv_get_state(ctx, cvar, pvar) BIND(C, NAME="fv_get_state")
       IMPORT :: c_int, c_ptr
       TYPE(c_ptr), VALUE :: ctx, cvar, pvar

@@ -118,6 +118,29 @@ int fv_step(fv_ctx *ctx, int *done);
 /* nsteps x fv_step; *taken = steps executed before the run ended */
 int fv_run(fv_ctx *ctx, int nsteps, int *taken);
 
+/* The same path cut at the Timedisc object, for a host that keeps fosite's
+ * own fosite%Step, Datafile and output (a timedisc_base extension whose
+ * IntegrationStep, ComputeRHS and CalcTimestep call these):
+ *
+ * fv_first_rhs: the device part of fosite%FirstStep (fosite.f90:305-321):
+ * CalcBackgroundVelocity for FARGO type 1, Timedisc%ComputeRHS with every
+ * data check, Timedisc%CalcTimestep; no output frame change.  *brk =
+ * Timedisc%break. */
+int fv_first_rhs(fv_ctx *ctx, int *brk);
+/* Timedisc%IntegrationStep (timedisc/timedisc_base.f90:649-750) with the
+ * host's time step: dt and dtcause as Datafile%AdjustTimestep left them
+ * (io/fileio_base.f90:604-620).  No stop test, output step or next CFL step
+ * (fosite%Step does those).  *brk = Timedisc%break. */
+int fv_integration_step(fv_ctx *ctx, double dt, int dtcause, int *brk);
+/* Timedisc%CalcTimestep (timedisc/timedisc_base.f90:818-909): the CFL time
+ * step of the last end-of-step evaluation, and its cause. */
+int fv_calc_timestep(fv_ctx *ctx, double *dt, int *dtcause);
+/* Physics%AddBackgroundVelocityY on Timedisc%pvar/cvar as WriteDataset
+ * applies it before writing (io/fileio_base.f90:523-533,
+ * physics/physics_eulerisotherm.f90:1263-1303): a no-op when the state is in
+ * the real frame already, or without FARGO. */
+int fv_add_background_velocity(fv_ctx *ctx);
+
 /* Device -> host copy of Timedisc%cvar and Timedisc%pvar, all cells (either
  * pointer may be NULL).  pvar is the one the last host-controlled ComputeRHS
  * left (timedisc_base.f90:1098-1164: after the PMIN clamp, ghost cells from

@@ -158,7 +158,41 @@ def main():
         exe = os.path.join(OUT, f[:-4])
         subprocess.check_call([FC, obj, lib, "-o", exe])
         print("built", exe)
+    build_device_driver(a.ref, objdir, moddir, lib)
     return 0
+
+
+REPO = os.path.dirname(HERE)
+FORTRAN = os.path.join(REPO, "fosite_amd", "fortran")
+LIBFV = os.path.join(REPO, "fosite_amd", "libfosite_fv.so")
+
+
+def build_device_driver(ref, objdir, moddir, lib):
+    """fv_driver_device: the same setup program with fosite's Timedisc
+    replaced by timedisc_gpu (fosite_amd/fortran/timedisc_gpu.f90, a
+    timedisc_base extension compiled against the reference's modules), linked
+    with the reference library and libfosite_fv.so.  Skipped while the HIP
+    library is not built."""
+    if not os.path.exists(LIBFV):
+        print("libfosite_fv.so not built; fv_driver_device skipped")
+        return None
+    ddir = os.path.join(OUT, "device")
+    os.makedirs(ddir, exist_ok=True)
+    objs = [compile_one(os.path.join(FORTRAN, f), ddir, moddir)
+            for f in ("fosite_fv_c.f90", "timedisc_gpu.f90")]
+    drv = os.path.join(HERE, "drivers", "fv_driver.f90")
+    obj = os.path.join(ddir, "fv_driver_device.o")
+    cmd = [FC] + FFLAGS + ["-DFV_DEVICE_TIMEDISC", "-J", ddir, "-I", moddir,
+                           "-I", os.path.join(ref, "common"), "-c", drv, "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("compile failed: %s\n%s" % (" ".join(cmd), r.stderr[-4000:]))
+    exe = os.path.join(OUT, "fv_driver_device")
+    libdir = os.path.dirname(LIBFV)
+    subprocess.check_call([FC, obj] + objs + [lib, "-L" + libdir, "-lfosite_fv",
+                                              "-Wl,-rpath,$ORIGIN/../../fosite_amd", "-o", exe])
+    print("built", exe)
+    return exe
 
 
 if __name__ == "__main__":

@@ -13,10 +13,13 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 DRIVER = os.path.join(HERE, "_ref", "fv_driver")
+# the same program with fosite's Timedisc replaced by timedisc_gpu (the HIP
+# library through fosite_amd/fortran/timedisc_gpu.f90); needs a GPU
+DEVICE_DRIVER = os.path.join(HERE, "_ref", "fv_driver_device")
 
 
-def available():
-    return os.path.exists(DRIVER)
+def available(device=False):
+    return os.path.exists(DEVICE_DRIVER if device else DRIVER)
 
 
 def _nml(cfg, nsteps, dump_every, infile, prefix):
@@ -47,11 +50,11 @@ def _nml(cfg, nsteps, dump_every, infile, prefix):
     return "\n".join(lines) + "\n"
 
 
-def run(cfg, init, nsteps, dump_every=1, workdir=None):
+def run(cfg, init, nsteps, dump_every=1, workdir=None, device=False):
     """Returns (mesh dict, list of step records {step, iter, n_adj, brk, time,
     dt, dtold, maxerrold, cvar, pvar}).  With cfg["binary_output"] = 1 the
     reference also writes its binary datasets (run_data_NNNN.bin, at time 0
-    and at the stop time) into workdir."""
+    and at the stop time) into workdir.  device=True runs fv_driver_device."""
     NI, NJ = cfg["inum"] + 4, cfg["jnum"] + 4
     N = NI * NJ
     V = 4 if cfg["physics"] == 17 else 3
@@ -68,7 +71,7 @@ def run(cfg, init, nsteps, dump_every=1, workdir=None):
     nml = os.path.join(tmp, "cfg.nml")
     with open(nml, "w") as f:
         f.write(_nml(cfg, nsteps, dump_every, infile, prefix))
-    r = subprocess.run([DRIVER, nml], cwd=tmp, capture_output=True, text=True)
+    r = subprocess.run([DEVICE_DRIVER if device else DRIVER, nml], cwd=tmp, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("reference driver failed:\n" + r.stdout[-3000:] + r.stderr[-3000:])
     step_seconds = None

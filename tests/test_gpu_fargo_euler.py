@@ -24,7 +24,8 @@ CASES = {
     "shear_euler_fargo2_ragged_prim": (S.shear_box_config(n=40, jnum=56, recvars=S.PRIMITIVE), 20, 0.0),
     "shear_iso_fargo2": (S.shear_box_config(n=48, physics=S.EULER_ISOTHERM), 30, 0.0),
     "shear_euler_fargo2_dp5": (S.shear_box_config(n=32, method=S.DORMAND_PRINCE, order=5, tol_rel=1e-4), 15, 0.0),
-    "euler_disk_fargo2": (dict(S.euler_disk_config(ni=40, nj=72), fargo=2), 20, 1e-12),
+    # (the coarse disk with FARGO 2 blows up: both runs break at step 16-17)
+    "euler_disk_fargo2": (dict(S.euler_disk_config(ni=40, nj=72), fargo=2), 14, 1e-12),
     "euler_disk_fargo1_prim": (dict(S.euler_disk_config(ni=40, nj=72), fargo=1, recvars=S.PRIMITIVE), 20, 1e-12),
 }
 
@@ -57,5 +58,6 @@ def test_gpu_fargo_euler_cartesian_host_steps(name):
 def test_gpu_fargo_euler_cartesian_device_rounds(name):
     cfg, nsteps, tol = CASES[name]
     print("rounds case", name, flush=True)
-    for n, cv, sc, rc, rs, ended in run_batched(cfg, 2 * nsteps, 11):
+    steps = nsteps if name == "euler_disk_fargo2" else 2 * nsteps
+    for n, cv, sc, rc, rs, ended in run_batched(cfg, steps, 11):
         check(n, cv, sc, rc, rs, tol, True)

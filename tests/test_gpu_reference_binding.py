@@ -1,0 +1,77 @@
+"""The drop-in at the Timedisc object: fosite's own setup program, Setup,
+FirstStep, Step(), Datafile and output, with its Timedisc replaced by
+timedisc_gpu (fosite_amd/fortran/timedisc_gpu.f90, a timedisc_base extension
+compiled against the reference's modules), against the same program with
+fosite's own Timedisc -- both built from the reference's sources by
+oracle/build_ref.py, the second run on the host CPU.
+
+Every step's state (cvar and pvar as the host holds them after Step()), the
+scalars (iter, n_adj, break, time, dt, dtold, maxerrold), the boundary fluxes
+and FARGO's w must agree: bit for bit on the paths without LOG/EXP
+(no absorbing boundary), relative L1 <= 1e-12 per field with it.  Several
+output times fall inside each run, so fosite's Datafile clips the steps and
+its WriteDataset moves the state to the real frame between steps; the
+device Timedisc has to follow both."""
+import numpy as np
+import pytest
+
+from fosite_amd import setups as S
+from oracle import refrun
+from tests.parity import rel_l1
+
+NG = [S.NO_GRADIENTS, S.NO_GRADIENTS, S.PERIODIC, S.PERIODIC]
+CASES = {
+    # Euler four-quadrant problem, 4 outputs in the run (the last at the stop time)
+    "riemann 48x48": (dict(S.riemann2d_config(n=48), stoptime=0.03, count=4), 40, 0.0),
+    # isothermal disk, FARGO type 2, point mass, 5 outputs
+    "disk fargo2 40x64": (dict(S.kepler_disk_config(ni=40, nj=64), bc=NG, stoptime=0.8, count=5), 40, 0.0),
+    # FARGO type 1 with absorbing radial boundaries and DP5
+    "disk fargo1 absorbing dp5": (dict(S.kepler_disk_config(ni=32, nj=48), fargo=1, method=S.DORMAND_PRINCE,
+                                       order=5, stoptime=0.5, count=4), 30, 1e-12),
+    # Euler with FARGO along y on a Cartesian shear box, SSPRK3
+    "euler shear box fargo2": (dict(S.shear_box_config(n=32), stoptime=0.09, count=3), 40, 0.0),
+}
+
+
+def initial(cfg):
+    if cfg.get("problem") == "shear_box":
+        return S.shear_box_initial(cfg)
+    return S.initial(cfg)
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not refrun.available(device=True), reason="reference build with the device driver absent")
+@pytest.mark.parametrize("name", list(CASES))
+def test_fosite_with_device_timedisc_matches_fosite(name):
+    cfg, nsteps, tol = CASES[name]
+    init = initial(cfg)
+    print("reference binding:", name, flush=True)
+    _, ref = refrun.run(cfg, init, nsteps)
+    _, dev = refrun.run(cfg, init, nsteps, device=True)
+    assert len(ref) == len(dev) and len(ref) > 5, (len(ref), len(dev))
+    I = (slice(None), slice(2, -2), slice(2, -2))
+    for r, d in zip(ref, dev):
+        for k in ("step", "iter", "n_adj", "brk"):
+            assert r[k] == d[k], (name, r["step"], k, r[k], d[k])
+        for k in ("time", "dt", "dtold", "maxerrold"):
+            if tol == 0.0:
+                assert r[k] == d[k], (name, r["step"], k, r[k], d[k])
+            else:
+                # the embedded-error controller magnifies the boundary's
+                # last-bit LOG/EXP differences in maxerrold (the difference
+                # of two solutions that agree to ~tol_rel), and the step
+                # size takes its 1/order power (see test_gpu_parity)
+                bar = 1e-6 if k == "maxerrold" else 1e-10
+                assert abs(r[k] - d[k]) <= bar * abs(r[k]), (name, r["step"], k, r[k], d[k])
+        if tol == 0.0:
+            assert np.array_equal(r["cvar"][I], d["cvar"][I]), (name, r["step"], rel_l1(d["cvar"], r["cvar"]))
+            assert np.array_equal(r["pvar"][I], d["pvar"][I]), (name, r["step"])
+            assert np.array_equal(r["bx"], d["bx"]) and np.array_equal(r["by"], d["by"]), (name, r["step"])
+            if "w" in r:
+                assert np.array_equal(r["w"], d["w"]), (name, r["step"])
+        else:
+            assert (rel_l1(d["cvar"], r["cvar"]) <= tol).all(), (name, r["step"], rel_l1(d["cvar"], r["cvar"]))
+            if "w" in r:
+                assert np.abs(r["w"] - d["w"]).sum() <= tol * np.abs(r["w"]).sum(), (name, r["step"])
+    last = ref[-1]
+    print(name, "steps", last["step"], "time", last["time"], "n_adj", last["n_adj"], flush=True)
