@@ -75,7 +75,8 @@ EXPORTS = ["fv_mesh_array", "fv_create", "fv_destroy", "fv_dims", "fv_set_sound_
            "fv_batch_get_info", "fv_batch_destroy", "fv_batch_last_error", "fv_device_bytes",
            "fv_server_create", "fv_server_submit", "fv_server_step", "fv_server_poll", "fv_server_result",
            "fv_server_get_stats", "fv_server_destroy", "fv_server_last_error",
-           "fv_first_rhs", "fv_integration_step", "fv_calc_timestep", "fv_add_background_velocity"]
+           "fv_first_rhs", "fv_integration_step", "fv_calc_timestep", "fv_add_background_velocity",
+           "fv_step_noalloc"]
 
 
 class FvServerOpts(ctypes.Structure):
@@ -130,6 +131,7 @@ def load_library():
     L.fv_set_pvar.argtypes = [V, P]
     L.fv_first_step.argtypes = [V, ctypes.POINTER(ctypes.c_int)]
     L.fv_step.argtypes = [V, ctypes.POINTER(ctypes.c_int)]
+    L.fv_step_noalloc.argtypes = [V, ctypes.POINTER(ctypes.c_int)]
     L.fv_run.argtypes = [V, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     L.fv_first_rhs.argtypes = [V, ctypes.POINTER(ctypes.c_int)]
     L.fv_integration_step.argtypes = [V, ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]

@@ -161,6 +161,7 @@ struct fv_ctx {
   double *d_red = nullptr, *h_red = nullptr;
   int nb_rhs = 0;
   hipStream_t st = nullptr;
+  int dev = 0;  // the device current at fv_create, which holds the state
   hipEvent_t e0 = nullptr, e1 = nullptr;
   Geom g{};
   // FARGO frame flags (fargo_transformation_applied) of each state vector
@@ -973,7 +974,8 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
   // device resources
   const size_t sbytes = elem_bytes(ctx) * ctx->V * ctx->FS;
   auto dalloc = [&](double **p, size_t b) { return hipMalloc((void **)p, b) == hipSuccess; };
-  bool ok = hipStreamCreateWithFlags(&ctx->st, hipStreamNonBlocking) == hipSuccess &&
+  bool ok = hipGetDevice(&ctx->dev) == hipSuccess &&
+            hipStreamCreateWithFlags(&ctx->st, hipStreamNonBlocking) == hipSuccess &&
             hipEventCreate(&ctx->e0) == hipSuccess &&
             hipEventCreate(&ctx->e1) == hipSuccess;
   auto zalloc = [&](double **p) {
@@ -1281,6 +1283,34 @@ extern "C" int fv_step(fv_ctx *ctx, int *done) {
   }
   ctx->dt = calc_timestep(ctx);
   return 0;
+}
+
+// fv_step with the reduction scratch of its kernels (per-workgroup partials
+// of the end-of-step evaluation and of the last stage, the reduction result)
+// in the library's static device array instead of the context's buffers: the
+// call allocates no device or pinned memory.  Every call on one device shares
+// that array; the partials are written by every workgroup before they are
+// reduced, so nothing in it has to start at zero.
+extern "C" int fv_step_noalloc(fv_ctx *ctx, int *done) {
+  if (!ctx) return -1;
+  if (ctx->prof) return fail(ctx, "fv_step_noalloc: kernel profiling (fv_profile) creates events; disable it");
+  if (ctx->nb_rhs > SCRATCH_MAX_BLOCKS)
+    return fail(ctx, "fv_step_noalloc: " + std::to_string(ctx->nb_rhs) + " workgroups per RHS launch exceed the " +
+                         std::to_string(SCRATCH_MAX_BLOCKS) + " the static scratch holds (use fv_step)");
+  int cur = -1;
+  HIPCHK(hipGetDevice(&cur));
+  if (cur != ctx->dev) return fail(ctx, "fv_step_noalloc: the context's device is not the current device");
+  double *s = nullptr;
+  HIPCHK(scratch_address(ctx->dev, &s));
+  double *const own[3] = {ctx->part_r, ctx->part_c, ctx->d_red};
+  ctx->part_r = s + SCRATCH_PART_R;
+  ctx->part_c = s + SCRATCH_PART_C;
+  ctx->d_red = s + SCRATCH_RED;
+  const int rc = fv_step(ctx, done);
+  ctx->part_r = own[0];
+  ctx->part_c = own[1];
+  ctx->d_red = own[2];
+  return rc;
 }
 
 extern "C" int fv_run(fv_ctx *ctx, int nsteps, int *taken) {

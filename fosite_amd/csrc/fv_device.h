@@ -221,4 +221,16 @@ void launch_rhs_multi(const RhsEntry *tab, const unsigned *tiles, int ntiles, in
 void launch_remap_multi(const RemapEntry *tab, const unsigned *tiles, int ntiles, int V, int dtype, hipStream_t s);
 void launch_control_multi(const ControlEntry *tab, int n, hipStream_t s);
 
+// Reduction scratch of the allocation-free step (fv_step_noalloc): a static
+// device array, one copy per device, laid out as the reduction result (2
+// values), then the per-workgroup partials of the end-of-step evaluation and
+// of the last stage (2 values per workgroup each).  At most
+// SCRATCH_MAX_BLOCKS workgroups per launch.
+constexpr int SCRATCH_MAX_BLOCKS = 16384;
+constexpr int SCRATCH_RED = 0, SCRATCH_PART_R = 32, SCRATCH_PART_C = SCRATCH_PART_R + 2 * SCRATCH_MAX_BLOCKS;
+constexpr int SCRATCH_DOUBLES = SCRATCH_PART_C + 2 * SCRATCH_MAX_BLOCKS;
+// device address of the scratch array on device dev (looked up once per
+// device, with that device current); hipSuccess or the lookup's error
+hipError_t scratch_address(int dev, double **out);
+
 }  // namespace fv

@@ -2,6 +2,7 @@
 // host code calls: they pick the element type (state dtype) and hand over to
 // the typed launchers instantiated in fv_inst_*.hip.
 #include <cstring>
+#include <mutex>
 
 #include "fv_kernels.h"
 
@@ -179,6 +180,33 @@ void launch_cfl_pvar(const Geom &g, const void *pv, int add_w, double *part, int
     if (V == 3) hipLaunchKernelGGL((k_cfl_pvar<double, 3>), dim3(nblocks), dim3(256), 0, s, g, (const double *)pv, add_w, part);
     else hipLaunchKernelGGL((k_cfl_pvar<double, 4>), dim3(nblocks), dim3(256), 0, s, g, (const double *)pv, add_w, part);
   }
+}
+
+// ------------------------------------------------- static reduction scratch
+// The array's name is only meaningful in device code; host code uses the
+// address hipGetSymbolAddress returns for each device (each device holds its
+// own copy at its own address).
+static __device__ __align__(256) double scratch[SCRATCH_DOUBLES];
+
+hipError_t scratch_address(int dev, double **out) {
+  constexpr int MAXDEV = 64;
+  static std::mutex mu;
+  static double *addr[MAXDEV] = {};
+  if (dev < 0 || dev >= MAXDEV) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[dev]) {
+    int cur = -1;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess) return e;
+    if (cur != dev && (e = hipSetDevice(dev)) != hipSuccess) return e;
+    double *p = nullptr;
+    e = hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch));
+    if (cur != dev) (void)hipSetDevice(cur);
+    if (e != hipSuccess) return e;
+    addr[dev] = p;
+  }
+  *out = addr[dev];
+  return hipSuccess;
 }
 
 // -------------------------------------------------------------- launchers

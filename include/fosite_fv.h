@@ -117,6 +117,17 @@ int fv_first_step(fv_ctx *ctx, int *brk);
 int fv_step(fv_ctx *ctx, int *done);
 /* nsteps x fv_step; *taken = steps executed before the run ended */
 int fv_run(fv_ctx *ctx, int nsteps, int *taken);
+/* fv_step that allocates nothing.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (tight serving loops, harnesses
+ * that forbid hipMalloc).  Its kernels' reduction scratch (per-workgroup
+ * partials of the end-of-step evaluation and of the last stage, and the
+ * reduction result) is a static device array of the library, one per
+ * device, instead of the context's buffers; the context itself still comes
+ * from fv_create.  Limit: 16384 workgroups per RHS launch (64 x 8 cells each,
+ * 8.4 M cells); larger meshes return an error.  Calls on one device share
+ * the array: calls that overlap on one device (other host threads, other
+ * streams) race.  Fails while fv_profile is enabled. */
+int fv_step_noalloc(fv_ctx *ctx, int *done);
 
 /* The same path cut at the Timedisc object, for a host that keeps fosite's
  * own fosite%Step, Datafile and output (a timedisc_base extension whose
