@@ -19,7 +19,8 @@ from . import setups
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.environ.get("FOSITE_FV_LIB") or os.path.join(HERE, "libfosite_fv.so")
 
-__all__ = ["Fosite", "Batch", "Server", "device_bytes", "FvConfig", "mesh_array", "setups", "load_library", "LIBPATH"]
+__all__ = ["Fosite", "Batch", "Server", "device_bytes", "FvConfig", "mesh_array", "setups", "load_library", "LIBPATH",
+           "restart"]
 
 
 class FvConfig(ctypes.Structure):
@@ -213,6 +214,23 @@ def mesh_array(cfg, name):
     if rc:
         raise ValueError("fv_mesh_array failed (%d)" % rc)
     return out
+
+
+def restart(path, bccsound=None, fcsound=None, w=None, **changes):
+    """A simulation restarted from a dataset in fosite's binary format, as
+    examples/restart.f90:89-170 restarts one: the file's /config tree with the
+    start time set to the file's time, its primitive variables as the initial
+    data (ghost cells from the boundaries at the first step), configuration
+    keys in ``changes`` overriding the file's.  The isothermal sound speeds
+    and the FARGO type-2 background velocity are not in the file: the caller
+    gives them, as restart.f90 computes its own.  Returns a Fosite ready for
+    first_step()."""
+    from . import datafile
+    r = datafile.load_restart(path)
+    cfg = dict(r["cfg"], **changes)
+    sim = Fosite(cfg)
+    sim.set_initial(r["pvar"], bccsound, fcsound, w)
+    return sim
 
 
 class Fosite:

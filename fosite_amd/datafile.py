@@ -296,3 +296,59 @@ def config_entries(cfg, filename=""):
         e += _tree("/config/", [("sources/grav/" + k, v) for k, v in grav])
     e.append(("/config/version", VERSION))
     return e
+
+
+def _val(d, key, default=None):
+    v = d.get(key)
+    if v is None:
+        return default
+    typ, x = v
+    if typ == CHAR:
+        return x.decode().rstrip()
+    return x
+
+
+def load_restart(path):
+    """A restart from a dataset, as examples/restart.f90:89-170 makes one:
+    the configuration is the file's /config tree with /timedisc/starttime set
+    to the file's /timedisc/time (restart.f90:100-106), the initial data its
+    primitive variables (interior cells; the ghost cells follow from the
+    boundaries at the first step) and the datafile step the number in the
+    file name (restart.f90:115-127).  Returns {"cfg", "pvar" ([V][NJ][NI],
+    ghosts zero), "time", "step"}.  What the file does not hold -- the
+    isothermal sound speeds, the FARGO type-2 background velocity -- the
+    caller supplies, as restart.f90 sets its sound speeds itself."""
+    d = read(path)
+    g = lambda k, dflt=None: _val(d, "/config/" + k, dflt)  # noqa: E731
+    physics = int(g("physics/problem"))
+    cfg = dict(physics=physics, geometry=int(g("mesh/geometry")), inum=int(g("mesh/inum")),
+               jnum=int(g("mesh/jnum")), xmin=float(g("mesh/xmin")), xmax=float(g("mesh/xmax")),
+               ymin=float(g("mesh/ymin")), ymax=float(g("mesh/ymax")), gparam=float(g("mesh/gparam")),
+               bc=[int(g("boundary/" + k)) for k in ("western", "eastern", "southern", "northern")],
+               limiter=int(g("fluxes/limiter")), theta=float(g("fluxes/theta")),
+               recvars=int(g("fluxes/variables")), method=int(g("timedisc/method")),
+               order=int(g("timedisc/order")), cfl=float(g("timedisc/cfl")),
+               stoptime=float(g("timedisc/stoptime")), tol_rel=float(g("timedisc/tol_rel")),
+               beta=float(g("timedisc/beta")), dtlimit=float(g("timedisc/dtlimit")),
+               pmin=float(g("timedisc/pmin")), rhomin=float(g("timedisc/rhomin")),
+               count=int(g("datafile/count", 1)), fargo=int(g("mesh/fargo/method", 0)))
+    tol = g("timedisc/tol_abs")
+    cfg["tol_abs"] = [float(x) for x in np.atleast_1d(tol)] + [0.001] * (4 - np.atleast_1d(tol).size)
+    if physics == setups.EULER:
+        cfg["gamma"] = float(g("physics/gamma"))
+    if _val(d, "/config/sources/grav/pmass/mass") is not None:
+        cfg.update(gravity=1, mass=float(g("sources/grav/pmass/mass")),
+                   outbound=int(g("sources/grav/pmass/outbound", 0)),
+                   acclimit=float(g("sources/grav/pmass/acclimit", -1.0)))
+        # the restart reads pvar only (restart.f90:373-720 LoadData): the
+        # point mass starts again from the configured mass
+    time = float(_val(d, "/timedisc/time"))
+    cfg["starttime"] = time
+    V = 4 if physics == setups.EULER else 3
+    NI, NJ = cfg["inum"] + 4, cfg["jnum"] + 4
+    pvar = np.zeros((V, NJ, NI))
+    for l, name in enumerate(PVAR_NAMES[physics]):
+        pvar[l, 2:-2, 2:-2] = np.asarray(_val(d, "/timedisc/" + name)).reshape(cfg["jnum"], cfg["inum"])
+    m = re.search(r"_(\d{4})\.bin$", path)
+    step = int(m.group(1)) if m else 0
+    return {"cfg": setups.full(cfg), "pvar": pvar, "time": time, "step": step}
