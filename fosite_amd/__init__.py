@@ -39,7 +39,8 @@ class FvConfig(ctypes.Structure):
                 ("mass", ctypes.c_double), ("GN", ctypes.c_double), ("dtype", ctypes.c_int),
                 ("outbound", ctypes.c_int), ("acclimit", ctypes.c_double),
                 ("pmin", ctypes.c_double), ("rhomin", ctypes.c_double),
-                ("count", ctypes.c_int), ("starttime", ctypes.c_double)]
+                ("count", ctypes.c_int), ("starttime", ctypes.c_double),
+                ("omega", ctypes.c_double), ("rotframe", ctypes.c_int)]
 
     @classmethod
     def from_dict(cls, cfg):

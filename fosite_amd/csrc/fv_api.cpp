@@ -947,6 +947,7 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
   if (c.gravity && c.geometry != LOGCYLINDRICAL) return bail("point-mass gravity requires a logcylindrical mesh");
   if (c.outbound < 0 || c.outbound > 4) return bail("outbound must be 0 (no accretion) or a boundary 1..4");
   if (c.outbound && !c.gravity) return bail("accretion (outbound) needs the point mass (gravity = 1)");
+  if (c.rotframe < 0 || c.rotframe > 2) return bail("rotframe must be 0 (off), 1 or 2 (disable_centaccel)");
   if (!set_tableau(c.method, c.order, ctx->tab)) return bail("unsupported time integrator (method/order)");
   if (mesh_build(c, ctx->mesh)) return bail("mesh construction failed");
   ctx->V = c.physics == EULER ? 4 : 3;
@@ -969,6 +970,19 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
     for (int i = 1; i < ctx->NI; i++)
       if (ctx->mesh.a[M_RDYS][(size_t)j * ctx->NI + i] != ctx->col_h[(size_t)C_RDYS * ctx->NJ + j])
         return bail("reconstruction offsets in y depend on i");
+  }
+  // rotating frame, centre at the origin (sources_rotframe.f90:110-175): the
+  // centrifugal acceleration caccel = (Omega*e_z x r) x Omega*e_z of the
+  // bary-centre position r = (posvec_x(i), posvec_y(j), 0); with the zero
+  // components of e_z the cross products reduce exactly to (Omega*x)*Omega
+  // and (Omega*y)*Omega.  Only interior cells read it (Omega*e_z is zero on
+  // ghost cells there).
+  if (c.rotframe) {
+    const double om = c.omega;
+    for (int i = 0; i < ctx->NI; i++)
+      ctx->ring_h[(size_t)R_CACX * ctx->NI + i] = (om * ctx->mesh.a[M_POSX][(size_t)GH * ctx->NI + i]) * om;
+    for (int j = 0; j < ctx->NJ; j++)
+      ctx->col_h[(size_t)C_CACY * ctx->NJ + j] = (om * ctx->mesh.a[M_POSY][(size_t)j * ctx->NI + GH]) * om;
   }
   if (const char *e = std::getenv("FV_HOST_CONTROL")) ctx->fast = std::atoi(e) == 0;
   // device resources
@@ -1012,6 +1026,8 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
   g.bdydz = ctx->mesh.dy * ctx->mesh.dz;
   g.bdzdx = ctx->mesh.dz * ctx->mesh.dx;
   g.fargo = c.fargo != 0; g.gravity = c.gravity != 0; g.curvilinear = c.geometry != CARTESIAN;
+  g.rotframe = c.rotframe;
+  g.twoomega = 2 * c.omega;
   g.dtype = c.dtype;
   g.stages = ctx->tab.m;
   g.tile_x = 64;

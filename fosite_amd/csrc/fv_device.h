@@ -18,12 +18,13 @@ enum Limiter { MINMOD = 1, MONOCENT = 2, SWEBY = 3, SUPERBEE = 4, OSPRE = 5, VAN
 // Per-ring (x index i, ghost-inclusive) metric and physics arrays.  On the
 // supported geometries every metric quantity depends on i only
 // (mesh/mesh_midpoint.f90:97-145 with geometry_logcylindrical.f90:310-319),
-// except the y reconstruction offsets, which depend on j only.
+// except the y reconstruction offsets and the y component of the rotating
+// frame's centrifugal acceleration, which depend on j only.
 enum RingId {
   R_DAXW, R_DAYS, R_DYDZDV, R_DZDXDV, R_DXDYDV, R_DLX, R_DLY, R_CXYX, R_CYXY, R_CZXZ, R_CZYZ,
-  R_RDXW, R_RDXE, R_ACCX, R_ACCY, R_CS, R_CSW, R_CSE, R_CSS, R_CSN, R_W, R_WFAC, R_COUNT
+  R_RDXW, R_RDXE, R_ACCX, R_ACCY, R_CS, R_CSW, R_CSE, R_CSS, R_CSN, R_W, R_WFAC, R_CACX, R_COUNT
 };
-enum ColId { C_RDYS, C_RDYN, C_COUNT };
+enum ColId { C_RDYS, C_RDYN, C_CACY, C_COUNT };
 
 struct Geom {
   int NI, NJ, P;        // ghost-inclusive cells per row / column, row pitch (elements)
@@ -36,6 +37,8 @@ struct Geom {
   int bc[4];
   double bdydz, bdzdx;  // Mesh%dy*Mesh%dz, Mesh%dz*Mesh%dx: boundary-flux weights
   int fargo, gravity, curvilinear;
+  int rotframe;         // rotating-frame source: 0 off, 1 centrifugal + Coriolis, 2 Coriolis only
+  double twoomega;      // 2*Omega (sources_rotframe.f90:162); centrifugal terms in R_CACX, C_CACY
   int tile_x;           // k_rhs tile width in cells: 64 or 32
   int stages;           // RK stages of the tableau (picks the kernels' coefficient slots)
   int dtype;            // stored state: 0 FP64 (plain drop-in), 1 FP32 (precision form)

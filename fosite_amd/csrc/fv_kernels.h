@@ -1047,6 +1047,24 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
       ext[2] = zero + ur[0] * ay;
       if (V == 4) ext[3] = zero + (ur[1] * ax + ur[2] * ay);
     }
+    // rotating frame, after gravity in the source list (sources_generic.f90:
+    // 133-199, 236-241; sources_rotframe.f90:226-234): centrifugal plus
+    // Coriolis acceleration of the real-frame velocities, through the same
+    // ExternalSources as gravity
+    if (g.rotframe) {
+      const T tw = (T)g.twoomega, vx = pr[1], vy = pr[2];
+      T ax, ay;
+      if (g.rotframe == 2) {
+        ax = tw * vy;
+        ay = -(tw * vx);
+      } else {
+        ax = rg<T>(g, R_CACX, i) + tw * vy;
+        ay = cl<T>(g, C_CACY, j) - tw * vx;
+      }
+      ext[1] = ext[1] + ur[0] * ax;
+      ext[2] = ext[2] + ur[0] * ay;
+      if (V == 4) ext[3] = ext[3] + (ur[1] * ax + ur[2] * ay);
+    }
     // CheckData (timedisc_base.f90:1666-1742) and CFL (timedisc_base.f90:818-909)
     if (want_check) {
       if (pr[0] < (T)g.rhomin) flags |= F_RHOMIN;

@@ -177,6 +177,7 @@ CVAR_NAMES = {setups.EULER_ISOTHERM: ("density", "xmomentum", "ymomentum"),
               setups.EULER: ("density", "xmomentum", "ymomentum", "energy")}
 VERSION = "0.8.3"  # CMakeLists.txt:18
 BINARY, LINEAR, KT, GRAVITY, POINTMASS, NEWTON = 1, 2, 1, 1, 1, 1
+ROTATING_FRAME = 20  # sources_generic.f90:77
 SI, GEOMETRICAL = 1, 3  # physics units (physics_base.f90:554)
 CHECK_ALL = -1
 
@@ -211,7 +212,7 @@ def config_entries(cfg, filename=""):
       timedisc   timedisc_base.f90:295-359 (starttime .. rhomin),
                  SetOutput 534-646 (output/*), timedisc_rkfehlberg.f90:121
       datafile   fileio_base.f90:420-470
-      sources    gravity_pointmass.f90, sources_gravity.f90
+      sources    gravity_pointmass.f90, sources_gravity.f90, sources_rotframe.f90
       version    fosite.f90:250
     The values are the ones fosite 0.8.3 writes (checked key by key against
     datasets the reference wrote)."""
@@ -229,10 +230,15 @@ def config_entries(cfg, filename=""):
         mesh.append(("fargo/method", fargo))
         if cfg["geometry"] == setups.CARTESIAN:  # along y (mesh_base.f90:390-404)
             mesh.append(("fargo/direction", 2))
+    omega = float(cfg.get("omega", 0.0))
+    if omega != 0.0:  # given by the setup
+        mesh.append(("omega", omega))
     if cfg["geometry"] == setups.CARTESIAN:
         mesh.append(("dz", 1.0))
-    mesh += [("shearingbox", 0), ("omega", 0.0), ("Qshear", 1.5),
-             ("rotation_center", np.zeros(3))]
+    mesh.append(("shearingbox", 0))
+    if omega == 0.0:
+        mesh.append(("omega", 0.0))
+    mesh += [("Qshear", 1.5), ("rotation_center", np.zeros(3))]
     if not fargo:
         mesh.append(("fargo/method", 0))
     mesh += [("output/" + k, v) for k, v in (
@@ -286,6 +292,7 @@ def config_entries(cfg, filename=""):
     e += _tree("/config/", [("mesh/" + k, v) for k, v in mesh] + [("physics/" + k, v) for k, v in phys]
                + [("boundary/" + k, v) for k, v in bnd] + [("fluxes/" + k, v) for k, v in flx]
                + [("timedisc/" + k, v) for k, v in td] + [("datafile/" + k, v) for k, v in dfile])
+    src = []
     if cfg.get("gravity", 0):
         pm = [("gtype", POINTMASS), ("potential", NEWTON), ("mass", float(cfg["mass"])),
               ("outbound", int(cfg.get("outbound", 0))), ("acclimit", float(cfg.get("acclimit", -1.0))),
@@ -293,7 +300,11 @@ def config_entries(cfg, filename=""):
               ("x", 0.0), ("y", 0.0), ("z", 0.0), ("softening", 0.0), ("switchon", -1.0)]
         grav = [("stype", GRAVITY)] + [("pmass/" + k, v) for k, v in pm]
         grav += [("update_disk_height", 0), ("energy", 1), ("output/height", 0), ("output/accel", 0)]
-        e += _tree("/config/", [("sources/grav/" + k, v) for k, v in grav])
+        src += [("grav/" + k, v) for k, v in grav]
+    if cfg.get("rotframe", 0):  # sources_rotframe.f90:86-89
+        src += [("rotframe/stype", ROTATING_FRAME),
+                ("rotframe/disable_centaccel", 1 if cfg["rotframe"] == 2 else 0)]
+    e += _tree("/config/", [("sources/" + k, v) for k, v in src])
     e.append(("/config/version", VERSION))
     return e
 
@@ -342,6 +353,9 @@ def load_restart(path):
                    acclimit=float(g("sources/grav/pmass/acclimit", -1.0)))
         # the restart reads pvar only (restart.f90:373-720 LoadData): the
         # point mass starts again from the configured mass
+    cfg["omega"] = float(g("mesh/omega", 0.0))
+    if _val(d, "/config/sources/rotframe/stype") is not None:
+        cfg["rotframe"] = 2 if int(g("sources/rotframe/disable_centaccel", 0)) else 1
     time = float(_val(d, "/timedisc/time"))
     cfg["starttime"] = time
     V = 4 if physics == setups.EULER else 3

@@ -32,6 +32,9 @@ MODULE fosite_fv_c
     REAL(c_double) :: pmin = -1, rhomin = -1
     INTEGER(c_int) :: count = 1
     REAL(c_double) :: starttime = 0
+    !> Mesh%omega and the rotating-frame source: 0 off, 1 on, 2 disable_centaccel
+    REAL(c_double) :: omega = 0
+    INTEGER(c_int) :: rotframe = 0
   END TYPE fv_config
 
   TYPE, BIND(C), PUBLIC :: fv_scalars

@@ -60,6 +60,10 @@ MODULE fosite_gpu_mod
     REAL(DP) :: mass = 1, GN = 1
     INTEGER  :: outbound = 0
     REAL(DP) :: acclimit = -1
+    !> rotating frame with angular speed omega about the origin
+    !! (sources_rotframe.f90:73-247): 0 off, 1 on, 2 disable_centaccel
+    INTEGER  :: rotframe = 0
+    REAL(DP) :: omega = 0
   END TYPE sources_gpu
 
   TYPE, PUBLIC :: timedisc_gpu
@@ -141,6 +145,8 @@ CONTAINS
     c%rhomin = this%Timedisc%rhomin
     c%count = this%Timedisc%count
     c%starttime = this%Timedisc%starttime
+    c%omega = this%Sources%omega
+    c%rotframe = this%Sources%rotframe
     c%dtype = 0
   END FUNCTION Config
 

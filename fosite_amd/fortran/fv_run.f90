@@ -31,13 +31,14 @@ PROGRAM fv_run
   REAL(DP) :: k_mass = 1, k_acclimit = -1
   INTEGER  :: k_outbound = 0
   INTEGER  :: k_nsteps = 10, k_dumpevery = 1, k_batch = 0, k_count = 1
-  REAL(DP) :: k_pmin = -1, k_rhomin = -1, k_starttime = 0
+  REAL(DP) :: k_pmin = -1, k_rhomin = -1, k_starttime = 0, k_omega = 0
+  INTEGER  :: k_rotframe = 0
   CHARACTER(LEN=512) :: k_infile = "", k_prefix = "out"
   NAMELIST /fvcfg/ k_physics, k_geometry, k_inum, k_jnum, k_gamma, k_xmin, k_xmax, k_ymin, k_ymax, &
        k_gparam, k_bcw, k_bce, k_bcs, k_bcn, k_limiter, k_recvars, k_theta, k_method, k_order,   &
        k_cfl, k_tolrel, k_tolabs, k_stoptime, k_dtlimit, k_beta, k_fargo, k_gravity, k_units,   &
        k_mass, k_outbound, k_acclimit, k_nsteps, k_dumpevery, k_batch, k_infile, k_prefix, &
-       k_count, k_pmin, k_rhomin, k_starttime
+       k_count, k_pmin, k_rhomin, k_starttime, k_omega, k_rotframe
 
   TYPE(fosite_gpu) :: Sim
   CHARACTER(LEN=512) :: cfgname
@@ -72,6 +73,8 @@ PROGRAM fv_run
   Sim%Sources%mass = k_mass
   Sim%Sources%outbound = k_outbound
   Sim%Sources%acclimit = k_acclimit
+  Sim%Sources%omega = k_omega
+  Sim%Sources%rotframe = k_rotframe
   Sim%Timedisc%method = k_method
   Sim%Timedisc%order = k_order
   Sim%Timedisc%cfl = k_cfl

@@ -56,6 +56,7 @@ MODULE timedisc_gpu_mod
   USE sources_generic_mod
   USE sources_gravity_mod, ONLY : sources_gravity
   USE gravity_pointmass_mod, ONLY : gravity_pointmass
+  USE sources_rotframe_mod, ONLY : sources_rotframe
   USE timedisc_base_mod
   USE timedisc_generic_mod
   USE fileio_generic_mod
@@ -177,8 +178,11 @@ CONTAINS
     c%outbound = 0
     c%acclimit = -1.0
     c%GN = Physics%constants%GN
-    ! the device evaluates one point mass at the origin: the only source term
-    ! it accepts
+    c%omega = Mesh%omega
+    c%rotframe = 0
+    ! the device evaluates one point mass at the origin and the rotating
+    ! frame about the origin, in that order (gravity heads the list,
+    ! sources_generic.f90:133-199): the source terms it accepts
     IF (ALLOCATED(Sources)) THEN
       sp => Sources%next
       DO WHILE (ASSOCIATED(sp))
@@ -195,8 +199,12 @@ CONTAINS
           CLASS DEFAULT
             CALL this%Error("timedisc_gpu","gravity other than a point mass")
           END SELECT
+        TYPE IS (sources_rotframe)
+          IF (ANY(Mesh%rotcent(:).NE.0.0)) &
+            CALL this%Error("timedisc_gpu","rotating frame about the origin only")
+          c%rotframe = MERGE(2, 1, sg%disable_centaccel)
         CLASS DEFAULT
-          CALL this%Error("timedisc_gpu","source terms other than point-mass gravity")
+          CALL this%Error("timedisc_gpu","source terms other than point-mass gravity and the rotating frame")
         END SELECT
         sp => sp%next
       END DO

@@ -16,6 +16,9 @@ A configuration is a plain dict whose keys follow the reference's Dict keys
             background velocity (mesh_base.f90:372-423)
   gravity, mass, GN                             (gravity_pointmass.f90:132-301)
   pmin, rhomin, starttime                       (timedisc_base.f90:295, 353-359)
+  omega     angular speed of the rotating frame  (mesh_base.f90:338)
+  rotframe  rotating-frame source: 0 off, 1 on, 2 with disable_centaccel
+            (sources_rotframe.f90:86-93, 212-247)
   count     datafile outputs between starttime and stoptime (fileio_base.f90:420-470)
 
 Initial data is returned as primitive variables on all cells (ghosts
@@ -43,7 +46,9 @@ DEFAULTS = dict(gamma=1.4, gparam=1.0, limiter=MINMOD, theta=1.0, recvars=CONSER
                 outbound=0, acclimit=-1.0,
                 # CheckData limits (<= 0: TINY, timedisc_base.f90:353-359),
                 # datafile count, start time (timedisc_base.f90:295)
-                pmin=-1.0, rhomin=-1.0, count=1, starttime=0.0)
+                pmin=-1.0, rhomin=-1.0, count=1, starttime=0.0,
+                # rotating frame (mesh_base.f90:338, sources_rotframe.f90)
+                omega=0.0, rotframe=0)
 
 
 def full(cfg):

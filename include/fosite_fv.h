@@ -66,6 +66,11 @@ typedef struct fv_config {
                        (io/fileio_base.f90:420-470, 487, 591-600); <= 0 selects 1 */
   double starttime; /* Timedisc%time at setup (timedisc/timedisc_base.f90:295); a restart
                        sets it to the dataset's time (examples/restart.f90:103-106) */
+  double omega;     /* Mesh%omega, angular speed of the rotating frame (mesh/mesh_base.f90:338) */
+  int rotframe;     /* rotating-frame source term (sources/sources_rotframe.f90:73-247), centre
+                       of rotation at the origin (the default rotation_center, mesh_base.f90:344):
+                       0 off, 1 centrifugal and Coriolis accelerations, 2 Coriolis only
+                       (disable_centaccel = 1) */
 } fv_config;
 
 typedef struct fv_scalars {

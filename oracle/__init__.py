@@ -43,7 +43,8 @@ class _Cfg(ctypes.Structure):
                 ("mass", ctypes.c_double), ("GN", ctypes.c_double),
                 ("outbound", ctypes.c_int), ("acclimit", ctypes.c_double),
                 ("pmin", ctypes.c_double), ("rhomin", ctypes.c_double),
-                ("count", ctypes.c_int), ("starttime", ctypes.c_double)]
+                ("count", ctypes.c_int), ("starttime", ctypes.c_double),
+                ("omega", ctypes.c_double), ("rotframe", ctypes.c_int)]
 
 
 class _Scalars(ctypes.Structure):
@@ -89,7 +90,7 @@ def _ptr(a):
 MESH_IDS = ["dAx_w", "dAx_e", "dAy_s", "dAy_n", "dydzdV", "dzdxdV", "dxdydV", "dlx", "dly",
             "cxyx", "cyxy", "czxz", "czyz", "rdx_w", "rdx_e", "rdx_s", "rdx_n", "radius",
             "radius_w", "radius_e", "radius_s", "radius_n", "posvec_x", "posvec_y",
-            "bcenter_x", "bcenter_y", "accel_x", "accel_y"]
+            "bcenter_x", "bcenter_y", "accel_x", "accel_y", "caccel_x", "caccel_y"]
 
 
 class Oracle:
@@ -128,6 +129,8 @@ class Oracle:
         c.rhomin = cfg.get("rhomin", -1.0)
         c.count = cfg.get("count", 1)
         c.starttime = cfg.get("starttime", 0.0)
+        c.omega = cfg.get("omega", 0.0)
+        c.rotframe = cfg.get("rotframe", 0)
         self._h = L.fvr_create(ctypes.byref(c))
         if not self._h:
             raise ValueError("oracle: unsupported configuration")

@@ -40,13 +40,15 @@ PROGRAM fv_driver
   INTEGER :: k_outbound = 0
   INTEGER :: k_binary = 0, k_count = 1
   REAL    :: k_pmin = -1.0, k_rhomin = -1.0, k_starttime = 0.0
+  REAL    :: k_omega = 0.0
+  INTEGER :: k_rotframe = 0
   INTEGER :: k_nsteps = 10, k_dumpevery = 1
   CHARACTER(LEN=512) :: k_infile = "", k_prefix = "out"
   NAMELIST /fvcfg/ k_physics, k_geometry, k_inum, k_jnum, k_gamma, k_xmin, k_xmax, k_ymin, k_ymax, &
        k_gparam, k_bcw, k_bce, k_bcs, k_bcn, k_limiter, k_recvars, k_theta, k_method, k_order,   &
        k_cfl, k_tolrel, k_tolabs, k_stoptime, k_dtlimit, k_beta, k_fargo, k_gravity, k_units,   &
        k_mass, k_outbound, k_acclimit, k_binary, k_nsteps, k_dumpevery, k_infile, k_prefix, &
-       k_count, k_pmin, k_rhomin, k_starttime
+       k_count, k_pmin, k_rhomin, k_starttime, k_omega, k_rotframe
   !--------------------------------------------------------------------------!
   CLASS(fosite), ALLOCATABLE :: Sim
   TYPE(Dict_TYP), POINTER    :: config, mesh, phys, bnd, flx, tdisc, dfile, &
@@ -77,6 +79,7 @@ PROGRAM fv_driver
   ! a Cartesian mesh has no azimuthal index: FARGO along y is named
   ! (mesh_base.f90:390-404)
   IF (k_fargo.GT.0 .AND. k_geometry.EQ.CARTESIAN) CALL SetAttr(mesh, "fargo/direction", 2)
+  IF (k_omega.NE.0.0) CALL SetAttr(mesh, "omega", k_omega)
   bnd => Dict("western" / k_bcw, "eastern" / k_bce, "southern" / k_bcs, &
        "northern" / k_bcn, "bottomer" / NO_GRADIENTS, "topper" / NO_GRADIENTS)
   IF (k_physics.EQ.EULER) THEN
@@ -113,12 +116,23 @@ PROGRAM fv_driver
   END IF
   config => Dict("mesh" / mesh, "physics" / phys, "boundary" / bnd, &
        "fluxes" / flx, "timedisc" / tdisc, "datafile" / dfile)
+  NULLIFY(src)
   IF (k_gravity.GT.0) THEN
     pmass => Dict("gtype" / POINTMASS, "potential" / NEWTON, "mass" / k_mass, &
          "outbound" / k_outbound, "acclimit" / k_acclimit)
     src => Dict("grav/stype" / GRAVITY, "grav/pmass" / pmass)
-    CALL SetAttr(config, "sources", src)
   END IF
+  ! k_rotframe = 1: the rotating-frame source, 2: the same with
+  ! disable_centaccel (sources/sources_rotframe.f90:86-93)
+  IF (k_rotframe.GT.0) THEN
+    IF (ASSOCIATED(src)) THEN
+      CALL SetAttr(src, "rotframe/stype", ROTATING_FRAME)
+    ELSE
+      src => Dict("rotframe/stype" / ROTATING_FRAME)
+    END IF
+    IF (k_rotframe.EQ.2) CALL SetAttr(src, "rotframe/disable_centaccel", 1)
+  END IF
+  IF (ASSOCIATED(src)) CALL SetAttr(config, "sources", src)
   Sim%config => config
   CALL Sim%Setup()
 

@@ -12,7 +12,8 @@
  * reconstruction (all non-PP limiters) of conservative or primitive
  * variables, embedded RK (RKF 3/5, Cash-Karp 5, Dormand-Prince 5, SSPRK 3/5)
  * with error control and reject path, no-gradient, periodic and absorbing
- * boundaries, point-mass gravity with accretion, FARGO along y for both
+ * boundaries, point-mass gravity with accretion, the rotating frame about
+ * the origin (centrifugal and Coriolis terms), FARGO along y for both
  * physics and geometries with the per-ring mean (type 1) or a user-supplied
  * (type 2) background velocity, CheckData breaks,
  * output times (datafile count) and a start time.
@@ -60,6 +61,7 @@ struct fvr {
   /* mesh arrays (all cells) */
   double *dAx[2], *dAy[2], *dydzdV, *dzdxdV, *dxdydV, *dlx, *dly;
   double *cxyx, *cyxy, *czxz, *czyz, *rdx[4], *rad, *radf[4], *posx, *posy, *bcx, *bcy;
+  double *caccel[2], *twoomega, *raccel[2]; /* rotating frame (sources_rotframe.f90:110-175, 226-234) */
   double *accel[2];
   double pm_mass, pm_accrate, pm_massloss, pm_mdot; /* gravity_pointmass.f90:79-85 */
   /* physics */
@@ -186,6 +188,25 @@ static void mesh_build(fvr *s) {
       s->accel[0][c] = 0.0 + (-(om2 * s->posx[c]));
       s->accel[1][c] = 0.0 + (-(om2 * s->posy[c]));
     }
+  }
+  /* sources_rotframe.f90:110-175 with the rotation centre at the origin:
+   * Omez = Omega*e_z on the interior cells, 0 on ghost cells, in curvilinear
+   * components (e_z is a basis vector of both geometries); vphi = Omez x r,
+   * caccel = vphi x Omez (marray_base.f90:855-897, axb1 = a2*b3 - a3*b2);
+   * r = posvec at the bary centres, z component 0 on these 2-D meshes;
+   * twoOmega = 2*Omez_z (VECTOR_X|VECTOR_Y) */
+  if (s->c.rotframe) {
+    for (int j = 0; j < s->NJ; j++)
+      for (int i = 0; i < s->NI; i++) {
+        const int c = IDX(s, i, j);
+        const int inner = i >= G && i < G + s->c.inum && j >= G && j < G + s->c.jnum;
+        const double o1 = 0.0, o2 = 0.0, o3 = inner ? s->c.omega : 0.0;
+        const double r1 = s->posx[c], r2 = s->posy[c], r3 = 0.0;
+        const double v1 = o2 * r3 - o3 * r2, v2 = o3 * r1 - o1 * r3, v3 = o1 * r2 - o2 * r1;
+        s->caccel[0][c] = v2 * o3 - v3 * o2;
+        s->caccel[1][c] = v3 * o1 - v1 * o3;
+        s->twoomega[c] = 2 * o3;
+      }
   }
 }
 
@@ -500,24 +521,53 @@ static void update_gravity(fvr *s, double dt) {
   s->pm_mdot = bflux;
 }
 
-/* sources_generic.f90:220-261, sources_gravity.f90:164-199,
- * physics_eulerisotherm.f90:1201-1226, physics_euler.f90:1849-1885;
+/* sources_generic.f90:220-261: sterm = 0, then every source of the list in
+ * order (gravity first, sources_generic.f90:133-199), each through
+ * Physics%ExternalSources (physics_eulerisotherm.f90:1201-1226,
+ * physics_euler.f90:1849-1885):
+ *  - point-mass gravity, sources_gravity.f90:164-199;
+ *  - the rotating frame, sources_rotframe.f90:212-247 (2-D acceleration:
+ *    caccel + twoOmega*v_y, caccel - twoOmega*v_x; with disable_centaccel
+ *    twoOmega*v_y, -twoOmega*v_x) with the real-frame velocities p;
  * dt < 0: no UpdateGravity (test aid) */
-static void external_sources(fvr *s, const double *c, double *st, double dt) {
-  int N = s->N, V = s->V;
-  if (!s->c.gravity) return;
-  if (dt >= 0.0) update_gravity(s, dt);
-  PAR_FOR
-  for (int k = 0; k < V * N; k++) st[k] = 0.0;
+static void source_term(const fvr *s, const double *c, const double *ax, const double *ay, int N, int V) {
   PAR_FOR
   for (int k = 0; k < N; k++) {
     s->sterm[k] = 0.0;
-    s->sterm[1 * N + k] = c[k] * s->accel[0][k];
-    s->sterm[2 * N + k] = c[k] * s->accel[1][k];
-    if (V == 4) s->sterm[3 * N + k] = c[1 * N + k] * s->accel[0][k] + c[2 * N + k] * s->accel[1][k];
+    s->sterm[1 * N + k] = c[k] * ax[k];
+    s->sterm[2 * N + k] = c[k] * ay[k];
+    if (V == 4) s->sterm[3 * N + k] = c[1 * N + k] * ax[k] + c[2 * N + k] * ay[k];
   }
+}
+
+static void external_sources(fvr *s, const double *p, const double *c, double *st, double dt) {
+  int N = s->N, V = s->V;
+  if (!s->c.gravity && !s->c.rotframe) return;
   PAR_FOR
-  for (int k = 0; k < V * N; k++) st[k] = st[k] + s->sterm[k];
+  for (int k = 0; k < V * N; k++) st[k] = 0.0;
+  if (s->c.gravity) {
+    if (dt >= 0.0) update_gravity(s, dt);
+    source_term(s, c, s->accel[0], s->accel[1], N, V);
+    PAR_FOR
+    for (int k = 0; k < V * N; k++) st[k] = st[k] + s->sterm[k];
+  }
+  if (s->c.rotframe) {
+    double *ax = s->raccel[0], *ay = s->raccel[1];
+    PAR_FOR
+    for (int k = 0; k < N; k++) {
+      const double tw = s->twoomega[k], vx = p[1 * N + k], vy = p[2 * N + k];
+      if (s->c.rotframe == 2) {
+        ax[k] = tw * vy;
+        ay[k] = -(tw * vx);
+      } else {
+        ax[k] = s->caccel[0][k] + tw * vy;
+        ay[k] = s->caccel[1][k] - tw * vx;
+      }
+    }
+    source_term(s, c, ax, ay, N, V);
+    PAR_FOR
+    for (int k = 0; k < V * N; k++) st[k] = st[k] + s->sterm[k];
+  }
   PAR_FOR
   for (int j = 0; j < s->NJ; j++)
     for (int i = 0; i < s->NI; i++)
@@ -766,7 +816,7 @@ static void compute_rhs(fvr *s, sv *p, sv *c, int checkbm, double *rhs, double d
   if (V == 4) update_bccsound(s, s->pvar.d);
   if (checkbm != CHECK_NOTHING) check_data(s, p->d, c->d, checkbm);
   geometrical_sources(s, p->d, c->d, s->geo_src);
-  external_sources(s, c->d, s->src, dt);
+  external_sources(s, p->d, c->d, s->src, dt);
   if (fargo) {
     if (s->c.geometry == CARTESIAN) memset(s->geo_src, 0, sizeof(double) * V * N);
     add_fargo_sources(s, p->d, c->d, s->geo_src);
@@ -1113,6 +1163,7 @@ fvr *fvr_create(const fvr_cfg *cfg) {
     if (b == ABSORBING && d >= 2) return NULL;
   }
   if (cfg->outbound < 0 || cfg->outbound > 4) return NULL;
+  if (cfg->rotframe < 0 || cfg->rotframe > 2) return NULL;
   s = (fvr *)calloc(1, sizeof(fvr));
   s->c = *cfg;
   s->pm_mass = cfg->mass; /* mdot, massloss, accrate start at 0 (gravity_pointmass.f90:206-209) */
@@ -1128,6 +1179,8 @@ fvr *fvr_create(const fvr_cfg *cfg) {
     s->cxyx = zalloc(N); s->cyxy = zalloc(N); s->czxz = zalloc(N); s->czyz = zalloc(N);
     for (int f = 0; f < 4; f++) { s->rdx[f] = zalloc(N); s->radf[f] = zalloc(N); }
     s->rad = zalloc(N); s->posx = zalloc(N); s->posy = zalloc(N); s->bcx = zalloc(N); s->bcy = zalloc(N);
+    s->caccel[0] = zalloc(N); s->caccel[1] = zalloc(N); s->twoomega = zalloc(N);
+    s->raccel[0] = zalloc(N); s->raccel[1] = zalloc(N);
     s->bccsound = zalloc(N); s->fcsound = zalloc(4 * N); s->w = zalloc(s->NI); s->wfac = zalloc(s->NI);
     s->pvar.d = zalloc(V * N); s->cvar.d = zalloc(V * N); s->ptmp.d = zalloc(V * N);
     s->ctmp.d = zalloc(V * N); s->cold.d = zalloc(V * N);
@@ -1178,6 +1231,7 @@ void fvr_destroy(fvr *s) {
   free(s->cxyx); free(s->cyxy); free(s->czxz); free(s->czyz);
   for (int f = 0; f < 4; f++) { free(s->rdx[f]); free(s->radf[f]); }
   free(s->rad); free(s->posx); free(s->posy); free(s->bcx); free(s->bcy);
+  free(s->caccel[0]); free(s->caccel[1]); free(s->twoomega); free(s->raccel[0]); free(s->raccel[1]);
   free(s->bccsound); free(s->fcsound); free(s->w); free(s->wfac);
   free(s->pvar.d); free(s->cvar.d); free(s->ptmp.d); free(s->ctmp.d); free(s->cold.d);
   free(s->geo_src); free(s->src); free(s->sterm); free(s->cerr); free(s->xflux); free(s->yflux);
@@ -1194,11 +1248,12 @@ int fvr_ncells(const fvr *s) { return s->N; }
 int fvr_vnum(const fvr *s) { return s->V; }
 
 int fvr_mesh(const fvr *s, int id, double *out) {
-  const double *src[28] = {s->dAx[0], s->dAx[1], s->dAy[0], s->dAy[1], s->dydzdV, s->dzdxdV, s->dxdydV,
+  const double *src[30] = {s->dAx[0], s->dAx[1], s->dAy[0], s->dAy[1], s->dydzdV, s->dzdxdV, s->dxdydV,
                            s->dlx, s->dly, s->cxyx, s->cyxy, s->czxz, s->czyz, s->rdx[0], s->rdx[1],
                            s->rdx[2], s->rdx[3], s->rad, s->radf[0], s->radf[1], s->radf[2], s->radf[3],
-                           s->posx, s->posy, s->bcx, s->bcy, s->accel[0], s->accel[1]};
-  if (id < 0 || id >= 28) return -1;
+                           s->posx, s->posy, s->bcx, s->bcy, s->accel[0], s->accel[1],
+                           s->caccel[0], s->caccel[1]};
+  if (id < 0 || id >= 30) return -1;
   memcpy(out, src[id], sizeof(double) * s->N);
   return 0;
 }

@@ -39,6 +39,9 @@ typedef struct fvr_cfg {
   double pmin, rhomin;/* CheckData limits, <= 0: TINY (timedisc_base.f90:353-359) */
   int count;          /* datafile count, <= 0: 1 (fileio_base.f90:420-470) */
   double starttime;   /* timedisc_base.f90:295 */
+  double omega;       /* angular speed of the rotating frame (mesh_base.f90:338) */
+  int rotframe;       /* rotating-frame source (sources_rotframe.f90): 0 off, 1 on,
+                         2 on with disable_centaccel; rotation centre at the origin */
 } fvr_cfg;
 
 typedef struct fvr_scalars {
@@ -55,7 +58,8 @@ int fvr_vnum(const fvr *s);
  * id 0 dAxdydz(W) 1 dAxdydz(E) 2 dAydzdx(S) 3 dAydzdx(N) 4 dydzdV 5 dzdxdV
  * 6 dxdydV 7 dlx 8 dly 9 cxyx 10 cyxy 11 czxz 12 czyz 13..16 reconstruction
  * offsets W,E,S,N 17 radius(bcenter) 18..21 radius(faces W,E,S,N)
- * 22 posvec_x 23 posvec_y 24 bcenter x 25 bcenter y 26 accel_x 27 accel_y */
+ * 22 posvec_x 23 posvec_y 24 bcenter x 25 bcenter y 26 accel_x 27 accel_y
+ * 28 29 centrifugal acceleration of the rotating frame, x and y */
 int fvr_mesh(const fvr *s, int id, double *out);
 void fvr_set_sound_speeds(fvr *s, const double *bccsound, const double *fcsound);
 void fvr_set_background_velocity(fvr *s, const double *w);

@@ -39,7 +39,8 @@ def _nml(cfg, nsteps, dump_every, infile, prefix):
                 k_acclimit=float(cfg.get("acclimit", -1.0)), k_binary=int(cfg.get("binary_output", 0)),
                 k_nsteps=nsteps, k_dumpevery=dump_every, k_count=int(cfg.get("count", 1)),
                 k_pmin=float(cfg.get("pmin", -1.0)), k_rhomin=float(cfg.get("rhomin", -1.0)),
-                k_starttime=float(cfg.get("starttime", 0.0)))
+                k_starttime=float(cfg.get("starttime", 0.0)), k_omega=float(cfg.get("omega", 0.0)),
+                k_rotframe=int(cfg.get("rotframe", 0)))
     lines = ["&fvcfg"]
     for k, v in vals.items():
         lines.append(" %s = %s" % (k, repr(float(v)) if isinstance(v, float) else v))

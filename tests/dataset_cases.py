@@ -19,4 +19,10 @@ CASES = {
     "disk_fargo1_16x32": (dict(S.kepler_disk_config(ni=16, nj=32), stoptime=0.05, fargo=1,
                                dtlimit=1e-10, outbound=2,
                                bc=[S.NO_GRADIENTS, S.NO_GRADIENTS, S.PERIODIC, S.PERIODIC]), 20),
+    # isothermal disk in a rotating frame (centrifugal and Coriolis terms)
+    # with the point mass and FARGO type 2
+    "disk_rotframe16x32": (dict(S.kepler_disk_config(ni=16, nj=32), stoptime=0.05, omega=0.3, rotframe=1), 3),
+    # Euler Riemann problem in a rotating frame, Coriolis terms only
+    # (disable_centaccel), no gravity
+    "riemann_rotframe16": (dict(S.riemann2d_config(n=16), stoptime=0.02, omega=2.0, rotframe=2), 8),
 }

@@ -87,6 +87,16 @@ CASES = {
     "shear_iso_fargo2": (S.shear_box_config(n=16, physics=S.EULER_ISOTHERM), 10, (10,)),
     "euler_disk_fargo2": (dict(S.euler_disk_config(ni=16, nj=32), fargo=2), 8, (8,)),
     "euler_disk_fargo1_prim": (dict(S.euler_disk_config(ni=16, nj=32), fargo=1, recvars=S.PRIMITIVE), 8, (8,)),
+    # the rotating frame (sources_rotframe.f90:73-247), after gravity in the
+    # source list; centrifugal and Coriolis terms, or Coriolis only
+    # (disable_centaccel)
+    "disk_rotframe": (dict(D, omega=0.3, rotframe=1), 8, (1, 8)),
+    "disk_rotframe_nocent_fargo1": (dict(D, omega=0.5, rotframe=2, fargo=1), 8, (8,)),
+    "disk_rotframe_nograd_no_fargo": (dict(D, omega=0.3, rotframe=1, fargo=0,
+                                           bc=[S.NO_GRADIENTS, S.NO_GRADIENTS, S.PERIODIC, S.PERIODIC]), 8, (8,)),
+    "euler_rotframe": (dict(E, omega=2.0, rotframe=1), 8, (1, 8)),
+    "euler_disk_rotframe_prim": (dict(S.euler_disk_config(ni=16, nj=32), omega=0.2, rotframe=1,
+                                      recvars=S.PRIMITIVE), 8, (8,)),
 }
 MESHES = {"cartesian_16": E, "logcyl_16x32": D}
 SCALARS = ("iter", "n_adj", "brk", "time", "dt", "dtold", "maxerrold")

@@ -26,6 +26,8 @@ CASES = {
     # FARGO type 1 with Dormand-Prince and absorbing boundaries; restarted from the 1st of 2
     "disk_fargo1_dp5": (dict(S.kepler_disk_config(ni=20, nj=32), fargo=1, method=S.DORMAND_PRINCE, order=5,
                              stoptime=0.5, count=2), 1),
+    # an Euler Riemann problem in a rotating frame (Coriolis terms only); restarted from the 2nd of 3
+    "riemann_rotframe": (dict(S.riemann2d_config(n=24), stoptime=0.06, count=3, omega=2.0, rotframe=2), 2),
 }
 
 
