@@ -69,8 +69,8 @@ class FvScalars(ctypes.Structure):
 
 # every entry point of include/fosite_fv.h
 EXPORTS = ["fv_mesh_array", "fv_create", "fv_destroy", "fv_dims", "fv_set_sound_speeds",
-           "fv_set_background_velocity", "fv_set_pvar", "fv_first_step", "fv_step", "fv_run",
-           "fv_get_state", "fv_get_scalars", "fv_get_boundary_flux", "fv_get_boundary_flux_total",
+           "fv_set_background_velocity", "fv_set_pvar", "fv_set_cvar", "fv_first_step", "fv_step",
+           "fv_run", "fv_get_state", "fv_get_scalars", "fv_get_boundary_flux", "fv_get_boundary_flux_total",
            "fv_get_background_velocity", "fv_get_point_mass",
            "fv_last_run_ms", "fv_last_error", "fv_profile",
            "fv_kernel_stats", "fv_batch_create", "fv_batch_add", "fv_batch_remove", "fv_batch_run",
@@ -131,6 +131,7 @@ def load_library():
     L.fv_set_sound_speeds.argtypes = [V, P, P]
     L.fv_set_background_velocity.argtypes = [V, P]
     L.fv_set_pvar.argtypes = [V, P]
+    L.fv_set_cvar.argtypes = [V, P]
     L.fv_first_step.argtypes = [V, ctypes.POINTER(ctypes.c_int)]
     L.fv_step.argtypes = [V, ctypes.POINTER(ctypes.c_int)]
     L.fv_step_noalloc.argtypes = [V, ctypes.POINTER(ctypes.c_int)]

@@ -1170,6 +1170,29 @@ extern "C" int fv_set_background_velocity(fv_ctx *ctx, const double *w) {
   return upload_geom(ctx);
 }
 
+// the conservative state of all cells into buffer 0 (rounded to FP32 in the
+// precision form); the stored state then starts in the real frame
+static int upload_state(fv_ctx *ctx, const double *cv_in) {
+  const int V = ctx->V;
+  const size_t N = (size_t)ctx->NI * ctx->NJ;
+  ctx->cvar = ctx->cold = 0;
+  const size_t es = elem_bytes(ctx);
+  std::vector<float> cf;
+  const void *src = cv_in;
+  if (ctx->c.dtype == 1) {  // precision form: conserved variables from FP64, stored rounded
+    cf.assign(cv_in, cv_in + V * N);
+    src = cf.data();
+  }
+  for (int l = 0; l < V; l++)
+    HIPCHK(hipMemcpy2DAsync(field(ctx, ctx->buf[0], l), es * ctx->P, static_cast<const char *>(src) + l * N * es,
+                            es * ctx->NI, es * ctx->NI, ctx->NJ, hipMemcpyHostToDevice, ctx->st));
+  HIPCHK(hipStreamSynchronize(ctx->st));
+  ctx->fl_pvar = ctx->fl_cvar = 0;
+  ctx->have_state = true;
+  ctx->pvalid = false;
+  return 0;
+}
+
 extern "C" int fv_set_pvar(fv_ctx *ctx, const double *pvar) {
   if (!ctx || !pvar) return -1;
   const int V = ctx->V;
@@ -1183,22 +1206,12 @@ extern "C" int fv_set_pvar(fv_ctx *ctx, const double *pvar) {
     cv[2 * N + k] = rho * v;
     if (V == 4) cv[3 * N + k] = pvar[3 * N + k] / (g - 1.) + 0.5 * rho * (u * u + v * v);
   }
-  ctx->cvar = ctx->cold = 0;
-  const size_t es = elem_bytes(ctx);
-  std::vector<float> cf;
-  const void *src = cv.data();
-  if (ctx->c.dtype == 1) {  // precision form: conserved variables from FP64, stored rounded
-    cf.assign(cv.begin(), cv.end());
-    src = cf.data();
-  }
-  for (int l = 0; l < V; l++)
-    HIPCHK(hipMemcpy2DAsync(field(ctx, ctx->buf[0], l), es * ctx->P, static_cast<const char *>(src) + l * N * es,
-                            es * ctx->NI, es * ctx->NI, ctx->NJ, hipMemcpyHostToDevice, ctx->st));
-  HIPCHK(hipStreamSynchronize(ctx->st));
-  ctx->fl_pvar = ctx->fl_cvar = 0;
-  ctx->have_state = true;
-  ctx->pvalid = false;
-  return 0;
+  return upload_state(ctx, cv.data());
+}
+
+extern "C" int fv_set_cvar(fv_ctx *ctx, const double *cvar) {
+  if (!ctx || !cvar) return -1;
+  return upload_state(ctx, cvar);
 }
 
 extern "C" int fv_first_rhs(fv_ctx *ctx, int *brk) {

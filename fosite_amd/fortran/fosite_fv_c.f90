@@ -43,7 +43,7 @@ MODULE fosite_fv_c
   END TYPE fv_scalars
 
   PUBLIC :: fv_mesh_array, fv_create, fv_destroy, fv_dims, fv_set_sound_speeds, &
-       fv_set_background_velocity, fv_set_pvar, fv_first_step, fv_step, fv_run, &
+       fv_set_background_velocity, fv_set_pvar, fv_set_cvar, fv_first_step, fv_step, fv_run, &
        fv_get_state, fv_get_scalars, fv_last_run_ms, fv_last_error, fv_profile, &
        fv_kernel_stats, fv_get_background_velocity, fv_get_boundary_flux, &
        fv_get_boundary_flux_total, fv_get_point_mass, c_string, &
@@ -85,6 +85,11 @@ MODULE fosite_fv_c
       TYPE(c_ptr), VALUE :: ctx
       REAL(c_double), INTENT(IN) :: pvar(*)
     END FUNCTION fv_set_pvar
+    INTEGER(c_int) FUNCTION fv_set_cvar(ctx, cvar) BIND(C, NAME="fv_set_cvar")
+      IMPORT :: c_int, c_ptr, c_double
+      TYPE(c_ptr), VALUE :: ctx
+      REAL(c_double), INTENT(IN) :: cvar(*)
+    END FUNCTION fv_set_cvar
     INTEGER(c_int) FUNCTION fv_first_step(ctx, brk) BIND(C, NAME="fv_first_step")
       IMPORT :: c_int, c_ptr
       TYPE(c_ptr), VALUE :: ctx

@@ -279,8 +279,10 @@ CONTAINS
       rc = fv_set_background_velocity(this%ctx, this%w(:,Mesh%KGMIN))
       CALL Check(rc, "fv_set_background_velocity")
     END IF
-    rc = fv_set_pvar(this%ctx, pvar%data1d)
-    CALL Check(rc, "fv_set_pvar")
+    ! the state itself: FirstStep's CenterBoundary has already replaced pvar
+    ! with Convert2Primitive(cvar) under FARGO type 1 (fosite.f90:307-313)
+    rc = fv_set_cvar(this%ctx, cvar%data1d)
+    CALL Check(rc, "fv_set_cvar")
     rc = fv_first_rhs(this%ctx, brk)
     CALL Check(rc, "fv_first_rhs")
     IF (brk.NE.0) this%break = .TRUE.

@@ -110,6 +110,13 @@ int fv_set_background_velocity(fv_ctx *ctx, const double *w);
 /* InitData: Timedisc%pvar = pvar, then Physics%Convert2Conservative
  * (physics/physics_eulerisotherm.f90:426-449). */
 int fv_set_pvar(fv_ctx *ctx, const double *pvar);
+/* Timedisc%cvar (timedisc/timedisc_base.f90:87-88), the conservative state
+ * as the host holds it, all cells; no conversion.  The Timedisc-level
+ * binding hands the state over this way: fosite's FirstStep has by then
+ * replaced pvar with Convert2Primitive(cvar) in CenterBoundary
+ * (fosite.f90:307-313), which a second Convert2Conservative would not
+ * invert bit for bit. */
+int fv_set_cvar(fv_ctx *ctx, const double *cvar);
 
 /* fosite%FirstStep (fosite.f90:258-348): full RHS with data checks, CFL
  * time step, initial output frame change.  Returns the break flag in *brk. */

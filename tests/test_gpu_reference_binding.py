@@ -30,6 +30,12 @@ CASES = {
                                        order=5, stoptime=0.5, count=4), 30, 1e-12),
     # Euler with FARGO along y on a Cartesian shear box, SSPRK3
     "euler shear box fargo2": (dict(S.shear_box_config(n=32), stoptime=0.09, count=3), 40, 0.0),
+    # a rotating frame (sources_rotframe.f90) after the point mass, FARGO type 1, 3 outputs
+    "disk rotframe fargo1": (dict(S.kepler_disk_config(ni=40, nj=64), bc=NG, fargo=1, omega=0.3, rotframe=1,
+                                  stoptime=0.8, count=3), 30, 0.0),
+    # Euler in a rotating frame with the Coriolis terms only (disable_centaccel)
+    "riemann rotframe nocent": (dict(S.riemann2d_config(n=32), stoptime=0.03, count=2, omega=2.0, rotframe=2),
+                                30, 0.0),
 }
 
 
