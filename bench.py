@@ -9,7 +9,9 @@ whole grid).  Inputs are resident in HBM before the timed region.
 
 Prints one JSON line:
   value      cell-updates/s (interior cells x accepted steps / device time)
-  roofline   fused RHS kernel: algorithmic bytes per launch / mean launch time
+  roofline   the dominant kernel (largest share of the profiled device time):
+             algorithmic bytes per launch / mean launch time against 8 TB/s,
+             with its measured HBM traffic and its FP64 VALU-issue floor
   cpu_baseline  the compiled reference (oracle/_ref/fv_driver), or the C
              oracle if the reference build is absent, on a bounded sample
 """
@@ -23,10 +25,22 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the stage-2 RHS kernel on this workload: rocprofv3
-# FETCH_SIZE x2.0 + WRITE_SIZE x1.0 (factors from a 1 GiB copy kernel),
-# 13547 KiB and 12411 KiB per dispatch (profiles/k_rhs_disk512x1024_hbm_traffic.md)
-TRAFFIC_BYTES_PER_LAUNCH = (2.0 * 13547 + 12411) * 1024
+# HBM bytes per launch on this workload: rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE
+# x1.0 (factors from a 1 GiB copy kernel), KiB per dispatch
+# (profiles/k_rhs_disk512x1024_hbm_traffic.md)
+TRAFFIC_BYTES_PER_LAUNCH = {"rhs": (2.0 * 13547 + 12425) * 1024, "rhs_combine": (2.0 * 20783 + 12519) * 1024,
+                            "remap": (2.0 * 6633 + 12384) * 1024, "rhs_end": (2.0 * 7726 + 28216) * 1024}
+# VALU-issue floor per launch in us: SQ_INSTS_VALU x 4 cycles + SQ_INSTS_VALU_TRANS_F64
+# x 12 more per wave, x waves per SIMD, at 2.4 GHz
+# (profiles/k_rhs_disk512x1024_sq_counters.md)
+VALU_FLOOR_US = {"rhs": 9.55, "rhs_combine": 12.87, "remap": 2.69, "rhs_end": 10.80}
+KERNEL_NAMES = {
+    "rhs": "fv::k_rhs<double,3,VANLEER,64,3,M_STAGE>: stage 2 (stage sum over k_1, ghost cells, RHS)",
+    "rhs_combine": "fv::k_rhs<double,3,VANLEER,64,3,M_LAST>: stage 3 with the final combination and "
+                   "embedded error",
+    "rhs_end": "fv::k_rhs<double,3,VANLEER,64,3,M_END>: end-of-step RHS (CFL, data checks, new state)",
+    "remap": "fv::k_remap<double,3>: FARGO azimuthal remap",
+}
 METRIC = "cell-updates/s & HBM GB/s vs plain FP64 at stated err; p50/p99 run latency"
 
 
@@ -191,12 +205,15 @@ def precision_form(ni, nj, steps):
             "mass_drift_fp64": (m64 - m0) / m0, "mass_drift_fp32": (m32 - m0) / m0}
 
 
-def batched_form(n=64, steps=500, alone_steps=50):
+def batched_form(n=64, steps=500, alone_steps=50, plain_fp64=None, err_members=4, err_steps=20):
     """The batched form: n independent isothermal disks of uneven sizes in
     one batch (every kernel of a step one launch over all disks, each round
     one HIP graph launch, tiling chosen by the autotuner), `steps` SSPRK3
-    steps with FARGO.  Beside it: the same disks as an FP64 batch, and each
-    disk alone on the plain FP64 path (fv_run, `alone_steps` steps each)."""
+    steps with FARGO.  Beside it: the same disks as an FP64 batch, each disk
+    alone on the plain FP64 path (fv_run, `alone_steps` steps each), the
+    plain FP64 rate of the 512x1024 disk (plain_fp64, cell-updates/s), and
+    the FP32 members' error: the first err_members disks after err_steps
+    steps in FP32 and in FP64, relative L1 per field (momentum as a vector)."""
     import fosite_amd
     from fosite_amd import setups
 
@@ -236,11 +253,37 @@ def batched_form(n=64, steps=500, alone_steps=50):
         sim.close()
     out["fp64_alone"] = {"cell_updates_per_s": upd / (ms * 1e-3), "ms": ms, "steps_per_disk": alone_steps}
     cells = sum(c["inum"] * c["jnum"] for c in cfgs)
-    return {"workload": "%d isothermal Keplerian disks, radial cells from {128,256,384,512}, azimuthal = 2x, "
-                        "cs/vK U(0.03,0.08), FARGO, SSPRK3, %d steps (%d cells in all)" % (n, steps, cells),
-            "fp32_batch": out["fp32"], "fp64_batch": out["fp64"], "fp64_alone": out["fp64_alone"],
-            "speedup_fp32_batch_over_fp64_alone": out["fp32"]["cell_updates_per_s"] /
-            out["fp64_alone"]["cell_updates_per_s"]}
+    # FP32 members against the same members in FP64 (inertial frame, interior cells)
+    import numpy as np
+    errs = []
+    for c32, c64 in zip(setups.batch_disks(n=n, steps=steps, dtype="fp32")[:err_members], cfgs[:err_members]):
+        st = []
+        for c in (c32, c64):
+            (sim,) = make([c])
+            sim.run(err_steps)
+            cv, _ = sim.state()
+            if c.get("fargo", 0):  # the stored azimuthal momentum is relative to rho*w
+                cv[2] = cv[2] + cv[0] * sim.background_velocity()[None, :]
+            st.append((cv[:, 2:-2, 2:-2], sim.scalars()["time"]))
+            sim.close()
+        (a, ta), (b, tb) = st
+        mom = np.sqrt(b[1] ** 2 + b[2] ** 2)
+        errs.append({"inum": c64["inum"], "jnum": c64["jnum"], "time_fp32": ta, "time_fp64": tb,
+                     "density": float(np.abs(a[0] - b[0]).sum() / np.abs(b[0]).sum()),
+                     "momentum": float(np.sqrt((a[1] - b[1]) ** 2 + (a[2] - b[2]) ** 2).sum() / mom.sum())})
+    fp32 = out["fp32"]["cell_updates_per_s"]
+    res = {"workload": "%d isothermal Keplerian disks, radial cells from {128,256,384,512}, azimuthal = 2x, "
+                       "cs/vK U(0.03,0.08), FARGO, SSPRK3, %d steps (%d cells in all)" % (n, steps, cells),
+           "fp32_batch": out["fp32"], "fp64_batch": out["fp64"], "fp64_alone": out["fp64_alone"],
+           "speedup_fp32_batch_over_fp64_alone": fp32 / out["fp64_alone"]["cell_updates_per_s"],
+           "speedup_fp32_batch_over_fp64_batch": fp32 / out["fp64"]["cell_updates_per_s"],
+           "fp32_error_vs_fp64": {"members": err_members, "steps": err_steps, "per_member": errs,
+                                  "max_density": max(e["density"] for e in errs),
+                                  "max_momentum": max(e["momentum"] for e in errs)}}
+    if plain_fp64:
+        res["plain_fp64_512x1024"] = plain_fp64
+        res["speedup_fp32_batch_over_plain_fp64_512x1024"] = fp32 / plain_fp64
+    return res
 
 
 def serving_form(n=300, rate=20.0, nmin=64, nmax=256, orbits=10.0, budget_gb=64.0, round_steps=32):
@@ -408,12 +451,16 @@ def main():
     sim.run(a.profile_steps)
     stats = {k: sim.kernel_stats(k) for k in ("rhs", "rhs_combine", "rhs_end", "remap", "control")}
     sim.profile(False)
-    n_rhs, ms_rhs, bytes_rhs = stats["rhs"]
-    achieved = bytes_rhs / (ms_rhs * 1e-3) / 1e9
     kernels = {k: {"launches": n, "mean_launch_us": 1e3 * t / max(n, 1),
                    "alg_bytes_per_launch": b / max(n, 1),
                    "alg_GBps": b / (t * 1e-3) / 1e9 if t > 0 else None}
                for k, (n, t, b) in stats.items()}
+    # the dominant kernel: the largest share of the profiled device time
+    dom = max(TRAFFIC_BYTES_PER_LAUNCH, key=lambda k: stats[k][1])
+    n_dom, ms_dom, bytes_dom = stats[dom]
+    mean_us = 1e3 * ms_dom / max(n_dom, 1)
+    achieved = bytes_dom / (ms_dom * 1e-3) / 1e9
+    floor_us = VALU_FLOOR_US[dom]
     out = {
         "metric": METRIC,
         "value": value,
@@ -436,11 +483,15 @@ def main():
                    "host_io_ms": {"upload": upload_ms, "download": download_ms},
                    "value_with_host_io": cells * taken / ((ms + upload_ms + download_ms) * 1e-3)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_BYTES_PER_LAUNCH,
-                     "kernel": "fv::k_rhs<double,3,VANLEER,64,3,M_STAGE> on a stage state (fused stage sum, "
-                               "ghost cells, RHS)",
-                     "launches": n_rhs, "mean_launch_us": 1e3 * ms_rhs / max(n_rhs, 1),
-                     "alg_bytes_per_launch": bytes_rhs / max(n_rhs, 1),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_BYTES_PER_LAUNCH[dom],
+                     "kernel": KERNEL_NAMES[dom],
+                     "launches": n_dom, "mean_launch_us": mean_us,
+                     "alg_bytes_per_launch": bytes_dom / max(n_dom, 1),
+                     "device_time_share": ms_dom / sum(t for _, t, _ in stats.values()),
+                     # second ceiling: FP64 VALU issue (IEEE divisions keep the
+                     # reciprocals on the transcendental unit)
+                     "valu_issue_floor_us": floor_us, "valu_issue_frac": floor_us / mean_us,
+                     "valu_ceiling_GBps": bytes_dom / max(n_dom, 1) / (floor_us * 1e-6) / 1e9,
                      "timing": "HIP events around each launch on the simulation stream, "
                                "%d device-controlled steps after the timed region" % a.profile_steps},
         "kernels": kernels,
@@ -448,7 +499,7 @@ def main():
     sim.close()
     if not a.no_forms:
         out["precision_form"] = precision_form(1024, 2048, a.form_steps)
-        out["batched_form"] = batched_form()
+        out["batched_form"] = batched_form(plain_fp64=value)
         out["serving_form"] = serving_form()
         out["serving_form_full_sizes"] = serving_form_full()
         out["accretion_variant"] = accretion_variant()

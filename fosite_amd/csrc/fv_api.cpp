@@ -433,7 +433,9 @@ void launch_stage(fv_ctx *ctx, const Geom &g, const StageArgs &a, int m) {
   const bool last = m == ctx->tab.m - 1;
   long pk = prof_begin(ctx);
   launch_rhs(g, a, ctx->V, ctx->st);
-  prof_end(ctx, pk, last ? K_RHS_COMBINE : K_RHS, (last ? m + 1.0 : m + 2.0) * word_bytes(ctx));
+  // stage m reads the state and m slopes and writes one slope; the last one
+  // writes the combined state instead (m + 2 words per cell either way)
+  prof_end(ctx, pk, last ? K_RHS_COMBINE : K_RHS, (m + 2.0) * word_bytes(ctx));
 }
 
 // ComputeRHS on the stage state (ptmp, ctmp), ctmp = cvar - sum a*dt*k
