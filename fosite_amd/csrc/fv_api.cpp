@@ -1533,6 +1533,12 @@ struct fv_batch {
   RhsEntry *d_rhs = nullptr, *h_rhs = nullptr;          // [2 parities][m launches][cap]
   RemapEntry *d_remap = nullptr, *h_remap = nullptr;    // [2][cap]
   ControlEntry *d_cent = nullptr, *h_cent = nullptr;    // [NCAND][cap]
+  // point-mass accretion members: the step's boundary-flux accumulation and
+  // UpdateGravity before the end-of-step evaluation, [2 parities][cap] each,
+  // acc_n entries used (members in order), acc_bpm workgroups per member
+  BfluxArgs *d_bfl = nullptr, *h_bfl = nullptr;
+  AccreteArgs *d_acc = nullptr, *h_acc = nullptr;
+  int acc_n = 0, acc_bpm = 0;
   unsigned *d_tiles = nullptr;
   size_t tiles_cap = 0;
   int cand_off[NCAND] = {}, cand_n[NCAND] = {}, remap_off = 0, remap_n = 0;
@@ -1581,6 +1587,12 @@ void free_tables(fv_batch *b) {
   if (b->h_rhs) (void)hipHostFree(b->h_rhs);
   if (b->h_remap) (void)hipHostFree(b->h_remap);
   if (b->h_cent) (void)hipHostFree(b->h_cent);
+  if (b->d_bfl) (void)hipFree(b->d_bfl);
+  if (b->h_bfl) (void)hipHostFree(b->h_bfl);
+  if (b->d_acc) (void)hipFree(b->d_acc);
+  if (b->h_acc) (void)hipHostFree(b->h_acc);
+  b->d_bfl = b->h_bfl = nullptr;
+  b->d_acc = b->h_acc = nullptr;
   b->d_ctl = b->h_ctl = nullptr;
   b->d_rhs = b->h_rhs = nullptr;
   b->d_remap = b->h_remap = nullptr;
@@ -1622,8 +1634,19 @@ int rebuild(fv_batch *b) {
     BCHK(hipHostMalloc((void **)&b->h_rhs, sizeof(RhsEntry) * 2 * m * cap));
     BCHK(hipHostMalloc((void **)&b->h_remap, sizeof(RemapEntry) * 2 * cap));
     BCHK(hipHostMalloc((void **)&b->h_cent, sizeof(ControlEntry) * NCAND * cap));
+    BCHK(hipMalloc((void **)&b->d_bfl, sizeof(BfluxArgs) * 2 * cap));
+    BCHK(hipMalloc((void **)&b->d_acc, sizeof(AccreteArgs) * 2 * cap));
+    BCHK(hipHostMalloc((void **)&b->h_bfl, sizeof(BfluxArgs) * 2 * cap));
+    BCHK(hipHostMalloc((void **)&b->h_acc, sizeof(AccreteArgs) * 2 * cap));
     BCHK(hipMemsetAsync(b->d_ctl, 0, sizeof(Ctl) * cap, b->st));
     b->cap = cap;
+  }
+  b->acc_n = b->acc_bpm = 0;
+  for (int d = 0; d < n; d++) {
+    const fv_ctx *x = b->sims[d];
+    if (!x->grav[0]) continue;
+    b->acc_n++;
+    b->acc_bpm = std::max(b->acc_bpm, (x->V * 2 * (x->c.inum + x->c.jnum) + 255) / 256);
   }
   std::vector<unsigned> all;
   for (int c = 0; c < NCAND; c++) {
@@ -1664,6 +1687,10 @@ void enqueue_batch_step(fv_batch *b, int p, int c) {
                      cand_tx(c), b->m, q == m - 2 ? 1 : 0, b->st);
   if (b->remap_n)
     launch_remap_multi(b->d_remap + (size_t)p * cap, b->d_tiles + b->remap_off, b->remap_n, b->V, b->dtype, b->st);
+  if (b->acc_n) {  // the step's boundary fluxes, then UpdateGravity
+    launch_bflux_multi(b->d_bfl + (size_t)p * cap, b->acc_n, b->acc_bpm, b->st);
+    launch_accrete_multi(b->d_acc + (size_t)p * cap, b->acc_n, b->st);
+  }
   launch_rhs_multi(b->d_rhs + (size_t)(p * m + m - 1) * cap, tl, b->cand_n[c], b->V, b->dtype, b->limiter,
                    cand_tx(c), b->m, 2, b->st);
   launch_control_multi(b->d_cent + (size_t)c * cap, n, b->st);
@@ -1696,6 +1723,7 @@ int batch_round(fv_batch *b, const std::vector<int> &want, int K, int c, bool ti
                 std::vector<int> &done, std::vector<int> &halted) {
   const int n = (int)b->sims.size(), m = b->m, cap = b->cap;
   b->roles.resize(n);
+  int ai = 0;  // entry of the next accreting member
   for (int d = 0; d < n; d++) {
     fv_ctx *x = b->sims[d];
     Ctl *h = &b->h_ctl[d];
@@ -1715,8 +1743,13 @@ int batch_round(fv_batch *b, const std::vector<int> &want, int K, int c, bool ti
     r.b0 = x->bcur;
     for (int p = 0; p < 2; p++) {
       FastStep f;
+      const int gin = (p && x->grav[0]) ? 1 - x->gcur : x->gcur;  // the point mass alternates as the state
       fast_step_args(x, dc, p ? r.x1 : r.x0, p ? r.x0 : r.x1, r.tb, r.rb, p ? 1 - r.k0 : r.k0,
-                     p ? r.k0 : 1 - r.k0, p ? 1 - r.b0 : r.b0, x->gcur, x->wcur, x->vcur, f);  // no accretion, no type 1
+                     p ? r.k0 : 1 - r.k0, p ? 1 - r.b0 : r.b0, gin, x->wcur, x->vcur, f);  // no FARGO type 1
+      if (f.accrete) {
+        b->h_bfl[(size_t)p * cap + ai] = f.bfl;
+        b->h_acc[(size_t)p * cap + ai] = f.acc;
+      }
       for (int q = 0; q < m - 1; q++) b->h_rhs[(size_t)(p * m + q) * cap + d] = RhsEntry{x->g, f.st[q]};
       b->h_rhs[(size_t)(p * m + m - 1) * cap + d] = RhsEntry{x->g, f.end};
       RemapEntry re;
@@ -1730,12 +1763,17 @@ int batch_round(fv_batch *b, const std::vector<int> &want, int K, int c, bool ti
       const int nb = ((x->c.inum + tx - 1) / tx) * ((x->c.jnum + TY_TILE - 1) / TY_TILE);
       b->h_cent[(size_t)cc * cap + d] = ControlEntry{dc, x->part_c, x->part_r, nb};
     }
+    if (x->grav[0]) ai++;
   }
   BCHK(hipMemcpy2DAsync(b->d_ctl, sizeof(Ctl), b->h_ctl, sizeof(Ctl), offsetof(Ctl, err_hist), n,
                         hipMemcpyHostToDevice, b->st));
   BCHK(hipMemcpyAsync(b->d_rhs, b->h_rhs, sizeof(RhsEntry) * 2 * m * cap, hipMemcpyHostToDevice, b->st));
   BCHK(hipMemcpyAsync(b->d_remap, b->h_remap, sizeof(RemapEntry) * 2 * cap, hipMemcpyHostToDevice, b->st));
   BCHK(hipMemcpyAsync(b->d_cent, b->h_cent, sizeof(ControlEntry) * NCAND * cap, hipMemcpyHostToDevice, b->st));
+  if (b->acc_n) {
+    BCHK(hipMemcpyAsync(b->d_bfl, b->h_bfl, sizeof(BfluxArgs) * 2 * cap, hipMemcpyHostToDevice, b->st));
+    BCHK(hipMemcpyAsync(b->d_acc, b->h_acc, sizeof(AccreteArgs) * 2 * cap, hipMemcpyHostToDevice, b->st));
+  }
   if (timed) {
     BCHK(hipEventRecord(b->t0, b->st));
     for (int s = 0; s < K; s++) enqueue_batch_step(b, s % 2, c);
@@ -1801,7 +1839,6 @@ extern "C" int fv_batch_create(fv_batch **out, const fv_batch_opts *opts) {
 extern "C" int fv_batch_add(fv_batch *b, fv_ctx *x) {
   if (!b || !x) return -1;
   if (x->batch) return bfail(b, "context already belongs to a batch");
-  if (x->grav[0]) return bfail(b, "point-mass accretion couples a boundary reduction into every step: not batched");
   if ((int)b->sims.size() >= MAX_BATCH) return bfail(b, "batch is full (" + std::to_string(MAX_BATCH) + " members)");
   const int nbx = (x->c.inum + 31) / 32, nby = (x->c.jnum + TY_TILE - 1) / TY_TILE, rby = remap_rows_blocks(x->c.jnum);
   if (nbx >= (1 << TILE_BX_BITS) || nby >= (1 << TILE_BY_BITS) || rby >= (1 << TILE_BY_BITS))

@@ -223,6 +223,11 @@ void launch_rhs_multi(const RhsEntry *tab, const unsigned *tiles, int ntiles, in
                       int tx, int stages, int mode, hipStream_t s);
 void launch_remap_multi(const RemapEntry *tab, const unsigned *tiles, int ntiles, int V, int dtype, hipStream_t s);
 void launch_control_multi(const ControlEntry *tab, int n, hipStream_t s);
+// batches with point-mass accretion: the step's boundary-flux accumulation
+// (bpm workgroups per member) and UpdateGravity (one workgroup per member) of
+// the n accreting members in tab
+void launch_bflux_multi(const BfluxArgs *tab, int n, int bpm, hipStream_t s);
+void launch_accrete_multi(const AccreteArgs *tab, int n, hipStream_t s);
 
 // Reduction scratch of the allocation-free step (fv_step_noalloc): a static
 // device array, one copy per device, laid out as the reduction result (2

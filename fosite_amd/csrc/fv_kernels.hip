@@ -65,6 +65,23 @@ __global__ __launch_bounds__(256) void k_accrete(AccreteArgs p) {
   const double sum = ordered_row_sum(p.bf + p.lo, p.hi - p.lo + 1, tid);
   if (tid < 64) accrete_update(p, sum, tid, 64);
 }
+// batches: the boundary-flux accumulation of the accreting members, bpm
+// workgroups per member striding over its boundary elements
+__global__ __launch_bounds__(256) void k_bflux_multi(const BfluxArgs *__restrict__ tab, int bpm) {
+  const BfluxArgs &bf = tab[blockIdx.x / bpm];
+  if (bf.ctl && bf.ctl->halt) return;
+  const int r = blockIdx.x % bpm;
+  bflux_accumulate(bf, bf.ctl ? bf.ctl->dt : bf.dt, r * blockDim.x + threadIdx.x, bpm * blockDim.x);
+}
+// batches: UpdateGravity of the accreting members, one workgroup each (the
+// same ordered chain as k_accrete)
+__global__ __launch_bounds__(256) void k_accrete_multi(const AccreteArgs *__restrict__ tab) {
+  const AccreteArgs &p = tab[blockIdx.x];
+  if (p.ctl && p.ctl->halt) return;
+  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+  const double sum = ordered_row_sum(p.bf + p.lo, p.hi - p.lo + 1, tid);
+  if (tid < 64) accrete_update(p, sum, tid, 64);
+}
 // one block per mesh of a batch
 __global__ __launch_bounds__(256) void k_control_multi(const ControlEntry *__restrict__ tab) {
   const ControlEntry &E = tab[blockIdx.x];
@@ -267,6 +284,12 @@ void launch_remap_multi(const RemapEntry *tab, const unsigned *tiles, int ntiles
 }
 void launch_control_multi(const ControlEntry *tab, int n, hipStream_t s) {
   hipLaunchKernelGGL(k_control_multi, dim3(n), dim3(256), 0, s, tab);
+}
+void launch_bflux_multi(const BfluxArgs *tab, int n, int bpm, hipStream_t s) {
+  hipLaunchKernelGGL(k_bflux_multi, dim3(n * bpm), dim3(256), 0, s, tab, bpm);
+}
+void launch_accrete_multi(const AccreteArgs *tab, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_accrete_multi, dim3(n), dim3(64, 4), 0, s, tab);
 }
 void launch_wsum(const Geom &g, const double *v, double *ring, float *ringf, hipStream_t s, const Ctl *ctl) {
   hipLaunchKernelGGL(k_wsum, dim3((g.NI + WS_RINGS - 1) / WS_RINGS), dim3(256), 0, s, g, v, ring, ringf, ctl);

@@ -151,7 +151,6 @@ extern "C" int fv_server_submit(fv_server *s, const fv_config *cfg, const double
   if (!s || !cfg || !pvar || !id) return -1;
   double bytes = 0;
   if (fv_device_bytes(cfg, &bytes)) return sfail(s, "unsupported configuration");
-  if (cfg->outbound) return sfail(s, "point-mass accretion runs alone (fv_run), not in the serving loop");
   if (bytes > s->opt.hbm_budget_bytes) return sfail(s, "run needs more device memory than the whole budget", -2);
   auto r = std::make_unique<Request>();
   r->cfg = *cfg;

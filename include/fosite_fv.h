@@ -225,8 +225,9 @@ int fv_kernel_stats(const fv_ctx *ctx, int kernel, int *launches, double *ms, do
  * the steps of a round are one HIP graph launch.  Each member keeps its own
  * mesh, time step control and host scalars; results are bit-identical to
  * running the member alone with fv_run.  Members must agree in physics,
- * dtype and time integrator (method, order); limiters, boundaries, FARGO and
- * sizes may differ.  A context belongs to at most one batch; fv_destroy
+ * dtype and time integrator (method, order); limiters, boundaries, FARGO,
+ * the rotating frame, point-mass accretion and sizes may differ (FARGO type 1
+ * members take their steps on the host path).  A context belongs to at most one batch; fv_destroy
  * removes it from its batch.
  *
  * Tilings of the RHS kernels: 0 = 64-wide tiles in row-major order, 1 =

@@ -134,24 +134,82 @@ def test_gpu_accretion_fp32_form_tracks_fp64():
 
 
 @pytest.mark.gpu
-def test_gpu_point_mass_without_accretion_and_batches_refuse_accretion():
+def test_gpu_point_mass_without_accretion():
     cfg = dict(D, bc=NG)
     sim, _ = pair(cfg)
     sim.run(5)
     assert np.array_equal(sim.point_mass(), [1.0, 0.0, 0.0, 0.0])
-    acc, _ = pair(dict(cfg, outbound=1))
-    b = fosite_amd.Batch()
-    with pytest.raises(RuntimeError, match="accretion"):
-        b.add(acc)
-    b.close()
-    acc.close()
     sim.close()
     nograv = fosite_amd.Fosite(S.riemann2d_config(n=16))
     with pytest.raises(RuntimeError, match="point mass"):
         nograv.point_mass()
     nograv.close()
-    srv = fosite_amd.Server(1e9)
-    acfg = dict(cfg, outbound=1)
-    with pytest.raises(RuntimeError, match="accretion"):
-        srv.submit(acfg, S.initial(acfg))
+
+
+BATCH = [dict(D, outbound=1, bc=NG, seed=41), dict(D, bc=NG, seed=42),
+         dict(S.kepler_disk_config(ni=56, nj=80), outbound=2, bc=NG, fargo=0, acclimit=0.001, seed=43),
+         dict(S.kepler_disk_config(ni=48, nj=96), outbound=1, seed=44)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype", ["fp64", "fp32"])
+@pytest.mark.parametrize("tiling", [1, 2])
+def test_gpu_batch_with_accreting_members_is_bit_identical(dtype, tiling):
+    """Members with point-mass accretion (west and east boundaries, an
+    accretion limit, absorbing boundaries) share the batch rounds with one
+    without: every member keeps the state, boundary fluxes and point mass of
+    its run alone, step for step over rounds of several sizes."""
+    cfgs = [dict(c, dtype=dtype) for c in BATCH]
+    print("accretion batch", dtype, "tiling", tiling, flush=True)
+    alone = []
+    for c in cfgs:
+        s, _ = pair(c)
+        s.run(14)
+        alone.append((s.state()[0], s.point_mass(), s.boundary_flux(), s.scalars()))
+        s.close()
+    sims = [pair(c)[0] for c in cfgs]
+    b = fosite_amd.Batch(sims, autotune=False, tiling=tiling)
+    for k in (1, 5, 8):
+        assert b.run(k) == [k] * len(sims)
+    for s, (cv, pm, bf, sc) in zip(sims, alone):
+        assert s.scalars() == sc
+        assert np.array_equal(s.state()[0], cv)
+        assert np.array_equal(s.point_mass(), pm)
+        for x, y in zip(s.boundary_flux(), bf):
+            assert np.array_equal(x, y)
+    assert b.info()["device_steps"] > 0
+    b.close()
+    for s in sims:
+        s.close()
+
+
+@pytest.mark.gpu
+def test_gpu_served_accretion_runs_equal_runs_alone():
+    """Disks set up with the reference's default accretion through the
+    western boundary (gravity_pointmass.f90:291-299) in the serving loop,
+    beside runs without: each result bit-identical to the run alone."""
+    reqs = [cfg for _, cfg in S.serving_requests(n=4, seed=11, nmin=24, nmax=64, orbits=0.3)]
+    cfgs = [dict(c, outbound=1 if k % 2 == 0 else 0, bc=NG) for k, c in enumerate(reqs)]
+    srv = fosite_amd.Server(1e10, round_steps=4, autotune=False)
+    ids = [srv.submit(c, S.initial(c)) for c in cfgs]
+    rounds = 0
+    while True:
+        srv.step()
+        st = srv.stats()
+        if st["completed"] + st["failed"] == len(cfgs):
+            break
+        rounds += 1
+        assert rounds < 5000
+    assert srv.stats()["failed"] == 0
+    for rid, c in zip(ids, cfgs):
+        cv, sc = srv.result(rid)
+        init = S.initial(c)
+        sim = fosite_amd.Fosite(c)
+        sim.set_initial(init["pvar"], init.get("bccsound"), init.get("fcsound"), init.get("w"))
+        sim.first_step()
+        while sim.run(256) == 256:
+            pass
+        assert sc == sim.scalars(), rid
+        assert np.array_equal(cv[I], sim.state()[0][I]), rid
+        sim.close()
     srv.close()
