@@ -82,7 +82,8 @@ EXPORTS = ["fv_mesh_array", "fv_create", "fv_destroy", "fv_dims", "fv_set_sound_
 
 
 class FvServerOpts(ctypes.Structure):
-    _fields_ = [("hbm_budget_bytes", ctypes.c_double), ("round_steps", ctypes.c_int), ("autotune", ctypes.c_int)]
+    _fields_ = [("hbm_budget_bytes", ctypes.c_double), ("round_steps", ctypes.c_int), ("autotune", ctypes.c_int),
+                ("backfill", ctypes.c_int)]
 
 
 class FvRequestInfo(ctypes.Structure):
@@ -437,13 +438,15 @@ def device_bytes(cfg):
 class Server:
     """Continuous batching of runs to their stop times (fv_server_*): runs are
     admitted in arrival order while they fit the device memory budget, stay
-    resident, and advance in the FP32 and FP64 batches, round by round."""
+    resident, and advance in the FP32 and FP64 batches, round by round.  With
+    backfill = k > 0, queued runs that fit may pass a head that does not,
+    until that head has been passed over in k rounds."""
 
     STATES = {0: "queued", 1: "running", 2: "done", 3: "failed", 4: "taken"}
 
-    def __init__(self, hbm_budget_bytes, round_steps=64, autotune=True):
+    def __init__(self, hbm_budget_bytes, round_steps=64, autotune=True, backfill=0):
         self.L = load_library()
-        o = FvServerOpts(float(hbm_budget_bytes), int(round_steps), int(bool(autotune)))
+        o = FvServerOpts(float(hbm_budget_bytes), int(round_steps), int(bool(autotune)), int(backfill))
         h = ctypes.c_void_p()
         rc = self.L.fv_server_create(ctypes.byref(h), ctypes.byref(o))
         if rc:

@@ -3,7 +3,8 @@
 // No reference counterpart (the reference runs one simulation per process).
 // Requests (a configuration, its initial data and a stop time) queue on the
 // host.  They are admitted in arrival order while their device footprint
-// (fv_device_bytes) fits the memory budget; an admitted run lives on the
+// (fv_device_bytes) fits the memory budget (or past a head that does not
+// fit, with backfilling); an admitted run lives on the
 // device until it reaches its stop time and joins the batch of its element
 // type (FP32 or FP64).  One scheduling round:
 //
@@ -12,7 +13,8 @@
 //      two host threads on their own streams, so their kernels overlap;
 //   2. meanwhile the calling thread admits queued requests that fit: it
 //      creates their contexts, uploads the initial data and takes the first
-//      step on each context's own stream;
+//      step on each context's own stream.  With backfilling, runs that fit
+//      may pass a queue head that does not, for a bounded number of rounds;
 //   3. after both batches returned, finished runs are copied back and their
 //      memory released, and the runs admitted in 2 join their batches.
 //
@@ -44,6 +46,7 @@ struct Request {
   double bytes = 0;
   double t_submit = 0, t_admit = -1, t_done = -1;
   int state = Q_QUEUED;
+  int passed = 0;  // rounds this request, at the head of the queue, was passed over
   fv_ctx *ctx = nullptr;
   long steps = 0;
   std::vector<double> cvar;  // result
@@ -128,8 +131,8 @@ void retire(fv_server *s, Request &r) {
 extern "C" int fv_server_create(fv_server **out, const fv_server_opts *o) {
   if (!out || !o) return -1;
   *out = nullptr;
-  if (!(o->hbm_budget_bytes > 0) || o->round_steps < 1 || o->round_steps > 512) {
-    std::fprintf(stderr, "fv_server_create: need hbm_budget_bytes > 0 and 1 <= round_steps <= 512\n");
+  if (!(o->hbm_budget_bytes > 0) || o->round_steps < 1 || o->round_steps > 512 || o->backfill < 0) {
+    std::fprintf(stderr, "fv_server_create: need hbm_budget_bytes > 0, 1 <= round_steps <= 512, backfill >= 0\n");
     return -1;
   }
   fv_server *s = new (std::nothrow) fv_server();
@@ -187,13 +190,20 @@ extern "C" int fv_server_step(fv_server *s, int *running, int *queued) {
     else
       th[d] = std::thread([s, d, K, &rc_b, &taken] { rc_b[d] = fv_batch_run(s->batch[d], K, taken[d].data()); });
   }
-  // 2. admit queued requests that fit, in arrival order, while the batches run
+  // 2. admit queued requests that fit while the batches run: in arrival
+  // order, and with backfilling past a head that does not fit until that
+  // head has been passed over in opt.backfill rounds
   std::vector<int> ready;
-  while (!s->queue.empty()) {
-    Request &r = *s->req[s->queue.front()];
-    if (s->used + r.bytes > s->opt.hbm_budget_bytes) break;
-    const int id = s->queue.front();
-    s->queue.pop_front();
+  size_t k = 0;
+  while (k < s->queue.size()) {
+    const int id = s->queue[k];
+    Request &r = *s->req[id];
+    if (s->used + r.bytes > s->opt.hbm_budget_bytes) {
+      if (k == 0 && ++r.passed > s->opt.backfill) break;  // the head waits; no one passes it
+      k++;
+      continue;
+    }
+    s->queue.erase(s->queue.begin() + (long)k);
     prepare(s, r);
     if (r.state == Q_FAILED) continue;
     s->used += r.bytes;

@@ -280,6 +280,9 @@ typedef struct fv_server_opts {
   double hbm_budget_bytes; /* device memory all admitted runs together may hold */
   int round_steps;         /* steps per batch round, 1..512 */
   int autotune;            /* batches choose their tiling by timing */
+  int backfill;            /* 0: admission strictly in arrival order; k > 0: queued runs that fit
+                              may pass a head that does not, until that head has been passed
+                              over in k rounds (then only the head may enter) */
 } fv_server_opts;
 typedef struct fv_request_info {
   int state;               /* 0 queued, 1 running, 2 done, 3 failed, 4 result taken */

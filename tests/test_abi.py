@@ -84,7 +84,7 @@ def test_batch_create_rejects_bad_options():
 def test_server_create_rejects_bad_options():
     L = fosite_amd.load_library()
     h = ctypes.c_void_p()
-    for o in ((0.0, 16, 0), (1e9, 0, 0), (1e9, 513, 1)):
+    for o in ((0.0, 16, 0, 0), (1e9, 0, 0, 0), (1e9, 513, 1, 0), (1e9, 16, 0, -1)):
         assert L.fv_server_create(ctypes.byref(h), ctypes.byref(fosite_amd.FvServerOpts(*o))) == -1, o
         assert not h.value
     assert L.fv_server_step(None, None, None) == -1

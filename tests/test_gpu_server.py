@@ -64,3 +64,36 @@ def test_gpu_server_results_equal_runs_alone():
         I = (slice(None), slice(2, -2), slice(2, -2))
         assert np.array_equal(cv[I], cv0[I]), rid
     srv.close()
+
+
+@pytest.mark.gpu
+def test_gpu_server_backfill_admits_small_runs_past_a_blocked_head():
+    """With backfilling, a queued run that fits the budget is admitted while
+    an earlier one that does not fit waits (for at most `backfill` rounds of
+    being passed over); every result stays bit-identical to the run alone."""
+    small = [dict(c, stoptime=c["stoptime"] * 0.2) for c in requests(4, 9)]
+    big = dict(S.kepler_disk_config(ni=128, nj=256), stoptime=0.05)
+    sizes = [fosite_amd.device_bytes(c) for c in small]
+    budget = fosite_amd.device_bytes(big) + 0.5 * min(sizes)  # big fits only alone
+    assert fosite_amd.device_bytes(big) >= 2 * max(sizes)  # two small runs fit together
+    srv = fosite_amd.Server(budget, round_steps=4, autotune=False, backfill=1000)
+    first = srv.submit(small[0], S.initial(small[0]))
+    srv.step()  # small[0] admitted and running
+    ids = [srv.submit(big, S.initial(big))] + [srv.submit(c, S.initial(c)) for c in small[1:]]
+    rounds = 0
+    while srv.stats()["completed"] + srv.stats()["failed"] < 1 + len(ids):
+        srv.step()
+        rounds += 1
+        assert rounds < 5000
+    assert srv.stats()["failed"] == 0
+    t_big = srv.poll(ids[0])["t_admit"]
+    passed = [srv.poll(rid)["t_admit"] < t_big for rid in ids[1:]]
+    print("backfill: admitted before the big run", passed, flush=True)
+    assert any(passed)
+    for rid, cfg in zip([first] + ids, [small[0], big] + small[1:]):
+        cv, sc = srv.result(rid)
+        cv0, sc0 = alone(cfg)
+        assert sc == sc0, (rid, sc, sc0)
+        I = (slice(None), slice(2, -2), slice(2, -2))
+        assert np.array_equal(cv[I], cv0[I]), rid
+    srv.close()
