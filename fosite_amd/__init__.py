@@ -100,7 +100,8 @@ class FvServerStats(ctypes.Structure):
 
 class FvBatchOpts(ctypes.Structure):
     """Mirror of ``fv_batch_opts``."""
-    _fields_ = [("autotune", ctypes.c_int), ("tiling", ctypes.c_int), ("max_graph_steps", ctypes.c_int)]
+    _fields_ = [("autotune", ctypes.c_int), ("tiling", ctypes.c_int), ("max_graph_steps", ctypes.c_int),
+                ("reuse_tuning", ctypes.c_int)]
 
 
 class FvBatchInfo(ctypes.Structure):
@@ -376,9 +377,9 @@ class Batch:
     the same with 32-wide tiles; with ``autotune`` the batch times all four
     on its first device steps and keeps the fastest."""
 
-    def __init__(self, sims=(), autotune=True, tiling=0, max_graph_steps=0):
+    def __init__(self, sims=(), autotune=True, tiling=0, max_graph_steps=0, reuse_tuning=False):
         self.L = load_library()
-        o = FvBatchOpts(int(bool(autotune)), int(tiling), int(max_graph_steps))
+        o = FvBatchOpts(int(bool(autotune)), int(tiling), int(max_graph_steps), int(bool(reuse_tuning)))
         h = ctypes.c_void_p()
         rc = self.L.fv_batch_create(ctypes.byref(h), ctypes.byref(o))
         if rc:

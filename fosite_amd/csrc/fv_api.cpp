@@ -14,6 +14,8 @@
 #include <cstddef>
 #include <cstdlib>
 #include <cmath>
+#include <map>
+#include <mutex>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -1715,6 +1717,19 @@ int batch_graph(fv_batch *b, int K, int c, hipGraphExec_t *out) {
   return 0;
 }
 
+// the autotuner's choices by batch class: element type, fields, stages, and
+// the log2 of the members' total cells and of their number; a later batch of
+// a class already tuned in this process takes the choice without trials
+std::mutex tune_mu;
+std::map<long, int> tune_cache;
+long tune_key(const fv_batch *b) {
+  double cells = 0;
+  for (const fv_ctx *x : b->sims) cells += (double)x->c.inum * x->c.jnum;
+  const long lc = (long)std::floor(std::log2(std::max(cells, 1.0)));
+  const long ln = (long)std::floor(std::log2((double)std::max<size_t>(b->sims.size(), 1)));
+  return ((((long)b->dtype * 8 + b->V) * 16 + b->m) * 64 + lc) * 16 + ln;
+}
+
 // one round: want[d] device steps of member d (0 = not in this round), K =
 // steps launched (>= every want), tiling c; timed rounds are issued eagerly
 // between events and return their device time in *ms.  done[d] = steps the
@@ -1886,6 +1901,14 @@ extern "C" int fv_batch_run(fv_batch *b, int nsteps, int *taken) {
     return 0;
   }
   if (b->dirty && (rc = rebuild(b))) return rc;
+  if (b->opt.autotune && b->opt.reuse_tuning && !b->tuned) {
+    std::lock_guard<std::mutex> lk(tune_mu);
+    auto it = tune_cache.find(tune_key(b));
+    if (it != tune_cache.end()) {
+      b->cand = it->second;
+      b->tuned = 2;
+    }
+  }
   BCHK(hipEventRecord(b->e0, b->st));
   auto host_step = [&](int d) -> int {  // one Step on the host path
     int dn = 0;
@@ -1933,6 +1956,10 @@ extern "C" int fv_batch_run(fv_batch *b, int nsteps, int *taken) {
           if (b->tune_ms[q] < b->tune_ms[best]) best = q;
         b->cand = best;
         b->tuned = 1;
+        if (b->opt.reuse_tuning) {
+          std::lock_guard<std::mutex> lk(tune_mu);
+          tune_cache[tune_key(b)] = best;
+        }
       }
     }
     for (int d = 0; d < n; d++) {

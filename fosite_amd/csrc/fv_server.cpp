@@ -139,7 +139,7 @@ extern "C" int fv_server_create(fv_server **out, const fv_server_opts *o) {
   if (!s) return -1;
   s->opt = *o;
   s->t0 = std::chrono::steady_clock::now();
-  fv_batch_opts bo{o->autotune, 0, o->round_steps};
+  fv_batch_opts bo{o->autotune, 0, o->round_steps, 1};  // both batches of a class tune once
   for (int d = 0; d < 2; d++)
     if (fv_batch_create(&s->batch[d], &bo)) {
       fv_server_destroy(s);

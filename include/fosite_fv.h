@@ -242,11 +242,15 @@ typedef struct fv_batch_opts {
   int autotune;        /* 1: choose the tiling by timing (see above) */
   int tiling;          /* tiling without autotune (0..3) */
   int max_graph_steps; /* device steps per graph launch, 1..512 (0 = 512) */
+  int reuse_tuning;    /* with autotune: take the choice of an earlier batch of the same class
+                          in this process (no trials), and record this batch's for later ones */
 } fv_batch_opts;
 typedef struct fv_batch_info {
   int members;
   int tiling;          /* tiling in use */
-  int tuned;           /* 1 once the autotuner has chosen */
+  int tuned;           /* 1 once the autotuner has chosen by its trials, 2 when it took the choice
+                          of an earlier batch of the same class (reuse_tuning: element type,
+                          fields, stages, log2 of the members' total cells and of their number) */
   double tune_ms[4];   /* best time per device step of each tiling in the trials (0 = not timed) */
   long device_steps;   /* member steps taken on the device-controlled path */
   long host_steps;     /* member steps taken on the host path (first steps, halts) */

@@ -100,6 +100,24 @@ def test_gpu_batch_autotune_and_short_graphs():
 
 
 @pytest.mark.gpu
+def test_gpu_batch_reuses_the_tuning_of_its_class():
+    """With reuse_tuning, the first batch of a class (element type, fields,
+    stages, total cells and member count in powers of two) tunes by trials
+    and a later batch of the same class takes its tiling without trials; the
+    bits stay those of the runs alone."""
+    cfgs = [dict(c, cfl=0.4) for c in members("fp64")[1:4]]
+    chunks = [12, 9]
+    res1, info1 = run_batch(cfgs, chunks, autotune=True, reuse_tuning=True)
+    res2, info2 = run_batch([dict(c, seed=c["seed"] + 100) for c in cfgs], chunks, autotune=True,
+                            reuse_tuning=True)
+    print("reuse tuning", info1, info2, flush=True)
+    assert info1["tuned"] in (1, 2)  # an earlier batch of the class may have tuned already
+    assert info2["tuned"] == 2 and info2["tiling"] == info1["tiling"]
+    assert all(t == 0 for t in info2["tune_ms"])
+    check_same(cfgs, res1, chunks)
+
+
+@pytest.mark.gpu
 def test_gpu_batch_member_reaching_its_stop_time():
     cfgs = members("fp64")[:3]
     cfgs[1] = dict(cfgs[1], stoptime=0.3)  # ends inside the run
