@@ -74,7 +74,7 @@ EXPORTS = ["fv_mesh_array", "fv_create", "fv_destroy", "fv_dims", "fv_set_sound_
            "fv_get_background_velocity", "fv_get_point_mass",
            "fv_last_run_ms", "fv_last_error", "fv_profile",
            "fv_kernel_stats", "fv_batch_create", "fv_batch_add", "fv_batch_remove", "fv_batch_run",
-           "fv_batch_get_info", "fv_batch_destroy", "fv_batch_last_error", "fv_device_bytes",
+           "fv_batch_get_info", "fv_batch_destroy", "fv_batch_last_error", "fv_device_bytes", "fv_build_id",
            "fv_server_create", "fv_server_submit", "fv_server_step", "fv_server_poll", "fv_server_result",
            "fv_server_get_stats", "fv_server_destroy", "fv_server_last_error",
            "fv_first_rhs", "fv_integration_step", "fv_calc_timestep", "fv_add_background_velocity",
@@ -164,6 +164,8 @@ def load_library():
     L.fv_batch_last_error.argtypes = [V]
     L.fv_batch_last_error.restype = ctypes.c_char_p
     L.fv_device_bytes.argtypes = [ctypes.POINTER(FvConfig), P]
+    L.fv_build_id.argtypes = []
+    L.fv_build_id.restype = ctypes.c_char_p
     L.fv_server_create.argtypes = [ctypes.POINTER(V), ctypes.POINTER(FvServerOpts)]
     L.fv_server_submit.argtypes = [V, ctypes.POINTER(FvConfig), P, P, P, P, ctypes.POINTER(ctypes.c_int)]
     L.fv_server_step.argtypes = [V, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
@@ -425,6 +427,12 @@ class Batch:
 
     def __del__(self):
         self.close()
+
+
+def build_id():
+    """SHA-256 of the sources and flags the loaded library was built from
+    (fv_build_id); fosite_amd.build.source_id() computes it for the tree."""
+    return load_library().fv_build_id().decode()
 
 
 def device_bytes(cfg):

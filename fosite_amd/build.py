@@ -1,8 +1,12 @@
 """In-tree build of libfosite_fv.so for gfx950 (hipcc, no JIT cache).
 
 Flags: -ffp-contract=off keeps every a*b+c unfused so the FP64 path rounds
-exactly like the reference; packed FP32 instructions are disabled.
+exactly like the reference; packed FP32 instructions are disabled.  The
+library carries the SHA-256 of the sources and compile flags it was built
+from (fv_build_id), so a prebuilt library can be checked against the tree.
 """
+import glob
+import hashlib
 import os
 import subprocess
 import sys
@@ -23,6 +27,22 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-ffp-contract=off", "-std=c++17", "-fP
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
+def source_id(defines=()):
+    """SHA-256 over the library's sources (csrc/*, include/*.h, by name and
+    content, in name order), the compile flags and the defines."""
+    h = hashlib.sha256()
+    files = sorted(glob.glob(os.path.join(HERE, "csrc", "*")) +
+                   glob.glob(os.path.join(os.path.dirname(HERE), "include", "*.h")))
+    for f in files:
+        if not os.path.isfile(f):
+            continue
+        h.update(os.path.basename(f).encode() + b"\0")
+        with open(f, "rb") as fh:
+            h.update(fh.read())
+    h.update(" ".join(FLAGS + list(defines)).encode())
+    return h.hexdigest()
+
+
 def _run(cmd, what):
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -35,7 +55,7 @@ def build(verbose=False, out=OUT, defines=(), jobs=None):
     library; the result replaces out atomically."""
     odir = os.path.join(OBJ, os.path.basename(out).replace(".", "_"))
     os.makedirs(odir, exist_ok=True)
-    defs = ["-D" + d for d in defines]
+    defs = ["-D" + d for d in defines] + ['-DFV_BUILD_ID="%s"' % source_id(defines)]
     objs = [os.path.join(odir, os.path.basename(f) + ".o") for f in SRC]
     jobs = jobs or min(len(SRC), max(1, os.cpu_count() or 1))
     with ThreadPoolExecutor(jobs) as ex:

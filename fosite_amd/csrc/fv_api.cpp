@@ -1088,6 +1088,11 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
   return 0;
 }
 
+#ifndef FV_BUILD_ID
+#define FV_BUILD_ID "unknown"
+#endif
+extern "C" const char *fv_build_id(void) { return FV_BUILD_ID; }
+
 extern "C" int fv_destroy(fv_ctx *ctx) {
   if (!ctx) return 0;
   if (ctx->batch) (void)fv_batch_remove(ctx->batch, ctx);

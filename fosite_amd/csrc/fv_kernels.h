@@ -1015,6 +1015,10 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
         const double cyd = ring(g, R_CYXY, i);
         double s1 = myd * (cyd * vyd - ring(g, R_CXYX, i) * (double)vx) + (cyd + ring(g, R_CZXZ, i)) * (double)P;
         if (g.gravity) s1 = s1 + rho * a.acc[i];
+        if (g.rotframe) {  // the rotating frame's radial acceleration joins the balance
+          const double ax = g.rotframe == 2 ? g.twoomega * vyd : ring(g, R_CACX, i) + g.twoomega * vyd;
+          s1 = s1 + rho * ax;
+        }
         geo[1] = (T)s1;
       } else {
         geo[1] = my * (cyxy * vy - cxyx * vx) + mz * (czxz * vz - zero * vx) + (cyxy + czxz) * P;
@@ -1061,7 +1065,8 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
         ax = rg<T>(g, R_CACX, i) + tw * vy;
         ay = cl<T>(g, C_CACY, j) - tw * vx;
       }
-      ext[1] = ext[1] + ur[0] * ax;
+      // FP32 curvilinear: the radial part is in the FP64 radial balance above
+      if (sizeof(T) == 8 || !g.curvilinear) ext[1] = ext[1] + ur[0] * ax;
       ext[2] = ext[2] + ur[0] * ay;
       if (V == 4) ext[3] = ext[3] + (ur[1] * ax + ur[2] * ay);
     }

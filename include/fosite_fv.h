@@ -99,6 +99,10 @@ int fv_dims(const fv_ctx *ctx, int *ni, int *nj, int *vnum);
 /* Device memory a context with this configuration allocates, in bytes
  * (no reference counterpart; used for admission by memory budget). */
 int fv_device_bytes(const fv_config *cfg, double *bytes);
+/* SHA-256 of the sources and compile flags the library was built from (no
+ * reference counterpart; lets a caller check a prebuilt library against its
+ * source tree). */
+const char *fv_build_id(void);
 
 /* physics_eulerisotherm%SetSoundSpeeds_center / _faces
  * (physics/physics_eulerisotherm.f90:288-309); isothermal physics only.

@@ -32,6 +32,14 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(L, name), name
 
 
+def test_library_was_built_from_this_tree():
+    """The library in the tree carries the SHA-256 of the sources and flags
+    it was compiled from; it must be this tree's (a stale or foreign prebuilt
+    library fails here)."""
+    from fosite_amd import build
+    assert fosite_amd.build_id() == build.source_id()
+
+
 @pytest.mark.parametrize("name", G.meshes())
 def test_host_mesh_matches_reference_fixture(name):
     cfg, arrays = G.mesh(name)

@@ -35,6 +35,9 @@ def oracle_states(cfg, steps):
     # boundaries keep LOG/EXP off the path, so the bits must agree
     ("disk_accretion_run", dict(S.kepler_disk_config(ni=48, nj=64), outbound=1,
                                 bc=[S.NO_GRADIENTS, S.NO_GRADIENTS, S.PERIODIC, S.PERIODIC]), 30, 10, 0.0),
+    # the rotating frame after the point mass (sources_rotframe.f90)
+    ("disk_rotframe_run", dict(S.kepler_disk_config(ni=48, nj=64), omega=0.3, rotframe=1,
+                               bc=[S.NO_GRADIENTS, S.NO_GRADIENTS, S.PERIODIC, S.PERIODIC]), 30, 10, 0.0),
 ])
 def test_fortran_host_matches_oracle(tmp_path, name, cfg, nsteps, batch, tol):
     print("fortran host case", name, flush=True)

@@ -100,6 +100,8 @@ CASES = {
     "euler_disk_absorbing_dp5": (S.euler_disk_config(ni=48, nj=96), 40),
     "riemann2d_euler": (S.riemann2d_config(n=64), 40),
     "kepler_disk_primitive_reconstruction": (dict(S.kepler_disk_config(ni=64, nj=128), recvars=S.PRIMITIVE), 60),
+    "kepler_disk_rotating_frame": (dict(S.kepler_disk_config(ni=64, nj=128), omega=0.3, rotframe=1), 60),
+    "riemann2d_rotating_frame": (dict(S.riemann2d_config(n=64), omega=2.0, rotframe=1), 40),
 }
 
 
