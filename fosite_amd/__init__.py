@@ -282,6 +282,14 @@ class Fosite:
                       "fv_set_background_velocity")
         self._chk(self.L.fv_set_pvar(self._h, _ptr(p)), "fv_set_pvar")
 
+    def set_conservative(self, cvar):
+        """Timedisc%cvar as given, all cells (V, NJ, NI), no conversion
+        (fv_set_cvar); sound speeds and w as for set_initial, beforehand."""
+        c = np.ascontiguousarray(cvar, dtype=np.float64)
+        if c.shape != (self.V, self.NJ, self.NI):
+            raise ValueError("cvar must have shape %s, not %s" % ((self.V, self.NJ, self.NI), c.shape))
+        self._chk(self.L.fv_set_cvar(self._h, _ptr(c)), "fv_set_cvar")
+
     def first_step(self):
         b = ctypes.c_int()
         self._chk(self.L.fv_first_step(self._h, ctypes.byref(b)), "fv_first_step")
