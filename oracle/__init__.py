@@ -44,7 +44,9 @@ class _Cfg(ctypes.Structure):
                 ("outbound", ctypes.c_int), ("acclimit", ctypes.c_double),
                 ("pmin", ctypes.c_double), ("rhomin", ctypes.c_double),
                 ("count", ctypes.c_int), ("starttime", ctypes.c_double),
-                ("omega", ctypes.c_double), ("rotframe", ctypes.c_int)]
+                ("omega", ctypes.c_double), ("rotframe", ctypes.c_int),
+                ("vismodel", ctypes.c_int), ("dynconst", ctypes.c_double), ("bulkconst", ctypes.c_double),
+                ("cvis", ctypes.c_double)]
 
 
 class _Scalars(ctypes.Structure):
@@ -131,6 +133,11 @@ class Oracle:
         c.starttime = cfg.get("starttime", 0.0)
         c.omega = cfg.get("omega", 0.0)
         c.rotframe = cfg.get("rotframe", 0)
+        c.vismodel = cfg.get("vismodel", 0)
+        c.dynconst = cfg.get("dynconst", 0.1)
+        bc = cfg.get("bulkconst")
+        c.bulkconst = -(2.0 / 3.0) * c.dynconst if bc is None else bc  # sources_viscosity.f90:150
+        c.cvis = cfg.get("cvis", 0.5)
         self._h = L.fvr_create(ctypes.byref(c))
         if not self._h:
             raise ValueError("oracle: unsupported configuration")

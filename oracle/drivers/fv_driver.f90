@@ -42,13 +42,18 @@ PROGRAM fv_driver
   REAL    :: k_pmin = -1.0, k_rhomin = -1.0, k_starttime = 0.0
   REAL    :: k_omega = 0.0
   INTEGER :: k_rotframe = 0
+  ! k_vismodel: 0 off, 1 MOLECULAR, 2 ALPHA (sources/sources_viscosity.f90:57-61);
+  ! k_bulkconst is passed only when k_setbulk = 1
+  INTEGER :: k_vismodel = 0, k_setbulk = 0
+  REAL    :: k_dynconst = 0.1, k_bulkconst = 0.0, k_cvis = 0.5
   INTEGER :: k_nsteps = 10, k_dumpevery = 1
   CHARACTER(LEN=512) :: k_infile = "", k_prefix = "out"
   NAMELIST /fvcfg/ k_physics, k_geometry, k_inum, k_jnum, k_gamma, k_xmin, k_xmax, k_ymin, k_ymax, &
        k_gparam, k_bcw, k_bce, k_bcs, k_bcn, k_limiter, k_recvars, k_theta, k_method, k_order,   &
        k_cfl, k_tolrel, k_tolabs, k_stoptime, k_dtlimit, k_beta, k_fargo, k_gravity, k_units,   &
        k_mass, k_outbound, k_acclimit, k_binary, k_nsteps, k_dumpevery, k_infile, k_prefix, &
-       k_count, k_pmin, k_rhomin, k_starttime, k_omega, k_rotframe
+       k_count, k_pmin, k_rhomin, k_starttime, k_omega, k_rotframe, k_vismodel, k_setbulk, &
+       k_dynconst, k_bulkconst, k_cvis
   !--------------------------------------------------------------------------!
   CLASS(fosite), ALLOCATABLE :: Sim
   TYPE(Dict_TYP), POINTER    :: config, mesh, phys, bnd, flx, tdisc, dfile, &
@@ -121,6 +126,19 @@ PROGRAM fv_driver
     pmass => Dict("gtype" / POINTMASS, "potential" / NEWTON, "mass" / k_mass, &
          "outbound" / k_outbound, "acclimit" / k_acclimit)
     src => Dict("grav/stype" / GRAVITY, "grav/pmass" / pmass)
+  END IF
+  ! the viscosity before the rotating frame in the Dict: the source list
+  ! holds them in reverse order (sources/sources_generic.f90:126-183)
+  IF (k_vismodel.GT.0) THEN
+    IF (ASSOCIATED(src)) THEN
+      CALL SetAttr(src, "vis/stype", VISCOSITY)
+    ELSE
+      src => Dict("vis/stype" / VISCOSITY)
+    END IF
+    CALL SetAttr(src, "vis/vismodel", k_vismodel)
+    CALL SetAttr(src, "vis/dynconst", k_dynconst)
+    IF (k_setbulk.EQ.1) CALL SetAttr(src, "vis/bulkconst", k_bulkconst)
+    CALL SetAttr(src, "vis/cvis", k_cvis)
   END IF
   ! k_rotframe = 1: the rotating-frame source, 2: the same with
   ! disable_centaccel (sources/sources_rotframe.f90:86-93)

@@ -13,7 +13,8 @@
  * variables, embedded RK (RKF 3/5, Cash-Karp 5, Dormand-Prince 5, SSPRK 3/5)
  * with error control and reject path, no-gradient, periodic and absorbing
  * boundaries, point-mass gravity with accretion, the rotating frame about
- * the origin (centrifugal and Coriolis terms), FARGO along y for both
+ * the origin (centrifugal and Coriolis terms), the viscosity source
+ * (MOLECULAR and ALPHA models, with its time-step limit), FARGO along y for both
  * physics and geometries with the per-ring mean (type 1) or a user-supplied
  * (type 2) background velocity, CheckData breaks,
  * output times (datafile count) and a start time.
@@ -62,6 +63,10 @@ struct fvr {
   double *dAx[2], *dAy[2], *dydzdV, *dzdxdV, *dxdydV, *dlx, *dly;
   double *cxyx, *cyxy, *czxz, *czyz, *rdx[4], *rad, *radf[4], *posx, *posy, *bcx, *bcy;
   double *caccel[2], *twoomega, *raccel[2]; /* rotating frame (sources_rotframe.f90:110-175, 226-234) */
+  /* viscosity (sources_viscosity.f90, physics_eulerisotherm.f90:982-1198):
+   * dynamic and bulk viscosity of the last update and its time, e_phi/r in
+   * curvilinear components, stresses, work arrays */
+  double *dynvis, *bulkvis, vis_time, *ephir[2], *btxx, *btxy, *btyy, *vtmp, *vtmp1, *vsterm;
   double *accel[2];
   double pm_mass, pm_accrate, pm_massloss, pm_mdot; /* gravity_pointmass.f90:79-85 */
   /* physics */
@@ -207,6 +212,38 @@ static void mesh_build(fvr *s) {
         s->caccel[1][c] = v3 * o1 - v1 * o3;
         s->twoomega[c] = 2 * o3;
       }
+  }
+  /* sources_viscosity.f90:169-178 (dynvis = dynconst, bulkvis = bulkconst,
+   * the update time -1) and 196-223 (ALPHA): e_phi/r = (-y/r^2, x/r^2, 0)
+   * of the Cartesian bary centres (geometry_logcylindrical.f90:344-356),
+   * in curvilinear components (geometry_cylindrical.f90:485-494; the
+   * Cartesian geometry keeps them) */
+  if (s->c.vismodel) {
+    s->vis_time = -1.0;
+    for (int c = 0; c < s->N; c++) {
+      s->dynvis[c] = s->c.dynconst;
+      s->bulkvis[c] = s->c.bulkconst;
+    }
+  }
+  if (s->c.vismodel == 2) {
+    for (int c = 0; c < s->N; c++) {
+      const double phi = s->bcy[c];
+      double x = s->bcx[c], y = s->bcy[c];
+      if (s->c.geometry == LOGCYLINDRICAL) {
+        const double r = s->c.gparam * exp(s->bcx[c]);
+        x = r * cos(phi);
+        y = r * sin(phi);
+      }
+      const double r2 = s->rad[c] * s->rad[c];
+      const double ex = -(y / r2), ey = x / r2;
+      if (s->c.geometry == LOGCYLINDRICAL) {
+        s->ephir[0][c] = ex * cos(phi) + ey * sin(phi);
+        s->ephir[1][c] = -(ex * sin(phi)) + ey * cos(phi);
+      } else {
+        s->ephir[0][c] = ex;
+        s->ephir[1][c] = ey;
+      }
+    }
   }
 }
 
@@ -529,7 +566,9 @@ static void update_gravity(fvr *s, double dt) {
  *  - the rotating frame, sources_rotframe.f90:212-247 (2-D acceleration:
  *    caccel + twoOmega*v_y, caccel - twoOmega*v_x; with disable_centaccel
  *    twoOmega*v_y, -twoOmega*v_x) with the real-frame velocities p;
- * dt < 0: no UpdateGravity (test aid) */
+ *  - the viscosity, sources_viscosity.f90:481-500 (configured before the
+ *    rotating frame; the list holds the configuration in reverse order);
+ * dt < 0: no UpdateGravity (test aid); t: ComputeRHS's time argument */
 static void source_term(const fvr *s, const double *c, const double *ax, const double *ay, int N, int V) {
   PAR_FOR
   for (int k = 0; k < N; k++) {
@@ -540,9 +579,86 @@ static void source_term(const fvr *s, const double *c, const double *ax, const d
   }
 }
 
-static void external_sources(fvr *s, const double *p, const double *c, double *st, double dt) {
+/* sources_viscosity.f90:356-460 (UpdateViscosity): the dynamic viscosity
+ * is recomputed only when the time differs from the last update's, or is 0;
+ * ALPHA: eta = alpha*rho*cs^2/|e_phi/r . v + Omega| on every cell, bulk
+ * viscosity -2/3 eta; MOLECULAR keeps its constants */
+static void update_viscosity(fvr *s, double time, const double *p) {
+  const int N = s->N;
+  if (!(time != s->vis_time || time == 0.0)) return;
+  if (s->c.vismodel == 2) {
+    for (int k = 0; k < N; k++) {
+      double om = s->ephir[0][k] * p[1 * N + k];
+      om = om + s->ephir[1][k] * p[2 * N + k];
+      om = om + s->c.omega;
+      const double cs = s->bccsound[k];
+      s->dynvis[k] = s->c.dynconst * p[k] * (cs * cs) / fabs(om);
+      s->bulkvis[k] = -(2. / 3.) * s->dynvis[k];
+    }
+  }
+  s->vis_time = time;
+}
+
+/* mesh_midpoint.f90:586-630 (Divergence3D) of a 2-D x-y vector field on
+ * face-averaged components (VectorDivergence2D_1, 156-232) */
+static double div2d(const fvr *s, int k, const double *vx, const double *vy) {
+  const int NI = s->NI;
+  const double xw = 0.5 * (vx[k - 1] + vx[k]), xe = 0.5 * (vx[k + 1] + vx[k]);
+  const double ys = 0.5 * (vy[k - NI] + vy[k]), yn = 0.5 * (vy[k + NI] + vy[k]);
+  return s->dydzdV[k] * (s->dAx[1][k] * xe - s->dAx[0][k] * xw) + s->dzdxdV[k] * (s->dAy[1][k] * yn - s->dAy[0][k] * ys) +
+         0.0 * (0.0 * 0.0 - 0.0 * 0.0) + 0.0 * 0.0 + 0.0 * 0.0 - 0.0 * 0.0 - 0.0 * 0.0;
+}
+
+/* physics_eulerisotherm.f90:1017-1072 (CalcStresses, 2-D x-y) and 982-1016
+ * (ViscositySources: TensorDivergence2D_1, mesh_midpoint.f90:279-335, with
+ * btyx = btxy), physics_euler.f90:1887-1918 (energy: divergence of T.v);
+ * result in s->vsterm */
+static void viscosity_sources(fvr *s, const double *p) {
+  const int N = s->N, NI = s->NI, NJ = s->NJ, V = s->V;
+  const double *vx = p + N, *vy = p + 2 * N;
+  for (int j = 1; j < NJ - 1; j++)
+    for (int i = 1; i < NI - 1; i++) s->vtmp[IDX(s, i, j)] = div2d(s, IDX(s, i, j), vx, vy);
+  for (int k = 0; k < N; k++) s->vtmp[k] = s->bulkvis[k] * s->vtmp[k];
+  for (int j = G - 1; j <= G + s->c.jnum; j++)
+    for (int i = G - 1; i <= G + s->c.inum; i++) {
+      const int k = IDX(s, i, j);
+      const double eta = s->dynvis[k];
+      s->btxx[k] = eta * ((vx[k + 1] - vx[k - 1]) / s->dlx[k] + 2.0 * s->cxyx[k] * vy[k]) + s->vtmp[k];
+      s->btyy[k] = eta * ((vy[k + NI] - vy[k - NI]) / s->dly[k] + 2.0 * s->cyxy[k] * vx[k]) + s->vtmp[k];
+      s->btxy[k] = eta * (0.5 * ((vy[k + 1] - vy[k - 1]) / s->dlx[k] + (vx[k + NI] - vx[k - NI]) / s->dly[k]) -
+                          s->cxyx[k] * vx[k] - s->cyxy[k] * vy[k]);
+    }
+  double *st = s->vsterm;
+  for (int k = 0; k < N; k++) st[k] = 0.0;
+  for (int j = 1; j < NJ - 1; j++)
+    for (int i = 1; i < NI - 1; i++) {
+      const int k = IDX(s, i, j);
+      const double *txx = s->btxx, *txy = s->btxy, *tyy = s->btyy;
+      st[N + k] = s->dydzdV[k] * (s->dAx[1][k] * (0.5 * (txx[k + 1] + txx[k])) -
+                                  s->dAx[0][k] * (0.5 * (txx[k - 1] + txx[k]))) +
+                  s->dzdxdV[k] * (s->dAy[1][k] * (0.5 * (txy[k + NI] + txy[k])) -
+                                  s->dAy[0][k] * (0.5 * (txy[k - NI] + txy[k]))) +
+                  0.0 * (0.0 * 0.0 - 0.0 * 0.0) + s->cxyx[k] * txy[k] + 0.0 * 0.0 - s->cyxy[k] * tyy[k] - 0.0 * 0.0;
+      st[2 * N + k] = s->dydzdV[k] * (s->dAx[1][k] * (0.5 * (txy[k + 1] + txy[k])) -
+                                      s->dAx[0][k] * (0.5 * (txy[k - 1] + txy[k]))) +
+                      s->dzdxdV[k] * (s->dAy[1][k] * (0.5 * (tyy[k + NI] + tyy[k])) -
+                                      s->dAy[0][k] * (0.5 * (tyy[k - NI] + tyy[k]))) +
+                      0.0 * (0.0 * 0.0 - 0.0 * 0.0) + (-s->cxyx[k]) * txx[k] + 0.0 * 0.0 - (-s->cyxy[k]) * txy[k] -
+                      0.0 * 0.0;
+    }
+  if (V == 4) {
+    for (int k = 0; k < N; k++) {
+      s->vtmp[k] = vx[k] * s->btxx[k] + vy[k] * s->btxy[k];
+      s->vtmp1[k] = vx[k] * s->btxy[k] + vy[k] * s->btyy[k];
+    }
+    for (int j = 1; j < NJ - 1; j++)
+      for (int i = 1; i < NI - 1; i++) st[3 * N + IDX(s, i, j)] = div2d(s, IDX(s, i, j), s->vtmp, s->vtmp1);
+  }
+}
+
+static void external_sources(fvr *s, const double *p, const double *c, double *st, double dt, double time) {
   int N = s->N, V = s->V;
-  if (!s->c.gravity && !s->c.rotframe) return;
+  if (!s->c.gravity && !s->c.rotframe && !s->c.vismodel) return;
   PAR_FOR
   for (int k = 0; k < V * N; k++) st[k] = 0.0;
   if (s->c.gravity) {
@@ -567,6 +683,11 @@ static void external_sources(fvr *s, const double *p, const double *c, double *s
     source_term(s, c, ax, ay, N, V);
     PAR_FOR
     for (int k = 0; k < V * N; k++) st[k] = st[k] + s->sterm[k];
+  }
+  if (s->c.vismodel) {  /* sources_viscosity.f90:481-500 */
+    update_viscosity(s, time, p);
+    viscosity_sources(s, p);
+    for (int k = 0; k < V * N; k++) st[k] = st[k] + s->vsterm[k];
   }
   PAR_FOR
   for (int j = 0; j < s->NJ; j++)
@@ -801,7 +922,7 @@ static void check_data(fvr *s, const double *p, const double *c, int bm) {
 /* timedisc_base.f90:1073-1271 (rhstype 0, 2-D) */
 /* dt: ComputeRHS's own dt argument (timedisc_base.f90:1073), read by the
  * point-mass accretion update */
-static void compute_rhs(fvr *s, sv *p, sv *c, int checkbm, double *rhs, double dt) {
+static void compute_rhs(fvr *s, sv *p, sv *c, int checkbm, double *rhs, double dt, double t) {
   int N = s->N, V = s->V, imin = G, imax = G + s->c.inum - 1, jmin = G, jmax = G + s->c.jnum - 1;
   int fargo = s->c.fargo == 1 || s->c.fargo == 2;
   if (fargo) add_bg_velocity(s, p, c);
@@ -816,7 +937,7 @@ static void compute_rhs(fvr *s, sv *p, sv *c, int checkbm, double *rhs, double d
   if (V == 4) update_bccsound(s, s->pvar.d);
   if (checkbm != CHECK_NOTHING) check_data(s, p->d, c->d, checkbm);
   geometrical_sources(s, p->d, c->d, s->geo_src);
-  external_sources(s, p->d, c->d, s->src, dt);
+  external_sources(s, p->d, c->d, s->src, dt, t);
   if (fargo) {
     if (s->c.geometry == CARTESIAN) memset(s->geo_src, 0, sizeof(double) * V * N);
     add_fargo_sources(s, p->d, c->d, s->geo_src);
@@ -862,7 +983,26 @@ static double calc_timestep(fvr *s) {
     if (x > invdt) invdt = x;
   }
   s->dtcause = 0;
-  return s->c.cfl / invdt; /* gravity: dt_src = HUGE (sources_c_accel.f90:125) */
+  double dt = s->c.cfl / invdt;
+  /* the source list's time steps (sources_generic.f90:264-293): gravity and
+   * the rotating frame give HUGE (sources_c_accel.f90:125); the viscosity
+   * its advection-diffusion limit over every cell (sources_viscosity.f90:
+   * 507-577, the 2-D x-y branch) with the kinematic viscosity of
+   * Timedisc%pvar, after an UpdateViscosity at the current time */
+  if (s->c.vismodel) {
+    update_viscosity(s, s->time, p);
+    double iv = -HUGE_;
+    for (int k = 0; k < N; k++) {
+      const double kin = s->dynvis[k] / p[k];
+      const double x = (fmax2(s->maxwav[k], -s->minwav[k]) + 2 * kin / s->dlx[k]) / s->dlx[k] +
+                       (fmax2(s->maxwav[N + k], -s->minwav[N + k]) + 2 * kin / s->dly[k]) / s->dly[k];
+      if (x > iv) iv = x;
+    }
+    const double dtv = iv > TINY_ ? s->c.cvis / iv : HUGE_;
+    if (dtv < dt) s->dtcause = 3; /* VISCOSITY (sources_generic.f90:74) */
+    dt = fmin2(dt, dtv);
+  }
+  return dt;
 }
 
 /* timedisc_rkfehlberg.f90:277-302 */
@@ -942,10 +1082,9 @@ static void accumulate_bflux(fvr *s, double dt) {
 /* timedisc_rkfehlberg.f90:170-268 */
 static void solve_ode(fvr *s, double time, double *dt, double *err) {
   size_t n = (size_t)s->V * s->N;
-  (void)time;
   for (int m = 1; m < s->m; m++) {
     compute_cvar(s, m, *dt);
-    compute_rhs(s, &s->ptmp, &s->ctmp, CHECK_NOTHING, s->coeff[m], *dt);
+    compute_rhs(s, &s->ptmp, &s->ctmp, CHECK_NOTHING, s->coeff[m], *dt, time + s->cc[m] * *dt);
   }
   memcpy(s->ctmp.d, s->cvar.d, sizeof(double) * n);
   s->ctmp.flag = s->cvar.flag;
@@ -1027,9 +1166,8 @@ static void fargo_advection_y(fvr *s) {
 
 /* timedisc_base.f90:961-1009 */
 static void reject_solution(fvr *s, double time, double dt) {
-  (void)time;
   copy_sv(s, &s->cvar, &s->cold);
-  compute_rhs(s, &s->pvar, &s->cvar, CHECK_NOTHING, s->coeff[0], dt);
+  compute_rhs(s, &s->pvar, &s->cvar, CHECK_NOTHING, s->coeff[0], dt, time);
   memcpy(s->bx, s->bxo, sizeof(double) * 2 * s->V * s->NJ);
   memcpy(s->by, s->byo, sizeof(double) * 2 * s->V * s->NI);
   s->n_adj++;
@@ -1041,7 +1179,7 @@ static void reject_solution(fvr *s, double time, double dt) {
 static void accept_solution(fvr *s, double *time, double dt) {
   *time = *time + dt;
   if (*time < s->time + s->dt) {
-    compute_rhs(s, &s->pvar, &s->cvar, CHECK_ALL, s->coeff[0], dt);
+    compute_rhs(s, &s->pvar, &s->cvar, CHECK_ALL, s->coeff[0], dt, *time);
     copy_sv(s, &s->cold, &s->cvar);
   }
   memcpy(s->bxo, s->bx, sizeof(double) * 2 * s->V * s->NJ);
@@ -1080,9 +1218,9 @@ static void integration_step(fvr *s) {
       PAR_FOR
       for (int k = 0; k < s->N; k++) cons2prim_pt(s, s->pvar.d, s->cvar.d, s->N, k);
       s->pvar.flag = s->cvar.flag;
-      compute_rhs(s, &s->pvar, &s->cvar, CHECK_ALL, s->coeff[0], 0.0);
+      compute_rhs(s, &s->pvar, &s->cvar, CHECK_ALL, s->coeff[0], 0.0, s->time);
     } else {
-      compute_rhs(s, &s->pvar, &s->cvar, CHECK_ALL, s->coeff[0], dtold);
+      compute_rhs(s, &s->pvar, &s->cvar, CHECK_ALL, s->coeff[0], dtold, s->time);
     }
     copy_sv(s, &s->cold, &s->cvar);
   }
@@ -1164,6 +1302,7 @@ fvr *fvr_create(const fvr_cfg *cfg) {
   }
   if (cfg->outbound < 0 || cfg->outbound > 4) return NULL;
   if (cfg->rotframe < 0 || cfg->rotframe > 2) return NULL;
+  if (cfg->vismodel < 0 || cfg->vismodel > 2) return NULL;
   s = (fvr *)calloc(1, sizeof(fvr));
   s->c = *cfg;
   s->pm_mass = cfg->mass; /* mdot, massloss, accrate start at 0 (gravity_pointmass.f90:206-209) */
@@ -1181,6 +1320,9 @@ fvr *fvr_create(const fvr_cfg *cfg) {
     s->rad = zalloc(N); s->posx = zalloc(N); s->posy = zalloc(N); s->bcx = zalloc(N); s->bcy = zalloc(N);
     s->caccel[0] = zalloc(N); s->caccel[1] = zalloc(N); s->twoomega = zalloc(N);
     s->raccel[0] = zalloc(N); s->raccel[1] = zalloc(N);
+    s->dynvis = zalloc(N); s->bulkvis = zalloc(N); s->ephir[0] = zalloc(N); s->ephir[1] = zalloc(N);
+    s->btxx = zalloc(N); s->btxy = zalloc(N); s->btyy = zalloc(N); s->vtmp = zalloc(N); s->vtmp1 = zalloc(N);
+    s->vsterm = zalloc(V * N);
     s->bccsound = zalloc(N); s->fcsound = zalloc(4 * N); s->w = zalloc(s->NI); s->wfac = zalloc(s->NI);
     s->pvar.d = zalloc(V * N); s->cvar.d = zalloc(V * N); s->ptmp.d = zalloc(V * N);
     s->ctmp.d = zalloc(V * N); s->cold.d = zalloc(V * N);
@@ -1232,6 +1374,8 @@ void fvr_destroy(fvr *s) {
   for (int f = 0; f < 4; f++) { free(s->rdx[f]); free(s->radf[f]); }
   free(s->rad); free(s->posx); free(s->posy); free(s->bcx); free(s->bcy);
   free(s->caccel[0]); free(s->caccel[1]); free(s->twoomega); free(s->raccel[0]); free(s->raccel[1]);
+  free(s->dynvis); free(s->bulkvis); free(s->ephir[0]); free(s->ephir[1]);
+  free(s->btxx); free(s->btxy); free(s->btyy); free(s->vtmp); free(s->vtmp1); free(s->vsterm);
   free(s->bccsound); free(s->fcsound); free(s->w); free(s->wfac);
   free(s->pvar.d); free(s->cvar.d); free(s->ptmp.d); free(s->ctmp.d); free(s->cold.d);
   free(s->geo_src); free(s->src); free(s->sterm); free(s->cerr); free(s->xflux); free(s->yflux);
@@ -1278,7 +1422,7 @@ int fvr_first_step(fvr *s) {
     center_boundary(s, &s->pvar, &s->cvar);
     calc_background_velocity(s);
   }
-  compute_rhs(s, &s->pvar, &s->cvar, CHECK_ALL, s->coeff[0], 0.0);
+  compute_rhs(s, &s->pvar, &s->cvar, CHECK_ALL, s->coeff[0], 0.0, s->time);
   s->dt = calc_timestep(s);
   copy_sv(s, &s->cold, &s->cvar);
   /* initial WriteDataset: fileio_base.f90:523-533 moves the state to the
@@ -1341,7 +1485,7 @@ void fvr_compute_rhs(fvr *s, double *rhs_out) {
   memcpy(p.d, s->pvar.d, sizeof(double) * n);
   memcpy(c.d, s->cvar.d, sizeof(double) * n);
   memset(rhs_out, 0, sizeof(double) * n);
-  compute_rhs(s, &p, &c, CHECK_NOTHING, rhs_out, -1.0);
+  compute_rhs(s, &p, &c, CHECK_NOTHING, rhs_out, -1.0, s->time);
   free(p.d);
   free(c.d);
 }

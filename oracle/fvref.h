@@ -42,6 +42,9 @@ typedef struct fvr_cfg {
   double omega;       /* angular speed of the rotating frame (mesh_base.f90:338) */
   int rotframe;       /* rotating-frame source (sources_rotframe.f90): 0 off, 1 on,
                          2 on with disable_centaccel; rotation centre at the origin */
+  int vismodel;       /* viscosity source (sources_viscosity.f90): 0 off, 1 MOLECULAR, 2 ALPHA */
+  double dynconst, bulkconst, cvis; /* dynamic viscosity constant (alpha), bulk viscosity
+                         (MOLECULAR), viscous Courant number (sources_viscosity.f90:145-162) */
 } fvr_cfg;
 
 typedef struct fvr_scalars {

@@ -40,7 +40,10 @@ def _nml(cfg, nsteps, dump_every, infile, prefix):
                 k_nsteps=nsteps, k_dumpevery=dump_every, k_count=int(cfg.get("count", 1)),
                 k_pmin=float(cfg.get("pmin", -1.0)), k_rhomin=float(cfg.get("rhomin", -1.0)),
                 k_starttime=float(cfg.get("starttime", 0.0)), k_omega=float(cfg.get("omega", 0.0)),
-                k_rotframe=int(cfg.get("rotframe", 0)))
+                k_rotframe=int(cfg.get("rotframe", 0)), k_vismodel=int(cfg.get("vismodel", 0)),
+                k_dynconst=float(cfg.get("dynconst", 0.1)), k_cvis=float(cfg.get("cvis", 0.5)),
+                k_setbulk=int(cfg.get("bulkconst") is not None),
+                k_bulkconst=float(cfg["bulkconst"]) if cfg.get("bulkconst") is not None else 0.0)
     lines = ["&fvcfg"]
     for k, v in vals.items():
         lines.append(" %s = %s" % (k, repr(float(v)) if isinstance(v, float) else v))

@@ -97,6 +97,20 @@ CASES = {
     "euler_rotframe": (dict(E, omega=2.0, rotframe=1), 8, (1, 8)),
     "euler_disk_rotframe_prim": (dict(S.euler_disk_config(ni=16, nj=32), omega=0.2, rotframe=1,
                                       recvars=S.PRIMITIVE), 8, (8,)),
+    # the viscosity source (sources_viscosity.f90): constant (MOLECULAR) and
+    # Shakura-Sunyaev (ALPHA) viscosity, its time-step limit and the
+    # update-only-when-the-time-changes rule (Dormand-Prince's stages 6, 7)
+    "euler_viscous_molecular": (dict(E, vismodel=1, dynconst=1e-3), 8, (1, 8)),
+    "euler_viscous_molecular_bulk_periodic": (dict(E, vismodel=1, dynconst=2e-3, bulkconst=5e-4,
+                                                   bc=[S.PERIODIC] * 4), 8, (8,)),
+    "euler_viscous_molecular_dp5": (dict(E, vismodel=1, dynconst=5e-3, cvis=0.2, method=S.DORMAND_PRINCE,
+                                         order=5, tol_rel=1e-4), 8, (8,)),
+    "disk_alpha_fargo1_rotframe": (dict(D, fargo=1, vismodel=2, dynconst=0.01, omega=0.3, rotframe=1,
+                                        bc=[S.NO_GRADIENTS, S.NO_GRADIENTS, S.PERIODIC, S.PERIODIC]), 8, (1, 8)),
+    "disk_alpha_no_fargo_limited": (dict(D, fargo=0, vismodel=2, dynconst=0.3, cvis=0.2,
+                                         bc=[S.NO_GRADIENTS, S.NO_GRADIENTS, S.PERIODIC, S.PERIODIC]), 8, (8,)),
+    "disk_alpha_dp5_absorbing": (dict(D, fargo=0, vismodel=2, dynconst=0.05, method=S.DORMAND_PRINCE, order=5,
+                                      tol_rel=1e-4), 8, (8,)),
 }
 MESHES = {"cartesian_16": E, "logcyl_16x32": D}
 SCALARS = ("iter", "n_adj", "brk", "time", "dt", "dtold", "maxerrold")
