@@ -40,7 +40,9 @@ class FvConfig(ctypes.Structure):
                 ("outbound", ctypes.c_int), ("acclimit", ctypes.c_double),
                 ("pmin", ctypes.c_double), ("rhomin", ctypes.c_double),
                 ("count", ctypes.c_int), ("starttime", ctypes.c_double),
-                ("omega", ctypes.c_double), ("rotframe", ctypes.c_int)]
+                ("omega", ctypes.c_double), ("rotframe", ctypes.c_int),
+                ("vismodel", ctypes.c_int), ("dynconst", ctypes.c_double), ("bulkconst", ctypes.c_double),
+                ("cvis", ctypes.c_double)]
 
     @classmethod
     def from_dict(cls, cfg):
@@ -311,7 +313,7 @@ class Fosite:
     def profile(self, enable=True):
         self._chk(self.L.fv_profile(self._h, int(enable)), "fv_profile")
 
-    KERNELS = {"rhs": 0, "rhs_end": 1, "rhs_combine": 2, "frame": 3, "control": 4, "remap": 5, "accrete": 6}
+    KERNELS = {"rhs": 0, "rhs_end": 1, "rhs_combine": 2, "frame": 3, "control": 4, "remap": 5, "accrete": 6, "visc": 7}
 
     def kernel_stats(self, kernel):
         n, ms, b = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()

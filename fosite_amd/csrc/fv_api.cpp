@@ -31,15 +31,17 @@ namespace {
 constexpr int NBUF = 4;
 // kernel classes of the timing statistics (fv_kernel_stats)
 constexpr int K_RHS = 0, K_RHS_END = 1, K_RHS_COMBINE = 2, K_FRAME = 3, K_CONTROL = 4, K_REMAP = 5,
-              K_ACCRETE = 6, K_CLASSES = 7;
+              K_ACCRETE = 6, K_VISC = 7, K_CLASSES = 8;
 // timedisc_base.f90:193-203
 constexpr int CHECK_NOTHING = 0, CHECK_ALL = ~0, CHECK_CSOUND = 1, CHECK_PMIN = 2;
 constexpr int DTCAUSE_ERRADJ = -1, DTCAUSE_SMALLERR = -2, DTCAUSE_FILEIO = -4;
+constexpr int DTCAUSE_VISCOSITY = 3;  // its source's position in sources_generic.f90:74
 enum { RK_FEHLBERG = 2, CASH_KARP = 3, DORMAND_PRINCE = 4, SSPRK = 5 };
 
 struct Tableau {
   int m;
   double a[MAXK][MAXK], bh[MAXK], bl[MAXK];
+  double c[MAXK];  // stage times: ComputeRHS(time + c(m)*dt) (timedisc_rkfehlberg.f90:277-302)
 };
 
 // Butcher tableaux: timedisc_ssprk.f90:93-126, timedisc_rkfehlberg.f90:305-335,
@@ -51,8 +53,8 @@ bool set_tableau(int method, int order, Tableau &t) {
     t.m = 3;
     t.a[1][0] = 1.0;
     t.a[2][0] = 0.25; t.a[2][1] = 0.25;
-    double bh[] = {1. / 6., 1. / 6., 2. / 3.}, bl[] = {0.5, 0.5, 0.};
-    std::memcpy(t.bh, bh, sizeof(bh)); std::memcpy(t.bl, bl, sizeof(bl));
+    double bh[] = {1. / 6., 1. / 6., 2. / 3.}, bl[] = {0.5, 0.5, 0.}, cc[] = {0., 1., 0.5};
+    std::memcpy(t.bh, bh, sizeof(bh)); std::memcpy(t.bl, bl, sizeof(bl)); std::memcpy(t.c, cc, sizeof(cc));
   } else if (method == SSPRK && order == 5) {
     t.m = 5;
     double a[5][5] = {{0., 0., 0., 0., 0.}, {0.36717, 0., 0., 0., 0.}, {0.26802, 0.3172, 0., 0., 0.},
@@ -60,13 +62,14 @@ bool set_tableau(int method, int order, Tableau &t) {
     for (int i = 0; i < 5; i++) for (int j = 0; j < 5; j++) t.a[i][j] = a[i][j];
     double bh[] = {0.17279, 0.094505, 0.12947, 0.29899, 0.30424};
     double bl[] = {0.12293, 0.31981, -0.15316, 0.31887, 0.39155};
-    std::memcpy(t.bh, bh, sizeof(bh)); std::memcpy(t.bl, bl, sizeof(bl));
+    double cc[] = {0., 0.36717, 0.58522, 0.44156, 0.8464};
+    std::memcpy(t.bh, bh, sizeof(bh)); std::memcpy(t.bl, bl, sizeof(bl)); std::memcpy(t.c, cc, sizeof(cc));
   } else if (method == RK_FEHLBERG && order == 3) {
     t.m = 3;
     t.a[1][0] = 0.5;
     t.a[2][0] = -1.0; t.a[2][1] = 2.0;
-    double bh[] = {1.0 / 6.0, 2.0 / 3.0, 1.0 / 6.0}, bl[] = {0.0, 1.0, 0.0};
-    std::memcpy(t.bh, bh, sizeof(bh)); std::memcpy(t.bl, bl, sizeof(bl));
+    double bh[] = {1.0 / 6.0, 2.0 / 3.0, 1.0 / 6.0}, bl[] = {0.0, 1.0, 0.0}, cc[] = {0.0, 0.5, 1.0};
+    std::memcpy(t.bh, bh, sizeof(bh)); std::memcpy(t.bl, bl, sizeof(bl)); std::memcpy(t.c, cc, sizeof(cc));
   } else if (method == RK_FEHLBERG && order == 5) {
     t.m = 6;
     double a[6][6] = {{0, 0, 0, 0, 0, 0},
@@ -78,7 +81,8 @@ bool set_tableau(int method, int order, Tableau &t) {
     for (int i = 0; i < 6; i++) for (int j = 0; j < 6; j++) t.a[i][j] = a[i][j];
     double bh[] = {16.0 / 135.0, 0.0, 6656.0 / 12825.0, 28561.0 / 56430.0, -9.0 / 50.0, 2.0 / 55.0};
     double bl[] = {25.0 / 216.0, 0.0, 1408.0 / 2565.0, 2197.0 / 4104.0, -0.2, 0.0};
-    std::memcpy(t.bh, bh, sizeof(bh)); std::memcpy(t.bl, bl, sizeof(bl));
+    double cc[] = {0.0, 0.25, 3.0 / 8.0, 12.0 / 13.0, 1.0, 0.5};
+    std::memcpy(t.bh, bh, sizeof(bh)); std::memcpy(t.bl, bl, sizeof(bl)); std::memcpy(t.c, cc, sizeof(cc));
   } else if (method == DORMAND_PRINCE && order == 5) {
     t.m = 7;
     double a[7][7] = {{0, 0, 0, 0, 0, 0, 0},
@@ -91,7 +95,8 @@ bool set_tableau(int method, int order, Tableau &t) {
     for (int i = 0; i < 7; i++) for (int j = 0; j < 7; j++) t.a[i][j] = a[i][j];
     double bh[] = {35. / 384., 0., 500. / 1113., 125. / 192., -2187. / 6784., 11. / 84., 0.};
     double bl[] = {5179. / 57600., 0., 7571. / 16695., 393. / 640., -92097. / 339200., 187. / 2100., 1. / 40.};
-    std::memcpy(t.bh, bh, sizeof(bh)); std::memcpy(t.bl, bl, sizeof(bl));
+    double cc[] = {0., .2, 3. / 10., 4. / 5., 8. / 9., 1., 1.};
+    std::memcpy(t.bh, bh, sizeof(bh)); std::memcpy(t.bl, bl, sizeof(bl)); std::memcpy(t.c, cc, sizeof(cc));
   } else if (method == CASH_KARP && order == 5) {  // timedisc_cashkarp.f90:101-113
     t.m = 6;
     double a[6][6] = {{0, 0, 0, 0, 0, 0},
@@ -103,7 +108,8 @@ bool set_tableau(int method, int order, Tableau &t) {
     for (int i = 0; i < 6; i++) for (int j = 0; j < 6; j++) t.a[i][j] = a[i][j];
     double bh[] = {37.0 / 378.0, 0.0, 250.0 / 621.0, 125.0 / 594.0, 0.0, 512.0 / 1771.0};
     double bl[] = {2825.0 / 27648.0, 0.0, 18575.0 / 48384.0, 13525.0 / 55296.0, 277.0 / 14336.0, 0.25};
-    std::memcpy(t.bh, bh, sizeof(bh)); std::memcpy(t.bl, bl, sizeof(bl));
+    double cc[] = {0.0, 0.2, 0.3, 0.6, 1.0, 7.0 / 8.0};
+    std::memcpy(t.bh, bh, sizeof(bh)); std::memcpy(t.bl, bl, sizeof(bl)); std::memcpy(t.c, cc, sizeof(cc));
   } else {
     return false;
   }
@@ -153,6 +159,11 @@ struct fv_ctx {
   int vcur = 0, wcur = 0;
   double *d_ring1 = nullptr;
   float *d_ringf1 = nullptr;
+  // viscosity (c.vismodel): the source of the evaluation in flight, and for
+  // ALPHA the dynamic viscosity of the last update and e_phi/r; vis_time is
+  // the time of that update (sources_viscosity.f90:371, -1 before the first)
+  double *d_vsrc = nullptr, *d_dyn = nullptr, *d_ephir = nullptr;
+  double vis_time = -1.0;
   Ctl *d_ctl = nullptr, *h_ctl = nullptr;
   struct GraphEntry { int x0, x1, t, k0, b0, g0, w0, v0, nsteps; hipGraphExec_t exec; };
   std::vector<GraphEntry> graphs;
@@ -291,6 +302,34 @@ StageArgs base_args(fv_ctx *ctx) {
   return a;
 }
 
+VisArgs vis_args(const fv_ctx *ctx, int update) {
+  VisArgs v;
+  std::memset(&v, 0, sizeof(v));
+  v.model = ctx->c.vismodel;
+  v.update = update;
+  v.dynconst = ctx->c.dynconst;
+  v.bulkconst = ctx->c.bulkconst;
+  v.omega = ctx->c.omega;
+  v.ephir = ctx->d_ephir;
+  v.dyn = ctx->d_dyn;
+  v.src = ctx->d_vsrc;
+  return v;
+}
+// the viscous source of the evaluation a at time t, ahead of its k_rhs
+// launch (UpdateViscosity recomputes the dynamic viscosity when t differs
+// from the last update's time or is 0, sources_viscosity.f90:371)
+int visc(fv_ctx *ctx, StageArgs &a, double t) {
+  if (!ctx->c.vismodel) return 0;
+  const int update = t != ctx->vis_time || t == 0.0;
+  ctx->vis_time = t;
+  long pk = prof_begin(ctx);
+  launch_visc(ctx->g, a, vis_args(ctx, update), ctx->V, ctx->st);
+  prof_end(ctx, pk, K_VISC, 8.0 * cells(ctx) * (ctx->V * (a.nk + 1.0) + ctx->V - 1.0));
+  HIPCHK(hipGetLastError());
+  a.vsrc = ctx->d_vsrc;
+  return 0;
+}
+
 // UpdateGravity_single of one ComputeRHS (gravity_pointmass.f90:383-428):
 // point-mass state gin -> 1 - gin, with the boundary fluxes bf[b] and the
 // call's dt argument
@@ -340,7 +379,7 @@ void rhs_flags(const fv_ctx *ctx, int &fp, int &fc, int &add, int &sub) {
 // (fresh buffer, ghost cells included); also the CFL inverse time step and
 // the CheckData flags (timedisc_base.f90:1666-1742).  remap = 1 evaluates it
 // on FargoAdvectionY(cvar) (the post-step remap, timedisc_base.f90:722-743).
-int rhs_main(fv_ctx *ctx, int checkbm, int remap) {
+int rhs_main(fv_ctx *ctx, int checkbm, int remap, double t) {
   int add, sub;
   rhs_flags(ctx, ctx->fl_pvar, ctx->fl_cvar, add, sub);
   const int pmin = (ctx->V == 4 && (checkbm & CHECK_PMIN)) ? 1 : 0;
@@ -367,6 +406,7 @@ int rhs_main(fv_ctx *ctx, int checkbm, int remap) {
   a.want_check = checkbm != CHECK_NOTHING;
   a.vout = ctx->vbufs[ctx->vcur];
   a.pout = ctx->pbuf;
+  if (int rc = visc(ctx, a, t)) return rc;
   long pk = prof_begin(ctx);
   launch_rhs(ctx->g, a, ctx->V, ctx->st);
   prof_end(ctx, pk, K_RHS_END, 3.0 * word_bytes(ctx));
@@ -440,8 +480,9 @@ void launch_stage(fv_ctx *ctx, const Geom &g, const StageArgs &a, int m) {
   prof_end(ctx, pk, last ? K_RHS_COMBINE : K_RHS, (m + 2.0) * word_bytes(ctx));
 }
 
-// ComputeRHS on the stage state (ptmp, ctmp), ctmp = cvar - sum a*dt*k
-int rhs_stage(fv_ctx *ctx, int m, double dt, int out) {
+// ComputeRHS on the stage state (ptmp, ctmp), ctmp = cvar - sum a*dt*k, at
+// time t
+int rhs_stage(fv_ctx *ctx, int m, double dt, int out, double t) {
   if (int rc = accrete(ctx, ctx->bcur, dt)) return rc;
   int add, sub;
   rhs_flags(ctx, ctx->fl_ptmp, ctx->fl_ctmp, add, sub);
@@ -449,6 +490,7 @@ int rhs_stage(fv_ctx *ctx, int m, double dt, int out) {
   a.dt = dt;
   a.add_bg = add;
   a.sub_bg = sub;
+  if (int rc = visc(ctx, a, t)) return rc;
   launch_stage(ctx, ctx->g, a, m);
   HIPCHK(hipGetLastError());
   return 0;
@@ -479,10 +521,10 @@ double adjust_timestep(fv_ctx *ctx, double maxerr, double dtold) {
 
 // timedisc_rkfehlberg.f90:170-268; the boundary fluxes of the step go to
 // bf[1 - bcur] (the last stage accumulates them)
-int solve_ode(fv_ctx *ctx, double *dt, double *err) {
+int solve_ode(fv_ctx *ctx, double time, double *dt, double *err) {
   const int outb = pick_free(ctx, ctx->cvar, ctx->cold, -1);
   for (int m = 1; m < ctx->tab.m; m++) {  // the last stage writes the combination
-    int rc = rhs_stage(ctx, m, *dt, outb);
+    int rc = rhs_stage(ctx, m, *dt, outb, time + ctx->tab.c[m] * *dt);
     if (rc) return rc;
   }
   launch_bflux(bflux_args(ctx, ctx->k0cur, ctx->bcur, *dt), ctx->st);
@@ -520,6 +562,31 @@ int cfl_real_frame(fv_ctx *ctx) {
   HIPCHK(hipGetLastError());
   int f = 0;
   return reduce(ctx, ctx->part_r, nb, &ctx->invdt, &f);
+}
+
+// CalcTimestep (timedisc_base.f90:818-909) into ctx->dt: the CFL limit, then
+// the source list's limits (sources_generic.f90:264-293), of which only the
+// viscosity's is finite: its advection-diffusion limit over every cell
+// (sources_viscosity.f90:507-577) on Timedisc%pvar, after UpdateViscosity at
+// the current time
+int timestep(fv_ctx *ctx) {
+  ctx->dt = calc_timestep(ctx);
+  if (!ctx->c.vismodel) return 0;
+  if (!ctx->pvalid) return fail(ctx, "viscous time step: no evaluation has stored Timedisc%pvar");
+  const int update = ctx->c.vismodel == VIS_ALPHA && (ctx->time != ctx->vis_time || ctx->time == 0.0);
+  ctx->vis_time = ctx->time;
+  const long cells = (long)ctx->NI * ctx->NJ;
+  const int nb = (int)std::min<long>(ctx->nb_rhs, (cells + 255) / 256);
+  launch_visc_cfl(ctx->g, vis_args(ctx, update), ctx->pbuf, pvar_moved_to_real_frame(ctx) ? 1 : 0, ctx->part_r, nb,
+                  ctx->V, ctx->st);
+  HIPCHK(hipGetLastError());
+  double iv = 0;
+  int f = 0;
+  if (int rc = reduce(ctx, ctx->part_r, nb, &iv, &f)) return rc;
+  const double dtv = iv > DBL_MIN ? ctx->c.cvis / iv : DBL_MAX;
+  if (dtv < ctx->dt) ctx->dtcause = DTCAUSE_VISCOSITY;
+  ctx->dt = dtv < ctx->dt ? dtv : ctx->dt;
+  return 0;
 }
 
 // out-of-place frame change of cvar (Physics%Add/SubtractBackgroundVelocityY)
@@ -568,7 +635,7 @@ int integration_step(fv_ctx *ctx) {
   if (dt < ctx->dtmin && ctx->dtcause != DTCAUSE_FILEIO) ctx->dtmin = dt;
   while (!ctx->brk && time + dt <= ctx->time + ctx->dt) {
     dtold = dt;
-    if ((rc = solve_ode(ctx, &dt, &err))) return rc;
+    if ((rc = solve_ode(ctx, time, &dt, &err))) return rc;
     if (err != err || err > DBL_MAX) {
       ctx->brk = 1;
     } else if (err >= 1.0) {  // RejectSolution (timedisc_base.f90:961-1009)
@@ -577,7 +644,7 @@ int integration_step(fv_ctx *ctx) {
       // its ComputeRHS runs before bxflux = bxfold: the point mass sees the
       // rejected step's boundary fluxes
       if ((rc = accrete(ctx, 1 - ctx->bcur, dt))) return rc;
-      if ((rc = rhs_main(ctx, CHECK_NOTHING, 0))) return rc;
+      if ((rc = rhs_main(ctx, CHECK_NOTHING, 0, time))) return rc;
       ctx->n_adj++;
       ctx->dtcause = DTCAUSE_ERRADJ;
       if (dt < ctx->dtmin) ctx->dtmin = dt;
@@ -586,7 +653,7 @@ int integration_step(fv_ctx *ctx) {
       ctx->bcur = 1 - ctx->bcur;  // bxfold = bxflux
       if (time < ctx->time + ctx->dt) {
         if ((rc = accrete(ctx, ctx->bcur, dtold))) return rc;
-        if ((rc = rhs_main(ctx, CHECK_ALL, 0))) return rc;
+        if ((rc = rhs_main(ctx, CHECK_ALL, 0, time))) return rc;
         ctx->cold = ctx->cvar;
         ctx->fl_cold = ctx->fl_cvar;
       }
@@ -605,7 +672,7 @@ int integration_step(fv_ctx *ctx) {
     if (remap) ctx->fl_pvar = ctx->fl_cvar;
     // ComputeRHS(..., this%time, 0. with FARGO, else dtold, ...) (timedisc_base.f90:742-746)
     if ((rc = accrete(ctx, ctx->bcur, remap ? 0.0 : dtold))) return rc;
-    if ((rc = rhs_main(ctx, CHECK_ALL, remap))) return rc;
+    if ((rc = rhs_main(ctx, CHECK_ALL, remap, ctx->time))) return rc;
     ctx->cold = ctx->cvar;
     ctx->fl_cold = ctx->fl_cvar;
   }
@@ -952,6 +1019,10 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
   if (c.outbound < 0 || c.outbound > 4) return bail("outbound must be 0 (no accretion) or a boundary 1..4");
   if (c.outbound && !c.gravity) return bail("accretion (outbound) needs the point mass (gravity = 1)");
   if (c.rotframe < 0 || c.rotframe > 2) return bail("rotframe must be 0 (off), 1 or 2 (disable_centaccel)");
+  if (c.vismodel < 0 || c.vismodel > 2) return bail("vismodel must be 0 (off), 1 MOLECULAR or 2 ALPHA");
+  if (c.vismodel && c.dtype != 0) return bail("viscosity is supported with the FP64 state (dtype 0)");
+  if (c.vismodel == VIS_ALPHA && c.physics != EULER_ISOTHERM)
+    return bail("ALPHA viscosity is supported with isothermal physics");
   if (!set_tableau(c.method, c.order, ctx->tab)) return bail("unsupported time integrator (method/order)");
   if (mesh_build(c, ctx->mesh)) return bail("mesh construction failed");
   ctx->V = c.physics == EULER ? 4 : 3;
@@ -965,7 +1036,8 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
   const struct { int r, m; } rm[] = {{R_DAXW, M_DAXW}, {R_DAYS, M_DAYS}, {R_DYDZDV, M_DYDZDV}, {R_DZDXDV, M_DZDXDV},
                                      {R_DXDYDV, M_DXDYDV}, {R_DLX, M_DLX}, {R_DLY, M_DLY}, {R_CXYX, M_CXYX},
                                      {R_CYXY, M_CYXY}, {R_CZXZ, M_CZXZ}, {R_CZYZ, M_CZYZ}, {R_RDXW, M_RDXW},
-                                     {R_RDXE, M_RDXE}, {R_ACCX, M_ACCX}, {R_ACCY, M_ACCY}, {R_WFAC, M_WFAC}};
+                                     {R_RDXE, M_RDXE}, {R_ACCX, M_ACCX}, {R_ACCY, M_ACCY}, {R_WFAC, M_WFAC},
+                                     {R_DAXE, M_DAXE}, {R_DAYN, M_DAYN}};
   for (auto &p : rm)
     if (!ring_from(ctx, p.r, ctx->mesh.a[p.m].data())) return bail("mesh metric depends on j");
   for (int j = 0; j < ctx->NJ; j++) {
@@ -989,6 +1061,8 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
       ctx->col_h[(size_t)C_CACY * ctx->NJ + j] = (om * ctx->mesh.a[M_POSY][(size_t)j * ctx->NI + GH]) * om;
   }
   if (const char *e = std::getenv("FV_HOST_CONTROL")) ctx->fast = std::atoi(e) == 0;
+  // the device-controlled steps carry no viscous source or time-step limit
+  if (c.vismodel) ctx->fast = false;
   // device resources
   const size_t sbytes = elem_bytes(ctx) * ctx->V * ctx->FS;
   auto dalloc = [&](double **p, size_t b) { return hipMalloc((void **)p, b) == hipSuccess; };
@@ -1050,6 +1124,9 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
     ok = dalloc(&ctx->grav[q], sizeof(double) * (PM_HDR + 2 * ctx->NI));
     if (ok && c.dtype == 1) ok = hipMalloc((void **)&ctx->gravf[q], sizeof(float) * 2 * ctx->NI) == hipSuccess;
   }
+  if (c.vismodel) ok = ok && dalloc(&ctx->d_vsrc, sizeof(double) * (ctx->V - 1) * ctx->FS);
+  if (c.vismodel == VIS_ALPHA)
+    ok = ok && dalloc(&ctx->d_dyn, sizeof(double) * ctx->FS) && dalloc(&ctx->d_ephir, sizeof(double) * 2 * ctx->FS);
   if (!ok) {
     fv_destroy(ctx);
     std::fprintf(stderr, "fv_create: device allocation failed\n");
@@ -1060,6 +1137,38 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
   g.ringf = ctx->d_ringf;
   g.colf = ctx->d_colf;
   if (upload_geom(ctx)) { fv_destroy(ctx); return -3; }
+  if (c.vismodel == VIS_ALPHA) {
+    // e_phi/r = (-y/r^2, x/r^2) at the Cartesian bary centres, r^2 = radius^2
+    // (sources_viscosity.f90:196-223; geometry_logcylindrical.f90:344-356),
+    // in curvilinear components (geometry_cylindrical.f90:485-494; the
+    // Cartesian geometry keeps them)
+    std::vector<double> e(2 * (size_t)ctx->FS, 0.0);
+    for (int j = 0; j < ctx->NJ; j++)
+      for (int i = 0; i < ctx->NI; i++) {
+        const size_t k = (size_t)j * ctx->NI + i, ix = (size_t)j * ctx->P + i;
+        const double phi = ctx->mesh.a[M_BCY][k], rad = ctx->mesh.a[M_RAD][k];
+        double x = ctx->mesh.a[M_BCX][k], y = phi;
+        if (c.geometry == LOGCYLINDRICAL) {
+          const double r = c.gparam * std::exp(x);
+          x = r * std::cos(phi);
+          y = r * std::sin(phi);
+        }
+        const double r2 = rad * rad, ex = -(y / r2), ey = x / r2;
+        if (c.geometry == LOGCYLINDRICAL) {
+          e[ix] = ex * std::cos(phi) + ey * std::sin(phi);
+          e[ctx->FS + ix] = -(ex * std::sin(phi)) + ey * std::cos(phi);
+        } else {
+          e[ix] = ex;
+          e[ctx->FS + ix] = ey;
+        }
+      }
+    if (hipMemcpyAsync(ctx->d_ephir, e.data(), sizeof(double) * e.size(), hipMemcpyHostToDevice, ctx->st) !=
+            hipSuccess ||
+        hipStreamSynchronize(ctx->st) != hipSuccess) {
+      fv_destroy(ctx);
+      return -3;
+    }
+  }
   if (c.outbound) {  // InitGravity_pointmass: mass; mdot, massloss, accrate = 0 (gravity_pointmass.f90:206-209)
     std::vector<double> gh(PM_HDR + 2 * (size_t)ctx->NI, 0.0);
     gh[PM_MASS] = c.mass;
@@ -1106,6 +1215,9 @@ extern "C" int fv_destroy(fv_ctx *ctx) {
   for (auto &b : ctx->gravf) if (b) (void)hipFree(b);
   for (auto &b : ctx->vbufs) if (b) (void)hipFree(b);
   if (ctx->pbuf) (void)hipFree(ctx->pbuf);
+  if (ctx->d_vsrc) (void)hipFree(ctx->d_vsrc);
+  if (ctx->d_dyn) (void)hipFree(ctx->d_dyn);
+  if (ctx->d_ephir) (void)hipFree(ctx->d_ephir);
   if (ctx->d_ring1) (void)hipFree(ctx->d_ring1);
   if (ctx->d_ringf1) (void)hipFree(ctx->d_ringf1);
   if (ctx->bks) (void)hipFree(ctx->bks);
@@ -1141,6 +1253,8 @@ extern "C" int fv_device_bytes(const fv_config *c, double *bytes) {
   if (c->fargo == 1)                                    // FARGO type 1: velocities, second ring arrays
     b += 2.0 * 8.0 * NJ * P + (8.0 + (c->dtype == 1 ? 4.0 : 0.0)) * R_COUNT * NI;
   if (c->outbound) b += 2.0 * (8.0 * (PM_HDR + 2.0 * NI) + (c->dtype == 1 ? 4.0 * 2.0 * NI : 0.0));
+  if (c->vismodel) b += 8.0 * (V - 1.0) * NJ * P;       // viscous source
+  if (c->vismodel == 2) b += 3.0 * 8.0 * NJ * P;        // ALPHA: dynamic viscosity, e_phi/r
   b += 2.0 * 2.0 * 8.0 * nb32;                         // CFL and error partials
   b += 8.0 * (R_COUNT * NI + C_COUNT * NJ);             // metric arrays
   if (c->dtype == 1) b += 4.0 * (R_COUNT * NI + C_COUNT * NJ);
@@ -1246,9 +1360,9 @@ extern "C" int fv_first_rhs(fv_ctx *ctx, int *brk) {
     if ((rc = calc_background_velocity(ctx))) return rc;
   }
   if ((rc = accrete(ctx, ctx->bcur, 0.0))) return rc;  // ComputeRHS(..., 0.0, ...) (fosite.f90:315-317)
-  rc = rhs_main(ctx, CHECK_ALL, 0);
+  rc = rhs_main(ctx, CHECK_ALL, 0, ctx->time);
   if (rc) return rc;
-  ctx->dt = calc_timestep(ctx);
+  if ((rc = timestep(ctx))) return rc;
   ctx->cold = ctx->cvar;
   ctx->fl_cold = ctx->fl_cvar;
   if (brk) *brk = ctx->brk;
@@ -1279,7 +1393,7 @@ extern "C" int fv_calc_timestep(fv_ctx *ctx, double *dt, int *dtcause) {
   if (ctx->iter < 0) return fail(ctx, "fv_calc_timestep before the first step");
   int rc;
   if (pvar_moved_to_real_frame(ctx) && (rc = cfl_real_frame(ctx))) return rc;
-  ctx->dt = calc_timestep(ctx);
+  if ((rc = timestep(ctx))) return rc;
   if (dt) *dt = ctx->dt;
   if (dtcause) *dtcause = ctx->dtcause;
   return 0;
@@ -1319,8 +1433,7 @@ extern "C" int fv_step(fv_ctx *ctx, int *done) {
     ctx->dtmin = ctx->c.stoptime;
     ctx->n_adj = 0;
   }
-  ctx->dt = calc_timestep(ctx);
-  return 0;
+  return timestep(ctx);
 }
 
 // fv_step with the reduction scratch of its kernels (per-workgroup partials
@@ -1860,6 +1973,7 @@ extern "C" int fv_batch_add(fv_batch *b, fv_ctx *x) {
   if (!b || !x) return -1;
   if (x->batch) return bfail(b, "context already belongs to a batch");
   if ((int)b->sims.size() >= MAX_BATCH) return bfail(b, "batch is full (" + std::to_string(MAX_BATCH) + " members)");
+  if (x->c.vismodel) return bfail(b, "viscosity is supported on single contexts, not in batches");
   const int nbx = (x->c.inum + 31) / 32, nby = (x->c.jnum + TY_TILE - 1) / TY_TILE, rby = remap_rows_blocks(x->c.jnum);
   if (nbx >= (1 << TILE_BX_BITS) || nby >= (1 << TILE_BY_BITS) || rby >= (1 << TILE_BY_BITS))
     return bfail(b, "mesh too large for a batch tile code");

@@ -22,7 +22,8 @@ enum Limiter { MINMOD = 1, MONOCENT = 2, SWEBY = 3, SUPERBEE = 4, OSPRE = 5, VAN
 // frame's centrifugal acceleration, which depend on j only.
 enum RingId {
   R_DAXW, R_DAYS, R_DYDZDV, R_DZDXDV, R_DXDYDV, R_DLX, R_DLY, R_CXYX, R_CYXY, R_CZXZ, R_CZYZ,
-  R_RDXW, R_RDXE, R_ACCX, R_ACCY, R_CS, R_CSW, R_CSE, R_CSS, R_CSN, R_W, R_WFAC, R_CACX, R_COUNT
+  R_RDXW, R_RDXE, R_ACCX, R_ACCY, R_CS, R_CSW, R_CSE, R_CSS, R_CSN, R_W, R_WFAC, R_CACX,
+  R_DAXE, R_DAYN, R_COUNT
 };
 enum ColId { C_RDYS, C_RDYN, C_CACY, C_COUNT };
 
@@ -144,8 +145,24 @@ struct StageArgsT {
   // the static rings, or the current point-mass state with accretion
   const double *acc;
   const float *accf;    // FP32 copies (dtype 1)
+  // viscous source of this evaluation (fields 1..V-1, [V-1][NJ][P], interior
+  // cells; k_visc writes it before the launch), or null
+  const double *vsrc;
 };
 using StageArgs = StageArgsT<double>;
+
+// Viscosity source (sources_viscosity.f90:73-577), FP64 state only.
+enum VisModel { VIS_MOLECULAR = 1, VIS_ALPHA = 2 };
+struct VisArgs {
+  int model;
+  int update;           // ALPHA: this evaluation recomputes the dynamic viscosity
+                        // (UpdateViscosity's time test, sources_viscosity.f90:371),
+                        // else it reads the one stored by the last update
+  double dynconst, bulkconst, omega;
+  const double *ephir;  // ALPHA: e_phi/r in curvilinear components, [2][NJ][P]
+  double *dyn;          // ALPHA: dynamic viscosity of the last update, all cells, [NJ][P]
+  double *src;          // k_visc output: StageArgs::vsrc
+};
 
 // FargoAdvectionY on the combined state (timedisc_base.f90:1935-2069):
 // comoving-frame CenterBoundary for the y-ghost rows, then per ring the
@@ -200,6 +217,12 @@ void launch_frame(const Geom &g, const void *src, void *dst, int sign, int V, hi
 void launch_reduce(const double *part, int n, double *out, hipStream_t s);
 // CalcTimestep partials (max per block) from a stored pvar, v + w when add_w
 void launch_cfl_pvar(const Geom &g, const void *pv, int add_w, double *part, int nblocks, int V, hipStream_t s);
+// viscous source of the evaluation a into v.src (FP64)
+void launch_visc(const Geom &g, const StageArgs &a, const VisArgs &v, int V, hipStream_t s);
+// viscous time-step limit (partials, max per block) from a stored FP64 pvar
+// (v + w when add_w); v.update: the dynamic viscosity from that pvar
+void launch_visc_cfl(const Geom &g, const VisArgs &v, const double *pv, int add_w, double *part, int nblocks, int V,
+                     hipStream_t s);
 void launch_control(Ctl *ctl, const double *part_c, const double *part_r, int nb, hipStream_t s);
 void launch_bflux(const BfluxArgs &bf, hipStream_t s);
 void launch_accrete(const AccreteArgs &p, hipStream_t s);

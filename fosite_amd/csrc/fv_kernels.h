@@ -1195,6 +1195,14 @@ __device__ __forceinline__ void rhs_tile(const Geom &g, const StageArgsT<T> &a, 
     const T dxdydV = rg<T>(g, R_DXDYDV, i);
     const unsigned ix = (unsigned)j * (unsigned)g.P + (unsigned)i;
     T rhs[V];
+    // the viscous source joins the source list's sum last (sources_generic.f90:
+    // 236-241); it has no density component
+    if constexpr (sizeof(T) == 8) {
+      if (a.vsrc) {
+#pragma unroll
+        for (int l = 1; l < V; l++) ext[l] = ext[l] + (a.vsrc + (l - 1) * g.FS)[ix];
+      }
+    }
 #pragma unroll
     for (int l = 0; l < V; l++) {
       const T xw = FLX(l, ty, tx), ys = FLY(l, ty, tx);
@@ -1512,6 +1520,174 @@ __global__ __launch_bounds__(256) void k_frame(Geom g, const T *src, T *dst, int
   if (w2) frame_sub<T, V>(u, w2[i]);
 #pragma unroll
   for (int l = 0; l < V; l++) dst[l * g.FS + ix] = u[l];
+}
+
+// ------------------------------------------------------------- viscosity
+// ALPHA dynamic viscosity of one cell (sources_viscosity.f90:377-386 with
+// etafkt_alpha): alpha*rho*cs^2 / |e_phi/r . v + Omega|
+__device__ __forceinline__ double alpha_dynvis(const VisArgs &va, long ix, long FS, double rho, double vx, double vy,
+                                               double cs) {
+  double om = va.ephir[ix] * vx;
+  om = om + va.ephir[FS + ix] * vy;
+  om = om + va.omega;
+  return va.dynconst * rho * (cs * cs) / fabs(om);
+}
+// Divergence of a vector field in 2-D curvilinear coordinates from its face
+// averages (mesh_midpoint.f90:156-232, 586-630): arguments are the west,
+// own, east x components and the south, own, north y components; the z terms
+// of the reference's expression are zero and kept as such (they turn a -0
+// into +0)
+__device__ __forceinline__ double div_faces(const Geom &g, int i, double xw, double x0, double xe, double ys,
+                                            double y0, double yn) {
+  const double fw = 0.5 * (xw + x0), fe = 0.5 * (xe + x0), fs = 0.5 * (ys + y0), fn = 0.5 * (yn + y0);
+  return ring(g, R_DYDZDV, i) * (ring(g, R_DAXE, i) * fe - ring(g, R_DAXW, i) * fw) +
+         ring(g, R_DZDXDV, i) * (ring(g, R_DAYN, i) * fn - ring(g, R_DAYS, i) * fs) + 0.0 * (0.0 * 0.0 - 0.0 * 0.0) +
+         0.0 * 0.0 + 0.0 * 0.0 - 0.0 * 0.0 - 0.0 * 0.0;
+}
+
+// Viscous source of one RHS evaluation (sources_viscosity.f90:481-500;
+// physics_eulerisotherm.f90:982-1072, physics_euler.f90:1887-1918), FP64:
+// ExternalSources sees Timedisc%pvar after CenterBoundary and the PMIN clamp,
+// in the real frame.  Tiles of VT_X x VT_Y cells cover the whole
+// ghost-inclusive mesh, one thread per cell; LDS holds
+//   density and velocities on the tile and a two-cell halo,
+//   the dynamic viscosity and the stresses on the tile and a one-cell halo
+// (the reference forms the stresses on every cell but the outermost ghost
+// layer, and the divergence reads them one cell around each interior cell).
+// Three phases: pvar (and, with an ALPHA update, the dynamic viscosity of
+// every own cell into va.dyn), stresses, divergence into va.src.
+constexpr int VT_X = 64, VT_Y = 8;
+template <int V, int KM>
+__global__ __launch_bounds__(VT_X *VT_Y) void k_visc(Geom g, StageArgs a, VisArgs va) {
+  constexpr int SX = VT_X + 4, SY = VT_Y + 4, RX = VT_X + 2, RY = VT_Y + 2, NT = VT_X * VT_Y;
+  __shared__ double Fr[SY][SX], Fu[SY][SX], Fv[SY][SX];  // rho, vx, vy; [r + 2][c + 2]
+  __shared__ double Dy[RY][RX], Txx[RY][RX], Tyy[RY][RX], Txy[RY][RX];  // [r + 1][c + 1]
+  if (a.ctl && a.ctl->halt) return;
+  const In<double, V, KM> in(g, a, false);
+  const int tx = threadIdx.x, ty = threadIdx.y, t = tx + VT_X * ty;
+  const int i0 = blockIdx.x * VT_X, j0 = blockIdx.y * VT_Y;
+  const int imin = GH, imax = GH + g.inum - 1, jmin = GH, jmax = GH + g.jnum - 1;
+  const bool alpha = va.model == VIS_ALPHA;
+
+  // ---- phase 1: pvar of the footprint; dynamic viscosity of the inner region
+#pragma unroll
+  for (int q0 = 0; q0 < SX * SY; q0 += NT) {
+    const int q = q0 + t;
+    if (q >= SX * SY) break;
+    const int r = q / SX - 2, c = q % SX - 2, ii = i0 + c, jj = j0 + r;
+    if (ii < 0 || ii >= g.NI || jj < 0 || jj >= g.NJ) continue;
+    double p[V];
+    if (ii >= imin && ii <= imax && jj >= jmin && jj <= jmax) {
+      double s[V], ur[V], uc[V];
+      in.interior(ii, jj, s);
+      in.transform(ii, s, ur, p, uc);
+    } else {
+      in.ghost_prim(ii, jj, p);
+      if (V == 4 && a.pmin_clamp) p[3] = fmax2(p[3], g.pmin);
+    }
+    Fr[r + 2][c + 2] = p[0];
+    Fu[r + 2][c + 2] = p[1];
+    Fv[r + 2][c + 2] = p[2];
+    if (r >= -1 && r <= VT_Y && c >= -1 && c <= VT_X) {
+      const long ix = cidx(g, ii, jj);
+      double d = va.dynconst;
+      if (alpha) {
+        if (va.update) {
+          d = alpha_dynvis(va, ix, g.FS, p[0], p[1], p[2], ring(g, R_CS, ii));
+          if (r >= 0 && r < VT_Y && c >= 0 && c < VT_X) va.dyn[ix] = d;
+        } else {
+          d = va.dyn[ix];
+        }
+      }
+      Dy[r + 1][c + 1] = d;
+    }
+  }
+  __syncthreads();
+
+  // ---- phase 2: stresses (physics_eulerisotherm.f90:1017-1072, 2-D x-y) on
+  // the inner region's cells that are not in the outermost ghost layer
+#pragma unroll
+  for (int q0 = 0; q0 < RX * RY; q0 += NT) {
+    const int q = q0 + t;
+    if (q >= RX * RY) break;
+    const int r = q / RX - 1, c = q % RX - 1, ii = i0 + c, jj = j0 + r;
+    if (ii < 1 || ii > g.NI - 2 || jj < 1 || jj > g.NJ - 2) continue;
+    const int fr = r + 2, fc = c + 2;
+    const double u0 = Fu[fr][fc], v0 = Fv[fr][fc];
+    const double uw = Fu[fr][fc - 1], ue = Fu[fr][fc + 1], us = Fu[fr - 1][fc], un = Fu[fr + 1][fc];
+    const double vw = Fv[fr][fc - 1], ve = Fv[fr][fc + 1], vs = Fv[fr - 1][fc], vn = Fv[fr + 1][fc];
+    const double eta = Dy[r + 1][c + 1];
+    const double bulk = alpha ? -(2. / 3.) * eta : va.bulkconst;
+    const double dlx = ring(g, R_DLX, ii), dly = ring(g, R_DLY, ii);
+    const double cxyx = ring(g, R_CXYX, ii), cyxy = ring(g, R_CYXY, ii);
+    const double dv = bulk * div_faces(g, ii, uw, u0, ue, vs, v0, vn);
+    Txx[r + 1][c + 1] = eta * ((ue - uw) / dlx + 2.0 * cxyx * v0) + dv;
+    Tyy[r + 1][c + 1] = eta * ((vn - vs) / dly + 2.0 * cyxy * u0) + dv;
+    Txy[r + 1][c + 1] = eta * (0.5 * ((ve - vw) / dlx + (un - us) / dly) - cxyx * u0 - cyxy * v0);
+  }
+  __syncthreads();
+
+  // ---- phase 3: tensor divergence (mesh_midpoint.f90:279-335, btyx = btxy)
+  // and, for Euler, the divergence of T.v (physics_euler.f90:1887-1918)
+  const int i = i0 + tx, j = j0 + ty;
+  if (i < imin || i > imax || j < jmin || j > jmax) return;
+  const int r = ty + 1, c = tx + 1;
+  const double dydzdV = ring(g, R_DYDZDV, i), dzdxdV = ring(g, R_DZDXDV, i);
+  const double axw = ring(g, R_DAXW, i), axe = ring(g, R_DAXE, i), ays = ring(g, R_DAYS, i), ayn = ring(g, R_DAYN, i);
+  const double cxyx = ring(g, R_CXYX, i), cyxy = ring(g, R_CYXY, i);
+  const double xx0 = Txx[r][c], xy0 = Txy[r][c], yy0 = Tyy[r][c];
+  const double sx = dydzdV * (axe * (0.5 * (Txx[r][c + 1] + xx0)) - axw * (0.5 * (Txx[r][c - 1] + xx0))) +
+                    dzdxdV * (ayn * (0.5 * (Txy[r + 1][c] + xy0)) - ays * (0.5 * (Txy[r - 1][c] + xy0))) +
+                    0.0 * (0.0 * 0.0 - 0.0 * 0.0) + cxyx * xy0 + 0.0 * 0.0 - cyxy * yy0 - 0.0 * 0.0;
+  const double sy = dydzdV * (axe * (0.5 * (Txy[r][c + 1] + xy0)) - axw * (0.5 * (Txy[r][c - 1] + xy0))) +
+                    dzdxdV * (ayn * (0.5 * (Tyy[r + 1][c] + yy0)) - ays * (0.5 * (Tyy[r - 1][c] + yy0))) +
+                    0.0 * (0.0 * 0.0 - 0.0 * 0.0) + (-cxyx) * xx0 + 0.0 * 0.0 - (-cyxy) * xy0 - 0.0 * 0.0;
+  const long ix = cidx(g, i, j);
+  va.src[ix] = sx;
+  va.src[g.FS + ix] = sy;
+  if constexpr (V == 4) {
+    // (T.v)_x and (T.v)_y at the cell (dr, dc) of the inner region
+    auto tvx = [&](int dr, int dc) {
+      return Fu[dr + 1][dc + 1] * Txx[dr][dc] + Fv[dr + 1][dc + 1] * Txy[dr][dc];
+    };
+    auto tvy = [&](int dr, int dc) {
+      return Fu[dr + 1][dc + 1] * Txy[dr][dc] + Fv[dr + 1][dc + 1] * Tyy[dr][dc];
+    };
+    va.src[2 * g.FS + ix] =
+        div_faces(g, i, tvx(r, c - 1), tvx(r, c), tvx(r, c + 1), tvy(r - 1, c), tvy(r, c), tvy(r + 1, c));
+  }
+}
+
+// Viscous time-step limit (sources_viscosity.f90:507-577, the 2-D x-y
+// branch) on the stored pvar pv of the last evaluation, all cells:
+// (max(a+, -a-) + 2 nu/dl)/dl summed over the directions, nu = eta/rho, with
+// the wave speeds CalcTimestep uses (v + w when add_w).  va.update: the
+// dynamic viscosity from pv itself (UpdateViscosity at the same time 0).
+template <int V>
+__global__ __launch_bounds__(256) void k_visc_cfl(Geom g, VisArgs va, const double *__restrict__ pv, int add_w,
+                                                  double *part) {
+  __shared__ double red[4];
+  double m = -DBL_MAX;
+  const long n = (long)g.NI * g.NJ;
+  for (long k = (long)blockIdx.x * 256 + threadIdx.x; k < n; k += (long)gridDim.x * 256) {
+    const int i = (int)(k % g.NI), j = (int)(k / g.NI);
+    const long ix = cidx(g, i, j);
+    const double rho = pv[ix], u = pv[g.FS + ix];
+    double v = pv[2 * g.FS + ix];
+    if (add_w) v = v + ring(g, R_W, i);
+    const double cs = (V == 4) ? sound_speed(g.gamma, rho, pv[3 * g.FS + ix]) : ring(g, R_CS, i);
+    const double mnx = fmin2(0.0, u - cs), mxx = fmax2(0.0, u + cs);
+    const double mny = fmin2(0.0, v - cs), mxy = fmax2(0.0, v + cs);
+    double d = va.dynconst;
+    if (va.model == VIS_ALPHA) d = va.update ? alpha_dynvis(va, ix, g.FS, rho, u, v, cs) : va.dyn[ix];
+    const double kin = d / rho, dlx = ring(g, R_DLX, i), dly = ring(g, R_DLY, i);
+    m = fmax2(m, (fmax2(mxx, -mnx) + 2 * kin / dlx) / dlx + (fmax2(mxy, -mny) + 2 * kin / dly) / dly);
+  }
+  m = blk_max(m, red);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = m;
+    part[2 * blockIdx.x + 1] = 0.0;
+  }
 }
 
 // typed launchers, explicitly instantiated in fv_inst_*.hip (one translation

@@ -199,6 +199,23 @@ void launch_cfl_pvar(const Geom &g, const void *pv, int add_w, double *part, int
   }
 }
 
+void launch_visc(const Geom &g, const StageArgs &a, const VisArgs &v, int V, hipStream_t s) {
+  const dim3 grid((g.NI + VT_X - 1) / VT_X, (g.NJ + VT_Y - 1) / VT_Y), block(VT_X, VT_Y);
+  const bool small = km_for(g.stages) == KM_SMALL;
+  if (V == 3) {
+    if (small) hipLaunchKernelGGL((k_visc<3, KM_SMALL>), grid, block, 0, s, g, a, v);
+    else hipLaunchKernelGGL((k_visc<3, MAXK>), grid, block, 0, s, g, a, v);
+  } else {
+    if (small) hipLaunchKernelGGL((k_visc<4, KM_SMALL>), grid, block, 0, s, g, a, v);
+    else hipLaunchKernelGGL((k_visc<4, MAXK>), grid, block, 0, s, g, a, v);
+  }
+}
+void launch_visc_cfl(const Geom &g, const VisArgs &v, const double *pv, int add_w, double *part, int nblocks, int V,
+                     hipStream_t s) {
+  if (V == 3) hipLaunchKernelGGL((k_visc_cfl<3>), dim3(nblocks), dim3(256), 0, s, g, v, pv, add_w, part);
+  else hipLaunchKernelGGL((k_visc_cfl<4>), dim3(nblocks), dim3(256), 0, s, g, v, pv, add_w, part);
+}
+
 // ------------------------------------------------- static reduction scratch
 // The array's name is only meaningful in device code; host code uses the
 // address hipGetSymbolAddress returns for each device (each device holds its

@@ -35,6 +35,10 @@ MODULE fosite_fv_c
     !> Mesh%omega and the rotating-frame source: 0 off, 1 on, 2 disable_centaccel
     REAL(c_double) :: omega = 0
     INTEGER(c_int) :: rotframe = 0
+    !> viscosity source: 0 off, 1 MOLECULAR, 2 ALPHA, its constants and Courant number
+    INTEGER(c_int) :: vismodel = 0
+    REAL(c_double) :: dynconst = 0.1_c_double, bulkconst = -2.0_c_double/3.0_c_double*0.1_c_double
+    REAL(c_double) :: cvis = 0.5_c_double
   END TYPE fv_config
 
   TYPE, BIND(C), PUBLIC :: fv_scalars

@@ -64,6 +64,10 @@ MODULE fosite_gpu_mod
     !! (sources_rotframe.f90:73-247): 0 off, 1 on, 2 disable_centaccel
     INTEGER  :: rotframe = 0
     REAL(DP) :: omega = 0
+    !> viscosity (sources_viscosity.f90:147-165): 0 off, 1 MOLECULAR, 2 ALPHA
+    !! (isothermal), its constants and the viscous Courant number
+    INTEGER  :: vismodel = 0
+    REAL(DP) :: dynconst = 0.1_DP, bulkconst = -2.0_DP/3.0_DP*0.1_DP, cvis = 0.5_DP
   END TYPE sources_gpu
 
   TYPE, PUBLIC :: timedisc_gpu
@@ -147,6 +151,10 @@ CONTAINS
     c%starttime = this%Timedisc%starttime
     c%omega = this%Sources%omega
     c%rotframe = this%Sources%rotframe
+    c%vismodel = this%Sources%vismodel
+    c%dynconst = this%Sources%dynconst
+    c%bulkconst = this%Sources%bulkconst
+    c%cvis = this%Sources%cvis
     c%dtype = 0
   END FUNCTION Config
 

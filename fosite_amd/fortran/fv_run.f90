@@ -33,12 +33,16 @@ PROGRAM fv_run
   INTEGER  :: k_nsteps = 10, k_dumpevery = 1, k_batch = 0, k_count = 1
   REAL(DP) :: k_pmin = -1, k_rhomin = -1, k_starttime = 0, k_omega = 0
   INTEGER  :: k_rotframe = 0
+  ! viscosity: 0 off, 1 MOLECULAR, 2 ALPHA; k_bulkconst is read only with k_setbulk = 1
+  INTEGER  :: k_vismodel = 0, k_setbulk = 0
+  REAL(DP) :: k_dynconst = 0.1_DP, k_bulkconst = 0, k_cvis = 0.5_DP
   CHARACTER(LEN=512) :: k_infile = "", k_prefix = "out"
   NAMELIST /fvcfg/ k_physics, k_geometry, k_inum, k_jnum, k_gamma, k_xmin, k_xmax, k_ymin, k_ymax, &
        k_gparam, k_bcw, k_bce, k_bcs, k_bcn, k_limiter, k_recvars, k_theta, k_method, k_order,   &
        k_cfl, k_tolrel, k_tolabs, k_stoptime, k_dtlimit, k_beta, k_fargo, k_gravity, k_units,   &
        k_mass, k_outbound, k_acclimit, k_nsteps, k_dumpevery, k_batch, k_infile, k_prefix, &
-       k_count, k_pmin, k_rhomin, k_starttime, k_omega, k_rotframe
+       k_count, k_pmin, k_rhomin, k_starttime, k_omega, k_rotframe, k_vismodel, k_setbulk, &
+       k_dynconst, k_bulkconst, k_cvis
 
   TYPE(fosite_gpu) :: Sim
   CHARACTER(LEN=512) :: cfgname
@@ -75,6 +79,11 @@ PROGRAM fv_run
   Sim%Sources%acclimit = k_acclimit
   Sim%Sources%omega = k_omega
   Sim%Sources%rotframe = k_rotframe
+  Sim%Sources%vismodel = k_vismodel
+  Sim%Sources%dynconst = k_dynconst
+  ! the reference's default bulk viscosity (sources_viscosity.f90:158)
+  Sim%Sources%bulkconst = MERGE(k_bulkconst, -(2.0_DP/3.0_DP)*k_dynconst, k_setbulk.EQ.1)
+  Sim%Sources%cvis = k_cvis
   Sim%Timedisc%method = k_method
   Sim%Timedisc%order = k_order
   Sim%Timedisc%cfl = k_cfl

@@ -20,6 +20,10 @@ A configuration is a plain dict whose keys follow the reference's Dict keys
   rotframe  rotating-frame source: 0 off, 1 on, 2 with disable_centaccel
             (sources_rotframe.f90:86-93, 212-247)
   count     datafile outputs between starttime and stoptime (fileio_base.f90:420-470)
+  vismodel  viscosity source: 0 off, 1 MOLECULAR, 2 ALPHA (sources_viscosity.f90:73-577)
+  dynconst, bulkconst, cvis
+            viscosity constants and viscous Courant number; bulkconst defaults
+            to -2/3 dynconst (sources_viscosity.f90:152-165)
 
 Initial data is returned as primitive variables on all cells (ghosts
 included), the isothermal sound speeds at cell centres and faces, and the
@@ -48,13 +52,17 @@ DEFAULTS = dict(gamma=1.4, gparam=1.0, limiter=MINMOD, theta=1.0, recvars=CONSER
                 # datafile count, start time (timedisc_base.f90:295)
                 pmin=-1.0, rhomin=-1.0, count=1, starttime=0.0,
                 # rotating frame (mesh_base.f90:338, sources_rotframe.f90)
-                omega=0.0, rotframe=0)
+                omega=0.0, rotframe=0,
+                # viscosity (sources_viscosity.f90:147-165)
+                vismodel=0, dynconst=0.1, cvis=0.5)
 
 
 def full(cfg):
     """Configuration with the reference's defaults filled in."""
     out = dict(DEFAULTS)
     out.update(cfg)
+    if out.get("bulkconst") is None:  # sources_viscosity.f90:158
+        out["bulkconst"] = -(2.0 / 3.0) * out["dynconst"]
     return out
 
 

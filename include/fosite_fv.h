@@ -71,6 +71,14 @@ typedef struct fv_config {
                        of rotation at the origin (the default rotation_center, mesh_base.f90:344):
                        0 off, 1 centrifugal and Coriolis accelerations, 2 Coriolis only
                        (disable_centaccel = 1) */
+  int vismodel;     /* viscosity source (sources/sources_viscosity.f90:73-577): 0 off,
+                       1 MOLECULAR (constant dynamic and bulk viscosity), 2 ALPHA
+                       (Shakura-Sunyaev, isothermal physics; uses omega above); FP64 state,
+                       single contexts on the host-controlled step only */
+  double dynconst, bulkconst; /* dynamic viscosity (MOLECULAR) or alpha (ALPHA), default 0.1;
+                       bulk viscosity (MOLECULAR), the reference's default being -2/3 dynconst
+                       (sources_viscosity.f90:152-158) */
+  double cvis;      /* viscous Courant number, default 0.5 (sources_viscosity.f90:165) */
 } fv_config;
 
 typedef struct fv_scalars {
@@ -215,7 +223,9 @@ const char *fv_last_error(const fv_ctx *ctx);
  * 3 FARGO frame change, 4 step control of the device-controlled batches (one
  * block), 5 FARGO azimuthal remap, 6 accretion update of a device-controlled
  * step (boundary-flux accumulation and UpdateGravity, two launches; no
- * algorithmic bytes counted).  fv_kernel_stats
+ * algorithmic bytes counted), 7 viscous source of an evaluation (its
+ * algorithmic bytes: the stage inputs read and V-1 source words written).
+ * fv_kernel_stats
  * returns the number of launches, their summed device time in ms and their
  * algorithmic bytes (state words each launch must read and write, 8 bytes
  * per word, interior cells). */

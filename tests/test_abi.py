@@ -61,6 +61,9 @@ BAD = {
     "tableau": dict(method=S.SSPRK, order=4),
     "tiny mesh": dict(inum=3),
     "dtype": dict(dtype=1),
+    "vismodel": dict(vismodel=3),
+    "viscosity in fp32": dict(vismodel=1, dtype="fp32"),
+    "alpha viscosity with euler": dict(vismodel=2),
 }
 
 
