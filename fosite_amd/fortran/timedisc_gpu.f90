@@ -57,6 +57,7 @@ MODULE timedisc_gpu_mod
   USE sources_gravity_mod, ONLY : sources_gravity
   USE gravity_pointmass_mod, ONLY : gravity_pointmass
   USE sources_rotframe_mod, ONLY : sources_rotframe
+  USE sources_viscosity_mod, ONLY : sources_viscosity
   USE timedisc_base_mod
   USE timedisc_generic_mod
   USE fileio_generic_mod
@@ -133,6 +134,7 @@ CONTAINS
     CLASS(fluxes_base),  INTENT(INOUT) :: Fluxes
     TYPE(fv_config),     INTENT(OUT)   :: c
     CLASS(sources_base), POINTER       :: sp
+    LOGICAL                            :: vis_seen
     INTEGER                            :: d
     c%physics = Physics%GetType()
     SELECT TYPE(p => Physics)
@@ -180,6 +182,8 @@ CONTAINS
     c%GN = Physics%constants%GN
     c%omega = Mesh%omega
     c%rotframe = 0
+    c%vismodel = 0
+    vis_seen = .FALSE.
     ! the device evaluates one point mass at the origin and the rotating
     ! frame about the origin, in that order (gravity heads the list,
     ! sources_generic.f90:133-199): the source terms it accepts
@@ -202,9 +206,22 @@ CONTAINS
         TYPE IS (sources_rotframe)
           IF (ANY(Mesh%rotcent(:).NE.0.0)) &
             CALL this%Error("timedisc_gpu","rotating frame about the origin only")
+          ! the device adds the terms as gravity, frame, viscosity
+          IF (vis_seen) &
+            CALL this%Error("timedisc_gpu","the viscosity before the rotating frame in the source list")
           c%rotframe = MERGE(2, 1, sg%disable_centaccel)
+        TYPE IS (sources_viscosity)
+          ! MOLECULAR (1) and ALPHA (2) (sources_viscosity.f90:75-76)
+          c%vismodel = sg%viscosity%GetType()
+          IF (c%vismodel.NE.1 .AND. c%vismodel.NE.2) &
+            CALL this%Error("timedisc_gpu","viscosity models other than MOLECULAR and ALPHA")
+          c%dynconst = sg%dynconst
+          c%bulkconst = sg%bulkconst
+          c%cvis = sg%cvis
+          vis_seen = .TRUE.
         CLASS DEFAULT
-          CALL this%Error("timedisc_gpu","source terms other than point-mass gravity and the rotating frame")
+          CALL this%Error("timedisc_gpu", &
+            "source terms other than point-mass gravity, the rotating frame and viscosity")
         END SELECT
         sp => sp%next
       END DO

@@ -38,6 +38,10 @@ def oracle_states(cfg, steps):
     # the rotating frame after the point mass (sources_rotframe.f90)
     ("disk_rotframe_run", dict(S.kepler_disk_config(ni=48, nj=64), omega=0.3, rotframe=1,
                                bc=[S.NO_GRADIENTS, S.NO_GRADIENTS, S.PERIODIC, S.PERIODIC]), 30, 10, 0.0),
+    # the viscosity source (sources_viscosity.f90): ALPHA after the frame, MOLECULAR on a shear box
+    ("disk_alpha_run", dict(S.kepler_disk_config(ni=48, nj=64), omega=0.3, rotframe=1, vismodel=2, dynconst=0.02,
+                            bc=[S.NO_GRADIENTS, S.NO_GRADIENTS, S.PERIODIC, S.PERIODIC]), 30, 10, 0.0),
+    ("shear_viscous_step", S.shear_box_config(n=32, vismodel=1, dynconst=1e-3), 20, 0, 0.0),
 ])
 def test_fortran_host_matches_oracle(tmp_path, name, cfg, nsteps, batch, tol):
     print("fortran host case", name, flush=True)

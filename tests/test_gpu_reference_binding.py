@@ -36,6 +36,12 @@ CASES = {
     # Euler in a rotating frame with the Coriolis terms only (disable_centaccel)
     "riemann rotframe nocent": (dict(S.riemann2d_config(n=32), stoptime=0.03, count=2, omega=2.0, rotframe=2),
                                 30, 0.0),
+    # ALPHA viscosity after the rotating frame and the point mass, FARGO type 1, 3 outputs
+    "disk alpha rotframe fargo1": (dict(S.kepler_disk_config(ni=40, nj=64), bc=NG, fargo=1, omega=0.3, rotframe=1,
+                                        vismodel=2, dynconst=0.01, stoptime=0.8, count=3), 30, 0.0),
+    # MOLECULAR viscosity on the Euler shear box with FARGO along y, 3 outputs
+    "euler shear box viscous": (dict(S.shear_box_config(n=32), stoptime=0.09, count=3, vismodel=1, dynconst=1e-3),
+                                40, 0.0),
 }
 
 
