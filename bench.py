@@ -384,6 +384,38 @@ def fargo1_variant(steps=300):
     return out
 
 
+def viscous_variant(steps=60, alpha=0.01, omega=0.3):
+    """The headline disk in a rotating frame with ALPHA viscosity:
+    host-controlled steps (the viscous source and its time-step limit run on
+    the host-driven path), ms per step, and the viscous-source kernel's mean
+    launch time and HBM rate (its algorithmic bytes: the stage inputs read and
+    the source written) from the library's HIP-event profiling."""
+    import fosite_amd
+    from fosite_amd import setups
+    cfg = dict(setups.kepler_disk_config(ni=512, nj=1024), vismodel=2, dynconst=alpha, omega=omega, rotframe=1)
+    out = {"workload": "isothermal Keplerian disk, logcylindrical 512x1024, FP64, SSPRK3, FARGO; rotating frame "
+                       "(omega %g), ALPHA viscosity (alpha %g), host-controlled steps" % (omega, alpha)}
+    init = setups.initial(cfg)
+    sim = fosite_amd.Fosite(cfg)
+    sim.set_initial(init["pvar"], init["bccsound"], init["fcsound"], init["w"])
+    sim.first_step()
+    sim.run(5)
+    taken = sim.run(steps)
+    out["ms_per_step"] = sim.last_run_ms() / max(taken, 1)
+    out["dtcause"] = sim.scalars()["dtcause"]
+    sim.profile(True)
+    sim.run(10)
+    for k in ("visc", "rhs", "rhs_combine", "rhs_end"):
+        n, ms, alg = sim.kernel_stats(k)
+        out[k + "_us"] = 1e3 * ms / max(n, 1)
+        if k == "visc" and ms > 0:
+            out["visc_GBps"] = alg / (ms * 1e-3) / 1e9
+            out["visc_hbm_frac"] = out["visc_GBps"] / 8000.0
+    sim.profile(False)
+    sim.close()
+    return out
+
+
 def serving_form_full(n=40):
     """The serving stream at the stated grid sizes, radial cells U[64,1024]
     (azimuthal 2x), 20 requests/s: the offered load (all cell updates of the
@@ -412,7 +444,7 @@ def main():
                     help="skip the precision, batched and serving forms")
     ap.add_argument("--form-steps", type=int, default=500)
     ap.add_argument("--only-form", choices=["precision", "batched", "serving", "serving_full", "accretion",
-                                             "fargo1"],
+                                             "fargo1", "viscous"],
                     help="run one form alone and print its object (for profiling)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -422,7 +454,8 @@ def main():
     if a.only_form:
         f = {"precision": lambda: precision_form(1024, 2048, a.form_steps), "batched": batched_form,
              "serving": serving_form, "serving_full": serving_form_full,
-             "accretion": accretion_variant, "fargo1": fargo1_variant}[a.only_form]
+             "accretion": accretion_variant, "fargo1": fargo1_variant,
+             "viscous": viscous_variant}[a.only_form]
         print(json.dumps(f()))
         return 0
 
@@ -504,6 +537,7 @@ def main():
         out["serving_form_full_sizes"] = serving_form_full()
         out["accretion_variant"] = accretion_variant()
         out["fargo1_variant"] = fargo1_variant()
+        out["viscous_variant"] = viscous_variant()
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg, init, a.cpu_steps, throughput=not a.no_cpu_throughput)
     print(json.dumps(out))

@@ -1557,8 +1557,11 @@ __device__ __forceinline__ double div_faces(const Geom &g, int i, double xw, dou
 // Three phases: pvar (and, with an ALPHA update, the dynamic viscosity of
 // every own cell into va.dyn), stresses, divergence into va.src.
 constexpr int VT_X = 64, VT_Y = 8;
+#ifndef FV_VISC_WAVES_PER_SIMD
+#define FV_VISC_WAVES_PER_SIMD 1
+#endif
 template <int V, int KM>
-__global__ __launch_bounds__(VT_X *VT_Y) void k_visc(Geom g, StageArgs a, VisArgs va) {
+__global__ __launch_bounds__(VT_X *VT_Y, FV_VISC_WAVES_PER_SIMD) void k_visc(Geom g, StageArgs a, VisArgs va) {
   constexpr int SX = VT_X + 4, SY = VT_Y + 4, RX = VT_X + 2, RY = VT_Y + 2, NT = VT_X * VT_Y;
   __shared__ double Fr[SY][SX], Fu[SY][SX], Fv[SY][SX];  // rho, vx, vy; [r + 2][c + 2]
   __shared__ double Dy[RY][RX], Txx[RY][RX], Tyy[RY][RX], Txy[RY][RX];  // [r + 1][c + 1]
@@ -1569,18 +1572,39 @@ __global__ __launch_bounds__(VT_X *VT_Y) void k_visc(Geom g, StageArgs a, VisArg
   const int imin = GH, imax = GH + g.inum - 1, jmin = GH, jmax = GH + g.jnum - 1;
   const bool alpha = va.model == VIS_ALPHA;
 
-  // ---- phase 1: pvar of the footprint; dynamic viscosity of the inner region
+  // ---- phase 1: pvar of the footprint; dynamic viscosity of the inner
+  // region.  At most two footprint cells per thread: every load of both
+  // (state, stage slopes, stored viscosity) is issued before any arithmetic.
+  static_assert(SX * SY <= 2 * NT, "two footprint cells per thread");
+  int fr[2], fc[2], fi[2], fj[2];
+  bool inm[2], inr[2], inreg[2];
 #pragma unroll
-  for (int q0 = 0; q0 < SX * SY; q0 += NT) {
-    const int q = q0 + t;
-    if (q >= SX * SY) break;
-    const int r = q / SX - 2, c = q % SX - 2, ii = i0 + c, jj = j0 + r;
-    if (ii < 0 || ii >= g.NI || jj < 0 || jj >= g.NJ) continue;
+  for (int h = 0; h < 2; h++) {
+    const int q = t + h * NT;
+    fr[h] = q / SX - 2;
+    fc[h] = q % SX - 2;
+    fi[h] = i0 + fc[h];
+    fj[h] = j0 + fr[h];
+    inm[h] = q < SX * SY && fi[h] >= 0 && fi[h] < g.NI && fj[h] >= 0 && fj[h] < g.NJ;
+    inr[h] = inm[h] && fi[h] >= imin && fi[h] <= imax && fj[h] >= jmin && fj[h] <= jmax;
+    inreg[h] = inm[h] && fr[h] >= -1 && fr[h] <= VT_Y && fc[h] >= -1 && fc[h] <= VT_X;
+  }
+  double dst[2] = {va.dynconst, va.dynconst};
+  if (alpha && !va.update) {
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+      if (inreg[h]) dst[h] = va.dyn[cidx(g, fi[h], fj[h])];
+  }
+  double s[2][V];
+  in.interior2(inr[0], fi[0], fj[0], s[0], inr[1], fi[1], fj[1], s[1]);
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    if (!inm[h]) continue;
+    const int r = fr[h], c = fc[h], ii = fi[h], jj = fj[h];
     double p[V];
-    if (ii >= imin && ii <= imax && jj >= jmin && jj <= jmax) {
-      double s[V], ur[V], uc[V];
-      in.interior(ii, jj, s);
-      in.transform(ii, s, ur, p, uc);
+    if (inr[h]) {
+      double ur[V], uc[V];
+      in.transform(ii, s[h], ur, p, uc);
     } else {
       in.ghost_prim(ii, jj, p);
       if (V == 4 && a.pmin_clamp) p[3] = fmax2(p[3], g.pmin);
@@ -1588,16 +1612,12 @@ __global__ __launch_bounds__(VT_X *VT_Y) void k_visc(Geom g, StageArgs a, VisArg
     Fr[r + 2][c + 2] = p[0];
     Fu[r + 2][c + 2] = p[1];
     Fv[r + 2][c + 2] = p[2];
-    if (r >= -1 && r <= VT_Y && c >= -1 && c <= VT_X) {
-      const long ix = cidx(g, ii, jj);
-      double d = va.dynconst;
-      if (alpha) {
-        if (va.update) {
-          d = alpha_dynvis(va, ix, g.FS, p[0], p[1], p[2], ring(g, R_CS, ii));
-          if (r >= 0 && r < VT_Y && c >= 0 && c < VT_X) va.dyn[ix] = d;
-        } else {
-          d = va.dyn[ix];
-        }
+    if (inreg[h]) {
+      double d = dst[h];
+      if (alpha && va.update) {
+        const long ix = cidx(g, ii, jj);
+        d = alpha_dynvis(va, ix, g.FS, p[0], p[1], p[2], ring(g, R_CS, ii));
+        if (r >= 0 && r < VT_Y && c >= 0 && c < VT_X) va.dyn[ix] = d;
       }
       Dy[r + 1][c + 1] = d;
     }
