@@ -178,6 +178,7 @@ CVAR_NAMES = {setups.EULER_ISOTHERM: ("density", "xmomentum", "ymomentum"),
 VERSION = "0.8.3"  # CMakeLists.txt:18
 BINARY, LINEAR, KT, GRAVITY, POINTMASS, NEWTON = 1, 2, 1, 1, 1, 1
 ROTATING_FRAME = 20  # sources_generic.f90:77
+VISCOSITY = 3  # sources_generic.f90:74
 SI, GEOMETRICAL = 1, 3  # physics units (physics_base.f90:554)
 CHECK_ALL = -1
 
@@ -212,7 +213,8 @@ def config_entries(cfg, filename=""):
       timedisc   timedisc_base.f90:295-359 (starttime .. rhomin),
                  SetOutput 534-646 (output/*), timedisc_rkfehlberg.f90:121
       datafile   fileio_base.f90:420-470
-      sources    gravity_pointmass.f90, sources_gravity.f90, sources_rotframe.f90
+      sources    gravity_pointmass.f90, sources_gravity.f90, sources_rotframe.f90,
+                 sources_viscosity.f90
       version    fosite.f90:250
     The values are the ones fosite 0.8.3 writes (checked key by key against
     datasets the reference wrote)."""
@@ -301,6 +303,13 @@ def config_entries(cfg, filename=""):
         grav = [("stype", GRAVITY)] + [("pmass/" + k, v) for k, v in pm]
         grav += [("update_disk_height", 0), ("energy", 1), ("output/height", 0), ("output/accel", 0)]
         src += [("grav/" + k, v) for k, v in grav]
+    if cfg.get("vismodel", 0):  # sources_viscosity.f90:147-165, 271-289
+        bulk = cfg.get("bulkconst")
+        bulk = -(2.0 / 3.0) * float(cfg.get("dynconst", 0.1)) if bulk is None else bulk
+        src += [("vis/stype", VISCOSITY), ("vis/vismodel", int(cfg["vismodel"])),
+                ("vis/dynconst", float(cfg.get("dynconst", 0.1))), ("vis/bulkconst", float(bulk)),
+                ("vis/cvis", float(cfg.get("cvis", 0.5))), ("vis/exponent", 0.0)]
+        src += [("vis/output/" + k, 0) for k in ("stress", "dynvis", "kinvis", "bulkvis")]
     if cfg.get("rotframe", 0):  # sources_rotframe.f90:86-89
         src += [("rotframe/stype", ROTATING_FRAME),
                 ("rotframe/disable_centaccel", 1 if cfg["rotframe"] == 2 else 0)]
@@ -356,6 +365,11 @@ def load_restart(path):
     cfg["omega"] = float(g("mesh/omega", 0.0))
     if _val(d, "/config/sources/rotframe/stype") is not None:
         cfg["rotframe"] = 2 if int(g("sources/rotframe/disable_centaccel", 0)) else 1
+    if _val(d, "/config/sources/vis/stype") is not None:
+        cfg.update(vismodel=int(g("sources/vis/vismodel")), dynconst=float(g("sources/vis/dynconst", 0.1)),
+                   cvis=float(g("sources/vis/cvis", 0.5)))
+        if _val(d, "/config/sources/vis/bulkconst") is not None:
+            cfg["bulkconst"] = float(g("sources/vis/bulkconst"))
     time = float(_val(d, "/timedisc/time"))
     cfg["starttime"] = time
     V = 4 if physics == setups.EULER else 3

@@ -25,4 +25,11 @@ CASES = {
     # Euler Riemann problem in a rotating frame, Coriolis terms only
     # (disable_centaccel), no gravity
     "riemann_rotframe16": (dict(S.riemann2d_config(n=16), stoptime=0.02, omega=2.0, rotframe=2), 8),
+    # isothermal disk with ALPHA viscosity in a rotating frame, the point
+    # mass and FARGO type 2
+    "disk_alpha16x32": (dict(S.kepler_disk_config(ni=16, nj=32), stoptime=0.05, omega=0.3, rotframe=1, vismodel=2,
+                             dynconst=0.01), 3),
+    # Euler Riemann problem with MOLECULAR viscosity and a given bulk viscosity
+    "riemann_viscous16": (dict(S.riemann2d_config(n=16), stoptime=0.02, vismodel=1, dynconst=1e-3, bulkconst=5e-4),
+                          8),
 }

@@ -28,6 +28,9 @@ CASES = {
                              stoptime=0.5, count=2), 1),
     # an Euler Riemann problem in a rotating frame (Coriolis terms only); restarted from the 2nd of 3
     "riemann_rotframe": (dict(S.riemann2d_config(n=24), stoptime=0.06, count=3, omega=2.0, rotframe=2), 2),
+    # an ALPHA-viscous disk in a rotating frame; restarted from the 2nd of 4
+    "disk_alpha_rotframe": (dict(S.kepler_disk_config(ni=24, nj=40), bc=NG, stoptime=0.8, count=4, omega=0.3,
+                                 rotframe=1, vismodel=2, dynconst=0.01), 2),
 }
 
 
@@ -67,9 +70,12 @@ def test_load_restart_reads_back_configuration_time_and_data(name):
     full = S.full(cfg)
     assert r["time"] == time and r["cfg"]["starttime"] == time and r["step"] == k
     for key in ("physics", "geometry", "inum", "jnum", "xmin", "xmax", "ymin", "ymax", "gparam", "limiter",
-                "recvars", "method", "order", "cfl", "stoptime", "tol_rel", "beta", "count", "fargo"):
+                "recvars", "method", "order", "cfl", "stoptime", "tol_rel", "beta", "count", "fargo", "omega",
+                "rotframe", "vismodel"):
         assert r["cfg"][key] == full[key], key
     assert list(r["cfg"]["bc"]) == list(full["bc"])
+    if full["vismodel"]:
+        assert all(r["cfg"][key] == full[key] for key in ("dynconst", "bulkconst", "cvis"))
     V = pv.shape[0]
     assert np.array_equal(np.asarray(r["cfg"]["tol_abs"][:V]), np.asarray(full["tol_abs"][:V]))
     I = (slice(None), slice(2, -2), slice(2, -2))
