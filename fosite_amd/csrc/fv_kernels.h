@@ -789,12 +789,11 @@ inline int xcd_grid(int nbx, int nby) { return ((nbx * nby + XCDS - 1) / XCDS) *
 // Output-aligned tiles keep every lane of the flux phases busy and divide
 // power-of-two meshes exactly.
 //
-// Occupancy: a CU has 160 KB of LDS.  The FP64 isothermal stage tiles
-// (64 x 8, V = 3) need 54.4 KB, so three tiles share a CU -- 6 waves per
-// SIMD -- when their VGPRs stay within 80, which the three-slot stage and
-// last-stage kernels do without spilling (the seven-slot ones would spill);
-// the ghost-cell CFL and flag arrays only exist in the end-of-step mode,
-// which keeps two tiles per CU.
+// Occupancy: a CU has 160 KB of LDS, allocated in 512-byte granules.  The
+// FP64 isothermal stage tiles (64 x 8, V = 3) take 54444 bytes, 107
+// granules, so two tiles share a CU (4 waves per SIMD); the launch bounds
+// compute that from the layout.  The ghost-cell CFL and flag arrays only
+// exist in the end-of-step mode.
 constexpr int TY_RHS = TY_TILE;
 // launch modes of the RHS kernel
 enum RhsMode { M_STAGE = 0, M_LAST = 1, M_END = 2 };
