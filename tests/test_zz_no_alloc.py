@@ -70,3 +70,22 @@ def test_noalloc_step_refuses_meshes_beyond_the_scratch():
     assert rc != 0
     assert b"static scratch" in o.L.fv_last_error(o._h)
     o.close()
+
+
+@pytest.mark.gpu
+def test_noalloc_step_with_the_viscosity_source():
+    """The viscous time-step limit reduces through the same static scratch:
+    an ALPHA-viscous disk, then a smaller MOLECULAR-viscous shear box, each
+    equal to fv_step bit for bit."""
+    print("fv_step_noalloc: viscous disk 64x96, then viscous shear box 40x40", flush=True)
+    disk = dict(S.kepler_disk_config(ni=64, nj=96), vismodel=2, dynconst=0.01, omega=0.3, rotframe=1)
+    box = S.shear_box_config(n=40, vismodel=1, dynconst=1e-3)
+    big, big_ref = make(disk), make(disk)
+    small, small_ref = make(box), make(box)
+    for n in range(5):
+        assert noalloc_step(big) == big_ref.step()
+        assert noalloc_step(small) == small_ref.step()
+        same(big, big_ref, ("viscous disk", n))
+        same(small, small_ref, ("viscous box", n))
+    for o in (big, big_ref, small, small_ref):
+        o.close()
