@@ -386,15 +386,16 @@ def fargo1_variant(steps=300):
 
 def viscous_variant(steps=60, alpha=0.01, omega=0.3):
     """The headline disk in a rotating frame with ALPHA viscosity:
-    host-controlled steps (the viscous source and its time-step limit run on
-    the host-driven path), ms per step, and the viscous-source kernel's mean
-    launch time and HBM rate (its algorithmic bytes: the stage inputs read and
-    the source written) from the library's HIP-event profiling."""
+    device-controlled steps (the viscous source before every RHS evaluation,
+    its time-step limit reduced in the step control), ms per step, and the
+    viscous-source kernel's mean launch time and HBM rate (its algorithmic
+    bytes: the stage inputs read and the source written) from the library's
+    HIP-event profiling."""
     import fosite_amd
     from fosite_amd import setups
     cfg = dict(setups.kepler_disk_config(ni=512, nj=1024), vismodel=2, dynconst=alpha, omega=omega, rotframe=1)
     out = {"workload": "isothermal Keplerian disk, logcylindrical 512x1024, FP64, SSPRK3, FARGO; rotating frame "
-                       "(omega %g), ALPHA viscosity (alpha %g), host-controlled steps" % (omega, alpha)}
+                       "(omega %g), ALPHA viscosity (alpha %g), device-controlled steps" % (omega, alpha)}
     init = setups.initial(cfg)
     sim = fosite_amd.Fosite(cfg)
     sim.set_initial(init["pvar"], init["bccsound"], init["fcsound"], init["w"])

@@ -18,6 +18,7 @@
 #include <mutex>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <new>
 #include <string>
 #include <vector>
@@ -164,6 +165,9 @@ struct fv_ctx {
   // the time of that update (sources_viscosity.f90:371, -1 before the first)
   double *d_vsrc = nullptr, *d_dyn = nullptr, *d_ephir = nullptr;
   double vis_time = -1.0;
+  // device-controlled steps: per-workgroup viscous 1/dt of the end-of-step k_visc
+  double *d_vpart = nullptr;
+  int nb_vis = 0;
   Ctl *d_ctl = nullptr, *h_ctl = nullptr;
   struct GraphEntry { int x0, x1, t, k0, b0, g0, w0, v0, nsteps; hipGraphExec_t exec; };
   std::vector<GraphEntry> graphs;
@@ -818,17 +822,37 @@ void fast_step_args(fv_ctx *ctx, Ctl *ctl, int xin, int xout, int tb, int rb, in
   f.end = a;
 }
 
+// the viscous source of a device-controlled evaluation at ctl->time +
+// c_cur*ctl->dt, the previous evaluation being at ctl->time + c_prev*ctl->dt
+// (part: the end-of-step one's viscous 1/dt partials)
+void visc_device(fv_ctx *ctx, const Geom &g, StageArgs &a, double c_cur, double c_prev, double *part) {
+  VisArgs v = vis_args(ctx, 0);
+  v.c_cur = c_cur;
+  v.c_prev = c_prev;
+  v.part = part;
+  long pk = prof_begin(ctx);
+  launch_visc(g, a, v, ctx->V, ctx->st);
+  prof_end(ctx, pk, K_VISC, 8.0 * cells(ctx) * (ctx->V * (a.nk + 1.0) + ctx->V - 1.0));
+  a.vsrc = ctx->d_vsrc;
+}
+
 void enqueue_fast_step(fv_ctx *ctx, int xin, int xout, int tb, int rb, int k0in, int k0out, int bin, int gin,
                        int win, int vin) {
   FastStep f;
   fast_step_args(ctx, ctx->d_ctl, xin, xout, tb, rb, k0in, k0out, bin, gin, win, vin, f);
+  const bool vis = ctx->c.vismodel != 0;
   if (f.cbv) {
     long pk = prof_begin(ctx);
     launch_wsum(f.gold, f.vin, f.wring, f.wringf, ctx->st, ctx->d_ctl);
     launch_frame(f.gold, f.fsrc, f.fdst, +1, ctx->V, ctx->st, ctx->d_ctl, f.wnew);
     prof_end(ctx, pk, K_FRAME, 2.0 * word_bytes(ctx));
   }
-  for (int m = 1; m < ctx->tab.m; m++) launch_stage(ctx, f.g, f.st[m - 1], m);
+  for (int m = 1; m < ctx->tab.m; m++) {
+    // stage m runs at time + c(m)*dt; stage 1 follows the end-of-step
+    // evaluation of the previous step, at time
+    if (vis) visc_device(ctx, f.g, f.st[m - 1], ctx->tab.c[m], m > 1 ? ctx->tab.c[m - 1] : 0.0, nullptr);
+    launch_stage(ctx, f.g, f.st[m - 1], m);
+  }
   if (f.remap) {
     long pk = prof_begin(ctx);
     launch_remap(f.g, f.rm, ctx->V, ctx->st);
@@ -840,11 +864,14 @@ void enqueue_fast_step(fv_ctx *ctx, int xin, int xout, int tb, int rb, int k0in,
     launch_accrete(f.acc, ctx->st);
     prof_end(ctx, pk, K_ACCRETE, 0.0);
   }
+  // the end of the step at time + dt (as time + 1*dt)
+  if (vis) visc_device(ctx, f.g, f.end, 1.0, ctx->tab.c[ctx->tab.m - 1], ctx->d_vpart);
   long pk = prof_begin(ctx);
   launch_rhs(f.g, f.end, ctx->V, ctx->st);
   prof_end(ctx, pk, K_RHS_END, 3.0 * word_bytes(ctx));
   pk = prof_begin(ctx);
-  launch_control(ctx->d_ctl, ctx->part_c, ctx->part_r, ctx->nb_rhs, ctx->st);
+  launch_control(ctx->d_ctl, ctx->part_c, ctx->part_r, ctx->nb_rhs, ctx->st, vis ? ctx->d_vpart : nullptr,
+                 ctx->nb_vis);
   prof_end(ctx, pk, K_CONTROL, 32.0 * ctx->nb_rhs);
 }
 
@@ -869,6 +896,8 @@ void prime_ctl(const fv_ctx *ctx, Ctl *h, int nmax) {
   h->beta = ctx->c.beta;
   h->order = ctx->c.order;
   h->nsteps = nmax;
+  h->vis = ctx->c.vismodel ? 1 : 0;
+  h->cvis = ctx->c.cvis;
 }
 
 // replays the host scalars of the h->step steps the device completed, exactly
@@ -900,7 +929,15 @@ int replay_device_steps(fv_ctx *ctx, const Ctl *h, int x0, int x1, int *halted) 
   if (k > 0) {  // CalcTimestep of the last accepted step (before output clipping)
     ctx->invdt = h->invdt;
     ctx->dt = calc_timestep(ctx);
+    if (ctx->c.vismodel) {  // and its viscous limit, as timestep() forms it
+      const double dtv = h->invdt_v > DBL_MIN ? ctx->c.cvis / h->invdt_v : DBL_MAX;
+      if (dtv < ctx->dt) ctx->dtcause = DTCAUSE_VISCOSITY;
+      ctx->dt = dtv < ctx->dt ? dtv : ctx->dt;
+    }
   }
+  // the device steps (a halted one included) updated the stored viscosity on
+  // their own schedule: the next host evaluation recomputes it
+  ctx->vis_time = std::numeric_limits<double>::quiet_NaN();
   *halted = h->reason != H_BATCH;
   ctx->batch_steps += k;
   ctx->batch_halts += *halted;
@@ -1061,8 +1098,6 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
       ctx->col_h[(size_t)C_CACY * ctx->NJ + j] = (om * ctx->mesh.a[M_POSY][(size_t)j * ctx->NI + GH]) * om;
   }
   if (const char *e = std::getenv("FV_HOST_CONTROL")) ctx->fast = std::atoi(e) == 0;
-  // the device-controlled steps carry no viscous source or time-step limit
-  if (c.vismodel) ctx->fast = false;
   // device resources
   const size_t sbytes = elem_bytes(ctx) * ctx->V * ctx->FS;
   auto dalloc = [&](double **p, size_t b) { return hipMalloc((void **)p, b) == hipSuccess; };
@@ -1124,7 +1159,11 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
     ok = dalloc(&ctx->grav[q], sizeof(double) * (PM_HDR + 2 * ctx->NI));
     if (ok && c.dtype == 1) ok = hipMalloc((void **)&ctx->gravf[q], sizeof(float) * 2 * ctx->NI) == hipSuccess;
   }
-  if (c.vismodel) ok = ok && dalloc(&ctx->d_vsrc, sizeof(double) * (ctx->V - 1) * ctx->FS);
+  if (c.vismodel) {
+    ctx->nb_vis = visc_blocks(g);
+    ok = ok && dalloc(&ctx->d_vsrc, sizeof(double) * (ctx->V - 1) * ctx->FS) &&
+         dalloc(&ctx->d_vpart, sizeof(double) * 2 * ctx->nb_vis);
+  }
   if (c.vismodel == VIS_ALPHA)
     ok = ok && dalloc(&ctx->d_dyn, sizeof(double) * ctx->FS) && dalloc(&ctx->d_ephir, sizeof(double) * 2 * ctx->FS);
   if (!ok) {
@@ -1216,6 +1255,7 @@ extern "C" int fv_destroy(fv_ctx *ctx) {
   for (auto &b : ctx->vbufs) if (b) (void)hipFree(b);
   if (ctx->pbuf) (void)hipFree(ctx->pbuf);
   if (ctx->d_vsrc) (void)hipFree(ctx->d_vsrc);
+  if (ctx->d_vpart) (void)hipFree(ctx->d_vpart);
   if (ctx->d_dyn) (void)hipFree(ctx->d_dyn);
   if (ctx->d_ephir) (void)hipFree(ctx->d_ephir);
   if (ctx->d_ring1) (void)hipFree(ctx->d_ring1);
@@ -1253,7 +1293,8 @@ extern "C" int fv_device_bytes(const fv_config *c, double *bytes) {
   if (c->fargo == 1)                                    // FARGO type 1: velocities, second ring arrays
     b += 2.0 * 8.0 * NJ * P + (8.0 + (c->dtype == 1 ? 4.0 : 0.0)) * R_COUNT * NI;
   if (c->outbound) b += 2.0 * (8.0 * (PM_HDR + 2.0 * NI) + (c->dtype == 1 ? 4.0 * 2.0 * NI : 0.0));
-  if (c->vismodel) b += 8.0 * (V - 1.0) * NJ * P;       // viscous source
+  if (c->vismodel)                                      // viscous source, its step partials
+    b += 8.0 * (V - 1.0) * NJ * P + 16.0 * std::ceil(NI / 64.0) * std::ceil(NJ / 8.0);
   if (c->vismodel == 2) b += 3.0 * 8.0 * NJ * P;        // ALPHA: dynamic viscosity, e_phi/r
   b += 2.0 * 2.0 * 8.0 * nb32;                         // CFL and error partials
   b += 8.0 * (R_COUNT * NI + C_COUNT * NJ);             // metric arrays

@@ -60,7 +60,8 @@ enum HaltReason { H_NONE = 0, H_ERR = 1, H_DTLIMIT = 2, H_FLAGS = 3, H_OUTPUT = 
 struct Ctl {
   double time, dt, dt_step, t_out, stoptime, cfl, dtlimit, maxerrold, beta, order;
   double invdt;
-  int halt, reason, step, nsteps, flags, pad;
+  double cvis, invdt_v;  // viscosity (vis = 1): its Courant number and the last step's viscous 1/dt
+  int halt, reason, step, nsteps, flags, vis;
   double err_hist[CTL_HIST], dt_hist[CTL_HIST];
 };
 
@@ -162,6 +163,14 @@ struct VisArgs {
   const double *ephir;  // ALPHA: e_phi/r in curvilinear components, [2][NJ][P]
   double *dyn;          // ALPHA: dynamic viscosity of the last update, all cells, [NJ][P]
   double *src;          // k_visc output: StageArgs::vsrc
+  // device-controlled steps (StageArgs::ctl set): the evaluation's time is
+  // ctl->time + c_cur*ctl->dt and the previous one's ctl->time + c_prev*ctl->dt,
+  // and UpdateViscosity's time test is decided there (update is unused)
+  double c_cur, c_prev;
+  // end-of-step evaluation of a device-controlled step: per-workgroup maxima
+  // of the viscous 1/dt (as k_visc_cfl on the pvar the evaluation leaves),
+  // [2*b], or null
+  double *part;
 };
 
 // FargoAdvectionY on the combined state (timedisc_base.f90:1935-2069):
@@ -223,7 +232,11 @@ void launch_visc(const Geom &g, const StageArgs &a, const VisArgs &v, int V, hip
 // (v + w when add_w); v.update: the dynamic viscosity from that pvar
 void launch_visc_cfl(const Geom &g, const VisArgs &v, const double *pv, int add_w, double *part, int nblocks, int V,
                      hipStream_t s);
-void launch_control(Ctl *ctl, const double *part_c, const double *part_r, int nb, hipStream_t s);
+// part_v/nbv: the viscous 1/dt partials of the end-of-step k_visc (or null)
+void launch_control(Ctl *ctl, const double *part_c, const double *part_r, int nb, hipStream_t s,
+                    const double *part_v = nullptr, int nbv = 0);
+// k_visc workgroups of a mesh (tiles of 64 x 8 cells over the ghost-inclusive mesh)
+int visc_blocks(const Geom &g);
 void launch_bflux(const BfluxArgs &bf, hipStream_t s);
 void launch_accrete(const AccreteArgs &p, hipStream_t s);
 // FARGO type 1: w(i) = SUM_j v(i,j)*wfactor(i) / JNUM per ring into the ring

@@ -629,11 +629,16 @@ static __device__ void control_accept(Ctl *c, double err) {
 // after the end-of-step RHS: data checks, output test, next CFL step,
 // stop-time test and output-time clipping (fosite.f90:383-462,
 // io/fileio_base.f90:604-620)
-static __device__ void control_next(Ctl *c, double invdt, int flags) {
+static __device__ void control_next(Ctl *c, double invdt, int flags, double invdt_v) {
   if (flags) return halt(c, H_FLAGS);
   const double t = c->time;
   if (fabs(c->t_out - t) <= 1.0E-5 * c->t_out) return halt(c, H_OUTPUT);
   double dt = c->cfl / invdt;
+  if (c->vis) {  // the viscous limit (sources_viscosity.f90:507-577)
+    const double dtv = invdt_v > DBL_MIN ? c->cvis / invdt_v : DBL_MAX;
+    dt = dtv < dt ? dtv : dt;
+    c->invdt_v = invdt_v;
+  }
   if (fabs(c->stoptime - t) <= 1.0E-05 * c->stoptime) return halt(c, H_STOP);
   if ((t + dt) / c->t_out > 1.0)
     dt = c->t_out - t;
@@ -1544,6 +1549,49 @@ __device__ __forceinline__ double div_faces(const Geom &g, int i, double xw, dou
          0.0 * 0.0 + 0.0 * 0.0 - 0.0 * 0.0 - 0.0 * 0.0;
 }
 
+// the divergence part of k_visc for the interior cell (i, j) at (r, c) of
+// the inner region
+template <int V, int SX, int RX>
+__device__ __forceinline__ void visc_divergence(const Geom &g, const VisArgs &va, int i, int j, int r, int c,
+                                                double (*Fu)[SX], double (*Fv)[SX], double (*Txx)[RX],
+                                                double (*Tyy)[RX], double (*Txy)[RX]) {
+  const double dydzdV = ring(g, R_DYDZDV, i), dzdxdV = ring(g, R_DZDXDV, i);
+  const double axw = ring(g, R_DAXW, i), axe = ring(g, R_DAXE, i), ays = ring(g, R_DAYS, i), ayn = ring(g, R_DAYN, i);
+  const double cxyx = ring(g, R_CXYX, i), cyxy = ring(g, R_CYXY, i);
+  const double xx0 = Txx[r][c], xy0 = Txy[r][c], yy0 = Tyy[r][c];
+  const double sx = dydzdV * (axe * (0.5 * (Txx[r][c + 1] + xx0)) - axw * (0.5 * (Txx[r][c - 1] + xx0))) +
+                    dzdxdV * (ayn * (0.5 * (Txy[r + 1][c] + xy0)) - ays * (0.5 * (Txy[r - 1][c] + xy0))) +
+                    0.0 * (0.0 * 0.0 - 0.0 * 0.0) + cxyx * xy0 + 0.0 * 0.0 - cyxy * yy0 - 0.0 * 0.0;
+  const double sy = dydzdV * (axe * (0.5 * (Txy[r][c + 1] + xy0)) - axw * (0.5 * (Txy[r][c - 1] + xy0))) +
+                    dzdxdV * (ayn * (0.5 * (Tyy[r + 1][c] + yy0)) - ays * (0.5 * (Tyy[r - 1][c] + yy0))) +
+                    0.0 * (0.0 * 0.0 - 0.0 * 0.0) + (-cxyx) * xx0 + 0.0 * 0.0 - (-cyxy) * xy0 - 0.0 * 0.0;
+  const long ix = cidx(g, i, j);
+  va.src[ix] = sx;
+  va.src[g.FS + ix] = sy;
+  if constexpr (V == 4) {
+    // (T.v)_x and (T.v)_y at the cell (dr, dc) of the inner region
+    auto tvx = [&](int dr, int dc) {
+      return Fu[dr + 1][dc + 1] * Txx[dr][dc] + Fv[dr + 1][dc + 1] * Txy[dr][dc];
+    };
+    auto tvy = [&](int dr, int dc) {
+      return Fu[dr + 1][dc + 1] * Txy[dr][dc] + Fv[dr + 1][dc + 1] * Tyy[dr][dc];
+    };
+    va.src[2 * g.FS + ix] =
+        div_faces(g, i, tvx(r, c - 1), tvx(r, c), tvx(r, c + 1), tvy(r - 1, c), tvy(r, c), tvy(r + 1, c));
+  }
+}
+
+// viscous 1/dt of one cell (sources_viscosity.f90:507-577, the 2-D x-y
+// branch): (max(a+, -a-) + 2 nu/dl)/dl summed over both directions, the wave
+// speeds of CalcTimestep, nu = eta/rho
+__device__ __forceinline__ double visc_invdt(const Geom &g, int i, double rho, double u, double v, double cs,
+                                             double eta) {
+  const double mnx = fmin2(0.0, u - cs), mxx = fmax2(0.0, u + cs);
+  const double mny = fmin2(0.0, v - cs), mxy = fmax2(0.0, v + cs);
+  const double kin = eta / rho, dlx = ring(g, R_DLX, i), dly = ring(g, R_DLY, i);
+  return (fmax2(mxx, -mnx) + 2 * kin / dlx) / dlx + (fmax2(mxy, -mny) + 2 * kin / dly) / dly;
+}
+
 // Viscous source of one RHS evaluation (sources_viscosity.f90:481-500;
 // physics_eulerisotherm.f90:982-1072, physics_euler.f90:1887-1918), FP64:
 // ExternalSources sees Timedisc%pvar after CenterBoundary and the PMIN clamp,
@@ -1564,12 +1612,21 @@ __global__ __launch_bounds__(VT_X *VT_Y, FV_VISC_WAVES_PER_SIMD) void k_visc(Geo
   constexpr int SX = VT_X + 4, SY = VT_Y + 4, RX = VT_X + 2, RY = VT_Y + 2, NT = VT_X * VT_Y;
   __shared__ double Fr[SY][SX], Fu[SY][SX], Fv[SY][SX];  // rho, vx, vy; [r + 2][c + 2]
   __shared__ double Dy[RY][RX], Txx[RY][RX], Tyy[RY][RX], Txy[RY][RX];  // [r + 1][c + 1]
+  __shared__ double Pown[V == 4 ? VT_Y : 1][V == 4 ? VT_X : 1];  // Euler: pressure of the own cells
+  __shared__ double red[VT_X * VT_Y / 64];
   if (a.ctl && a.ctl->halt) return;
   const In<double, V, KM> in(g, a, false);
   const int tx = threadIdx.x, ty = threadIdx.y, t = tx + VT_X * ty;
   const int i0 = blockIdx.x * VT_X, j0 = blockIdx.y * VT_Y;
   const int imin = GH, imax = GH + g.inum - 1, jmin = GH, jmax = GH + g.jnum - 1;
   const bool alpha = va.model == VIS_ALPHA;
+  // UpdateViscosity's time test (sources_viscosity.f90:371): decided by the
+  // host, or here from the step's time and dt in device-controlled steps
+  bool update = va.update;
+  if (a.ctl) {
+    const double tc = a.ctl->time + va.c_cur * a.ctl->dt, tp = a.ctl->time + va.c_prev * a.ctl->dt;
+    update = tc != tp || tc == 0.0;
+  }
 
   // ---- phase 1: pvar of the footprint; dynamic viscosity of the inner
   // region.  At most two footprint cells per thread: every load of both
@@ -1589,7 +1646,7 @@ __global__ __launch_bounds__(VT_X *VT_Y, FV_VISC_WAVES_PER_SIMD) void k_visc(Geo
     inreg[h] = inm[h] && fr[h] >= -1 && fr[h] <= VT_Y && fc[h] >= -1 && fc[h] <= VT_X;
   }
   double dst[2] = {va.dynconst, va.dynconst};
-  if (alpha && !va.update) {
+  if (alpha && !update) {
 #pragma unroll
     for (int h = 0; h < 2; h++)
       if (inreg[h]) dst[h] = va.dyn[cidx(g, fi[h], fj[h])];
@@ -1611,9 +1668,12 @@ __global__ __launch_bounds__(VT_X *VT_Y, FV_VISC_WAVES_PER_SIMD) void k_visc(Geo
     Fr[r + 2][c + 2] = p[0];
     Fu[r + 2][c + 2] = p[1];
     Fv[r + 2][c + 2] = p[2];
+    if constexpr (V == 4) {
+      if (va.part && r >= 0 && r < VT_Y && c >= 0 && c < VT_X) Pown[r][c] = p[3];
+    }
     if (inreg[h]) {
       double d = dst[h];
-      if (alpha && va.update) {
+      if (alpha && update) {
         const long ix = cidx(g, ii, jj);
         d = alpha_dynvis(va, ix, g.FS, p[0], p[1], p[2], ring(g, R_CS, ii));
         if (r >= 0 && r < VT_Y && c >= 0 && c < VT_X) va.dyn[ix] = d;
@@ -1649,31 +1709,27 @@ __global__ __launch_bounds__(VT_X *VT_Y, FV_VISC_WAVES_PER_SIMD) void k_visc(Geo
   // ---- phase 3: tensor divergence (mesh_midpoint.f90:279-335, btyx = btxy)
   // and, for Euler, the divergence of T.v (physics_euler.f90:1887-1918)
   const int i = i0 + tx, j = j0 + ty;
-  if (i < imin || i > imax || j < jmin || j > jmax) return;
-  const int r = ty + 1, c = tx + 1;
-  const double dydzdV = ring(g, R_DYDZDV, i), dzdxdV = ring(g, R_DZDXDV, i);
-  const double axw = ring(g, R_DAXW, i), axe = ring(g, R_DAXE, i), ays = ring(g, R_DAYS, i), ayn = ring(g, R_DAYN, i);
-  const double cxyx = ring(g, R_CXYX, i), cyxy = ring(g, R_CYXY, i);
-  const double xx0 = Txx[r][c], xy0 = Txy[r][c], yy0 = Tyy[r][c];
-  const double sx = dydzdV * (axe * (0.5 * (Txx[r][c + 1] + xx0)) - axw * (0.5 * (Txx[r][c - 1] + xx0))) +
-                    dzdxdV * (ayn * (0.5 * (Txy[r + 1][c] + xy0)) - ays * (0.5 * (Txy[r - 1][c] + xy0))) +
-                    0.0 * (0.0 * 0.0 - 0.0 * 0.0) + cxyx * xy0 + 0.0 * 0.0 - cyxy * yy0 - 0.0 * 0.0;
-  const double sy = dydzdV * (axe * (0.5 * (Txy[r][c + 1] + xy0)) - axw * (0.5 * (Txy[r][c - 1] + xy0))) +
-                    dzdxdV * (ayn * (0.5 * (Tyy[r + 1][c] + yy0)) - ays * (0.5 * (Tyy[r - 1][c] + yy0))) +
-                    0.0 * (0.0 * 0.0 - 0.0 * 0.0) + (-cxyx) * xx0 + 0.0 * 0.0 - (-cyxy) * xy0 - 0.0 * 0.0;
-  const long ix = cidx(g, i, j);
-  va.src[ix] = sx;
-  va.src[g.FS + ix] = sy;
-  if constexpr (V == 4) {
-    // (T.v)_x and (T.v)_y at the cell (dr, dc) of the inner region
-    auto tvx = [&](int dr, int dc) {
-      return Fu[dr + 1][dc + 1] * Txx[dr][dc] + Fv[dr + 1][dc + 1] * Txy[dr][dc];
-    };
-    auto tvy = [&](int dr, int dc) {
-      return Fu[dr + 1][dc + 1] * Txy[dr][dc] + Fv[dr + 1][dc + 1] * Tyy[dr][dc];
-    };
-    va.src[2 * g.FS + ix] =
-        div_faces(g, i, tvx(r, c - 1), tvx(r, c), tvx(r, c + 1), tvy(r - 1, c), tvy(r, c), tvy(r + 1, c));
+  if (i >= imin && i <= imax && j >= jmin && j <= jmax)
+    visc_divergence<V, SX, RX>(g, va, i, j, ty + 1, tx + 1, Fu, Fv, Txx, Tyy, Txy);
+  if (!va.part) return;
+  // the viscous limit of this evaluation's pvar on the own cell, every cell of
+  // the mesh (as k_visc_cfl on Timedisc%pvar: comoving velocities under FARGO)
+  double m = -DBL_MAX;
+  if (i < g.NI && j < g.NJ) {
+    const double rho = Fr[ty + 2][tx + 2], u = Fu[ty + 2][tx + 2];
+    const double v = a.sub_bg ? Fv[ty + 2][tx + 2] - ring(g, R_W, i) : Fv[ty + 2][tx + 2];
+    double cs;
+    if constexpr (V == 4)
+      cs = sound_speed(g.gamma, rho, Pown[ty][tx]);
+    else
+      cs = ring(g, R_CS, i);
+    m = visc_invdt(g, i, rho, u, v, cs, Dy[ty + 1][tx + 1]);
+  }
+  m = blk_max(m, red);
+  if (t == 0) {
+    const int b = blockIdx.x + blockIdx.y * gridDim.x;
+    va.part[2 * b] = m;
+    va.part[2 * b + 1] = 0.0;
   }
 }
 
@@ -1695,12 +1751,9 @@ __global__ __launch_bounds__(256) void k_visc_cfl(Geom g, VisArgs va, const doub
     double v = pv[2 * g.FS + ix];
     if (add_w) v = v + ring(g, R_W, i);
     const double cs = (V == 4) ? sound_speed(g.gamma, rho, pv[3 * g.FS + ix]) : ring(g, R_CS, i);
-    const double mnx = fmin2(0.0, u - cs), mxx = fmax2(0.0, u + cs);
-    const double mny = fmin2(0.0, v - cs), mxy = fmax2(0.0, v + cs);
     double d = va.dynconst;
     if (va.model == VIS_ALPHA) d = va.update ? alpha_dynvis(va, ix, g.FS, rho, u, v, cs) : va.dyn[ix];
-    const double kin = d / rho, dlx = ring(g, R_DLX, i), dly = ring(g, R_DLY, i);
-    m = fmax2(m, (fmax2(mxx, -mnx) + 2 * kin / dlx) / dlx + (fmax2(mxy, -mny) + 2 * kin / dly) / dly);
+    m = fmax2(m, visc_invdt(g, i, rho, u, v, cs, d));
   }
   m = blk_max(m, red);
   if (threadIdx.x == 0) {

@@ -11,11 +11,12 @@ namespace fv {
 // Step control of a device-controlled batch (one block): embedded error of
 // the step from the partials of the last stage, accept test, then CFL and
 // data checks from the end-of-step k_rhs partials and the next time step.
-__device__ __forceinline__ void control_block(Ctl *ctl, const double *part_c, const double *part_r, int nb) {
-  __shared__ double rede[4], redm[4];
+__device__ __forceinline__ void control_block(Ctl *ctl, const double *part_c, const double *part_r, int nb,
+                                              const double *part_v = nullptr, int nbv = 0) {
+  __shared__ double rede[4], redm[4], redv[4];
   __shared__ int redi[4];
   if (ctl->halt) return;
-  double e = -DBL_MAX, m = -DBL_MAX;
+  double e = -DBL_MAX, m = -DBL_MAX, v = -DBL_MAX;
   int f = 0;
 #pragma unroll 4
   for (int b = threadIdx.x; b < nb; b += blockDim.x) {
@@ -23,31 +24,38 @@ __device__ __forceinline__ void control_block(Ctl *ctl, const double *part_c, co
     m = fmax2(m, part_r[2 * b]);
     f |= (int)part_r[2 * b + 1];
   }
+  if (part_v) {
+    for (int b = threadIdx.x; b < nbv; b += blockDim.x) v = fmax2(v, part_v[2 * b]);
+  }
   // one barrier: wave results to LDS, thread 0 combines
   e = wave_max(e);
   m = wave_max(m);
+  v = wave_max(v);
   f = wave_or(f);
   const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
   if ((threadIdx.x & 63) == 0) {
     rede[w] = e;
     redm[w] = m;
+    redv[w] = v;
     redi[w] = f;
   }
   __syncthreads();
   if (threadIdx.x == 0) {  // from -DBL_MAX in wave order, as blk_max
-    e = m = -DBL_MAX;
+    e = m = v = -DBL_MAX;
     f = 0;
     for (int k = 0; k < nw; k++) {
       e = fmax2(e, rede[k]);
       m = fmax2(m, redm[k]);
+      v = fmax2(v, redv[k]);
       f |= redi[k];
     }
     control_accept(ctl, e);
-    if (!ctl->halt) control_next(ctl, m, f);
+    if (!ctl->halt) control_next(ctl, m, f, v);
   }
 }
-__global__ __launch_bounds__(256) void k_control(Ctl *ctl, const double *part_c, const double *part_r, int nb) {
-  control_block(ctl, part_c, part_r, nb);
+__global__ __launch_bounds__(256) void k_control(Ctl *ctl, const double *part_c, const double *part_r, int nb,
+                                                 const double *part_v, int nbv) {
+  control_block(ctl, part_c, part_r, nb, part_v, nbv);
 }
 // host-controlled steps: the boundary-flux accumulation of a step
 __global__ __launch_bounds__(256) void k_bflux(BfluxArgs bf) {
@@ -279,9 +287,11 @@ void launch_frame(const Geom &g, const void *src, void *dst, int sign, int V, hi
   else
     frame_single<double>(g, src, dst, sign, V, ctl, w2, s);
 }
-void launch_control(Ctl *ctl, const double *part_c, const double *part_r, int nb, hipStream_t s) {
-  hipLaunchKernelGGL(k_control, dim3(1), dim3(256), 0, s, ctl, part_c, part_r, nb);
+void launch_control(Ctl *ctl, const double *part_c, const double *part_r, int nb, hipStream_t s,
+                    const double *part_v, int nbv) {
+  hipLaunchKernelGGL(k_control, dim3(1), dim3(256), 0, s, ctl, part_c, part_r, nb, part_v, nbv);
 }
+int visc_blocks(const Geom &g) { return ((g.NI + VT_X - 1) / VT_X) * ((g.NJ + VT_Y - 1) / VT_Y); }
 void launch_rhs_multi(const RhsEntry *tab, const unsigned *tiles, int ntiles, int V, int dtype, int limiter,
                       int tx, int stages, int mode, hipStream_t s) {
   const bool small = km_for(stages) == KM_SMALL;

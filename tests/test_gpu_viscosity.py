@@ -78,20 +78,30 @@ def test_gpu_viscosity_steps_match_oracle(name):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["shear_euler_molecular_2x_tiles", "disk_alpha_fargo1_rotframe"])
-def test_gpu_viscosity_run_matches_oracle(name):
-    """fv_run with viscosity takes host-controlled steps, identical to the
-    oracle's."""
+@pytest.mark.parametrize("name", ["shear_euler_molecular_2x_tiles", "disk_alpha_fargo1_rotframe",
+                                  "euler_molecular_bulk_periodic_dp5", "disk_alpha_no_fargo_limited",
+                                  "shear_iso_molecular_fargo1_limited", "euler_disk_molecular_fargo1_prim",
+                                  "disk_alpha_dp5_absorbing"])
+def test_gpu_viscosity_device_rounds_match_oracle(name):
+    """fv_run with viscosity: device-controlled steps (k_visc before every
+    k_rhs, UpdateViscosity's time test and the viscous limit of the next step
+    decided on the device, the host replaying the scalars), identical to the
+    oracle's steps; the profiled chunk shows that device rounds ran."""
     cfg, _, tol = CASES[name]
     print("viscosity run()", name, flush=True)
     sim, ref = pair(cfg)
     done = 0
-    for chunk in (1, 6, 7):
+    for k, chunk in enumerate((1, 6, 7, 5)):
+        if k == 3:
+            sim.profile(True)
         done += sim.run(chunk)
         for _ in range(chunk):
             ref.step()
-        check(sim, ref, tol, (name, done))
-    assert done == 14
+        sc = check(sim, ref, tol, (name, done))
+        assert sc["dtcause"] == ref.state()[2]["dtcause"], (name, done)
+    assert done == 19
+    assert sim.kernel_stats("control")[0] > 0 and sim.kernel_stats("visc")[0] > 0
+    sim.profile(False)
     sim.close()
 
 
