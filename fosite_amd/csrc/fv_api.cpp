@@ -168,6 +168,10 @@ struct fv_ctx {
   // device-controlled steps: per-workgroup viscous 1/dt of the end-of-step k_visc
   double *d_vpart = nullptr;
   int nb_vis = 0;
+  // ALPHA with Euler physics: the sound speed of Timedisc%pvar, in d_cs[k0cur]
+  // (a device step's stages read one, its end-of-step evaluation writes the
+  // other, like the coeff(1) pair)
+  double *d_cs[2] = {};
   Ctl *d_ctl = nullptr, *h_ctl = nullptr;
   struct GraphEntry { int x0, x1, t, k0, b0, g0, w0, v0, nsteps; hipGraphExec_t exec; };
   std::vector<GraphEntry> graphs;
@@ -317,6 +321,7 @@ VisArgs vis_args(const fv_ctx *ctx, int update) {
   v.ephir = ctx->d_ephir;
   v.dyn = ctx->d_dyn;
   v.src = ctx->d_vsrc;
+  v.cs_in = v.cs_out = ctx->d_cs[ctx->k0cur];
   return v;
 }
 // the viscous source of the evaluation a at time t, ahead of its k_rhs
@@ -825,11 +830,14 @@ void fast_step_args(fv_ctx *ctx, Ctl *ctl, int xin, int xout, int tb, int rb, in
 // the viscous source of a device-controlled evaluation at ctl->time +
 // c_cur*ctl->dt, the previous evaluation being at ctl->time + c_prev*ctl->dt
 // (part: the end-of-step one's viscous 1/dt partials)
-void visc_device(fv_ctx *ctx, const Geom &g, StageArgs &a, double c_cur, double c_prev, double *part) {
+void visc_device(fv_ctx *ctx, const Geom &g, StageArgs &a, double c_cur, double c_prev, double *part, int csin,
+                 int csout) {
   VisArgs v = vis_args(ctx, 0);
   v.c_cur = c_cur;
   v.c_prev = c_prev;
   v.part = part;
+  v.cs_in = ctx->d_cs[csin];
+  v.cs_out = ctx->d_cs[csout];
   long pk = prof_begin(ctx);
   launch_visc(g, a, v, ctx->V, ctx->st);
   prof_end(ctx, pk, K_VISC, 8.0 * cells(ctx) * (ctx->V * (a.nk + 1.0) + ctx->V - 1.0));
@@ -850,7 +858,7 @@ void enqueue_fast_step(fv_ctx *ctx, int xin, int xout, int tb, int rb, int k0in,
   for (int m = 1; m < ctx->tab.m; m++) {
     // stage m runs at time + c(m)*dt; stage 1 follows the end-of-step
     // evaluation of the previous step, at time
-    if (vis) visc_device(ctx, f.g, f.st[m - 1], ctx->tab.c[m], m > 1 ? ctx->tab.c[m - 1] : 0.0, nullptr);
+    if (vis) visc_device(ctx, f.g, f.st[m - 1], ctx->tab.c[m], m > 1 ? ctx->tab.c[m - 1] : 0.0, nullptr, k0in, k0in);
     launch_stage(ctx, f.g, f.st[m - 1], m);
   }
   if (f.remap) {
@@ -865,7 +873,7 @@ void enqueue_fast_step(fv_ctx *ctx, int xin, int xout, int tb, int rb, int k0in,
     prof_end(ctx, pk, K_ACCRETE, 0.0);
   }
   // the end of the step at time + dt (as time + 1*dt)
-  if (vis) visc_device(ctx, f.g, f.end, 1.0, ctx->tab.c[ctx->tab.m - 1], ctx->d_vpart);
+  if (vis) visc_device(ctx, f.g, f.end, 1.0, ctx->tab.c[ctx->tab.m - 1], ctx->d_vpart, k0out, k0out);
   long pk = prof_begin(ctx);
   launch_rhs(f.g, f.end, ctx->V, ctx->st);
   prof_end(ctx, pk, K_RHS_END, 3.0 * word_bytes(ctx));
@@ -1058,8 +1066,6 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
   if (c.rotframe < 0 || c.rotframe > 2) return bail("rotframe must be 0 (off), 1 or 2 (disable_centaccel)");
   if (c.vismodel < 0 || c.vismodel > 2) return bail("vismodel must be 0 (off), 1 MOLECULAR or 2 ALPHA");
   if (c.vismodel && c.dtype != 0) return bail("viscosity is supported with the FP64 state (dtype 0)");
-  if (c.vismodel == VIS_ALPHA && c.physics != EULER_ISOTHERM)
-    return bail("ALPHA viscosity is supported with isothermal physics");
   if (!set_tableau(c.method, c.order, ctx->tab)) return bail("unsupported time integrator (method/order)");
   if (mesh_build(c, ctx->mesh)) return bail("mesh construction failed");
   ctx->V = c.physics == EULER ? 4 : 3;
@@ -1166,6 +1172,8 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
   }
   if (c.vismodel == VIS_ALPHA)
     ok = ok && dalloc(&ctx->d_dyn, sizeof(double) * ctx->FS) && dalloc(&ctx->d_ephir, sizeof(double) * 2 * ctx->FS);
+  for (int q = 0; q < 2 && c.vismodel == VIS_ALPHA && ctx->V == 4; q++)
+    ok = ok && dalloc(&ctx->d_cs[q], sizeof(double) * ctx->FS);
   if (!ok) {
     fv_destroy(ctx);
     std::fprintf(stderr, "fv_create: device allocation failed\n");
@@ -1256,6 +1264,7 @@ extern "C" int fv_destroy(fv_ctx *ctx) {
   if (ctx->pbuf) (void)hipFree(ctx->pbuf);
   if (ctx->d_vsrc) (void)hipFree(ctx->d_vsrc);
   if (ctx->d_vpart) (void)hipFree(ctx->d_vpart);
+  for (auto &b : ctx->d_cs) if (b) (void)hipFree(b);
   if (ctx->d_dyn) (void)hipFree(ctx->d_dyn);
   if (ctx->d_ephir) (void)hipFree(ctx->d_ephir);
   if (ctx->d_ring1) (void)hipFree(ctx->d_ring1);
@@ -1295,7 +1304,7 @@ extern "C" int fv_device_bytes(const fv_config *c, double *bytes) {
   if (c->outbound) b += 2.0 * (8.0 * (PM_HDR + 2.0 * NI) + (c->dtype == 1 ? 4.0 * 2.0 * NI : 0.0));
   if (c->vismodel)                                      // viscous source, its step partials
     b += 8.0 * (V - 1.0) * NJ * P + 16.0 * std::ceil(NI / 64.0) * std::ceil(NJ / 8.0);
-  if (c->vismodel == 2) b += 3.0 * 8.0 * NJ * P;        // ALPHA: dynamic viscosity, e_phi/r
+  if (c->vismodel == 2) b += (V == 4 ? 5.0 : 3.0) * 8.0 * NJ * P;  // ALPHA: viscosity, e_phi/r, Euler: sound speeds
   b += 2.0 * 2.0 * 8.0 * nb32;                         // CFL and error partials
   b += 8.0 * (R_COUNT * NI + C_COUNT * NJ);             // metric arrays
   if (c->dtype == 1) b += 4.0 * (R_COUNT * NI + C_COUNT * NJ);

@@ -163,6 +163,12 @@ struct VisArgs {
   const double *ephir;  // ALPHA: e_phi/r in curvilinear components, [2][NJ][P]
   double *dyn;          // ALPHA: dynamic viscosity of the last update, all cells, [NJ][P]
   double *src;          // k_visc output: StageArgs::vsrc
+  // ALPHA with Euler physics: Physics%bccsound, the sound speed of
+  // Timedisc%pvar that every ComputeRHS of the main state updates
+  // (timedisc_base.f90:1166-1174); a main-state evaluation (nk = 0) writes it
+  // to cs_out from its own pvar, a stage evaluation reads it from cs_in
+  const double *cs_in;
+  double *cs_out;
   // device-controlled steps (StageArgs::ctl set): the evaluation's time is
   // ctl->time + c_cur*ctl->dt and the previous one's ctl->time + c_prev*ctl->dt,
   // and UpdateViscosity's time test is decided there (update is unused)

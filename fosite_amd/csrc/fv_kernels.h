@@ -1668,14 +1668,21 @@ __global__ __launch_bounds__(VT_X *VT_Y, FV_VISC_WAVES_PER_SIMD) void k_visc(Geo
     Fr[r + 2][c + 2] = p[0];
     Fu[r + 2][c + 2] = p[1];
     Fv[r + 2][c + 2] = p[2];
+    const bool own = r >= 0 && r < VT_Y && c >= 0 && c < VT_X;
     if constexpr (V == 4) {
-      if (va.part && r >= 0 && r < VT_Y && c >= 0 && c < VT_X) Pown[r][c] = p[3];
+      if (va.part && own) Pown[r][c] = p[3];
+      if (alpha && a.nk == 0 && own) va.cs_out[cidx(g, ii, jj)] = sound_speed(g.gamma, p[0], p[3]);
     }
     if (inreg[h]) {
       double d = dst[h];
       if (alpha && update) {
         const long ix = cidx(g, ii, jj);
-        d = alpha_dynvis(va, ix, g.FS, p[0], p[1], p[2], ring(g, R_CS, ii));
+        double cs;
+        if constexpr (V == 4)
+          cs = a.nk == 0 ? sound_speed(g.gamma, p[0], p[3]) : va.cs_in[ix];
+        else
+          cs = ring(g, R_CS, ii);
+        d = alpha_dynvis(va, ix, g.FS, p[0], p[1], p[2], cs);
         if (r >= 0 && r < VT_Y && c >= 0 && c < VT_X) va.dyn[ix] = d;
       }
       Dy[r + 1][c + 1] = d;

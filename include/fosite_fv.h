@@ -73,8 +73,8 @@ typedef struct fv_config {
                        (disable_centaccel = 1) */
   int vismodel;     /* viscosity source (sources/sources_viscosity.f90:73-577): 0 off,
                        1 MOLECULAR (constant dynamic and bulk viscosity), 2 ALPHA
-                       (Shakura-Sunyaev, isothermal physics; uses omega above); FP64 state,
-                       single contexts (not in batches) */
+                       (Shakura-Sunyaev; uses omega above); FP64 state, single contexts
+                       (not in batches) */
   double dynconst, bulkconst; /* dynamic viscosity (MOLECULAR) or alpha (ALPHA), default 0.1;
                        bulk viscosity (MOLECULAR), the reference's default being -2/3 dynconst
                        (sources_viscosity.f90:152-158) */

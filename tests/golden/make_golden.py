@@ -111,6 +111,15 @@ CASES = {
                                          bc=[S.NO_GRADIENTS, S.NO_GRADIENTS, S.PERIODIC, S.PERIODIC]), 8, (8,)),
     "disk_alpha_dp5_absorbing": (dict(D, fargo=0, vismodel=2, dynconst=0.05, method=S.DORMAND_PRINCE, order=5,
                                       tol_rel=1e-4), 8, (8,)),
+    # ALPHA with Euler physics: the sound speed of Timedisc%pvar, updated by
+    # every ComputeRHS of the main state (timedisc_base.f90:1166-1174), also
+    # in the stage evaluations
+    "euler_disk_alpha": (dict(S.euler_disk_config(ni=16, nj=32), vismodel=2, dynconst=0.02,
+                              bc=[S.NO_GRADIENTS, S.NO_GRADIENTS, S.PERIODIC, S.PERIODIC]), 8, (1, 8)),
+    "euler_disk_alpha_fargo1_rotframe": (dict(S.euler_disk_config(ni=16, nj=32), vismodel=2, dynconst=0.01,
+                                              fargo=1, omega=0.3, rotframe=1, method=S.SSPRK, order=3,
+                                              bc=[S.NO_GRADIENTS, S.NO_GRADIENTS, S.PERIODIC, S.PERIODIC]),
+                                         8, (8,)),
 }
 MESHES = {"cartesian_16": E, "logcyl_16x32": D}
 SCALARS = ("iter", "n_adj", "brk", "time", "dt", "dtold", "maxerrold")
@@ -119,8 +128,16 @@ SCALARS = ("iter", "n_adj", "brk", "time", "dt", "dtold", "maxerrold")
 def main():
     if not refrun.available():
         sys.exit("reference build missing: run oracle/build_ref.py first")
+    # --only NAME ...: recompute those cases and keep the others from the
+    # existing file
+    only = sys.argv[sys.argv.index("--only") + 1:] if "--only" in sys.argv else None
     out = {}
+    if only:
+        with np.load(OUT, allow_pickle=False) as old:
+            out = {k: old[k] for k in old.files if k.split("/")[0] not in only}
     for name, (cfg, steps, keep) in CASES.items():
+        if only and name not in only:
+            continue
         init = S.initial(cfg)
         _, recs = refrun.run(cfg, init, steps, dump_every=1)
         assert [r["step"] for r in recs] == list(range(steps + 1)), name
@@ -142,7 +159,7 @@ def main():
             if cfg.get("outbound", 0):  # accel(:,:,:,1:2) of the point mass
                 out["%s/accel_%d" % (name, s)] = recs[s]["accel"]
         print(name, "n_adj", recs[-1]["n_adj"], "time %.6e" % recs[-1]["time"])
-    for name, cfg in MESHES.items():
+    for name, cfg in (MESHES.items() if not only else ()):
         mesh, _ = refrun.run(cfg, S.initial(cfg), 0, dump_every=1)
         for k, v in mesh.items():
             if isinstance(v, np.ndarray):

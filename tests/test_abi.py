@@ -63,7 +63,6 @@ BAD = {
     "dtype": dict(dtype=1),
     "vismodel": dict(vismodel=3),
     "viscosity in fp32": dict(vismodel=1, dtype="fp32"),
-    "alpha viscosity with euler": dict(vismodel=2),
 }
 
 

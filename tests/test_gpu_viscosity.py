@@ -14,7 +14,9 @@ Dormand-Prince whose last two stages share a time, one limited by the
 viscous step: dtcause 3), an Euler disk with FARGO type 1 and PRIMITIVE
 reconstruction, and isothermal ALPHA disks with the rotating frame and FARGO
 type 1, without FARGO and limited by the viscous step, with SSPRK(5) on two
-tiles, and with absorbing boundaries."""
+tiles, and with absorbing boundaries; ALPHA on Euler disks (Dormand-Prince;
+FARGO type 1 in a rotating frame), whose stages take the sound speed of the
+step's initial state."""
 import ctypes
 
 import numpy as np
@@ -53,6 +55,11 @@ CASES = {
                                     method=S.SSPRK, order=5, bc=NG), 15, 0.0),
     "disk_alpha_dp5_absorbing": (dict(D, fargo=0, vismodel=2, dynconst=0.05, method=S.DORMAND_PRINCE, order=5,
                                       tol_rel=1e-4), 12, 1e-12),
+    # ALPHA with Euler physics: the sound speed of Timedisc%pvar in the stages
+    "euler_disk_alpha_dp5": (dict(S.euler_disk_config(ni=32, nj=64), vismodel=2, dynconst=0.02, bc=NG), 12, 0.0),
+    "euler_disk_alpha_fargo1_rotframe": (dict(S.euler_disk_config(ni=32, nj=64), vismodel=2, dynconst=0.01,
+                                              fargo=1, omega=0.3, rotframe=1, method=S.SSPRK, order=3, bc=NG),
+                                         12, 0.0),
 }
 
 
@@ -81,7 +88,8 @@ def test_gpu_viscosity_steps_match_oracle(name):
 @pytest.mark.parametrize("name", ["shear_euler_molecular_2x_tiles", "disk_alpha_fargo1_rotframe",
                                   "euler_molecular_bulk_periodic_dp5", "disk_alpha_no_fargo_limited",
                                   "shear_iso_molecular_fargo1_limited", "euler_disk_molecular_fargo1_prim",
-                                  "disk_alpha_dp5_absorbing"])
+                                  "disk_alpha_dp5_absorbing", "euler_disk_alpha_dp5",
+                                  "euler_disk_alpha_fargo1_rotframe"])
 def test_gpu_viscosity_device_rounds_match_oracle(name):
     """fv_run with viscosity: device-controlled steps (k_visc before every
     k_rhs, UpdateViscosity's time test and the viscous limit of the next step
