@@ -28,12 +28,12 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch on this workload: rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE
 # x1.0 (factors from a 1 GiB copy kernel), KiB per dispatch
 # (profiles/k_rhs_disk512x1024_hbm_traffic.md)
-TRAFFIC_BYTES_PER_LAUNCH = {"rhs": (2.0 * 13547 + 12425) * 1024, "rhs_combine": (2.0 * 20783 + 12519) * 1024,
-                            "remap": (2.0 * 6633 + 12384) * 1024, "rhs_end": (2.0 * 7726 + 28216) * 1024}
+TRAFFIC_BYTES_PER_LAUNCH = {"rhs": (2.0 * 13548 + 12420) * 1024, "rhs_combine": (2.0 * 21428 + 12523) * 1024,
+                            "remap": (2.0 * 6633 + 12384) * 1024, "rhs_end": (2.0 * 7733 + 28220) * 1024}
 # VALU-issue floor per launch in us: SQ_INSTS_VALU x 4 cycles + SQ_INSTS_VALU_TRANS_F64
 # x 12 more per wave, x waves per SIMD, at 2.4 GHz
 # (profiles/k_rhs_disk512x1024_sq_counters.md)
-VALU_FLOOR_US = {"rhs": 9.55, "rhs_combine": 12.87, "remap": 2.69, "rhs_end": 10.80}
+VALU_FLOOR_US = {"rhs": 9.55, "rhs_combine": 12.87, "remap": 2.69, "rhs_end": 10.89}
 KERNEL_NAMES = {
     "rhs": "fv::k_rhs<double,3,VANLEER,64,3,M_STAGE>: stage 2 (stage sum over k_1, ghost cells, RHS)",
     "rhs_combine": "fv::k_rhs<double,3,VANLEER,64,3,M_LAST>: stage 3 with the final combination and "

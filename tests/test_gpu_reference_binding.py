@@ -42,6 +42,9 @@ CASES = {
     # MOLECULAR viscosity on the Euler shear box with FARGO along y, 3 outputs
     "euler shear box viscous": (dict(S.shear_box_config(n=32), stoptime=0.09, count=3, vismodel=1, dynconst=1e-3),
                                 40, 0.0),
+    # ALPHA viscosity on an Euler disk (the sound speed of Timedisc%pvar), DP5, 3 outputs
+    "euler disk alpha": (dict(S.euler_disk_config(ni=32, nj=48), vismodel=2, dynconst=0.02, bc=NG, stoptime=0.6,
+                              count=3), 30, 0.0),
 }
 
 
