@@ -194,6 +194,7 @@ struct fv_ctx {
   double cs_min = 0;
   double last_ms = 0;
   double invdt = 0;
+  double invdt_v = 0;  // viscous 1/dt of the last device-controlled step (Ctl::invdt_v)
   int flags = 0;
   // per-kernel timing (HIP events around each launch on the context stream)
   bool prof = false;
@@ -581,17 +582,18 @@ int cfl_real_frame(fv_ctx *ctx) {
 int timestep(fv_ctx *ctx) {
   ctx->dt = calc_timestep(ctx);
   if (!ctx->c.vismodel) return 0;
-  if (!ctx->pvalid) return fail(ctx, "viscous time step: no evaluation has stored Timedisc%pvar");
-  const int update = ctx->c.vismodel == VIS_ALPHA && (ctx->time != ctx->vis_time || ctx->time == 0.0);
-  ctx->vis_time = ctx->time;
-  const long cells = (long)ctx->NI * ctx->NJ;
-  const int nb = (int)std::min<long>(ctx->nb_rhs, (cells + 255) / 256);
-  launch_visc_cfl(ctx->g, vis_args(ctx, update), ctx->pbuf, pvar_moved_to_real_frame(ctx) ? 1 : 0, ctx->part_r, nb,
-                  ctx->V, ctx->st);
-  HIPCHK(hipGetLastError());
-  double iv = 0;
-  int f = 0;
-  if (int rc = reduce(ctx, ctx->part_r, nb, &iv, &f)) return rc;
+  double iv = ctx->invdt_v;  // after device-controlled steps: the last one's, from the step control
+  if (ctx->pvalid) {
+    const int update = ctx->c.vismodel == VIS_ALPHA && (ctx->time != ctx->vis_time || ctx->time == 0.0);
+    ctx->vis_time = ctx->time;
+    const long cells = (long)ctx->NI * ctx->NJ;
+    const int nb = (int)std::min<long>(ctx->nb_rhs, (cells + 255) / 256);
+    launch_visc_cfl(ctx->g, vis_args(ctx, update), ctx->pbuf, pvar_moved_to_real_frame(ctx) ? 1 : 0, ctx->part_r,
+                    nb, ctx->V, ctx->st);
+    HIPCHK(hipGetLastError());
+    int f = 0;
+    if (int rc = reduce(ctx, ctx->part_r, nb, &iv, &f)) return rc;
+  }
   const double dtv = iv > DBL_MIN ? ctx->c.cvis / iv : DBL_MAX;
   if (dtv < ctx->dt) ctx->dtcause = DTCAUSE_VISCOSITY;
   ctx->dt = dtv < ctx->dt ? dtv : ctx->dt;
@@ -938,6 +940,7 @@ int replay_device_steps(fv_ctx *ctx, const Ctl *h, int x0, int x1, int *halted) 
     ctx->invdt = h->invdt;
     ctx->dt = calc_timestep(ctx);
     if (ctx->c.vismodel) {  // and its viscous limit, as timestep() forms it
+      ctx->invdt_v = h->invdt_v;
       const double dtv = h->invdt_v > DBL_MIN ? ctx->c.cvis / h->invdt_v : DBL_MAX;
       if (dtv < ctx->dt) ctx->dtcause = DTCAUSE_VISCOSITY;
       ctx->dt = dtv < ctx->dt ? dtv : ctx->dt;

@@ -110,6 +110,13 @@ def test_gpu_viscosity_device_rounds_match_oracle(name):
     assert done == 19
     assert sim.kernel_stats("control")[0] > 0 and sim.kernel_stats("visc")[0] > 0
     sim.profile(False)
+    # CalcTimestep right after a device round: the step the round's control
+    # formed (viscous limit included), as the oracle's
+    dt, cause = ctypes.c_double(), ctypes.c_int()
+    assert sim.L.fv_calc_timestep(sim._h, ctypes.byref(dt), ctypes.byref(cause)) == 0
+    rs = ref.state()[2]
+    if tol == 0.0:
+        assert dt.value == rs["dt"] and cause.value == rs["dtcause"], (name, dt.value, cause.value, rs)
     sim.close()
 
 
