@@ -14,7 +14,7 @@
  * with error control and reject path, no-gradient, periodic and absorbing
  * boundaries, point-mass gravity with accretion, the rotating frame about
  * the origin (centrifugal and Coriolis terms), the viscosity source
- * (MOLECULAR and ALPHA models, with its time-step limit), FARGO along y for both
+ * (MOLECULAR, ALPHA and BETA models, with its time-step limit), FARGO along y for both
  * physics and geometries with the per-ring mean (type 1) or a user-supplied
  * (type 2) background velocity, CheckData breaks,
  * output times (datafile count) and a start time.
@@ -225,7 +225,7 @@ static void mesh_build(fvr *s) {
       s->bulkvis[c] = s->c.bulkconst;
     }
   }
-  if (s->c.vismodel == 2) {
+  if (s->c.vismodel == 2 || s->c.vismodel == 3) {
     for (int c = 0; c < s->N; c++) {
       const double phi = s->bcy[c];
       double x = s->bcx[c], y = s->bcy[c];
@@ -593,6 +593,14 @@ static void update_viscosity(fvr *s, double time, const double *p) {
       om = om + s->c.omega;
       const double cs = s->bccsound[k];
       s->dynvis[k] = s->c.dynconst * p[k] * (cs * cs) / fabs(om);
+      s->bulkvis[k] = -(2. / 3.) * s->dynvis[k];
+    }
+  } else if (s->c.vismodel == 3) { /* BETA: beta*rho*r^2*|Omega| (sources_viscosity.f90:388-394, 457-466) */
+    for (int k = 0; k < N; k++) {
+      double om = s->ephir[0][k] * p[1 * N + k];
+      om = om + s->ephir[1][k] * p[2 * N + k];
+      om = om + s->c.omega;
+      s->dynvis[k] = s->c.dynconst * p[k] * (s->rad[k] * s->rad[k]) * fabs(om);
       s->bulkvis[k] = -(2. / 3.) * s->dynvis[k];
     }
   }
@@ -1302,7 +1310,7 @@ fvr *fvr_create(const fvr_cfg *cfg) {
   }
   if (cfg->outbound < 0 || cfg->outbound > 4) return NULL;
   if (cfg->rotframe < 0 || cfg->rotframe > 2) return NULL;
-  if (cfg->vismodel < 0 || cfg->vismodel > 2) return NULL;
+  if (cfg->vismodel < 0 || cfg->vismodel > 3) return NULL;
   s = (fvr *)calloc(1, sizeof(fvr));
   s->c = *cfg;
   s->pm_mass = cfg->mass; /* mdot, massloss, accrate start at 0 (gravity_pointmass.f90:206-209) */

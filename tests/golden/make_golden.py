@@ -120,6 +120,12 @@ CASES = {
                                               fargo=1, omega=0.3, rotframe=1, method=S.SSPRK, order=3,
                                               bc=[S.NO_GRADIENTS, S.NO_GRADIENTS, S.PERIODIC, S.PERIODIC]),
                                          8, (8,)),
+    # BETA (Duschl): beta*rho*r^2*|Omega| with the bary-centre radius
+    "disk_beta": (dict(D, fargo=0, vismodel=3, dynconst=1e-3,
+                       bc=[S.NO_GRADIENTS, S.NO_GRADIENTS, S.PERIODIC, S.PERIODIC]), 8, (1, 8)),
+    "euler_disk_beta_fargo1": (dict(S.euler_disk_config(ni=16, nj=32), vismodel=3, dynconst=1e-3, fargo=1,
+                                    method=S.SSPRK, order=3,
+                                    bc=[S.NO_GRADIENTS, S.NO_GRADIENTS, S.PERIODIC, S.PERIODIC]), 8, (8,)),
 }
 MESHES = {"cartesian_16": E, "logcyl_16x32": D}
 SCALARS = ("iter", "n_adj", "brk", "time", "dt", "dtold", "maxerrold")
