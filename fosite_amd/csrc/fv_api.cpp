@@ -584,7 +584,7 @@ int timestep(fv_ctx *ctx) {
   if (!ctx->c.vismodel) return 0;
   double iv = ctx->invdt_v;  // after device-controlled steps: the last one's, from the step control
   if (ctx->pvalid) {
-    const int update = ctx->c.vismodel == VIS_ALPHA && (ctx->time != ctx->vis_time || ctx->time == 0.0);
+    const int update = ctx->c.vismodel >= VIS_ALPHA && (ctx->time != ctx->vis_time || ctx->time == 0.0);
     ctx->vis_time = ctx->time;
     const long cells = (long)ctx->NI * ctx->NJ;
     const int nb = (int)std::min<long>(ctx->nb_rhs, (cells + 255) / 256);
@@ -1067,7 +1067,7 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
   if (c.outbound < 0 || c.outbound > 4) return bail("outbound must be 0 (no accretion) or a boundary 1..4");
   if (c.outbound && !c.gravity) return bail("accretion (outbound) needs the point mass (gravity = 1)");
   if (c.rotframe < 0 || c.rotframe > 2) return bail("rotframe must be 0 (off), 1 or 2 (disable_centaccel)");
-  if (c.vismodel < 0 || c.vismodel > 2) return bail("vismodel must be 0 (off), 1 MOLECULAR or 2 ALPHA");
+  if (c.vismodel < 0 || c.vismodel > 3) return bail("vismodel must be 0 (off), 1 MOLECULAR, 2 ALPHA or 3 BETA");
   if (c.vismodel && c.dtype != 0) return bail("viscosity is supported with the FP64 state (dtype 0)");
   if (!set_tableau(c.method, c.order, ctx->tab)) return bail("unsupported time integrator (method/order)");
   if (mesh_build(c, ctx->mesh)) return bail("mesh construction failed");
@@ -1173,8 +1173,8 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
     ok = ok && dalloc(&ctx->d_vsrc, sizeof(double) * (ctx->V - 1) * ctx->FS) &&
          dalloc(&ctx->d_vpart, sizeof(double) * 2 * ctx->nb_vis);
   }
-  if (c.vismodel == VIS_ALPHA)
-    ok = ok && dalloc(&ctx->d_dyn, sizeof(double) * ctx->FS) && dalloc(&ctx->d_ephir, sizeof(double) * 2 * ctx->FS);
+  if (c.vismodel >= VIS_ALPHA)
+    ok = ok && dalloc(&ctx->d_dyn, sizeof(double) * ctx->FS) && dalloc(&ctx->d_ephir, sizeof(double) * 3 * ctx->FS);
   for (int q = 0; q < 2 && c.vismodel == VIS_ALPHA && ctx->V == 4; q++)
     ok = ok && dalloc(&ctx->d_cs[q], sizeof(double) * ctx->FS);
   if (!ok) {
@@ -1187,12 +1187,13 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
   g.ringf = ctx->d_ringf;
   g.colf = ctx->d_colf;
   if (upload_geom(ctx)) { fv_destroy(ctx); return -3; }
-  if (c.vismodel == VIS_ALPHA) {
+  if (c.vismodel >= VIS_ALPHA) {
     // e_phi/r = (-y/r^2, x/r^2) at the Cartesian bary centres, r^2 = radius^2
+    // (kept as the third array, BETA's r**2)
     // (sources_viscosity.f90:196-223; geometry_logcylindrical.f90:344-356),
     // in curvilinear components (geometry_cylindrical.f90:485-494; the
     // Cartesian geometry keeps them)
-    std::vector<double> e(2 * (size_t)ctx->FS, 0.0);
+    std::vector<double> e(3 * (size_t)ctx->FS, 0.0);
     for (int j = 0; j < ctx->NJ; j++)
       for (int i = 0; i < ctx->NI; i++) {
         const size_t k = (size_t)j * ctx->NI + i, ix = (size_t)j * ctx->P + i;
@@ -1204,6 +1205,7 @@ extern "C" int fv_create(fv_ctx **out, const fv_config *cfg) {
           y = r * std::sin(phi);
         }
         const double r2 = rad * rad, ex = -(y / r2), ey = x / r2;
+        e[2 * ctx->FS + ix] = r2;
         if (c.geometry == LOGCYLINDRICAL) {
           e[ix] = ex * std::cos(phi) + ey * std::sin(phi);
           e[ctx->FS + ix] = -(ex * std::sin(phi)) + ey * std::cos(phi);
@@ -1307,7 +1309,8 @@ extern "C" int fv_device_bytes(const fv_config *c, double *bytes) {
   if (c->outbound) b += 2.0 * (8.0 * (PM_HDR + 2.0 * NI) + (c->dtype == 1 ? 4.0 * 2.0 * NI : 0.0));
   if (c->vismodel)                                      // viscous source, its step partials
     b += 8.0 * (V - 1.0) * NJ * P + 16.0 * std::ceil(NI / 64.0) * std::ceil(NJ / 8.0);
-  if (c->vismodel == 2) b += (V == 4 ? 5.0 : 3.0) * 8.0 * NJ * P;  // ALPHA: viscosity, e_phi/r, Euler: sound speeds
+  if (c->vismodel >= 2)                                 // viscosity, e_phi/r, r^2; Euler ALPHA: sound speeds
+    b += (c->vismodel == 2 && V == 4 ? 6.0 : 4.0) * 8.0 * NJ * P;
   b += 2.0 * 2.0 * 8.0 * nb32;                         // CFL and error partials
   b += 8.0 * (R_COUNT * NI + C_COUNT * NJ);             // metric arrays
   if (c->dtype == 1) b += 4.0 * (R_COUNT * NI + C_COUNT * NJ);

@@ -153,15 +153,15 @@ struct StageArgsT {
 using StageArgs = StageArgsT<double>;
 
 // Viscosity source (sources_viscosity.f90:73-577), FP64 state only.
-enum VisModel { VIS_MOLECULAR = 1, VIS_ALPHA = 2 };
+enum VisModel { VIS_MOLECULAR = 1, VIS_ALPHA = 2, VIS_BETA = 3 };
 struct VisArgs {
   int model;
   int update;           // ALPHA: this evaluation recomputes the dynamic viscosity
                         // (UpdateViscosity's time test, sources_viscosity.f90:371),
                         // else it reads the one stored by the last update
   double dynconst, bulkconst, omega;
-  const double *ephir;  // ALPHA: e_phi/r in curvilinear components, [2][NJ][P]
-  double *dyn;          // ALPHA: dynamic viscosity of the last update, all cells, [NJ][P]
+  const double *ephir;  // ALPHA, BETA: e_phi/r in curvilinear components, then radius^2, [3][NJ][P]
+  double *dyn;          // ALPHA, BETA: dynamic viscosity of the last update, all cells, [NJ][P]
   double *src;          // k_visc output: StageArgs::vsrc
   // ALPHA with Euler physics: Physics%bccsound, the sound speed of
   // Timedisc%pvar that every ComputeRHS of the main state updates

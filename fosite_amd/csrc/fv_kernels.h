@@ -1527,13 +1527,16 @@ __global__ __launch_bounds__(256) void k_frame(Geom g, const T *src, T *dst, int
 }
 
 // ------------------------------------------------------------- viscosity
-// ALPHA dynamic viscosity of one cell (sources_viscosity.f90:377-386 with
-// etafkt_alpha): alpha*rho*cs^2 / |e_phi/r . v + Omega|
+// dynamic viscosity of one cell from the local angular velocity
+// e_phi/r . v + Omega (sources_viscosity.f90:377-394): ALPHA (etafkt_alpha)
+// alpha*rho*cs^2/|Omega_l|, BETA (etafkt_beta, 457-466) beta*rho*r^2*|Omega_l|
+// with the bary-centre radius
 __device__ __forceinline__ double alpha_dynvis(const VisArgs &va, long ix, long FS, double rho, double vx, double vy,
                                                double cs) {
   double om = va.ephir[ix] * vx;
   om = om + va.ephir[FS + ix] * vy;
   om = om + va.omega;
+  if (va.model == VIS_BETA) return va.dynconst * rho * va.ephir[2 * FS + ix] * fabs(om);
   return va.dynconst * rho * (cs * cs) / fabs(om);
 }
 // Divergence of a vector field in 2-D curvilinear coordinates from its face
@@ -1619,7 +1622,8 @@ __global__ __launch_bounds__(VT_X *VT_Y, FV_VISC_WAVES_PER_SIMD) void k_visc(Geo
   const int tx = threadIdx.x, ty = threadIdx.y, t = tx + VT_X * ty;
   const int i0 = blockIdx.x * VT_X, j0 = blockIdx.y * VT_Y;
   const int imin = GH, imax = GH + g.inum - 1, jmin = GH, jmax = GH + g.jnum - 1;
-  const bool alpha = va.model == VIS_ALPHA;
+  const bool alpha = va.model == VIS_ALPHA || va.model == VIS_BETA;  // viscosity from the local rotation
+  const bool euler_cs = V == 4 && va.model == VIS_ALPHA;               // Physics%bccsound of Euler physics
   // UpdateViscosity's time test (sources_viscosity.f90:371): decided by the
   // host, or here from the step's time and dt in device-controlled steps
   bool update = va.update;
@@ -1671,17 +1675,18 @@ __global__ __launch_bounds__(VT_X *VT_Y, FV_VISC_WAVES_PER_SIMD) void k_visc(Geo
     const bool own = r >= 0 && r < VT_Y && c >= 0 && c < VT_X;
     if constexpr (V == 4) {
       if (va.part && own) Pown[r][c] = p[3];
-      if (alpha && a.nk == 0 && own) va.cs_out[cidx(g, ii, jj)] = sound_speed(g.gamma, p[0], p[3]);
+      if (euler_cs && a.nk == 0 && own) va.cs_out[cidx(g, ii, jj)] = sound_speed(g.gamma, p[0], p[3]);
     }
     if (inreg[h]) {
       double d = dst[h];
       if (alpha && update) {
         const long ix = cidx(g, ii, jj);
-        double cs;
-        if constexpr (V == 4)
-          cs = a.nk == 0 ? sound_speed(g.gamma, p[0], p[3]) : va.cs_in[ix];
-        else
+        double cs = 0.0;  // (BETA does not use it)
+        if constexpr (V == 4) {
+          if (euler_cs) cs = a.nk == 0 ? sound_speed(g.gamma, p[0], p[3]) : va.cs_in[ix];
+        } else {
           cs = ring(g, R_CS, ii);
+        }
         d = alpha_dynvis(va, ix, g.FS, p[0], p[1], p[2], cs);
         if (r >= 0 && r < VT_Y && c >= 0 && c < VT_X) va.dyn[ix] = d;
       }
@@ -1759,7 +1764,8 @@ __global__ __launch_bounds__(256) void k_visc_cfl(Geom g, VisArgs va, const doub
     if (add_w) v = v + ring(g, R_W, i);
     const double cs = (V == 4) ? sound_speed(g.gamma, rho, pv[3 * g.FS + ix]) : ring(g, R_CS, i);
     double d = va.dynconst;
-    if (va.model == VIS_ALPHA) d = va.update ? alpha_dynvis(va, ix, g.FS, rho, u, v, cs) : va.dyn[ix];
+    if (va.model == VIS_ALPHA || va.model == VIS_BETA)
+      d = va.update ? alpha_dynvis(va, ix, g.FS, rho, u, v, cs) : va.dyn[ix];
     m = fmax2(m, visc_invdt(g, i, rho, u, v, cs, d));
   }
   m = blk_max(m, red);

@@ -211,10 +211,10 @@ CONTAINS
             CALL this%Error("timedisc_gpu","the viscosity before the rotating frame in the source list")
           c%rotframe = MERGE(2, 1, sg%disable_centaccel)
         TYPE IS (sources_viscosity)
-          ! MOLECULAR (1) and ALPHA (2) (sources_viscosity.f90:75-76)
+          ! MOLECULAR (1), ALPHA (2) and BETA (3) (sources_viscosity.f90:75-77)
           c%vismodel = sg%viscosity%GetType()
-          IF (c%vismodel.NE.1 .AND. c%vismodel.NE.2) &
-            CALL this%Error("timedisc_gpu","viscosity models other than MOLECULAR and ALPHA")
+          IF (c%vismodel.LT.1 .OR. c%vismodel.GT.3) &
+            CALL this%Error("timedisc_gpu","viscosity models other than MOLECULAR, ALPHA and BETA")
           c%dynconst = sg%dynconst
           c%bulkconst = sg%bulkconst
           c%cvis = sg%cvis

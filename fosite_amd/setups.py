@@ -20,7 +20,7 @@ A configuration is a plain dict whose keys follow the reference's Dict keys
   rotframe  rotating-frame source: 0 off, 1 on, 2 with disable_centaccel
             (sources_rotframe.f90:86-93, 212-247)
   count     datafile outputs between starttime and stoptime (fileio_base.f90:420-470)
-  vismodel  viscosity source: 0 off, 1 MOLECULAR, 2 ALPHA (sources_viscosity.f90:73-577)
+  vismodel  viscosity source: 0 off, 1 MOLECULAR, 2 ALPHA, 3 BETA (sources_viscosity.f90:73-577)
   dynconst, bulkconst, cvis
             viscosity constants and viscous Courant number; bulkconst defaults
             to -2/3 dynconst (sources_viscosity.f90:152-165)

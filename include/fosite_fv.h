@@ -73,9 +73,10 @@ typedef struct fv_config {
                        (disable_centaccel = 1) */
   int vismodel;     /* viscosity source (sources/sources_viscosity.f90:73-577): 0 off,
                        1 MOLECULAR (constant dynamic and bulk viscosity), 2 ALPHA
-                       (Shakura-Sunyaev; uses omega above); FP64 state, single contexts
-                       (not in batches) */
-  double dynconst, bulkconst; /* dynamic viscosity (MOLECULAR) or alpha (ALPHA), default 0.1;
+                       (Shakura-Sunyaev), 3 BETA (Duschl); ALPHA and BETA use omega above;
+                       FP64 state, single contexts (not in batches) */
+  double dynconst, bulkconst; /* dynamic viscosity (MOLECULAR), alpha (ALPHA) or beta (BETA),
+                       default 0.1;
                        bulk viscosity (MOLECULAR), the reference's default being -2/3 dynconst
                        (sources_viscosity.f90:152-158) */
   double cvis;      /* viscous Courant number, default 0.5 (sources_viscosity.f90:165) */

@@ -61,7 +61,7 @@ BAD = {
     "tableau": dict(method=S.SSPRK, order=4),
     "tiny mesh": dict(inum=3),
     "dtype": dict(dtype=1),
-    "vismodel": dict(vismodel=3),
+    "vismodel": dict(vismodel=4),
     "viscosity in fp32": dict(vismodel=1, dtype="fp32"),
 }
 

@@ -60,6 +60,10 @@ CASES = {
     "euler_disk_alpha_fargo1_rotframe": (dict(S.euler_disk_config(ni=32, nj=64), vismodel=2, dynconst=0.01,
                                               fargo=1, omega=0.3, rotframe=1, method=S.SSPRK, order=3, bc=NG),
                                          12, 0.0),
+    # BETA: beta*rho*r^2*|Omega| (an isothermal disk; an Euler disk with FARGO type 1)
+    "disk_beta_ssprk3": (dict(D, fargo=0, vismodel=3, dynconst=1e-3, bc=NG), 15, 0.0),
+    "euler_disk_beta_fargo1": (dict(S.euler_disk_config(ni=32, nj=64), vismodel=3, dynconst=1e-3, fargo=1,
+                                    method=S.SSPRK, order=3, bc=NG), 12, 0.0),
 }
 
 
@@ -89,7 +93,7 @@ def test_gpu_viscosity_steps_match_oracle(name):
                                   "euler_molecular_bulk_periodic_dp5", "disk_alpha_no_fargo_limited",
                                   "shear_iso_molecular_fargo1_limited", "euler_disk_molecular_fargo1_prim",
                                   "disk_alpha_dp5_absorbing", "euler_disk_alpha_dp5",
-                                  "euler_disk_alpha_fargo1_rotframe"])
+                                  "euler_disk_alpha_fargo1_rotframe", "disk_beta_ssprk3", "euler_disk_beta_fargo1"])
 def test_gpu_viscosity_device_rounds_match_oracle(name):
     """fv_run with viscosity: device-controlled steps (k_visc before every
     k_rhs, UpdateViscosity's time test and the viscous limit of the next step
